@@ -1,0 +1,198 @@
+/*
+ * w90gpu.h — C ABI of the MI355X wannierisation library (libw90gpu.so).
+ *
+ * Scope: the spread minimisation of wannier90 without disentanglement
+ * (wann_main, non-gamma, no preconditioner / site symmetry / SLWF / guiding
+ * centres).  Every entry point takes plain pointers and sizes; complex
+ * arrays are interleaved (re, im) doubles in Fortran (column-major) order,
+ * exactly the memory of wannier90's complex(kind=dp) arrays, so a Fortran
+ * caller passes its arrays through ISO_C_BINDING without copies and a C or
+ * Python caller passes numpy arrays in order='F'.
+ *
+ * Status convention (src/error.F90:27-36): 0 ok, >0 error, <0 warning.
+ *
+ * Threading: a context may be used by one host thread at a time.  Sessions of
+ * one context share its device and run on their own HIP streams.
+ */
+#ifndef W90GPU_H
+#define W90GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes: same values as w90_error codes (src/error.F90:27-36) */
+#define W90GPU_OK 0
+#define W90GPU_ERR_FATAL 1
+#define W90GPU_ERR_ALLOC 2
+#define W90GPU_ERR_DEALLOC 3
+#define W90GPU_ERR_INPUT 5
+#define W90GPU_ERR_FILE 6
+#define W90GPU_WARN_UNCONVERGED (-1)
+#define W90GPU_WARN (-2)
+
+#define W90GPU_NUM_NNMAX 12      /* types.F90:116 num_nnmax */
+#define W90GPU_MAX_WANN 128      /* largest num_wann the kernels accept */
+#define W90GPU_MAX_CONV_WINDOW 64
+
+/* arithmetic of the rotation / gradient pass */
+#define W90GPU_PREC_FP64 0       /* plain drop-in: everything in FP64 */
+#define W90GPU_PREC_MIXED 1      /* FP32 MFMA bulk + FP64 phases, retraction and polish */
+
+typedef struct w90gpu_ctx w90gpu_ctx;
+typedef struct w90gpu_session w90gpu_session;
+
+/* Control parameters of the minimiser on this path; field meaning and
+ * defaults follow wann_control_type (src/wannier90_types.F90:170-192) and
+ * the .win keyword defaults (src/wannier90_readwrite.F90:628-804). */
+typedef struct {
+  int num_iter;          /* num_iter            (100)   */
+  int num_cg_steps;      /* num_cg_steps        (5)     */
+  int conv_window;       /* conv_window         (-1: never stop early) */
+  int lfixstep;          /* fixed_step given -> 1 */
+  double conv_tol;       /* conv_tol            (1e-10) */
+  double trial_step;     /* trial_step          (2.0)   */
+  double fixed_step;     /* fixed_step          (-999)  */
+  double conv_noise_amp; /* must be <= 0 on this path  */
+  int precond;           /* must be 0 */
+  int guiding_centres;   /* must be 0 */
+  int selective_loc;     /* must be 0 */
+  int precision;         /* W90GPU_PREC_FP64 or W90GPU_PREC_MIXED */
+} w90gpu_control;
+
+/* One wannierisation problem: sizes and the k-mesh tables of kmesh_info_type
+ * (src/types.F90:161-176).  Arrays are read during the call only. */
+typedef struct {
+  int num_wann;
+  int num_kpts;
+  int nntot;
+  const int* nnlist;     /* (num_kpts, nntot), 1-based k indices */
+  const double* bk;      /* (3, nntot, num_kpts) Cartesian b-vectors, 1/Ang */
+  const double* wb;      /* (nntot) weights, Ang^2 */
+  double wbtot;          /* sum of wb */
+} w90gpu_problem;
+
+/* Spread decomposition and loop statistics (wann_omega_type and the
+ * localisation_vars_type of src/wannierise.F90:35-43). */
+typedef struct {
+  double omega_total;     /* Omega = Omega_I + Omega_D + Omega_OD        */
+  double omega_invariant; /* Omega_I                                     */
+  double omega_d;         /* Omega_D                                     */
+  double omega_od;        /* Omega_OD                                    */
+  double omega_tilde;     /* Omega_D + Omega_OD                          */
+  double gcnorm1;         /* last squared gradient norm (RMS^2 on CONV)  */
+  double delta_omega;     /* last iteration's change of Omega            */
+  double solve_seconds;   /* wall time of the loop on the device         */
+  int iterations;         /* iterations performed                        */
+  int converged;          /* 1 if the conv_window test was satisfied     */
+  int unitarity_ok;       /* 1 if the final U passed the 1e-5 check      */
+  int status;             /* status of the solve                         */
+} w90gpu_result;
+
+/* ---------------------------------------------------------------- setup */
+
+/* Fill defaults of wann_control_type. */
+void w90gpu_control_defaults(w90gpu_control* ctl);
+
+/* Create a context on HIP device `device`.  Replaces the library state
+ * object of w90_create (src/c_interface.F90:10) for the GPU path. */
+int w90gpu_create(int device, w90gpu_ctx** ctx);
+int w90gpu_destroy(w90gpu_ctx* ctx);
+
+/* Message of the last error on this context (empty if none). */
+const char* w90gpu_last_error(const w90gpu_ctx* ctx);
+
+/* Library version string. */
+const char* w90gpu_version(void);
+
+/* -------------------------------------------------------------- k-mesh */
+
+/* Shells, B1 weights and neighbour lists; replaces kmesh_get
+ * (src/kmesh.F90:61-740).  real_lattice(3,3): row i is lattice vector a_i
+ * (Fortran real_lattice(i,:)), Ang.  kpt_latt(3,num_kpts) fractional.
+ * num_shells <= 0 selects shells automatically (kmesh_shell_automatic);
+ * otherwise shell_list(num_shells) is used (kmesh_shell_fixed).
+ * Outputs must hold W90GPU_NUM_NNMAX neighbours; they are written packed
+ * with the returned nntot: nnlist(num_kpts,nntot) 1-based,
+ * nncell(3,num_kpts,nntot), bk(3,nntot,num_kpts), wb(nntot), wbtot. */
+int w90gpu_kmesh_get(const double* real_lattice, int num_kpts, const double* kpt_latt,
+                     int search_shells, double tol, int num_shells, const int* shell_list,
+                     int skip_b1_tests, int* nntot, int* nnlist, int* nncell, double* bk,
+                     double* wb, double* wbtot, char* errmsg, int errlen);
+
+/* ------------------------------------------------ one-shot drop-in solve */
+
+/* The spread minimisation loop of wann_main (src/wannierise.F90:48-1072)
+ * for one problem.  m_matrix (num_wann,num_wann,nntot,num_kpts) complex and
+ * u_matrix (num_wann,num_wann,num_kpts) complex are updated in place, as
+ * w90_wannierise does with the caller's arrays
+ * (src/library_interface.F90:464-530).  centres(3,num_wann) Ang and
+ * spreads(num_wann) Ang^2 receive wannier_data%centres/spreads.
+ * Returns W90GPU_WARN_UNCONVERGED never (the reference does not flag a
+ * loop that ran out of iterations); >0 on error with a message in
+ * w90gpu_last_error. */
+int w90gpu_wann_main(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90gpu_control* ctl,
+                     double* m_matrix, double* u_matrix, double* centres, double* spreads,
+                     w90gpu_result* result);
+
+/* ------------------------------------------- per-phase entry points (FP64) */
+
+/* wann_omega (src/wannierise.F90:1979-2284), non-SLWF branch, csheet=1,
+ * sheet=0.  If first_pass, Omega_I is computed from M; otherwise
+ * omega_invariant is used.  Outputs: rave(3,num_wann), r2ave(num_wann),
+ * omega[4] = {Omega_I, Omega_D, Omega_OD, Omega_total}. */
+int w90gpu_omega(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* m_matrix,
+                 int first_pass, double omega_invariant, double* rave, double* r2ave,
+                 double* omega);
+
+/* wann_domega (src/wannierise.F90:2287-2569), non-SLWF gradient:
+ * cdodq(num_wann,num_wann,num_kpts) complex. */
+int w90gpu_domega(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* m_matrix,
+                  double* cdodq);
+
+/* internal_new_u_and_m (src/wannierise.F90:1599-1751): with cdq the
+ * anti-Hermitian step (num_wann,num_wann,num_kpts), U(k) <- U(k) exp(cdq_k)
+ * and M(k,b) <- exp(cdq_k)^+ M(k,b) exp(cdq_{k+b}), in place. */
+int w90gpu_new_u_and_m(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* cdq,
+                       double* u_matrix, double* m_matrix);
+
+/* ------------------------------------------------ device-resident sessions */
+
+/* A session keeps one problem's M, U and k-mesh tables resident in HBM and
+ * reruns the minimisation on them without host traffic.  Used by the
+ * drop-in solve, by benchmarks (inputs resident in HBM) and by serving. */
+int w90gpu_session_create(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90gpu_control* ctl,
+                          w90gpu_session** sess);
+int w90gpu_session_destroy(w90gpu_session* sess);
+/* Upload the initial M and U (host, Fortran layout). */
+int w90gpu_session_load(w90gpu_session* sess, const double* m_matrix, const double* u_matrix);
+/* Run the minimisation from the loaded initial state (which is kept). */
+int w90gpu_session_solve(w90gpu_session* sess, w90gpu_result* result);
+/* Enqueue the solve without waiting; w90gpu_session_wait completes it. */
+int w90gpu_session_launch(w90gpu_session* sess);
+int w90gpu_session_wait(w90gpu_session* sess, w90gpu_result* result);
+/* Download the final state of the last solve. */
+int w90gpu_session_fetch(w90gpu_session* sess, double* m_matrix, double* u_matrix,
+                         double* centres, double* spreads);
+/* Per-iteration record of the last solve: row i (i = 0 .. iterations) holds
+ * {Omega, Omega_D, Omega_OD, gcnorm1} after iteration i (row 0: initial
+ * state), i.e. the numbers of the CONV/SPRD lines of wann_main
+ * (src/wannierise.F90:703-756).  Writes min(rows, iterations+1) rows;
+ * returns the number written or a negative status. */
+int w90gpu_session_history(w90gpu_session* sess, double* out, int rows);
+/* Device bytes held by a session of this problem shape. */
+size_t w90gpu_session_bytes(int num_wann, int num_kpts, int nntot);
+/* Kernel timing: when enabled, HIP events bracket every launch of the
+ * M-rotation kernel on the session's stream; *avg_ms and *launches report
+ * the average duration and count since the last reset. */
+int w90gpu_session_profile(w90gpu_session* sess, int enable);
+int w90gpu_session_kernel_time(w90gpu_session* sess, double* avg_ms, int* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* W90GPU_H */

@@ -1,0 +1,141 @@
+"""ctypes access to the C restatement (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  Build with `make -C oracle`.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+REF_EXE = os.path.join(HERE, "_ref", "wannier90.x")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class OProblem(C.Structure):
+    _fields_ = [("nw", C.c_int), ("nk", C.c_int), ("nn", C.c_int), ("nnlist", _ip), ("bk", _dp),
+                ("wb", _dp), ("wbtot", C.c_double)]
+
+
+class OControl(C.Structure):
+    _fields_ = [("num_iter", C.c_int), ("num_cg_steps", C.c_int), ("conv_window", C.c_int),
+                ("lfixstep", C.c_int), ("conv_tol", C.c_double), ("trial_step", C.c_double),
+                ("fixed_step", C.c_double)]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise OSError(f"{LIB} not built: run `make -C oracle`")
+        # MKL selects its CPU kernels by dlopen; they need the core symbols global
+        lib = C.CDLL(LIB, mode=C.RTLD_GLOBAL)
+        P = C.POINTER(OProblem)
+        lib.oracle_omega.argtypes = [P, _dp, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp]
+        lib.oracle_omega.restype = None
+        lib.oracle_domega.argtypes = [P, _dp, _dp, _dp]
+        lib.oracle_domega.restype = None
+        lib.oracle_new_u_and_m.argtypes = [P, _dp, _dp, _dp]
+        lib.oracle_new_u_and_m.restype = C.c_int
+        lib.oracle_check_unitarity.argtypes = [P, _dp]
+        lib.oracle_check_unitarity.restype = C.c_double
+        lib.oracle_lowdin.argtypes = [P, _dp, _dp, _dp]
+        lib.oracle_lowdin.restype = C.c_int
+        lib.oracle_wann_main.argtypes = [P, C.POINTER(OControl), _dp, _dp, _dp, _dp, _dp, _dp, _ip]
+        lib.oracle_wann_main.restype = C.c_int
+        _lib = lib
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Problem:
+    def __init__(self, nw, nk, kmesh):
+        self.nnl = np.asfortranarray(kmesh.nnlist, dtype=np.int32)
+        self.bk = np.asfortranarray(kmesh.bk, dtype=np.float64)
+        self.wb = np.ascontiguousarray(kmesh.wb, dtype=np.float64)
+        self.s = OProblem(nw, nk, kmesh.nntot, self.nnl.ctypes.data_as(_ip), _d(self.bk),
+                          _d(self.wb), float(kmesh.wbtot))
+        self.nw, self.nk, self.nn = nw, nk, kmesh.nntot
+
+
+def omega(prob, m, first_pass=True, omega_invariant=0.0):
+    lib = load()
+    nw, nk, nn = prob.nw, prob.nk, prob.nn
+    m = np.asfortranarray(m, dtype=np.complex128)
+    rave = np.zeros((3, nw), order="F")
+    r2ave = np.zeros(nw)
+    rave2 = np.zeros(nw)
+    ln = np.zeros((nw, nn, nk), order="F")
+    om = np.zeros(4)
+    lib.oracle_omega(C.byref(prob.s), _d(m), 1 if first_pass else 0, omega_invariant, _d(rave),
+                     _d(r2ave), _d(rave2), _d(ln), _d(om))
+    return rave, r2ave, om, ln
+
+
+def domega(prob, m):
+    lib = load()
+    m = np.asfortranarray(m, dtype=np.complex128)
+    g = np.zeros((prob.nw, prob.nw, prob.nk), dtype=np.complex128, order="F")
+    rave = np.zeros((3, prob.nw), order="F")
+    lib.oracle_domega(C.byref(prob.s), _d(m), _d(g), _d(rave))
+    return g, rave
+
+
+def new_u_and_m(prob, cdq, u, m):
+    """Returns rotated copies (U, M)."""
+    lib = load()
+    cdq = np.array(cdq, dtype=np.complex128, order="F")
+    u = np.array(u, dtype=np.complex128, order="F")
+    m = np.array(m, dtype=np.complex128, order="F")
+    info = lib.oracle_new_u_and_m(C.byref(prob.s), _d(cdq), _d(u), _d(m))
+    if info:
+        raise RuntimeError(f"zheev info={info}")
+    return u, m
+
+
+def lowdin(prob, a_nwk, m):
+    """a (nw,nw,nk) Fortran; returns (U, rotated M)."""
+    lib = load()
+    a = np.array(a_nwk, dtype=np.complex128, order="F")
+    u = np.zeros_like(a)
+    m = np.array(m, dtype=np.complex128, order="F")
+    info = lib.oracle_lowdin(C.byref(prob.s), _d(a), _d(u), _d(m))
+    if info:
+        raise RuntimeError(f"zgesvd info={info}")
+    return u, m
+
+
+def unitarity(prob, u):
+    return load().oracle_check_unitarity(C.byref(prob.s), _d(np.asfortranarray(u)))
+
+
+def wann_main(prob, ctl, m, u):
+    """Returns dict with final m, u, centres, spreads, omega, history, iterations."""
+    lib = load()
+    m = np.array(m, dtype=np.complex128, order="F")
+    u = np.array(u, dtype=np.complex128, order="F")
+    c = OControl(int(ctl["num_iter"]), int(ctl["num_cg_steps"]), int(ctl["conv_window"]),
+                 int(ctl.get("lfixstep", 0)), float(ctl["conv_tol"]), float(ctl["trial_step"]),
+                 float(ctl.get("fixed_step", -999.0)))
+    cen = np.zeros((3, prob.nw), order="F")
+    spr = np.zeros(prob.nw)
+    om = np.zeros(4)
+    hist = np.zeros((c.num_iter + 1, 4))
+    stats = np.zeros(2, dtype=np.int32)
+    info = lib.oracle_wann_main(C.byref(prob.s), C.byref(c), _d(m), _d(u), _d(cen), _d(spr), _d(om),
+                                _d(hist), stats.ctypes.data_as(_ip))
+    if info:
+        raise RuntimeError(f"LAPACK info={info}")
+    it = int(stats[0])
+    return dict(m=m, u=u, centres=cen, spreads=spr, omega=om, history=hist[: it + 1],
+                iterations=it, converged=bool(stats[1]))

@@ -1,0 +1,1489 @@
+// libw90gpu: MI355X wannierisation (spread minimisation without
+// disentanglement).  Device code and the host side of the C ABI declared in
+// include/w90gpu.h.
+//
+// One problem = one session.  M(k,b), U(k), the gradient, the search
+// direction and every reduction buffer stay in HBM for the whole solve; the
+// k-mesh tables are uploaded once.  All scalar control of wann_main — the
+// Fletcher-Reeves coefficient and its resets, the uphill test, the parabolic
+// line search and the convergence window — runs on the device in
+// single-workgroup control kernels, so one iteration is a fixed sequence of
+// launches that needs no host round trip.  The host only polls a done flag.
+//
+// Reference: src/wannierise.F90 (wann_main :48-1072, internal_new_u_and_m
+// :1599-1751, wann_omega :1979-2284, wann_domega :2287-2569,
+// wann_check_unitarity :2594-2661).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/w90gpu.h"
+
+typedef double2 cplx;
+
+#define W90_NT 256        // threads per workgroup for every kernel
+#define W90_PANEL 16      // output rows per rotation workgroup
+
+// --------------------------------------------------------------- helpers
+
+__device__ __forceinline__ cplx c_make(double r, double i) { return make_double2(r, i); }
+__device__ __forceinline__ cplx c_add(cplx a, cplx b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cplx c_sub(cplx a, cplx b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cplx c_scale(cplx a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ cplx c_conj(cplx a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ cplx c_mul(cplx a, cplx b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// conj(a) * b
+__device__ __forceinline__ cplx c_cmul(cplx a, cplx b) {
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ cplx c_div(cplx a, cplx b) {
+  const double d = b.x * b.x + b.y * b.y;
+  return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+__device__ __forceinline__ double c_abs2(cplx a) { return a.x * a.x + a.y * a.y; }
+
+// Block-wide sum of one double per thread; deterministic (fixed tree).
+__device__ double block_sum(double v, double* red) {
+  const int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) red[t] += red[t + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ double block_max(double v, double* red) {
+  const int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) red[t] = fmax(red[t], red[t + s]);
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// --------------------------------------------------------------- state
+
+struct Spread {
+  double om_i, om_d, om_od, om_tot;
+};
+
+enum Mode { MODE_INIT = 0, MODE_TRIAL = 1, MODE_SECOND = 2 };
+
+// Per-problem solver state, resident in device memory.
+struct State {
+  // control (wann_control_type)
+  int num_iter, num_cg_steps, conv_window, lfixstep;
+  double conv_tol, trial_step, fixed_step, wbtot;
+  // loop state of wann_main
+  int iter;        // completed iterations
+  int ncg;
+  int cur;         // buffer holding the current U, M
+  int done;        // loop finished (converged or num_iter)
+  int converged;
+  int lquad;
+  int second;      // parabolic step pending in this iteration
+  int conv_count;
+  int first_pass;
+  int pad0;
+  double gcnorm0, gcnorm1, gcfac, doda0, alphamin, falphamin;
+  double beta, dir_sign, scale;
+  double omega_invariant;
+  Spread wann, old, trial;
+  double history[W90GPU_MAX_CONV_WINDOW];
+};
+
+// Device pointers of one problem, passed by value to every kernel.
+struct Prob {
+  int nw, nk, nn, npan;
+  int nch, kc, nw2, pad;
+  const int* nnl;     // [k][b] 0-based neighbour
+  const double* bk;   // [k][b][3]
+  const double* wb;   // [b]
+  cplx* M[2];         // [k][b][nw*nw] column-major blocks
+  cplx* U[2];         // [k][nw*nw]
+  cplx* G;            // gradient
+  cplx* D;            // search direction (cdqkeep)
+  cplx* E;            // exp(dW) per k
+  cplx* W1;           // exp workspace per k
+  cplx* W2;           // exp workspace per k
+  double* ln[2];      // [k][b][n] Im ln M_nn of each buffer
+  double* d2;         // [k][b][n] |M_nn|^2 of the last rotated buffer
+  double* part_od;    // [k][b][panel]
+  double* part_all;   // [k][b][panel]
+  double* chunk;      // [chunk][8*nw+8]
+  double* part_gg;    // [k]
+  double* part_gd;    // [k]
+  double* rave[2];    // [3][nw] per buffer (Fortran rave(3,nw): ind fastest)
+  double* r2ave[2];   // [nw]
+  double* unit;       // [k] unitarity deviation
+  double* log;        // [num_iter+1][4] om_tot, om_d, om_od, gcnorm1
+  State* st;
+};
+
+__device__ __forceinline__ bool gate(const State* st, int mode) {
+  if (st->done) return false;
+  if (mode == MODE_SECOND) return st->second != 0;
+  return true;
+}
+
+// --------------------------------------------------------------- kernels
+
+// Reset the loop state for a new solve.
+__global__ void k_reset(Prob p, w90gpu_control ctl, int first_pass, double omega_invariant) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  State* st = p.st;
+  st->num_iter = ctl.num_iter;
+  st->num_cg_steps = ctl.num_cg_steps;
+  st->conv_window = ctl.conv_window;
+  st->lfixstep = ctl.lfixstep;
+  st->conv_tol = ctl.conv_tol;
+  st->trial_step = ctl.trial_step;
+  st->fixed_step = ctl.fixed_step;
+  st->iter = 0;
+  st->ncg = 0;
+  st->cur = 0;
+  st->done = 0;
+  st->converged = 0;
+  st->lquad = 0;
+  st->second = 0;
+  st->conv_count = 0;
+  st->first_pass = first_pass;
+  st->gcnorm0 = st->gcnorm1 = st->gcfac = st->doda0 = 0.0;
+  st->alphamin = st->falphamin = 0.0;
+  st->beta = 0.0;
+  st->dir_sign = 1.0;
+  st->scale = 0.0;
+  st->omega_invariant = omega_invariant;
+  st->wann = Spread{0, 0, 0, 0};
+  st->old = Spread{0, 0, 0, 0};
+  st->trial = Spread{0, 0, 0, 0};
+  for (int i = 0; i < W90GPU_MAX_CONV_WINDOW; ++i) st->history[i] = 0.0;
+}
+
+// Direct-step state for internal_new_u_and_m as a standalone phase.
+__global__ void k_set_direct(Prob p, double scale) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  State* st = p.st;
+  st->done = 0;
+  st->second = 1;
+  st->cur = 0;
+  st->scale = scale;
+}
+
+__global__ void k_zero(cplx* x, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    x[i] = c_make(0.0, 0.0);
+}
+
+// M rotation M(k,b) <- E_k^+ M(k,b) E_{k+b} for one (k, b, row panel),
+// fused with the per-block partial sums of wann_omega: Im ln M_nn, |M_nn|^2,
+// sum_{m!=n}|M_mn|^2 and sum_mn |M_mn|^2 of the rotated block.
+// MODE_INIT only evaluates the partial sums of the current buffer.
+// (internal_new_u_and_m :1737-1748; wann_omega :2034-2043, :2076-2090,
+//  :2214-2258)
+__global__ void __launch_bounds__(W90_NT) k_rotate(Prob p, int mode) {
+  const State* st = p.st;
+  if (!gate(st, mode)) return;
+  const int nw = p.nw, nw2 = p.nw2;
+  const int panel = blockIdx.x % p.npan;
+  const int kb = blockIdx.x / p.npan;
+  const int k = kb / p.nn;
+  const int cur = st->cur;
+  const int dst = (mode == MODE_INIT) ? cur : 1 - cur;
+  const int i0 = panel * W90_PANEL;
+  const int prows = min(W90_PANEL, nw - i0);
+  const cplx* Ms = p.M[cur] + (size_t)kb * nw2;
+  cplx* Md = p.M[dst] + (size_t)kb * nw2;
+  __shared__ cplx T[W90_PANEL * W90GPU_MAX_WANN];
+  __shared__ double red[W90_NT];
+
+  const int nout = prows * nw;
+  double od = 0.0, all = 0.0;
+  if (mode != MODE_INIT) {
+    const cplx* Ek = p.E + (size_t)k * nw2;
+    const cplx* Ekb = p.E + (size_t)p.nnl[kb] * nw2;
+    for (int e = threadIdx.x; e < nout; e += blockDim.x) {
+      const int i = e % prows, j = e / prows;
+      const cplx* ec = Ek + (size_t)(i0 + i) * nw;
+      const cplx* mc = Ms + (size_t)j * nw;
+      cplx acc = c_make(0.0, 0.0);
+      for (int l = 0; l < nw; ++l) acc = c_add(acc, c_cmul(ec[l], mc[l]));
+      T[i + j * W90_PANEL] = acc;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nout; e += blockDim.x) {
+      const int i = e % prows, j = e / prows;
+      const cplx* ec = Ekb + (size_t)j * nw;
+      cplx acc = c_make(0.0, 0.0);
+      for (int l = 0; l < nw; ++l) acc = c_add(acc, c_mul(T[i + l * W90_PANEL], ec[l]));
+      Md[(i0 + i) + (size_t)j * nw] = acc;
+      const double a2 = c_abs2(acc);
+      all += a2;
+      if (i0 + i == j) {
+        p.ln[dst][(size_t)kb * nw + j] = atan2(acc.y, acc.x);
+        p.d2[(size_t)kb * nw + j] = a2;
+      } else {
+        od += a2;
+      }
+    }
+  } else {
+    for (int e = threadIdx.x; e < nout; e += blockDim.x) {
+      const int i = e % prows, j = e / prows;
+      const cplx v = Ms[(i0 + i) + (size_t)j * nw];
+      const double a2 = c_abs2(v);
+      all += a2;
+      if (i0 + i == j) {
+        p.ln[dst][(size_t)kb * nw + j] = atan2(v.y, v.x);
+        p.d2[(size_t)kb * nw + j] = a2;
+      } else {
+        od += a2;
+      }
+    }
+  }
+  od = block_sum(od, red);
+  all = block_sum(all, red);
+  if (threadIdx.x == 0) {
+    p.part_od[(size_t)kb * p.npan + panel] = od;
+    p.part_all[(size_t)kb * p.npan + panel] = all;
+  }
+}
+
+// First reduction stage of wann_omega over a chunk of k-points.  Per
+// Wannier function n it accumulates
+//   S_rave[ind] = sum wb b_ind ln,  S_r2 = sum wb (1 - |M_nn|^2 + ln^2),
+//   A = sum wb (ln + b.r0)^2,  V[ind] = sum wb b_ind (ln + b.r0)
+// with r0 the centres of the source buffer, plus the scalars
+//   S_od = sum wb od,  S_i = sum wb (nw - sum|M|^2),  B_ij = sum wb b_i b_j.
+// Omega_D = (1/nk) sum_n [A + 2 d.V + d^T B d] with d = rave - r0 is then
+// exact algebra for sum wb (ln + b.rave)^2 (wannierise.F90:2260-2275) without
+// a second pass over ln.
+__global__ void __launch_bounds__(W90_NT) k_omega_chunks(Prob p, int mode) {
+  const State* st = p.st;
+  if (!gate(st, mode)) return;
+  const int nw = p.nw, nn = p.nn;
+  const int cur = st->cur;
+  const int dst = (mode == MODE_INIT) ? cur : 1 - cur;
+  const double* ln = p.ln[dst];
+  const double* r0 = (mode == MODE_INIT) ? nullptr : p.rave[cur];
+  const int c = blockIdx.x;
+  const int kbeg = c * p.kc, kend = min(p.nk, kbeg + p.kc);
+  const int nitems = (kend - kbeg) * nn;
+  const int psz = 8 * nw + 8;
+  double* out = p.chunk + (size_t)c * psz;
+  __shared__ double red[W90_NT];
+  __shared__ double acc8[8 * W90GPU_MAX_WANN];
+
+  // per-n accumulation: thread (n, sub)
+  const int nsub = blockDim.x / nw;
+  const int n = threadIdx.x % nw;
+  const int sub = threadIdx.x / nw;
+  double s_r[3] = {0, 0, 0}, s_r2 = 0, s_a = 0, s_v[3] = {0, 0, 0};
+  if (sub < nsub) {
+    double rx = 0, ry = 0, rz = 0;
+    if (r0) {
+      rx = r0[0 + 3 * n];
+      ry = r0[1 + 3 * n];
+      rz = r0[2 + 3 * n];
+    }
+    for (int it = sub; it < nitems; it += nsub) {
+      const int kb = kbeg * nn + it;
+      const int b = kb % nn;
+      const double w = p.wb[b];
+      const double* bv = p.bk + (size_t)kb * 3;
+      const double l = ln[(size_t)kb * nw + n];
+      const double d2 = p.d2[(size_t)kb * nw + n];
+      s_r[0] += w * bv[0] * l;
+      s_r[1] += w * bv[1] * l;
+      s_r[2] += w * bv[2] * l;
+      s_r2 += w * (1.0 - d2 + l * l);
+      const double x = l + (bv[0] * rx + bv[1] * ry + bv[2] * rz);
+      s_a += w * x * x;
+      s_v[0] += w * bv[0] * x;
+      s_v[1] += w * bv[1] * x;
+      s_v[2] += w * bv[2] * x;
+    }
+  }
+  // combine the nsub partial sums of each n in fixed order
+  for (int q = 0; q < 8; ++q) {
+    double v;
+    switch (q) {
+      case 0: v = s_r[0]; break;
+      case 1: v = s_r[1]; break;
+      case 2: v = s_r[2]; break;
+      case 3: v = s_r2; break;
+      case 4: v = s_a; break;
+      case 5: v = s_v[0]; break;
+      case 6: v = s_v[1]; break;
+      default: v = s_v[2]; break;
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    if (threadIdx.x < nw) {
+      double s = 0.0;
+      for (int u = 0; u < nsub; ++u) s += red[threadIdx.x + u * nw];
+      acc8[q * nw + threadIdx.x] = s;
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < 8 * nw; e += blockDim.x) out[e] = acc8[e];
+
+  // scalar sums over (k,b)
+  double sod = 0, si = 0, bxx = 0, byy = 0, bzz = 0, bxy = 0, byz = 0, bzx = 0;
+  for (int it = threadIdx.x; it < nitems; it += blockDim.x) {
+    const int kb = kbeg * nn + it;
+    const int b = kb % nn;
+    const double w = p.wb[b];
+    double od = 0, al = 0;
+    for (int q = 0; q < p.npan; ++q) {
+      od += p.part_od[(size_t)kb * p.npan + q];
+      al += p.part_all[(size_t)kb * p.npan + q];
+    }
+    sod += w * od;
+    si += w * ((double)nw - al);
+    const double* bv = p.bk + (size_t)kb * 3;
+    bxx += w * bv[0] * bv[0];
+    byy += w * bv[1] * bv[1];
+    bzz += w * bv[2] * bv[2];
+    bxy += w * bv[0] * bv[1];
+    byz += w * bv[1] * bv[2];
+    bzx += w * bv[2] * bv[0];
+  }
+  double r;
+  r = block_sum(sod, red); if (threadIdx.x == 0) out[8 * nw + 0] = r;
+  r = block_sum(si, red);  if (threadIdx.x == 0) out[8 * nw + 1] = r;
+  r = block_sum(bxx, red); if (threadIdx.x == 0) out[8 * nw + 2] = r;
+  r = block_sum(byy, red); if (threadIdx.x == 0) out[8 * nw + 3] = r;
+  r = block_sum(bzz, red); if (threadIdx.x == 0) out[8 * nw + 4] = r;
+  r = block_sum(bxy, red); if (threadIdx.x == 0) out[8 * nw + 5] = r;
+  r = block_sum(byz, red); if (threadIdx.x == 0) out[8 * nw + 6] = r;
+  r = block_sum(bzx, red); if (threadIdx.x == 0) out[8 * nw + 7] = r;
+}
+
+// internal_test_convergence (wannierise.F90:1076-1164) with the noise branch
+// off (conv_noise_amp <= 0), then the loop-exit bookkeeping.
+__device__ void conv_step(State* st, double* logrow, double gc) {
+  const int iter = st->iter + 1;
+  if (st->conv_window > 1) {
+    const double delta = st->wann.om_tot - st->old.om_tot;
+    const int cw = st->conv_window;
+    if (iter <= cw) {
+      st->history[iter - 1] = delta;
+    } else {
+      for (int j = 0; j < cw - 1; ++j) st->history[j] = st->history[j + 1];
+      st->history[cw - 1] = delta;
+    }
+    st->conv_count += 1;
+    if (st->conv_count >= cw) {
+      bool ok = true;
+      for (int j = 0; j < cw; ++j)
+        if (fabs(st->history[j]) > st->conv_tol) ok = false;
+      if (ok) st->converged = 1;
+    }
+  }
+  st->iter = iter;
+  if (logrow) {
+    logrow[0] = st->wann.om_tot;
+    logrow[1] = st->wann.om_d;
+    logrow[2] = st->wann.om_od;
+    logrow[3] = gc;
+  }
+  if (st->converged || iter >= st->num_iter) st->done = 1;
+}
+
+// Final reduction of wann_omega (single workgroup) fused with the control
+// logic that follows each wann_omega call in wann_main: initial state
+// (:452-470), internal_optimal_step (:1528-1596) after the trial step and
+// the acceptance of the trial or parabolic step (:660-701).
+__global__ void __launch_bounds__(W90_NT) k_omega_final(Prob p, int mode) {
+  State* st = p.st;
+  if (!gate(st, mode)) return;
+  const int nw = p.nw;
+  const int cur = st->cur;
+  const int dst = (mode == MODE_INIT) ? cur : 1 - cur;
+  const int psz = 8 * nw + 8;
+  const double rnk = (double)p.nk;
+  __shared__ double red[W90_NT];
+  __shared__ double sc[8];
+
+  double omd_n = 0.0;
+  for (int n = threadIdx.x; n < nw; n += blockDim.x) {
+    double q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < p.nch; ++c) {
+      const double* pc = p.chunk + (size_t)c * psz;
+      for (int u = 0; u < 8; ++u) q[u] += pc[u * nw + n];
+    }
+    double bb[6] = {0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < p.nch; ++c) {
+      const double* pc = p.chunk + (size_t)c * psz + 8 * nw + 2;
+      for (int u = 0; u < 6; ++u) bb[u] += pc[u];
+    }
+    const double rx = -q[0] / rnk, ry = -q[1] / rnk, rz = -q[2] / rnk;
+    p.rave[dst][0 + 3 * n] = rx;
+    p.rave[dst][1 + 3 * n] = ry;
+    p.rave[dst][2 + 3 * n] = rz;
+    p.r2ave[dst][n] = q[3] / rnk;
+    double r0x = 0, r0y = 0, r0z = 0;
+    if (mode != MODE_INIT) {
+      r0x = p.rave[cur][0 + 3 * n];
+      r0y = p.rave[cur][1 + 3 * n];
+      r0z = p.rave[cur][2 + 3 * n];
+    }
+    const double dx = rx - r0x, dy = ry - r0y, dz = rz - r0z;
+    const double dbd = bb[0] * dx * dx + bb[1] * dy * dy + bb[2] * dz * dz +
+                       2.0 * (bb[3] * dx * dy + bb[4] * dy * dz + bb[5] * dz * dx);
+    omd_n += q[4] + 2.0 * (dx * q[5] + dy * q[6] + dz * q[7]) + dbd;
+  }
+  // Omega_D summed over n in a fixed tree
+  const double omd = block_sum(omd_n, red);
+  if (threadIdx.x == 0) {
+    double sod = 0, si = 0;
+    for (int c = 0; c < p.nch; ++c) {
+      const double* pc = p.chunk + (size_t)c * psz + 8 * nw;
+      sod += pc[0];
+      si += pc[1];
+    }
+    sc[0] = sod;
+    sc[1] = si;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+
+  Spread sp;
+  sp.om_od = sc[0] / rnk;
+  sp.om_d = omd / rnk;
+  if (mode == MODE_INIT && st->first_pass) {
+    sp.om_i = sc[1] / rnk;
+  } else {
+    sp.om_i = st->omega_invariant;
+  }
+  sp.om_tot = sp.om_i + sp.om_d + sp.om_od;
+
+  if (mode == MODE_INIT) {
+    st->omega_invariant = sp.om_i;
+    st->first_pass = 0;
+    st->wann = sp;
+    st->old.om_tot = 0.0;
+    if (p.log) {
+      p.log[0] = sp.om_tot;
+      p.log[1] = sp.om_d;
+      p.log[2] = sp.om_od;
+      p.log[3] = 0.0;
+    }
+    return;
+  }
+  double* logrow = p.log ? p.log + 4 * (size_t)(st->iter + 1) : nullptr;
+  if (mode == MODE_TRIAL) {
+    st->trial = sp;
+    if (st->lfixstep) {
+      st->old = st->wann;
+      st->wann = sp;
+      st->cur = dst;
+      conv_step(st, logrow, st->gcnorm1);
+      return;
+    }
+    // internal_optimal_step
+    const double trial_step = st->trial_step;
+    double fac = st->trial.om_tot - st->wann.om_tot;
+    double shift;
+    if (fabs(fac) > 2.2250738585072014e-308) {  // tiny(1.0_dp)
+      fac = 1.0 / fac;
+      shift = 1.0;
+    } else {
+      fac = 1.0e6;
+      shift = fac * st->trial.om_tot - fac * st->wann.om_tot;
+    }
+    const double eqb = fac * st->doda0;
+    const double eqa = shift - eqb * trial_step;
+    double alphamin, falphamin;
+    int lquad;
+    if (fabs(eqa / (fac * st->wann.om_tot)) > 2.220446049250313e-16) {  // epsilon(1.0_dp)
+      lquad = 1;
+      alphamin = -0.5 * eqb / eqa * (trial_step * trial_step);
+      falphamin = st->wann.om_tot - 0.25 * eqb * eqb / (fac * eqa) * (trial_step * trial_step);
+    } else {
+      lquad = 0;
+      alphamin = trial_step;
+      falphamin = st->trial.om_tot;
+    }
+    if (st->doda0 * alphamin > 0.0) {
+      lquad = 0;
+      alphamin = trial_step;
+      falphamin = st->trial.om_tot;
+    }
+    st->alphamin = alphamin;
+    st->falphamin = falphamin;
+    st->lquad = lquad;
+    if (lquad) {
+      st->second = 1;
+      st->scale = alphamin / (4.0 * st->wbtot);
+    } else {
+      st->old = st->wann;
+      st->wann = st->trial;
+      st->cur = dst;
+      conv_step(st, logrow, st->gcnorm1);
+    }
+    return;
+  }
+  // MODE_SECOND: the parabolic step replaced the trial step
+  st->old = st->wann;
+  st->wann = sp;
+  st->cur = dst;
+  st->second = 0;
+  conv_step(st, logrow, st->gcnorm1);
+}
+
+// wann_domega (wannierise.F90:2287-2569), non-SLWF gradient, one k per
+// workgroup, fused with the per-k partial sums Re<G,G> and Re<G,D_old>
+// needed by internal_search_direction.
+__global__ void __launch_bounds__(W90_NT) k_domega(Prob p) {
+  const State* st = p.st;
+  if (st->done) return;
+  const int nw = p.nw, nn = p.nn, nw2 = p.nw2;
+  const int k = blockIdx.x;
+  const int cur = st->cur;
+  const double* ln = p.ln[cur];
+  const double* rave = p.rave[cur];
+  const cplx* Mk = p.M[cur] + (size_t)k * nn * nw2;
+  __shared__ double red[W90_NT];
+  const double inv_nk = 1.0 / (double)p.nk;
+  const cplx mhalf_i = c_make(0.0, -0.5);
+  double gg = 0.0, gd = 0.0;
+  for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
+    const int m = e % nw, n = e / nw;
+    cplx acc = c_make(0.0, 0.0);
+    for (int b = 0; b < nn; ++b) {
+      const cplx* Mb = Mk + (size_t)b * nw2;
+      const double w = p.wb[b];
+      const double* bv = p.bk + ((size_t)k * nn + b) * 3;
+      const cplx mmn = Mb[m + n * nw];
+      const cplx mnm = Mb[n + m * nw];
+      const cplx mnn = Mb[n + n * nw];
+      const cplx mmm = Mb[m + m * nw];
+      const double* lnb = ln + ((size_t)k * nn + b) * nw;
+      const double qn = w * lnb[n];
+      const double qm = w * lnb[m];
+      const double rn = bv[0] * rave[0 + 3 * n] + bv[1] * rave[1 + 3 * n] + bv[2] * rave[2 + 3 * n];
+      const double rm = bv[0] * rave[0 + 3 * m] + bv[1] * rave[1 + 3 * m] + bv[2] * rave[2 + 3 * m];
+      const cplx cr_mn = c_mul(mmn, c_conj(mnn));
+      const cplx cr_nm = c_mul(mnm, c_conj(mmm));
+      const cplx crt_mn = c_div(mmn, mnn);
+      const cplx crt_nm = c_div(mnm, mmm);
+      // A[R] = (R - R^+)/2
+      acc = c_add(acc, c_scale(c_sub(cr_mn, c_conj(cr_nm)), w * 0.5));
+      // -S[T] = -(T + T^+)/2i, T_mn = Rt_mn q_n
+      acc = c_sub(acc, c_mul(c_add(c_scale(crt_mn, qn), c_conj(c_scale(crt_nm, qm))), mhalf_i));
+      acc = c_sub(acc, c_scale(c_mul(c_add(c_scale(crt_mn, rn), c_conj(c_scale(crt_nm, rm))), mhalf_i), w));
+    }
+    const cplx g = c_scale(c_scale(acc, inv_nk), 4.0);
+    p.G[(size_t)k * nw2 + e] = g;
+    gg += c_abs2(g);
+    const cplx dold = p.D[(size_t)k * nw2 + e];
+    gd += g.x * dold.x + g.y * dold.y;
+  }
+  gg = block_sum(gg, red);
+  gd = block_sum(gd, red);
+  if (threadIdx.x == 0) {
+    p.part_gg[k] = gg;
+    p.part_gd[k] = gd;
+  }
+}
+
+// internal_search_direction (wannierise.F90:1371-1525) without
+// preconditioning or noise: Fletcher-Reeves coefficient with its resets,
+// slope along D and the uphill handling.  D itself is formed by k_step.
+__global__ void __launch_bounds__(W90_NT) k_search(Prob p) {
+  State* st = p.st;
+  if (st->done) return;
+  __shared__ double red[W90_NT];
+  double gg = 0.0, gd = 0.0;
+  // fixed-order chunked sums over k
+  const int per = (p.nk + blockDim.x - 1) / blockDim.x;
+  const int kb = threadIdx.x * per, ke = min(p.nk, kb + per);
+  for (int k = kb; k < ke; ++k) {
+    gg += p.part_gg[k];
+    gd += p.part_gd[k];
+  }
+  gg = block_sum(gg, red);
+  gd = block_sum(gd, red);
+  if (threadIdx.x != 0) return;
+  const int iter = st->iter + 1;
+  st->second = 0;
+  st->gcnorm1 = gg;
+  double gcfac;
+  if (iter == 1 || st->ncg >= st->num_cg_steps) {
+    gcfac = 0.0;
+    st->ncg = 0;
+  } else {
+    if (st->gcnorm0 > 2.220446049250313e-16) {
+      gcfac = gg / st->gcnorm0;
+      if (gcfac > 3.0) {
+        gcfac = 0.0;
+        st->ncg = 0;
+      } else {
+        st->ncg += 1;
+      }
+    } else {
+      gcfac = 0.0;
+      st->ncg = 0;
+    }
+  }
+  st->gcnorm0 = gg;
+  const double w4 = 4.0 * st->wbtot;
+  double doda0 = -(gg + gcfac * gd) / w4;
+  double beta = gcfac, sign = 1.0;
+  if (doda0 > 0.0) {
+    if (st->ncg > 0) {
+      beta = 0.0;
+      st->ncg = 0;
+      gcfac = 0.0;
+      doda0 = -gg / w4;
+      if (doda0 > 0.0) {
+        sign = -1.0;
+        doda0 = -doda0;
+      }
+    } else {
+      sign = -sign;
+      doda0 = -doda0;
+    }
+  }
+  st->gcfac = gcfac;
+  st->beta = beta;
+  st->dir_sign = sign;
+  st->doda0 = doda0;
+  st->scale = (st->lfixstep ? st->fixed_step : st->trial_step) / w4;
+}
+
+// C = A * B for n x n column-major complex matrices held by one workgroup.
+__device__ void wg_gemm(cplx* C, const cplx* A, const cplx* B, int n) {
+  const int n2 = n * n;
+  for (int e = threadIdx.x; e < n2; e += blockDim.x) {
+    const int i = e % n, j = e / n;
+    const cplx* bc = B + (size_t)j * n;
+    cplx acc = c_make(0.0, 0.0);
+    for (int l = 0; l < n; ++l) acc = c_add(acc, c_mul(A[i + (size_t)l * n], bc[l]));
+    C[e] = acc;
+  }
+  __syncthreads();
+}
+
+// Retraction of one k: forms the search direction D (trial step only),
+// X = scale * D, E = exp(X) by scaling and squaring of a Taylor polynomial,
+// and U_new = U E.  X is anti-Hermitian, so exp(X) is the same unitary that
+// internal_new_u_and_m builds from the eigenvectors of i X
+// (wannierise.F90:1662-1689, 1729-1734).
+__global__ void __launch_bounds__(W90_NT) k_step(Prob p, int mode) {
+  const State* st = p.st;
+  if (!gate(st, mode)) return;
+  const int nw = p.nw, nw2 = p.nw2;
+  const int k = blockIdx.x;
+  const int cur = st->cur;
+  __shared__ double red[W90_NT];
+  cplx* Dk = p.D + (size_t)k * nw2;
+  cplx* X = p.W1 + (size_t)k * nw2;
+  cplx* P = p.W2 + (size_t)k * nw2;
+  cplx* Ek = p.E + (size_t)k * nw2;
+  const double scale = st->scale;
+  if (mode == MODE_TRIAL) {
+    const cplx* Gk = p.G + (size_t)k * nw2;
+    const double beta = st->beta, sign = st->dir_sign;
+    for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
+      cplx d = c_add(Gk[e], c_scale(Dk[e], beta));
+      d = c_scale(d, sign);
+      Dk[e] = d;
+      X[e] = c_scale(d, scale);
+    }
+  } else {
+    for (int e = threadIdx.x; e < nw2; e += blockDim.x) X[e] = c_scale(Dk[e], scale);
+  }
+  __syncthreads();
+  // 1-norm of X
+  double cs = 0.0;
+  for (int j = threadIdx.x; j < nw; j += blockDim.x) {
+    double s = 0.0;
+    for (int i = 0; i < nw; ++i) {
+      const cplx v = X[i + (size_t)j * nw];
+      s += sqrt(c_abs2(v));
+    }
+    cs = fmax(cs, s);
+  }
+  double nrm = block_max(cs, red);
+  // a non-finite step leaves a non-finite result; the loop stays bounded
+  if (!isfinite(nrm)) nrm = 0.0;
+  int sq = 0;
+  while (nrm > 0.5 && sq < 64) {
+    nrm *= 0.5;
+    ++sq;
+  }
+  // Taylor degree: nrm^(m+1)/(m+1)! < 1e-18
+  int m = 1;
+  {
+    double term = nrm;  // nrm^1/1!
+    while (m < 24) {
+      term = term * nrm / (double)(m + 1);
+      if (term < 1e-18) break;
+      ++m;
+    }
+  }
+  if (sq > 0) {
+    const double f = ldexp(1.0, -sq);
+    for (int e = threadIdx.x; e < nw2; e += blockDim.x) X[e] = c_scale(X[e], f);
+  }
+  __syncthreads();
+  // Horner: P = I + X/m; P = I + X P / j, j = m-1..1
+  const double invm = 1.0 / (double)m;
+  for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
+    cplx v = c_scale(X[e], invm);
+    if (e % nw == e / nw) v.x += 1.0;
+    P[e] = v;
+  }
+  __syncthreads();
+  for (int j = m - 1; j >= 1; --j) {
+    wg_gemm(Ek, X, P, nw);
+    const double invj = 1.0 / (double)j;
+    for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
+      cplx v = c_scale(Ek[e], invj);
+      if (e % nw == e / nw) v.x += 1.0;
+      P[e] = v;
+    }
+    __syncthreads();
+  }
+  // squaring: alternate between P and X (X no longer needed)
+  cplx* a = P;
+  cplx* b = X;
+  for (int s = 0; s < sq; ++s) {
+    wg_gemm(b, a, a, nw);
+    cplx* t = a;
+    a = b;
+    b = t;
+  }
+  for (int e = threadIdx.x; e < nw2; e += blockDim.x) Ek[e] = a[e];
+  __syncthreads();
+  // U_new = U_cur E
+  wg_gemm(p.U[1 - cur] + (size_t)k * nw2, p.U[cur] + (size_t)k * nw2, Ek, nw);
+}
+
+// wann_check_unitarity (wannierise.F90:2594-2661): per-k maximum deviation
+// of U U^+ and U^+ U from the identity.
+__global__ void __launch_bounds__(W90_NT) k_unitarity(Prob p) {
+  const int nw = p.nw, nw2 = p.nw2;
+  const int k = blockIdx.x;
+  const cplx* U = p.U[p.st->cur] + (size_t)k * nw2;
+  __shared__ double red[W90_NT];
+  double dev = 0.0;
+  for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
+    const int i = e % nw, j = e / nw;
+    cplx c1 = c_make(0, 0), c2 = c_make(0, 0);
+    for (int m = 0; m < nw; ++m) {
+      c1 = c_add(c1, c_mul(U[i + m * nw], c_conj(U[j + m * nw])));
+      c2 = c_add(c2, c_mul(U[m + j * nw], c_conj(U[m + i * nw])));
+    }
+    if (i == j) {
+      c1.x -= 1.0;
+      c2.x -= 1.0;
+    }
+    // NaN-propagating maximum: a NaN entry must fail the check
+    const double v = fmax(sqrt(c_abs2(c1)), sqrt(c_abs2(c2)));
+    if (!(v <= dev)) dev = isnan(v) ? INFINITY : fmax(v, dev);
+    if (isnan(c1.x) || isnan(c1.y) || isnan(c2.x) || isnan(c2.y)) dev = INFINITY;
+  }
+  dev = block_max(dev, red);
+  if (threadIdx.x == 0) p.unit[k] = dev;
+}
+
+// =============================================================== host side
+
+struct w90gpu_ctx {
+  int device = 0;
+  std::string err;
+};
+
+struct w90gpu_session {
+  w90gpu_ctx* ctx = nullptr;
+  w90gpu_control ctl{};
+  Prob p{};
+  int nw = 0, nk = 0, nn = 0;
+  double wbtot = 0.0;
+  hipStream_t stream = nullptr;
+  void* arena = nullptr;
+  size_t arena_bytes = 0;
+  cplx* M0 = nullptr;
+  cplx* U0 = nullptr;
+  State* st_host = nullptr;  // pinned
+  bool loaded = false;
+  bool prof = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+  size_t ev_used = 0;
+  double prof_ms = 0.0;
+  int prof_launches = 0;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  w90gpu_result last{};
+};
+
+namespace {
+
+thread_local std::string g_noctx_err;
+
+void set_error(w90gpu_ctx* ctx, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  else g_noctx_err = buf;
+}
+
+#define HIPCHK(ctx, call)                                                                 \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      set_error((ctx), "HIP error %s at %s:%d (%s)", hipGetErrorString(e_), __FILE__, __LINE__, \
+                #call);                                                                   \
+      return W90GPU_ERR_FATAL;                                                            \
+    }                                                                                     \
+  } while (0)
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct Layout {
+  size_t off[40];
+  int n = 0;
+  size_t total = 0;
+  size_t add(size_t bytes) {
+    const size_t o = total;
+    off[n++] = o;
+    total += align_up(bytes);
+    return o;
+  }
+};
+
+int npanels(int nw) { return (nw + W90_PANEL - 1) / W90_PANEL; }
+
+void chunking(int nk, int* kc, int* nch) {
+  int c = (nk + 127) / 128;
+  if (c < 1) c = 1;
+  *kc = c;
+  *nch = (nk + c - 1) / c;
+}
+
+// Compute the arena layout of a session; fills offsets in `lay` in a fixed
+// order that session_create mirrors.
+size_t session_layout(int nw, int nk, int nn, int num_iter, Layout& lay) {
+  const size_t nw2 = (size_t)nw * nw;
+  const size_t mb = nw2 * nn * nk * sizeof(cplx);
+  const size_t ub = nw2 * nk * sizeof(cplx);
+  const size_t lnb = (size_t)nk * nn * nw * sizeof(double);
+  int kc, nch;
+  chunking(nk, &kc, &nch);
+  lay.add(mb);   // M[0]
+  lay.add(mb);   // M[1]
+  lay.add(mb);   // M0
+  lay.add(ub);   // U[0]
+  lay.add(ub);   // U[1]
+  lay.add(ub);   // U0
+  lay.add(ub);   // G
+  lay.add(ub);   // D
+  lay.add(ub);   // E
+  lay.add(ub);   // W1
+  lay.add(ub);   // W2
+  lay.add(lnb);  // ln[0]
+  lay.add(lnb);  // ln[1]
+  lay.add(lnb);  // d2
+  lay.add((size_t)nk * nn * npanels(nw) * sizeof(double));  // part_od
+  lay.add((size_t)nk * nn * npanels(nw) * sizeof(double));  // part_all
+  lay.add((size_t)nch * (8 * nw + 8) * sizeof(double));     // chunk
+  lay.add((size_t)nk * sizeof(double));                     // part_gg
+  lay.add((size_t)nk * sizeof(double));                     // part_gd
+  lay.add(3 * (size_t)nw * sizeof(double));                 // rave[0]
+  lay.add(3 * (size_t)nw * sizeof(double));                 // rave[1]
+  lay.add((size_t)nw * sizeof(double));                     // r2ave[0]
+  lay.add((size_t)nw * sizeof(double));                     // r2ave[1]
+  lay.add((size_t)nk * sizeof(double));                     // unit
+  lay.add(4 * (size_t)(num_iter + 2) * sizeof(double));     // log
+  lay.add(sizeof(State));                                   // state
+  lay.add((size_t)nk * nn * sizeof(int));                   // nnl
+  lay.add((size_t)nk * nn * 3 * sizeof(double));            // bk
+  lay.add((size_t)nn * sizeof(double));                     // wb
+  return lay.total;
+}
+
+int validate(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90gpu_control* ctl) {
+  if (!prob || !ctl) {
+    set_error(ctx, "null problem or control");
+    return W90GPU_ERR_INPUT;
+  }
+  if (prob->num_wann < 1 || prob->num_wann > W90GPU_MAX_WANN) {
+    set_error(ctx, "num_wann=%d outside [1, %d]", prob->num_wann, W90GPU_MAX_WANN);
+    return W90GPU_ERR_INPUT;
+  }
+  if (prob->num_kpts < 1 || prob->nntot < 1 || prob->nntot > W90GPU_NUM_NNMAX) {
+    set_error(ctx, "invalid num_kpts=%d or nntot=%d", prob->num_kpts, prob->nntot);
+    return W90GPU_ERR_INPUT;
+  }
+  if (!prob->nnlist || !prob->bk || !prob->wb || !(prob->wbtot > 0.0)) {
+    set_error(ctx, "missing k-mesh tables (nnlist, bk, wb, wbtot)");
+    return W90GPU_ERR_INPUT;
+  }
+  for (size_t i = 0; i < (size_t)prob->num_kpts * prob->nntot; ++i) {
+    if (prob->nnlist[i] < 1 || prob->nnlist[i] > prob->num_kpts) {
+      set_error(ctx, "nnlist entry %d out of range [1, %d]", prob->nnlist[i], prob->num_kpts);
+      return W90GPU_ERR_INPUT;
+    }
+  }
+  if (ctl->precond || ctl->guiding_centres || ctl->selective_loc || ctl->conv_noise_amp > 0.0) {
+    set_error(ctx, "precond, guiding centres, selective localisation and conv_noise are not on this path");
+    return W90GPU_ERR_INPUT;
+  }
+  if (ctl->conv_window > W90GPU_MAX_CONV_WINDOW) {
+    set_error(ctx, "conv_window=%d exceeds %d", ctl->conv_window, W90GPU_MAX_CONV_WINDOW);
+    return W90GPU_ERR_INPUT;
+  }
+  if (ctl->num_iter < 0) {
+    set_error(ctx, "num_iter < 0");
+    return W90GPU_ERR_INPUT;
+  }
+  if (ctl->precision != W90GPU_PREC_FP64 && ctl->precision != W90GPU_PREC_MIXED) {
+    set_error(ctx, "unknown precision %d", ctl->precision);
+    return W90GPU_ERR_INPUT;
+  }
+  return W90GPU_OK;
+}
+
+// launch helpers ----------------------------------------------------------
+
+int launch_rotate(w90gpu_session* s, int mode) {
+  const unsigned nblk = (unsigned)((size_t)s->nk * s->nn * s->p.npan);
+  hipEvent_t a = nullptr, b = nullptr;
+  if (s->prof) {
+    if (s->ev_used >= s->evs.size()) {
+      hipEvent_t e0, e1;
+      HIPCHK(s->ctx, hipEventCreate(&e0));
+      HIPCHK(s->ctx, hipEventCreate(&e1));
+      s->evs.emplace_back(e0, e1);
+    }
+    a = s->evs[s->ev_used].first;
+    b = s->evs[s->ev_used].second;
+    ++s->ev_used;
+    HIPCHK(s->ctx, hipEventRecord(a, s->stream));
+  }
+  hipLaunchKernelGGL(k_rotate, dim3(nblk), dim3(W90_NT), 0, s->stream, s->p, mode);
+  HIPCHK(s->ctx, hipGetLastError());
+  if (s->prof) HIPCHK(s->ctx, hipEventRecord(b, s->stream));
+  return W90GPU_OK;
+}
+
+int launch_omega(w90gpu_session* s, int mode) {
+  int rc = launch_rotate(s, mode);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_omega_chunks, dim3(s->p.nch), dim3(W90_NT), 0, s->stream, s->p, mode);
+  HIPCHK(s->ctx, hipGetLastError());
+  hipLaunchKernelGGL(k_omega_final, dim3(1), dim3(W90_NT), 0, s->stream, s->p, mode);
+  HIPCHK(s->ctx, hipGetLastError());
+  return W90GPU_OK;
+}
+
+int launch_step(w90gpu_session* s, int mode) {
+  hipLaunchKernelGGL(k_step, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p, mode);
+  HIPCHK(s->ctx, hipGetLastError());
+  return W90GPU_OK;
+}
+
+int enqueue_init(w90gpu_session* s, int first_pass, double omega_invariant) {
+  w90gpu_ctx* ctx = s->ctx;
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s->stream, s->p, s->ctl, first_pass, omega_invariant);
+  HIPCHK(ctx, hipGetLastError());
+  const size_t mb = (size_t)s->nw * s->nw * s->nn * s->nk * sizeof(cplx);
+  const size_t ub = (size_t)s->nw * s->nw * s->nk * sizeof(cplx);
+  HIPCHK(ctx, hipMemcpyAsync(s->p.M[0], s->M0, mb, hipMemcpyDeviceToDevice, s->stream));
+  HIPCHK(ctx, hipMemcpyAsync(s->p.U[0], s->U0, ub, hipMemcpyDeviceToDevice, s->stream));
+  const size_t nd = (size_t)s->nw * s->nw * s->nk;
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(1024, (nd + 255) / 256)), dim3(256), 0,
+                     s->stream, s->p.D, nd);
+  HIPCHK(ctx, hipGetLastError());
+  return launch_omega(s, MODE_INIT);
+}
+
+int enqueue_iteration(w90gpu_session* s) {
+  w90gpu_ctx* ctx = s->ctx;
+  hipLaunchKernelGGL(k_domega, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p);
+  HIPCHK(ctx, hipGetLastError());
+  hipLaunchKernelGGL(k_search, dim3(1), dim3(W90_NT), 0, s->stream, s->p);
+  HIPCHK(ctx, hipGetLastError());
+  int rc;
+  if ((rc = launch_step(s, MODE_TRIAL))) return rc;
+  if ((rc = launch_omega(s, MODE_TRIAL))) return rc;
+  if (!s->ctl.lfixstep) {
+    if ((rc = launch_step(s, MODE_SECOND))) return rc;
+    if ((rc = launch_omega(s, MODE_SECOND))) return rc;
+  }
+  return W90GPU_OK;
+}
+
+int read_state(w90gpu_session* s) {
+  HIPCHK(s->ctx, hipMemcpyAsync(s->st_host, s->p.st, sizeof(State), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s->ctx, hipStreamSynchronize(s->stream));
+  return W90GPU_OK;
+}
+
+int finish_result(w90gpu_session* s, w90gpu_result* r, float elapsed_ms) {
+  int rc = read_state(s);
+  if (rc) return rc;
+  const State* st = s->st_host;
+  w90gpu_result res{};
+  res.omega_total = st->wann.om_tot;
+  res.omega_invariant = st->wann.om_i;
+  res.omega_d = st->wann.om_d;
+  res.omega_od = st->wann.om_od;
+  res.omega_tilde = st->wann.om_d + st->wann.om_od;
+  res.gcnorm1 = st->gcnorm1;
+  res.delta_omega = st->wann.om_tot - st->old.om_tot;
+  res.iterations = st->iter;
+  res.converged = st->converged;
+  res.solve_seconds = elapsed_ms * 1e-3;
+  // unitarity check of the final U (wann_check_unitarity, eps5)
+  hipLaunchKernelGGL(k_unitarity, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p);
+  HIPCHK(s->ctx, hipGetLastError());
+  std::vector<double> dev(s->nk);
+  HIPCHK(s->ctx, hipMemcpyAsync(dev.data(), s->p.unit, sizeof(double) * s->nk, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s->ctx, hipStreamSynchronize(s->stream));
+  double mx = 0.0;
+  for (double d : dev) mx = std::max(mx, d);
+  res.unitarity_ok = (mx <= 1e-5) ? 1 : 0;
+  res.status = W90GPU_OK;
+  if (!res.unitarity_ok) {
+    set_error(s->ctx, "wann_check_unitarity: final U deviates from unitarity by %.3e", mx);
+    res.status = W90GPU_ERR_FATAL;
+  }
+  if (s->prof) {
+    for (size_t i = 0; i < s->ev_used; ++i) {
+      float ms = 0.f;
+      HIPCHK(s->ctx, hipEventElapsedTime(&ms, s->evs[i].first, s->evs[i].second));
+      s->prof_ms += ms;
+      s->prof_launches += 1;
+    }
+    s->ev_used = 0;
+  }
+  s->last = res;
+  if (r) *r = res;
+  return res.status;
+}
+
+}  // namespace
+
+// =============================================================== C ABI
+
+extern "C" {
+
+const char* w90gpu_version(void) { return "w90gpu 0.1 (gfx950)"; }
+
+void w90gpu_control_defaults(w90gpu_control* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->num_iter = 100;
+  c->num_cg_steps = 5;
+  c->conv_window = -1;
+  c->lfixstep = 0;
+  c->conv_tol = 1.0e-10;
+  c->trial_step = 2.0;
+  c->fixed_step = -999.0;
+  c->conv_noise_amp = -1.0;
+  c->precision = W90GPU_PREC_FP64;
+}
+
+int w90gpu_create(int device, w90gpu_ctx** ctx) {
+  if (!ctx) return W90GPU_ERR_INPUT;
+  *ctx = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0) {
+    set_error(nullptr, "no HIP device available (%s)", hipGetErrorString(e));
+    return W90GPU_ERR_FATAL;
+  }
+  if (device < 0 || device >= ndev) {
+    set_error(nullptr, "device %d out of range (%d devices)", device, ndev);
+    return W90GPU_ERR_INPUT;
+  }
+  w90gpu_ctx* c = new (std::nothrow) w90gpu_ctx();
+  if (!c) return W90GPU_ERR_ALLOC;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete c;
+    set_error(nullptr, "hipSetDevice(%d) failed", device);
+    return W90GPU_ERR_FATAL;
+  }
+  *ctx = c;
+  return W90GPU_OK;
+}
+
+int w90gpu_destroy(w90gpu_ctx* ctx) {
+  delete ctx;
+  return W90GPU_OK;
+}
+
+const char* w90gpu_last_error(const w90gpu_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : g_noctx_err.c_str();
+}
+
+size_t w90gpu_session_bytes(int num_wann, int num_kpts, int nntot) {
+  Layout lay;
+  return session_layout(num_wann, num_kpts, nntot, 0, lay);
+}
+
+int w90gpu_session_create(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90gpu_control* ctl,
+                          w90gpu_session** out) {
+  if (!ctx || !out) return W90GPU_ERR_INPUT;
+  *out = nullptr;
+  int rc = validate(ctx, prob, ctl);
+  if (rc) return rc;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  w90gpu_session* s = new (std::nothrow) w90gpu_session();
+  if (!s) return W90GPU_ERR_ALLOC;
+  s->ctx = ctx;
+  s->ctl = *ctl;
+  s->nw = prob->num_wann;
+  s->nk = prob->num_kpts;
+  s->nn = prob->nntot;
+  s->wbtot = prob->wbtot;
+  Layout lay;
+  const size_t bytes = session_layout(s->nw, s->nk, s->nn, std::max(ctl->num_iter, 0), lay);
+  hipError_t e = hipMalloc(&s->arena, bytes);
+  if (e != hipSuccess) {
+    delete s;
+    set_error(ctx, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    return W90GPU_ERR_ALLOC;
+  }
+  s->arena_bytes = bytes;
+  char* a = (char*)s->arena;
+  int i = 0;
+  Prob& p = s->p;
+  p.nw = s->nw;
+  p.nk = s->nk;
+  p.nn = s->nn;
+  p.nw2 = s->nw * s->nw;
+  p.npan = npanels(s->nw);
+  chunking(s->nk, &p.kc, &p.nch);
+  p.M[0] = (cplx*)(a + lay.off[i++]);
+  p.M[1] = (cplx*)(a + lay.off[i++]);
+  s->M0 = (cplx*)(a + lay.off[i++]);
+  p.U[0] = (cplx*)(a + lay.off[i++]);
+  p.U[1] = (cplx*)(a + lay.off[i++]);
+  s->U0 = (cplx*)(a + lay.off[i++]);
+  p.G = (cplx*)(a + lay.off[i++]);
+  p.D = (cplx*)(a + lay.off[i++]);
+  p.E = (cplx*)(a + lay.off[i++]);
+  p.W1 = (cplx*)(a + lay.off[i++]);
+  p.W2 = (cplx*)(a + lay.off[i++]);
+  p.ln[0] = (double*)(a + lay.off[i++]);
+  p.ln[1] = (double*)(a + lay.off[i++]);
+  p.d2 = (double*)(a + lay.off[i++]);
+  p.part_od = (double*)(a + lay.off[i++]);
+  p.part_all = (double*)(a + lay.off[i++]);
+  p.chunk = (double*)(a + lay.off[i++]);
+  p.part_gg = (double*)(a + lay.off[i++]);
+  p.part_gd = (double*)(a + lay.off[i++]);
+  p.rave[0] = (double*)(a + lay.off[i++]);
+  p.rave[1] = (double*)(a + lay.off[i++]);
+  p.r2ave[0] = (double*)(a + lay.off[i++]);
+  p.r2ave[1] = (double*)(a + lay.off[i++]);
+  p.unit = (double*)(a + lay.off[i++]);
+  p.log = (double*)(a + lay.off[i++]);
+  p.st = (State*)(a + lay.off[i++]);
+  int* nnl = (int*)(a + lay.off[i++]);
+  double* bk = (double*)(a + lay.off[i++]);
+  double* wb = (double*)(a + lay.off[i++]);
+  p.nnl = nnl;
+  p.bk = bk;
+  p.wb = wb;
+
+  auto fail = [&](int code) {
+    if (s->stream) hipStreamDestroy(s->stream);
+    if (s->st_host) hipHostFree(s->st_host);
+    hipFree(s->arena);
+    delete s;
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error(ctx, "hipStreamCreate failed");
+    return fail(W90GPU_ERR_FATAL);
+  }
+  if (hipHostMalloc((void**)&s->st_host, sizeof(State), hipHostMallocDefault) != hipSuccess) {
+    s->st_host = nullptr;
+    set_error(ctx, "hipHostMalloc failed");
+    return fail(W90GPU_ERR_ALLOC);
+  }
+  if (hipEventCreate(&s->t0) != hipSuccess || hipEventCreate(&s->t1) != hipSuccess) {
+    set_error(ctx, "hipEventCreate failed");
+    return fail(W90GPU_ERR_FATAL);
+  }
+  // k-mesh tables: nnlist (nk,nn) Fortran -> [k][b] 0-based
+  std::vector<int> hn((size_t)s->nk * s->nn);
+  for (int k = 0; k < s->nk; ++k)
+    for (int b = 0; b < s->nn; ++b) hn[(size_t)k * s->nn + b] = prob->nnlist[k + (size_t)s->nk * b] - 1;
+  std::vector<double> hst(sizeof(State) / sizeof(double) + 1, 0.0);
+  State* st0 = (State*)hst.data();
+  st0->wbtot = s->wbtot;
+  if (hipMemcpy(nnl, hn.data(), hn.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(bk, prob->bk, (size_t)s->nk * s->nn * 3 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(wb, prob->wb, (size_t)s->nn * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p.st, st0, sizeof(State), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error(ctx, "upload of k-mesh tables failed");
+    return fail(W90GPU_ERR_FATAL);
+  }
+  *out = s;
+  return W90GPU_OK;
+}
+
+int w90gpu_session_destroy(w90gpu_session* s) {
+  if (!s) return W90GPU_OK;
+  if (s->stream) hipStreamSynchronize(s->stream);
+  for (auto& e : s->evs) {
+    hipEventDestroy(e.first);
+    hipEventDestroy(e.second);
+  }
+  if (s->t0) hipEventDestroy(s->t0);
+  if (s->t1) hipEventDestroy(s->t1);
+  if (s->st_host) hipHostFree(s->st_host);
+  if (s->stream) hipStreamDestroy(s->stream);
+  if (s->arena) hipFree(s->arena);
+  delete s;
+  return W90GPU_OK;
+}
+
+int w90gpu_session_load(w90gpu_session* s, const double* m_matrix, const double* u_matrix) {
+  if (!s || !m_matrix || !u_matrix) return W90GPU_ERR_INPUT;
+  const size_t mb = (size_t)s->nw * s->nw * s->nn * s->nk * sizeof(cplx);
+  const size_t ub = (size_t)s->nw * s->nw * s->nk * sizeof(cplx);
+  HIPCHK(s->ctx, hipMemcpyAsync(s->M0, m_matrix, mb, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(s->ctx, hipMemcpyAsync(s->U0, u_matrix, ub, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(s->ctx, hipStreamSynchronize(s->stream));
+  s->loaded = true;
+  return W90GPU_OK;
+}
+
+int w90gpu_session_launch(w90gpu_session* s) {
+  if (!s) return W90GPU_ERR_INPUT;
+  if (!s->loaded) {
+    set_error(s->ctx, "session has no loaded M/U");
+    return W90GPU_ERR_INPUT;
+  }
+  HIPCHK(s->ctx, hipEventRecord(s->t0, s->stream));
+  int rc = enqueue_init(s, 1, 0.0);
+  if (rc) return rc;
+  return W90GPU_OK;
+}
+
+int w90gpu_session_wait(w90gpu_session* s, w90gpu_result* r) {
+  if (!s) return W90GPU_ERR_INPUT;
+  // iterate in blocks, polling the done flag between blocks
+  const int block = 8;
+  int issued = 0;
+  int rc;
+  if ((rc = read_state(s))) return rc;
+  while (!s->st_host->done && issued < s->ctl.num_iter) {
+    const int nb = std::min(block, s->ctl.num_iter - issued);
+    for (int i = 0; i < nb; ++i)
+      if ((rc = enqueue_iteration(s))) return rc;
+    issued += nb;
+    if ((rc = read_state(s))) return rc;
+  }
+  HIPCHK(s->ctx, hipEventRecord(s->t1, s->stream));
+  HIPCHK(s->ctx, hipEventSynchronize(s->t1));
+  float ms = 0.f;
+  HIPCHK(s->ctx, hipEventElapsedTime(&ms, s->t0, s->t1));
+  return finish_result(s, r, ms);
+}
+
+int w90gpu_session_solve(w90gpu_session* s, w90gpu_result* r) {
+  int rc = w90gpu_session_launch(s);
+  if (rc) return rc;
+  return w90gpu_session_wait(s, r);
+}
+
+int w90gpu_session_fetch(w90gpu_session* s, double* m_matrix, double* u_matrix, double* centres,
+                         double* spreads) {
+  if (!s) return W90GPU_ERR_INPUT;
+  int rc = read_state(s);
+  if (rc) return rc;
+  const int cur = s->st_host->cur;
+  const size_t mb = (size_t)s->nw * s->nw * s->nn * s->nk * sizeof(cplx);
+  const size_t ub = (size_t)s->nw * s->nw * s->nk * sizeof(cplx);
+  if (m_matrix) HIPCHK(s->ctx, hipMemcpyAsync(m_matrix, s->p.M[cur], mb, hipMemcpyDeviceToHost, s->stream));
+  if (u_matrix) HIPCHK(s->ctx, hipMemcpyAsync(u_matrix, s->p.U[cur], ub, hipMemcpyDeviceToHost, s->stream));
+  std::vector<double> rave(3 * s->nw), r2(s->nw);
+  HIPCHK(s->ctx, hipMemcpyAsync(rave.data(), s->p.rave[cur], 3 * s->nw * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s->ctx, hipMemcpyAsync(r2.data(), s->p.r2ave[cur], s->nw * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s->ctx, hipStreamSynchronize(s->stream));
+  if (centres) std::memcpy(centres, rave.data(), 3 * s->nw * sizeof(double));
+  if (spreads)
+    for (int n = 0; n < s->nw; ++n) {
+      const double rx = rave[3 * n], ry = rave[3 * n + 1], rz = rave[3 * n + 2];
+      spreads[n] = r2[n] - (rx * rx + ry * ry + rz * rz);
+    }
+  return W90GPU_OK;
+}
+
+int w90gpu_session_history(w90gpu_session* s, double* out, int rows) {
+  if (!s || !out || rows < 0) return -W90GPU_ERR_INPUT;
+  if (read_state(s)) return -W90GPU_ERR_FATAL;
+  const int n = std::min(rows, s->st_host->iter + 1);
+  if (n <= 0) return 0;
+  if (hipMemcpy(out, s->p.log, sizeof(double) * 4 * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error(s->ctx, "history download failed");
+    return -W90GPU_ERR_FATAL;
+  }
+  return n;
+}
+
+int w90gpu_session_profile(w90gpu_session* s, int enable) {
+  if (!s) return W90GPU_ERR_INPUT;
+  s->prof = enable != 0;
+  s->prof_ms = 0.0;
+  s->prof_launches = 0;
+  s->ev_used = 0;
+  return W90GPU_OK;
+}
+
+int w90gpu_session_kernel_time(w90gpu_session* s, double* avg_ms, int* launches) {
+  if (!s) return W90GPU_ERR_INPUT;
+  if (avg_ms) *avg_ms = s->prof_launches ? s->prof_ms / s->prof_launches : 0.0;
+  if (launches) *launches = s->prof_launches;
+  return W90GPU_OK;
+}
+
+int w90gpu_wann_main(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90gpu_control* ctl,
+                     double* m_matrix, double* u_matrix, double* centres, double* spreads,
+                     w90gpu_result* result) {
+  if (!ctx) return W90GPU_ERR_INPUT;
+  if (!m_matrix || !u_matrix) {
+    set_error(ctx, "m_matrix and u_matrix must be associated");
+    return W90GPU_ERR_INPUT;
+  }
+  w90gpu_session* s = nullptr;
+  int rc = w90gpu_session_create(ctx, prob, ctl, &s);
+  if (rc) return rc;
+  rc = w90gpu_session_load(s, m_matrix, u_matrix);
+  if (!rc) rc = w90gpu_session_solve(s, result);
+  if (rc <= 0) {
+    const int rc2 = w90gpu_session_fetch(s, m_matrix, u_matrix, centres, spreads);
+    if (rc2) rc = rc2;
+  }
+  w90gpu_session_destroy(s);
+  return rc;
+}
+
+// ---------------------------------------------------- per-phase entry points
+
+int w90gpu_omega(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* m_matrix, int first_pass,
+                 double omega_invariant, double* rave, double* r2ave, double* omega) {
+  if (!ctx || !m_matrix) return W90GPU_ERR_INPUT;
+  w90gpu_control ctl;
+  w90gpu_control_defaults(&ctl);
+  ctl.num_iter = 0;
+  w90gpu_session* s = nullptr;
+  int rc = w90gpu_session_create(ctx, prob, &ctl, &s);
+  if (rc) return rc;
+  const size_t mb = (size_t)s->nw * s->nw * s->nn * s->nk * sizeof(cplx);
+  const size_t nd = (size_t)s->nw * s->nw * s->nk;
+  auto run = [&]() -> int {
+    HIPCHK(ctx, hipMemcpyAsync(s->p.M[0], m_matrix, mb, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s->stream, s->p, s->ctl, first_pass ? 1 : 0,
+                       omega_invariant);
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(1024, (nd + 255) / 256)), dim3(256), 0,
+                       s->stream, s->p.U[0], nd);
+    int r = launch_omega(s, MODE_INIT);
+    if (r) return r;
+    if ((r = read_state(s))) return r;
+    if (rave) HIPCHK(ctx, hipMemcpy(rave, s->p.rave[0], 3 * s->nw * sizeof(double), hipMemcpyDeviceToHost));
+    if (r2ave) HIPCHK(ctx, hipMemcpy(r2ave, s->p.r2ave[0], s->nw * sizeof(double), hipMemcpyDeviceToHost));
+    if (omega) {
+      omega[0] = s->st_host->wann.om_i;
+      omega[1] = s->st_host->wann.om_d;
+      omega[2] = s->st_host->wann.om_od;
+      omega[3] = s->st_host->wann.om_tot;
+    }
+    return W90GPU_OK;
+  };
+  rc = run();
+  w90gpu_session_destroy(s);
+  return rc;
+}
+
+int w90gpu_domega(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* m_matrix, double* cdodq) {
+  if (!ctx || !m_matrix || !cdodq) return W90GPU_ERR_INPUT;
+  w90gpu_control ctl;
+  w90gpu_control_defaults(&ctl);
+  ctl.num_iter = 1;
+  w90gpu_session* s = nullptr;
+  int rc = w90gpu_session_create(ctx, prob, &ctl, &s);
+  if (rc) return rc;
+  const size_t mb = (size_t)s->nw * s->nw * s->nn * s->nk * sizeof(cplx);
+  const size_t ub = (size_t)s->nw * s->nw * s->nk * sizeof(cplx);
+  const size_t nd = (size_t)s->nw * s->nw * s->nk;
+  auto run = [&]() -> int {
+    HIPCHK(ctx, hipMemcpyAsync(s->p.M[0], m_matrix, mb, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s->stream, s->p, s->ctl, 1, 0.0);
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(1024, (nd + 255) / 256)), dim3(256), 0,
+                       s->stream, s->p.D, nd);
+    int r = launch_omega(s, MODE_INIT);
+    if (r) return r;
+    hipLaunchKernelGGL(k_domega, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpyAsync(cdodq, s->p.G, ub, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    return W90GPU_OK;
+  };
+  rc = run();
+  w90gpu_session_destroy(s);
+  return rc;
+}
+
+int w90gpu_new_u_and_m(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* cdq, double* u_matrix,
+                       double* m_matrix) {
+  if (!ctx || !cdq || !u_matrix || !m_matrix) return W90GPU_ERR_INPUT;
+  w90gpu_control ctl;
+  w90gpu_control_defaults(&ctl);
+  ctl.num_iter = 1;
+  w90gpu_session* s = nullptr;
+  int rc = w90gpu_session_create(ctx, prob, &ctl, &s);
+  if (rc) return rc;
+  const size_t mb = (size_t)s->nw * s->nw * s->nn * s->nk * sizeof(cplx);
+  const size_t ub = (size_t)s->nw * s->nw * s->nk * sizeof(cplx);
+  auto run = [&]() -> int {
+    HIPCHK(ctx, hipMemcpyAsync(s->p.M[0], m_matrix, mb, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(ctx, hipMemcpyAsync(s->p.U[0], u_matrix, ub, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(ctx, hipMemcpyAsync(s->p.D, cdq, ub, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s->stream, s->p, s->ctl, 1, 0.0);
+    hipLaunchKernelGGL(k_set_direct, dim3(1), dim3(64), 0, s->stream, s->p, 1.0);
+    int r = launch_step(s, MODE_SECOND);
+    if (r) return r;
+    if ((r = launch_rotate(s, MODE_SECOND))) return r;
+    HIPCHK(ctx, hipMemcpyAsync(m_matrix, s->p.M[1], mb, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(ctx, hipMemcpyAsync(u_matrix, s->p.U[1], ub, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    return W90GPU_OK;
+  };
+  rc = run();
+  w90gpu_session_destroy(s);
+  return rc;
+}
+
+}  // extern "C"
