@@ -149,6 +149,33 @@ def test_pinned_prefix_matches_oracle(ctx, name):
     assert np.abs(r["history"][:, 0] - o["history"][:, 0]).max() <= 1e-8
 
 
+def test_benchmark_workload_matches_reference(ctx):
+    """The benchmark problem (fcc, 16 WFs, 512 k, nntot 8, 500 iterations),
+    regenerated from its seed: Omega after each of the 500 iterations, the
+    final Omega, centres, spreads and the stored U blocks equal the reference
+    run's.  The trajectory is insensitive to rounding (sens_u ~ 4e-14)."""
+    from wannier90_amd import capi, synth
+
+    c = Case("cfg2")
+    prob = synth.config(2)
+    assert np.array_equal(prob.kmesh.nnlist, c.kmesh.nnlist)
+    u0, m0 = synth.initial_state(prob)
+    ctl = c.gpu_control()
+    s = ctx.session(prob.kmesh, prob.num_wann, prob.num_kpts, ctl)
+    s.load(m0, u0)
+    res = s.solve()
+    _, u, cen, spr = s.fetch(want_m=False)
+    hist = s.history()
+    s.close()
+    assert res.status == capi.OK and res.iterations == 500
+    it = c.z["conv_iter"]
+    assert np.abs(hist[it, 0] - c.z["conv_omega"]).max() <= 1e-9
+    assert abs(res.omega_total - c.z["omega"][3]) <= 1e-8
+    assert np.abs(cen - c.z["centres"]).max() <= 1e-6
+    assert np.abs(spr - c.z["spreads"]).max() <= 1e-8
+    assert np.abs(u[:, :, :8] - c.z["u_final"]).max() <= 1e-7
+
+
 def test_closed_form_minimum_on_gpu(ctx):
     """bcc, 6 WFs: the solve reaches Omega = 0 with centres at the orbital
     positions, like the reference."""

@@ -294,4 +294,5 @@ def read_wout_final(path):
     om["iterations"] = int(conv[-1][0]) if conv else 0
     om["conv_iter"] = [int(r[0]) for r in conv]
     om["conv_omega"] = [float(r[3]) for r in conv]
+    om["conv_time"] = [float(r[4]) for r in conv]  # io_wallclocktime, seconds
     return c, s, om
