@@ -125,7 +125,12 @@ def test_final_state_matches_reference(ctx, name):
     if c.sens_u > 1e-3:
         pytest.skip("final U not reproducible by any FP64 implementation (sensitivity %.1e); "
                     "pinned by test_pinned_prefix_matches_oracle" % c.sens_u)
-    assert res.iterations == int(c.z["iterations"])
+    if c.well_conditioned():
+        assert res.iterations == int(c.z["iterations"])
+    else:
+        # |dOmega| <= conv_tol decides the stop; on a rounding-sensitive
+        # trajectory that test can flip one iteration either way
+        assert abs(res.iterations - int(c.z["iterations"])) <= 2
     assert np.abs(r["u"] - c.z["u_final"]).max() <= tol(1e-7, c.sens_u)
     assert np.abs(r["m"] - c.z["m_final"]).max() <= tol(1e-7, c.sens_u)
     assert np.abs(r["centres"] - c.z["centres"]).max() <= tol(1e-6, c.sens_centres)
@@ -204,6 +209,33 @@ def test_deterministic_and_resolvable(ctx):
     s.close()
     assert r1.omega_total == r2.omega_total and r1.iterations == r2.iterations
     assert np.array_equal(u1, u2) and np.array_equal(c1, c2)
+
+
+def test_graph_replays_match_eager_launches(ctx, monkeypatch):
+    """The captured block of iterations, replayed over several solves with
+    other eager GPU work and synchronisations in between, gives exactly the
+    result of launching every kernel directly."""
+    c = Case("cfg1_nowindow")
+
+    def solve():
+        s = ctx.session(c.kmesh, c.nw, c.nk, c.gpu_control())
+        s.load(c.m0, c.u0)
+        outs = []
+        for _ in range(3):
+            r = s.solve()
+            _, u, cen, _ = s.fetch()
+            outs.append((r.omega_total, r.iterations, u, cen, s.history()))
+            ctx.domega(c.kmesh, c.m0)  # unrelated eager work between replays
+        s.close()
+        return outs
+
+    monkeypatch.setenv("W90GPU_NO_GRAPH", "1")
+    eager = solve()[0]
+    monkeypatch.delenv("W90GPU_NO_GRAPH")
+    for om, it, u, cen, hist in solve():
+        assert (om, it) == eager[:2]
+        assert np.array_equal(u, eager[2]) and np.array_equal(cen, eager[3])
+        assert np.array_equal(hist, eager[4])
 
 
 def test_num_iter_zero_returns_initial_state(ctx):

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -53,12 +54,18 @@ __device__ __forceinline__ cplx c_div(cplx a, cplx b) {
 }
 __device__ __forceinline__ double c_abs2(cplx a) { return a.x * a.x + a.y * a.y; }
 
-// Block-wide sum of one double per thread; deterministic (fixed tree).
+// Block-wide sum of one double per thread; deterministic (fixed tree), any
+// block size: the part above the largest power of two is folded in first.
 __device__ double block_sum(double v, double* red) {
   const int t = threadIdx.x;
+  const int n = blockDim.x;
+  int p2 = 1;
+  while (p2 * 2 <= n) p2 *= 2;
   red[t] = v;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+  if (t < n - p2) red[t] += red[t + p2];
+  __syncthreads();
+  for (int s = p2 / 2; s > 0; s >>= 1) {
     if (t < s) red[t] += red[t + s];
     __syncthreads();
   }
@@ -69,9 +76,14 @@ __device__ double block_sum(double v, double* red) {
 
 __device__ double block_max(double v, double* red) {
   const int t = threadIdx.x;
+  const int n = blockDim.x;
+  int p2 = 1;
+  while (p2 * 2 <= n) p2 *= 2;
   red[t] = v;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+  if (t < n - p2) red[t] = fmax(red[t], red[t + p2]);
+  __syncthreads();
+  for (int s = p2 / 2; s > 0; s >>= 1) {
     if (t < s) red[t] = fmax(red[t], red[t + s]);
     __syncthreads();
   }
@@ -114,7 +126,8 @@ struct State {
 // Device pointers of one problem, passed by value to every kernel.
 struct Prob {
   int nw, nk, nn, npan;
-  int nch, kc, nw2, pad;
+  int psz, nw2, lds, ngroup;  // psz: length of a per-k partial vector; lds: per-k LDS kernels;
+                              // ngroup: k-ranges of the first reduction level
   const int* nnl;     // [k][b] 0-based neighbour
   const double* bk;   // [k][b][3]
   const double* wb;   // [b]
@@ -129,7 +142,8 @@ struct Prob {
   double* d2;         // [k][b][n] |M_nn|^2 of the last rotated buffer
   double* part_od;    // [k][b][panel]
   double* part_all;   // [k][b][panel]
-  double* chunk;      // [chunk][8*nw+8]
+  double* kpart;      // [k][psz] per-k partial sums of wann_omega (layout: DiagAcc below)
+  double* gpart;      // [ngroup][psz] sums of the per-k vectors over contiguous k-ranges
   double* part_gg;    // [k]
   double* part_gd;    // [k]
   double* rave[2];    // [3][nw] per buffer (Fortran rave(3,nw): ind fastest)
@@ -267,115 +281,368 @@ __global__ void __launch_bounds__(W90_NT) k_rotate(Prob p, int mode) {
   }
 }
 
-// First reduction stage of wann_omega over a chunk of k-points.  Per
-// Wannier function n it accumulates
-//   S_rave[ind] = sum wb b_ind ln,  S_r2 = sum wb (1 - |M_nn|^2 + ln^2),
-//   A = sum wb (ln + b.r0)^2,  V[ind] = sum wb b_ind (ln + b.r0)
-// with r0 the centres of the source buffer, plus the scalars
-//   S_od = sum wb od,  S_i = sum wb (nw - sum|M|^2),  B_ij = sum wb b_i b_j.
-// Omega_D = (1/nk) sum_n [A + 2 d.V + d^T B d] with d = rave - r0 is then
-// exact algebra for sum wb (ln + b.rave)^2 (wannierise.F90:2260-2275) without
-// a second pass over ln.
-__global__ void __launch_bounds__(W90_NT) k_omega_chunks(Prob p, int mode) {
+// Per-k partial sums of wann_omega.  For Wannier function n, over the b of
+// one k:
+//   [q*nw + n]     S_rave[q] = sum_b wb b_q ln_n              (q = 0,1,2)
+//   [3*nw + n]     S_r2      = sum_b wb (1 - |M_nn|^2 + ln_n^2)
+//   [4*nw + n]     A         = sum_b wb (ln_n + b.r0_n)^2
+//   [(5+q)*nw + n] V[q]      = sum_b wb b_q (ln_n + b.r0_n)
+// and the scalars
+//   [8*nw + 0]     S_od      = sum_b wb sum_{m!=n} |M_mn|^2
+//   [8*nw + 1]     S_i       = sum_b wb (nw - sum_mn |M_mn|^2)
+//   [8*nw + 2..7]  B         = sum_b wb b b^T  (xx, yy, zz, xy, yz, zx)
+// with ln_n = Im ln M_nn and r0 the centres of the source buffer.  With
+// d = rave - r0, Omega_D = (1/nk) sum_n [A + 2 d.V + d^T B d] is exact
+// algebra for (1/nk) sum wb (ln + b.rave)^2 (wannierise.F90:2260-2275), and
+// r0 close to rave keeps it free of cancellation, so no second pass over ln
+// is needed once rave is known.
+struct DiagAcc {
+  double r[3], r2, a, v[3];
+};
+
+__device__ __forceinline__ void diag_add(DiagAcc& d, double w, const double* bv, double l, double d2,
+                                         double r0x, double r0y, double r0z) {
+  d.r[0] += w * bv[0] * l;
+  d.r[1] += w * bv[1] * l;
+  d.r[2] += w * bv[2] * l;
+  d.r2 += w * (1.0 - d2 + l * l);
+  const double x = l + (bv[0] * r0x + bv[1] * r0y + bv[2] * r0z);
+  d.a += w * x * x;
+  d.v[0] += w * bv[0] * x;
+  d.v[1] += w * bv[1] * x;
+  d.v[2] += w * bv[2] * x;
+}
+
+__device__ __forceinline__ void diag_store(double* out, int nw, int n, const DiagAcc& d) {
+  out[0 * nw + n] = d.r[0];
+  out[1 * nw + n] = d.r[1];
+  out[2 * nw + n] = d.r[2];
+  out[3 * nw + n] = d.r2;
+  out[4 * nw + n] = d.a;
+  out[5 * nw + n] = d.v[0];
+  out[6 * nw + n] = d.v[1];
+  out[7 * nw + n] = d.v[2];
+}
+
+// Scalars of the per-k vector that depend only on the k-mesh: B = sum wb b b^T.
+__device__ void kpart_bmat(const Prob& p, int k, double* out) {
+  double b6[6] = {0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < p.nn; ++b) {
+    const double w = p.wb[b];
+    const double* bv = p.bk + ((size_t)k * p.nn + b) * 3;
+    b6[0] += w * bv[0] * bv[0];
+    b6[1] += w * bv[1] * bv[1];
+    b6[2] += w * bv[2] * bv[2];
+    b6[3] += w * bv[0] * bv[1];
+    b6[4] += w * bv[1] * bv[2];
+    b6[5] += w * bv[2] * bv[0];
+  }
+  for (int u = 0; u < 6; ++u) out[2 + u] = b6[u];
+}
+
+// Per-k partial vector from the outputs of the panel rotation kernel
+// (ln, |M_nn|^2 and the per-panel sums), for num_wann above the LDS kernels.
+__global__ void __launch_bounds__(W90_NT) k_kpart(Prob p, int mode) {
   const State* st = p.st;
   if (!gate(st, mode)) return;
   const int nw = p.nw, nn = p.nn;
+  const int k = blockIdx.x;
   const int cur = st->cur;
   const int dst = (mode == MODE_INIT) ? cur : 1 - cur;
   const double* ln = p.ln[dst];
-  const double* r0 = (mode == MODE_INIT) ? nullptr : p.rave[cur];
-  const int c = blockIdx.x;
-  const int kbeg = c * p.kc, kend = min(p.nk, kbeg + p.kc);
-  const int nitems = (kend - kbeg) * nn;
-  const int psz = 8 * nw + 8;
-  double* out = p.chunk + (size_t)c * psz;
+  double* out = p.kpart + (size_t)k * p.psz;
+  for (int n = threadIdx.x; n < nw; n += blockDim.x) {
+    double r0x = 0, r0y = 0, r0z = 0;
+    if (mode != MODE_INIT) {
+      r0x = p.rave[cur][0 + 3 * n];
+      r0y = p.rave[cur][1 + 3 * n];
+      r0z = p.rave[cur][2 + 3 * n];
+    }
+    DiagAcc d = {};
+    for (int b = 0; b < nn; ++b) {
+      const size_t kb = (size_t)k * nn + b;
+      diag_add(d, p.wb[b], p.bk + kb * 3, ln[kb * nw + n], p.d2[kb * nw + n], r0x, r0y, r0z);
+    }
+    diag_store(out, nw, n, d);
+  }
+  if (threadIdx.x == 0) {
+    double sod = 0.0, si = 0.0;
+    for (int b = 0; b < nn; ++b) {
+      const size_t kb = (size_t)k * nn + b;
+      double od = 0.0, al = 0.0;
+      for (int q = 0; q < p.npan; ++q) {
+        od += p.part_od[kb * p.npan + q];
+        al += p.part_all[kb * p.npan + q];
+      }
+      sod += p.wb[b] * od;
+      si += p.wb[b] * ((double)nw - al);
+    }
+    out[8 * nw + 0] = sod;
+    out[8 * nw + 1] = si;
+    kpart_bmat(p, k, out + 8 * nw);
+  }
+}
+
+// C = A B of n x n column-major complex matrices in LDS; thread slots as in
+// the calling kernel (element e = tid + s*W90_NT).
+template <int NWM>
+__device__ __forceinline__ void lds_gemm(cplx* C, const cplx* A, const cplx* B, int n) {
+  constexpr int SL = (NWM * NWM + W90_NT - 1) / W90_NT;
+  const int n2 = n * n, ld = n + 1;  // padded leading dimension
+#pragma unroll
+  for (int s = 0; s < SL; ++s) {
+    const int e = threadIdx.x + s * W90_NT;
+    if (e < n2) {
+      const int i = e % n, j = e / n;
+      cplx acc = c_make(0.0, 0.0);
+      for (int l = 0; l < n; ++l) acc = c_add(acc, c_mul(A[i + l * ld], B[l + j * ld]));
+      C[i + j * ld] = acc;
+    }
+  }
+  __syncthreads();
+}
+
+// M rotation with the omega partial sums, one workgroup per k, all operands
+// in LDS (num_wann <= NWM).  E_k is loaded once for the nn blocks of k;
+// each block M(k,b) and E_{k+b} are staged in LDS, T = E_k^+ M and
+// R = T E_{k+b} are formed there, R is written once and reduced in
+// registers into the per-k partial vector.  MODE_INIT only reduces.
+// (internal_new_u_and_m :1737-1748; wann_omega :2034-2090, :2214-2275)
+template <int NWM>
+__global__ void __launch_bounds__(W90_NT) k_rotate_k(Prob p, int mode) {
+  const State* st = p.st;
+  if (!gate(st, mode)) return;
+  constexpr int SL = (NWM * NWM + W90_NT - 1) / W90_NT;
+  const int nw = p.nw, nw2 = p.nw2, nn = p.nn;
+  const int k = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int cur = st->cur;
+  const bool rot = (mode != MODE_INIT);
+  const int dst = rot ? 1 - cur : cur;
+  // padded column-major LDS blocks (leading dimension nw + 1), E_k stored
+  // conjugate-transposed: consecutive lanes read consecutive addresses
+  __shared__ cplx EkH[NWM * (NWM + 1)];
+  __shared__ cplx Eb[NWM * (NWM + 1)];
+  __shared__ cplx Ms[NWM * (NWM + 1)];
+  __shared__ cplx T[NWM * (NWM + 1)];
+  const int ld = nw + 1;
   __shared__ double red[W90_NT];
-  __shared__ double acc8[8 * W90GPU_MAX_WANN];
 
-  // per-n accumulation: thread (n, sub)
-  const int nsub = blockDim.x / nw;
-  const int n = threadIdx.x % nw;
-  const int sub = threadIdx.x / nw;
-  double s_r[3] = {0, 0, 0}, s_r2 = 0, s_a = 0, s_v[3] = {0, 0, 0};
-  if (sub < nsub) {
-    double rx = 0, ry = 0, rz = 0;
-    if (r0) {
-      rx = r0[0 + 3 * n];
-      ry = r0[1 + 3 * n];
-      rz = r0[2 + 3 * n];
-    }
-    for (int it = sub; it < nitems; it += nsub) {
-      const int kb = kbeg * nn + it;
-      const int b = kb % nn;
-      const double w = p.wb[b];
-      const double* bv = p.bk + (size_t)kb * 3;
-      const double l = ln[(size_t)kb * nw + n];
-      const double d2 = p.d2[(size_t)kb * nw + n];
-      s_r[0] += w * bv[0] * l;
-      s_r[1] += w * bv[1] * l;
-      s_r[2] += w * bv[2] * l;
-      s_r2 += w * (1.0 - d2 + l * l);
-      const double x = l + (bv[0] * rx + bv[1] * ry + bv[2] * rz);
-      s_a += w * x * x;
-      s_v[0] += w * bv[0] * x;
-      s_v[1] += w * bv[1] * x;
-      s_v[2] += w * bv[2] * x;
+  DiagAcc acc[SL];
+  double r0[SL][3];
+#pragma unroll
+  for (int s = 0; s < SL; ++s) {
+    acc[s] = DiagAcc{};
+    r0[s][0] = r0[s][1] = r0[s][2] = 0.0;
+    const int e = tid + s * W90_NT;
+    if (rot && e < nw2 && e % nw == e / nw) {
+      const int n = e % nw;
+      r0[s][0] = p.rave[cur][0 + 3 * n];
+      r0[s][1] = p.rave[cur][1 + 3 * n];
+      r0[s][2] = p.rave[cur][2 + 3 * n];
     }
   }
-  // combine the nsub partial sums of each n in fixed order
-  for (int q = 0; q < 8; ++q) {
-    double v;
-    switch (q) {
-      case 0: v = s_r[0]; break;
-      case 1: v = s_r[1]; break;
-      case 2: v = s_r[2]; break;
-      case 3: v = s_r2; break;
-      case 4: v = s_a; break;
-      case 5: v = s_v[0]; break;
-      case 6: v = s_v[1]; break;
-      default: v = s_v[2]; break;
-    }
-    red[threadIdx.x] = v;
-    __syncthreads();
-    if (threadIdx.x < nw) {
-      double s = 0.0;
-      for (int u = 0; u < nsub; ++u) s += red[threadIdx.x + u * nw];
-      acc8[q * nw + threadIdx.x] = s;
-    }
-    __syncthreads();
+  if (rot) {
+    const cplx* ek = p.E + (size_t)k * nw2;
+    for (int e = tid; e < nw2; e += W90_NT) EkH[(e / nw) + (e % nw) * ld] = c_conj(ek[e]);
   }
-  for (int e = threadIdx.x; e < 8 * nw; e += blockDim.x) out[e] = acc8[e];
-
-  // scalar sums over (k,b)
-  double sod = 0, si = 0, bxx = 0, byy = 0, bzz = 0, bxy = 0, byz = 0, bzx = 0;
-  for (int it = threadIdx.x; it < nitems; it += blockDim.x) {
-    const int kb = kbeg * nn + it;
-    const int b = kb % nn;
+  double s_od = 0.0, s_all = 0.0;
+  for (int b = 0; b < nn; ++b) {
+    const size_t kb = (size_t)k * nn + b;
+    const cplx* src = p.M[cur] + kb * nw2;
+    __syncthreads();
+    for (int e = tid; e < nw2; e += W90_NT) Ms[(e % nw) + (e / nw) * ld] = src[e];
+    if (rot) {
+      const cplx* eb = p.E + (size_t)p.nnl[kb] * nw2;
+      for (int e = tid; e < nw2; e += W90_NT) Eb[(e % nw) + (e / nw) * ld] = eb[e];
+    }
+    __syncthreads();
+    if (rot) {
+#pragma unroll
+      for (int s = 0; s < SL; ++s) {
+        const int e = tid + s * W90_NT;
+        if (e < nw2) {
+          const int i = e % nw, j = e / nw;
+          cplx a = c_make(0.0, 0.0);
+          for (int l = 0; l < nw; ++l) a = c_add(a, c_mul(EkH[i + l * ld], Ms[l + j * ld]));
+          T[i + j * ld] = a;
+        }
+      }
+      __syncthreads();
+    }
     const double w = p.wb[b];
-    double od = 0, al = 0;
-    for (int q = 0; q < p.npan; ++q) {
-      od += p.part_od[(size_t)kb * p.npan + q];
-      al += p.part_all[(size_t)kb * p.npan + q];
+    const double* bv = p.bk + kb * 3;
+    cplx* out = p.M[dst] + kb * nw2;
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int e = tid + s * W90_NT;
+      if (e < nw2) {
+        const int i = e % nw, j = e / nw;
+        cplx r;
+        if (rot) {
+          r = c_make(0.0, 0.0);
+          for (int l = 0; l < nw; ++l) r = c_add(r, c_mul(T[i + l * ld], Eb[l + j * ld]));
+          out[e] = r;
+        } else {
+          r = Ms[i + j * ld];
+        }
+        const double a2 = c_abs2(r);
+        s_all += w * a2;
+        if (i != j) {
+          s_od += w * a2;
+        } else {
+          const double l = atan2(r.y, r.x);
+          p.ln[dst][kb * nw + i] = l;
+          diag_add(acc[s], w, bv, l, a2, r0[s][0], r0[s][1], r0[s][2]);
+        }
+      }
     }
-    sod += w * od;
-    si += w * ((double)nw - al);
-    const double* bv = p.bk + (size_t)kb * 3;
-    bxx += w * bv[0] * bv[0];
-    byy += w * bv[1] * bv[1];
-    bzz += w * bv[2] * bv[2];
-    bxy += w * bv[0] * bv[1];
-    byz += w * bv[1] * bv[2];
-    bzx += w * bv[2] * bv[0];
   }
-  double r;
-  r = block_sum(sod, red); if (threadIdx.x == 0) out[8 * nw + 0] = r;
-  r = block_sum(si, red);  if (threadIdx.x == 0) out[8 * nw + 1] = r;
-  r = block_sum(bxx, red); if (threadIdx.x == 0) out[8 * nw + 2] = r;
-  r = block_sum(byy, red); if (threadIdx.x == 0) out[8 * nw + 3] = r;
-  r = block_sum(bzz, red); if (threadIdx.x == 0) out[8 * nw + 4] = r;
-  r = block_sum(bxy, red); if (threadIdx.x == 0) out[8 * nw + 5] = r;
-  r = block_sum(byz, red); if (threadIdx.x == 0) out[8 * nw + 6] = r;
-  r = block_sum(bzx, red); if (threadIdx.x == 0) out[8 * nw + 7] = r;
+  double* outk = p.kpart + (size_t)k * p.psz;
+#pragma unroll
+  for (int s = 0; s < SL; ++s) {
+    const int e = tid + s * W90_NT;
+    if (e < nw2 && e % nw == e / nw) diag_store(outk, nw, e % nw, acc[s]);
+  }
+  s_od = block_sum(s_od, red);
+  s_all = block_sum(s_all, red);
+  if (tid == 0) {
+    double wsum = 0.0;
+    for (int b = 0; b < nn; ++b) wsum += p.wb[b];
+    outk[8 * nw + 0] = s_od;
+    outk[8 * nw + 1] = (double)nw * wsum - s_all;
+    kpart_bmat(p, k, outk + 8 * nw);
+  }
+}
+
+// M rotation with the omega partial sums for small blocks (num_wann <= NWM
+// <= 16): one workgroup per k with one wavefront per neighbour b, so the nn
+// rotations of a k proceed concurrently.  E_k, the nn blocks M(k,b), the nn
+// E_{k+b} and the nn intermediates T are all staged in LDS by one round of
+// loads; the diagonal of each rotated block goes through LDS to threads
+// n < num_wann, which sum it over b in order.
+// LDS matrices are column-major with leading dimension nw + 1: consecutive
+// lanes (consecutive rows i) read consecutive addresses, and the columns read
+// as broadcasts by the four 16-lane groups of a wavefront land in different
+// banks.  E_k is stored conjugate-transposed so that E_k^+ M reads rows of it
+// the same way.  LDS is sized by nntot at launch (dynamic shared memory).
+template <int NWM>
+__host__ __device__ constexpr size_t rotate_kb_lds(int nn) {
+  return sizeof(cplx) * (size_t)NWM * (NWM + 1) * (1 + 2 * (size_t)nn) +
+         sizeof(double) * (2 * (size_t)nn * NWM + 64 * (size_t)nn);
+}
+
+template <int NWM>
+__global__ void __launch_bounds__(64 * W90GPU_NUM_NNMAX) k_rotate_kb(Prob p, int mode) {
+  const State* st = p.st;
+  if (!gate(st, mode)) return;
+  constexpr int B2 = NWM * NWM;
+  constexpr int BP = NWM * (NWM + 1);  // padded block
+  constexpr int SL = (B2 + 63) / 64;
+  const int nw = p.nw, nw2 = p.nw2, nn = p.nn;
+  const int ld = nw + 1;
+  const int k = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int b = tid >> 6;  // wavefront index = neighbour index
+  const int lane = tid & 63;
+  const int cur = st->cur;
+  const bool rot = (mode != MODE_INIT);
+  const int dst = rot ? 1 - cur : cur;
+  extern __shared__ double lds_raw[];
+  cplx* EkH = reinterpret_cast<cplx*>(lds_raw);   // conj(E_k)^T
+  cplx* Ms = EkH + BP + (size_t)b * 2 * BP;        // M(k,b), then T = E_k^+ M(k,b)
+  cplx* Eb = Ms + BP;                              // E_{k+b}
+  double* lnL = reinterpret_cast<double*>(EkH + BP + (size_t)nn * 2 * BP);  // [b][NWM]
+  double* d2L = lnL + nn * NWM;
+  double* red = d2L + nn * NWM;  // 64 * nn doubles for block_sum
+  const size_t kb = (size_t)k * nn + b;
+  if (rot)
+    for (int e = tid; e < nw2; e += blockDim.x) {
+      const int l = e % nw, i = e / nw;
+      EkH[i + l * ld] = c_conj(p.E[(size_t)k * nw2 + e]);
+    }
+  {
+    const cplx* src = p.M[cur] + kb * nw2;
+    for (int e = lane; e < nw2; e += 64) Ms[(e % nw) + (e / nw) * ld] = src[e];
+    if (rot) {
+      const cplx* eb = p.E + (size_t)p.nnl[kb] * nw2;
+      for (int e = lane; e < nw2; e += 64) Eb[(e % nw) + (e / nw) * ld] = eb[e];
+    }
+  }
+  __syncthreads();
+  if (rot) {
+    cplx tr[SL];
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int e = lane + 64 * s;
+      tr[s] = c_make(0.0, 0.0);
+      if (e < nw2) {
+        const int i = e % nw, j = e / nw;
+        cplx a = c_make(0.0, 0.0);
+        for (int l = 0; l < nw; ++l) a = c_add(a, c_mul(EkH[i + l * ld], Ms[l + j * ld]));
+        tr[s] = a;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int e = lane + 64 * s;
+      if (e < nw2) Ms[(e % nw) + (e / nw) * ld] = tr[s];
+    }
+    __syncthreads();
+  }
+  const double w = p.wb[b];
+  double s_od = 0.0, s_all = 0.0;
+  cplx* out = p.M[dst] + kb * nw2;
+#pragma unroll
+  for (int s = 0; s < SL; ++s) {
+    const int e = lane + 64 * s;
+    if (e < nw2) {
+      const int i = e % nw, j = e / nw;
+      cplx r;
+      if (rot) {
+        r = c_make(0.0, 0.0);
+        for (int l = 0; l < nw; ++l) r = c_add(r, c_mul(Ms[i + l * ld], Eb[l + j * ld]));
+        out[e] = r;
+      } else {
+        r = Ms[i + j * ld];
+      }
+      const double a2 = c_abs2(r);
+      s_all += w * a2;
+      if (i != j) {
+        s_od += w * a2;
+      } else {
+        const double l = atan2(r.y, r.x);
+        p.ln[dst][kb * nw + i] = l;
+        lnL[b * NWM + i] = l;
+        d2L[b * NWM + i] = a2;
+      }
+    }
+  }
+  __syncthreads();
+  double* outk = p.kpart + (size_t)k * p.psz;
+  if (tid < nw) {
+    const int n = tid;
+    double r0x = 0, r0y = 0, r0z = 0;
+    if (rot) {
+      r0x = p.rave[cur][0 + 3 * n];
+      r0y = p.rave[cur][1 + 3 * n];
+      r0z = p.rave[cur][2 + 3 * n];
+    }
+    DiagAcc d = {};
+    for (int bb = 0; bb < nn; ++bb)
+      diag_add(d, p.wb[bb], p.bk + ((size_t)k * nn + bb) * 3, lnL[bb * NWM + n], d2L[bb * NWM + n], r0x, r0y, r0z);
+    diag_store(outk, nw, n, d);
+  }
+  s_od = block_sum(s_od, red);
+  s_all = block_sum(s_all, red);
+  if (tid == 0) {
+    double wsum = 0.0;
+    for (int bb = 0; bb < nn; ++bb) wsum += p.wb[bb];
+    outk[8 * nw + 0] = s_od;
+    outk[8 * nw + 1] = (double)nw * wsum - s_all;
+    kpart_bmat(p, k, outk + 8 * nw);
+  }
 }
 
 // internal_test_convergence (wannierise.F90:1076-1164) with the noise branch
@@ -413,29 +680,62 @@ __device__ void conv_step(State* st, double* logrow, double gc) {
 // logic that follows each wann_omega call in wann_main: initial state
 // (:452-470), internal_optimal_step (:1528-1596) after the trial step and
 // the acceptance of the trial or parabolic step (:660-701).
-__global__ void __launch_bounds__(W90_NT) k_omega_final(Prob p, int mode) {
+#define W90_FINAL_NT 1024
+#define W90_MAX_GROUPS 32
+
+// First level of the omega reduction: workgroup g sums the per-k vectors of
+// a contiguous k-range, in k order (rows read coalesced along v).
+__global__ void __launch_bounds__(W90_NT) k_omega_group(Prob p, int mode) {
+  const State* st = p.st;
+  if (!gate(st, mode)) return;
+  const int g = blockIdx.x, G = p.ngroup, psz = p.psz;
+  const int k0 = (int)(((long)g * p.nk) / G), k1 = (int)(((long)(g + 1) * p.nk) / G);
+  for (int v = threadIdx.x; v < psz; v += blockDim.x) {
+    double s = 0.0;
+    for (int k = k0; k < k1; ++k) s += p.kpart[(size_t)k * psz + v];
+    p.gpart[(size_t)g * psz + v] = s;
+  }
+}
+
+__global__ void __launch_bounds__(W90_FINAL_NT) k_omega_final(Prob p, int mode) {
   State* st = p.st;
   if (!gate(st, mode)) return;
-  const int nw = p.nw;
+  const int nw = p.nw, nk = p.nk, psz = p.psz;
   const int cur = st->cur;
   const int dst = (mode == MODE_INIT) ? cur : 1 - cur;
-  const int psz = 8 * nw + 8;
-  const double rnk = (double)p.nk;
-  __shared__ double red[W90_NT];
+  const double rnk = (double)nk;
+  const int tid = threadIdx.x;
+  __shared__ double red[W90_FINAL_NT];
+  __shared__ double sums[8 * W90GPU_MAX_WANN + 8];
   __shared__ double sc[8];
 
+  // sum of the group vectors: VB values per pass (coalesced along a row),
+  // J lanes over groups, lanes combined in fixed order
+  const int VB = min(psz, W90_FINAL_NT);
+  const int J = W90_FINAL_NT / VB;
+  const int G = p.ngroup;
+  for (int v0 = 0; v0 < psz; v0 += VB) {
+    const int v = v0 + tid % VB;
+    const int j = tid / VB;
+    double s = 0.0;
+    if (j < J && v < psz)
+      for (int g = j; g < G; g += J) s += p.gpart[(size_t)g * psz + v];
+    red[tid] = s;
+    __syncthreads();
+    if (tid < VB && v0 + tid < psz) {
+      double t = 0.0;
+      for (int jj = 0; jj < J; ++jj) t += red[tid + jj * VB];
+      sums[v0 + tid] = t;
+    }
+    __syncthreads();
+  }
+
   double omd_n = 0.0;
-  for (int n = threadIdx.x; n < nw; n += blockDim.x) {
-    double q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int c = 0; c < p.nch; ++c) {
-      const double* pc = p.chunk + (size_t)c * psz;
-      for (int u = 0; u < 8; ++u) q[u] += pc[u * nw + n];
-    }
-    double bb[6] = {0, 0, 0, 0, 0, 0};
-    for (int c = 0; c < p.nch; ++c) {
-      const double* pc = p.chunk + (size_t)c * psz + 8 * nw + 2;
-      for (int u = 0; u < 6; ++u) bb[u] += pc[u];
-    }
+  for (int n = tid; n < nw; n += blockDim.x) {
+    double q[8];
+    for (int u = 0; u < 8; ++u) q[u] = sums[u * nw + n];
+    double bb[6];
+    for (int u = 0; u < 6; ++u) bb[u] = sums[8 * nw + 2 + u];
     const double rx = -q[0] / rnk, ry = -q[1] / rnk, rz = -q[2] / rnk;
     p.rave[dst][0 + 3 * n] = rx;
     p.rave[dst][1 + 3 * n] = ry;
@@ -455,14 +755,8 @@ __global__ void __launch_bounds__(W90_NT) k_omega_final(Prob p, int mode) {
   // Omega_D summed over n in a fixed tree
   const double omd = block_sum(omd_n, red);
   if (threadIdx.x == 0) {
-    double sod = 0, si = 0;
-    for (int c = 0; c < p.nch; ++c) {
-      const double* pc = p.chunk + (size_t)c * psz + 8 * nw;
-      sod += pc[0];
-      si += pc[1];
-    }
-    sc[0] = sod;
-    sc[1] = si;
+    sc[0] = sums[8 * nw + 0];
+    sc[1] = sums[8 * nw + 1];
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
@@ -564,7 +858,7 @@ __global__ void __launch_bounds__(W90_NT) k_domega(Prob p) {
   const double* rave = p.rave[cur];
   const cplx* Mk = p.M[cur] + (size_t)k * nn * nw2;
   __shared__ double red[W90_NT];
-  const double inv_nk = 1.0 / (double)p.nk;
+  const double rnk = (double)p.nk;
   const cplx mhalf_i = c_make(0.0, -0.5);
   double gg = 0.0, gd = 0.0;
   for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
@@ -593,7 +887,7 @@ __global__ void __launch_bounds__(W90_NT) k_domega(Prob p) {
       acc = c_sub(acc, c_mul(c_add(c_scale(crt_mn, qn), c_conj(c_scale(crt_nm, qm))), mhalf_i));
       acc = c_sub(acc, c_scale(c_mul(c_add(c_scale(crt_mn, rn), c_conj(c_scale(crt_nm, rm))), mhalf_i), w));
     }
-    const cplx g = c_scale(c_scale(acc, inv_nk), 4.0);
+    const cplx g = c_scale(make_double2(acc.x / rnk, acc.y / rnk), 4.0);
     p.G[(size_t)k * nw2 + e] = g;
     gg += c_abs2(g);
     const cplx dold = p.D[(size_t)k * nw2 + e];
@@ -602,6 +896,85 @@ __global__ void __launch_bounds__(W90_NT) k_domega(Prob p) {
   gg = block_sum(gg, red);
   gd = block_sum(gd, red);
   if (threadIdx.x == 0) {
+    p.part_gg[k] = gg;
+    p.part_gd[k] = gd;
+  }
+}
+
+// wann_domega with each block M(k,b) staged in LDS (num_wann <= NWM): the
+// transposed and diagonal reads of the gradient formula hit LDS, and the
+// scaled phases wb Im ln M_nn and b.rave_n of the k are precomputed.
+template <int NWM>
+__global__ void __launch_bounds__(W90_NT) k_domega_k(Prob p) {
+  const State* st = p.st;
+  if (st->done) return;
+  constexpr int SL = (NWM * NWM + W90_NT - 1) / W90_NT;
+  const int nw = p.nw, nn = p.nn, nw2 = p.nw2;
+  const int k = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int cur = st->cur;
+  __shared__ cplx Ms[NWM * (NWM + 1)];  // padded, leading dimension nw + 1
+  const int ld = nw + 1;
+  __shared__ double q[W90GPU_NUM_NNMAX * NWM];
+  __shared__ double rb[W90GPU_NUM_NNMAX * NWM];
+  __shared__ double red[W90_NT];
+  const double* rave = p.rave[cur];
+  for (int t = tid; t < nn * nw; t += W90_NT) {
+    const int b = t / nw, n = t % nw;
+    const size_t kb = (size_t)k * nn + b;
+    const double* bv = p.bk + kb * 3;
+    q[t] = p.wb[b] * p.ln[cur][kb * nw + n];
+    rb[t] = bv[0] * rave[0 + 3 * n] + bv[1] * rave[1 + 3 * n] + bv[2] * rave[2 + 3 * n];
+  }
+  cplx acc[SL];
+#pragma unroll
+  for (int s = 0; s < SL; ++s) acc[s] = c_make(0.0, 0.0);
+  const cplx mhalf_i = c_make(0.0, -0.5);
+  for (int b = 0; b < nn; ++b) {
+    const cplx* src = p.M[cur] + ((size_t)k * nn + b) * nw2;
+    __syncthreads();
+    for (int e = tid; e < nw2; e += W90_NT) Ms[(e % nw) + (e / nw) * ld] = src[e];
+    __syncthreads();
+    const double w = p.wb[b];
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int e = tid + s * W90_NT;
+      if (e < nw2) {
+        const int m = e % nw, n = e / nw;
+        const cplx mmn = Ms[m + n * ld];
+        const cplx mnm = Ms[n + m * ld];
+        const cplx mnn = Ms[n + n * ld];
+        const cplx mmm = Ms[m + m * ld];
+        const double qn = q[b * nw + n], qm = q[b * nw + m];
+        const double rn = rb[b * nw + n], rm = rb[b * nw + m];
+        const cplx cr_mn = c_mul(mmn, c_conj(mnn));
+        const cplx cr_nm = c_mul(mnm, c_conj(mmm));
+        const cplx crt_mn = c_div(mmn, mnn);
+        const cplx crt_nm = c_div(mnm, mmm);
+        cplx a = acc[s];
+        a = c_add(a, c_scale(c_sub(cr_mn, c_conj(cr_nm)), w * 0.5));
+        a = c_sub(a, c_mul(c_add(c_scale(crt_mn, qn), c_conj(c_scale(crt_nm, qm))), mhalf_i));
+        a = c_sub(a, c_scale(c_mul(c_add(c_scale(crt_mn, rn), c_conj(c_scale(crt_nm, rm))), mhalf_i), w));
+        acc[s] = a;
+      }
+    }
+  }
+  const double rnk = (double)p.nk;
+  double gg = 0.0, gd = 0.0;
+#pragma unroll
+  for (int s = 0; s < SL; ++s) {
+    const int e = tid + s * W90_NT;
+    if (e < nw2) {
+      const cplx g = c_scale(make_double2(acc[s].x / rnk, acc[s].y / rnk), 4.0);
+      p.G[(size_t)k * nw2 + e] = g;
+      gg += c_abs2(g);
+      const cplx dold = p.D[(size_t)k * nw2 + e];
+      gd += g.x * dold.x + g.y * dold.y;
+    }
+  }
+  gg = block_sum(gg, red);
+  gd = block_sum(gd, red);
+  if (tid == 0) {
     p.part_gg[k] = gg;
     p.part_gd[k] = gd;
   }
@@ -672,6 +1045,8 @@ __global__ void __launch_bounds__(W90_NT) k_search(Prob p) {
   st->scale = (st->lfixstep ? st->fixed_step : st->trial_step) / w4;
 }
 
+__device__ __forceinline__ void taylor_plan(double nrm, int* sq_out, int* m_out);
+
 // C = A * B for n x n column-major complex matrices held by one workgroup.
 __device__ void wg_gemm(cplx* C, const cplx* A, const cplx* B, int n) {
   const int n2 = n * n;
@@ -725,24 +1100,8 @@ __global__ void __launch_bounds__(W90_NT) k_step(Prob p, int mode) {
     }
     cs = fmax(cs, s);
   }
-  double nrm = block_max(cs, red);
-  // a non-finite step leaves a non-finite result; the loop stays bounded
-  if (!isfinite(nrm)) nrm = 0.0;
-  int sq = 0;
-  while (nrm > 0.5 && sq < 64) {
-    nrm *= 0.5;
-    ++sq;
-  }
-  // Taylor degree: nrm^(m+1)/(m+1)! < 1e-18
-  int m = 1;
-  {
-    double term = nrm;  // nrm^1/1!
-    while (m < 24) {
-      term = term * nrm / (double)(m + 1);
-      if (term < 1e-18) break;
-      ++m;
-    }
-  }
+  int sq, m;
+  taylor_plan(block_max(cs, red), &sq, &m);
   if (sq > 0) {
     const double f = ldexp(1.0, -sq);
     for (int e = threadIdx.x; e < nw2; e += blockDim.x) X[e] = c_scale(X[e], f);
@@ -779,6 +1138,110 @@ __global__ void __launch_bounds__(W90_NT) k_step(Prob p, int mode) {
   __syncthreads();
   // U_new = U_cur E
   wg_gemm(p.U[1 - cur] + (size_t)k * nw2, p.U[cur] + (size_t)k * nw2, Ek, nw);
+}
+
+// Degree and squarings of the Taylor approximant of exp(X) for ||X||_1 = nrm:
+// X is scaled by 2^-sq to norm <= 1/8, and the degree m is the smallest with
+// (nrm/2^sq)^(m+1)/(m+1)! < 1e-18, far below the FP64 rounding of the
+// result (the exponential of an anti-Hermitian matrix is unitary, norm 1).
+__device__ __forceinline__ void taylor_plan(double nrm, int* sq_out, int* m_out) {
+  if (!isfinite(nrm)) nrm = 0.0;  // a non-finite step yields a non-finite U
+  int sq = 0;
+  while (nrm > 0.125 && sq < 64) {
+    nrm *= 0.5;
+    ++sq;
+  }
+  int m = 1;
+  double term = nrm;
+  while (m < 24) {
+    term = term * nrm / (double)(m + 1);
+    if (term < 1e-18) break;
+    ++m;
+  }
+  *sq_out = sq;
+  *m_out = m;
+}
+
+// Retraction of one k with every matrix in LDS (num_wann <= NWM): forms D
+// (trial step), X = scale D, E = exp(X) and U_new = U E.
+template <int NWM>
+__global__ void __launch_bounds__(W90_NT) k_step_k(Prob p, int mode) {
+  const State* st = p.st;
+  if (!gate(st, mode)) return;
+  constexpr int SL = (NWM * NWM + W90_NT - 1) / W90_NT;
+  const int nw = p.nw, nw2 = p.nw2;
+  const int k = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int cur = st->cur;
+  // padded column-major LDS matrices, leading dimension nw + 1
+  __shared__ cplx X[NWM * (NWM + 1)];
+  __shared__ cplx P[NWM * (NWM + 1)];
+  __shared__ cplx T[NWM * (NWM + 1)];
+  const int ld = nw + 1;
+  __shared__ double red[W90_NT];
+  cplx* Dk = p.D + (size_t)k * nw2;
+  const double scale = st->scale;
+  if (mode == MODE_TRIAL) {
+    const cplx* Gk = p.G + (size_t)k * nw2;
+    const double beta = st->beta, sign = st->dir_sign;
+    for (int e = tid; e < nw2; e += W90_NT) {
+      cplx d = c_scale(c_add(Gk[e], c_scale(Dk[e], beta)), sign);
+      Dk[e] = d;
+      X[(e % nw) + (e / nw) * ld] = c_scale(d, scale);
+    }
+  } else {
+    for (int e = tid; e < nw2; e += W90_NT) X[(e % nw) + (e / nw) * ld] = c_scale(Dk[e], scale);
+  }
+  __syncthreads();
+  double cs = 0.0;
+  for (int j = tid; j < nw; j += W90_NT) {
+    double s = 0.0;
+    for (int i = 0; i < nw; ++i) s += sqrt(c_abs2(X[i + j * ld]));
+    cs = fmax(cs, s);
+  }
+  int sq, m;
+  taylor_plan(block_max(cs, red), &sq, &m);
+  const double f = ldexp(1.0, -sq);
+  const double invm = 1.0 / (double)m;
+  for (int e = tid; e < nw2; e += W90_NT) {
+    const int q = (e % nw) + (e / nw) * ld;
+    const cplx x = c_scale(X[q], f);
+    X[q] = x;
+    cplx v = c_scale(x, invm);
+    if (e % nw == e / nw) v.x += 1.0;
+    P[q] = v;
+  }
+  __syncthreads();
+  for (int j = m - 1; j >= 1; --j) {
+    lds_gemm<NWM>(T, X, P, nw);
+    const double invj = 1.0 / (double)j;
+    for (int e = tid; e < nw2; e += W90_NT) {
+      const int q = (e % nw) + (e / nw) * ld;
+      cplx v = c_scale(T[q], invj);
+      if (e % nw == e / nw) v.x += 1.0;
+      P[q] = v;
+    }
+    __syncthreads();
+  }
+  cplx* a = P;
+  cplx* b = T;
+  for (int s = 0; s < sq; ++s) {
+    lds_gemm<NWM>(b, a, a, nw);
+    cplx* t = a;
+    a = b;
+    b = t;
+  }
+  cplx* Ek = p.E + (size_t)k * nw2;
+  const cplx* Uc = p.U[cur] + (size_t)k * nw2;
+  for (int e = tid; e < nw2; e += W90_NT) {
+    const int q = (e % nw) + (e / nw) * ld;
+    Ek[e] = a[q];
+    X[q] = Uc[e];
+  }
+  __syncthreads();
+  lds_gemm<NWM>(b, X, a, nw);
+  cplx* Un = p.U[1 - cur] + (size_t)k * nw2;
+  for (int e = tid; e < nw2; e += W90_NT) Un[e] = b[(e % nw) + (e / nw) * ld];
 }
 
 // wann_check_unitarity (wannierise.F90:2594-2661): per-k maximum deviation
@@ -836,7 +1299,17 @@ struct w90gpu_session {
   int prof_launches = 0;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   w90gpu_result last{};
+  // a captured block of kGraphBlock iterations, replayed until done
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_failed = false;
+  // pinned copies of the done flag of the two blocks in flight
+  int* flags_host = nullptr;
+  hipEvent_t flag_ev[2] = {nullptr, nullptr};
 };
+
+// iterations per captured block; the solve is polled once per block
+constexpr int kGraphBlock = 8;
 
 namespace {
 
@@ -878,12 +1351,9 @@ struct Layout {
 
 int npanels(int nw) { return (nw + W90_PANEL - 1) / W90_PANEL; }
 
-void chunking(int nk, int* kc, int* nch) {
-  int c = (nk + 127) / 128;
-  if (c < 1) c = 1;
-  *kc = c;
-  *nch = (nk + c - 1) / c;
-}
+// LDS-resident per-k kernels cover num_wann up to 32 (four 32x32 complex
+// blocks = 64 KiB of LDS); larger num_wann use the panel kernels.
+constexpr int kLdsMaxWann = 32;
 
 // Compute the arena layout of a session; fills offsets in `lay` in a fixed
 // order that session_create mirrors.
@@ -892,8 +1362,6 @@ size_t session_layout(int nw, int nk, int nn, int num_iter, Layout& lay) {
   const size_t mb = nw2 * nn * nk * sizeof(cplx);
   const size_t ub = nw2 * nk * sizeof(cplx);
   const size_t lnb = (size_t)nk * nn * nw * sizeof(double);
-  int kc, nch;
-  chunking(nk, &kc, &nch);
   lay.add(mb);   // M[0]
   lay.add(mb);   // M[1]
   lay.add(mb);   // M0
@@ -910,7 +1378,8 @@ size_t session_layout(int nw, int nk, int nn, int num_iter, Layout& lay) {
   lay.add(lnb);  // d2
   lay.add((size_t)nk * nn * npanels(nw) * sizeof(double));  // part_od
   lay.add((size_t)nk * nn * npanels(nw) * sizeof(double));  // part_all
-  lay.add((size_t)nch * (8 * nw + 8) * sizeof(double));     // chunk
+  lay.add((size_t)nk * (8 * nw + 8) * sizeof(double));      // kpart
+  lay.add((size_t)W90_MAX_GROUPS * (8 * nw + 8) * sizeof(double));  // gpart
   lay.add((size_t)nk * sizeof(double));                     // part_gg
   lay.add((size_t)nk * sizeof(double));                     // part_gd
   lay.add(3 * (size_t)nw * sizeof(double));                 // rave[0]
@@ -970,8 +1439,9 @@ int validate(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90gpu_control* 
 
 // launch helpers ----------------------------------------------------------
 
+// The rotation of M and the reduction of its diagonal into the per-k
+// partial sums of wann_omega (one fused kernel when the blocks fit in LDS).
 int launch_rotate(w90gpu_session* s, int mode) {
-  const unsigned nblk = (unsigned)((size_t)s->nk * s->nn * s->p.npan);
   hipEvent_t a = nullptr, b = nullptr;
   if (s->prof) {
     if (s->ev_used >= s->evs.size()) {
@@ -985,24 +1455,58 @@ int launch_rotate(w90gpu_session* s, int mode) {
     ++s->ev_used;
     HIPCHK(s->ctx, hipEventRecord(a, s->stream));
   }
-  hipLaunchKernelGGL(k_rotate, dim3(nblk), dim3(W90_NT), 0, s->stream, s->p, mode);
+  const dim3 perk(s->nk);
+  if (s->nw <= 8) {
+    hipLaunchKernelGGL(k_rotate_kb<8>, perk, dim3(64 * s->nn), rotate_kb_lds<8>(s->nn), s->stream, s->p,
+                       mode);
+  } else if (s->nw <= 16) {
+    hipLaunchKernelGGL(k_rotate_kb<16>, perk, dim3(64 * s->nn), rotate_kb_lds<16>(s->nn), s->stream, s->p,
+                       mode);
+  } else if (s->nw <= kLdsMaxWann) {
+    hipLaunchKernelGGL(k_rotate_k<kLdsMaxWann>, perk, dim3(W90_NT), 0, s->stream, s->p, mode);
+  } else {
+    const unsigned nblk = (unsigned)((size_t)s->nk * s->nn * s->p.npan);
+    hipLaunchKernelGGL(k_rotate, dim3(nblk), dim3(W90_NT), 0, s->stream, s->p, mode);
+  }
   HIPCHK(s->ctx, hipGetLastError());
   if (s->prof) HIPCHK(s->ctx, hipEventRecord(b, s->stream));
+  if (s->nw > kLdsMaxWann) {
+    hipLaunchKernelGGL(k_kpart, perk, dim3(W90_NT), 0, s->stream, s->p, mode);
+    HIPCHK(s->ctx, hipGetLastError());
+  }
   return W90GPU_OK;
 }
 
 int launch_omega(w90gpu_session* s, int mode) {
   int rc = launch_rotate(s, mode);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_omega_chunks, dim3(s->p.nch), dim3(W90_NT), 0, s->stream, s->p, mode);
+  hipLaunchKernelGGL(k_omega_group, dim3(s->p.ngroup), dim3(W90_NT), 0, s->stream, s->p, mode);
   HIPCHK(s->ctx, hipGetLastError());
-  hipLaunchKernelGGL(k_omega_final, dim3(1), dim3(W90_NT), 0, s->stream, s->p, mode);
+  hipLaunchKernelGGL(k_omega_final, dim3(1), dim3(W90_FINAL_NT), 0, s->stream, s->p, mode);
   HIPCHK(s->ctx, hipGetLastError());
   return W90GPU_OK;
 }
 
 int launch_step(w90gpu_session* s, int mode) {
-  hipLaunchKernelGGL(k_step, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p, mode);
+  if (s->nw <= 16) {
+    hipLaunchKernelGGL(k_step_k<16>, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p, mode);
+  } else if (s->nw <= kLdsMaxWann) {
+    hipLaunchKernelGGL(k_step_k<kLdsMaxWann>, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p, mode);
+  } else {
+    hipLaunchKernelGGL(k_step, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p, mode);
+  }
+  HIPCHK(s->ctx, hipGetLastError());
+  return W90GPU_OK;
+}
+
+int launch_domega(w90gpu_session* s) {
+  if (s->nw <= 16) {
+    hipLaunchKernelGGL(k_domega_k<16>, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p);
+  } else if (s->nw <= kLdsMaxWann) {
+    hipLaunchKernelGGL(k_domega_k<kLdsMaxWann>, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p);
+  } else {
+    hipLaunchKernelGGL(k_domega, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p);
+  }
   HIPCHK(s->ctx, hipGetLastError());
   return W90GPU_OK;
 }
@@ -1024,17 +1528,47 @@ int enqueue_init(w90gpu_session* s, int first_pass, double omega_invariant) {
 
 int enqueue_iteration(w90gpu_session* s) {
   w90gpu_ctx* ctx = s->ctx;
-  hipLaunchKernelGGL(k_domega, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p);
-  HIPCHK(ctx, hipGetLastError());
+  int rc;
+  if ((rc = launch_domega(s))) return rc;
   hipLaunchKernelGGL(k_search, dim3(1), dim3(W90_NT), 0, s->stream, s->p);
   HIPCHK(ctx, hipGetLastError());
-  int rc;
   if ((rc = launch_step(s, MODE_TRIAL))) return rc;
   if ((rc = launch_omega(s, MODE_TRIAL))) return rc;
   if (!s->ctl.lfixstep) {
     if ((rc = launch_step(s, MODE_SECOND))) return rc;
     if ((rc = launch_omega(s, MODE_SECOND))) return rc;
   }
+  return W90GPU_OK;
+}
+
+// Capture kGraphBlock iterations of the loop into an executable graph.  The
+// kernels read the problem from their by-value argument and the loop state
+// from device memory, so one graph serves every block and every solve of
+// the session.  On a capture failure the session keeps launching directly.
+int capture_block(w90gpu_session* s) {
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    s->graph_failed = true;
+    return W90GPU_OK;
+  }
+  int rc = W90GPU_OK;
+  for (int i = 0; i < kGraphBlock && rc == W90GPU_OK; ++i) rc = enqueue_iteration(s);
+  const hipError_t e = hipStreamEndCapture(s->stream, &g);
+  if (rc != W90GPU_OK || e != hipSuccess || !g) {
+    (void)hipGetLastError();
+    if (g) (void)hipGraphDestroy(g);
+    s->graph_failed = true;
+    return W90GPU_OK;
+  }
+  if (hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipGraphDestroy(g);
+    s->gexec = nullptr;
+    s->graph_failed = true;
+    return W90GPU_OK;
+  }
+  s->graph = g;
   return W90GPU_OK;
 }
 
@@ -1155,6 +1689,11 @@ int w90gpu_session_create(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90
   int rc = validate(ctx, prob, ctl);
   if (rc) return rc;
   HIPCHK(ctx, hipSetDevice(ctx->device));
+  // dynamic LDS above 64 KiB must be enabled per kernel
+  HIPCHK(ctx, hipFuncSetAttribute((const void*)k_rotate_kb<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)rotate_kb_lds<16>(W90GPU_NUM_NNMAX)));
+  HIPCHK(ctx, hipFuncSetAttribute((const void*)k_rotate_kb<8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)rotate_kb_lds<8>(W90GPU_NUM_NNMAX)));
   w90gpu_session* s = new (std::nothrow) w90gpu_session();
   if (!s) return W90GPU_ERR_ALLOC;
   s->ctx = ctx;
@@ -1180,7 +1719,8 @@ int w90gpu_session_create(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90
   p.nn = s->nn;
   p.nw2 = s->nw * s->nw;
   p.npan = npanels(s->nw);
-  chunking(s->nk, &p.kc, &p.nch);
+  p.psz = 8 * s->nw + 8;
+  p.lds = s->nw <= kLdsMaxWann ? 1 : 0;
   p.M[0] = (cplx*)(a + lay.off[i++]);
   p.M[1] = (cplx*)(a + lay.off[i++]);
   s->M0 = (cplx*)(a + lay.off[i++]);
@@ -1197,7 +1737,9 @@ int w90gpu_session_create(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90
   p.d2 = (double*)(a + lay.off[i++]);
   p.part_od = (double*)(a + lay.off[i++]);
   p.part_all = (double*)(a + lay.off[i++]);
-  p.chunk = (double*)(a + lay.off[i++]);
+  p.kpart = (double*)(a + lay.off[i++]);
+  p.gpart = (double*)(a + lay.off[i++]);
+  p.ngroup = std::min(s->nk, W90_MAX_GROUPS);
   p.part_gg = (double*)(a + lay.off[i++]);
   p.part_gd = (double*)(a + lay.off[i++]);
   p.rave[0] = (double*)(a + lay.off[i++]);
@@ -1215,10 +1757,7 @@ int w90gpu_session_create(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90
   p.wb = wb;
 
   auto fail = [&](int code) {
-    if (s->stream) hipStreamDestroy(s->stream);
-    if (s->st_host) hipHostFree(s->st_host);
-    hipFree(s->arena);
-    delete s;
+    w90gpu_session_destroy(s);
     return code;
   };
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1230,9 +1769,16 @@ int w90gpu_session_create(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90
     set_error(ctx, "hipHostMalloc failed");
     return fail(W90GPU_ERR_ALLOC);
   }
-  if (hipEventCreate(&s->t0) != hipSuccess || hipEventCreate(&s->t1) != hipSuccess) {
+  if (hipEventCreate(&s->t0) != hipSuccess || hipEventCreate(&s->t1) != hipSuccess ||
+      hipEventCreateWithFlags(&s->flag_ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->flag_ev[1], hipEventDisableTiming) != hipSuccess) {
     set_error(ctx, "hipEventCreate failed");
     return fail(W90GPU_ERR_FATAL);
+  }
+  if (hipHostMalloc((void**)&s->flags_host, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    s->flags_host = nullptr;
+    set_error(ctx, "hipHostMalloc failed");
+    return fail(W90GPU_ERR_ALLOC);
   }
   // k-mesh tables: nnlist (nk,nn) Fortran -> [k][b] 0-based
   std::vector<int> hn((size_t)s->nk * s->nn);
@@ -1254,16 +1800,21 @@ int w90gpu_session_create(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90
 
 int w90gpu_session_destroy(w90gpu_session* s) {
   if (!s) return W90GPU_OK;
-  if (s->stream) hipStreamSynchronize(s->stream);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (auto& e : s->evs) {
-    hipEventDestroy(e.first);
-    hipEventDestroy(e.second);
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
   }
-  if (s->t0) hipEventDestroy(s->t0);
-  if (s->t1) hipEventDestroy(s->t1);
-  if (s->st_host) hipHostFree(s->st_host);
-  if (s->stream) hipStreamDestroy(s->stream);
-  if (s->arena) hipFree(s->arena);
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  for (hipEvent_t e : s->flag_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (s->t0) (void)hipEventDestroy(s->t0);
+  if (s->t1) (void)hipEventDestroy(s->t1);
+  if (s->flags_host) (void)hipHostFree(s->flags_host);
+  if (s->st_host) (void)hipHostFree(s->st_host);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  if (s->arena) (void)hipFree(s->arena);
   delete s;
   return W90GPU_OK;
 }
@@ -1293,17 +1844,42 @@ int w90gpu_session_launch(w90gpu_session* s) {
 
 int w90gpu_session_wait(w90gpu_session* s, w90gpu_result* r) {
   if (!s) return W90GPU_ERR_INPUT;
-  // iterate in blocks, polling the done flag between blocks
-  const int block = 8;
-  int issued = 0;
+  // Iterate in blocks of kGraphBlock iterations (a replayed graph when
+  // possible).  Two blocks are kept in flight: after enqueuing block j the
+  // host waits for the done flag copied at the end of block j-1, so the
+  // device never idles on the poll.  Kernels of a finished solve exit at
+  // their first instruction, so at most one block of them runs after the
+  // loop has finished.
+  int issued = 0, blocks = 0;
   int rc;
-  if ((rc = read_state(s))) return rc;
-  while (!s->st_host->done && issued < s->ctl.num_iter) {
-    const int nb = std::min(block, s->ctl.num_iter - issued);
-    for (int i = 0; i < nb; ++i)
-      if ((rc = enqueue_iteration(s))) return rc;
+  const bool use_graph = !s->prof && !s->graph_failed && std::getenv("W90GPU_NO_GRAPH") == nullptr;
+  while (issued < s->ctl.num_iter) {
+    const int nb = std::min(kGraphBlock, s->ctl.num_iter - issued);
+    if (use_graph && nb == kGraphBlock) {
+      if (!s->gexec) {
+        if ((rc = capture_block(s))) return rc;
+      }
+      if (s->gexec) {
+        HIPCHK(s->ctx, hipGraphLaunch(s->gexec, s->stream));
+      } else {
+        for (int i = 0; i < nb; ++i)
+          if ((rc = enqueue_iteration(s))) return rc;
+      }
+    } else {
+      for (int i = 0; i < nb; ++i)
+        if ((rc = enqueue_iteration(s))) return rc;
+    }
     issued += nb;
-    if ((rc = read_state(s))) return rc;
+    const int slot = blocks & 1;
+    HIPCHK(s->ctx, hipMemcpyAsync(&s->flags_host[slot], &s->p.st->done, sizeof(int), hipMemcpyDeviceToHost,
+                                  s->stream));
+    HIPCHK(s->ctx, hipEventRecord(s->flag_ev[slot], s->stream));
+    ++blocks;
+    if (blocks >= 2) {
+      const int prev = (blocks - 2) & 1;
+      HIPCHK(s->ctx, hipEventSynchronize(s->flag_ev[prev]));
+      if (s->flags_host[prev]) break;
+    }
   }
   HIPCHK(s->ctx, hipEventRecord(s->t1, s->stream));
   HIPCHK(s->ctx, hipEventSynchronize(s->t1));
@@ -1445,8 +2021,7 @@ int w90gpu_domega(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* m_m
                        s->stream, s->p.D, nd);
     int r = launch_omega(s, MODE_INIT);
     if (r) return r;
-    hipLaunchKernelGGL(k_domega, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p);
-    HIPCHK(ctx, hipGetLastError());
+    if ((r = launch_domega(s))) return r;
     HIPCHK(ctx, hipMemcpyAsync(cdodq, s->p.G, ub, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(ctx, hipStreamSynchronize(s->stream));
     return W90GPU_OK;
