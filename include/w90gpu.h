@@ -159,6 +159,15 @@ int w90gpu_domega(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* m_m
 int w90gpu_new_u_and_m(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* cdq,
                        double* u_matrix, double* m_matrix);
 
+/* overlap_project (src/overlap.F90:758-952) for num_bands = num_wann: the
+ * Lowdin-orthonormalised U(k) = Z V^+ of the SVD A(k) = Z S V^+ (the unitary
+ * polar factor of A), then M(k,b) <- U(k)^+ M(k,b) U(k+b).
+ * a_matrix (num_wann,num_wann,num_kpts) complex in, u_matrix out, m_matrix
+ * (num_wann,num_wann,nntot,num_kpts) rotated in place.  Fails like the
+ * reference when the resulting U is not unitary to 1e-5 (singular A). */
+int w90gpu_project(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* a_matrix,
+                   double* u_matrix, double* m_matrix);
+
 /* ------------------------------------------------ device-resident sessions */
 
 /* A session keeps one problem's M, U and k-mesh tables resident in HBM and

@@ -277,3 +277,30 @@ def test_nan_input_is_reported(ctx):
     m[0, 0, 0, 0] = np.nan
     with pytest.raises(capi.W90Error):
         ctx.wann_main(c.kmesh, m, np.array(c.u0, order="F"), c.gpu_control(num_iter=5))
+
+
+@pytest.mark.parametrize("name", ["cfg1", "bcc_nw6", "fcc_nw8"])
+def test_project_matches_reference_and_oracle(ctx, name):
+    """GPU Lowdin projection (polar factor by Newton-Schulz) equals the SVD
+    construction of overlap_project: the reference's initial U and M."""
+    from test_oracle_cpu import lowdin_inputs
+
+    c = Case(name)
+    prob, a, m = lowdin_inputs(name)
+    m_gpu = np.array(m, order="F")
+    u_gpu = ctx.project(c.kmesh, a, m_gpu)
+    assert np.abs(u_gpu - c.u0).max() < 1e-12
+    assert np.abs(m_gpu - c.m0).max() < 1e-12
+    u_o, m_o = po.lowdin(oracle_problem(c), a, m)
+    assert np.abs(u_gpu - u_o).max() < 1e-12
+
+
+def test_project_rejects_singular_projections(ctx):
+    from wannier90_amd import capi
+
+    c = Case("cfg1")
+    a = np.zeros((c.nw, c.nw, c.nk), dtype=np.complex128, order="F")
+    a[0, 0, :] = 1.0  # rank one
+    with pytest.raises(capi.W90Error) as e:
+        ctx.project(c.kmesh, a, np.array(c.m0, order="F"))
+    assert e.value.code == capi.ERR_FATAL

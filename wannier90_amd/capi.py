@@ -81,6 +81,7 @@ SIGNATURES = [
     ("w90gpu_omega", C.c_int, [_vp, C.POINTER(Problem), _dp, C.c_int, C.c_double, _dp, _dp, _dp]),
     ("w90gpu_domega", C.c_int, [_vp, C.POINTER(Problem), _dp, _dp]),
     ("w90gpu_new_u_and_m", C.c_int, [_vp, C.POINTER(Problem), _dp, _dp, _dp]),
+    ("w90gpu_project", C.c_int, [_vp, C.POINTER(Problem), _dp, _dp, _dp]),
     ("w90gpu_session_create", C.c_int, [_vp, C.POINTER(Problem), C.POINTER(Control), C.POINTER(_vp)]),
     ("w90gpu_session_destroy", C.c_int, [_vp]),
     ("w90gpu_session_load", C.c_int, [_vp, _dp, _dp]),
@@ -265,6 +266,16 @@ class Context:
         nw, nk = u.shape[0], u.shape[2]
         prob = self.problem(nw, kmesh, nk)
         self._check(self.lib.w90gpu_new_u_and_m(self.h, C.byref(prob), _d(cdq), _d(u), _d(m)))
+
+    def project(self, kmesh, a, m):
+        """Lowdin projection: returns U from A (nw,nw,nk); rotates m in place."""
+        _check_fortran(a, 3)
+        _check_fortran(m, 4)
+        nw, nk = a.shape[0], a.shape[2]
+        prob = self.problem(nw, kmesh, nk)
+        u = np.zeros_like(a, order="F")
+        self._check(self.lib.w90gpu_project(self.h, C.byref(prob), _d(a), _d(u), _d(m)))
+        return u
 
     def session(self, kmesh, num_wann, num_kpts, ctl=None):
         return Session(self, kmesh, num_wann, num_kpts, ctl or control())

@@ -1244,6 +1244,60 @@ __global__ void __launch_bounds__(W90_NT) k_step_k(Prob p, int mode) {
   for (int e = tid; e < nw2; e += W90_NT) Un[e] = b[(e % nw) + (e / nw) * ld];
 }
 
+// C = A^+ B for n x n column-major complex matrices held by one workgroup.
+__device__ void wg_gemm_cn(cplx* C, const cplx* A, const cplx* B, int n) {
+  const int n2 = n * n;
+  for (int e = threadIdx.x; e < n2; e += blockDim.x) {
+    const int i = e % n, j = e / n;
+    const cplx* ac = A + (size_t)i * n;
+    const cplx* bc = B + (size_t)j * n;
+    cplx acc = c_make(0.0, 0.0);
+    for (int l = 0; l < n; ++l) acc = c_add(acc, c_cmul(ac[l], bc[l]));
+    C[e] = acc;
+  }
+  __syncthreads();
+}
+
+// Lowdin orthonormalisation of overlap_project (src/overlap.F90:758-952)
+// for num_bands = num_wann: the unitary polar factor U = A (A^+ A)^-1/2 of
+// A(k), which is the Z V^+ built there from the SVD A = Z S V^+.  Newton-
+// Schulz iteration X <- X (3 - X^+ X)/2 from X = A/||A||_F (all singular
+// values in (0,1]) converges to it for non-singular A, quadratically near
+// the end: once X^+ X = 1 to 1e-12 one more step reaches rounding level and
+// the iteration stops (at most 200 steps).  Writes U(k) to E.
+__global__ void __launch_bounds__(W90_NT) k_polar(Prob p, const cplx* A) {
+  const int nw = p.nw, nw2 = p.nw2;
+  const int k = blockIdx.x;
+  cplx* X = p.E + (size_t)k * nw2;
+  cplx* Y = p.W1 + (size_t)k * nw2;
+  cplx* Z = p.W2 + (size_t)k * nw2;
+  __shared__ double red[W90_NT];
+  const cplx* Ak = A + (size_t)k * nw2;
+  double fro = 0.0;
+  for (int e = threadIdx.x; e < nw2; e += blockDim.x) fro += c_abs2(Ak[e]);
+  fro = sqrt(block_sum(fro, red));
+  const double inv = (fro > 0.0 && isfinite(fro)) ? 1.0 / fro : 0.0;
+  for (int e = threadIdx.x; e < nw2; e += blockDim.x) X[e] = c_scale(Ak[e], inv);
+  __syncthreads();
+  for (int it = 0; it < 200; ++it) {
+    wg_gemm_cn(Y, X, X, nw);
+    double dev = 0.0;
+    for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
+      cplx y = Y[e];
+      if (e % nw == e / nw) y.x -= 1.0;
+      dev = fmax(dev, sqrt(c_abs2(y)));
+      // 3 - X^+ X, halved below
+      Y[e] = c_make(-Y[e].x + ((e % nw == e / nw) ? 3.0 : 0.0), -Y[e].y);
+    }
+    dev = block_max(dev, red);
+    if (!(dev == dev)) break;  // non-finite input: leave it to the unitarity check
+    wg_gemm(Z, X, Y, nw);
+    for (int e = threadIdx.x; e < nw2; e += blockDim.x) X[e] = c_scale(Z[e], 0.5);
+    __syncthreads();
+    if (dev < 1e-12) break;
+  }
+}
+
 // wann_check_unitarity (wannierise.F90:2594-2661): per-k maximum deviation
 // of U U^+ and U^+ U from the identity.
 __global__ void __launch_bounds__(W90_NT) k_unitarity(Prob p) {
@@ -2054,6 +2108,48 @@ int w90gpu_new_u_and_m(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double
     HIPCHK(ctx, hipMemcpyAsync(m_matrix, s->p.M[1], mb, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(ctx, hipMemcpyAsync(u_matrix, s->p.U[1], ub, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    return W90GPU_OK;
+  };
+  rc = run();
+  w90gpu_session_destroy(s);
+  return rc;
+}
+
+int w90gpu_project(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* a_matrix, double* u_matrix,
+                   double* m_matrix) {
+  if (!ctx || !a_matrix || !u_matrix || !m_matrix) return W90GPU_ERR_INPUT;
+  w90gpu_control ctl;
+  w90gpu_control_defaults(&ctl);
+  ctl.num_iter = 1;
+  w90gpu_session* s = nullptr;
+  int rc = w90gpu_session_create(ctx, prob, &ctl, &s);
+  if (rc) return rc;
+  const size_t mb = (size_t)s->nw * s->nw * s->nn * s->nk * sizeof(cplx);
+  const size_t ub = (size_t)s->nw * s->nw * s->nk * sizeof(cplx);
+  auto run = [&]() -> int {
+    // A is staged in the gradient buffer, which this phase does not use
+    HIPCHK(ctx, hipMemcpyAsync(s->p.G, a_matrix, ub, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(ctx, hipMemcpyAsync(s->p.M[0], m_matrix, mb, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s->stream, s->p, s->ctl, 1, 0.0);
+    hipLaunchKernelGGL(k_polar, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p, (const cplx*)s->p.G);
+    HIPCHK(ctx, hipGetLastError());
+    hipLaunchKernelGGL(k_set_direct, dim3(1), dim3(64), 0, s->stream, s->p, 1.0);
+    int r = launch_rotate(s, MODE_SECOND);
+    if (r) return r;
+    HIPCHK(ctx, hipMemcpyAsync(s->p.U[0], s->p.E, ub, hipMemcpyDeviceToDevice, s->stream));
+    hipLaunchKernelGGL(k_unitarity, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p);
+    HIPCHK(ctx, hipGetLastError());
+    std::vector<double> dev(s->nk);
+    HIPCHK(ctx, hipMemcpyAsync(dev.data(), s->p.unit, sizeof(double) * s->nk, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(ctx, hipMemcpyAsync(u_matrix, s->p.U[0], ub, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(ctx, hipMemcpyAsync(m_matrix, s->p.M[1], mb, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    double mx = 0.0;
+    for (double d : dev) mx = (d > mx || d != d) ? d : mx;
+    if (!(mx <= 1e-5)) {
+      set_error(ctx, "Error in unitarity of initial U in overlap_project (deviation %.3e)", mx);
+      return W90GPU_ERR_FATAL;
+    }
     return W90GPU_OK;
   };
   rc = run();

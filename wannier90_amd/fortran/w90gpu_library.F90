@@ -1,0 +1,842 @@
+!> Library interface of the wannierisation step, MI355X build.
+!>
+!> Keeps the procedure names, arguments and error behaviour of wannier90's
+!> module w90_library (src/library_interface.F90) for the path without
+!> disentanglement: options through w90_set_option + w90_input_setopt or a
+!> .win file through w90_input_reader, caller-owned M/U/A arrays associated
+!> by w90_set_*, k-mesh through w90_create_kmesh/w90_get_nn/w90_get_nnkp,
+!> the Lowdin projection in w90_project_overlap and the minimisation in
+!> w90_wannierise.  The arithmetic runs in libw90gpu.so through
+!> ISO_C_BINDING (module w90gpu_c); nothing falls back to the CPU.
+!> Status: ierr = 0 ok, > 0 error (codes of src/error.F90:27-36), < 0 warning.
+module w90_library
+  use iso_c_binding
+  use w90gpu_c
+  use w90gpu_io
+  implicit none
+  private
+
+  public :: lib_common_type
+  public :: w90_get_fortran_stdout, w90_get_fortran_stderr
+  public :: w90_set_option, w90_input_setopt, w90_input_reader
+  public :: w90_create_kmesh, w90_get_nn, w90_get_nnkp, w90_get_gkpb
+  public :: w90_set_m_local, w90_set_u_matrix, w90_set_u_opt, w90_set_eigval
+  public :: w90_project_overlap, w90_wannierise, w90_disentangle, w90_plot, w90_transport
+  public :: w90_get_centres, w90_get_spreads, w90_get_omega, w90_write_chkpt, w90_read_chkpt
+  public :: w90_write_nnkp, w90_print_conv
+
+  type lib_common_type
+    character(len=128) :: seedname = 'wannier'
+    complex(dp), pointer :: a_matrix(:, :, :) => null()
+    complex(dp), pointer :: m_matrix_local(:, :, :, :) => null()
+    complex(dp), pointer :: u_matrix(:, :, :) => null()
+    complex(dp), pointer :: u_opt(:, :, :) => null()
+    real(dp), pointer :: eigval(:, :) => null()
+    integer :: mp_grid(3) = 0
+    integer :: num_bands = 0
+    integer :: num_kpts = 0
+    integer :: num_wann = 0
+    integer :: num_proj = 0
+    logical :: gamma_only = .false.
+    logical :: have_disentangled = .false.
+    logical :: setup_complete = .false.
+    real(dp), allocatable :: kpt_latt(:, :)
+    real(dp) :: real_lattice(3, 3) = 0.0_dp
+    ! kmesh_info_type / kmesh_input_type
+    integer :: nntot = 0
+    integer(c_int), allocatable :: nnlist(:, :)
+    integer(c_int), allocatable :: nncell(:, :, :)
+    real(dp), allocatable :: bk(:, :, :)
+    real(dp), allocatable :: wb(:)
+    real(dp) :: wbtot = 0.0_dp
+    integer :: search_shells = 36
+    integer :: num_shells = 0
+    integer :: shell_list(6) = 0
+    real(dp) :: kmesh_tol = 1.0e-6_dp
+    logical :: skip_b1_tests = .false.
+    ! wann_control_type and print settings
+    type(w90gpu_control) :: control
+    logical :: control_set = .false.
+    integer :: iprint = 1
+    integer :: num_print_cycles = 1
+    integer :: device = 0
+    logical :: write_xyz = .false.
+    logical :: write_u_matrices = .false.
+    logical :: translate_home_cell = .false.
+    character(len=16) :: restart = ''
+    ! atoms (written to _centres.xyz)
+    integer :: num_atoms = 0
+    character(len=20), allocatable :: atom_symbol(:)
+    real(dp), allocatable :: atom_cart(:, :)
+    ! wannier_data / wann_omega_type
+    real(dp), allocatable :: centres(:, :)
+    real(dp), allocatable :: spreads(:)
+    type(w90gpu_result) :: result
+    real(dp), allocatable :: history(:, :)
+    ! options (set_option entries or .win data)
+    type(w90_settings) :: settings
+    logical :: have_win = .false.
+    type(c_ptr) :: ctx = c_null_ptr
+  end type lib_common_type
+
+  interface w90_set_option
+    module procedure w90_set_option_text
+    module procedure w90_set_option_logical
+    module procedure w90_set_option_int
+    module procedure w90_set_option_i1d
+    module procedure w90_set_option_i2d
+    module procedure w90_set_option_real
+    module procedure w90_set_option_r1d
+    module procedure w90_set_option_r2d
+  end interface w90_set_option
+
+contains
+
+  subroutine w90_get_fortran_stdout(istdout)
+    use iso_fortran_env, only: output_unit
+    integer, intent(out) :: istdout
+    istdout = output_unit
+  end subroutine w90_get_fortran_stdout
+
+  subroutine w90_get_fortran_stderr(istderr)
+    use iso_fortran_env, only: error_unit
+    integer, intent(out) :: istderr
+    istderr = error_unit
+  end subroutine w90_get_fortran_stderr
+
+  !> report an error like prterr (src/io.F90:395-452): message on stderr
+  subroutine report(istdout, istderr, code, msg, ierr)
+    integer, intent(in) :: istdout, istderr, code
+    character(len=*), intent(in) :: msg
+    integer, intent(out) :: ierr
+    ierr = code
+    if (code > 0) then
+      write (istderr, '(1x,a)') 'Exiting.......'
+      write (istderr, '(1x,a)') trim(msg)
+      write (istdout, '(1x,a)') 'Exiting.......'
+      write (istdout, '(1x,a)') trim(msg)
+    else if (code < 0) then
+      write (istdout, '(1x,a)') 'Warning: '//trim(msg)
+    end if
+  end subroutine report
+
+  ! ------------------------------------------------------------ options
+
+  subroutine w90_set_option_text(common_data, keyword, text)
+    type(lib_common_type), intent(inout) :: common_data
+    character(*), intent(in) :: keyword, text
+    type(w90_setting) :: e
+    e%keyword = keyword
+    e%kind = K_TEXT
+    e%text = text
+    call settings_add(common_data%settings, e)
+  end subroutine w90_set_option_text
+
+  subroutine w90_set_option_logical(common_data, keyword, bool)
+    type(lib_common_type), intent(inout) :: common_data
+    character(*), intent(in) :: keyword
+    logical, intent(in) :: bool
+    type(w90_setting) :: e
+    e%keyword = keyword
+    e%kind = K_LOGICAL
+    e%lval = bool
+    call settings_add(common_data%settings, e)
+  end subroutine w90_set_option_logical
+
+  subroutine w90_set_option_int(common_data, keyword, ival)
+    type(lib_common_type), intent(inout) :: common_data
+    character(*), intent(in) :: keyword
+    integer, intent(in) :: ival
+    type(w90_setting) :: e
+    e%keyword = keyword
+    e%kind = K_INT
+    e%ival = ival
+    call settings_add(common_data%settings, e)
+  end subroutine w90_set_option_int
+
+  subroutine w90_set_option_i1d(common_data, keyword, arr)
+    type(lib_common_type), intent(inout) :: common_data
+    character(*), intent(in) :: keyword
+    integer, intent(in) :: arr(:)
+    type(w90_setting) :: e
+    e%keyword = keyword
+    e%kind = K_I1D
+    e%i1d = arr
+    call settings_add(common_data%settings, e)
+  end subroutine w90_set_option_i1d
+
+  subroutine w90_set_option_i2d(common_data, keyword, arr)
+    type(lib_common_type), intent(inout) :: common_data
+    character(*), intent(in) :: keyword
+    integer, intent(in) :: arr(:, :)
+    type(w90_setting) :: e
+    e%keyword = keyword
+    e%kind = K_I2D
+    e%i2d = arr
+    call settings_add(common_data%settings, e)
+  end subroutine w90_set_option_i2d
+
+  subroutine w90_set_option_real(common_data, keyword, rval)
+    type(lib_common_type), intent(inout) :: common_data
+    character(*), intent(in) :: keyword
+    real(dp), intent(in) :: rval
+    type(w90_setting) :: e
+    e%keyword = keyword
+    e%kind = K_REAL
+    e%rval = rval
+    call settings_add(common_data%settings, e)
+  end subroutine w90_set_option_real
+
+  subroutine w90_set_option_r1d(common_data, keyword, arr)
+    type(lib_common_type), intent(inout) :: common_data
+    character(*), intent(in) :: keyword
+    real(dp), intent(in) :: arr(:)
+    type(w90_setting) :: e
+    e%keyword = keyword
+    e%kind = K_R1D
+    e%r1d = arr
+    call settings_add(common_data%settings, e)
+  end subroutine w90_set_option_r1d
+
+  !> blocks (unit_cell_cart, kpoints) are given as r(columns, rows), e.g.
+  !> kpoints(3, num_kpts), as the reference stores them
+  subroutine w90_set_option_r2d(common_data, keyword, arr)
+    type(lib_common_type), intent(inout) :: common_data
+    character(*), intent(in) :: keyword
+    real(dp), intent(in) :: arr(:, :)
+    type(w90_setting) :: e
+    e%keyword = keyword
+    e%kind = K_R2D
+    e%r2d = arr
+    call settings_add(common_data%settings, e)
+  end subroutine w90_set_option_r2d
+
+  !> Apply the option entries (w90_input_setopt, src/library_interface.F90:161)
+  subroutine w90_input_setopt(common_data, seedname, comm, istdout, istderr, ierr)
+    type(lib_common_type), intent(inout) :: common_data
+    character(len=*), intent(in) :: seedname
+    integer, intent(in) :: comm
+    integer, intent(in) :: istdout, istderr
+    integer, intent(out) :: ierr
+    character(len=256) :: msg
+    ierr = 0
+    if (common_data%have_win) then
+      call report(istdout, istderr, 1, ' readinput and setopt clash at input_setopt call', ierr)
+      return
+    end if
+    if (common_data%settings%n == 0) then
+      call report(istdout, istderr, 1, ' input_setopt called with no input', ierr)
+      return
+    end if
+    common_data%seedname = seedname
+    call apply_settings(common_data, ierr, msg)
+    if (ierr > 0) call report(istdout, istderr, ierr, msg, ierr)
+    if (comm < -huge(comm)) ierr = ierr  ! single device: the communicator is not used
+  end subroutine w90_input_setopt
+
+  !> Read seedname.win (w90_input_reader, src/library_interface.F90:294)
+  subroutine w90_input_reader(common_data, istdout, istderr, ierr)
+    type(lib_common_type), intent(inout) :: common_data
+    integer, intent(in) :: istdout, istderr
+    integer, intent(out) :: ierr
+    character(len=256) :: msg
+    ierr = 0
+    if (common_data%settings%n > 0 .and. .not. common_data%have_win) then
+      call report(istdout, istderr, 1, 'Error: (library use) options interface and .win parsing clash.', ierr)
+      return
+    end if
+    call settings_clear(common_data%settings)
+    call read_win(trim(common_data%seedname)//'.win', common_data%settings, ierr, msg)
+    if (ierr > 0) then
+      call report(istdout, istderr, ierr, msg, ierr)
+      return
+    end if
+    common_data%have_win = .true.
+    call apply_settings(common_data, ierr, msg)
+    if (ierr > 0) call report(istdout, istderr, ierr, msg, ierr)
+  end subroutine w90_input_reader
+
+  !> Keywords of the wannierise path (src/wannier90_readwrite.F90:157-384,
+  !> :628-804, src/readwrite.F90 lattice/k-point/atom blocks).  Keywords of
+  !> the branches this build does not carry are rejected with an input error.
+  subroutine apply_settings(cd, ierr, msg)
+    type(lib_common_type), intent(inout) :: cd
+    integer, intent(out) :: ierr
+    character(len=*), intent(out) :: msg
+    logical :: found, lval, found_trial
+    integer :: nk, i, n, ios, nsh
+    real(dp) :: rval, uc(3, 3), pos(3)
+    character(len=64) :: text
+    character(len=512) :: line
+    integer :: sl(6)
+    ierr = 0
+    msg = ''
+    call w90gpu_control_defaults(cd%control)
+    cd%control_set = .true.
+    call get_int(cd%settings, 'num_wann', cd%num_wann, found, ierr)
+    if (.not. found .or. cd%num_wann <= 0) then
+      ierr = 5
+      msg = 'Error: num_wann must be provided and positive'
+      return
+    end if
+    cd%num_bands = cd%num_wann
+    call get_int(cd%settings, 'num_bands', cd%num_bands, found, ierr)
+    if (cd%num_bands > cd%num_wann) then
+      ierr = 5
+      msg = 'Error: num_bands > num_wann needs disentanglement, which this build does not carry'
+      return
+    else if (cd%num_bands < cd%num_wann) then
+      ierr = 5
+      msg = 'Error: num_bands must be greater than or equal to num_wann'
+      return
+    end if
+    call get_i1d(cd%settings, 'mp_grid', cd%mp_grid, 3, found, ierr)
+    if (.not. found .or. ierr /= 0) then
+      ierr = 5
+      msg = 'Error: Did not find the kpoint grid (mp_grid) in input file'
+      return
+    end if
+    call get_r2d(cd%settings, 'unit_cell_cart', uc, 3, 3, .true., found, ierr)
+    if (.not. found .or. ierr /= 0) then
+      ierr = 5
+      msg = 'Error: Did not find the cell information (unit_cell_cart) in the input file'
+      return
+    end if
+    cd%real_lattice = transpose(uc)  ! real_lattice(i,:) is lattice vector i
+    nk = product(cd%mp_grid)
+    call get_int(cd%settings, 'num_kpts', nk, found, ierr)
+    cd%num_kpts = nk
+    if (allocated(cd%kpt_latt)) deallocate (cd%kpt_latt)
+    allocate (cd%kpt_latt(3, nk))
+    call get_r2d(cd%settings, 'kpoints', cd%kpt_latt, 3, nk, .false., found, ierr)
+    if (.not. found .or. ierr /= 0) then
+      ierr = 5
+      msg = 'Error: Did not find the kpoint information in the input file'
+      return
+    end if
+    ! wann_control (defaults of src/wannier90_types.F90:170-192)
+    call get_int(cd%settings, 'num_iter', cd%control%num_iter, found, ierr)
+    call get_int(cd%settings, 'num_cg_steps', cd%control%num_cg_steps, found, ierr)
+    call get_int(cd%settings, 'conv_window', cd%control%conv_window, found, ierr)
+    call get_real(cd%settings, 'conv_tol', cd%control%conv_tol, found, ierr)
+    call get_real(cd%settings, 'trial_step', cd%control%trial_step, found_trial, ierr)
+    rval = -999.0_dp
+    call get_real(cd%settings, 'fixed_step', rval, found, ierr)
+    if (found) then
+      if (found_trial) then
+        ierr = 5
+        msg = 'Error: cannot specify both fixed_step and trial_step'
+        return
+      end if
+      cd%control%fixed_step = rval
+      cd%control%lfixstep = 1
+      if (rval <= 0.0_dp) then
+        ierr = 5
+        msg = 'Error: fixed_step must be greater than zero'
+        return
+      end if
+    end if
+    rval = -1.0_dp
+    call get_real(cd%settings, 'conv_noise_amp', rval, found, ierr)
+    cd%control%conv_noise_amp = rval
+    if (ierr /= 0) then
+      msg = 'Error: malformed wannierise keyword'
+      return
+    end if
+    ! branches of wann_main outside this path
+    do i = 1, 5
+      lval = .false.
+      select case (i)
+      case (1)
+        text = 'precond'
+      case (2)
+        text = 'guiding_centres'
+      case (3)
+        text = 'slwf_constrain'
+      case (4)
+        text = 'gamma_only'
+      case (5)
+        text = 'site_symmetry'
+      end select
+      call get_logical(cd%settings, trim(text), lval, found, ierr)
+      if (lval) then
+        ierr = 5
+        msg = 'Error: '//trim(text)//' is not supported by this build (wannierise path without it)'
+        return
+      end if
+    end do
+    n = 0
+    call get_int(cd%settings, 'slwf_num', n, found, ierr)
+    if (found .and. n /= cd%num_wann) then
+      ierr = 5
+      msg = 'Error: selective localisation (slwf_num) is not supported by this build'
+      return
+    end if
+    if (settings_find(cd%settings, 'exclude_bands') > 0) then
+      ierr = 5
+      msg = 'Error: exclude_bands is not supported by this build'
+      return
+    end if
+    if (cd%control%conv_noise_amp > 0.0_dp) then
+      ierr = 5
+      msg = 'Error: conv_noise_amp > 0 is not supported by this build'
+      return
+    end if
+    ! print and output options
+    call get_int(cd%settings, 'iprint', cd%iprint, found, ierr)
+    call get_int(cd%settings, 'num_print_cycles', cd%num_print_cycles, found, ierr)
+    if (cd%num_print_cycles <= 0) cd%num_print_cycles = 1
+    call get_logical(cd%settings, 'write_xyz', cd%write_xyz, found, ierr)
+    call get_logical(cd%settings, 'write_u_matrices', cd%write_u_matrices, found, ierr)
+    call get_logical(cd%settings, 'translate_home_cell', cd%translate_home_cell, found, ierr)
+    call get_text(cd%settings, 'restart', cd%restart, found)
+    ! k-mesh input
+    call get_int(cd%settings, 'search_shells', cd%search_shells, found, ierr)
+    call get_real(cd%settings, 'kmesh_tol', cd%kmesh_tol, found, ierr)
+    call get_logical(cd%settings, 'skip_b1_tests', cd%skip_b1_tests, found, ierr)
+    sl = 0
+    i = settings_find(cd%settings, 'shell_list')
+    if (i > 0) then
+      if (cd%settings%e(i)%kind == K_I1D) then
+        nsh = min(6, size(cd%settings%e(i)%i1d))
+        sl(1:nsh) = cd%settings%e(i)%i1d(1:nsh)
+      else
+        nsh = 0
+        do n = 6, 1, -1
+          read (cd%settings%e(i)%text, *, iostat=ios) sl(1:n)
+          if (ios == 0) then
+            nsh = n
+            exit
+          end if
+        end do
+      end if
+      cd%num_shells = nsh
+      cd%shell_list = sl
+    end if
+    ! MI355X-specific options
+    call get_int(cd%settings, 'gpu_device', cd%device, found, ierr)
+    text = 'fp64'
+    call get_text(cd%settings, 'precision', text, found)
+    if (trim(text) == 'mixed') then
+      cd%control%precision = W90GPU_PREC_MIXED
+    else
+      cd%control%precision = W90GPU_PREC_FP64
+    end if
+    ! atoms for _centres.xyz
+    cd%num_atoms = 0
+    if (allocated(cd%atom_symbol)) deallocate (cd%atom_symbol)
+    if (allocated(cd%atom_cart)) deallocate (cd%atom_cart)
+    i = settings_find(cd%settings, 'atoms_frac')
+    if (i == 0) i = settings_find(cd%settings, 'atoms_cart')
+    if (i > 0) then
+      if (cd%settings%e(i)%kind == K_BLOCK) then
+        n = size(cd%settings%e(i)%lines)
+        allocate (cd%atom_symbol(n), cd%atom_cart(3, n))
+        rval = 1.0_dp
+        do nk = 1, n
+          line = cd%settings%e(i)%lines(nk)
+          if (lower(adjustl(line(1:4))) == 'bohr') then
+            rval = bohr_ang
+            cycle
+          end if
+          if (lower(adjustl(line(1:3))) == 'ang') cycle
+          read (line, *, iostat=ios) text, pos
+          if (ios /= 0) cycle
+          cd%num_atoms = cd%num_atoms + 1
+          cd%atom_symbol(cd%num_atoms) = text
+          if (trim(cd%settings%e(i)%keyword) == 'atoms_frac') then
+            cd%atom_cart(:, cd%num_atoms) = matmul(pos, cd%real_lattice)
+          else
+            cd%atom_cart(:, cd%num_atoms) = pos * rval
+          end if
+        end do
+      end if
+    end if
+    ierr = 0
+  end subroutine apply_settings
+
+  ! ------------------------------------------------------------ k-mesh
+
+  subroutine ensure_context(cd, istdout, istderr, ierr)
+    type(lib_common_type), intent(inout) :: cd
+    integer, intent(in) :: istdout, istderr
+    integer, intent(out) :: ierr
+    integer(c_int) :: rc
+    ierr = 0
+    if (c_associated(cd%ctx)) return
+    rc = w90gpu_create(int(cd%device, c_int), cd%ctx)
+    if (rc /= W90GPU_OK) then
+      call report(istdout, istderr, int(rc), 'Error: no MI355X device for the wannierisation library: '// &
+                  w90gpu_error_message(c_null_ptr), ierr)
+      cd%ctx = c_null_ptr
+    end if
+  end subroutine ensure_context
+
+  !> w90_create_kmesh (src/library_interface.F90:810): shells, weights and
+  !> neighbour lists, computed by the library's k-mesh (kmesh_get).
+  subroutine w90_create_kmesh(common_data, istdout, istderr, ierr)
+    type(lib_common_type), intent(inout) :: common_data
+    integer, intent(in) :: istdout, istderr
+    integer, intent(out) :: ierr
+    integer(c_int) :: rc, nntot, nk
+    integer(c_int), allocatable :: nnl(:), nnc(:), sl(:)
+    real(c_double), allocatable :: bk(:), wb(:)
+    real(c_double) :: wbtot
+    character(kind=c_char) :: err(256)
+    character(len=256) :: msg
+    integer :: i
+    ierr = 0
+    nk = int(common_data%num_kpts, c_int)
+    if (nk <= 0 .or. .not. allocated(common_data%kpt_latt)) then
+      call report(istdout, istderr, 1, 'Error: kmesh requested before the k-points are set', ierr)
+      return
+    end if
+    allocate (nnl(nk * W90GPU_NUM_NNMAX), nnc(3 * nk * W90GPU_NUM_NNMAX), bk(3 * W90GPU_NUM_NNMAX * nk), &
+              wb(W90GPU_NUM_NNMAX), sl(6))
+    sl = common_data%shell_list
+    rc = w90gpu_kmesh_get(common_data%real_lattice, nk, common_data%kpt_latt, &
+                          int(common_data%search_shells, c_int), real(common_data%kmesh_tol, c_double), &
+                          int(common_data%num_shells, c_int), sl, &
+                          merge(1_c_int, 0_c_int, common_data%skip_b1_tests), nntot, nnl, nnc, bk, wb, &
+                          wbtot, err, 256_c_int)
+    if (rc /= W90GPU_OK) then
+      msg = ''
+      do i = 1, 256
+        if (err(i) == c_null_char) exit
+        msg(i:i) = err(i)
+      end do
+      call report(istdout, istderr, int(rc), 'Error: '//trim(msg), ierr)
+      return
+    end if
+    common_data%nntot = nntot
+    if (allocated(common_data%nnlist)) deallocate (common_data%nnlist, common_data%nncell, common_data%bk, &
+                                                   common_data%wb)
+    allocate (common_data%nnlist(nk, nntot), common_data%nncell(3, nk, nntot), &
+              common_data%bk(3, nntot, nk), common_data%wb(nntot))
+    common_data%nnlist = reshape(nnl(1:nk * nntot), [nk, nntot])
+    common_data%nncell = reshape(nnc(1:3 * nk * nntot), [3, nk, nntot])
+    common_data%bk = reshape(bk(1:3 * nntot * nk), [3, nntot, nk])
+    common_data%wb = wb(1:nntot)
+    common_data%wbtot = wbtot
+    common_data%setup_complete = .true.
+  end subroutine w90_create_kmesh
+
+  subroutine w90_get_nn(common_data, nn, istdout, istderr, ierr)
+    type(lib_common_type), intent(inout) :: common_data
+    integer, intent(out) :: nn, ierr
+    integer, intent(in) :: istdout, istderr
+    ierr = 0
+    if (.not. common_data%setup_complete) then
+      call w90_create_kmesh(common_data, istdout, istderr, ierr)
+      if (ierr > 0) return
+    end if
+    nn = common_data%nntot
+  end subroutine w90_get_nn
+
+  subroutine w90_get_nnkp(common_data, nnkp, istdout, istderr, ierr)
+    type(lib_common_type), intent(inout) :: common_data
+    integer, intent(out) :: nnkp(:, :), ierr
+    integer, intent(in) :: istdout, istderr
+    ierr = 0
+    if (.not. common_data%setup_complete) then
+      call w90_create_kmesh(common_data, istdout, istderr, ierr)
+      if (ierr > 0) return
+    end if
+    nnkp = common_data%nnlist
+  end subroutine w90_get_nnkp
+
+  subroutine w90_get_gkpb(common_data, gkpb, istdout, istderr, ierr)
+    type(lib_common_type), intent(inout) :: common_data
+    integer, intent(out) :: gkpb(:, :, :), ierr
+    integer, intent(in) :: istdout, istderr
+    ierr = 0
+    if (.not. common_data%setup_complete) then
+      call w90_create_kmesh(common_data, istdout, istderr, ierr)
+      if (ierr > 0) return
+    end if
+    gkpb = common_data%nncell
+  end subroutine w90_get_gkpb
+
+  ! ------------------------------------------------------------ arrays
+
+  subroutine w90_set_m_local(common_data, m_orig)
+    type(lib_common_type), intent(inout) :: common_data
+    complex(dp), intent(inout), target :: m_orig(:, :, :, :)
+    common_data%m_matrix_local => m_orig
+  end subroutine w90_set_m_local
+
+  subroutine w90_set_u_matrix(common_data, u_matrix)
+    type(lib_common_type), intent(inout) :: common_data
+    complex(dp), intent(inout), target :: u_matrix(:, :, :)
+    common_data%u_matrix => u_matrix
+  end subroutine w90_set_u_matrix
+
+  subroutine w90_set_u_opt(common_data, u_opt)
+    type(lib_common_type), intent(inout) :: common_data
+    complex(dp), intent(inout), target :: u_opt(:, :, :)
+    common_data%u_opt => u_opt
+  end subroutine w90_set_u_opt
+
+  subroutine w90_set_eigval(common_data, eigval)
+    type(lib_common_type), intent(inout) :: common_data
+    real(dp), intent(inout), target :: eigval(:, :)
+    common_data%eigval => eigval
+  end subroutine w90_set_eigval
+
+  subroutine make_problem(cd, prob)
+    type(lib_common_type), intent(inout), target :: cd
+    type(w90gpu_problem), intent(out) :: prob
+    prob%num_wann = int(cd%num_wann, c_int)
+    prob%num_kpts = int(cd%num_kpts, c_int)
+    prob%nntot = int(cd%nntot, c_int)
+    prob%nnlist = c_loc(cd%nnlist)
+    prob%bk = c_loc(cd%bk)
+    prob%wb = c_loc(cd%wb)
+    prob%wbtot = real(cd%wbtot, c_double)
+  end subroutine make_problem
+
+  !> w90_project_overlap (src/library_interface.F90:422): on entry u_matrix
+  !> holds A, on exit the Lowdin U; m_matrix_local is rotated with it.
+  subroutine w90_project_overlap(common_data, istdout, istderr, ierr)
+    type(lib_common_type), intent(inout), target :: common_data
+    integer, intent(in) :: istdout, istderr
+    integer, intent(out) :: ierr
+    type(w90gpu_problem) :: prob
+    complex(dp), allocatable :: a(:, :, :)
+    integer(c_int) :: rc
+    ierr = 0
+    if (.not. associated(common_data%m_matrix_local)) then
+      call report(istdout, istderr, 1, 'm_matrix_local not set for projovlp call', ierr)
+      return
+    else if (.not. associated(common_data%u_matrix)) then
+      call report(istdout, istderr, 1, 'u_matrix not set for projovlp call', ierr)
+      return
+    end if
+    if (common_data%have_disentangled) return
+    if (.not. common_data%setup_complete) then
+      call w90_create_kmesh(common_data, istdout, istderr, ierr)
+      if (ierr > 0) return
+    end if
+    call ensure_context(common_data, istdout, istderr, ierr)
+    if (ierr > 0) return
+    call make_problem(common_data, prob)
+    a = common_data%u_matrix
+    rc = w90gpu_project(common_data%ctx, prob, a, common_data%u_matrix, common_data%m_matrix_local)
+    if (rc /= W90GPU_OK) call report(istdout, istderr, int(rc), w90gpu_error_message(common_data%ctx), ierr)
+  end subroutine w90_project_overlap
+
+  !> w90_wannierise (src/library_interface.F90:464): the minimisation of
+  !> wann_main on the GPU, in place on the caller's u_matrix and
+  !> m_matrix_local; centres, spreads and Omega kept in common_data.
+  subroutine w90_wannierise(common_data, istdout, istderr, ierr)
+    type(lib_common_type), intent(inout), target :: common_data
+    integer, intent(in) :: istdout, istderr
+    integer, intent(out) :: ierr
+    type(w90gpu_problem) :: prob
+    type(c_ptr) :: sess
+    integer(c_int) :: rc, n
+    real(c_double), allocatable :: hist(:)
+    ierr = 0
+    if (.not. associated(common_data%m_matrix_local)) then
+      call report(istdout, istderr, 1, 'm_matrix_local not set for wannierise', ierr)
+      return
+    else if (.not. associated(common_data%u_opt)) then
+      call report(istdout, istderr, 1, 'u_opt not set for wannierise', ierr)
+      return
+    else if (.not. associated(common_data%u_matrix)) then
+      call report(istdout, istderr, 1, 'u_matrix not set for wannierise', ierr)
+      return
+    end if
+    if (common_data%gamma_only) then
+      call report(istdout, istderr, 5, 'Error: gamma_only wannierisation is not supported by this build', ierr)
+      return
+    end if
+    if (.not. common_data%setup_complete) then
+      call w90_create_kmesh(common_data, istdout, istderr, ierr)
+      if (ierr > 0) return
+    end if
+    if (.not. common_data%control_set) then
+      call w90gpu_control_defaults(common_data%control)
+      common_data%control_set = .true.
+    end if
+    call ensure_context(common_data, istdout, istderr, ierr)
+    if (ierr > 0) return
+    call make_problem(common_data, prob)
+    rc = w90gpu_session_create(common_data%ctx, prob, common_data%control, sess)
+    if (rc /= W90GPU_OK) then
+      call report(istdout, istderr, int(rc), w90gpu_error_message(common_data%ctx), ierr)
+      return
+    end if
+    rc = w90gpu_session_load(sess, common_data%m_matrix_local, common_data%u_matrix)
+    if (rc == W90GPU_OK) rc = w90gpu_session_solve(sess, common_data%result)
+    if (allocated(common_data%centres)) deallocate (common_data%centres, common_data%spreads)
+    allocate (common_data%centres(3, common_data%num_wann), common_data%spreads(common_data%num_wann))
+    if (rc <= 0) rc = w90gpu_session_fetch(sess, common_data%m_matrix_local, common_data%u_matrix, &
+                                          common_data%centres, common_data%spreads)
+    if (rc <= 0) then
+      allocate (hist(4 * (common_data%control%num_iter + 1)))
+      n = w90gpu_session_history(sess, hist, int(common_data%control%num_iter + 1, c_int))
+      if (allocated(common_data%history)) deallocate (common_data%history)
+      allocate (common_data%history(4, 0:max(n - 1, 0)))
+      if (n > 0) common_data%history = reshape(hist(1:4 * n), [4, n])
+    end if
+    if (rc /= W90GPU_OK) call report(istdout, istderr, int(rc), w90gpu_error_message(common_data%ctx), ierr)
+    rc = w90gpu_session_destroy(sess)
+  end subroutine w90_wannierise
+
+  !> Disentanglement is not carried by this build: a no-op when
+  !> num_bands = num_wann (nothing to disentangle), an input error otherwise.
+  subroutine w90_disentangle(common_data, istdout, istderr, ierr)
+    type(lib_common_type), intent(inout) :: common_data
+    integer, intent(in) :: istdout, istderr
+    integer, intent(out) :: ierr
+    ierr = 0
+    if (common_data%num_bands > common_data%num_wann) &
+      call report(istdout, istderr, 5, 'Error: disentanglement is not supported by this build', ierr)
+  end subroutine w90_disentangle
+
+  !> The output writers of plot_main on this path: _centres.xyz (write_xyz)
+  !> and _u.mat (write_u_matrices).  Band/Fermi-surface/WF plotting is not
+  !> carried by this build.
+  subroutine w90_plot(common_data, istdout, istderr, ierr)
+    type(lib_common_type), intent(inout) :: common_data
+    integer, intent(in) :: istdout, istderr
+    integer, intent(out) :: ierr
+    character(len=20), allocatable :: sym(:)
+    real(dp), allocatable :: pos(:, :)
+    ierr = 0
+    if (common_data%write_xyz .and. allocated(common_data%centres)) then
+      if (common_data%num_atoms > 0) then
+        sym = common_data%atom_symbol(1:common_data%num_atoms)
+        pos = common_data%atom_cart(:, 1:common_data%num_atoms)
+      else
+        allocate (sym(0), pos(3, 0))
+      end if
+      call write_xyz(trim(common_data%seedname)//'_centres.xyz', common_data%num_wann, &
+                     common_data%centres, common_data%real_lattice, common_data%translate_home_cell, &
+                     common_data%num_atoms, sym, pos)
+    end if
+    if (common_data%write_u_matrices .and. associated(common_data%u_matrix)) then
+      call write_umat(trim(common_data%seedname)//'_u.mat', common_data%num_wann, common_data%num_kpts, &
+                      common_data%kpt_latt, common_data%u_matrix)
+    end if
+    if (istdout < 0 .and. istderr < 0) ierr = 0
+  end subroutine w90_plot
+
+  subroutine w90_transport(common_data, istdout, istderr, ierr)
+    type(lib_common_type), intent(inout) :: common_data
+    integer, intent(in) :: istdout, istderr
+    integer, intent(out) :: ierr
+    call report(istdout, istderr, 5, 'Error: transport is not supported by this build', ierr)
+    if (common_data%num_wann < 0) ierr = 5
+  end subroutine w90_transport
+
+  ! ------------------------------------------------------------ results
+
+  subroutine w90_get_centres(common_data, centres)
+    type(lib_common_type), intent(in) :: common_data
+    real(dp), intent(out) :: centres(:, :)
+    if (allocated(common_data%centres)) then
+      centres = common_data%centres
+    else
+      centres = 0.0_dp
+    end if
+  end subroutine w90_get_centres
+
+  subroutine w90_get_spreads(common_data, spreads)
+    type(lib_common_type), intent(in) :: common_data
+    real(dp), intent(out) :: spreads(:)
+    if (allocated(common_data%spreads)) then
+      spreads = common_data%spreads
+    else
+      spreads = 0.0_dp
+    end if
+  end subroutine w90_get_spreads
+
+  !> Omega decomposition: omega(1:4) = Omega_I, Omega_D, Omega_OD, Omega;
+  !> the reference keeps these in common_data%omega without a getter.
+  subroutine w90_get_omega(common_data, omega)
+    type(lib_common_type), intent(in) :: common_data
+    real(dp), intent(out) :: omega(4)
+    omega = [common_data%result%omega_invariant, common_data%result%omega_d, &
+             common_data%result%omega_od, common_data%result%omega_total]
+  end subroutine w90_get_omega
+
+  subroutine w90_write_chkpt(common_data, chkpt)
+    type(lib_common_type), intent(in) :: common_data
+    character(len=*), intent(in) :: chkpt
+    call write_chk(trim(common_data%seedname)//'.chk', chkpt, common_data%num_bands, &
+                   common_data%real_lattice, common_data%num_kpts, common_data%mp_grid, &
+                   common_data%kpt_latt, common_data%nntot, common_data%num_wann, common_data%u_matrix, &
+                   common_data%m_matrix_local, common_data%centres, common_data%spreads)
+  end subroutine w90_write_chkpt
+
+  subroutine w90_read_chkpt(common_data, istdout, istderr, ierr)
+    type(lib_common_type), intent(inout) :: common_data
+    integer, intent(in) :: istdout, istderr
+    integer, intent(out) :: ierr
+    character(len=256) :: msg
+    call read_chk(trim(common_data%seedname)//'.chk', common_data%num_wann, common_data%num_kpts, &
+                  common_data%nntot, common_data%u_matrix, common_data%m_matrix_local, ierr, msg)
+    if (ierr > 0) call report(istdout, istderr, ierr, msg, ierr)
+  end subroutine w90_read_chkpt
+
+  subroutine w90_write_nnkp(common_data)
+    type(lib_common_type), intent(in) :: common_data
+    call write_nnkp(trim(common_data%seedname)//'.nnkp', common_data%real_lattice, common_data%num_kpts, &
+                    common_data%kpt_latt, common_data%nntot, common_data%nnlist, common_data%nncell)
+  end subroutine w90_write_nnkp
+
+  !> The CONV/SPRD lines and final state of wann_main
+  !> (src/wannierise.F90:486-523, :703-756, :838-880) from the per-iteration
+  !> record of the solve.
+  subroutine w90_print_conv(common_data, istdout, wall)
+    type(lib_common_type), intent(in) :: common_data
+    integer, intent(in) :: istdout
+    real(dp), intent(in) :: wall
+    integer :: it, iw, nit
+    real(dp) :: delta
+    if (.not. allocated(common_data%history)) return
+    nit = ubound(common_data%history, 2)
+    write (istdout, *)
+    write (istdout, '(1x,a)') '*------------------------------- WANNIERISE ---------------------------------*'
+    write (istdout, '(1x,a)') '+--------------------------------------------------------------------+<-- CONV'
+    write (istdout, '(1x,a)') '| Iter  Delta Spread     RMS Gradient      Spread (Ang^2)      Time  |<-- CONV'
+    write (istdout, '(1x,a)') '+--------------------------------------------------------------------+<-- CONV'
+    write (istdout, *)
+    do it = 0, nit
+      if (.not. (it == 0 .or. it == 1 .or. it == nit .or. mod(it, common_data%num_print_cycles) == 0)) cycle
+      if (it == 0) then
+        delta = common_data%history(1, 0)
+      else
+        delta = common_data%history(1, it) - common_data%history(1, it - 1)
+      end if
+      write (istdout, '(1x,i6,2x,E12.3,2x,F15.10,2x,F18.10,3x,F8.2,2x,a)') it, delta, &
+        sqrt(abs(common_data%history(4, it))), common_data%history(1, it), wall, '<-- CONV'
+      write (istdout, '(8x,a,F15.7,a,F15.7,a,F15.7,a)') 'O_D=', common_data%history(2, it), ' O_OD=', &
+        common_data%history(3, it), ' O_TOT=', common_data%history(1, it), ' <-- SPRD'
+    end do
+    if (common_data%result%converged == 1) then
+      write (istdout, '(/13x,a,es10.3,a,i2,a)') '<<<     Delta <', common_data%control%conv_tol, &
+        '  over ', common_data%control%conv_window, ' iterations     >>>'
+      write (istdout, '(13x,a/)') '<<< Wannierisation convergence criteria satisfied >>>'
+    end if
+    write (istdout, '(1x,a)') 'Final State'
+    do iw = 1, common_data%num_wann
+      write (istdout, '(2x,"WF centre and spread",i5,2x,"(",f10.6,",",f10.6,",",f10.6," )",f15.8)') iw, &
+        common_data%centres(:, iw), common_data%spreads(iw)
+    end do
+    write (istdout, '(2x,"Sum of centres and spreads",1x,"(",f10.6,",",f10.6,",",f10.6," )",f15.8)') &
+      sum(common_data%centres(1, :)), sum(common_data%centres(2, :)), sum(common_data%centres(3, :)), &
+      sum(common_data%spreads)
+    write (istdout, *)
+    write (istdout, '(3x,a21,a,f15.9)') '     Spreads (Ang^2)', '       Omega I      = ', &
+      common_data%result%omega_invariant
+    write (istdout, '(3x,a,f15.9)') '     ================       Omega D      = ', common_data%result%omega_d
+    write (istdout, '(3x,a,f15.9)') '                            Omega OD     = ', common_data%result%omega_od
+    write (istdout, '(3x,a21,a,f15.9)') 'Final Spread (Ang^2)', '       Omega Total  = ', &
+      common_data%result%omega_total
+    write (istdout, '(1x,a78)') repeat('-', 78)
+  end subroutine w90_print_conv
+
+end module w90_library
