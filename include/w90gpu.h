@@ -90,6 +90,8 @@ typedef struct {
   int converged;          /* 1 if the conv_window test was satisfied     */
   int unitarity_ok;       /* 1 if the final U passed the 1e-5 check      */
   int status;             /* status of the solve                         */
+  int bulk_iterations;    /* iterations of the FP32 bulk phase (mixed)   */
+  int reserved;
 } w90gpu_result;
 
 /* ---------------------------------------------------------------- setup */
@@ -199,6 +201,48 @@ size_t w90gpu_session_bytes(int num_wann, int num_kpts, int nntot);
  * the average duration and count since the last reset. */
 int w90gpu_session_profile(w90gpu_session* sess, int enable);
 int w90gpu_session_kernel_time(w90gpu_session* sess, double* avg_ms, int* launches);
+
+/* ------------------------------------------------ batches and serving
+ *
+ * No counterpart in the reference (which runs one problem per process):
+ * many independent problems of any num_wann / k-mesh solved together.  A
+ * batch owns one device pool of pool_bytes; a problem is admitted when its
+ * w90gpu_problem_bytes fit in the free part of the pool.  Every phase of an
+ * iteration is one launch over all running problems; each problem keeps its
+ * own loop state, convergence test and precision, and its results are the
+ * same as those of a session running it alone (the kernels sum in an order
+ * that does not depend on the other problems).  Problems may be added while
+ * others run (continuous batching): w90gpu_batch_step starts them at the next
+ * block of iterations.  Not thread-safe: one host thread per batch. */
+typedef struct w90gpu_batch w90gpu_batch;
+
+/* Device bytes one problem holds (inputs, state, workspaces, and the
+ * per-iteration record of num_iter iterations). */
+size_t w90gpu_problem_bytes(int num_wann, int num_kpts, int nntot, int num_iter);
+int w90gpu_batch_create(w90gpu_ctx* ctx, size_t pool_bytes, w90gpu_batch** batch);
+int w90gpu_batch_destroy(w90gpu_batch* batch);
+/* Admit a problem with its initial M and U (host, Fortran layout; copied
+ * before the call returns).  *id >= 0 identifies it.  Returns
+ * W90GPU_ERR_ALLOC, with the batch unchanged, when the pool has no room. */
+int w90gpu_batch_add(w90gpu_batch* batch, const w90gpu_problem* prob, const w90gpu_control* ctl,
+                     const double* m_matrix, const double* u_matrix, int* id);
+/* Start the added problems and run max_blocks blocks of iterations over all
+ * running problems (0: only start them).  ids of problems finished since
+ * the last call go to finished[0 .. *nfinished) (at most cap; the rest are
+ * reported by the next call). */
+int w90gpu_batch_step(w90gpu_batch* batch, int max_blocks, int* finished, int cap, int* nfinished);
+/* Run until every added problem has finished. */
+int w90gpu_batch_run(w90gpu_batch* batch);
+/* Queue every finished problem again from its initial M and U. */
+int w90gpu_batch_restart(w90gpu_batch* batch);
+/* Result and final state of a finished problem (M and U stay on the device
+ * until removed). */
+int w90gpu_batch_result(w90gpu_batch* batch, int id, w90gpu_result* result);
+int w90gpu_batch_fetch(w90gpu_batch* batch, int id, double* m_matrix, double* u_matrix, double* centres,
+                       double* spreads);
+/* Release a problem that is not running; its bytes return to the pool. */
+int w90gpu_batch_remove(w90gpu_batch* batch, int id);
+int w90gpu_batch_info(w90gpu_batch* batch, size_t* pool_bytes, size_t* used_bytes, int* running, int* queued);
 
 #ifdef __cplusplus
 }

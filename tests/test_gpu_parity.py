@@ -148,8 +148,11 @@ def test_pinned_prefix_matches_oracle(ctx, name):
     r = gpu_solve(ctx, c, num_iter=n)
     o = po.wann_main(oracle_problem(c), c.oracle_control(num_iter=n), c.m0, c.u0)
     assert r["res"].iterations == o["iterations"]
-    assert np.abs(r["u"] - o["u"]).max() <= 1e-7
-    assert np.abs(r["centres"] - o["centres"]).max() <= 1e-6
+    # pinned_iters pins the Omega trajectory; U and the centres may still
+    # carry the case's measured sensitivity to rounding (near-degenerate
+    # gauge directions), which bounds them as in the full-run tests
+    assert np.abs(r["u"] - o["u"]).max() <= tol(1e-7, c.sens_u)
+    assert np.abs(r["centres"] - o["centres"]).max() <= tol(1e-6, c.sens_centres)
     assert abs(r["res"].omega_total - o["omega"][3]) <= 1e-8
     assert np.abs(r["history"][:, 0] - o["history"][:, 0]).max() <= 1e-8
 

@@ -56,6 +56,7 @@ class Result(C.Structure):
         ("omega_od", C.c_double), ("omega_tilde", C.c_double), ("gcnorm1", C.c_double),
         ("delta_omega", C.c_double), ("solve_seconds", C.c_double), ("iterations", C.c_int),
         ("converged", C.c_int), ("unitarity_ok", C.c_int), ("status", C.c_int),
+        ("bulk_iterations", C.c_int), ("reserved", C.c_int),
     ]
 
     def as_dict(self):
@@ -93,6 +94,17 @@ SIGNATURES = [
     ("w90gpu_session_history", C.c_int, [_vp, _dp, C.c_int]),
     ("w90gpu_session_profile", C.c_int, [_vp, C.c_int]),
     ("w90gpu_session_kernel_time", C.c_int, [_vp, _dp, _ip]),
+    ("w90gpu_problem_bytes", C.c_size_t, [C.c_int, C.c_int, C.c_int, C.c_int]),
+    ("w90gpu_batch_create", C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    ("w90gpu_batch_destroy", C.c_int, [_vp]),
+    ("w90gpu_batch_add", C.c_int, [_vp, C.POINTER(Problem), C.POINTER(Control), _dp, _dp, _ip]),
+    ("w90gpu_batch_step", C.c_int, [_vp, C.c_int, _ip, C.c_int, _ip]),
+    ("w90gpu_batch_run", C.c_int, [_vp]),
+    ("w90gpu_batch_restart", C.c_int, [_vp]),
+    ("w90gpu_batch_result", C.c_int, [_vp, C.c_int, C.POINTER(Result)]),
+    ("w90gpu_batch_fetch", C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
+    ("w90gpu_batch_remove", C.c_int, [_vp, C.c_int]),
+    ("w90gpu_batch_info", C.c_int, [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), _ip, _ip]),
 ]
 
 
@@ -280,6 +292,9 @@ class Context:
     def session(self, kmesh, num_wann, num_kpts, ctl=None):
         return Session(self, kmesh, num_wann, num_kpts, ctl or control())
 
+    def batch(self, pool_bytes):
+        return Batch(self, pool_bytes)
+
 
 class Session:
     """Device-resident problem (w90gpu_session_*)."""
@@ -350,6 +365,90 @@ class Session:
         n = C.c_int(0)
         self.ctx._check(self.lib.w90gpu_session_kernel_time(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+class Batch:
+    """Many problems solved together on one device (w90gpu_batch_*)."""
+
+    def __init__(self, ctx, pool_bytes):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = _vp()
+        ctx._check(self.lib.w90gpu_batch_create(ctx.h, int(pool_bytes), C.byref(h)))
+        self.h = h
+        self.shapes = {}
+
+    def close(self):
+        if self.h:
+            self.lib.w90gpu_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, kmesh, m, u, ctl=None):
+        """Admit a problem; returns its id, or None when the pool has no room."""
+        _check_fortran(m, 4)
+        _check_fortran(u, 3)
+        nw, nk = u.shape[0], u.shape[2]
+        prob = self.ctx.problem(nw, kmesh, nk)
+        ctl = ctl or control()
+        i = C.c_int(-1)
+        rc = self.lib.w90gpu_batch_add(self.h, C.byref(prob), C.byref(ctl), _d(m), _d(u), C.byref(i))
+        if rc == ERR_ALLOC:
+            return None
+        self.ctx._check(rc)
+        self.shapes[i.value] = (nw, nk, kmesh.nntot)
+        return i.value
+
+    def step(self, blocks=1, cap=4096):
+        """Start added problems and run `blocks` blocks; ids finished since
+        the last call."""
+        out = np.zeros(cap, dtype=np.int32)
+        n = C.c_int(0)
+        self.ctx._check(self.lib.w90gpu_batch_step(self.h, blocks, _i(out), cap, C.byref(n)))
+        return [int(x) for x in out[: n.value]]
+
+    def run(self):
+        self.ctx._check(self.lib.w90gpu_batch_run(self.h))
+
+    def restart(self):
+        self.ctx._check(self.lib.w90gpu_batch_restart(self.h))
+
+    def result(self, i):
+        res = Result()
+        self.ctx._check(self.lib.w90gpu_batch_result(self.h, i, C.byref(res)))
+        return res
+
+    def fetch(self, i, want_m=False):
+        nw, nk, nn = self.shapes[i]
+        m = np.zeros((nw, nw, nn, nk), dtype=np.complex128, order="F") if want_m else None
+        u = np.zeros((nw, nw, nk), dtype=np.complex128, order="F")
+        centres = np.zeros((3, nw), order="F")
+        spreads = np.zeros(nw)
+        self.ctx._check(self.lib.w90gpu_batch_fetch(self.h, i, _d(m) if want_m else None, _d(u),
+                                                    _d(centres), _d(spreads)))
+        return m, u, centres, spreads
+
+    def remove(self, i):
+        self.ctx._check(self.lib.w90gpu_batch_remove(self.h, i))
+        self.shapes.pop(i, None)
+
+    def info(self):
+        pool = C.c_size_t(0)
+        used = C.c_size_t(0)
+        nr = C.c_int(0)
+        nq = C.c_int(0)
+        self.ctx._check(self.lib.w90gpu_batch_info(self.h, C.byref(pool), C.byref(used), C.byref(nr),
+                                                   C.byref(nq)))
+        return {"pool_bytes": pool.value, "used_bytes": used.value, "running": nr.value, "queued": nq.value}
+
+
+def problem_bytes(num_wann, num_kpts, nntot, num_iter):
+    return int(load().w90gpu_problem_bytes(num_wann, num_kpts, nntot, num_iter))
 
 
 def session_bytes(num_wann, num_kpts, nntot):

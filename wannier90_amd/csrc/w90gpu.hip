@@ -1,18 +1,10 @@
 // libw90gpu: MI355X wannierisation (spread minimisation without
-// disentanglement).  Device code and the host side of the C ABI declared in
-// include/w90gpu.h.
+// disentanglement) behind the C ABI of include/w90gpu.h.  Device code is in
+// w90gpu_kernels.h; this file holds the host side.
 //
-// One problem = one session.  M(k,b), U(k), the gradient, the search
-// direction and every reduction buffer stay in HBM for the whole solve; the
-// k-mesh tables are uploaded once.  All scalar control of wann_main — the
-// Fletcher-Reeves coefficient and its resets, the uphill test, the parabolic
-// line search and the convergence window — runs on the device in
-// single-workgroup control kernels, so one iteration is a fixed sequence of
-// launches that needs no host round trip.  The host only polls a done flag.
-//
-// Reference: src/wannierise.F90 (wann_main :48-1072, internal_new_u_and_m
-// :1599-1751, wann_omega :1979-2284, wann_domega :2287-2569,
-// wann_check_unitarity :2594-2661).
+// Reference: src/wannierise.F90 (wann_main :48-1072), src/overlap.F90
+// (overlap_project :758-952), src/library_interface.F90 (w90_wannierise
+// :464-530).
 
 #include <hip/hip_runtime.h>
 
@@ -23,1347 +15,167 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <new>
 #include <string>
 #include <vector>
 
-#include "../../include/w90gpu.h"
-
-typedef double2 cplx;
-
-#define W90_NT 256        // threads per workgroup for every kernel
-#define W90_PANEL 16      // output rows per rotation workgroup
-
-// --------------------------------------------------------------- helpers
-
-__device__ __forceinline__ cplx c_make(double r, double i) { return make_double2(r, i); }
-__device__ __forceinline__ cplx c_add(cplx a, cplx b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ cplx c_sub(cplx a, cplx b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ cplx c_scale(cplx a, double s) { return make_double2(a.x * s, a.y * s); }
-__device__ __forceinline__ cplx c_conj(cplx a) { return make_double2(a.x, -a.y); }
-__device__ __forceinline__ cplx c_mul(cplx a, cplx b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-// conj(a) * b
-__device__ __forceinline__ cplx c_cmul(cplx a, cplx b) {
-  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
-}
-__device__ __forceinline__ cplx c_div(cplx a, cplx b) {
-  const double d = b.x * b.x + b.y * b.y;
-  return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
-}
-__device__ __forceinline__ double c_abs2(cplx a) { return a.x * a.x + a.y * a.y; }
-
-// Block-wide sum of one double per thread; deterministic (fixed tree), any
-// block size: the part above the largest power of two is folded in first.
-__device__ double block_sum(double v, double* red) {
-  const int t = threadIdx.x;
-  const int n = blockDim.x;
-  int p2 = 1;
-  while (p2 * 2 <= n) p2 *= 2;
-  red[t] = v;
-  __syncthreads();
-  if (t < n - p2) red[t] += red[t + p2];
-  __syncthreads();
-  for (int s = p2 / 2; s > 0; s >>= 1) {
-    if (t < s) red[t] += red[t + s];
-    __syncthreads();
-  }
-  const double r = red[0];
-  __syncthreads();
-  return r;
-}
-
-__device__ double block_max(double v, double* red) {
-  const int t = threadIdx.x;
-  const int n = blockDim.x;
-  int p2 = 1;
-  while (p2 * 2 <= n) p2 *= 2;
-  red[t] = v;
-  __syncthreads();
-  if (t < n - p2) red[t] = fmax(red[t], red[t + p2]);
-  __syncthreads();
-  for (int s = p2 / 2; s > 0; s >>= 1) {
-    if (t < s) red[t] = fmax(red[t], red[t + s]);
-    __syncthreads();
-  }
-  const double r = red[0];
-  __syncthreads();
-  return r;
-}
-
-// --------------------------------------------------------------- state
-
-struct Spread {
-  double om_i, om_d, om_od, om_tot;
-};
-
-enum Mode { MODE_INIT = 0, MODE_TRIAL = 1, MODE_SECOND = 2 };
-
-// Per-problem solver state, resident in device memory.
-struct State {
-  // control (wann_control_type)
-  int num_iter, num_cg_steps, conv_window, lfixstep;
-  double conv_tol, trial_step, fixed_step, wbtot;
-  // loop state of wann_main
-  int iter;        // completed iterations
-  int ncg;
-  int cur;         // buffer holding the current U, M
-  int done;        // loop finished (converged or num_iter)
-  int converged;
-  int lquad;
-  int second;      // parabolic step pending in this iteration
-  int conv_count;
-  int first_pass;
-  int pad0;
-  double gcnorm0, gcnorm1, gcfac, doda0, alphamin, falphamin;
-  double beta, dir_sign, scale;
-  double omega_invariant;
-  Spread wann, old, trial;
-  double history[W90GPU_MAX_CONV_WINDOW];
-};
-
-// Device pointers of one problem, passed by value to every kernel.
-struct Prob {
-  int nw, nk, nn, npan;
-  int psz, nw2, lds, ngroup;  // psz: length of a per-k partial vector; lds: per-k LDS kernels;
-                              // ngroup: k-ranges of the first reduction level
-  const int* nnl;     // [k][b] 0-based neighbour
-  const double* bk;   // [k][b][3]
-  const double* wb;   // [b]
-  cplx* M[2];         // [k][b][nw*nw] column-major blocks
-  cplx* U[2];         // [k][nw*nw]
-  cplx* G;            // gradient
-  cplx* D;            // search direction (cdqkeep)
-  cplx* E;            // exp(dW) per k
-  cplx* W1;           // exp workspace per k
-  cplx* W2;           // exp workspace per k
-  double* ln[2];      // [k][b][n] Im ln M_nn of each buffer
-  double* d2;         // [k][b][n] |M_nn|^2 of the last rotated buffer
-  double* part_od;    // [k][b][panel]
-  double* part_all;   // [k][b][panel]
-  double* kpart;      // [k][psz] per-k partial sums of wann_omega (layout: DiagAcc below)
-  double* gpart;      // [ngroup][psz] sums of the per-k vectors over contiguous k-ranges
-  double* part_gg;    // [k]
-  double* part_gd;    // [k]
-  double* rave[2];    // [3][nw] per buffer (Fortran rave(3,nw): ind fastest)
-  double* r2ave[2];   // [nw]
-  double* unit;       // [k] unitarity deviation
-  double* log;        // [num_iter+1][4] om_tot, om_d, om_od, gcnorm1
-  State* st;
-};
-
-__device__ __forceinline__ bool gate(const State* st, int mode) {
-  if (st->done) return false;
-  if (mode == MODE_SECOND) return st->second != 0;
-  return true;
-}
-
-// --------------------------------------------------------------- kernels
-
-// Reset the loop state for a new solve.
-__global__ void k_reset(Prob p, w90gpu_control ctl, int first_pass, double omega_invariant) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  State* st = p.st;
-  st->num_iter = ctl.num_iter;
-  st->num_cg_steps = ctl.num_cg_steps;
-  st->conv_window = ctl.conv_window;
-  st->lfixstep = ctl.lfixstep;
-  st->conv_tol = ctl.conv_tol;
-  st->trial_step = ctl.trial_step;
-  st->fixed_step = ctl.fixed_step;
-  st->iter = 0;
-  st->ncg = 0;
-  st->cur = 0;
-  st->done = 0;
-  st->converged = 0;
-  st->lquad = 0;
-  st->second = 0;
-  st->conv_count = 0;
-  st->first_pass = first_pass;
-  st->gcnorm0 = st->gcnorm1 = st->gcfac = st->doda0 = 0.0;
-  st->alphamin = st->falphamin = 0.0;
-  st->beta = 0.0;
-  st->dir_sign = 1.0;
-  st->scale = 0.0;
-  st->omega_invariant = omega_invariant;
-  st->wann = Spread{0, 0, 0, 0};
-  st->old = Spread{0, 0, 0, 0};
-  st->trial = Spread{0, 0, 0, 0};
-  for (int i = 0; i < W90GPU_MAX_CONV_WINDOW; ++i) st->history[i] = 0.0;
-}
-
-// Direct-step state for internal_new_u_and_m as a standalone phase.
-__global__ void k_set_direct(Prob p, double scale) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  State* st = p.st;
-  st->done = 0;
-  st->second = 1;
-  st->cur = 0;
-  st->scale = scale;
-}
-
-__global__ void k_zero(cplx* x, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    x[i] = c_make(0.0, 0.0);
-}
-
-// M rotation M(k,b) <- E_k^+ M(k,b) E_{k+b} for one (k, b, row panel),
-// fused with the per-block partial sums of wann_omega: Im ln M_nn, |M_nn|^2,
-// sum_{m!=n}|M_mn|^2 and sum_mn |M_mn|^2 of the rotated block.
-// MODE_INIT only evaluates the partial sums of the current buffer.
-// (internal_new_u_and_m :1737-1748; wann_omega :2034-2043, :2076-2090,
-//  :2214-2258)
-__global__ void __launch_bounds__(W90_NT) k_rotate(Prob p, int mode) {
-  const State* st = p.st;
-  if (!gate(st, mode)) return;
-  const int nw = p.nw, nw2 = p.nw2;
-  const int panel = blockIdx.x % p.npan;
-  const int kb = blockIdx.x / p.npan;
-  const int k = kb / p.nn;
-  const int cur = st->cur;
-  const int dst = (mode == MODE_INIT) ? cur : 1 - cur;
-  const int i0 = panel * W90_PANEL;
-  const int prows = min(W90_PANEL, nw - i0);
-  const cplx* Ms = p.M[cur] + (size_t)kb * nw2;
-  cplx* Md = p.M[dst] + (size_t)kb * nw2;
-  __shared__ cplx T[W90_PANEL * W90GPU_MAX_WANN];
-  __shared__ double red[W90_NT];
-
-  const int nout = prows * nw;
-  double od = 0.0, all = 0.0;
-  if (mode != MODE_INIT) {
-    const cplx* Ek = p.E + (size_t)k * nw2;
-    const cplx* Ekb = p.E + (size_t)p.nnl[kb] * nw2;
-    for (int e = threadIdx.x; e < nout; e += blockDim.x) {
-      const int i = e % prows, j = e / prows;
-      const cplx* ec = Ek + (size_t)(i0 + i) * nw;
-      const cplx* mc = Ms + (size_t)j * nw;
-      cplx acc = c_make(0.0, 0.0);
-      for (int l = 0; l < nw; ++l) acc = c_add(acc, c_cmul(ec[l], mc[l]));
-      T[i + j * W90_PANEL] = acc;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < nout; e += blockDim.x) {
-      const int i = e % prows, j = e / prows;
-      const cplx* ec = Ekb + (size_t)j * nw;
-      cplx acc = c_make(0.0, 0.0);
-      for (int l = 0; l < nw; ++l) acc = c_add(acc, c_mul(T[i + l * W90_PANEL], ec[l]));
-      Md[(i0 + i) + (size_t)j * nw] = acc;
-      const double a2 = c_abs2(acc);
-      all += a2;
-      if (i0 + i == j) {
-        p.ln[dst][(size_t)kb * nw + j] = atan2(acc.y, acc.x);
-        p.d2[(size_t)kb * nw + j] = a2;
-      } else {
-        od += a2;
-      }
-    }
-  } else {
-    for (int e = threadIdx.x; e < nout; e += blockDim.x) {
-      const int i = e % prows, j = e / prows;
-      const cplx v = Ms[(i0 + i) + (size_t)j * nw];
-      const double a2 = c_abs2(v);
-      all += a2;
-      if (i0 + i == j) {
-        p.ln[dst][(size_t)kb * nw + j] = atan2(v.y, v.x);
-        p.d2[(size_t)kb * nw + j] = a2;
-      } else {
-        od += a2;
-      }
-    }
-  }
-  od = block_sum(od, red);
-  all = block_sum(all, red);
-  if (threadIdx.x == 0) {
-    p.part_od[(size_t)kb * p.npan + panel] = od;
-    p.part_all[(size_t)kb * p.npan + panel] = all;
-  }
-}
-
-// Per-k partial sums of wann_omega.  For Wannier function n, over the b of
-// one k:
-//   [q*nw + n]     S_rave[q] = sum_b wb b_q ln_n              (q = 0,1,2)
-//   [3*nw + n]     S_r2      = sum_b wb (1 - |M_nn|^2 + ln_n^2)
-//   [4*nw + n]     A         = sum_b wb (ln_n + b.r0_n)^2
-//   [(5+q)*nw + n] V[q]      = sum_b wb b_q (ln_n + b.r0_n)
-// and the scalars
-//   [8*nw + 0]     S_od      = sum_b wb sum_{m!=n} |M_mn|^2
-//   [8*nw + 1]     S_i       = sum_b wb (nw - sum_mn |M_mn|^2)
-//   [8*nw + 2..7]  B         = sum_b wb b b^T  (xx, yy, zz, xy, yz, zx)
-// with ln_n = Im ln M_nn and r0 the centres of the source buffer.  With
-// d = rave - r0, Omega_D = (1/nk) sum_n [A + 2 d.V + d^T B d] is exact
-// algebra for (1/nk) sum wb (ln + b.rave)^2 (wannierise.F90:2260-2275), and
-// r0 close to rave keeps it free of cancellation, so no second pass over ln
-// is needed once rave is known.
-struct DiagAcc {
-  double r[3], r2, a, v[3];
-};
-
-__device__ __forceinline__ void diag_add(DiagAcc& d, double w, const double* bv, double l, double d2,
-                                         double r0x, double r0y, double r0z) {
-  d.r[0] += w * bv[0] * l;
-  d.r[1] += w * bv[1] * l;
-  d.r[2] += w * bv[2] * l;
-  d.r2 += w * (1.0 - d2 + l * l);
-  const double x = l + (bv[0] * r0x + bv[1] * r0y + bv[2] * r0z);
-  d.a += w * x * x;
-  d.v[0] += w * bv[0] * x;
-  d.v[1] += w * bv[1] * x;
-  d.v[2] += w * bv[2] * x;
-}
-
-__device__ __forceinline__ void diag_store(double* out, int nw, int n, const DiagAcc& d) {
-  out[0 * nw + n] = d.r[0];
-  out[1 * nw + n] = d.r[1];
-  out[2 * nw + n] = d.r[2];
-  out[3 * nw + n] = d.r2;
-  out[4 * nw + n] = d.a;
-  out[5 * nw + n] = d.v[0];
-  out[6 * nw + n] = d.v[1];
-  out[7 * nw + n] = d.v[2];
-}
-
-// Scalars of the per-k vector that depend only on the k-mesh: B = sum wb b b^T.
-__device__ void kpart_bmat(const Prob& p, int k, double* out) {
-  double b6[6] = {0, 0, 0, 0, 0, 0};
-  for (int b = 0; b < p.nn; ++b) {
-    const double w = p.wb[b];
-    const double* bv = p.bk + ((size_t)k * p.nn + b) * 3;
-    b6[0] += w * bv[0] * bv[0];
-    b6[1] += w * bv[1] * bv[1];
-    b6[2] += w * bv[2] * bv[2];
-    b6[3] += w * bv[0] * bv[1];
-    b6[4] += w * bv[1] * bv[2];
-    b6[5] += w * bv[2] * bv[0];
-  }
-  for (int u = 0; u < 6; ++u) out[2 + u] = b6[u];
-}
-
-// Per-k partial vector from the outputs of the panel rotation kernel
-// (ln, |M_nn|^2 and the per-panel sums), for num_wann above the LDS kernels.
-__global__ void __launch_bounds__(W90_NT) k_kpart(Prob p, int mode) {
-  const State* st = p.st;
-  if (!gate(st, mode)) return;
-  const int nw = p.nw, nn = p.nn;
-  const int k = blockIdx.x;
-  const int cur = st->cur;
-  const int dst = (mode == MODE_INIT) ? cur : 1 - cur;
-  const double* ln = p.ln[dst];
-  double* out = p.kpart + (size_t)k * p.psz;
-  for (int n = threadIdx.x; n < nw; n += blockDim.x) {
-    double r0x = 0, r0y = 0, r0z = 0;
-    if (mode != MODE_INIT) {
-      r0x = p.rave[cur][0 + 3 * n];
-      r0y = p.rave[cur][1 + 3 * n];
-      r0z = p.rave[cur][2 + 3 * n];
-    }
-    DiagAcc d = {};
-    for (int b = 0; b < nn; ++b) {
-      const size_t kb = (size_t)k * nn + b;
-      diag_add(d, p.wb[b], p.bk + kb * 3, ln[kb * nw + n], p.d2[kb * nw + n], r0x, r0y, r0z);
-    }
-    diag_store(out, nw, n, d);
-  }
-  if (threadIdx.x == 0) {
-    double sod = 0.0, si = 0.0;
-    for (int b = 0; b < nn; ++b) {
-      const size_t kb = (size_t)k * nn + b;
-      double od = 0.0, al = 0.0;
-      for (int q = 0; q < p.npan; ++q) {
-        od += p.part_od[kb * p.npan + q];
-        al += p.part_all[kb * p.npan + q];
-      }
-      sod += p.wb[b] * od;
-      si += p.wb[b] * ((double)nw - al);
-    }
-    out[8 * nw + 0] = sod;
-    out[8 * nw + 1] = si;
-    kpart_bmat(p, k, out + 8 * nw);
-  }
-}
-
-// C = A B of n x n column-major complex matrices in LDS; thread slots as in
-// the calling kernel (element e = tid + s*W90_NT).
-template <int NWM>
-__device__ __forceinline__ void lds_gemm(cplx* C, const cplx* A, const cplx* B, int n) {
-  constexpr int SL = (NWM * NWM + W90_NT - 1) / W90_NT;
-  const int n2 = n * n, ld = n + 1;  // padded leading dimension
-#pragma unroll
-  for (int s = 0; s < SL; ++s) {
-    const int e = threadIdx.x + s * W90_NT;
-    if (e < n2) {
-      const int i = e % n, j = e / n;
-      cplx acc = c_make(0.0, 0.0);
-      for (int l = 0; l < n; ++l) acc = c_add(acc, c_mul(A[i + l * ld], B[l + j * ld]));
-      C[i + j * ld] = acc;
-    }
-  }
-  __syncthreads();
-}
-
-// M rotation with the omega partial sums, one workgroup per k, all operands
-// in LDS (num_wann <= NWM).  E_k is loaded once for the nn blocks of k;
-// each block M(k,b) and E_{k+b} are staged in LDS, T = E_k^+ M and
-// R = T E_{k+b} are formed there, R is written once and reduced in
-// registers into the per-k partial vector.  MODE_INIT only reduces.
-// (internal_new_u_and_m :1737-1748; wann_omega :2034-2090, :2214-2275)
-template <int NWM>
-__global__ void __launch_bounds__(W90_NT) k_rotate_k(Prob p, int mode) {
-  const State* st = p.st;
-  if (!gate(st, mode)) return;
-  constexpr int SL = (NWM * NWM + W90_NT - 1) / W90_NT;
-  const int nw = p.nw, nw2 = p.nw2, nn = p.nn;
-  const int k = blockIdx.x;
-  const int tid = threadIdx.x;
-  const int cur = st->cur;
-  const bool rot = (mode != MODE_INIT);
-  const int dst = rot ? 1 - cur : cur;
-  // padded column-major LDS blocks (leading dimension nw + 1), E_k stored
-  // conjugate-transposed: consecutive lanes read consecutive addresses
-  __shared__ cplx EkH[NWM * (NWM + 1)];
-  __shared__ cplx Eb[NWM * (NWM + 1)];
-  __shared__ cplx Ms[NWM * (NWM + 1)];
-  __shared__ cplx T[NWM * (NWM + 1)];
-  const int ld = nw + 1;
-  __shared__ double red[W90_NT];
-
-  DiagAcc acc[SL];
-  double r0[SL][3];
-#pragma unroll
-  for (int s = 0; s < SL; ++s) {
-    acc[s] = DiagAcc{};
-    r0[s][0] = r0[s][1] = r0[s][2] = 0.0;
-    const int e = tid + s * W90_NT;
-    if (rot && e < nw2 && e % nw == e / nw) {
-      const int n = e % nw;
-      r0[s][0] = p.rave[cur][0 + 3 * n];
-      r0[s][1] = p.rave[cur][1 + 3 * n];
-      r0[s][2] = p.rave[cur][2 + 3 * n];
-    }
-  }
-  if (rot) {
-    const cplx* ek = p.E + (size_t)k * nw2;
-    for (int e = tid; e < nw2; e += W90_NT) EkH[(e / nw) + (e % nw) * ld] = c_conj(ek[e]);
-  }
-  double s_od = 0.0, s_all = 0.0;
-  for (int b = 0; b < nn; ++b) {
-    const size_t kb = (size_t)k * nn + b;
-    const cplx* src = p.M[cur] + kb * nw2;
-    __syncthreads();
-    for (int e = tid; e < nw2; e += W90_NT) Ms[(e % nw) + (e / nw) * ld] = src[e];
-    if (rot) {
-      const cplx* eb = p.E + (size_t)p.nnl[kb] * nw2;
-      for (int e = tid; e < nw2; e += W90_NT) Eb[(e % nw) + (e / nw) * ld] = eb[e];
-    }
-    __syncthreads();
-    if (rot) {
-#pragma unroll
-      for (int s = 0; s < SL; ++s) {
-        const int e = tid + s * W90_NT;
-        if (e < nw2) {
-          const int i = e % nw, j = e / nw;
-          cplx a = c_make(0.0, 0.0);
-          for (int l = 0; l < nw; ++l) a = c_add(a, c_mul(EkH[i + l * ld], Ms[l + j * ld]));
-          T[i + j * ld] = a;
-        }
-      }
-      __syncthreads();
-    }
-    const double w = p.wb[b];
-    const double* bv = p.bk + kb * 3;
-    cplx* out = p.M[dst] + kb * nw2;
-#pragma unroll
-    for (int s = 0; s < SL; ++s) {
-      const int e = tid + s * W90_NT;
-      if (e < nw2) {
-        const int i = e % nw, j = e / nw;
-        cplx r;
-        if (rot) {
-          r = c_make(0.0, 0.0);
-          for (int l = 0; l < nw; ++l) r = c_add(r, c_mul(T[i + l * ld], Eb[l + j * ld]));
-          out[e] = r;
-        } else {
-          r = Ms[i + j * ld];
-        }
-        const double a2 = c_abs2(r);
-        s_all += w * a2;
-        if (i != j) {
-          s_od += w * a2;
-        } else {
-          const double l = atan2(r.y, r.x);
-          p.ln[dst][kb * nw + i] = l;
-          diag_add(acc[s], w, bv, l, a2, r0[s][0], r0[s][1], r0[s][2]);
-        }
-      }
-    }
-  }
-  double* outk = p.kpart + (size_t)k * p.psz;
-#pragma unroll
-  for (int s = 0; s < SL; ++s) {
-    const int e = tid + s * W90_NT;
-    if (e < nw2 && e % nw == e / nw) diag_store(outk, nw, e % nw, acc[s]);
-  }
-  s_od = block_sum(s_od, red);
-  s_all = block_sum(s_all, red);
-  if (tid == 0) {
-    double wsum = 0.0;
-    for (int b = 0; b < nn; ++b) wsum += p.wb[b];
-    outk[8 * nw + 0] = s_od;
-    outk[8 * nw + 1] = (double)nw * wsum - s_all;
-    kpart_bmat(p, k, outk + 8 * nw);
-  }
-}
-
-// M rotation with the omega partial sums for small blocks (num_wann <= NWM
-// <= 16): one workgroup per k with one wavefront per neighbour b, so the nn
-// rotations of a k proceed concurrently.  E_k, the nn blocks M(k,b), the nn
-// E_{k+b} and the nn intermediates T are all staged in LDS by one round of
-// loads; the diagonal of each rotated block goes through LDS to threads
-// n < num_wann, which sum it over b in order.
-// LDS matrices are column-major with leading dimension nw + 1: consecutive
-// lanes (consecutive rows i) read consecutive addresses, and the columns read
-// as broadcasts by the four 16-lane groups of a wavefront land in different
-// banks.  E_k is stored conjugate-transposed so that E_k^+ M reads rows of it
-// the same way.  LDS is sized by nntot at launch (dynamic shared memory).
-template <int NWM>
-__host__ __device__ constexpr size_t rotate_kb_lds(int nn) {
-  return sizeof(cplx) * (size_t)NWM * (NWM + 1) * (1 + 2 * (size_t)nn) +
-         sizeof(double) * (2 * (size_t)nn * NWM + 64 * (size_t)nn);
-}
-
-template <int NWM>
-__global__ void __launch_bounds__(64 * W90GPU_NUM_NNMAX) k_rotate_kb(Prob p, int mode) {
-  const State* st = p.st;
-  if (!gate(st, mode)) return;
-  constexpr int B2 = NWM * NWM;
-  constexpr int BP = NWM * (NWM + 1);  // padded block
-  constexpr int SL = (B2 + 63) / 64;
-  const int nw = p.nw, nw2 = p.nw2, nn = p.nn;
-  const int ld = nw + 1;
-  const int k = blockIdx.x;
-  const int tid = threadIdx.x;
-  const int b = tid >> 6;  // wavefront index = neighbour index
-  const int lane = tid & 63;
-  const int cur = st->cur;
-  const bool rot = (mode != MODE_INIT);
-  const int dst = rot ? 1 - cur : cur;
-  extern __shared__ double lds_raw[];
-  cplx* EkH = reinterpret_cast<cplx*>(lds_raw);   // conj(E_k)^T
-  cplx* Ms = EkH + BP + (size_t)b * 2 * BP;        // M(k,b), then T = E_k^+ M(k,b)
-  cplx* Eb = Ms + BP;                              // E_{k+b}
-  double* lnL = reinterpret_cast<double*>(EkH + BP + (size_t)nn * 2 * BP);  // [b][NWM]
-  double* d2L = lnL + nn * NWM;
-  double* red = d2L + nn * NWM;  // 64 * nn doubles for block_sum
-  const size_t kb = (size_t)k * nn + b;
-  if (rot)
-    for (int e = tid; e < nw2; e += blockDim.x) {
-      const int l = e % nw, i = e / nw;
-      EkH[i + l * ld] = c_conj(p.E[(size_t)k * nw2 + e]);
-    }
-  {
-    const cplx* src = p.M[cur] + kb * nw2;
-    for (int e = lane; e < nw2; e += 64) Ms[(e % nw) + (e / nw) * ld] = src[e];
-    if (rot) {
-      const cplx* eb = p.E + (size_t)p.nnl[kb] * nw2;
-      for (int e = lane; e < nw2; e += 64) Eb[(e % nw) + (e / nw) * ld] = eb[e];
-    }
-  }
-  __syncthreads();
-  if (rot) {
-    cplx tr[SL];
-#pragma unroll
-    for (int s = 0; s < SL; ++s) {
-      const int e = lane + 64 * s;
-      tr[s] = c_make(0.0, 0.0);
-      if (e < nw2) {
-        const int i = e % nw, j = e / nw;
-        cplx a = c_make(0.0, 0.0);
-        for (int l = 0; l < nw; ++l) a = c_add(a, c_mul(EkH[i + l * ld], Ms[l + j * ld]));
-        tr[s] = a;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < SL; ++s) {
-      const int e = lane + 64 * s;
-      if (e < nw2) Ms[(e % nw) + (e / nw) * ld] = tr[s];
-    }
-    __syncthreads();
-  }
-  const double w = p.wb[b];
-  double s_od = 0.0, s_all = 0.0;
-  cplx* out = p.M[dst] + kb * nw2;
-#pragma unroll
-  for (int s = 0; s < SL; ++s) {
-    const int e = lane + 64 * s;
-    if (e < nw2) {
-      const int i = e % nw, j = e / nw;
-      cplx r;
-      if (rot) {
-        r = c_make(0.0, 0.0);
-        for (int l = 0; l < nw; ++l) r = c_add(r, c_mul(Ms[i + l * ld], Eb[l + j * ld]));
-        out[e] = r;
-      } else {
-        r = Ms[i + j * ld];
-      }
-      const double a2 = c_abs2(r);
-      s_all += w * a2;
-      if (i != j) {
-        s_od += w * a2;
-      } else {
-        const double l = atan2(r.y, r.x);
-        p.ln[dst][kb * nw + i] = l;
-        lnL[b * NWM + i] = l;
-        d2L[b * NWM + i] = a2;
-      }
-    }
-  }
-  __syncthreads();
-  double* outk = p.kpart + (size_t)k * p.psz;
-  if (tid < nw) {
-    const int n = tid;
-    double r0x = 0, r0y = 0, r0z = 0;
-    if (rot) {
-      r0x = p.rave[cur][0 + 3 * n];
-      r0y = p.rave[cur][1 + 3 * n];
-      r0z = p.rave[cur][2 + 3 * n];
-    }
-    DiagAcc d = {};
-    for (int bb = 0; bb < nn; ++bb)
-      diag_add(d, p.wb[bb], p.bk + ((size_t)k * nn + bb) * 3, lnL[bb * NWM + n], d2L[bb * NWM + n], r0x, r0y, r0z);
-    diag_store(outk, nw, n, d);
-  }
-  s_od = block_sum(s_od, red);
-  s_all = block_sum(s_all, red);
-  if (tid == 0) {
-    double wsum = 0.0;
-    for (int bb = 0; bb < nn; ++bb) wsum += p.wb[bb];
-    outk[8 * nw + 0] = s_od;
-    outk[8 * nw + 1] = (double)nw * wsum - s_all;
-    kpart_bmat(p, k, outk + 8 * nw);
-  }
-}
-
-// internal_test_convergence (wannierise.F90:1076-1164) with the noise branch
-// off (conv_noise_amp <= 0), then the loop-exit bookkeeping.
-__device__ void conv_step(State* st, double* logrow, double gc) {
-  const int iter = st->iter + 1;
-  if (st->conv_window > 1) {
-    const double delta = st->wann.om_tot - st->old.om_tot;
-    const int cw = st->conv_window;
-    if (iter <= cw) {
-      st->history[iter - 1] = delta;
-    } else {
-      for (int j = 0; j < cw - 1; ++j) st->history[j] = st->history[j + 1];
-      st->history[cw - 1] = delta;
-    }
-    st->conv_count += 1;
-    if (st->conv_count >= cw) {
-      bool ok = true;
-      for (int j = 0; j < cw; ++j)
-        if (fabs(st->history[j]) > st->conv_tol) ok = false;
-      if (ok) st->converged = 1;
-    }
-  }
-  st->iter = iter;
-  if (logrow) {
-    logrow[0] = st->wann.om_tot;
-    logrow[1] = st->wann.om_d;
-    logrow[2] = st->wann.om_od;
-    logrow[3] = gc;
-  }
-  if (st->converged || iter >= st->num_iter) st->done = 1;
-}
-
-// Final reduction of wann_omega (single workgroup) fused with the control
-// logic that follows each wann_omega call in wann_main: initial state
-// (:452-470), internal_optimal_step (:1528-1596) after the trial step and
-// the acceptance of the trial or parabolic step (:660-701).
-#define W90_FINAL_NT 1024
-#define W90_MAX_GROUPS 32
-
-// First level of the omega reduction: workgroup g sums the per-k vectors of
-// a contiguous k-range, in k order (rows read coalesced along v).
-__global__ void __launch_bounds__(W90_NT) k_omega_group(Prob p, int mode) {
-  const State* st = p.st;
-  if (!gate(st, mode)) return;
-  const int g = blockIdx.x, G = p.ngroup, psz = p.psz;
-  const int k0 = (int)(((long)g * p.nk) / G), k1 = (int)(((long)(g + 1) * p.nk) / G);
-  for (int v = threadIdx.x; v < psz; v += blockDim.x) {
-    double s = 0.0;
-    for (int k = k0; k < k1; ++k) s += p.kpart[(size_t)k * psz + v];
-    p.gpart[(size_t)g * psz + v] = s;
-  }
-}
-
-__global__ void __launch_bounds__(W90_FINAL_NT) k_omega_final(Prob p, int mode) {
-  State* st = p.st;
-  if (!gate(st, mode)) return;
-  const int nw = p.nw, nk = p.nk, psz = p.psz;
-  const int cur = st->cur;
-  const int dst = (mode == MODE_INIT) ? cur : 1 - cur;
-  const double rnk = (double)nk;
-  const int tid = threadIdx.x;
-  __shared__ double red[W90_FINAL_NT];
-  __shared__ double sums[8 * W90GPU_MAX_WANN + 8];
-  __shared__ double sc[8];
-
-  // sum of the group vectors: VB values per pass (coalesced along a row),
-  // J lanes over groups, lanes combined in fixed order
-  const int VB = min(psz, W90_FINAL_NT);
-  const int J = W90_FINAL_NT / VB;
-  const int G = p.ngroup;
-  for (int v0 = 0; v0 < psz; v0 += VB) {
-    const int v = v0 + tid % VB;
-    const int j = tid / VB;
-    double s = 0.0;
-    if (j < J && v < psz)
-      for (int g = j; g < G; g += J) s += p.gpart[(size_t)g * psz + v];
-    red[tid] = s;
-    __syncthreads();
-    if (tid < VB && v0 + tid < psz) {
-      double t = 0.0;
-      for (int jj = 0; jj < J; ++jj) t += red[tid + jj * VB];
-      sums[v0 + tid] = t;
-    }
-    __syncthreads();
-  }
-
-  double omd_n = 0.0;
-  for (int n = tid; n < nw; n += blockDim.x) {
-    double q[8];
-    for (int u = 0; u < 8; ++u) q[u] = sums[u * nw + n];
-    double bb[6];
-    for (int u = 0; u < 6; ++u) bb[u] = sums[8 * nw + 2 + u];
-    const double rx = -q[0] / rnk, ry = -q[1] / rnk, rz = -q[2] / rnk;
-    p.rave[dst][0 + 3 * n] = rx;
-    p.rave[dst][1 + 3 * n] = ry;
-    p.rave[dst][2 + 3 * n] = rz;
-    p.r2ave[dst][n] = q[3] / rnk;
-    double r0x = 0, r0y = 0, r0z = 0;
-    if (mode != MODE_INIT) {
-      r0x = p.rave[cur][0 + 3 * n];
-      r0y = p.rave[cur][1 + 3 * n];
-      r0z = p.rave[cur][2 + 3 * n];
-    }
-    const double dx = rx - r0x, dy = ry - r0y, dz = rz - r0z;
-    const double dbd = bb[0] * dx * dx + bb[1] * dy * dy + bb[2] * dz * dz +
-                       2.0 * (bb[3] * dx * dy + bb[4] * dy * dz + bb[5] * dz * dx);
-    omd_n += q[4] + 2.0 * (dx * q[5] + dy * q[6] + dz * q[7]) + dbd;
-  }
-  // Omega_D summed over n in a fixed tree
-  const double omd = block_sum(omd_n, red);
-  if (threadIdx.x == 0) {
-    sc[0] = sums[8 * nw + 0];
-    sc[1] = sums[8 * nw + 1];
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-
-  Spread sp;
-  sp.om_od = sc[0] / rnk;
-  sp.om_d = omd / rnk;
-  if (mode == MODE_INIT && st->first_pass) {
-    sp.om_i = sc[1] / rnk;
-  } else {
-    sp.om_i = st->omega_invariant;
-  }
-  sp.om_tot = sp.om_i + sp.om_d + sp.om_od;
-
-  if (mode == MODE_INIT) {
-    st->omega_invariant = sp.om_i;
-    st->first_pass = 0;
-    st->wann = sp;
-    st->old.om_tot = 0.0;
-    if (p.log) {
-      p.log[0] = sp.om_tot;
-      p.log[1] = sp.om_d;
-      p.log[2] = sp.om_od;
-      p.log[3] = 0.0;
-    }
-    return;
-  }
-  double* logrow = p.log ? p.log + 4 * (size_t)(st->iter + 1) : nullptr;
-  if (mode == MODE_TRIAL) {
-    st->trial = sp;
-    if (st->lfixstep) {
-      st->old = st->wann;
-      st->wann = sp;
-      st->cur = dst;
-      conv_step(st, logrow, st->gcnorm1);
-      return;
-    }
-    // internal_optimal_step
-    const double trial_step = st->trial_step;
-    double fac = st->trial.om_tot - st->wann.om_tot;
-    double shift;
-    if (fabs(fac) > 2.2250738585072014e-308) {  // tiny(1.0_dp)
-      fac = 1.0 / fac;
-      shift = 1.0;
-    } else {
-      fac = 1.0e6;
-      shift = fac * st->trial.om_tot - fac * st->wann.om_tot;
-    }
-    const double eqb = fac * st->doda0;
-    const double eqa = shift - eqb * trial_step;
-    double alphamin, falphamin;
-    int lquad;
-    if (fabs(eqa / (fac * st->wann.om_tot)) > 2.220446049250313e-16) {  // epsilon(1.0_dp)
-      lquad = 1;
-      alphamin = -0.5 * eqb / eqa * (trial_step * trial_step);
-      falphamin = st->wann.om_tot - 0.25 * eqb * eqb / (fac * eqa) * (trial_step * trial_step);
-    } else {
-      lquad = 0;
-      alphamin = trial_step;
-      falphamin = st->trial.om_tot;
-    }
-    if (st->doda0 * alphamin > 0.0) {
-      lquad = 0;
-      alphamin = trial_step;
-      falphamin = st->trial.om_tot;
-    }
-    st->alphamin = alphamin;
-    st->falphamin = falphamin;
-    st->lquad = lquad;
-    if (lquad) {
-      st->second = 1;
-      st->scale = alphamin / (4.0 * st->wbtot);
-    } else {
-      st->old = st->wann;
-      st->wann = st->trial;
-      st->cur = dst;
-      conv_step(st, logrow, st->gcnorm1);
-    }
-    return;
-  }
-  // MODE_SECOND: the parabolic step replaced the trial step
-  st->old = st->wann;
-  st->wann = sp;
-  st->cur = dst;
-  st->second = 0;
-  conv_step(st, logrow, st->gcnorm1);
-}
-
-// wann_domega (wannierise.F90:2287-2569), non-SLWF gradient, one k per
-// workgroup, fused with the per-k partial sums Re<G,G> and Re<G,D_old>
-// needed by internal_search_direction.
-__global__ void __launch_bounds__(W90_NT) k_domega(Prob p) {
-  const State* st = p.st;
-  if (st->done) return;
-  const int nw = p.nw, nn = p.nn, nw2 = p.nw2;
-  const int k = blockIdx.x;
-  const int cur = st->cur;
-  const double* ln = p.ln[cur];
-  const double* rave = p.rave[cur];
-  const cplx* Mk = p.M[cur] + (size_t)k * nn * nw2;
-  __shared__ double red[W90_NT];
-  const double rnk = (double)p.nk;
-  const cplx mhalf_i = c_make(0.0, -0.5);
-  double gg = 0.0, gd = 0.0;
-  for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
-    const int m = e % nw, n = e / nw;
-    cplx acc = c_make(0.0, 0.0);
-    for (int b = 0; b < nn; ++b) {
-      const cplx* Mb = Mk + (size_t)b * nw2;
-      const double w = p.wb[b];
-      const double* bv = p.bk + ((size_t)k * nn + b) * 3;
-      const cplx mmn = Mb[m + n * nw];
-      const cplx mnm = Mb[n + m * nw];
-      const cplx mnn = Mb[n + n * nw];
-      const cplx mmm = Mb[m + m * nw];
-      const double* lnb = ln + ((size_t)k * nn + b) * nw;
-      const double qn = w * lnb[n];
-      const double qm = w * lnb[m];
-      const double rn = bv[0] * rave[0 + 3 * n] + bv[1] * rave[1 + 3 * n] + bv[2] * rave[2 + 3 * n];
-      const double rm = bv[0] * rave[0 + 3 * m] + bv[1] * rave[1 + 3 * m] + bv[2] * rave[2 + 3 * m];
-      const cplx cr_mn = c_mul(mmn, c_conj(mnn));
-      const cplx cr_nm = c_mul(mnm, c_conj(mmm));
-      const cplx crt_mn = c_div(mmn, mnn);
-      const cplx crt_nm = c_div(mnm, mmm);
-      // A[R] = (R - R^+)/2
-      acc = c_add(acc, c_scale(c_sub(cr_mn, c_conj(cr_nm)), w * 0.5));
-      // -S[T] = -(T + T^+)/2i, T_mn = Rt_mn q_n
-      acc = c_sub(acc, c_mul(c_add(c_scale(crt_mn, qn), c_conj(c_scale(crt_nm, qm))), mhalf_i));
-      acc = c_sub(acc, c_scale(c_mul(c_add(c_scale(crt_mn, rn), c_conj(c_scale(crt_nm, rm))), mhalf_i), w));
-    }
-    const cplx g = c_scale(make_double2(acc.x / rnk, acc.y / rnk), 4.0);
-    p.G[(size_t)k * nw2 + e] = g;
-    gg += c_abs2(g);
-    const cplx dold = p.D[(size_t)k * nw2 + e];
-    gd += g.x * dold.x + g.y * dold.y;
-  }
-  gg = block_sum(gg, red);
-  gd = block_sum(gd, red);
-  if (threadIdx.x == 0) {
-    p.part_gg[k] = gg;
-    p.part_gd[k] = gd;
-  }
-}
-
-// wann_domega with each block M(k,b) staged in LDS (num_wann <= NWM): the
-// transposed and diagonal reads of the gradient formula hit LDS, and the
-// scaled phases wb Im ln M_nn and b.rave_n of the k are precomputed.
-template <int NWM>
-__global__ void __launch_bounds__(W90_NT) k_domega_k(Prob p) {
-  const State* st = p.st;
-  if (st->done) return;
-  constexpr int SL = (NWM * NWM + W90_NT - 1) / W90_NT;
-  const int nw = p.nw, nn = p.nn, nw2 = p.nw2;
-  const int k = blockIdx.x;
-  const int tid = threadIdx.x;
-  const int cur = st->cur;
-  __shared__ cplx Ms[NWM * (NWM + 1)];  // padded, leading dimension nw + 1
-  const int ld = nw + 1;
-  __shared__ double q[W90GPU_NUM_NNMAX * NWM];
-  __shared__ double rb[W90GPU_NUM_NNMAX * NWM];
-  __shared__ double red[W90_NT];
-  const double* rave = p.rave[cur];
-  for (int t = tid; t < nn * nw; t += W90_NT) {
-    const int b = t / nw, n = t % nw;
-    const size_t kb = (size_t)k * nn + b;
-    const double* bv = p.bk + kb * 3;
-    q[t] = p.wb[b] * p.ln[cur][kb * nw + n];
-    rb[t] = bv[0] * rave[0 + 3 * n] + bv[1] * rave[1 + 3 * n] + bv[2] * rave[2 + 3 * n];
-  }
-  cplx acc[SL];
-#pragma unroll
-  for (int s = 0; s < SL; ++s) acc[s] = c_make(0.0, 0.0);
-  const cplx mhalf_i = c_make(0.0, -0.5);
-  for (int b = 0; b < nn; ++b) {
-    const cplx* src = p.M[cur] + ((size_t)k * nn + b) * nw2;
-    __syncthreads();
-    for (int e = tid; e < nw2; e += W90_NT) Ms[(e % nw) + (e / nw) * ld] = src[e];
-    __syncthreads();
-    const double w = p.wb[b];
-#pragma unroll
-    for (int s = 0; s < SL; ++s) {
-      const int e = tid + s * W90_NT;
-      if (e < nw2) {
-        const int m = e % nw, n = e / nw;
-        const cplx mmn = Ms[m + n * ld];
-        const cplx mnm = Ms[n + m * ld];
-        const cplx mnn = Ms[n + n * ld];
-        const cplx mmm = Ms[m + m * ld];
-        const double qn = q[b * nw + n], qm = q[b * nw + m];
-        const double rn = rb[b * nw + n], rm = rb[b * nw + m];
-        const cplx cr_mn = c_mul(mmn, c_conj(mnn));
-        const cplx cr_nm = c_mul(mnm, c_conj(mmm));
-        const cplx crt_mn = c_div(mmn, mnn);
-        const cplx crt_nm = c_div(mnm, mmm);
-        cplx a = acc[s];
-        a = c_add(a, c_scale(c_sub(cr_mn, c_conj(cr_nm)), w * 0.5));
-        a = c_sub(a, c_mul(c_add(c_scale(crt_mn, qn), c_conj(c_scale(crt_nm, qm))), mhalf_i));
-        a = c_sub(a, c_scale(c_mul(c_add(c_scale(crt_mn, rn), c_conj(c_scale(crt_nm, rm))), mhalf_i), w));
-        acc[s] = a;
-      }
-    }
-  }
-  const double rnk = (double)p.nk;
-  double gg = 0.0, gd = 0.0;
-#pragma unroll
-  for (int s = 0; s < SL; ++s) {
-    const int e = tid + s * W90_NT;
-    if (e < nw2) {
-      const cplx g = c_scale(make_double2(acc[s].x / rnk, acc[s].y / rnk), 4.0);
-      p.G[(size_t)k * nw2 + e] = g;
-      gg += c_abs2(g);
-      const cplx dold = p.D[(size_t)k * nw2 + e];
-      gd += g.x * dold.x + g.y * dold.y;
-    }
-  }
-  gg = block_sum(gg, red);
-  gd = block_sum(gd, red);
-  if (tid == 0) {
-    p.part_gg[k] = gg;
-    p.part_gd[k] = gd;
-  }
-}
-
-// internal_search_direction (wannierise.F90:1371-1525) without
-// preconditioning or noise: Fletcher-Reeves coefficient with its resets,
-// slope along D and the uphill handling.  D itself is formed by k_step.
-__global__ void __launch_bounds__(W90_NT) k_search(Prob p) {
-  State* st = p.st;
-  if (st->done) return;
-  __shared__ double red[W90_NT];
-  double gg = 0.0, gd = 0.0;
-  // fixed-order chunked sums over k
-  const int per = (p.nk + blockDim.x - 1) / blockDim.x;
-  const int kb = threadIdx.x * per, ke = min(p.nk, kb + per);
-  for (int k = kb; k < ke; ++k) {
-    gg += p.part_gg[k];
-    gd += p.part_gd[k];
-  }
-  gg = block_sum(gg, red);
-  gd = block_sum(gd, red);
-  if (threadIdx.x != 0) return;
-  const int iter = st->iter + 1;
-  st->second = 0;
-  st->gcnorm1 = gg;
-  double gcfac;
-  if (iter == 1 || st->ncg >= st->num_cg_steps) {
-    gcfac = 0.0;
-    st->ncg = 0;
-  } else {
-    if (st->gcnorm0 > 2.220446049250313e-16) {
-      gcfac = gg / st->gcnorm0;
-      if (gcfac > 3.0) {
-        gcfac = 0.0;
-        st->ncg = 0;
-      } else {
-        st->ncg += 1;
-      }
-    } else {
-      gcfac = 0.0;
-      st->ncg = 0;
-    }
-  }
-  st->gcnorm0 = gg;
-  const double w4 = 4.0 * st->wbtot;
-  double doda0 = -(gg + gcfac * gd) / w4;
-  double beta = gcfac, sign = 1.0;
-  if (doda0 > 0.0) {
-    if (st->ncg > 0) {
-      beta = 0.0;
-      st->ncg = 0;
-      gcfac = 0.0;
-      doda0 = -gg / w4;
-      if (doda0 > 0.0) {
-        sign = -1.0;
-        doda0 = -doda0;
-      }
-    } else {
-      sign = -sign;
-      doda0 = -doda0;
-    }
-  }
-  st->gcfac = gcfac;
-  st->beta = beta;
-  st->dir_sign = sign;
-  st->doda0 = doda0;
-  st->scale = (st->lfixstep ? st->fixed_step : st->trial_step) / w4;
-}
-
-__device__ __forceinline__ void taylor_plan(double nrm, int* sq_out, int* m_out);
-
-// C = A * B for n x n column-major complex matrices held by one workgroup.
-__device__ void wg_gemm(cplx* C, const cplx* A, const cplx* B, int n) {
-  const int n2 = n * n;
-  for (int e = threadIdx.x; e < n2; e += blockDim.x) {
-    const int i = e % n, j = e / n;
-    const cplx* bc = B + (size_t)j * n;
-    cplx acc = c_make(0.0, 0.0);
-    for (int l = 0; l < n; ++l) acc = c_add(acc, c_mul(A[i + (size_t)l * n], bc[l]));
-    C[e] = acc;
-  }
-  __syncthreads();
-}
-
-// Retraction of one k: forms the search direction D (trial step only),
-// X = scale * D, E = exp(X) by scaling and squaring of a Taylor polynomial,
-// and U_new = U E.  X is anti-Hermitian, so exp(X) is the same unitary that
-// internal_new_u_and_m builds from the eigenvectors of i X
-// (wannierise.F90:1662-1689, 1729-1734).
-__global__ void __launch_bounds__(W90_NT) k_step(Prob p, int mode) {
-  const State* st = p.st;
-  if (!gate(st, mode)) return;
-  const int nw = p.nw, nw2 = p.nw2;
-  const int k = blockIdx.x;
-  const int cur = st->cur;
-  __shared__ double red[W90_NT];
-  cplx* Dk = p.D + (size_t)k * nw2;
-  cplx* X = p.W1 + (size_t)k * nw2;
-  cplx* P = p.W2 + (size_t)k * nw2;
-  cplx* Ek = p.E + (size_t)k * nw2;
-  const double scale = st->scale;
-  if (mode == MODE_TRIAL) {
-    const cplx* Gk = p.G + (size_t)k * nw2;
-    const double beta = st->beta, sign = st->dir_sign;
-    for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
-      cplx d = c_add(Gk[e], c_scale(Dk[e], beta));
-      d = c_scale(d, sign);
-      Dk[e] = d;
-      X[e] = c_scale(d, scale);
-    }
-  } else {
-    for (int e = threadIdx.x; e < nw2; e += blockDim.x) X[e] = c_scale(Dk[e], scale);
-  }
-  __syncthreads();
-  // 1-norm of X
-  double cs = 0.0;
-  for (int j = threadIdx.x; j < nw; j += blockDim.x) {
-    double s = 0.0;
-    for (int i = 0; i < nw; ++i) {
-      const cplx v = X[i + (size_t)j * nw];
-      s += sqrt(c_abs2(v));
-    }
-    cs = fmax(cs, s);
-  }
-  int sq, m;
-  taylor_plan(block_max(cs, red), &sq, &m);
-  if (sq > 0) {
-    const double f = ldexp(1.0, -sq);
-    for (int e = threadIdx.x; e < nw2; e += blockDim.x) X[e] = c_scale(X[e], f);
-  }
-  __syncthreads();
-  // Horner: P = I + X/m; P = I + X P / j, j = m-1..1
-  const double invm = 1.0 / (double)m;
-  for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
-    cplx v = c_scale(X[e], invm);
-    if (e % nw == e / nw) v.x += 1.0;
-    P[e] = v;
-  }
-  __syncthreads();
-  for (int j = m - 1; j >= 1; --j) {
-    wg_gemm(Ek, X, P, nw);
-    const double invj = 1.0 / (double)j;
-    for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
-      cplx v = c_scale(Ek[e], invj);
-      if (e % nw == e / nw) v.x += 1.0;
-      P[e] = v;
-    }
-    __syncthreads();
-  }
-  // squaring: alternate between P and X (X no longer needed)
-  cplx* a = P;
-  cplx* b = X;
-  for (int s = 0; s < sq; ++s) {
-    wg_gemm(b, a, a, nw);
-    cplx* t = a;
-    a = b;
-    b = t;
-  }
-  for (int e = threadIdx.x; e < nw2; e += blockDim.x) Ek[e] = a[e];
-  __syncthreads();
-  // U_new = U_cur E
-  wg_gemm(p.U[1 - cur] + (size_t)k * nw2, p.U[cur] + (size_t)k * nw2, Ek, nw);
-}
-
-// Degree and squarings of the Taylor approximant of exp(X) for ||X||_1 = nrm:
-// X is scaled by 2^-sq to norm <= 1/8, and the degree m is the smallest with
-// (nrm/2^sq)^(m+1)/(m+1)! < 1e-18, far below the FP64 rounding of the
-// result (the exponential of an anti-Hermitian matrix is unitary, norm 1).
-__device__ __forceinline__ void taylor_plan(double nrm, int* sq_out, int* m_out) {
-  if (!isfinite(nrm)) nrm = 0.0;  // a non-finite step yields a non-finite U
-  int sq = 0;
-  while (nrm > 0.125 && sq < 64) {
-    nrm *= 0.5;
-    ++sq;
-  }
-  int m = 1;
-  double term = nrm;
-  while (m < 24) {
-    term = term * nrm / (double)(m + 1);
-    if (term < 1e-18) break;
-    ++m;
-  }
-  *sq_out = sq;
-  *m_out = m;
-}
-
-// Retraction of one k with every matrix in LDS (num_wann <= NWM): forms D
-// (trial step), X = scale D, E = exp(X) and U_new = U E.
-template <int NWM>
-__global__ void __launch_bounds__(W90_NT) k_step_k(Prob p, int mode) {
-  const State* st = p.st;
-  if (!gate(st, mode)) return;
-  constexpr int SL = (NWM * NWM + W90_NT - 1) / W90_NT;
-  const int nw = p.nw, nw2 = p.nw2;
-  const int k = blockIdx.x;
-  const int tid = threadIdx.x;
-  const int cur = st->cur;
-  // padded column-major LDS matrices, leading dimension nw + 1
-  __shared__ cplx X[NWM * (NWM + 1)];
-  __shared__ cplx P[NWM * (NWM + 1)];
-  __shared__ cplx T[NWM * (NWM + 1)];
-  const int ld = nw + 1;
-  __shared__ double red[W90_NT];
-  cplx* Dk = p.D + (size_t)k * nw2;
-  const double scale = st->scale;
-  if (mode == MODE_TRIAL) {
-    const cplx* Gk = p.G + (size_t)k * nw2;
-    const double beta = st->beta, sign = st->dir_sign;
-    for (int e = tid; e < nw2; e += W90_NT) {
-      cplx d = c_scale(c_add(Gk[e], c_scale(Dk[e], beta)), sign);
-      Dk[e] = d;
-      X[(e % nw) + (e / nw) * ld] = c_scale(d, scale);
-    }
-  } else {
-    for (int e = tid; e < nw2; e += W90_NT) X[(e % nw) + (e / nw) * ld] = c_scale(Dk[e], scale);
-  }
-  __syncthreads();
-  double cs = 0.0;
-  for (int j = tid; j < nw; j += W90_NT) {
-    double s = 0.0;
-    for (int i = 0; i < nw; ++i) s += sqrt(c_abs2(X[i + j * ld]));
-    cs = fmax(cs, s);
-  }
-  int sq, m;
-  taylor_plan(block_max(cs, red), &sq, &m);
-  const double f = ldexp(1.0, -sq);
-  const double invm = 1.0 / (double)m;
-  for (int e = tid; e < nw2; e += W90_NT) {
-    const int q = (e % nw) + (e / nw) * ld;
-    const cplx x = c_scale(X[q], f);
-    X[q] = x;
-    cplx v = c_scale(x, invm);
-    if (e % nw == e / nw) v.x += 1.0;
-    P[q] = v;
-  }
-  __syncthreads();
-  for (int j = m - 1; j >= 1; --j) {
-    lds_gemm<NWM>(T, X, P, nw);
-    const double invj = 1.0 / (double)j;
-    for (int e = tid; e < nw2; e += W90_NT) {
-      const int q = (e % nw) + (e / nw) * ld;
-      cplx v = c_scale(T[q], invj);
-      if (e % nw == e / nw) v.x += 1.0;
-      P[q] = v;
-    }
-    __syncthreads();
-  }
-  cplx* a = P;
-  cplx* b = T;
-  for (int s = 0; s < sq; ++s) {
-    lds_gemm<NWM>(b, a, a, nw);
-    cplx* t = a;
-    a = b;
-    b = t;
-  }
-  cplx* Ek = p.E + (size_t)k * nw2;
-  const cplx* Uc = p.U[cur] + (size_t)k * nw2;
-  for (int e = tid; e < nw2; e += W90_NT) {
-    const int q = (e % nw) + (e / nw) * ld;
-    Ek[e] = a[q];
-    X[q] = Uc[e];
-  }
-  __syncthreads();
-  lds_gemm<NWM>(b, X, a, nw);
-  cplx* Un = p.U[1 - cur] + (size_t)k * nw2;
-  for (int e = tid; e < nw2; e += W90_NT) Un[e] = b[(e % nw) + (e / nw) * ld];
-}
-
-// C = A^+ B for n x n column-major complex matrices held by one workgroup.
-__device__ void wg_gemm_cn(cplx* C, const cplx* A, const cplx* B, int n) {
-  const int n2 = n * n;
-  for (int e = threadIdx.x; e < n2; e += blockDim.x) {
-    const int i = e % n, j = e / n;
-    const cplx* ac = A + (size_t)i * n;
-    const cplx* bc = B + (size_t)j * n;
-    cplx acc = c_make(0.0, 0.0);
-    for (int l = 0; l < n; ++l) acc = c_add(acc, c_cmul(ac[l], bc[l]));
-    C[e] = acc;
-  }
-  __syncthreads();
-}
-
-// Lowdin orthonormalisation of overlap_project (src/overlap.F90:758-952)
-// for num_bands = num_wann: the unitary polar factor U = A (A^+ A)^-1/2 of
-// A(k), which is the Z V^+ built there from the SVD A = Z S V^+.  Newton-
-// Schulz iteration X <- X (3 - X^+ X)/2 from X = A/||A||_F (all singular
-// values in (0,1]) converges to it for non-singular A, quadratically near
-// the end: once X^+ X = 1 to 1e-12 one more step reaches rounding level and
-// the iteration stops (at most 200 steps).  Writes U(k) to E.
-__global__ void __launch_bounds__(W90_NT) k_polar(Prob p, const cplx* A) {
-  const int nw = p.nw, nw2 = p.nw2;
-  const int k = blockIdx.x;
-  cplx* X = p.E + (size_t)k * nw2;
-  cplx* Y = p.W1 + (size_t)k * nw2;
-  cplx* Z = p.W2 + (size_t)k * nw2;
-  __shared__ double red[W90_NT];
-  const cplx* Ak = A + (size_t)k * nw2;
-  double fro = 0.0;
-  for (int e = threadIdx.x; e < nw2; e += blockDim.x) fro += c_abs2(Ak[e]);
-  fro = sqrt(block_sum(fro, red));
-  const double inv = (fro > 0.0 && isfinite(fro)) ? 1.0 / fro : 0.0;
-  for (int e = threadIdx.x; e < nw2; e += blockDim.x) X[e] = c_scale(Ak[e], inv);
-  __syncthreads();
-  for (int it = 0; it < 200; ++it) {
-    wg_gemm_cn(Y, X, X, nw);
-    double dev = 0.0;
-    for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
-      cplx y = Y[e];
-      if (e % nw == e / nw) y.x -= 1.0;
-      dev = fmax(dev, sqrt(c_abs2(y)));
-      // 3 - X^+ X, halved below
-      Y[e] = c_make(-Y[e].x + ((e % nw == e / nw) ? 3.0 : 0.0), -Y[e].y);
-    }
-    dev = block_max(dev, red);
-    if (!(dev == dev)) break;  // non-finite input: leave it to the unitarity check
-    wg_gemm(Z, X, Y, nw);
-    for (int e = threadIdx.x; e < nw2; e += blockDim.x) X[e] = c_scale(Z[e], 0.5);
-    __syncthreads();
-    if (dev < 1e-12) break;
-  }
-}
-
-// wann_check_unitarity (wannierise.F90:2594-2661): per-k maximum deviation
-// of U U^+ and U^+ U from the identity.
-__global__ void __launch_bounds__(W90_NT) k_unitarity(Prob p) {
-  const int nw = p.nw, nw2 = p.nw2;
-  const int k = blockIdx.x;
-  const cplx* U = p.U[p.st->cur] + (size_t)k * nw2;
-  __shared__ double red[W90_NT];
-  double dev = 0.0;
-  for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
-    const int i = e % nw, j = e / nw;
-    cplx c1 = c_make(0, 0), c2 = c_make(0, 0);
-    for (int m = 0; m < nw; ++m) {
-      c1 = c_add(c1, c_mul(U[i + m * nw], c_conj(U[j + m * nw])));
-      c2 = c_add(c2, c_mul(U[m + j * nw], c_conj(U[m + i * nw])));
-    }
-    if (i == j) {
-      c1.x -= 1.0;
-      c2.x -= 1.0;
-    }
-    // NaN-propagating maximum: a NaN entry must fail the check
-    const double v = fmax(sqrt(c_abs2(c1)), sqrt(c_abs2(c2)));
-    if (!(v <= dev)) dev = isnan(v) ? INFINITY : fmax(v, dev);
-    if (isnan(c1.x) || isnan(c1.y) || isnan(c2.x) || isnan(c2.y)) dev = INFINITY;
-  }
-  dev = block_max(dev, red);
-  if (threadIdx.x == 0) p.unit[k] = dev;
-}
-
-// =============================================================== host side
+#include "w90gpu_kernels.h"
 
 struct w90gpu_ctx {
   int device = 0;
   std::string err;
 };
 
-struct w90gpu_session {
-  w90gpu_ctx* ctx = nullptr;
-  w90gpu_control ctl{};
-  Prob p{};
+// Host side of libw90gpu: the engine that runs a set of problems on one
+// device, and the C ABI of include/w90gpu.h on top of it.
+//
+// Engine.  Problems live in one device pool (sub-allocated first-fit, so
+// admission is a question of free bytes).  Each problem has a slot in a
+// device array of Prob; the problems that are running form a membership
+// epoch, for which the host builds one item list (problem, k) sorted by
+// kernel class and storage precision, and one problem list.  An iteration
+// is the fixed launch sequence of wann_main over these lists; a block of
+// kGraphBlock iterations is captured once per epoch into a HIP graph and
+// replayed.  After each block a status kernel copies done/hold flags to
+// pinned host memory; two blocks are kept in flight, so the host reads the
+// flags of block j while block j+1 runs.  Finished problems are retired
+// (final state gathered, unitarity checked) and held problems get their
+// FP64 rebuild of M; either starts a new epoch.  A session is an engine
+// with one problem.
+
+struct Member {
+  int id = -1;
+  int slot = -1;
   int nw = 0, nk = 0, nn = 0;
+  int cls = 0;     // kernel class by num_wann
+  int phase = 1;   // 0: mixed bulk, 1: FP64 (host view)
+  int stage = 0;   // 0 idle, 1 queued, 2 running, 3 finished
+  w90gpu_control ctl{};
   double wbtot = 0.0;
-  hipStream_t stream = nullptr;
-  void* arena = nullptr;
-  size_t arena_bytes = 0;
+  size_t off = 0, bytes = 0;  // sub-allocation in the pool
+  Prob p{};
+  State* st_init = nullptr;   // device template of the loop state
   cplx* M0 = nullptr;
   cplx* U0 = nullptr;
-  State* st_host = nullptr;  // pinned
-  bool loaded = false;
+  w90gpu_result res{};
+  double t_start = 0.0, t_end = 0.0;
+};
+
+// kernel classes by num_wann: b-parallel LDS rotation up to 8 and 16, the
+// per-k LDS kernels up to 32, the panel kernels above
+constexpr int kClasses = 4;
+constexpr int kLdsMaxWann = 32;
+// iterations per captured block; the solve is polled once per block
+constexpr int kGraphBlock = 8;
+
+int wann_class(int nw) { return nw <= 8 ? 0 : nw <= 16 ? 1 : nw <= kLdsMaxWann ? 2 : 3; }
+int npanels(int nw) { return (nw + W90_PANEL - 1) / W90_PANEL; }
+
+// A contiguous range of an item list with the launch shape its kernels need.
+struct Seg {
+  int off = 0, n = 0, nnmax = 0, npanmax = 0;
+};
+
+// Device lists of one launch sequence: items sorted by (class, precision),
+// problem indices for the per-problem kernels.
+struct Lists {
+  const Item* items = nullptr;
+  const int* plist = nullptr;
+  int np = 0;
+  Seg seg[kClasses][2];  // [class][0: FP64, 1: FP32]
+  bool any_second = false;
+};
+
+struct Pool {
+  char* base = nullptr;
+  size_t size = 0;
+  std::map<size_t, size_t> free_;  // offset -> bytes
+  size_t used = 0;
+  bool alloc(size_t bytes, size_t* off) {
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        *off = it->first;
+        const size_t rest = it->second - bytes;
+        const size_t o = it->first + bytes;
+        free_.erase(it);
+        if (rest) free_[o] = rest;
+        used += bytes;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(size_t off, size_t bytes) {
+    used -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
+
+struct Engine {
+  w90gpu_ctx* ctx = nullptr;
+  hipStream_t stream = nullptr;   // compute
+  hipStream_t copy = nullptr;     // uploads of admitted problems
+  Pool pool;
+  std::vector<Member*> slots;     // by slot; nullptr = free
+  std::map<int, Member*> by_id;
+  int next_id = 0;
+  // device tables
+  Prob* d_probs = nullptr;
+  int cap_probs = 0;
+  Item* d_items = nullptr;        // run lists
+  int* d_plist = nullptr;
+  Item* d_aux_items = nullptr;    // start / rebuild / retire lists
+  int* d_aux_plist = nullptr;
+  int cap_items = 0, cap_plist = 0;
+  Status* d_status = nullptr;     // [2][cap_plist]
+  Status* h_status = nullptr;     // pinned
+  Final* d_final = nullptr;       // [cap_plist]
+  Final* h_final = nullptr;       // pinned
+  // run lists of the current epoch
+  long epoch = 0, lists_epoch = -1;
+  Lists run;
+  std::vector<int> run_slots;
+  int epoch_blocks = 0;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  std::vector<std::pair<hipGraph_t, hipGraphExec_t>> retired;
+  bool graph_failed = false;
+  bool use_graph = true;
+  hipEvent_t blk_ev[2] = {nullptr, nullptr};
+  std::vector<int> blk_slots[2];
+  // profiling of the rotation kernel
   bool prof = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
   size_t ev_used = 0;
   double prof_ms = 0.0;
   int prof_launches = 0;
   hipEvent_t t0 = nullptr, t1 = nullptr;
-  w90gpu_result last{};
-  // a captured block of kGraphBlock iterations, replayed until done
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  bool graph_failed = false;
-  // pinned copies of the done flag of the two blocks in flight
-  int* flags_host = nullptr;
-  hipEvent_t flag_ev[2] = {nullptr, nullptr};
+  std::vector<int> finished_ids;  // retired since the last report
 };
 
-// iterations per captured block; the solve is polled once per block
-constexpr int kGraphBlock = 8;
+struct w90gpu_session {
+  Engine* eng = nullptr;
+  Member* m = nullptr;
+  bool loaded = false;
+  w90gpu_result last{};
+};
+
+struct w90gpu_batch {
+  Engine* eng = nullptr;
+};
 
 namespace {
 
@@ -1379,45 +191,41 @@ void set_error(w90gpu_ctx* ctx, const char* fmt, ...) {
   else g_noctx_err = buf;
 }
 
-#define HIPCHK(ctx, call)                                                                 \
-  do {                                                                                    \
-    hipError_t e_ = (call);                                                               \
-    if (e_ != hipSuccess) {                                                               \
+#define HIPCHK(ctx, call)                                                                         \
+  do {                                                                                            \
+    hipError_t e_ = (call);                                                                       \
+    if (e_ != hipSuccess) {                                                                       \
       set_error((ctx), "HIP error %s at %s:%d (%s)", hipGetErrorString(e_), __FILE__, __LINE__, \
-                #call);                                                                   \
-      return W90GPU_ERR_FATAL;                                                            \
-    }                                                                                     \
+                #call);                                                                           \
+      return W90GPU_ERR_FATAL;                                                                    \
+    }                                                                                             \
   } while (0)
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Arena of one problem: offsets in a fixed order that member_bind mirrors.
 struct Layout {
   size_t off[40];
   int n = 0;
   size_t total = 0;
-  size_t add(size_t bytes) {
-    const size_t o = total;
-    off[n++] = o;
+  void add(size_t bytes) {
+    off[n++] = total;
     total += align_up(bytes);
-    return o;
   }
 };
 
-int npanels(int nw) { return (nw + W90_PANEL - 1) / W90_PANEL; }
-
-// LDS-resident per-k kernels cover num_wann up to 32 (four 32x32 complex
-// blocks = 64 KiB of LDS); larger num_wann use the panel kernels.
-constexpr int kLdsMaxWann = 32;
-
-// Compute the arena layout of a session; fills offsets in `lay` in a fixed
-// order that session_create mirrors.
-size_t session_layout(int nw, int nk, int nn, int num_iter, Layout& lay) {
+size_t problem_layout(int nw, int nk, int nn, int num_iter, Layout& lay) {
   const size_t nw2 = (size_t)nw * nw;
   const size_t mb = nw2 * nn * nk * sizeof(cplx);
   const size_t ub = nw2 * nk * sizeof(cplx);
   const size_t lnb = (size_t)nk * nn * nw * sizeof(double);
+  const size_t psz = 8 * (size_t)nw + 8;
   lay.add(mb);   // M[0]
-  lay.add(mb);   // M[1]
+  lay.add(mb);   // M[1] (holds the two FP32 buffers of the mixed bulk phase)
   lay.add(mb);   // M0
   lay.add(ub);   // U[0]
   lay.add(ub);   // U[1]
@@ -1432,8 +240,8 @@ size_t session_layout(int nw, int nk, int nn, int num_iter, Layout& lay) {
   lay.add(lnb);  // d2
   lay.add((size_t)nk * nn * npanels(nw) * sizeof(double));  // part_od
   lay.add((size_t)nk * nn * npanels(nw) * sizeof(double));  // part_all
-  lay.add((size_t)nk * (8 * nw + 8) * sizeof(double));      // kpart
-  lay.add((size_t)W90_MAX_GROUPS * (8 * nw + 8) * sizeof(double));  // gpart
+  lay.add((size_t)nk * psz * sizeof(double));               // kpart
+  lay.add((size_t)W90_MAX_GROUPS * psz * sizeof(double));   // gpart
   lay.add((size_t)nk * sizeof(double));                     // part_gg
   lay.add((size_t)nk * sizeof(double));                     // part_gd
   lay.add(3 * (size_t)nw * sizeof(double));                 // rave[0]
@@ -1443,6 +251,7 @@ size_t session_layout(int nw, int nk, int nn, int num_iter, Layout& lay) {
   lay.add((size_t)nk * sizeof(double));                     // unit
   lay.add(4 * (size_t)(num_iter + 2) * sizeof(double));     // log
   lay.add(sizeof(State));                                   // state
+  lay.add(sizeof(State));                                   // state template
   lay.add((size_t)nk * nn * sizeof(int));                   // nnl
   lay.add((size_t)nk * nn * 3 * sizeof(double));            // bk
   lay.add((size_t)nn * sizeof(double));                     // wb
@@ -1491,189 +300,798 @@ int validate(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90gpu_control* 
   return W90GPU_OK;
 }
 
-// launch helpers ----------------------------------------------------------
+// ------------------------------------------------------------ engine setup
+
+int grow_tables(Engine* E, int nprob, int nitems) {
+  w90gpu_ctx* ctx = E->ctx;
+  if (nprob > E->cap_probs) {
+    const int cap = std::max(nprob, 2 * E->cap_probs);
+    Prob* np_ = nullptr;
+    HIPCHK(ctx, hipMalloc(&np_, sizeof(Prob) * cap));
+    if (E->d_probs) {
+      HIPCHK(ctx, hipMemcpyAsync(np_, E->d_probs, sizeof(Prob) * E->cap_probs, hipMemcpyDeviceToDevice, E->stream));
+      HIPCHK(ctx, hipStreamSynchronize(E->stream));
+      HIPCHK(ctx, hipFree(E->d_probs));
+    }
+    E->d_probs = np_;
+    E->cap_probs = cap;
+  }
+  if (nprob > E->cap_plist) {
+    const int cap = std::max(nprob, 2 * E->cap_plist);
+    HIPCHK(ctx, hipStreamSynchronize(E->stream));
+    for (void* q : {(void*)E->d_plist, (void*)E->d_aux_plist, (void*)E->d_status, (void*)E->d_final})
+      if (q) HIPCHK(ctx, hipFree(q));
+    if (E->h_status) HIPCHK(ctx, hipHostFree(E->h_status));
+    if (E->h_final) HIPCHK(ctx, hipHostFree(E->h_final));
+    E->d_plist = E->d_aux_plist = nullptr;
+    E->d_status = nullptr;
+    E->d_final = nullptr;
+    E->h_status = nullptr;
+    E->h_final = nullptr;
+    HIPCHK(ctx, hipMalloc(&E->d_plist, sizeof(int) * cap));
+    HIPCHK(ctx, hipMalloc(&E->d_aux_plist, sizeof(int) * cap));
+    HIPCHK(ctx, hipMalloc(&E->d_status, sizeof(Status) * 2 * cap));
+    HIPCHK(ctx, hipMalloc(&E->d_final, sizeof(Final) * cap));
+    HIPCHK(ctx, hipHostMalloc((void**)&E->h_status, sizeof(Status) * 2 * cap, hipHostMallocDefault));
+    HIPCHK(ctx, hipHostMalloc((void**)&E->h_final, sizeof(Final) * cap, hipHostMallocDefault));
+    E->cap_plist = cap;
+    E->epoch++;  // run lists must be uploaded again
+  }
+  if (nitems > E->cap_items) {
+    const int cap = std::max(nitems, 2 * E->cap_items);
+    HIPCHK(ctx, hipStreamSynchronize(E->stream));
+    if (E->d_items) HIPCHK(ctx, hipFree(E->d_items));
+    if (E->d_aux_items) HIPCHK(ctx, hipFree(E->d_aux_items));
+    E->d_items = E->d_aux_items = nullptr;
+    HIPCHK(ctx, hipMalloc(&E->d_items, sizeof(Item) * cap));
+    HIPCHK(ctx, hipMalloc(&E->d_aux_items, sizeof(Item) * cap));
+    E->cap_items = cap;
+    E->epoch++;
+  }
+  return W90GPU_OK;
+}
+
+void drop_graph(Engine* E) {
+  if (E->gexec || E->graph) E->retired.emplace_back(E->graph, E->gexec);
+  E->graph = nullptr;
+  E->gexec = nullptr;
+  E->epoch_blocks = 0;
+}
+
+// Graphs of past epochs are destroyed once the stream has drained.
+void reap_graphs(Engine* E) {
+  for (auto& g : E->retired) {
+    if (g.second) (void)hipGraphExecDestroy(g.second);
+    if (g.first) (void)hipGraphDestroy(g.first);
+  }
+  E->retired.clear();
+}
+
+void engine_destroy(Engine* E);
+
+int engine_create(w90gpu_ctx* ctx, size_t pool_bytes, Engine** out) {
+  *out = nullptr;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  Engine* E = new (std::nothrow) Engine();
+  if (!E) return W90GPU_ERR_ALLOC;
+  E->ctx = ctx;
+  auto fail = [&](int code) {
+    // partial engine: release what was made
+    if (E->stream) (void)hipStreamDestroy(E->stream);
+    if (E->copy) (void)hipStreamDestroy(E->copy);
+    if (E->pool.base) (void)hipFree(E->pool.base);
+    delete E;
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&E->copy, hipStreamNonBlocking) != hipSuccess) {
+    set_error(ctx, "hipStreamCreate failed");
+    return fail(W90GPU_ERR_FATAL);
+  }
+  pool_bytes = align_up(std::max<size_t>(pool_bytes, 256));
+  hipError_t e = hipMalloc((void**)&E->pool.base, pool_bytes);
+  if (e != hipSuccess) {
+    E->pool.base = nullptr;
+    (void)hipGetLastError();
+    set_error(ctx, "hipMalloc of a %zu-byte pool failed: %s", pool_bytes, hipGetErrorString(e));
+    return fail(W90GPU_ERR_ALLOC);
+  }
+  E->pool.size = pool_bytes;
+  E->pool.free_[0] = pool_bytes;
+  if (hipEventCreate(&E->t0) != hipSuccess || hipEventCreate(&E->t1) != hipSuccess ||
+      hipEventCreateWithFlags(&E->blk_ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&E->blk_ev[1], hipEventDisableTiming) != hipSuccess) {
+    set_error(ctx, "hipEventCreate failed");
+    return fail(W90GPU_ERR_FATAL);
+  }
+  E->use_graph = std::getenv("W90GPU_NO_GRAPH") == nullptr;
+  if (grow_tables(E, 4, 64)) {
+    engine_destroy(E);
+    return W90GPU_ERR_FATAL;
+  }
+  *out = E;
+  return W90GPU_OK;
+}
+
+void engine_destroy(Engine* E) {
+  if (!E) return;
+  if (E->stream) (void)hipStreamSynchronize(E->stream);
+  if (E->copy) (void)hipStreamSynchronize(E->copy);
+  drop_graph(E);
+  reap_graphs(E);
+  for (auto& kv : E->by_id) delete kv.second;
+  for (auto& ev : E->evs) {
+    (void)hipEventDestroy(ev.first);
+    (void)hipEventDestroy(ev.second);
+  }
+  for (hipEvent_t ev : {E->t0, E->t1, E->blk_ev[0], E->blk_ev[1]})
+    if (ev) (void)hipEventDestroy(ev);
+  for (void* q : {(void*)E->d_probs, (void*)E->d_items, (void*)E->d_plist, (void*)E->d_aux_items,
+                  (void*)E->d_aux_plist, (void*)E->d_status, (void*)E->d_final, (void*)E->pool.base})
+    if (q) (void)hipFree(q);
+  if (E->h_status) (void)hipHostFree(E->h_status);
+  if (E->h_final) (void)hipHostFree(E->h_final);
+  if (E->stream) (void)hipStreamDestroy(E->stream);
+  if (E->copy) (void)hipStreamDestroy(E->copy);
+  delete E;
+}
+
+// Point the Prob of a member at its arena.
+void member_bind(Member* m, char* a, const Layout& lay) {
+  int i = 0;
+  Prob& p = m->p;
+  p.nw = m->nw;
+  p.nk = m->nk;
+  p.nn = m->nn;
+  p.nw2 = m->nw * m->nw;
+  p.npan = npanels(m->nw);
+  p.psz = 8 * m->nw + 8;
+  p.ngroup = std::min(m->nk, W90_MAX_GROUPS);
+  p.M[0] = (cplx*)(a + lay.off[i++]);
+  p.M[1] = (cplx*)(a + lay.off[i++]);
+  m->M0 = (cplx*)(a + lay.off[i++]);
+  p.M0 = m->M0;
+  const size_t mcount = (size_t)p.nw2 * m->nn * m->nk;
+  p.Mf[0] = (float2*)p.M[1];
+  p.Mf[1] = p.Mf[0] + mcount;
+  p.U[0] = (cplx*)(a + lay.off[i++]);
+  p.U[1] = (cplx*)(a + lay.off[i++]);
+  m->U0 = (cplx*)(a + lay.off[i++]);
+  p.U0 = m->U0;
+  p.G = (cplx*)(a + lay.off[i++]);
+  p.D = (cplx*)(a + lay.off[i++]);
+  p.E = (cplx*)(a + lay.off[i++]);
+  p.W1 = (cplx*)(a + lay.off[i++]);
+  p.W2 = (cplx*)(a + lay.off[i++]);
+  p.ln[0] = (double*)(a + lay.off[i++]);
+  p.ln[1] = (double*)(a + lay.off[i++]);
+  p.d2 = (double*)(a + lay.off[i++]);
+  p.part_od = (double*)(a + lay.off[i++]);
+  p.part_all = (double*)(a + lay.off[i++]);
+  p.kpart = (double*)(a + lay.off[i++]);
+  p.gpart = (double*)(a + lay.off[i++]);
+  p.part_gg = (double*)(a + lay.off[i++]);
+  p.part_gd = (double*)(a + lay.off[i++]);
+  p.rave[0] = (double*)(a + lay.off[i++]);
+  p.rave[1] = (double*)(a + lay.off[i++]);
+  p.r2ave[0] = (double*)(a + lay.off[i++]);
+  p.r2ave[1] = (double*)(a + lay.off[i++]);
+  p.unit = (double*)(a + lay.off[i++]);
+  p.log = (double*)(a + lay.off[i++]);
+  p.st = (State*)(a + lay.off[i++]);
+  m->st_init = (State*)(a + lay.off[i++]);
+  p.st_init = m->st_init;
+  p.nnl = (const int*)(a + lay.off[i++]);
+  p.bk = (const double*)(a + lay.off[i++]);
+  p.wb = (const double*)(a + lay.off[i++]);
+}
+
+State state_template(const Member* m, int first_pass, double omega_invariant) {
+  State s;
+  std::memset(&s, 0, sizeof(s));
+  s.num_iter = m->ctl.num_iter;
+  s.num_cg_steps = m->ctl.num_cg_steps;
+  s.conv_window = m->ctl.conv_window;
+  s.lfixstep = m->ctl.lfixstep;
+  s.conv_tol = m->ctl.conv_tol;
+  s.trial_step = m->ctl.trial_step;
+  s.fixed_step = m->ctl.fixed_step;
+  s.wbtot = m->wbtot;
+  s.first_pass = first_pass;
+  s.omega_invariant = omega_invariant;
+  s.dir_sign = 1.0;
+  s.phase = (m->ctl.precision == W90GPU_PREC_MIXED) ? 0 : 1;
+  return s;
+}
+
+int upload_state_template(Engine* E, Member* m, int first_pass, double omega_invariant) {
+  const State s = state_template(m, first_pass, omega_invariant);
+  HIPCHK(E->ctx, hipMemcpyAsync(m->st_init, &s, sizeof(State), hipMemcpyHostToDevice, E->stream));
+  HIPCHK(E->ctx, hipStreamSynchronize(E->stream));
+  return W90GPU_OK;
+}
+
+// Admit a problem: pool bytes, a slot, its k-mesh tables and state template.
+// M0/U0 are uploaded separately (member_upload).
+int engine_add(Engine* E, const w90gpu_problem* prob, const w90gpu_control* ctl, Member** out) {
+  w90gpu_ctx* ctx = E->ctx;
+  int rc = validate(ctx, prob, ctl);
+  if (rc) return rc;
+  Layout lay;
+  const size_t bytes = problem_layout(prob->num_wann, prob->num_kpts, prob->nntot, ctl->num_iter, lay);
+  size_t off = 0;
+  if (!E->pool.alloc(bytes, &off)) {
+    set_error(ctx, "no room: problem needs %zu bytes, pool has %zu of %zu free", bytes,
+              E->pool.size - E->pool.used, E->pool.size);
+    return W90GPU_ERR_ALLOC;
+  }
+  Member* m = new (std::nothrow) Member();
+  if (!m) {
+    E->pool.release(off, bytes);
+    return W90GPU_ERR_ALLOC;
+  }
+  m->nw = prob->num_wann;
+  m->nk = prob->num_kpts;
+  m->nn = prob->nntot;
+  m->cls = wann_class(m->nw);
+  m->ctl = *ctl;
+  m->wbtot = prob->wbtot;
+  m->off = off;
+  m->bytes = bytes;
+  m->phase = (ctl->precision == W90GPU_PREC_MIXED) ? 0 : 1;
+  member_bind(m, E->pool.base + off, lay);
+  int slot = -1;
+  for (size_t i = 0; i < E->slots.size(); ++i)
+    if (!E->slots[i]) {
+      slot = (int)i;
+      break;
+    }
+  if (slot < 0) {
+    slot = (int)E->slots.size();
+    E->slots.push_back(nullptr);
+  }
+  auto fail = [&](int code) {
+    E->pool.release(off, bytes);
+    delete m;
+    return code;
+  };
+  if (grow_tables(E, (int)E->slots.size(), 0)) return fail(W90GPU_ERR_FATAL);
+  // k-mesh tables: nnlist (nk,nn) Fortran -> [k][b] 0-based
+  std::vector<int> hn((size_t)m->nk * m->nn);
+  for (int k = 0; k < m->nk; ++k)
+    for (int b = 0; b < m->nn; ++b) hn[(size_t)k * m->nn + b] = prob->nnlist[k + (size_t)m->nk * b] - 1;
+  const State s = state_template(m, 1, 0.0);
+  if (hipMemcpyAsync((void*)m->p.nnl, hn.data(), hn.size() * sizeof(int), hipMemcpyHostToDevice, E->stream) !=
+          hipSuccess ||
+      hipMemcpyAsync((void*)m->p.bk, prob->bk, (size_t)m->nk * m->nn * 3 * sizeof(double), hipMemcpyHostToDevice,
+                     E->stream) != hipSuccess ||
+      hipMemcpyAsync((void*)m->p.wb, prob->wb, (size_t)m->nn * sizeof(double), hipMemcpyHostToDevice,
+                     E->stream) != hipSuccess ||
+      hipMemcpyAsync(m->st_init, &s, sizeof(State), hipMemcpyHostToDevice, E->stream) != hipSuccess ||
+      hipMemcpyAsync(m->p.st, &s, sizeof(State), hipMemcpyHostToDevice, E->stream) != hipSuccess ||
+      hipMemcpyAsync(E->d_probs + slot, &m->p, sizeof(Prob), hipMemcpyHostToDevice, E->stream) != hipSuccess ||
+      hipStreamSynchronize(E->stream) != hipSuccess) {
+    set_error(ctx, "upload of k-mesh tables failed");
+    return fail(W90GPU_ERR_FATAL);
+  }
+  m->slot = slot;
+  m->id = E->next_id++;
+  E->slots[slot] = m;
+  E->by_id[m->id] = m;
+  *out = m;
+  return W90GPU_OK;
+}
+
+// Upload M0/U0 of a member from the host on the copy stream; the call
+// returns once the caller's arrays have been read.
+int member_upload(Engine* E, Member* m, const double* m_matrix, const double* u_matrix) {
+  const size_t mb = (size_t)m->nw * m->nw * m->nn * m->nk * sizeof(cplx);
+  const size_t ub = (size_t)m->nw * m->nw * m->nk * sizeof(cplx);
+  HIPCHK(E->ctx, hipMemcpyAsync(m->M0, m_matrix, mb, hipMemcpyHostToDevice, E->copy));
+  HIPCHK(E->ctx, hipMemcpyAsync(m->U0, u_matrix, ub, hipMemcpyHostToDevice, E->copy));
+  HIPCHK(E->ctx, hipStreamSynchronize(E->copy));
+  return W90GPU_OK;
+}
+
+void engine_remove(Engine* E, Member* m) {
+  E->pool.release(m->off, m->bytes);
+  E->slots[m->slot] = nullptr;
+  E->by_id.erase(m->id);
+  if (m->stage == 2) E->epoch++;
+  delete m;
+}
+
+// ------------------------------------------------------------- lists
+
+// Build lists for a set of members: items sorted by (class, precision).
+// prec_of(m) gives 0 (FP64) or 1 (FP32) for each member.
+template <class F>
+void build_lists(const std::vector<Member*>& ms, F prec_of, std::vector<Item>& items, std::vector<int>& plist,
+                 Lists& L) {
+  items.clear();
+  plist.clear();
+  L = Lists{};
+  for (int c = 0; c < kClasses; ++c)
+    for (int q = 0; q < 2; ++q) {
+      Seg& s = L.seg[c][q];
+      s.off = (int)items.size();
+      for (Member* m : ms) {
+        if (m->cls != c || prec_of(m) != q) continue;
+        for (int k = 0; k < m->nk; ++k) items.push_back(Item{m->slot, k});
+        s.nnmax = std::max(s.nnmax, m->nn);
+        s.npanmax = std::max(s.npanmax, npanels(m->nw));
+      }
+      s.n = (int)items.size() - s.off;
+    }
+  for (Member* m : ms) {
+    plist.push_back(m->slot);
+    if (!m->ctl.lfixstep) L.any_second = true;
+  }
+  L.np = (int)plist.size();
+}
+
+int upload_lists(Engine* E, const std::vector<Item>& items, const std::vector<int>& plist, Item* d_items,
+                 int* d_plist, Lists& L) {
+  if (!items.empty())
+    HIPCHK(E->ctx, hipMemcpyAsync(d_items, items.data(), sizeof(Item) * items.size(), hipMemcpyHostToDevice,
+                                  E->stream));
+  if (!plist.empty())
+    HIPCHK(E->ctx, hipMemcpyAsync(d_plist, plist.data(), sizeof(int) * plist.size(), hipMemcpyHostToDevice,
+                                  E->stream));
+  L.items = d_items;
+  L.plist = d_plist;
+  return W90GPU_OK;
+}
+
+// Aux lists (start, rebuild, retire) for a set of members, uploaded in
+// stream order.  Every member in FP64 precision unless fp32 is set.
+int aux_lists(Engine* E, const std::vector<Member*>& ms, Lists& L) {
+  int nitems = 0;
+  for (Member* m : ms) nitems += m->nk;
+  int rc = grow_tables(E, (int)E->slots.size(), nitems);
+  if (rc) return rc;
+  std::vector<Item> items;
+  std::vector<int> plist;
+  build_lists(ms, [](const Member*) { return 0; }, items, plist, L);
+  // pageable sources are staged by the copy call itself; keep them alive
+  // until the stream reaches the copy
+  rc = upload_lists(E, items, plist, E->d_aux_items, E->d_aux_plist, L);
+  if (rc) return rc;
+  HIPCHK(E->ctx, hipStreamSynchronize(E->stream));
+  return W90GPU_OK;
+}
+
+// ---------------------------------------------------------- launches
+
+int prof_begin(Engine* E, hipEvent_t* b) {
+  if (!E->prof) return W90GPU_OK;
+  if (E->ev_used >= E->evs.size()) {
+    hipEvent_t e0, e1;
+    HIPCHK(E->ctx, hipEventCreate(&e0));
+    HIPCHK(E->ctx, hipEventCreate(&e1));
+    E->evs.emplace_back(e0, e1);
+  }
+  HIPCHK(E->ctx, hipEventRecord(E->evs[E->ev_used].first, E->stream));
+  *b = E->evs[E->ev_used].second;
+  ++E->ev_used;
+  return W90GPU_OK;
+}
 
 // The rotation of M and the reduction of its diagonal into the per-k
-// partial sums of wann_omega (one fused kernel when the blocks fit in LDS).
-int launch_rotate(w90gpu_session* s, int mode) {
-  hipEvent_t a = nullptr, b = nullptr;
-  if (s->prof) {
-    if (s->ev_used >= s->evs.size()) {
-      hipEvent_t e0, e1;
-      HIPCHK(s->ctx, hipEventCreate(&e0));
-      HIPCHK(s->ctx, hipEventCreate(&e1));
-      s->evs.emplace_back(e0, e1);
-    }
-    a = s->evs[s->ev_used].first;
-    b = s->evs[s->ev_used].second;
-    ++s->ev_used;
-    HIPCHK(s->ctx, hipEventRecord(a, s->stream));
-  }
-  const dim3 perk(s->nk);
-  if (s->nw <= 8) {
-    hipLaunchKernelGGL(k_rotate_kb<8>, perk, dim3(64 * s->nn), rotate_kb_lds<8>(s->nn), s->stream, s->p,
-                       mode);
-  } else if (s->nw <= 16) {
-    hipLaunchKernelGGL(k_rotate_kb<16>, perk, dim3(64 * s->nn), rotate_kb_lds<16>(s->nn), s->stream, s->p,
-                       mode);
-  } else if (s->nw <= kLdsMaxWann) {
-    hipLaunchKernelGGL(k_rotate_k<kLdsMaxWann>, perk, dim3(W90_NT), 0, s->stream, s->p, mode);
-  } else {
-    const unsigned nblk = (unsigned)((size_t)s->nk * s->nn * s->p.npan);
-    hipLaunchKernelGGL(k_rotate, dim3(nblk), dim3(W90_NT), 0, s->stream, s->p, mode);
-  }
-  HIPCHK(s->ctx, hipGetLastError());
-  if (s->prof) HIPCHK(s->ctx, hipEventRecord(b, s->stream));
-  if (s->nw > kLdsMaxWann) {
-    hipLaunchKernelGGL(k_kpart, perk, dim3(W90_NT), 0, s->stream, s->p, mode);
-    HIPCHK(s->ctx, hipGetLastError());
-  }
-  return W90GPU_OK;
-}
-
-int launch_omega(w90gpu_session* s, int mode) {
-  int rc = launch_rotate(s, mode);
+// partial sums of wann_omega, over every class and precision of the lists.
+int launch_rotate(Engine* E, const Lists& L, int mode) {
+  hipStream_t s = E->stream;
+  const Prob* P = E->d_probs;
+  hipEvent_t pe = nullptr;
+  int rc = prof_begin(E, &pe);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_omega_group, dim3(s->p.ngroup), dim3(W90_NT), 0, s->stream, s->p, mode);
-  HIPCHK(s->ctx, hipGetLastError());
-  hipLaunchKernelGGL(k_omega_final, dim3(1), dim3(W90_FINAL_NT), 0, s->stream, s->p, mode);
-  HIPCHK(s->ctx, hipGetLastError());
-  return W90GPU_OK;
-}
-
-int launch_step(w90gpu_session* s, int mode) {
-  if (s->nw <= 16) {
-    hipLaunchKernelGGL(k_step_k<16>, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p, mode);
-  } else if (s->nw <= kLdsMaxWann) {
-    hipLaunchKernelGGL(k_step_k<kLdsMaxWann>, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p, mode);
-  } else {
-    hipLaunchKernelGGL(k_step, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p, mode);
+  for (int c = 0; c < kClasses; ++c)
+    for (int q = 0; q < 2; ++q) {
+      const Seg& g = L.seg[c][q];
+      if (g.n == 0) continue;
+      const Item* it = L.items + g.off;
+      const dim3 grid(g.n);
+      if (c == 0) {
+        const dim3 blk(64 * g.nnmax);
+        if (q == 0)
+          hipLaunchKernelGGL((k_rotate_kb<8, cplx>), grid, blk, (rotate_kb_lds<8, cplx>(g.nnmax)), s, P, it, g.n,
+                             mode);
+        else
+          hipLaunchKernelGGL((k_rotate_kb<8, float2>), grid, blk, (rotate_kb_lds<8, float2>(g.nnmax)), s, P, it,
+                             g.n, mode);
+      } else if (c == 1) {
+        const dim3 blk(64 * g.nnmax);
+        if (q == 0)
+          hipLaunchKernelGGL((k_rotate_kb<16, cplx>), grid, blk, (rotate_kb_lds<16, cplx>(g.nnmax)), s, P, it,
+                             g.n, mode);
+        else
+          hipLaunchKernelGGL((k_rotate_kb<16, float2>), grid, blk, (rotate_kb_lds<16, float2>(g.nnmax)), s, P,
+                             it, g.n, mode);
+      } else if (c == 2) {
+        if (q == 0)
+          hipLaunchKernelGGL((k_rotate_k<kLdsMaxWann, cplx>), grid, dim3(W90_NT), 0, s, P, it, g.n, mode);
+        else
+          hipLaunchKernelGGL((k_rotate_k<kLdsMaxWann, float2>), grid, dim3(W90_NT), 0, s, P, it, g.n, mode);
+      } else {
+        const dim3 g2(g.n, g.nnmax * g.npanmax);
+        if (q == 0)
+          hipLaunchKernelGGL(k_rotate<cplx>, g2, dim3(W90_NT), 0, s, P, it, g.n, mode);
+        else
+          hipLaunchKernelGGL(k_rotate<float2>, g2, dim3(W90_NT), 0, s, P, it, g.n, mode);
+      }
+      HIPCHK(E->ctx, hipGetLastError());
+    }
+  if (pe) HIPCHK(E->ctx, hipEventRecord(pe, s));
+  // per-k partial vectors of the panel class
+  for (int q = 0; q < 2; ++q) {
+    const Seg& g = L.seg[3][q];
+    if (g.n == 0) continue;
+    hipLaunchKernelGGL(k_kpart, dim3(g.n), dim3(W90_NT), 0, s, P, L.items + g.off, g.n, mode);
+    HIPCHK(E->ctx, hipGetLastError());
   }
-  HIPCHK(s->ctx, hipGetLastError());
   return W90GPU_OK;
 }
 
-int launch_domega(w90gpu_session* s) {
-  if (s->nw <= 16) {
-    hipLaunchKernelGGL(k_domega_k<16>, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p);
-  } else if (s->nw <= kLdsMaxWann) {
-    hipLaunchKernelGGL(k_domega_k<kLdsMaxWann>, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p);
-  } else {
-    hipLaunchKernelGGL(k_domega, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p);
-  }
-  HIPCHK(s->ctx, hipGetLastError());
+int launch_omega(Engine* E, const Lists& L, int mode) {
+  int rc = launch_rotate(E, L, mode);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_omega_group, dim3(L.np * W90_MAX_GROUPS), dim3(W90_NT), 0, E->stream, E->d_probs, L.plist,
+                     L.np, mode);
+  HIPCHK(E->ctx, hipGetLastError());
+  hipLaunchKernelGGL(k_omega_final, dim3(L.np), dim3(W90_FINAL_NT), 0, E->stream, E->d_probs, L.plist, L.np, mode);
+  HIPCHK(E->ctx, hipGetLastError());
   return W90GPU_OK;
 }
 
-int enqueue_init(w90gpu_session* s, int first_pass, double omega_invariant) {
-  w90gpu_ctx* ctx = s->ctx;
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s->stream, s->p, s->ctl, first_pass, omega_invariant);
-  HIPCHK(ctx, hipGetLastError());
-  const size_t mb = (size_t)s->nw * s->nw * s->nn * s->nk * sizeof(cplx);
-  const size_t ub = (size_t)s->nw * s->nw * s->nk * sizeof(cplx);
-  HIPCHK(ctx, hipMemcpyAsync(s->p.M[0], s->M0, mb, hipMemcpyDeviceToDevice, s->stream));
-  HIPCHK(ctx, hipMemcpyAsync(s->p.U[0], s->U0, ub, hipMemcpyDeviceToDevice, s->stream));
-  const size_t nd = (size_t)s->nw * s->nw * s->nk;
-  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(1024, (nd + 255) / 256)), dim3(256), 0,
-                     s->stream, s->p.D, nd);
-  HIPCHK(ctx, hipGetLastError());
-  return launch_omega(s, MODE_INIT);
+// A range of whole classes [c0, c1] of the item list (both precisions).
+Seg class_range(const Lists& L, int c0, int c1) {
+  Seg r;
+  r.off = L.seg[c0][0].off;
+  const Seg& last = L.seg[c1][1];
+  r.n = last.off + last.n - r.off;
+  return r;
 }
 
-int enqueue_iteration(w90gpu_session* s) {
-  w90gpu_ctx* ctx = s->ctx;
+int launch_step(Engine* E, const Lists& L, int mode) {
+  hipStream_t s = E->stream;
+  const Seg a = class_range(L, 0, 1), b = class_range(L, 2, 2), c = class_range(L, 3, 3);
+  if (a.n)
+    hipLaunchKernelGGL(k_step_k<16>, dim3(a.n), dim3(W90_NT), 0, s, E->d_probs, L.items + a.off, a.n, mode);
+  if (b.n)
+    hipLaunchKernelGGL(k_step_k<kLdsMaxWann>, dim3(b.n), dim3(W90_NT), 0, s, E->d_probs, L.items + b.off, b.n,
+                       mode);
+  if (c.n) hipLaunchKernelGGL(k_step, dim3(c.n), dim3(W90_NT), 0, s, E->d_probs, L.items + c.off, c.n, mode);
+  HIPCHK(E->ctx, hipGetLastError());
+  return W90GPU_OK;
+}
+
+int launch_domega(Engine* E, const Lists& L) {
+  hipStream_t s = E->stream;
+  const Prob* P = E->d_probs;
+  for (int c = 0; c < kClasses; ++c)
+    for (int q = 0; q < 2; ++q) {
+      const Seg& g = L.seg[c][q];
+      if (g.n == 0) continue;
+      const Item* it = L.items + g.off;
+      if (c <= 1) {
+        if (q == 0) hipLaunchKernelGGL((k_domega_k<16, cplx>), dim3(g.n), dim3(W90_NT), 0, s, P, it, g.n);
+        else hipLaunchKernelGGL((k_domega_k<16, float2>), dim3(g.n), dim3(W90_NT), 0, s, P, it, g.n);
+      } else if (c == 2) {
+        if (q == 0) hipLaunchKernelGGL((k_domega_k<kLdsMaxWann, cplx>), dim3(g.n), dim3(W90_NT), 0, s, P, it, g.n);
+        else hipLaunchKernelGGL((k_domega_k<kLdsMaxWann, float2>), dim3(g.n), dim3(W90_NT), 0, s, P, it, g.n);
+      } else {
+        if (q == 0) hipLaunchKernelGGL(k_domega<cplx>, dim3(g.n), dim3(W90_NT), 0, s, P, it, g.n);
+        else hipLaunchKernelGGL(k_domega<float2>, dim3(g.n), dim3(W90_NT), 0, s, P, it, g.n);
+      }
+      HIPCHK(E->ctx, hipGetLastError());
+    }
+  return W90GPU_OK;
+}
+
+int all_items(const Lists& L) {
+  const Seg r = class_range(L, 0, kClasses - 1);
+  return r.n;
+}
+
+// One iteration of wann_main over the lists (wannierise.F90:555-817).
+int enqueue_iteration(Engine* E, const Lists& L) {
   int rc;
-  if ((rc = launch_domega(s))) return rc;
-  hipLaunchKernelGGL(k_search, dim3(1), dim3(W90_NT), 0, s->stream, s->p);
-  HIPCHK(ctx, hipGetLastError());
-  if ((rc = launch_step(s, MODE_TRIAL))) return rc;
-  if ((rc = launch_omega(s, MODE_TRIAL))) return rc;
-  if (!s->ctl.lfixstep) {
-    if ((rc = launch_step(s, MODE_SECOND))) return rc;
-    if ((rc = launch_omega(s, MODE_SECOND))) return rc;
+  if ((rc = launch_domega(E, L))) return rc;
+  hipLaunchKernelGGL(k_search, dim3(L.np), dim3(W90_NT), 0, E->stream, E->d_probs, L.plist, L.np);
+  HIPCHK(E->ctx, hipGetLastError());
+  if ((rc = launch_step(E, L, MODE_TRIAL))) return rc;
+  if ((rc = launch_omega(E, L, MODE_TRIAL))) return rc;
+  if (L.any_second) {
+    if ((rc = launch_step(E, L, MODE_SECOND))) return rc;
+    if ((rc = launch_omega(E, L, MODE_SECOND))) return rc;
   }
   return W90GPU_OK;
 }
 
-// Capture kGraphBlock iterations of the loop into an executable graph.  The
-// kernels read the problem from their by-value argument and the loop state
-// from device memory, so one graph serves every block and every solve of
-// the session.  On a capture failure the session keeps launching directly.
-int capture_block(w90gpu_session* s) {
+// Start the queued members: state from the template, M and U from M0/U0,
+// the initial wann_omega in FP64, and the FP32 copy of M for the mixed
+// bulk phase.
+int enqueue_start(Engine* E, const std::vector<Member*>& ms) {
+  Lists L;
+  int rc = aux_lists(E, ms, L);
+  if (rc) return rc;
+  hipStream_t s = E->stream;
+  hipLaunchKernelGGL(k_reset, dim3((L.np + 255) / 256), dim3(256), 0, s, E->d_probs, L.plist, L.np);
+  HIPCHK(E->ctx, hipGetLastError());
+  const int n = all_items(L);
+  hipLaunchKernelGGL(k_load_initial, dim3(n), dim3(W90_NT), 0, s, E->d_probs, L.items, n);
+  HIPCHK(E->ctx, hipGetLastError());
+  if ((rc = launch_omega(E, L, MODE_INIT))) return rc;
+  bool any_mixed = false;
+  for (Member* m : ms) any_mixed |= (m->ctl.precision == W90GPU_PREC_MIXED);
+  if (any_mixed) {
+    hipLaunchKernelGGL(k_to_f32, dim3(n), dim3(W90_NT), 0, s, E->d_probs, L.items, n);
+    HIPCHK(E->ctx, hipGetLastError());
+  }
+  const double t = now_s();
+  for (Member* m : ms) {
+    m->stage = 2;
+    m->phase = (m->ctl.precision == W90GPU_PREC_MIXED) ? 0 : 1;
+    m->t_start = t;
+  }
+  E->epoch++;
+  return W90GPU_OK;
+}
+
+// FP64 rebuild of M for members whose mixed bulk phase has ended:
+// E(k) = U0(k)^+ U(k), M(k,b) = E(k)^+ M0(k,b) E(k+b), and the wann_omega
+// of it.
+int enqueue_remix(Engine* E, const std::vector<Member*>& ms) {
+  Lists L;
+  int rc = aux_lists(E, ms, L);
+  if (rc) return rc;
+  const int n = all_items(L);
+  hipLaunchKernelGGL(k_delta_u, dim3(n), dim3(W90_NT), 0, E->stream, E->d_probs, L.items, n);
+  HIPCHK(E->ctx, hipGetLastError());
+  if ((rc = launch_omega(E, L, MODE_REMIX))) return rc;
+  for (Member* m : ms) m->phase = 1;
+  E->epoch++;
+  return W90GPU_OK;
+}
+
+// Retire finished members: unitarity check of U and the final state.
+int retire(Engine* E, const std::vector<Member*>& ms) {
+  Lists L;
+  int rc = aux_lists(E, ms, L);
+  if (rc) return rc;
+  const int n = all_items(L);
+  hipLaunchKernelGGL(k_unitarity, dim3(n), dim3(W90_NT), 0, E->stream, E->d_probs, L.items, n);
+  HIPCHK(E->ctx, hipGetLastError());
+  hipLaunchKernelGGL(k_gather_final, dim3(L.np), dim3(W90_NT), 0, E->stream, E->d_probs, L.plist, L.np,
+                     E->d_final);
+  HIPCHK(E->ctx, hipGetLastError());
+  HIPCHK(E->ctx, hipMemcpyAsync(E->h_final, E->d_final, sizeof(Final) * L.np, hipMemcpyDeviceToHost, E->stream));
+  HIPCHK(E->ctx, hipStreamSynchronize(E->stream));
+  const double t = now_s();
+  for (size_t i = 0; i < ms.size(); ++i) {
+    Member* m = ms[i];
+    const State& st = E->h_final[i].st;
+    w90gpu_result r{};
+    r.omega_total = st.wann.om_tot;
+    r.omega_invariant = st.wann.om_i;
+    r.omega_d = st.wann.om_d;
+    r.omega_od = st.wann.om_od;
+    r.omega_tilde = st.wann.om_d + st.wann.om_od;
+    r.gcnorm1 = st.gcnorm1;
+    r.delta_omega = st.wann.om_tot - st.old.om_tot;
+    r.iterations = st.iter;
+    r.converged = st.converged;
+    r.bulk_iterations = (m->ctl.precision == W90GPU_PREC_MIXED) ? st.bulk_iter : 0;
+    const double mx = E->h_final[i].unit_max;
+    r.unitarity_ok = (mx <= 1e-5) ? 1 : 0;
+    r.status = r.unitarity_ok ? W90GPU_OK : W90GPU_ERR_FATAL;
+    if (!r.unitarity_ok)
+      set_error(E->ctx, "wann_check_unitarity: final U deviates from unitarity by %.3e", mx);
+    m->t_end = t;
+    r.solve_seconds = m->t_end - m->t_start;
+    m->res = r;
+    m->stage = 3;
+    E->finished_ids.push_back(m->id);
+  }
+  E->epoch++;
+  return W90GPU_OK;
+}
+
+int build_run_lists(Engine* E) {
+  std::vector<Member*> ms;
+  int nitems = 0;
+  for (Member* m : E->slots)
+    if (m && m->stage == 2) {
+      ms.push_back(m);
+      nitems += m->nk;
+    }
+  int rc = grow_tables(E, (int)E->slots.size(), nitems);
+  if (rc) return rc;
+  static thread_local std::vector<Item> items;
+  static thread_local std::vector<int> plist;
+  build_lists(ms, [](const Member* m) { return m->phase == 0 ? 1 : 0; }, items, plist, E->run);
+  rc = upload_lists(E, items, plist, E->d_items, E->d_plist, E->run);
+  if (rc) return rc;
+  HIPCHK(E->ctx, hipStreamSynchronize(E->stream));
+  E->run_slots = plist;
+  drop_graph(E);
+  E->lists_epoch = E->epoch;
+  return W90GPU_OK;
+}
+
+// Capture kGraphBlock iterations over the run lists.  The kernels read the
+// loop state from device memory, so one graph serves every block of the
+// epoch.  On a capture failure the engine keeps launching directly.
+int capture_block(Engine* E) {
   hipGraph_t g = nullptr;
-  if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+  if (hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
     (void)hipGetLastError();
-    s->graph_failed = true;
+    E->graph_failed = true;
     return W90GPU_OK;
   }
   int rc = W90GPU_OK;
-  for (int i = 0; i < kGraphBlock && rc == W90GPU_OK; ++i) rc = enqueue_iteration(s);
-  const hipError_t e = hipStreamEndCapture(s->stream, &g);
+  for (int i = 0; i < kGraphBlock && rc == W90GPU_OK; ++i) rc = enqueue_iteration(E, E->run);
+  const hipError_t e = hipStreamEndCapture(E->stream, &g);
   if (rc != W90GPU_OK || e != hipSuccess || !g) {
     (void)hipGetLastError();
     if (g) (void)hipGraphDestroy(g);
-    s->graph_failed = true;
+    E->graph_failed = true;
     return W90GPU_OK;
   }
-  if (hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+  if (hipGraphInstantiate(&E->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
     (void)hipGetLastError();
     (void)hipGraphDestroy(g);
-    s->gexec = nullptr;
-    s->graph_failed = true;
+    E->gexec = nullptr;
+    E->graph_failed = true;
     return W90GPU_OK;
   }
-  s->graph = g;
+  E->graph = g;
   return W90GPU_OK;
 }
 
-int read_state(w90gpu_session* s) {
-  HIPCHK(s->ctx, hipMemcpyAsync(s->st_host, s->p.st, sizeof(State), hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(s->ctx, hipStreamSynchronize(s->stream));
-  return W90GPU_OK;
-}
-
-int finish_result(w90gpu_session* s, w90gpu_result* r, float elapsed_ms) {
-  int rc = read_state(s);
-  if (rc) return rc;
-  const State* st = s->st_host;
-  w90gpu_result res{};
-  res.omega_total = st->wann.om_tot;
-  res.omega_invariant = st->wann.om_i;
-  res.omega_d = st->wann.om_d;
-  res.omega_od = st->wann.om_od;
-  res.omega_tilde = st->wann.om_d + st->wann.om_od;
-  res.gcnorm1 = st->gcnorm1;
-  res.delta_omega = st->wann.om_tot - st->old.om_tot;
-  res.iterations = st->iter;
-  res.converged = st->converged;
-  res.solve_seconds = elapsed_ms * 1e-3;
-  // unitarity check of the final U (wann_check_unitarity, eps5)
-  hipLaunchKernelGGL(k_unitarity, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p);
-  HIPCHK(s->ctx, hipGetLastError());
-  std::vector<double> dev(s->nk);
-  HIPCHK(s->ctx, hipMemcpyAsync(dev.data(), s->p.unit, sizeof(double) * s->nk, hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(s->ctx, hipStreamSynchronize(s->stream));
-  double mx = 0.0;
-  for (double d : dev) mx = std::max(mx, d);
-  res.unitarity_ok = (mx <= 1e-5) ? 1 : 0;
-  res.status = W90GPU_OK;
-  if (!res.unitarity_ok) {
-    set_error(s->ctx, "wann_check_unitarity: final U deviates from unitarity by %.3e", mx);
-    res.status = W90GPU_ERR_FATAL;
+int enqueue_block(Engine* E, int slot) {
+  int rc;
+  // the first block of an epoch is launched directly; the graph is
+  // captured when the membership has stayed the same for a block
+  const bool graph = E->use_graph && !E->prof && !E->graph_failed && E->epoch_blocks >= 1;
+  if (graph && !E->gexec) {
+    if ((rc = capture_block(E))) return rc;
   }
-  if (s->prof) {
-    for (size_t i = 0; i < s->ev_used; ++i) {
-      float ms = 0.f;
-      HIPCHK(s->ctx, hipEventElapsedTime(&ms, s->evs[i].first, s->evs[i].second));
-      s->prof_ms += ms;
-      s->prof_launches += 1;
+  if (graph && E->gexec) {
+    HIPCHK(E->ctx, hipGraphLaunch(E->gexec, E->stream));
+  } else {
+    for (int i = 0; i < kGraphBlock; ++i)
+      if ((rc = enqueue_iteration(E, E->run))) return rc;
+  }
+  E->epoch_blocks++;
+  const int np = E->run.np;
+  hipLaunchKernelGGL(k_status, dim3((np + 255) / 256), dim3(256), 0, E->stream, E->d_probs, E->run.plist, np,
+                     E->d_status + (size_t)slot * E->cap_plist);
+  HIPCHK(E->ctx, hipGetLastError());
+  HIPCHK(E->ctx, hipMemcpyAsync(E->h_status + (size_t)slot * E->cap_plist, E->d_status + (size_t)slot * E->cap_plist,
+                                sizeof(Status) * np, hipMemcpyDeviceToHost, E->stream));
+  HIPCHK(E->ctx, hipEventRecord(E->blk_ev[slot], E->stream));
+  E->blk_slots[slot] = E->run_slots;
+  return W90GPU_OK;
+}
+
+// Read the status of a completed block: finished members are retired,
+// members whose bulk phase ended get their FP64 rebuild queued.
+int process_block(Engine* E, int slot, std::vector<Member*>& remix) {
+  HIPCHK(E->ctx, hipEventSynchronize(E->blk_ev[slot]));
+  const Status* st = E->h_status + (size_t)slot * E->cap_plist;
+  std::vector<Member*> done;
+  const std::vector<int>& sl = E->blk_slots[slot];
+  for (size_t i = 0; i < sl.size(); ++i) {
+    Member* m = E->slots[sl[i]];
+    if (!m || m->stage != 2) continue;
+    if (st[i].done) {
+      done.push_back(m);
+    } else if (st[i].hold && m->phase == 0 &&
+               std::find(remix.begin(), remix.end(), m) == remix.end()) {
+      remix.push_back(m);
     }
-    s->ev_used = 0;
   }
-  s->last = res;
-  if (r) *r = res;
-  return res.status;
+  if (!done.empty()) return retire(E, done);
+  return W90GPU_OK;
 }
+
+int running_count(const Engine* E) {
+  int n = 0;
+  for (const Member* m : E->slots)
+    if (m && m->stage == 2) ++n;
+  return n;
+}
+
+// Start queued members, then run blocks over the running ones until
+// max_blocks blocks were issued (max_blocks < 0: until none is running).
+// Returns with the stream drained.
+int engine_advance(Engine* E, int max_blocks) {
+  int rc;
+  int issued = 0;
+  int inflight = 0, head = 0;  // blocks in flight: slots head, head^1
+  std::vector<Member*> remix;
+  while (true) {
+    std::vector<Member*> queued;
+    for (Member* m : E->slots)
+      if (m && m->stage == 1) queued.push_back(m);
+    if (!queued.empty() && (rc = enqueue_start(E, queued))) return rc;
+    if (!remix.empty()) {
+      if ((rc = enqueue_remix(E, remix))) return rc;
+      remix.clear();
+    }
+    if (running_count(E) == 0) break;
+    if (max_blocks >= 0 && issued >= max_blocks) break;
+    if (E->lists_epoch != E->epoch) {
+      // a new epoch: let the blocks in flight finish with the old lists
+      while (inflight > 0) {
+        if ((rc = process_block(E, head, remix))) return rc;
+        head ^= 1;
+        --inflight;
+      }
+      if (!remix.empty()) continue;
+      if (running_count(E) == 0) break;
+      if ((rc = build_run_lists(E))) return rc;
+    }
+    const int slot = (head + inflight) & 1;
+    if ((rc = enqueue_block(E, slot))) return rc;
+    ++inflight;
+    ++issued;
+    if (inflight == 2) {
+      if ((rc = process_block(E, head, remix))) return rc;
+      head ^= 1;
+      --inflight;
+    }
+  }
+  while (inflight > 0) {
+    if ((rc = process_block(E, head, remix))) return rc;
+    head ^= 1;
+    --inflight;
+  }
+  if (!remix.empty() && (rc = enqueue_remix(E, remix))) return rc;
+  HIPCHK(E->ctx, hipStreamSynchronize(E->stream));
+  reap_graphs(E);
+  return W90GPU_OK;
+}
+
+int read_member_state(Engine* E, Member* m, State* out) {
+  HIPCHK(E->ctx, hipMemcpyAsync(out, m->p.st, sizeof(State), hipMemcpyDeviceToHost, E->stream));
+  HIPCHK(E->ctx, hipStreamSynchronize(E->stream));
+  return W90GPU_OK;
+}
+
+int member_fetch(Engine* E, Member* m, double* m_matrix, double* u_matrix, double* centres, double* spreads) {
+  State st;
+  int rc = read_member_state(E, m, &st);
+  if (rc) return rc;
+  const int cur = st.cur;
+  const size_t mb = (size_t)m->nw * m->nw * m->nn * m->nk * sizeof(cplx);
+  const size_t ub = (size_t)m->nw * m->nw * m->nk * sizeof(cplx);
+  if (m_matrix) {
+    if (st.phase == 0 && st.iter > 0) {
+      set_error(E->ctx, "M of a mixed solve is available once its FP64 phase has started");
+      return W90GPU_ERR_INPUT;
+    }
+    HIPCHK(E->ctx, hipMemcpyAsync(m_matrix, m->p.M[cur], mb, hipMemcpyDeviceToHost, E->stream));
+  }
+  if (u_matrix) HIPCHK(E->ctx, hipMemcpyAsync(u_matrix, m->p.U[cur], ub, hipMemcpyDeviceToHost, E->stream));
+  std::vector<double> rave(3 * m->nw), r2(m->nw);
+  HIPCHK(E->ctx, hipMemcpyAsync(rave.data(), m->p.rave[cur], 3 * m->nw * sizeof(double), hipMemcpyDeviceToHost,
+                                E->stream));
+  HIPCHK(E->ctx, hipMemcpyAsync(r2.data(), m->p.r2ave[cur], m->nw * sizeof(double), hipMemcpyDeviceToHost,
+                                E->stream));
+  HIPCHK(E->ctx, hipStreamSynchronize(E->stream));
+  if (centres) std::memcpy(centres, rave.data(), 3 * m->nw * sizeof(double));
+  if (spreads)
+    for (int n = 0; n < m->nw; ++n) {
+      const double rx = rave[3 * n], ry = rave[3 * n + 1], rz = rave[3 * n + 2];
+      spreads[n] = r2[n] - (rx * rx + ry * ry + rz * rz);
+    }
+  return W90GPU_OK;
+}
+
+// A one-member list of a member for the per-phase entry points.
+int single_lists(Engine* E, Member* m, Lists& L) { return aux_lists(E, std::vector<Member*>{m}, L); }
 
 }  // namespace
 
@@ -1681,7 +1099,7 @@ int finish_result(w90gpu_session* s, w90gpu_result* r, float elapsed_ms) {
 
 extern "C" {
 
-const char* w90gpu_version(void) { return "w90gpu 0.1 (gfx950)"; }
+const char* w90gpu_version(void) { return "w90gpu 0.2 (gfx950)"; }
 
 void w90gpu_control_defaults(w90gpu_control* c) {
   if (!c) return;
@@ -1710,14 +1128,29 @@ int w90gpu_create(int device, w90gpu_ctx** ctx) {
     set_error(nullptr, "device %d out of range (%d devices)", device, ndev);
     return W90GPU_ERR_INPUT;
   }
-  w90gpu_ctx* c = new (std::nothrow) w90gpu_ctx();
-  if (!c) return W90GPU_ERR_ALLOC;
-  c->device = device;
   if (hipSetDevice(device) != hipSuccess) {
-    delete c;
     set_error(nullptr, "hipSetDevice(%d) failed", device);
     return W90GPU_ERR_FATAL;
   }
+  // dynamic LDS above 64 KiB must be enabled per kernel
+  const struct {
+    const void* f;
+    size_t bytes;
+  } lds[] = {
+      {(const void*)k_rotate_kb<8, cplx>, rotate_kb_lds<8, cplx>(W90GPU_NUM_NNMAX)},
+      {(const void*)k_rotate_kb<16, cplx>, rotate_kb_lds<16, cplx>(W90GPU_NUM_NNMAX)},
+      {(const void*)k_rotate_kb<8, float2>, rotate_kb_lds<8, float2>(W90GPU_NUM_NNMAX)},
+      {(const void*)k_rotate_kb<16, float2>, rotate_kb_lds<16, float2>(W90GPU_NUM_NNMAX)},
+  };
+  for (const auto& l : lds) {
+    if (hipFuncSetAttribute(l.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.bytes) != hipSuccess) {
+      set_error(nullptr, "hipFuncSetAttribute(MaxDynamicSharedMemorySize=%zu) failed", l.bytes);
+      return W90GPU_ERR_FATAL;
+    }
+  }
+  w90gpu_ctx* c = new (std::nothrow) w90gpu_ctx();
+  if (!c) return W90GPU_ERR_ALLOC;
+  c->device = device;
   *ctx = c;
   return W90GPU_OK;
 }
@@ -1727,14 +1160,19 @@ int w90gpu_destroy(w90gpu_ctx* ctx) {
   return W90GPU_OK;
 }
 
-const char* w90gpu_last_error(const w90gpu_ctx* ctx) {
-  return ctx ? ctx->err.c_str() : g_noctx_err.c_str();
+const char* w90gpu_last_error(const w90gpu_ctx* ctx) { return ctx ? ctx->err.c_str() : g_noctx_err.c_str(); }
+
+size_t w90gpu_problem_bytes(int num_wann, int num_kpts, int nntot, int num_iter) {
+  if (num_wann < 1 || num_kpts < 1 || nntot < 1 || num_iter < 0) return 0;
+  Layout lay;
+  return problem_layout(num_wann, num_kpts, nntot, num_iter, lay);
 }
 
 size_t w90gpu_session_bytes(int num_wann, int num_kpts, int nntot) {
-  Layout lay;
-  return session_layout(num_wann, num_kpts, nntot, 0, lay);
+  return w90gpu_problem_bytes(num_wann, num_kpts, nntot, 0);
 }
+
+// ------------------------------------------------------------ sessions
 
 int w90gpu_session_create(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90gpu_control* ctl,
                           w90gpu_session** out) {
@@ -1742,144 +1180,37 @@ int w90gpu_session_create(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90
   *out = nullptr;
   int rc = validate(ctx, prob, ctl);
   if (rc) return rc;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  // dynamic LDS above 64 KiB must be enabled per kernel
-  HIPCHK(ctx, hipFuncSetAttribute((const void*)k_rotate_kb<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)rotate_kb_lds<16>(W90GPU_NUM_NNMAX)));
-  HIPCHK(ctx, hipFuncSetAttribute((const void*)k_rotate_kb<8>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)rotate_kb_lds<8>(W90GPU_NUM_NNMAX)));
-  w90gpu_session* s = new (std::nothrow) w90gpu_session();
-  if (!s) return W90GPU_ERR_ALLOC;
-  s->ctx = ctx;
-  s->ctl = *ctl;
-  s->nw = prob->num_wann;
-  s->nk = prob->num_kpts;
-  s->nn = prob->nntot;
-  s->wbtot = prob->wbtot;
   Layout lay;
-  const size_t bytes = session_layout(s->nw, s->nk, s->nn, std::max(ctl->num_iter, 0), lay);
-  hipError_t e = hipMalloc(&s->arena, bytes);
-  if (e != hipSuccess) {
-    delete s;
-    set_error(ctx, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+  const size_t bytes = problem_layout(prob->num_wann, prob->num_kpts, prob->nntot, ctl->num_iter, lay);
+  Engine* E = nullptr;
+  if ((rc = engine_create(ctx, bytes, &E))) return rc;
+  Member* m = nullptr;
+  if ((rc = engine_add(E, prob, ctl, &m))) {
+    engine_destroy(E);
+    return rc;
+  }
+  w90gpu_session* s = new (std::nothrow) w90gpu_session();
+  if (!s) {
+    engine_destroy(E);
     return W90GPU_ERR_ALLOC;
   }
-  s->arena_bytes = bytes;
-  char* a = (char*)s->arena;
-  int i = 0;
-  Prob& p = s->p;
-  p.nw = s->nw;
-  p.nk = s->nk;
-  p.nn = s->nn;
-  p.nw2 = s->nw * s->nw;
-  p.npan = npanels(s->nw);
-  p.psz = 8 * s->nw + 8;
-  p.lds = s->nw <= kLdsMaxWann ? 1 : 0;
-  p.M[0] = (cplx*)(a + lay.off[i++]);
-  p.M[1] = (cplx*)(a + lay.off[i++]);
-  s->M0 = (cplx*)(a + lay.off[i++]);
-  p.U[0] = (cplx*)(a + lay.off[i++]);
-  p.U[1] = (cplx*)(a + lay.off[i++]);
-  s->U0 = (cplx*)(a + lay.off[i++]);
-  p.G = (cplx*)(a + lay.off[i++]);
-  p.D = (cplx*)(a + lay.off[i++]);
-  p.E = (cplx*)(a + lay.off[i++]);
-  p.W1 = (cplx*)(a + lay.off[i++]);
-  p.W2 = (cplx*)(a + lay.off[i++]);
-  p.ln[0] = (double*)(a + lay.off[i++]);
-  p.ln[1] = (double*)(a + lay.off[i++]);
-  p.d2 = (double*)(a + lay.off[i++]);
-  p.part_od = (double*)(a + lay.off[i++]);
-  p.part_all = (double*)(a + lay.off[i++]);
-  p.kpart = (double*)(a + lay.off[i++]);
-  p.gpart = (double*)(a + lay.off[i++]);
-  p.ngroup = std::min(s->nk, W90_MAX_GROUPS);
-  p.part_gg = (double*)(a + lay.off[i++]);
-  p.part_gd = (double*)(a + lay.off[i++]);
-  p.rave[0] = (double*)(a + lay.off[i++]);
-  p.rave[1] = (double*)(a + lay.off[i++]);
-  p.r2ave[0] = (double*)(a + lay.off[i++]);
-  p.r2ave[1] = (double*)(a + lay.off[i++]);
-  p.unit = (double*)(a + lay.off[i++]);
-  p.log = (double*)(a + lay.off[i++]);
-  p.st = (State*)(a + lay.off[i++]);
-  int* nnl = (int*)(a + lay.off[i++]);
-  double* bk = (double*)(a + lay.off[i++]);
-  double* wb = (double*)(a + lay.off[i++]);
-  p.nnl = nnl;
-  p.bk = bk;
-  p.wb = wb;
-
-  auto fail = [&](int code) {
-    w90gpu_session_destroy(s);
-    return code;
-  };
-  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
-    set_error(ctx, "hipStreamCreate failed");
-    return fail(W90GPU_ERR_FATAL);
-  }
-  if (hipHostMalloc((void**)&s->st_host, sizeof(State), hipHostMallocDefault) != hipSuccess) {
-    s->st_host = nullptr;
-    set_error(ctx, "hipHostMalloc failed");
-    return fail(W90GPU_ERR_ALLOC);
-  }
-  if (hipEventCreate(&s->t0) != hipSuccess || hipEventCreate(&s->t1) != hipSuccess ||
-      hipEventCreateWithFlags(&s->flag_ev[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&s->flag_ev[1], hipEventDisableTiming) != hipSuccess) {
-    set_error(ctx, "hipEventCreate failed");
-    return fail(W90GPU_ERR_FATAL);
-  }
-  if (hipHostMalloc((void**)&s->flags_host, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
-    s->flags_host = nullptr;
-    set_error(ctx, "hipHostMalloc failed");
-    return fail(W90GPU_ERR_ALLOC);
-  }
-  // k-mesh tables: nnlist (nk,nn) Fortran -> [k][b] 0-based
-  std::vector<int> hn((size_t)s->nk * s->nn);
-  for (int k = 0; k < s->nk; ++k)
-    for (int b = 0; b < s->nn; ++b) hn[(size_t)k * s->nn + b] = prob->nnlist[k + (size_t)s->nk * b] - 1;
-  std::vector<double> hst(sizeof(State) / sizeof(double) + 1, 0.0);
-  State* st0 = (State*)hst.data();
-  st0->wbtot = s->wbtot;
-  if (hipMemcpy(nnl, hn.data(), hn.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(bk, prob->bk, (size_t)s->nk * s->nn * 3 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(wb, prob->wb, (size_t)s->nn * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(p.st, st0, sizeof(State), hipMemcpyHostToDevice) != hipSuccess) {
-    set_error(ctx, "upload of k-mesh tables failed");
-    return fail(W90GPU_ERR_FATAL);
-  }
+  s->eng = E;
+  s->m = m;
   *out = s;
   return W90GPU_OK;
 }
 
 int w90gpu_session_destroy(w90gpu_session* s) {
   if (!s) return W90GPU_OK;
-  if (s->stream) (void)hipStreamSynchronize(s->stream);
-  for (auto& e : s->evs) {
-    (void)hipEventDestroy(e.first);
-    (void)hipEventDestroy(e.second);
-  }
-  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
-  if (s->graph) (void)hipGraphDestroy(s->graph);
-  for (hipEvent_t e : s->flag_ev)
-    if (e) (void)hipEventDestroy(e);
-  if (s->t0) (void)hipEventDestroy(s->t0);
-  if (s->t1) (void)hipEventDestroy(s->t1);
-  if (s->flags_host) (void)hipHostFree(s->flags_host);
-  if (s->st_host) (void)hipHostFree(s->st_host);
-  if (s->stream) (void)hipStreamDestroy(s->stream);
-  if (s->arena) (void)hipFree(s->arena);
+  engine_destroy(s->eng);
   delete s;
   return W90GPU_OK;
 }
 
 int w90gpu_session_load(w90gpu_session* s, const double* m_matrix, const double* u_matrix) {
   if (!s || !m_matrix || !u_matrix) return W90GPU_ERR_INPUT;
-  const size_t mb = (size_t)s->nw * s->nw * s->nn * s->nk * sizeof(cplx);
-  const size_t ub = (size_t)s->nw * s->nw * s->nk * sizeof(cplx);
-  HIPCHK(s->ctx, hipMemcpyAsync(s->M0, m_matrix, mb, hipMemcpyHostToDevice, s->stream));
-  HIPCHK(s->ctx, hipMemcpyAsync(s->U0, u_matrix, ub, hipMemcpyHostToDevice, s->stream));
-  HIPCHK(s->ctx, hipStreamSynchronize(s->stream));
+  int rc = member_upload(s->eng, s->m, m_matrix, u_matrix);
+  if (rc) return rc;
   s->loaded = true;
   return W90GPU_OK;
 }
@@ -1887,59 +1218,44 @@ int w90gpu_session_load(w90gpu_session* s, const double* m_matrix, const double*
 int w90gpu_session_launch(w90gpu_session* s) {
   if (!s) return W90GPU_ERR_INPUT;
   if (!s->loaded) {
-    set_error(s->ctx, "session has no loaded M/U");
+    set_error(s->eng->ctx, "session has no loaded M/U");
     return W90GPU_ERR_INPUT;
   }
-  HIPCHK(s->ctx, hipEventRecord(s->t0, s->stream));
-  int rc = enqueue_init(s, 1, 0.0);
-  if (rc) return rc;
-  return W90GPU_OK;
+  Engine* E = s->eng;
+  HIPCHK(E->ctx, hipEventRecord(E->t0, E->stream));
+  s->m->stage = 1;
+  std::vector<Member*> q{s->m};
+  return enqueue_start(E, q);
 }
 
 int w90gpu_session_wait(w90gpu_session* s, w90gpu_result* r) {
   if (!s) return W90GPU_ERR_INPUT;
-  // Iterate in blocks of kGraphBlock iterations (a replayed graph when
-  // possible).  Two blocks are kept in flight: after enqueuing block j the
-  // host waits for the done flag copied at the end of block j-1, so the
-  // device never idles on the poll.  Kernels of a finished solve exit at
-  // their first instruction, so at most one block of them runs after the
-  // loop has finished.
-  int issued = 0, blocks = 0;
-  int rc;
-  const bool use_graph = !s->prof && !s->graph_failed && std::getenv("W90GPU_NO_GRAPH") == nullptr;
-  while (issued < s->ctl.num_iter) {
-    const int nb = std::min(kGraphBlock, s->ctl.num_iter - issued);
-    if (use_graph && nb == kGraphBlock) {
-      if (!s->gexec) {
-        if ((rc = capture_block(s))) return rc;
-      }
-      if (s->gexec) {
-        HIPCHK(s->ctx, hipGraphLaunch(s->gexec, s->stream));
-      } else {
-        for (int i = 0; i < nb; ++i)
-          if ((rc = enqueue_iteration(s))) return rc;
-      }
-    } else {
-      for (int i = 0; i < nb; ++i)
-        if ((rc = enqueue_iteration(s))) return rc;
-    }
-    issued += nb;
-    const int slot = blocks & 1;
-    HIPCHK(s->ctx, hipMemcpyAsync(&s->flags_host[slot], &s->p.st->done, sizeof(int), hipMemcpyDeviceToHost,
-                                  s->stream));
-    HIPCHK(s->ctx, hipEventRecord(s->flag_ev[slot], s->stream));
-    ++blocks;
-    if (blocks >= 2) {
-      const int prev = (blocks - 2) & 1;
-      HIPCHK(s->ctx, hipEventSynchronize(s->flag_ev[prev]));
-      if (s->flags_host[prev]) break;
-    }
-  }
-  HIPCHK(s->ctx, hipEventRecord(s->t1, s->stream));
-  HIPCHK(s->ctx, hipEventSynchronize(s->t1));
+  Engine* E = s->eng;
+  int rc = engine_advance(E, -1);
+  if (rc) return rc;
+  HIPCHK(E->ctx, hipEventRecord(E->t1, E->stream));
+  HIPCHK(E->ctx, hipEventSynchronize(E->t1));
   float ms = 0.f;
-  HIPCHK(s->ctx, hipEventElapsedTime(&ms, s->t0, s->t1));
-  return finish_result(s, r, ms);
+  HIPCHK(E->ctx, hipEventElapsedTime(&ms, E->t0, E->t1));
+  if (E->prof) {
+    for (size_t i = 0; i < E->ev_used; ++i) {
+      float t = 0.f;
+      HIPCHK(E->ctx, hipEventElapsedTime(&t, E->evs[i].first, E->evs[i].second));
+      E->prof_ms += t;
+      E->prof_launches += 1;
+    }
+    E->ev_used = 0;
+  }
+  E->finished_ids.clear();
+  if (s->m->stage != 3) {
+    set_error(E->ctx, "solve did not finish");
+    return W90GPU_ERR_FATAL;
+  }
+  w90gpu_result res = s->m->res;
+  res.solve_seconds = ms * 1e-3;
+  s->last = res;
+  if (r) *r = res;
+  return res.status;
 }
 
 int w90gpu_session_solve(w90gpu_session* s, w90gpu_result* r) {
@@ -1948,36 +1264,19 @@ int w90gpu_session_solve(w90gpu_session* s, w90gpu_result* r) {
   return w90gpu_session_wait(s, r);
 }
 
-int w90gpu_session_fetch(w90gpu_session* s, double* m_matrix, double* u_matrix, double* centres,
-                         double* spreads) {
+int w90gpu_session_fetch(w90gpu_session* s, double* m_matrix, double* u_matrix, double* centres, double* spreads) {
   if (!s) return W90GPU_ERR_INPUT;
-  int rc = read_state(s);
-  if (rc) return rc;
-  const int cur = s->st_host->cur;
-  const size_t mb = (size_t)s->nw * s->nw * s->nn * s->nk * sizeof(cplx);
-  const size_t ub = (size_t)s->nw * s->nw * s->nk * sizeof(cplx);
-  if (m_matrix) HIPCHK(s->ctx, hipMemcpyAsync(m_matrix, s->p.M[cur], mb, hipMemcpyDeviceToHost, s->stream));
-  if (u_matrix) HIPCHK(s->ctx, hipMemcpyAsync(u_matrix, s->p.U[cur], ub, hipMemcpyDeviceToHost, s->stream));
-  std::vector<double> rave(3 * s->nw), r2(s->nw);
-  HIPCHK(s->ctx, hipMemcpyAsync(rave.data(), s->p.rave[cur], 3 * s->nw * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(s->ctx, hipMemcpyAsync(r2.data(), s->p.r2ave[cur], s->nw * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(s->ctx, hipStreamSynchronize(s->stream));
-  if (centres) std::memcpy(centres, rave.data(), 3 * s->nw * sizeof(double));
-  if (spreads)
-    for (int n = 0; n < s->nw; ++n) {
-      const double rx = rave[3 * n], ry = rave[3 * n + 1], rz = rave[3 * n + 2];
-      spreads[n] = r2[n] - (rx * rx + ry * ry + rz * rz);
-    }
-  return W90GPU_OK;
+  return member_fetch(s->eng, s->m, m_matrix, u_matrix, centres, spreads);
 }
 
 int w90gpu_session_history(w90gpu_session* s, double* out, int rows) {
   if (!s || !out || rows < 0) return -W90GPU_ERR_INPUT;
-  if (read_state(s)) return -W90GPU_ERR_FATAL;
-  const int n = std::min(rows, s->st_host->iter + 1);
+  State st;
+  if (read_member_state(s->eng, s->m, &st)) return -W90GPU_ERR_FATAL;
+  const int n = std::min(rows, st.iter + 1);
   if (n <= 0) return 0;
-  if (hipMemcpy(out, s->p.log, sizeof(double) * 4 * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess) {
-    set_error(s->ctx, "history download failed");
+  if (hipMemcpy(out, s->m->p.log, sizeof(double) * 4 * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error(s->eng->ctx, "history download failed");
     return -W90GPU_ERR_FATAL;
   }
   return n;
@@ -1985,23 +1284,24 @@ int w90gpu_session_history(w90gpu_session* s, double* out, int rows) {
 
 int w90gpu_session_profile(w90gpu_session* s, int enable) {
   if (!s) return W90GPU_ERR_INPUT;
-  s->prof = enable != 0;
-  s->prof_ms = 0.0;
-  s->prof_launches = 0;
-  s->ev_used = 0;
+  Engine* E = s->eng;
+  E->prof = enable != 0;
+  E->prof_ms = 0.0;
+  E->prof_launches = 0;
+  E->ev_used = 0;
   return W90GPU_OK;
 }
 
 int w90gpu_session_kernel_time(w90gpu_session* s, double* avg_ms, int* launches) {
   if (!s) return W90GPU_ERR_INPUT;
-  if (avg_ms) *avg_ms = s->prof_launches ? s->prof_ms / s->prof_launches : 0.0;
-  if (launches) *launches = s->prof_launches;
+  const Engine* E = s->eng;
+  if (avg_ms) *avg_ms = E->prof_launches ? E->prof_ms / E->prof_launches : 0.0;
+  if (launches) *launches = E->prof_launches;
   return W90GPU_OK;
 }
 
-int w90gpu_wann_main(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90gpu_control* ctl,
-                     double* m_matrix, double* u_matrix, double* centres, double* spreads,
-                     w90gpu_result* result) {
+int w90gpu_wann_main(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90gpu_control* ctl, double* m_matrix,
+                     double* u_matrix, double* centres, double* spreads, w90gpu_result* result) {
   if (!ctx) return W90GPU_ERR_INPUT;
   if (!m_matrix || !u_matrix) {
     set_error(ctx, "m_matrix and u_matrix must be associated");
@@ -2020,6 +1320,135 @@ int w90gpu_wann_main(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90gpu_c
   return rc;
 }
 
+// ------------------------------------------------------------- batches
+
+int w90gpu_batch_create(w90gpu_ctx* ctx, size_t pool_bytes, w90gpu_batch** out) {
+  if (!ctx || !out) return W90GPU_ERR_INPUT;
+  *out = nullptr;
+  Engine* E = nullptr;
+  int rc = engine_create(ctx, pool_bytes, &E);
+  if (rc) return rc;
+  w90gpu_batch* b = new (std::nothrow) w90gpu_batch();
+  if (!b) {
+    engine_destroy(E);
+    return W90GPU_ERR_ALLOC;
+  }
+  b->eng = E;
+  *out = b;
+  return W90GPU_OK;
+}
+
+int w90gpu_batch_destroy(w90gpu_batch* b) {
+  if (!b) return W90GPU_OK;
+  engine_destroy(b->eng);
+  delete b;
+  return W90GPU_OK;
+}
+
+int w90gpu_batch_add(w90gpu_batch* b, const w90gpu_problem* prob, const w90gpu_control* ctl,
+                     const double* m_matrix, const double* u_matrix, int* id) {
+  if (!b || !id || !m_matrix || !u_matrix) return W90GPU_ERR_INPUT;
+  *id = -1;
+  Member* m = nullptr;
+  int rc = engine_add(b->eng, prob, ctl, &m);
+  if (rc) return rc;
+  if ((rc = member_upload(b->eng, m, m_matrix, u_matrix))) {
+    engine_remove(b->eng, m);
+    return rc;
+  }
+  m->stage = 1;
+  *id = m->id;
+  return W90GPU_OK;
+}
+
+int w90gpu_batch_step(w90gpu_batch* b, int max_blocks, int* finished, int cap, int* nfinished) {
+  if (!b) return W90GPU_ERR_INPUT;
+  Engine* E = b->eng;
+  int rc = engine_advance(E, max_blocks < 0 ? 0 : max_blocks);
+  if (rc) return rc;
+  int n = 0;
+  for (int id : E->finished_ids) {
+    if (finished && n < cap) finished[n] = id;
+    ++n;
+  }
+  if (nfinished) *nfinished = std::min(n, cap);
+  if (n <= cap) E->finished_ids.clear();
+  else E->finished_ids.erase(E->finished_ids.begin(), E->finished_ids.begin() + cap);
+  return W90GPU_OK;
+}
+
+int w90gpu_batch_run(w90gpu_batch* b) {
+  if (!b) return W90GPU_ERR_INPUT;
+  int rc = engine_advance(b->eng, -1);
+  b->eng->finished_ids.clear();
+  return rc;
+}
+
+int w90gpu_batch_restart(w90gpu_batch* b) {
+  if (!b) return W90GPU_ERR_INPUT;
+  for (Member* m : b->eng->slots)
+    if (m && m->stage != 2) m->stage = 1;
+  return W90GPU_OK;
+}
+
+int w90gpu_batch_result(w90gpu_batch* b, int id, w90gpu_result* r) {
+  if (!b || !r) return W90GPU_ERR_INPUT;
+  auto it = b->eng->by_id.find(id);
+  if (it == b->eng->by_id.end()) {
+    set_error(b->eng->ctx, "no problem with id %d", id);
+    return W90GPU_ERR_INPUT;
+  }
+  if (it->second->stage != 3) {
+    set_error(b->eng->ctx, "problem %d has not finished", id);
+    return W90GPU_ERR_INPUT;
+  }
+  *r = it->second->res;
+  return r->status;
+}
+
+int w90gpu_batch_fetch(w90gpu_batch* b, int id, double* m_matrix, double* u_matrix, double* centres,
+                       double* spreads) {
+  if (!b) return W90GPU_ERR_INPUT;
+  auto it = b->eng->by_id.find(id);
+  if (it == b->eng->by_id.end()) {
+    set_error(b->eng->ctx, "no problem with id %d", id);
+    return W90GPU_ERR_INPUT;
+  }
+  return member_fetch(b->eng, it->second, m_matrix, u_matrix, centres, spreads);
+}
+
+int w90gpu_batch_remove(w90gpu_batch* b, int id) {
+  if (!b) return W90GPU_ERR_INPUT;
+  auto it = b->eng->by_id.find(id);
+  if (it == b->eng->by_id.end()) {
+    set_error(b->eng->ctx, "no problem with id %d", id);
+    return W90GPU_ERR_INPUT;
+  }
+  if (it->second->stage == 2) {
+    set_error(b->eng->ctx, "problem %d is running", id);
+    return W90GPU_ERR_INPUT;
+  }
+  HIPCHK(b->eng->ctx, hipStreamSynchronize(b->eng->copy));
+  engine_remove(b->eng, it->second);
+  return W90GPU_OK;
+}
+
+int w90gpu_batch_info(w90gpu_batch* b, size_t* pool_bytes, size_t* used_bytes, int* running, int* queued) {
+  if (!b) return W90GPU_ERR_INPUT;
+  const Engine* E = b->eng;
+  if (pool_bytes) *pool_bytes = E->pool.size;
+  if (used_bytes) *used_bytes = E->pool.used;
+  int nr = 0, nq = 0;
+  for (const Member* m : E->slots) {
+    if (!m) continue;
+    nr += (m->stage == 2);
+    nq += (m->stage == 1);
+  }
+  if (running) *running = nr;
+  if (queued) *queued = nq;
+  return W90GPU_OK;
+}
+
 // ---------------------------------------------------- per-phase entry points
 
 int w90gpu_omega(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* m_matrix, int first_pass,
@@ -2031,24 +1460,29 @@ int w90gpu_omega(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* m_ma
   w90gpu_session* s = nullptr;
   int rc = w90gpu_session_create(ctx, prob, &ctl, &s);
   if (rc) return rc;
-  const size_t mb = (size_t)s->nw * s->nw * s->nn * s->nk * sizeof(cplx);
-  const size_t nd = (size_t)s->nw * s->nw * s->nk;
+  Engine* E = s->eng;
+  Member* m = s->m;
+  const size_t mb = (size_t)m->nw * m->nw * m->nn * m->nk * sizeof(cplx);
+  const size_t nd = (size_t)m->nw * m->nw * m->nk;
   auto run = [&]() -> int {
-    HIPCHK(ctx, hipMemcpyAsync(s->p.M[0], m_matrix, mb, hipMemcpyHostToDevice, s->stream));
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s->stream, s->p, s->ctl, first_pass ? 1 : 0,
-                       omega_invariant);
-    hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(1024, (nd + 255) / 256)), dim3(256), 0,
-                       s->stream, s->p.U[0], nd);
-    int r = launch_omega(s, MODE_INIT);
+    int r = upload_state_template(E, m, first_pass ? 1 : 0, omega_invariant);
     if (r) return r;
-    if ((r = read_state(s))) return r;
-    if (rave) HIPCHK(ctx, hipMemcpy(rave, s->p.rave[0], 3 * s->nw * sizeof(double), hipMemcpyDeviceToHost));
-    if (r2ave) HIPCHK(ctx, hipMemcpy(r2ave, s->p.r2ave[0], s->nw * sizeof(double), hipMemcpyDeviceToHost));
+    Lists L;
+    if ((r = single_lists(E, m, L))) return r;
+    HIPCHK(ctx, hipMemcpyAsync(m->p.M[0], m_matrix, mb, hipMemcpyHostToDevice, E->stream));
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, E->stream, E->d_probs, L.plist, 1);
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(1024, (nd + 255) / 256)), dim3(256), 0, E->stream,
+                       m->p.U[0], nd);
+    if ((r = launch_omega(E, L, MODE_INIT))) return r;
+    State st;
+    if ((r = read_member_state(E, m, &st))) return r;
+    if (rave) HIPCHK(ctx, hipMemcpy(rave, m->p.rave[0], 3 * m->nw * sizeof(double), hipMemcpyDeviceToHost));
+    if (r2ave) HIPCHK(ctx, hipMemcpy(r2ave, m->p.r2ave[0], m->nw * sizeof(double), hipMemcpyDeviceToHost));
     if (omega) {
-      omega[0] = s->st_host->wann.om_i;
-      omega[1] = s->st_host->wann.om_d;
-      omega[2] = s->st_host->wann.om_od;
-      omega[3] = s->st_host->wann.om_tot;
+      omega[0] = st.wann.om_i;
+      omega[1] = st.wann.om_d;
+      omega[2] = st.wann.om_od;
+      omega[3] = st.wann.om_tot;
     }
     return W90GPU_OK;
   };
@@ -2065,19 +1499,23 @@ int w90gpu_domega(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* m_m
   w90gpu_session* s = nullptr;
   int rc = w90gpu_session_create(ctx, prob, &ctl, &s);
   if (rc) return rc;
-  const size_t mb = (size_t)s->nw * s->nw * s->nn * s->nk * sizeof(cplx);
-  const size_t ub = (size_t)s->nw * s->nw * s->nk * sizeof(cplx);
-  const size_t nd = (size_t)s->nw * s->nw * s->nk;
+  Engine* E = s->eng;
+  Member* m = s->m;
+  const size_t mb = (size_t)m->nw * m->nw * m->nn * m->nk * sizeof(cplx);
+  const size_t ub = (size_t)m->nw * m->nw * m->nk * sizeof(cplx);
+  const size_t nd = (size_t)m->nw * m->nw * m->nk;
   auto run = [&]() -> int {
-    HIPCHK(ctx, hipMemcpyAsync(s->p.M[0], m_matrix, mb, hipMemcpyHostToDevice, s->stream));
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s->stream, s->p, s->ctl, 1, 0.0);
-    hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(1024, (nd + 255) / 256)), dim3(256), 0,
-                       s->stream, s->p.D, nd);
-    int r = launch_omega(s, MODE_INIT);
+    Lists L;
+    int r = single_lists(E, m, L);
     if (r) return r;
-    if ((r = launch_domega(s))) return r;
-    HIPCHK(ctx, hipMemcpyAsync(cdodq, s->p.G, ub, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    HIPCHK(ctx, hipMemcpyAsync(m->p.M[0], m_matrix, mb, hipMemcpyHostToDevice, E->stream));
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, E->stream, E->d_probs, L.plist, 1);
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(1024, (nd + 255) / 256)), dim3(256), 0, E->stream,
+                       m->p.D, nd);
+    if ((r = launch_omega(E, L, MODE_INIT))) return r;
+    if ((r = launch_domega(E, L))) return r;
+    HIPCHK(ctx, hipMemcpyAsync(cdodq, m->p.G, ub, hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(ctx, hipStreamSynchronize(E->stream));
     return W90GPU_OK;
   };
   rc = run();
@@ -2094,20 +1532,24 @@ int w90gpu_new_u_and_m(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double
   w90gpu_session* s = nullptr;
   int rc = w90gpu_session_create(ctx, prob, &ctl, &s);
   if (rc) return rc;
-  const size_t mb = (size_t)s->nw * s->nw * s->nn * s->nk * sizeof(cplx);
-  const size_t ub = (size_t)s->nw * s->nw * s->nk * sizeof(cplx);
+  Engine* E = s->eng;
+  Member* m = s->m;
+  const size_t mb = (size_t)m->nw * m->nw * m->nn * m->nk * sizeof(cplx);
+  const size_t ub = (size_t)m->nw * m->nw * m->nk * sizeof(cplx);
   auto run = [&]() -> int {
-    HIPCHK(ctx, hipMemcpyAsync(s->p.M[0], m_matrix, mb, hipMemcpyHostToDevice, s->stream));
-    HIPCHK(ctx, hipMemcpyAsync(s->p.U[0], u_matrix, ub, hipMemcpyHostToDevice, s->stream));
-    HIPCHK(ctx, hipMemcpyAsync(s->p.D, cdq, ub, hipMemcpyHostToDevice, s->stream));
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s->stream, s->p, s->ctl, 1, 0.0);
-    hipLaunchKernelGGL(k_set_direct, dim3(1), dim3(64), 0, s->stream, s->p, 1.0);
-    int r = launch_step(s, MODE_SECOND);
+    Lists L;
+    int r = single_lists(E, m, L);
     if (r) return r;
-    if ((r = launch_rotate(s, MODE_SECOND))) return r;
-    HIPCHK(ctx, hipMemcpyAsync(m_matrix, s->p.M[1], mb, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(ctx, hipMemcpyAsync(u_matrix, s->p.U[1], ub, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    HIPCHK(ctx, hipMemcpyAsync(m->p.M[0], m_matrix, mb, hipMemcpyHostToDevice, E->stream));
+    HIPCHK(ctx, hipMemcpyAsync(m->p.U[0], u_matrix, ub, hipMemcpyHostToDevice, E->stream));
+    HIPCHK(ctx, hipMemcpyAsync(m->p.D, cdq, ub, hipMemcpyHostToDevice, E->stream));
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, E->stream, E->d_probs, L.plist, 1);
+    hipLaunchKernelGGL(k_set_direct, dim3(1), dim3(64), 0, E->stream, E->d_probs + m->slot, 1.0);
+    if ((r = launch_step(E, L, MODE_SECOND))) return r;
+    if ((r = launch_rotate(E, L, MODE_SECOND))) return r;
+    HIPCHK(ctx, hipMemcpyAsync(m_matrix, m->p.M[1], mb, hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(ctx, hipMemcpyAsync(u_matrix, m->p.U[1], ub, hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(ctx, hipStreamSynchronize(E->stream));
     return W90GPU_OK;
   };
   rc = run();
@@ -2124,26 +1566,31 @@ int w90gpu_project(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* a_
   w90gpu_session* s = nullptr;
   int rc = w90gpu_session_create(ctx, prob, &ctl, &s);
   if (rc) return rc;
-  const size_t mb = (size_t)s->nw * s->nw * s->nn * s->nk * sizeof(cplx);
-  const size_t ub = (size_t)s->nw * s->nw * s->nk * sizeof(cplx);
+  Engine* E = s->eng;
+  Member* m = s->m;
+  const size_t mb = (size_t)m->nw * m->nw * m->nn * m->nk * sizeof(cplx);
+  const size_t ub = (size_t)m->nw * m->nw * m->nk * sizeof(cplx);
   auto run = [&]() -> int {
-    // A is staged in the gradient buffer, which this phase does not use
-    HIPCHK(ctx, hipMemcpyAsync(s->p.G, a_matrix, ub, hipMemcpyHostToDevice, s->stream));
-    HIPCHK(ctx, hipMemcpyAsync(s->p.M[0], m_matrix, mb, hipMemcpyHostToDevice, s->stream));
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s->stream, s->p, s->ctl, 1, 0.0);
-    hipLaunchKernelGGL(k_polar, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p, (const cplx*)s->p.G);
-    HIPCHK(ctx, hipGetLastError());
-    hipLaunchKernelGGL(k_set_direct, dim3(1), dim3(64), 0, s->stream, s->p, 1.0);
-    int r = launch_rotate(s, MODE_SECOND);
+    Lists L;
+    int r = single_lists(E, m, L);
     if (r) return r;
-    HIPCHK(ctx, hipMemcpyAsync(s->p.U[0], s->p.E, ub, hipMemcpyDeviceToDevice, s->stream));
-    hipLaunchKernelGGL(k_unitarity, dim3(s->nk), dim3(W90_NT), 0, s->stream, s->p);
+    const int n = all_items(L);
+    // A is staged in the gradient buffer, which this phase does not use
+    HIPCHK(ctx, hipMemcpyAsync(m->p.G, a_matrix, ub, hipMemcpyHostToDevice, E->stream));
+    HIPCHK(ctx, hipMemcpyAsync(m->p.M[0], m_matrix, mb, hipMemcpyHostToDevice, E->stream));
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, E->stream, E->d_probs, L.plist, 1);
+    hipLaunchKernelGGL(k_polar, dim3(n), dim3(W90_NT), 0, E->stream, E->d_probs, L.items, n);
     HIPCHK(ctx, hipGetLastError());
-    std::vector<double> dev(s->nk);
-    HIPCHK(ctx, hipMemcpyAsync(dev.data(), s->p.unit, sizeof(double) * s->nk, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(ctx, hipMemcpyAsync(u_matrix, s->p.U[0], ub, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(ctx, hipMemcpyAsync(m_matrix, s->p.M[1], mb, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    hipLaunchKernelGGL(k_set_direct, dim3(1), dim3(64), 0, E->stream, E->d_probs + m->slot, 1.0);
+    if ((r = launch_rotate(E, L, MODE_SECOND))) return r;
+    HIPCHK(ctx, hipMemcpyAsync(m->p.U[0], m->p.E, ub, hipMemcpyDeviceToDevice, E->stream));
+    hipLaunchKernelGGL(k_unitarity, dim3(n), dim3(W90_NT), 0, E->stream, E->d_probs, L.items, n);
+    HIPCHK(ctx, hipGetLastError());
+    std::vector<double> dev(m->nk);
+    HIPCHK(ctx, hipMemcpyAsync(dev.data(), m->p.unit, sizeof(double) * m->nk, hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(ctx, hipMemcpyAsync(u_matrix, m->p.U[0], ub, hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(ctx, hipMemcpyAsync(m_matrix, m->p.M[1], mb, hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(ctx, hipStreamSynchronize(E->stream));
     double mx = 0.0;
     for (double d : dev) mx = (d > mx || d != d) ? d : mx;
     if (!(mx <= 1e-5)) {

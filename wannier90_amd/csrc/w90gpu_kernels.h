@@ -1,0 +1,1583 @@
+// Device code of libw90gpu: the kernels of the spread minimisation
+// (wann_main without disentanglement) on MI355X (gfx950).
+//
+// Every kernel works on a set of problems.  A problem's device state is a
+// Prob (pointers into its HBM arena) in a device array; per-k kernels take
+// a list of work items (problem, k) and run one workgroup per item,
+// per-problem kernels take a list of problem indices and run one workgroup
+// per problem.  One launch per phase therefore covers every active problem
+// of a batch, and a single problem is a batch of one.  A problem whose loop
+// has finished (State::done) or waits for the host (State::hold) makes its
+// workgroups exit at once: the per-problem convergence mask.
+//
+// All scalar control of wann_main -- the Fletcher-Reeves coefficient and its
+// resets, the uphill test, the parabolic line search and the convergence
+// window -- runs in per-problem control kernels, so an iteration is a fixed
+// sequence of launches with no host round trip.
+//
+// Storage precision.  The kernels that read or write M(k,b) are templates
+// on the complex storage type T2: double2 for the FP64 path, float2 for the
+// bulk phase of the mixed-precision path, whose rotation runs in FP32 with
+// FP32 accumulation.  Everything else is FP64 in both: U, the step
+// exp(dW) (so U stays unitary to FP64 rounding), Im ln M_nn, the centres,
+// every spread sum and the gradient arithmetic.  The mixed bulk phase ends
+// when Omega stops decreasing by more than its FP32 noise floor; M is then
+// rebuilt in FP64 from the initial overlaps, M(k,b) = E(k)^+ M0(k,b)
+// E(k+b) with E = U0^+ U (MODE_REMIX), and the FP64 loop polishes to
+// conv_tol.  A bulk
+// step that does not lower Omega beyond the FP32 noise floor is not taken.
+//
+// Reference: src/wannierise.F90 (wann_main :48-1072, internal_new_u_and_m
+// :1599-1751, wann_omega :1979-2284, wann_domega :2287-2569,
+// wann_check_unitarity :2594-2661), src/overlap.F90 (overlap_project
+// :758-952).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "../../include/w90gpu.h"
+
+typedef double2 cplx;
+
+#define W90_NT 256          // threads per workgroup of the per-k kernels
+#define W90_PANEL 16        // output rows per panel-rotation workgroup
+#define W90_FINAL_NT 1024   // threads of the per-problem omega kernel
+#define W90_MAX_GROUPS 32   // k-ranges of the first omega reduction level
+#define W90_MIXED_RTOL 1e-6 // relative FP32 noise floor of Omega in the bulk phase
+
+// --------------------------------------------------------------- helpers
+
+__device__ __forceinline__ cplx c_make(double r, double i) { return make_double2(r, i); }
+__device__ __forceinline__ cplx c_add(cplx a, cplx b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cplx c_sub(cplx a, cplx b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cplx c_scale(cplx a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ cplx c_conj(cplx a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ cplx c_mul(cplx a, cplx b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// conj(a) * b
+__device__ __forceinline__ cplx c_cmul(cplx a, cplx b) {
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ cplx c_div(cplx a, cplx b) {
+  const double d = b.x * b.x + b.y * b.y;
+  return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+__device__ __forceinline__ double c_abs2(cplx a) { return a.x * a.x + a.y * a.y; }
+
+// FP32 complex (storage of the mixed bulk phase)
+__device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 c_mul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <class T2>
+__device__ __forceinline__ T2 c_zero() {
+  T2 z;
+  z.x = 0;
+  z.y = 0;
+  return z;
+}
+// storage -> FP64 and FP64 -> storage
+__device__ __forceinline__ cplx c_wide(cplx a) { return a; }
+__device__ __forceinline__ cplx c_wide(float2 a) { return make_double2((double)a.x, (double)a.y); }
+template <class T2>
+__device__ __forceinline__ T2 c_narrow(cplx a);
+template <>
+__device__ __forceinline__ cplx c_narrow<cplx>(cplx a) {
+  return a;
+}
+template <>
+__device__ __forceinline__ float2 c_narrow<float2>(cplx a) {
+  return make_float2((float)a.x, (float)a.y);
+}
+
+// Block-wide sum of one double per thread; deterministic (fixed tree), any
+// block size: the part above the largest power of two is folded in first.
+__device__ double block_sum(double v, double* red) {
+  const int t = threadIdx.x;
+  const int n = blockDim.x;
+  int p2 = 1;
+  while (p2 * 2 <= n) p2 *= 2;
+  red[t] = v;
+  __syncthreads();
+  if (t < n - p2) red[t] += red[t + p2];
+  __syncthreads();
+  for (int s = p2 / 2; s > 0; s >>= 1) {
+    if (t < s) red[t] += red[t + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ double block_max(double v, double* red) {
+  const int t = threadIdx.x;
+  const int n = blockDim.x;
+  int p2 = 1;
+  while (p2 * 2 <= n) p2 *= 2;
+  red[t] = v;
+  __syncthreads();
+  if (t < n - p2) red[t] = fmax(red[t], red[t + p2]);
+  __syncthreads();
+  for (int s = p2 / 2; s > 0; s >>= 1) {
+    if (t < s) red[t] = fmax(red[t], red[t + s]);
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// Sum over the 64 lanes of a wavefront (butterfly; lane 0's value is used).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// --------------------------------------------------------------- state
+
+struct Spread {
+  double om_i, om_d, om_od, om_tot;
+};
+
+enum Mode { MODE_INIT = 0, MODE_TRIAL = 1, MODE_SECOND = 2, MODE_REMIX = 3 };
+
+// Per-problem solver state, resident in device memory.
+struct State {
+  // control (wann_control_type)
+  int num_iter, num_cg_steps, conv_window, lfixstep;
+  double conv_tol, trial_step, fixed_step, wbtot;
+  // loop state of wann_main
+  int iter;        // completed iterations
+  int ncg;
+  int cur;         // buffer holding the current U, M
+  int done;        // loop finished (converged or num_iter)
+  int converged;
+  int lquad;
+  int second;      // parabolic step pending in this iteration
+  int conv_count;
+  int first_pass;
+  int phase;       // 0: mixed bulk (FP32 M), 1: FP64
+  int hold;        // bulk phase ended, waiting for the FP64 rebuild of M
+  int bulk_iter;   // iterations done in the bulk phase
+  int pad0, pad1;
+  double gcnorm0, gcnorm1, gcfac, doda0, alphamin, falphamin;
+  double beta, dir_sign, scale;
+  double omega_invariant;
+  Spread wann, old, trial;
+  double history[W90GPU_MAX_CONV_WINDOW];
+};
+
+// Device view of one problem: sizes and pointers into its HBM arena.
+struct Prob {
+  int nw, nk, nn, npan;
+  int psz, nw2, ngroup, pad;  // psz: per-k partial vector length; ngroup: first-level k-ranges
+  const int* nnl;     // [k][b] 0-based neighbour
+  const double* bk;   // [k][b][3]
+  const double* wb;   // [b]
+  cplx* M[2];         // [k][b][nw*nw] column-major blocks, FP64
+  float2* Mf[2];      // FP32 blocks of the mixed bulk phase (inside M[1])
+  const cplx* M0;     // initial M of the problem
+  cplx* U[2];         // [k][nw*nw]
+  const cplx* U0;     // initial U
+  cplx* G;            // gradient
+  cplx* D;            // search direction (cdqkeep)
+  cplx* E;            // exp(dW) per k
+  cplx* W1;           // workspace per k
+  cplx* W2;           // workspace per k
+  double* ln[2];      // [k][b][n] Im ln M_nn of each buffer
+  double* d2;         // [k][b][n] |M_nn|^2 (panel path)
+  double* part_od;    // [k][b][panel] (panel path)
+  double* part_all;   // [k][b][panel] (panel path)
+  double* kpart;      // [k][psz] per-k partial sums of wann_omega (DiagAcc layout)
+  double* gpart;      // [ngroup][psz] sums over contiguous k-ranges
+  double* part_gg;    // [k]
+  double* part_gd;    // [k]
+  double* rave[2];    // [3][nw] per buffer (Fortran rave(3,nw): ind fastest)
+  double* r2ave[2];   // [nw]
+  double* unit;       // [k] unitarity deviation
+  double* log;        // [num_iter+1][4] om_tot, om_d, om_od, gcnorm1
+  State* st;
+  const State* st_init;  // state at the start of a solve (control + first_pass)
+};
+
+// A per-k work item.
+struct Item {
+  int prob;
+  int k;
+};
+
+// Per-problem status the host reads after each block of iterations.
+struct Status {
+  int done, hold, iter, converged;
+};
+
+// Final state of a problem gathered for the host.
+struct Final {
+  State st;
+  double unit_max;
+  double pad;
+};
+
+template <class T2>
+__device__ __forceinline__ T2* mbuf(const Prob& p, int i);
+template <>
+__device__ __forceinline__ cplx* mbuf<cplx>(const Prob& p, int i) {
+  return p.M[i];
+}
+template <>
+__device__ __forceinline__ float2* mbuf<float2>(const Prob& p, int i) {
+  return p.Mf[i];
+}
+
+__device__ __forceinline__ bool gate(const State* st, int mode) {
+  if (mode == MODE_REMIX) return st->hold != 0 && !st->done;
+  if (st->done || st->hold) return false;
+  if (mode == MODE_SECOND) return st->second != 0;
+  return true;
+}
+
+// buffer written by a rotation of this mode: the next buffer for a step;
+// the current one for the initial evaluation and for the FP64 rebuild
+// (whose M then pairs with the current U)
+__device__ __forceinline__ int dst_buf(int mode, int cur) {
+  return (mode == MODE_INIT || mode == MODE_REMIX) ? cur : 1 - cur;
+}
+
+// --------------------------------------------------------- setup kernels
+
+// Start a solve: the loop state of each listed problem from its template.
+__global__ void k_reset(const Prob* probs, const int* plist, int np) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const Prob& p = probs[plist[i]];
+  *p.st = *p.st_init;
+}
+
+// Direct-step state for internal_new_u_and_m as a standalone phase.
+__global__ void k_set_direct(const Prob* probs, double scale) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  State* st = probs[0].st;
+  st->done = 0;
+  st->hold = 0;
+  st->second = 1;
+  st->cur = 0;
+  st->scale = scale;
+}
+
+__global__ void k_zero(cplx* x, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    x[i] = c_make(0.0, 0.0);
+}
+
+// Per item: M[0](k) <- M0(k), U[0](k) <- U0(k), D(k) <- 0 (start of a solve).
+__global__ void __launch_bounds__(W90_NT) k_load_initial(const Prob* probs, const Item* items, int nitems) {
+  if (blockIdx.x >= (unsigned)nitems) return;
+  const Item it = items[blockIdx.x];
+  const Prob& p = probs[it.prob];
+  const size_t nw2 = p.nw2, mb = nw2 * p.nn;
+  const cplx* m0 = p.M0 + (size_t)it.k * mb;
+  cplx* m = p.M[0] + (size_t)it.k * mb;
+  for (size_t e = threadIdx.x; e < mb; e += blockDim.x) m[e] = m0[e];
+  const cplx* u0 = p.U0 + (size_t)it.k * nw2;
+  cplx* u = p.U[0] + (size_t)it.k * nw2;
+  cplx* d = p.D + (size_t)it.k * nw2;
+  for (size_t e = threadIdx.x; e < nw2; e += blockDim.x) {
+    u[e] = u0[e];
+    d[e] = c_make(0.0, 0.0);
+  }
+}
+
+// Per item: FP32 copy of the current FP64 M (entry to the mixed bulk phase).
+__global__ void __launch_bounds__(W90_NT) k_to_f32(const Prob* probs, const Item* items, int nitems) {
+  if (blockIdx.x >= (unsigned)nitems) return;
+  const Item it = items[blockIdx.x];
+  const Prob& p = probs[it.prob];
+  const State* st = p.st;
+  if (st->done || st->phase != 0) return;
+  const size_t mb = (size_t)p.nw2 * p.nn;
+  const cplx* src = p.M[st->cur] + (size_t)it.k * mb;
+  float2* dst = p.Mf[st->cur] + (size_t)it.k * mb;
+  for (size_t e = threadIdx.x; e < mb; e += blockDim.x) dst[e] = c_narrow<float2>(src[e]);
+}
+
+// Per item of a problem on hold: E(k) = U0(k)^+ U(k), the rotation that
+// takes the initial overlaps M0 to the gauge of the current U.
+__global__ void __launch_bounds__(W90_NT) k_delta_u(const Prob* probs, const Item* items, int nitems);
+
+// Status of each listed problem (one thread per problem).
+__global__ void k_status(const Prob* probs, const int* plist, int np, Status* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const State* st = probs[plist[i]].st;
+  Status s;
+  s.done = st->done;
+  s.hold = st->hold;
+  s.iter = st->iter;
+  s.converged = st->converged;
+  out[i] = s;
+}
+
+// Final state and largest unitarity deviation of each listed problem (one
+// workgroup per problem; run after k_unitarity).
+__global__ void __launch_bounds__(W90_NT) k_gather_final(const Prob* probs, const int* plist, int np,
+                                                         Final* out) {
+  if (blockIdx.x >= (unsigned)np) return;
+  const Prob& p = probs[plist[blockIdx.x]];
+  __shared__ double red[W90_NT];
+  double mx = 0.0;
+  for (int k = threadIdx.x; k < p.nk; k += blockDim.x) {
+    const double v = p.unit[k];
+    mx = (v != v) ? INFINITY : fmax(mx, v);
+  }
+  mx = block_max(mx, red);
+  if (threadIdx.x == 0) {
+    out[blockIdx.x].st = *p.st;
+    out[blockIdx.x].unit_max = mx;
+  }
+}
+
+// ------------------------------------------------- omega partial vectors
+
+// Per-k partial sums of wann_omega.  For Wannier function n, over the b of
+// one k:
+//   [q*nw + n]     S_rave[q] = sum_b wb b_q ln_n              (q = 0,1,2)
+//   [3*nw + n]     S_r2      = sum_b wb (1 - |M_nn|^2 + ln_n^2)
+//   [4*nw + n]     A         = sum_b wb (ln_n + b.r0_n)^2
+//   [(5+q)*nw + n] V[q]      = sum_b wb b_q (ln_n + b.r0_n)
+// and the scalars
+//   [8*nw + 0]     S_od      = sum_b wb sum_{m!=n} |M_mn|^2
+//   [8*nw + 1]     S_i       = sum_b wb (nw - sum_mn |M_mn|^2)
+//   [8*nw + 2..7]  B         = sum_b wb b b^T  (xx, yy, zz, xy, yz, zx)
+// with ln_n = Im ln M_nn and r0 the centres of the source buffer.  With
+// d = rave - r0, Omega_D = (1/nk) sum_n [A + 2 d.V + d^T B d] is exact
+// algebra for (1/nk) sum wb (ln + b.rave)^2 (wannierise.F90:2260-2275), and
+// r0 close to rave keeps it free of cancellation, so no second pass over ln
+// is needed once rave is known.
+struct DiagAcc {
+  double r[3], r2, a, v[3];
+};
+
+__device__ __forceinline__ void diag_add(DiagAcc& d, double w, const double* bv, double l, double d2,
+                                         double r0x, double r0y, double r0z) {
+  d.r[0] += w * bv[0] * l;
+  d.r[1] += w * bv[1] * l;
+  d.r[2] += w * bv[2] * l;
+  d.r2 += w * (1.0 - d2 + l * l);
+  const double x = l + (bv[0] * r0x + bv[1] * r0y + bv[2] * r0z);
+  d.a += w * x * x;
+  d.v[0] += w * bv[0] * x;
+  d.v[1] += w * bv[1] * x;
+  d.v[2] += w * bv[2] * x;
+}
+
+__device__ __forceinline__ void diag_store(double* out, int nw, int n, const DiagAcc& d) {
+  out[0 * nw + n] = d.r[0];
+  out[1 * nw + n] = d.r[1];
+  out[2 * nw + n] = d.r[2];
+  out[3 * nw + n] = d.r2;
+  out[4 * nw + n] = d.a;
+  out[5 * nw + n] = d.v[0];
+  out[6 * nw + n] = d.v[1];
+  out[7 * nw + n] = d.v[2];
+}
+
+// Scalars of the per-k vector that depend only on the k-mesh: B = sum wb b b^T.
+__device__ void kpart_bmat(const Prob& p, int k, double* out) {
+  double b6[6] = {0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < p.nn; ++b) {
+    const double w = p.wb[b];
+    const double* bv = p.bk + ((size_t)k * p.nn + b) * 3;
+    b6[0] += w * bv[0] * bv[0];
+    b6[1] += w * bv[1] * bv[1];
+    b6[2] += w * bv[2] * bv[2];
+    b6[3] += w * bv[0] * bv[1];
+    b6[4] += w * bv[1] * bv[2];
+    b6[5] += w * bv[2] * bv[0];
+  }
+  for (int u = 0; u < 6; ++u) out[2 + u] = b6[u];
+}
+
+// Reference centres r0 of the partial vectors: the centres of the source
+// buffer (zero for MODE_INIT, where no earlier centres exist).
+__device__ __forceinline__ void ref_centre(const Prob& p, int mode, int cur, int n, double* r0) {
+  if (mode == MODE_INIT) {
+    r0[0] = r0[1] = r0[2] = 0.0;
+  } else {
+    r0[0] = p.rave[cur][0 + 3 * n];
+    r0[1] = p.rave[cur][1 + 3 * n];
+    r0[2] = p.rave[cur][2 + 3 * n];
+  }
+}
+
+// Source block of a rotation: M(k,b) of the current buffer, rotated by
+// E = exp(dW); for MODE_REMIX the initial M0(k,b), rotated by
+// E = U0^+ U (k_delta_u), since M0 is the overlap in the gauge of U0.
+template <class T2>
+__device__ __forceinline__ const T2* rot_src(const Prob& p, int mode, int cur) {
+  if constexpr (std::is_same<T2, cplx>::value) {
+    if (mode == MODE_REMIX) return p.M0;
+  }
+  return mbuf<T2>(p, cur);
+}
+__device__ __forceinline__ const cplx* rot_factor(const Prob& p, int, int) { return p.E; }
+
+// --------------------------------------------------------- M rotation
+
+// M rotation M(k,b) <- E_k^+ M(k,b) E_{k+b} for one (problem, k, b, row
+// panel) -- gridDim.y covers b and the panels -- with the per-block partial
+// sums of wann_omega: Im ln M_nn, |M_nn|^2, sum_{m!=n}|M_mn|^2 and
+// sum_mn |M_mn|^2.  Used for num_wann above the LDS kernels.
+// MODE_INIT only evaluates the partial sums of the current buffer.
+// (internal_new_u_and_m :1737-1748; wann_omega :2034-2043, :2076-2090,
+//  :2214-2258)
+template <class T2>
+__global__ void __launch_bounds__(W90_NT) k_rotate(const Prob* probs, const Item* items, int nitems, int mode) {
+  if (blockIdx.x >= (unsigned)nitems) return;
+  const Item it = items[blockIdx.x];
+  const Prob& p = probs[it.prob];
+  const State* st = p.st;
+  if (!gate(st, mode)) return;
+  const int nw = p.nw, nw2 = p.nw2;
+  const int bp = blockIdx.y;
+  if (bp >= p.nn * p.npan) return;
+  const int panel = bp % p.npan;
+  const int kb = it.k * p.nn + bp / p.npan;
+  const int k = it.k;
+  const int cur = st->cur;
+  const int dst = dst_buf(mode, cur);
+  const int i0 = panel * W90_PANEL;
+  const int prows = min(W90_PANEL, nw - i0);
+  const T2* Ms = rot_src<T2>(p, mode, cur) + (size_t)kb * nw2;
+  T2* Md = mbuf<T2>(p, dst) + (size_t)kb * nw2;
+  __shared__ T2 T[W90_PANEL * W90GPU_MAX_WANN];
+  __shared__ double red[W90_NT];
+
+  const int nout = prows * nw;
+  double od = 0.0, all = 0.0;
+  if (mode != MODE_INIT) {
+    const cplx* F = rot_factor(p, mode, cur);
+    const cplx* Ek = F + (size_t)k * nw2;
+    const cplx* Ekb = F + (size_t)p.nnl[kb] * nw2;
+    for (int e = threadIdx.x; e < nout; e += blockDim.x) {
+      const int i = e % prows, j = e / prows;
+      const cplx* ec = Ek + (size_t)(i0 + i) * nw;
+      const T2* mc = Ms + (size_t)j * nw;
+      T2 acc = c_zero<T2>();
+      for (int l = 0; l < nw; ++l) acc = c_add(acc, c_mul(c_narrow<T2>(c_conj(ec[l])), mc[l]));
+      T[i + j * W90_PANEL] = acc;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nout; e += blockDim.x) {
+      const int i = e % prows, j = e / prows;
+      const cplx* ec = Ekb + (size_t)j * nw;
+      T2 acc = c_zero<T2>();
+      for (int l = 0; l < nw; ++l) acc = c_add(acc, c_mul(T[i + l * W90_PANEL], c_narrow<T2>(ec[l])));
+      Md[(i0 + i) + (size_t)j * nw] = acc;
+      const cplx a = c_wide(acc);
+      const double a2 = c_abs2(a);
+      all += a2;
+      if (i0 + i == j) {
+        p.ln[dst][(size_t)kb * nw + j] = atan2(a.y, a.x);
+        p.d2[(size_t)kb * nw + j] = a2;
+      } else {
+        od += a2;
+      }
+    }
+  } else {
+    for (int e = threadIdx.x; e < nout; e += blockDim.x) {
+      const int i = e % prows, j = e / prows;
+      const cplx v = c_wide(Ms[(i0 + i) + (size_t)j * nw]);
+      const double a2 = c_abs2(v);
+      all += a2;
+      if (i0 + i == j) {
+        p.ln[dst][(size_t)kb * nw + j] = atan2(v.y, v.x);
+        p.d2[(size_t)kb * nw + j] = a2;
+      } else {
+        od += a2;
+      }
+    }
+  }
+  od = block_sum(od, red);
+  all = block_sum(all, red);
+  if (threadIdx.x == 0) {
+    p.part_od[(size_t)kb * p.npan + panel] = od;
+    p.part_all[(size_t)kb * p.npan + panel] = all;
+  }
+}
+
+// Per-k partial vector from the outputs of the panel rotation kernel.
+__global__ void __launch_bounds__(W90_NT) k_kpart(const Prob* probs, const Item* items, int nitems, int mode) {
+  if (blockIdx.x >= (unsigned)nitems) return;
+  const Item it = items[blockIdx.x];
+  const Prob& p = probs[it.prob];
+  const State* st = p.st;
+  if (!gate(st, mode)) return;
+  const int nw = p.nw, nn = p.nn;
+  const int k = it.k;
+  const int cur = st->cur;
+  const int dst = dst_buf(mode, cur);
+  const double* ln = p.ln[dst];
+  double* out = p.kpart + (size_t)k * p.psz;
+  for (int n = threadIdx.x; n < nw; n += blockDim.x) {
+    double r0[3];
+    ref_centre(p, mode, cur, n, r0);
+    DiagAcc d = {};
+    for (int b = 0; b < nn; ++b) {
+      const size_t kb = (size_t)k * nn + b;
+      diag_add(d, p.wb[b], p.bk + kb * 3, ln[kb * nw + n], p.d2[kb * nw + n], r0[0], r0[1], r0[2]);
+    }
+    diag_store(out, nw, n, d);
+  }
+  if (threadIdx.x == 0) {
+    double sod = 0.0, si = 0.0;
+    for (int b = 0; b < nn; ++b) {
+      const size_t kb = (size_t)k * nn + b;
+      double od = 0.0, al = 0.0;
+      for (int q = 0; q < p.npan; ++q) {
+        od += p.part_od[kb * p.npan + q];
+        al += p.part_all[kb * p.npan + q];
+      }
+      sod += p.wb[b] * od;
+      si += p.wb[b] * ((double)nw - al);
+    }
+    out[8 * nw + 0] = sod;
+    out[8 * nw + 1] = si;
+    kpart_bmat(p, k, out + 8 * nw);
+  }
+}
+
+// C = A B of n x n column-major complex matrices in LDS with leading
+// dimension n + 1; thread slots e = tid + s*W90_NT.
+template <int NWM>
+__device__ __forceinline__ void lds_gemm(cplx* C, const cplx* A, const cplx* B, int n) {
+  constexpr int SL = (NWM * NWM + W90_NT - 1) / W90_NT;
+  const int n2 = n * n, ld = n + 1;
+#pragma unroll
+  for (int s = 0; s < SL; ++s) {
+    const int e = threadIdx.x + s * W90_NT;
+    if (e < n2) {
+      const int i = e % n, j = e / n;
+      cplx acc = c_make(0.0, 0.0);
+      for (int l = 0; l < n; ++l) acc = c_add(acc, c_mul(A[i + l * ld], B[l + j * ld]));
+      C[i + j * ld] = acc;
+    }
+  }
+  __syncthreads();
+}
+
+// M rotation with the omega partial sums, one workgroup per (problem, k),
+// all operands in LDS (num_wann <= NWM): E_k is loaded once for the nn
+// blocks of k; each block M(k,b) and E_{k+b} are staged in LDS, T = E_k^+ M
+// and R = T E_{k+b} are formed there, R is written once and reduced in
+// registers into the per-k partial vector.  LDS matrices are column-major
+// with leading dimension nw + 1 (conflict-free rows and broadcast columns),
+// E_k stored conjugate-transposed.  MODE_INIT only reduces.
+// (internal_new_u_and_m :1737-1748; wann_omega :2034-2090, :2214-2275)
+template <int NWM, class T2>
+__global__ void __launch_bounds__(W90_NT) k_rotate_k(const Prob* probs, const Item* items, int nitems, int mode) {
+  if (blockIdx.x >= (unsigned)nitems) return;
+  const Item it = items[blockIdx.x];
+  const Prob& p = probs[it.prob];
+  const State* st = p.st;
+  if (!gate(st, mode)) return;
+  constexpr int SL = (NWM * NWM + W90_NT - 1) / W90_NT;
+  const int nw = p.nw, nw2 = p.nw2, nn = p.nn;
+  const int ld = nw + 1;
+  const int k = it.k;
+  const int tid = threadIdx.x;
+  const int cur = st->cur;
+  const bool rot = (mode != MODE_INIT);
+  const int dst = dst_buf(mode, cur);
+  const T2* src0 = rot_src<T2>(p, mode, cur);
+  const cplx* F = rot_factor(p, mode, cur);
+  __shared__ T2 EkH[NWM * (NWM + 1)];
+  __shared__ T2 Eb[NWM * (NWM + 1)];
+  __shared__ T2 Ms[NWM * (NWM + 1)];
+  __shared__ T2 T[NWM * (NWM + 1)];
+  __shared__ double red[W90_NT];
+
+  DiagAcc acc[SL];
+  double r0[SL][3];
+#pragma unroll
+  for (int s = 0; s < SL; ++s) {
+    acc[s] = DiagAcc{};
+    r0[s][0] = r0[s][1] = r0[s][2] = 0.0;
+    const int e = tid + s * W90_NT;
+    if (e < nw2 && e % nw == e / nw) ref_centre(p, mode, cur, e % nw, r0[s]);
+  }
+  if (rot) {
+    const cplx* ek = F + (size_t)k * nw2;
+    for (int e = tid; e < nw2; e += W90_NT) EkH[(e / nw) + (e % nw) * ld] = c_narrow<T2>(c_conj(ek[e]));
+  }
+  double s_od = 0.0, s_all = 0.0;
+  for (int b = 0; b < nn; ++b) {
+    const size_t kb = (size_t)k * nn + b;
+    const T2* src = src0 + kb * nw2;
+    __syncthreads();
+    for (int e = tid; e < nw2; e += W90_NT) Ms[(e % nw) + (e / nw) * ld] = src[e];
+    if (rot) {
+      const cplx* eb = F + (size_t)p.nnl[kb] * nw2;
+      for (int e = tid; e < nw2; e += W90_NT) Eb[(e % nw) + (e / nw) * ld] = c_narrow<T2>(eb[e]);
+    }
+    __syncthreads();
+    if (rot) {
+#pragma unroll
+      for (int s = 0; s < SL; ++s) {
+        const int e = tid + s * W90_NT;
+        if (e < nw2) {
+          const int i = e % nw, j = e / nw;
+          T2 a = c_zero<T2>();
+          for (int l = 0; l < nw; ++l) a = c_add(a, c_mul(EkH[i + l * ld], Ms[l + j * ld]));
+          T[i + j * ld] = a;
+        }
+      }
+      __syncthreads();
+    }
+    const double w = p.wb[b];
+    const double* bv = p.bk + kb * 3;
+    T2* out = mbuf<T2>(p, dst) + kb * nw2;
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int e = tid + s * W90_NT;
+      if (e < nw2) {
+        const int i = e % nw, j = e / nw;
+        T2 r;
+        if (rot) {
+          r = c_zero<T2>();
+          for (int l = 0; l < nw; ++l) r = c_add(r, c_mul(T[i + l * ld], Eb[l + j * ld]));
+          out[e] = r;
+        } else {
+          r = Ms[i + j * ld];
+        }
+        const cplx rw = c_wide(r);
+        const double a2 = c_abs2(rw);
+        s_all += w * a2;
+        if (i != j) {
+          s_od += w * a2;
+        } else {
+          const double l = atan2(rw.y, rw.x);
+          p.ln[dst][kb * nw + i] = l;
+          diag_add(acc[s], w, bv, l, a2, r0[s][0], r0[s][1], r0[s][2]);
+        }
+      }
+    }
+  }
+  double* outk = p.kpart + (size_t)k * p.psz;
+#pragma unroll
+  for (int s = 0; s < SL; ++s) {
+    const int e = tid + s * W90_NT;
+    if (e < nw2 && e % nw == e / nw) diag_store(outk, nw, e % nw, acc[s]);
+  }
+  s_od = block_sum(s_od, red);
+  s_all = block_sum(s_all, red);
+  if (tid == 0) {
+    double wsum = 0.0;
+    for (int b = 0; b < nn; ++b) wsum += p.wb[b];
+    outk[8 * nw + 0] = s_od;
+    outk[8 * nw + 1] = (double)nw * wsum - s_all;
+    kpart_bmat(p, k, outk + 8 * nw);
+  }
+}
+
+// LDS bytes of k_rotate_kb for up to nn neighbours (dynamic shared memory).
+template <int NWM, class T2>
+__host__ __device__ constexpr size_t rotate_kb_lds(int nn) {
+  return sizeof(T2) * (size_t)NWM * (NWM + 1) * (1 + 2 * (size_t)nn) +
+         sizeof(double) * (2 * (size_t)nn * NWM + 2 * (size_t)nn);
+}
+
+// M rotation with the omega partial sums for small blocks (num_wann <= NWM
+// <= 16): one workgroup per (problem, k) with one wavefront per neighbour b,
+// so the nn rotations of a k proceed concurrently.  E_k, the nn blocks
+// M(k,b) and the nn E_{k+b} are staged in LDS by one round of loads; T =
+// E_k^+ M(k,b) replaces M(k,b) in LDS; the diagonal of each rotated block
+// goes through LDS to threads n < num_wann, which sum it over b in order.
+// blockDim.x = 64 * (largest nntot of the launch); wavefronts beyond the
+// problem's nntot only take part in the barriers, and every sum is formed
+// per wavefront and then over b in order, so the result of a problem does
+// not depend on the other problems of the launch.
+template <int NWM, class T2>
+__global__ void __launch_bounds__(64 * W90GPU_NUM_NNMAX) k_rotate_kb(const Prob* probs, const Item* items,
+                                                                     int nitems, int mode) {
+  if (blockIdx.x >= (unsigned)nitems) return;
+  const Item it = items[blockIdx.x];
+  const Prob& p = probs[it.prob];
+  const State* st = p.st;
+  if (!gate(st, mode)) return;
+  constexpr int B2 = NWM * NWM;
+  constexpr int BP = NWM * (NWM + 1);
+  constexpr int SL = (B2 + 63) / 64;
+  const int nw = p.nw, nw2 = p.nw2, nn = p.nn;
+  const int nnl = blockDim.x >> 6;  // wavefronts of this launch (>= nn)
+  const int ld = nw + 1;
+  const int k = it.k;
+  const int tid = threadIdx.x;
+  const int b = tid >> 6;
+  const bool active = b < nn;
+  const int lane = tid & 63;
+  const int cur = st->cur;
+  const bool rot = (mode != MODE_INIT);
+  const int dst = dst_buf(mode, cur);
+  const cplx* F = rot_factor(p, mode, cur);
+  extern __shared__ double lds_raw[];
+  T2* EkH = reinterpret_cast<T2*>(lds_raw);
+  T2* Ms = EkH + BP + (size_t)b * 2 * BP;  // M(k,b), then T = E_k^+ M(k,b)
+  T2* Eb = Ms + BP;                        // E_{k+b}
+  double* lnL = reinterpret_cast<double*>(EkH + BP + (size_t)nnl * 2 * BP);  // [b][NWM]
+  double* d2L = lnL + nnl * NWM;
+  double* wod = d2L + nnl * NWM;  // [b] per-wave sum_{m!=n} |M_mn|^2
+  double* wall = wod + nnl;       // [b] per-wave sum_mn |M_mn|^2
+  const size_t kb = (size_t)k * nn + b;
+  if (rot)
+    for (int e = tid; e < nw2; e += blockDim.x) {
+      const int l = e % nw, i = e / nw;
+      EkH[i + l * ld] = c_narrow<T2>(c_conj(F[(size_t)k * nw2 + e]));
+    }
+  if (active) {
+    const T2* src = rot_src<T2>(p, mode, cur) + kb * nw2;
+    for (int e = lane; e < nw2; e += 64) Ms[(e % nw) + (e / nw) * ld] = src[e];
+    if (rot) {
+      const cplx* eb = F + (size_t)p.nnl[kb] * nw2;
+      for (int e = lane; e < nw2; e += 64) Eb[(e % nw) + (e / nw) * ld] = c_narrow<T2>(eb[e]);
+    }
+  }
+  __syncthreads();
+  if (rot) {
+    T2 tr[SL];
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int e = lane + 64 * s;
+      tr[s] = c_zero<T2>();
+      if (active && e < nw2) {
+        const int i = e % nw, j = e / nw;
+        T2 a = c_zero<T2>();
+        for (int l = 0; l < nw; ++l) a = c_add(a, c_mul(EkH[i + l * ld], Ms[l + j * ld]));
+        tr[s] = a;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int e = lane + 64 * s;
+      if (active && e < nw2) Ms[(e % nw) + (e / nw) * ld] = tr[s];
+    }
+    __syncthreads();
+  }
+  double s_od = 0.0, s_all = 0.0;
+  if (active) {
+    T2* out = mbuf<T2>(p, dst) + kb * nw2;
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int e = lane + 64 * s;
+      if (e < nw2) {
+        const int i = e % nw, j = e / nw;
+        T2 r;
+        if (rot) {
+          r = c_zero<T2>();
+          for (int l = 0; l < nw; ++l) r = c_add(r, c_mul(Ms[i + l * ld], Eb[l + j * ld]));
+          out[e] = r;
+        } else {
+          r = Ms[i + j * ld];
+        }
+        const cplx rw = c_wide(r);
+        const double a2 = c_abs2(rw);
+        s_all += a2;
+        if (i != j) {
+          s_od += a2;
+        } else {
+          const double l = atan2(rw.y, rw.x);
+          p.ln[dst][kb * nw + i] = l;
+          lnL[b * NWM + i] = l;
+          d2L[b * NWM + i] = a2;
+        }
+      }
+    }
+  }
+  s_od = wave_sum(s_od);
+  s_all = wave_sum(s_all);
+  if (active && lane == 0) {
+    wod[b] = s_od;
+    wall[b] = s_all;
+  }
+  __syncthreads();
+  double* outk = p.kpart + (size_t)k * p.psz;
+  if (tid < nw) {
+    const int n = tid;
+    double r0[3];
+    ref_centre(p, mode, cur, n, r0);
+    DiagAcc d = {};
+    for (int bb = 0; bb < nn; ++bb)
+      diag_add(d, p.wb[bb], p.bk + ((size_t)k * nn + bb) * 3, lnL[bb * NWM + n], d2L[bb * NWM + n], r0[0],
+               r0[1], r0[2]);
+    diag_store(outk, nw, n, d);
+  }
+  if (tid == 0) {
+    double sod = 0.0, sall = 0.0, wsum = 0.0;
+    for (int bb = 0; bb < nn; ++bb) {
+      const double w = p.wb[bb];
+      sod += w * wod[bb];
+      sall += w * wall[bb];
+      wsum += w;
+    }
+    outk[8 * nw + 0] = sod;
+    outk[8 * nw + 1] = (double)nw * wsum - sall;
+    kpart_bmat(p, k, outk + 8 * nw);
+  }
+}
+
+// ---------------------------------------------------------- omega / control
+
+// internal_test_convergence (wannierise.F90:1076-1164) with the noise branch
+// off (conv_noise_amp <= 0), then the loop-exit bookkeeping.
+__device__ void conv_step(State* st, double* logrow, double gc) {
+  const int iter = st->iter + 1;
+  const double delta = st->wann.om_tot - st->old.om_tot;
+  st->iter = iter;
+  if (logrow) {
+    logrow[0] = st->wann.om_tot;
+    logrow[1] = st->wann.om_d;
+    logrow[2] = st->wann.om_od;
+    logrow[3] = gc;
+  }
+  if (st->conv_window > 1) {
+    const int cw = st->conv_window;
+    if (st->conv_count < cw) {
+      st->history[st->conv_count] = delta;
+    } else {
+      for (int j = 0; j < cw - 1; ++j) st->history[j] = st->history[j + 1];
+      st->history[cw - 1] = delta;
+    }
+    st->conv_count += 1;
+    if (st->conv_count >= cw) {
+      bool ok = true;
+      for (int j = 0; j < cw; ++j)
+        if (fabs(st->history[j]) > st->conv_tol) ok = false;
+      if (ok) st->converged = 1;
+    }
+  }
+  if (st->converged || iter >= st->num_iter) st->done = 1;
+}
+
+// Acceptance of a step with spread sp into buffer dst (wannierise.F90
+// :660-701), then the convergence test.  In the mixed bulk phase a step is
+// taken only if it lowers Omega by more than the FP32 noise floor of Omega
+// (W90_MIXED_RTOL relative, at least conv_tol); the first step that does
+// not -- or the end of the iteration budget -- ends the bulk phase: the
+// problem holds at its current point for the FP64 rebuild of M, from which
+// the FP64 loop continues.  A noise-level line search is never followed.
+__device__ void accept_step(State* st, const Spread& sp, int dst, double* logrow) {
+  if (st->phase == 0) {
+    const double floor_tol = fmax(st->conv_tol, W90_MIXED_RTOL * fabs(st->wann.om_tot));
+    const bool progress = sp.om_tot - st->wann.om_tot < -floor_tol;
+    if (progress) {
+      st->old = st->wann;
+      st->wann = sp;
+      st->cur = dst;
+    }
+    st->iter += 1;
+    st->bulk_iter = st->iter;
+    if (logrow) {
+      logrow[0] = st->wann.om_tot;
+      logrow[1] = st->wann.om_d;
+      logrow[2] = st->wann.om_od;
+      logrow[3] = st->gcnorm1;
+    }
+    if (!progress || st->iter >= st->num_iter) st->hold = 1;
+    return;
+  }
+  st->old = st->wann;
+  st->wann = sp;
+  st->cur = dst;
+  conv_step(st, logrow, st->gcnorm1);
+}
+
+// First level of the omega reduction: workgroup (problem, g) sums the per-k
+// vectors of a contiguous k-range, in k order (rows read coalesced).
+__global__ void __launch_bounds__(W90_NT) k_omega_group(const Prob* probs, const int* plist, int np, int mode) {
+  const int pi = blockIdx.x / W90_MAX_GROUPS;
+  const int g = blockIdx.x % W90_MAX_GROUPS;
+  if (pi >= np) return;
+  const Prob& p = probs[plist[pi]];
+  if (g >= p.ngroup || !gate(p.st, mode)) return;
+  const int G = p.ngroup, psz = p.psz;
+  const int k0 = (int)(((long)g * p.nk) / G), k1 = (int)(((long)(g + 1) * p.nk) / G);
+  for (int v = threadIdx.x; v < psz; v += blockDim.x) {
+    double s = 0.0;
+    for (int k = k0; k < k1; ++k) s += p.kpart[(size_t)k * psz + v];
+    p.gpart[(size_t)g * psz + v] = s;
+  }
+}
+
+// Final reduction of wann_omega (one workgroup per problem) fused with the
+// control logic that follows each wann_omega call in wann_main: initial
+// state (:452-470), internal_optimal_step (:1528-1596) after the trial step
+// and the acceptance of the trial or parabolic step (:660-701).  MODE_REMIX
+// installs the FP64 rebuild of M at the end of the mixed bulk phase.
+__global__ void __launch_bounds__(W90_FINAL_NT) k_omega_final(const Prob* probs, const int* plist, int np,
+                                                              int mode) {
+  if (blockIdx.x >= (unsigned)np) return;
+  const Prob& p = probs[plist[blockIdx.x]];
+  State* st = p.st;
+  if (!gate(st, mode)) return;
+  const int nw = p.nw, nk = p.nk, psz = p.psz;
+  const int cur = st->cur;
+  const int dst = dst_buf(mode, cur);
+  const double rnk = (double)nk;
+  const int tid = threadIdx.x;
+  __shared__ double red[W90_FINAL_NT];
+  __shared__ double sums[8 * W90GPU_MAX_WANN + 8];
+  __shared__ double sc[8];
+
+  // sum of the group vectors: VB values per pass (coalesced along a row),
+  // J lanes over groups, lanes combined in fixed order
+  const int VB = min(psz, W90_FINAL_NT);
+  const int J = W90_FINAL_NT / VB;
+  const int G = p.ngroup;
+  for (int v0 = 0; v0 < psz; v0 += VB) {
+    const int v = v0 + tid % VB;
+    const int j = tid / VB;
+    double s = 0.0;
+    if (j < J && v < psz)
+      for (int g = j; g < G; g += J) s += p.gpart[(size_t)g * psz + v];
+    red[tid] = s;
+    __syncthreads();
+    if (tid < VB && v0 + tid < psz) {
+      double t = 0.0;
+      for (int jj = 0; jj < J; ++jj) t += red[tid + jj * VB];
+      sums[v0 + tid] = t;
+    }
+    __syncthreads();
+  }
+
+  double omd_n = 0.0;
+  for (int n = tid; n < nw; n += blockDim.x) {
+    double q[8];
+    for (int u = 0; u < 8; ++u) q[u] = sums[u * nw + n];
+    double bb[6];
+    for (int u = 0; u < 6; ++u) bb[u] = sums[8 * nw + 2 + u];
+    // reference centres are read before rave[dst] is written (dst == cur
+    // for the rebuild)
+    double r0[3];
+    ref_centre(p, mode, cur, n, r0);
+    const double rx = -q[0] / rnk, ry = -q[1] / rnk, rz = -q[2] / rnk;
+    p.rave[dst][0 + 3 * n] = rx;
+    p.rave[dst][1 + 3 * n] = ry;
+    p.rave[dst][2 + 3 * n] = rz;
+    p.r2ave[dst][n] = q[3] / rnk;
+    const double dx = rx - r0[0], dy = ry - r0[1], dz = rz - r0[2];
+    const double dbd = bb[0] * dx * dx + bb[1] * dy * dy + bb[2] * dz * dz +
+                       2.0 * (bb[3] * dx * dy + bb[4] * dy * dz + bb[5] * dz * dx);
+    omd_n += q[4] + 2.0 * (dx * q[5] + dy * q[6] + dz * q[7]) + dbd;
+  }
+  // Omega_D summed over n in a fixed tree
+  const double omd = block_sum(omd_n, red);
+  if (threadIdx.x == 0) {
+    sc[0] = sums[8 * nw + 0];
+    sc[1] = sums[8 * nw + 1];
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+
+  Spread sp;
+  sp.om_od = sc[0] / rnk;
+  sp.om_d = omd / rnk;
+  if (mode == MODE_INIT && st->first_pass) {
+    sp.om_i = sc[1] / rnk;
+  } else {
+    sp.om_i = st->omega_invariant;
+  }
+  sp.om_tot = sp.om_i + sp.om_d + sp.om_od;
+
+  if (mode == MODE_INIT) {
+    st->omega_invariant = sp.om_i;
+    st->first_pass = 0;
+    st->wann = sp;
+    st->old.om_tot = 0.0;
+    if (p.log) {
+      p.log[0] = sp.om_tot;
+      p.log[1] = sp.om_d;
+      p.log[2] = sp.om_od;
+      p.log[3] = 0.0;
+    }
+    if (st->num_iter <= 0) st->done = 1;
+    return;
+  }
+  if (mode == MODE_REMIX) {
+    // FP64 state from the rebuilt M; the FP64 loop restarts its conjugate
+    // gradient and its convergence window from here
+    st->wann = sp;
+    st->cur = dst;
+    st->phase = 1;
+    st->hold = 0;
+    st->ncg = st->num_cg_steps;
+    st->conv_count = 0;
+    for (int j = 0; j < W90GPU_MAX_CONV_WINDOW; ++j) st->history[j] = 0.0;
+    if (p.log) {
+      double* row = p.log + 4 * (size_t)st->iter;
+      row[0] = sp.om_tot;
+      row[1] = sp.om_d;
+      row[2] = sp.om_od;
+    }
+    if (st->iter >= st->num_iter) st->done = 1;
+    return;
+  }
+  double* logrow = p.log ? p.log + 4 * (size_t)(st->iter + 1) : nullptr;
+  if (mode == MODE_TRIAL) {
+    st->trial = sp;
+    if (st->lfixstep) {
+      accept_step(st, sp, dst, logrow);
+      return;
+    }
+    // internal_optimal_step
+    const double trial_step = st->trial_step;
+    double fac = st->trial.om_tot - st->wann.om_tot;
+    double shift;
+    if (fabs(fac) > 2.2250738585072014e-308) {  // tiny(1.0_dp)
+      fac = 1.0 / fac;
+      shift = 1.0;
+    } else {
+      fac = 1.0e6;
+      shift = fac * st->trial.om_tot - fac * st->wann.om_tot;
+    }
+    const double eqb = fac * st->doda0;
+    const double eqa = shift - eqb * trial_step;
+    double alphamin, falphamin;
+    int lquad;
+    if (fabs(eqa / (fac * st->wann.om_tot)) > 2.220446049250313e-16) {  // epsilon(1.0_dp)
+      lquad = 1;
+      alphamin = -0.5 * eqb / eqa * (trial_step * trial_step);
+      falphamin = st->wann.om_tot - 0.25 * eqb * eqb / (fac * eqa) * (trial_step * trial_step);
+    } else {
+      lquad = 0;
+      alphamin = trial_step;
+      falphamin = st->trial.om_tot;
+    }
+    if (st->doda0 * alphamin > 0.0) {
+      lquad = 0;
+      alphamin = trial_step;
+      falphamin = st->trial.om_tot;
+    }
+    st->alphamin = alphamin;
+    st->falphamin = falphamin;
+    st->lquad = lquad;
+    if (lquad) {
+      st->second = 1;
+      st->scale = alphamin / (4.0 * st->wbtot);
+    } else {
+      accept_step(st, st->trial, dst, logrow);
+    }
+    return;
+  }
+  // MODE_SECOND: the parabolic step replaced the trial step
+  st->second = 0;
+  accept_step(st, sp, dst, logrow);
+}
+
+// ---------------------------------------------------------- gradient
+
+// One (m, n) term of the non-SLWF gradient of wann_domega
+// (wannierise.F90:2510-2528) for one block: A[R] - S[T] with
+// R_mn = M_mn conj(M_nn), Rt_mn = M_mn / M_nn, qn = wb Im ln M_nn and
+// rn = b.rave_n.
+__device__ __forceinline__ cplx domega_term(cplx acc, double w, cplx mmn, cplx mnm, cplx mnn, cplx mmm, double qn,
+                                            double qm, double rn, double rm) {
+  const cplx mhalf_i = c_make(0.0, -0.5);
+  const cplx cr_mn = c_mul(mmn, c_conj(mnn));
+  const cplx cr_nm = c_mul(mnm, c_conj(mmm));
+  const cplx crt_mn = c_div(mmn, mnn);
+  const cplx crt_nm = c_div(mnm, mmm);
+  // A[R] = (R - R^+)/2
+  acc = c_add(acc, c_scale(c_sub(cr_mn, c_conj(cr_nm)), w * 0.5));
+  // -S[T] = -(T + T^+)/2i, T_mn = Rt_mn q_n
+  acc = c_sub(acc, c_mul(c_add(c_scale(crt_mn, qn), c_conj(c_scale(crt_nm, qm))), mhalf_i));
+  acc = c_sub(acc, c_scale(c_mul(c_add(c_scale(crt_mn, rn), c_conj(c_scale(crt_nm, rm))), mhalf_i), w));
+  return acc;
+}
+
+// wann_domega (wannierise.F90:2287-2569), non-SLWF gradient, one
+// workgroup per (problem, k), fused with the per-k partial sums Re<G,G> and
+// Re<G,D_old> needed by internal_search_direction.
+template <class T2>
+__global__ void __launch_bounds__(W90_NT) k_domega(const Prob* probs, const Item* items, int nitems) {
+  if (blockIdx.x >= (unsigned)nitems) return;
+  const Item it = items[blockIdx.x];
+  const Prob& p = probs[it.prob];
+  const State* st = p.st;
+  if (st->done || st->hold) return;
+  const int nw = p.nw, nn = p.nn, nw2 = p.nw2;
+  const int k = it.k;
+  const int cur = st->cur;
+  const double* ln = p.ln[cur];
+  const double* rave = p.rave[cur];
+  const T2* Mk = mbuf<T2>(p, cur) + (size_t)k * nn * nw2;
+  __shared__ double red[W90_NT];
+  const double rnk = (double)p.nk;
+  double gg = 0.0, gd = 0.0;
+  for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
+    const int m = e % nw, n = e / nw;
+    cplx acc = c_make(0.0, 0.0);
+    for (int b = 0; b < nn; ++b) {
+      const T2* Mb = Mk + (size_t)b * nw2;
+      const double w = p.wb[b];
+      const double* bv = p.bk + ((size_t)k * nn + b) * 3;
+      const double* lnb = ln + ((size_t)k * nn + b) * nw;
+      const double rn = bv[0] * rave[0 + 3 * n] + bv[1] * rave[1 + 3 * n] + bv[2] * rave[2 + 3 * n];
+      const double rm = bv[0] * rave[0 + 3 * m] + bv[1] * rave[1 + 3 * m] + bv[2] * rave[2 + 3 * m];
+      acc = domega_term(acc, w, c_wide(Mb[m + n * nw]), c_wide(Mb[n + m * nw]), c_wide(Mb[n + n * nw]),
+                        c_wide(Mb[m + m * nw]), w * lnb[n], w * lnb[m], rn, rm);
+    }
+    const cplx g = c_scale(make_double2(acc.x / rnk, acc.y / rnk), 4.0);
+    p.G[(size_t)k * nw2 + e] = g;
+    gg += c_abs2(g);
+    const cplx dold = p.D[(size_t)k * nw2 + e];
+    gd += g.x * dold.x + g.y * dold.y;
+  }
+  gg = block_sum(gg, red);
+  gd = block_sum(gd, red);
+  if (threadIdx.x == 0) {
+    p.part_gg[k] = gg;
+    p.part_gd[k] = gd;
+  }
+}
+
+// wann_domega with each block M(k,b) staged in LDS (num_wann <= NWM): the
+// transposed and diagonal reads of the gradient formula hit LDS (padded
+// leading dimension), and the scaled phases wb Im ln M_nn and b.rave_n of
+// the k are precomputed.
+template <int NWM, class T2>
+__global__ void __launch_bounds__(W90_NT) k_domega_k(const Prob* probs, const Item* items, int nitems) {
+  if (blockIdx.x >= (unsigned)nitems) return;
+  const Item it = items[blockIdx.x];
+  const Prob& p = probs[it.prob];
+  const State* st = p.st;
+  if (st->done || st->hold) return;
+  constexpr int SL = (NWM * NWM + W90_NT - 1) / W90_NT;
+  const int nw = p.nw, nn = p.nn, nw2 = p.nw2;
+  const int ld = nw + 1;
+  const int k = it.k;
+  const int tid = threadIdx.x;
+  const int cur = st->cur;
+  __shared__ T2 Ms[NWM * (NWM + 1)];
+  __shared__ double q[W90GPU_NUM_NNMAX * NWM];
+  __shared__ double rb[W90GPU_NUM_NNMAX * NWM];
+  __shared__ double red[W90_NT];
+  const double* rave = p.rave[cur];
+  for (int t = tid; t < nn * nw; t += W90_NT) {
+    const int b = t / nw, n = t % nw;
+    const size_t kb = (size_t)k * nn + b;
+    const double* bv = p.bk + kb * 3;
+    q[t] = p.wb[b] * p.ln[cur][kb * nw + n];
+    rb[t] = bv[0] * rave[0 + 3 * n] + bv[1] * rave[1 + 3 * n] + bv[2] * rave[2 + 3 * n];
+  }
+  cplx acc[SL];
+#pragma unroll
+  for (int s = 0; s < SL; ++s) acc[s] = c_make(0.0, 0.0);
+  const T2* Mk = mbuf<T2>(p, cur) + (size_t)k * nn * nw2;
+  for (int b = 0; b < nn; ++b) {
+    const T2* src = Mk + (size_t)b * nw2;
+    __syncthreads();
+    for (int e = tid; e < nw2; e += W90_NT) Ms[(e % nw) + (e / nw) * ld] = src[e];
+    __syncthreads();
+    const double w = p.wb[b];
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int e = tid + s * W90_NT;
+      if (e < nw2) {
+        const int m = e % nw, n = e / nw;
+        acc[s] = domega_term(acc[s], w, c_wide(Ms[m + n * ld]), c_wide(Ms[n + m * ld]), c_wide(Ms[n + n * ld]),
+                             c_wide(Ms[m + m * ld]), q[b * nw + n], q[b * nw + m], rb[b * nw + n],
+                             rb[b * nw + m]);
+      }
+    }
+  }
+  const double rnk = (double)p.nk;
+  double gg = 0.0, gd = 0.0;
+#pragma unroll
+  for (int s = 0; s < SL; ++s) {
+    const int e = tid + s * W90_NT;
+    if (e < nw2) {
+      const cplx g = c_scale(make_double2(acc[s].x / rnk, acc[s].y / rnk), 4.0);
+      p.G[(size_t)k * nw2 + e] = g;
+      gg += c_abs2(g);
+      const cplx dold = p.D[(size_t)k * nw2 + e];
+      gd += g.x * dold.x + g.y * dold.y;
+    }
+  }
+  gg = block_sum(gg, red);
+  gd = block_sum(gd, red);
+  if (tid == 0) {
+    p.part_gg[k] = gg;
+    p.part_gd[k] = gd;
+  }
+}
+
+// internal_search_direction (wannierise.F90:1371-1525) without
+// preconditioning or noise, one workgroup per problem: Fletcher-Reeves
+// coefficient with its resets, slope along D and the uphill handling.  D
+// itself is formed by the step kernel.
+__global__ void __launch_bounds__(W90_NT) k_search(const Prob* probs, const int* plist, int np) {
+  if (blockIdx.x >= (unsigned)np) return;
+  const Prob& p = probs[plist[blockIdx.x]];
+  State* st = p.st;
+  if (st->done || st->hold) return;
+  __shared__ double red[W90_NT];
+  double gg = 0.0, gd = 0.0;
+  // fixed-order chunked sums over k
+  const int per = (p.nk + blockDim.x - 1) / blockDim.x;
+  const int kb = threadIdx.x * per, ke = min(p.nk, kb + per);
+  for (int k = kb; k < ke; ++k) {
+    gg += p.part_gg[k];
+    gd += p.part_gd[k];
+  }
+  gg = block_sum(gg, red);
+  gd = block_sum(gd, red);
+  if (threadIdx.x != 0) return;
+  const int iter = st->iter + 1;
+  st->second = 0;
+  st->gcnorm1 = gg;
+  double gcfac;
+  if (iter == 1 || st->ncg >= st->num_cg_steps) {
+    gcfac = 0.0;
+    st->ncg = 0;
+  } else {
+    if (st->gcnorm0 > 2.220446049250313e-16) {
+      gcfac = gg / st->gcnorm0;
+      if (gcfac > 3.0) {
+        gcfac = 0.0;
+        st->ncg = 0;
+      } else {
+        st->ncg += 1;
+      }
+    } else {
+      gcfac = 0.0;
+      st->ncg = 0;
+    }
+  }
+  st->gcnorm0 = gg;
+  const double w4 = 4.0 * st->wbtot;
+  double doda0 = -(gg + gcfac * gd) / w4;
+  double beta = gcfac, sign = 1.0;
+  if (doda0 > 0.0) {
+    if (st->ncg > 0) {
+      beta = 0.0;
+      st->ncg = 0;
+      gcfac = 0.0;
+      doda0 = -gg / w4;
+      if (doda0 > 0.0) {
+        sign = -1.0;
+        doda0 = -doda0;
+      }
+    } else {
+      sign = -sign;
+      doda0 = -doda0;
+    }
+  }
+  st->gcfac = gcfac;
+  st->beta = beta;
+  st->dir_sign = sign;
+  st->doda0 = doda0;
+  st->scale = (st->lfixstep ? st->fixed_step : st->trial_step) / w4;
+}
+
+// ---------------------------------------------------------- retraction
+
+// Degree and squarings of the Taylor approximant of exp(X) for ||X||_1 = nrm:
+// X is scaled by 2^-sq to norm <= 1/8, and the degree m is the smallest with
+// (nrm/2^sq)^(m+1)/(m+1)! < 1e-18, far below the FP64 rounding of the
+// result (the exponential of an anti-Hermitian matrix is unitary, norm 1).
+__device__ __forceinline__ void taylor_plan(double nrm, int* sq_out, int* m_out) {
+  if (!isfinite(nrm)) nrm = 0.0;  // a non-finite step yields a non-finite U
+  int sq = 0;
+  while (nrm > 0.125 && sq < 64) {
+    nrm *= 0.5;
+    ++sq;
+  }
+  int m = 1;
+  double term = nrm;
+  while (m < 24) {
+    term = term * nrm / (double)(m + 1);
+    if (term < 1e-18) break;
+    ++m;
+  }
+  *sq_out = sq;
+  *m_out = m;
+}
+
+// C = A * B for n x n column-major complex matrices held by one workgroup.
+__device__ void wg_gemm(cplx* C, const cplx* A, const cplx* B, int n) {
+  const int n2 = n * n;
+  for (int e = threadIdx.x; e < n2; e += blockDim.x) {
+    const int i = e % n, j = e / n;
+    const cplx* bc = B + (size_t)j * n;
+    cplx acc = c_make(0.0, 0.0);
+    for (int l = 0; l < n; ++l) acc = c_add(acc, c_mul(A[i + (size_t)l * n], bc[l]));
+    C[e] = acc;
+  }
+  __syncthreads();
+}
+
+// C = A^+ B for n x n column-major complex matrices held by one workgroup.
+__device__ void wg_gemm_cn(cplx* C, const cplx* A, const cplx* B, int n) {
+  const int n2 = n * n;
+  for (int e = threadIdx.x; e < n2; e += blockDim.x) {
+    const int i = e % n, j = e / n;
+    const cplx* ac = A + (size_t)i * n;
+    const cplx* bc = B + (size_t)j * n;
+    cplx acc = c_make(0.0, 0.0);
+    for (int l = 0; l < n; ++l) acc = c_add(acc, c_cmul(ac[l], bc[l]));
+    C[e] = acc;
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(W90_NT) k_delta_u(const Prob* probs, const Item* items, int nitems) {
+  if (blockIdx.x >= (unsigned)nitems) return;
+  const Item it = items[blockIdx.x];
+  const Prob& p = probs[it.prob];
+  const State* st = p.st;
+  if (!gate(st, MODE_REMIX)) return;
+  const size_t o = (size_t)it.k * p.nw2;
+  wg_gemm_cn(p.E + o, p.U0 + o, p.U[st->cur] + o, p.nw);
+}
+
+// Retraction of one (problem, k) with workspaces in global memory (any
+// num_wann): forms the search direction D (trial step only), X = scale * D,
+// E = exp(X) by scaling and squaring of a Taylor polynomial, and U_new = U E.
+// X is anti-Hermitian, so exp(X) is the same unitary that
+// internal_new_u_and_m builds from the eigenvectors of i X
+// (wannierise.F90:1662-1689, 1729-1734).
+__global__ void __launch_bounds__(W90_NT) k_step(const Prob* probs, const Item* items, int nitems, int mode) {
+  if (blockIdx.x >= (unsigned)nitems) return;
+  const Item it = items[blockIdx.x];
+  const Prob& p = probs[it.prob];
+  const State* st = p.st;
+  if (!gate(st, mode)) return;
+  const int nw = p.nw, nw2 = p.nw2;
+  const int k = it.k;
+  const int cur = st->cur;
+  __shared__ double red[W90_NT];
+  cplx* Dk = p.D + (size_t)k * nw2;
+  cplx* X = p.W1 + (size_t)k * nw2;
+  cplx* P = p.W2 + (size_t)k * nw2;
+  cplx* Ek = p.E + (size_t)k * nw2;
+  const double scale = st->scale;
+  if (mode == MODE_TRIAL) {
+    const cplx* Gk = p.G + (size_t)k * nw2;
+    const double beta = st->beta, sign = st->dir_sign;
+    for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
+      cplx d = c_scale(c_add(Gk[e], c_scale(Dk[e], beta)), sign);
+      Dk[e] = d;
+      X[e] = c_scale(d, scale);
+    }
+  } else {
+    for (int e = threadIdx.x; e < nw2; e += blockDim.x) X[e] = c_scale(Dk[e], scale);
+  }
+  __syncthreads();
+  double cs = 0.0;
+  for (int j = threadIdx.x; j < nw; j += blockDim.x) {
+    double s = 0.0;
+    for (int i = 0; i < nw; ++i) s += sqrt(c_abs2(X[i + (size_t)j * nw]));
+    cs = fmax(cs, s);
+  }
+  int sq, m;
+  taylor_plan(block_max(cs, red), &sq, &m);
+  if (sq > 0) {
+    const double f = ldexp(1.0, -sq);
+    for (int e = threadIdx.x; e < nw2; e += blockDim.x) X[e] = c_scale(X[e], f);
+  }
+  __syncthreads();
+  const double invm = 1.0 / (double)m;
+  for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
+    cplx v = c_scale(X[e], invm);
+    if (e % nw == e / nw) v.x += 1.0;
+    P[e] = v;
+  }
+  __syncthreads();
+  for (int j = m - 1; j >= 1; --j) {
+    wg_gemm(Ek, X, P, nw);
+    const double invj = 1.0 / (double)j;
+    for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
+      cplx v = c_scale(Ek[e], invj);
+      if (e % nw == e / nw) v.x += 1.0;
+      P[e] = v;
+    }
+    __syncthreads();
+  }
+  cplx* a = P;
+  cplx* b = X;
+  for (int s = 0; s < sq; ++s) {
+    wg_gemm(b, a, a, nw);
+    cplx* t = a;
+    a = b;
+    b = t;
+  }
+  for (int e = threadIdx.x; e < nw2; e += blockDim.x) Ek[e] = a[e];
+  __syncthreads();
+  wg_gemm(p.U[1 - cur] + (size_t)k * nw2, p.U[cur] + (size_t)k * nw2, Ek, nw);
+}
+
+// Retraction of one (problem, k) with every matrix in LDS (num_wann <= NWM):
+// forms D (trial step), X = scale D, E = exp(X) and U_new = U E.  Padded
+// column-major LDS matrices (leading dimension nw + 1).
+template <int NWM>
+__global__ void __launch_bounds__(W90_NT) k_step_k(const Prob* probs, const Item* items, int nitems, int mode) {
+  if (blockIdx.x >= (unsigned)nitems) return;
+  const Item it = items[blockIdx.x];
+  const Prob& p = probs[it.prob];
+  const State* st = p.st;
+  if (!gate(st, mode)) return;
+  const int nw = p.nw, nw2 = p.nw2;
+  const int ld = nw + 1;
+  const int k = it.k;
+  const int tid = threadIdx.x;
+  const int cur = st->cur;
+  __shared__ cplx X[NWM * (NWM + 1)];
+  __shared__ cplx P[NWM * (NWM + 1)];
+  __shared__ cplx T[NWM * (NWM + 1)];
+  __shared__ double red[W90_NT];
+  cplx* Dk = p.D + (size_t)k * nw2;
+  const double scale = st->scale;
+  if (mode == MODE_TRIAL) {
+    const cplx* Gk = p.G + (size_t)k * nw2;
+    const double beta = st->beta, sign = st->dir_sign;
+    for (int e = tid; e < nw2; e += W90_NT) {
+      cplx d = c_scale(c_add(Gk[e], c_scale(Dk[e], beta)), sign);
+      Dk[e] = d;
+      X[(e % nw) + (e / nw) * ld] = c_scale(d, scale);
+    }
+  } else {
+    for (int e = tid; e < nw2; e += W90_NT) X[(e % nw) + (e / nw) * ld] = c_scale(Dk[e], scale);
+  }
+  __syncthreads();
+  double cs = 0.0;
+  for (int j = tid; j < nw; j += W90_NT) {
+    double s = 0.0;
+    for (int i = 0; i < nw; ++i) s += sqrt(c_abs2(X[i + j * ld]));
+    cs = fmax(cs, s);
+  }
+  int sq, m;
+  taylor_plan(block_max(cs, red), &sq, &m);
+  const double f = ldexp(1.0, -sq);
+  const double invm = 1.0 / (double)m;
+  for (int e = tid; e < nw2; e += W90_NT) {
+    const int q = (e % nw) + (e / nw) * ld;
+    const cplx x = c_scale(X[q], f);
+    X[q] = x;
+    cplx v = c_scale(x, invm);
+    if (e % nw == e / nw) v.x += 1.0;
+    P[q] = v;
+  }
+  __syncthreads();
+  for (int j = m - 1; j >= 1; --j) {
+    lds_gemm<NWM>(T, X, P, nw);
+    const double invj = 1.0 / (double)j;
+    for (int e = tid; e < nw2; e += W90_NT) {
+      const int q = (e % nw) + (e / nw) * ld;
+      cplx v = c_scale(T[q], invj);
+      if (e % nw == e / nw) v.x += 1.0;
+      P[q] = v;
+    }
+    __syncthreads();
+  }
+  cplx* a = P;
+  cplx* b = T;
+  for (int s = 0; s < sq; ++s) {
+    lds_gemm<NWM>(b, a, a, nw);
+    cplx* t = a;
+    a = b;
+    b = t;
+  }
+  cplx* Ek = p.E + (size_t)k * nw2;
+  const cplx* Uc = p.U[cur] + (size_t)k * nw2;
+  for (int e = tid; e < nw2; e += W90_NT) {
+    const int q = (e % nw) + (e / nw) * ld;
+    Ek[e] = a[q];
+    X[q] = Uc[e];
+  }
+  __syncthreads();
+  lds_gemm<NWM>(b, X, a, nw);
+  cplx* Un = p.U[1 - cur] + (size_t)k * nw2;
+  for (int e = tid; e < nw2; e += W90_NT) Un[e] = b[(e % nw) + (e / nw) * ld];
+}
+
+// ---------------------------------------------------------- projection
+
+// Lowdin orthonormalisation of overlap_project (src/overlap.F90:758-952)
+// for num_bands = num_wann: the unitary polar factor U = A (A^+ A)^-1/2 of
+// A(k) (staged in G), which is the Z V^+ built there from the SVD
+// A = Z S V^+.  Newton-Schulz iteration X <- X (3 - X^+ X)/2 from
+// X = A/||A||_F (all singular values in (0,1]) converges to it for
+// non-singular A, quadratically near the end: once X^+ X = 1 to 1e-12 one
+// more step reaches rounding level and the iteration stops (at most 200
+// steps).  Writes U(k) to E.
+__global__ void __launch_bounds__(W90_NT) k_polar(const Prob* probs, const Item* items, int nitems) {
+  if (blockIdx.x >= (unsigned)nitems) return;
+  const Item it = items[blockIdx.x];
+  const Prob& p = probs[it.prob];
+  const int nw = p.nw, nw2 = p.nw2;
+  const int k = it.k;
+  cplx* X = p.E + (size_t)k * nw2;
+  cplx* Y = p.W1 + (size_t)k * nw2;
+  cplx* Z = p.W2 + (size_t)k * nw2;
+  __shared__ double red[W90_NT];
+  const cplx* Ak = p.G + (size_t)k * nw2;
+  double fro = 0.0;
+  for (int e = threadIdx.x; e < nw2; e += blockDim.x) fro += c_abs2(Ak[e]);
+  fro = sqrt(block_sum(fro, red));
+  const double inv = (fro > 0.0 && isfinite(fro)) ? 1.0 / fro : 0.0;
+  for (int e = threadIdx.x; e < nw2; e += blockDim.x) X[e] = c_scale(Ak[e], inv);
+  __syncthreads();
+  for (int iter = 0; iter < 200; ++iter) {
+    wg_gemm_cn(Y, X, X, nw);
+    double dev = 0.0;
+    for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
+      cplx y = Y[e];
+      if (e % nw == e / nw) y.x -= 1.0;
+      dev = fmax(dev, sqrt(c_abs2(y)));
+      // 3 - X^+ X, halved below
+      Y[e] = c_make(-Y[e].x + ((e % nw == e / nw) ? 3.0 : 0.0), -Y[e].y);
+    }
+    dev = block_max(dev, red);
+    if (!(dev == dev)) break;  // non-finite input: left to the unitarity check
+    wg_gemm(Z, X, Y, nw);
+    for (int e = threadIdx.x; e < nw2; e += blockDim.x) X[e] = c_scale(Z[e], 0.5);
+    __syncthreads();
+    if (dev < 1e-12) break;
+  }
+}
+
+// wann_check_unitarity (wannierise.F90:2594-2661): per-k maximum deviation
+// of U U^+ and U^+ U from the identity (NaN-propagating).
+__global__ void __launch_bounds__(W90_NT) k_unitarity(const Prob* probs, const Item* items, int nitems) {
+  if (blockIdx.x >= (unsigned)nitems) return;
+  const Item it = items[blockIdx.x];
+  const Prob& p = probs[it.prob];
+  const int nw = p.nw, nw2 = p.nw2;
+  const int k = it.k;
+  const cplx* U = p.U[p.st->cur] + (size_t)k * nw2;
+  __shared__ double red[W90_NT];
+  double dev = 0.0;
+  for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
+    const int i = e % nw, j = e / nw;
+    cplx c1 = c_make(0, 0), c2 = c_make(0, 0);
+    for (int m = 0; m < nw; ++m) {
+      c1 = c_add(c1, c_mul(U[i + m * nw], c_conj(U[j + m * nw])));
+      c2 = c_add(c2, c_mul(U[m + j * nw], c_conj(U[m + i * nw])));
+    }
+    if (i == j) {
+      c1.x -= 1.0;
+      c2.x -= 1.0;
+    }
+    const double v = fmax(sqrt(c_abs2(c1)), sqrt(c_abs2(c2)));
+    if (!(v <= dev)) dev = isnan(v) ? INFINITY : fmax(v, dev);
+    if (isnan(c1.x) || isnan(c1.y) || isnan(c2.x) || isnan(c2.y)) dev = INFINITY;
+  }
+  dev = block_max(dev, red);
+  if (threadIdx.x == 0) p.unit[k] = dev;
+}

@@ -56,6 +56,8 @@ module w90gpu_c
     integer(c_int) :: converged
     integer(c_int) :: unitarity_ok
     integer(c_int) :: status
+    integer(c_int) :: bulk_iterations
+    integer(c_int) :: reserved
   end type w90gpu_result
 
   interface

@@ -1,5 +1,5 @@
-﻿!mod$ v1 sum:c03ee65961c6ae93
-!need$ 0f11ed11b95c8d68 n w90gpu_c
+﻿!mod$ v1 sum:e34ef4e13c24167b
+!need$ ad019ea2035d982d n w90gpu_c
 !need$ 7d55b91d8c4c39ed n w90gpu_io
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ bb381bf46e508468 i __fortran_builtins

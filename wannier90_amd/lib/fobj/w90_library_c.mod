@@ -1,5 +1,5 @@
-﻿!mod$ v1 sum:6db4cfe1dffa7038
-!need$ c03ee65961c6ae93 n w90_library
+﻿!mod$ v1 sum:b92451e51e165643
+!need$ e34ef4e13c24167b n w90_library
 !need$ 0bde2ac47243ead2 i iso_c_binding
 module w90_library_c
 use,intrinsic::iso_c_binding,only:c_associated

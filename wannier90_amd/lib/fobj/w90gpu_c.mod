@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:0f11ed11b95c8d68
+﻿!mod$ v1 sum:ad019ea2035d982d
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ bb381bf46e508468 i __fortran_builtins
 module w90gpu_c
@@ -123,6 +123,8 @@ integer(4)::iterations
 integer(4)::converged
 integer(4)::unitarity_ok
 integer(4)::status
+integer(4)::bulk_iterations
+integer(4)::reserved
 end type
 interface
 subroutine w90gpu_control_defaults(ctl) bind(c,name="w90gpu_control_defaults")
