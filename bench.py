@@ -1,19 +1,24 @@
 """Benchmark of the MI355X wannierisation path.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
-                    [--no-cpu-baseline]
+                    [--no-cpu-baseline] [--workload single|batch|serve]
 
-Workload: the benchmark configuration of the metric, a seeded fcc
+single (default): the benchmark configuration of the metric, a seeded fcc
 tight-binding problem (a = 5.43 Ang, num_wann = num_bands = 16, decay 2 Ang,
 8x8x8 MP grid = 512 k, nntot = 8), solved with num_iter = 500,
 conv_tol = 1e-10, conv_window = 3, trial_step = 2, num_cg_steps = 5.
 One step = one complete minimisation (wann_main) from the Lowdin-projected
-initial state, with M and U resident in HBM before the timed region.
+initial state, with M and U resident in HBM before the timed region.  The
+same problem is then solved in mixed precision and its Omega and centres
+compared with the FP64 solve.
 
-Prints one JSON line (value = solves/s on one GPU) with the roofline of the
-dominant kernel (the M rotation, timed with HIP events on its own stream)
-and the CPU baseline (the reference wannier90.x on a bounded sample, timed
-on this host).
+Prints one JSON line (value = FP64 solves/s on one GPU) with the roofline
+of the dominant kernel (the M rotation, timed with HIP events on its own
+stream) and the CPU baseline (the reference wannier90.x on a bounded sample,
+timed on this host).
+
+batch and serve (wannier90_amd/workloads.py): the ragged batch of 256
+problems and the open-loop serving loop.
 """
 import argparse
 import json
@@ -43,6 +48,12 @@ def parse():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-iters", type=int, default=100)
+    ap.add_argument("--workload", choices=["single", "batch", "serve"], default="single")
+    ap.add_argument("--problems", type=int, default=256, help="batch: number of problems")
+    ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second (0: from a probe)")
+    ap.add_argument("--num-iter", type=int, default=500)
+    ap.add_argument("--pool-gb", type=float, default=160.0, help="serve: device pool for admission")
     return ap.parse_args()
 
 
@@ -97,12 +108,46 @@ def cpu_baseline(prob, params, sample_iters):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def mixed_comparison(ctx, prob, m0, u0, sess_fp64, r64, args):
+    """The same problem in mixed precision: solves/s, and Omega / centres
+    against the FP64 solve of the same inputs (sess_fp64 holds its state)."""
+    from wannier90_amd import capi
+
+    _, _, cen64, _ = sess_fp64.fetch(want_m=False)
+    ctl = capi.control(num_iter=500, conv_tol=1e-10, conv_window=3, trial_step=2.0, num_cg_steps=5,
+                       precision=capi.PREC_MIXED)
+    s = ctx.session(prob.kmesh, prob.num_wann, prob.num_kpts, ctl)
+    s.load(m0, u0)
+    for _ in range(args.warmup):
+        s.solve()
+    t0 = time.perf_counter()
+    res = [s.solve() for _ in range(args.steps)]
+    dt = time.perf_counter() - t0
+    _, _, cen, _ = s.fetch(want_m=False)
+    s.close()
+    r = res[-1]
+    d_om = abs(r.omega_total - r64.omega_total)
+    d_c = float(np.abs(cen - cen64).max())
+    log(f"[bench] mixed: {args.steps / dt:.2f} solves/s, bulk {r.bulk_iterations}/{r.iterations} iterations, "
+        f"dOmega {d_om:.2e}, dcentres {d_c:.2e}")
+    return dict(value=args.steps / dt, unit="solves/s", ms_per_step=1e3 * dt / args.steps,
+                iterations=r.iterations, bulk_iterations=r.bulk_iterations, omega_total=r.omega_total,
+                delta_omega_vs_fp64=d_om, delta_centres_vs_fp64=d_c,
+                within_1e8=bool(d_om <= 1e-8 and d_c <= 1e-6))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(batched and serving workloads shard as whole problems per GPU)", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "single":
+        from wannier90_amd import workloads
+
+        out = workloads.run(args)
+        print(json.dumps(out), flush=True)
+        return
     from wannier90_amd import capi, synth
 
     t = time.perf_counter()
@@ -181,6 +226,8 @@ def main():
         },
         "roofline": roof,
     }
+    if prec == capi.PREC_FP64:
+        out["mixed"] = mixed_comparison(ctx, prob, m0, u0, sess, results[-1], args)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(prob, params, args.cpu_sample_iters)
         per_solve = cb["seconds"] * its / max(cb["iters"], 1)

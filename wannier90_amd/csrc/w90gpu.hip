@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "w90gpu_kernels.h"
+#include "w90gpu_mfma.h"
 
 struct w90gpu_ctx {
   int device = 0;
@@ -63,14 +64,17 @@ struct Member {
 };
 
 // kernel classes by num_wann: b-parallel LDS rotation up to 8 and 16, the
-// per-k LDS kernels up to 32, the panel kernels above
-constexpr int kClasses = 4;
+// per-k LDS kernels up to 32, the MFMA rotation with the MFMA retraction
+// and LDS gradient up to 64, the MFMA rotation with the global-memory
+// retraction and gradient up to W90GPU_MAX_WANN
+constexpr int kClasses = 5;
 constexpr int kLdsMaxWann = 32;
 // iterations per captured block; the solve is polled once per block
 constexpr int kGraphBlock = 8;
 
-int wann_class(int nw) { return nw <= 8 ? 0 : nw <= 16 ? 1 : nw <= kLdsMaxWann ? 2 : 3; }
-int npanels(int nw) { return (nw + W90_PANEL - 1) / W90_PANEL; }
+int wann_class(int nw) { return nw <= 8 ? 0 : nw <= 16 ? 1 : nw <= kLdsMaxWann ? 2 : nw <= W90_STEP_N ? 3 : 4; }
+// per-(k,b) partial sums of the rotation: one per MFMA tile
+int npanels(int nw) { return wann_class(nw) >= 3 ? gemm_tiles(nw) : 1; }
 
 // A contiguous range of an item list with the launch shape its kernels need.
 struct Seg {
@@ -238,6 +242,7 @@ size_t problem_layout(int nw, int nk, int nn, int num_iter, Layout& lay) {
   lay.add(lnb);  // ln[0]
   lay.add(lnb);  // ln[1]
   lay.add(lnb);  // d2
+  lay.add(wann_class(nw) >= 3 ? mb : 256);                  // Tm (MFMA rotation scratch)
   lay.add((size_t)nk * nn * npanels(nw) * sizeof(double));  // part_od
   lay.add((size_t)nk * nn * npanels(nw) * sizeof(double));  // part_all
   lay.add((size_t)nk * psz * sizeof(double));               // kpart
@@ -466,6 +471,7 @@ void member_bind(Member* m, char* a, const Layout& lay) {
   p.ln[0] = (double*)(a + lay.off[i++]);
   p.ln[1] = (double*)(a + lay.off[i++]);
   p.d2 = (double*)(a + lay.off[i++]);
+  p.Tm = (void*)(a + lay.off[i++]);
   p.part_od = (double*)(a + lay.off[i++]);
   p.part_all = (double*)(a + lay.off[i++]);
   p.kpart = (double*)(a + lay.off[i++]);
@@ -677,6 +683,15 @@ int prof_begin(Engine* E, hipEvent_t* b) {
   return W90GPU_OK;
 }
 
+// A range of whole classes [c0, c1] of the item list (both precisions).
+Seg class_range(const Lists& L, int c0, int c1) {
+  Seg r;
+  r.off = L.seg[c0][0].off;
+  const Seg& last = L.seg[c1][1];
+  r.n = last.off + last.n - r.off;
+  return r;
+}
+
 // The rotation of M and the reduction of its diagonal into the per-k
 // partial sums of wann_omega, over every class and precision of the lists.
 int launch_rotate(Engine* E, const Lists& L, int mode) {
@@ -713,20 +728,26 @@ int launch_rotate(Engine* E, const Lists& L, int mode) {
         else
           hipLaunchKernelGGL((k_rotate_k<kLdsMaxWann, float2>), grid, dim3(W90_NT), 0, s, P, it, g.n, mode);
       } else {
+        // MFMA GEMM tiles; gridDim.y covers b and the tiles of a block
         const dim3 g2(g.n, g.nnmax * g.npanmax);
-        if (q == 0)
-          hipLaunchKernelGGL(k_rotate<cplx>, g2, dim3(W90_NT), 0, s, P, it, g.n, mode);
-        else
-          hipLaunchKernelGGL(k_rotate<float2>, g2, dim3(W90_NT), 0, s, P, it, g.n, mode);
+        if (mode == MODE_INIT) {
+          if (q == 0) hipLaunchKernelGGL(k_reduce_tiles<cplx>, g2, dim3(256), 0, s, P, it, g.n, mode);
+          else hipLaunchKernelGGL(k_reduce_tiles<float2>, g2, dim3(256), 0, s, P, it, g.n, mode);
+        } else if (q == 0) {
+          hipLaunchKernelGGL((k_rot_gemm<cplx, 1>), g2, dim3(256), 0, s, P, it, g.n, mode);
+          hipLaunchKernelGGL((k_rot_gemm<cplx, 2>), g2, dim3(256), 0, s, P, it, g.n, mode);
+        } else {
+          hipLaunchKernelGGL((k_rot_gemm<float2, 1>), g2, dim3(256), 0, s, P, it, g.n, mode);
+          hipLaunchKernelGGL((k_rot_gemm<float2, 2>), g2, dim3(256), 0, s, P, it, g.n, mode);
+        }
       }
       HIPCHK(E->ctx, hipGetLastError());
     }
   if (pe) HIPCHK(E->ctx, hipEventRecord(pe, s));
-  // per-k partial vectors of the panel class
-  for (int q = 0; q < 2; ++q) {
-    const Seg& g = L.seg[3][q];
-    if (g.n == 0) continue;
-    hipLaunchKernelGGL(k_kpart, dim3(g.n), dim3(W90_NT), 0, s, P, L.items + g.off, g.n, mode);
+  // per-k partial vectors of the MFMA classes
+  const Seg gk = class_range(L, 3, kClasses - 1);
+  if (gk.n) {
+    hipLaunchKernelGGL(k_kpart, dim3(gk.n), dim3(W90_NT), 0, s, P, L.items + gk.off, gk.n, mode);
     HIPCHK(E->ctx, hipGetLastError());
   }
   return W90GPU_OK;
@@ -743,24 +764,19 @@ int launch_omega(Engine* E, const Lists& L, int mode) {
   return W90GPU_OK;
 }
 
-// A range of whole classes [c0, c1] of the item list (both precisions).
-Seg class_range(const Lists& L, int c0, int c1) {
-  Seg r;
-  r.off = L.seg[c0][0].off;
-  const Seg& last = L.seg[c1][1];
-  r.n = last.off + last.n - r.off;
-  return r;
-}
-
 int launch_step(Engine* E, const Lists& L, int mode) {
   hipStream_t s = E->stream;
-  const Seg a = class_range(L, 0, 1), b = class_range(L, 2, 2), c = class_range(L, 3, 3);
+  const Seg a = class_range(L, 0, 1), b = class_range(L, 2, 2), c = class_range(L, 3, 3),
+            d = class_range(L, 4, 4);
   if (a.n)
     hipLaunchKernelGGL(k_step_k<16>, dim3(a.n), dim3(W90_NT), 0, s, E->d_probs, L.items + a.off, a.n, mode);
   if (b.n)
     hipLaunchKernelGGL(k_step_k<kLdsMaxWann>, dim3(b.n), dim3(W90_NT), 0, s, E->d_probs, L.items + b.off, b.n,
                        mode);
-  if (c.n) hipLaunchKernelGGL(k_step, dim3(c.n), dim3(W90_NT), 0, s, E->d_probs, L.items + c.off, c.n, mode);
+  if (c.n)
+    hipLaunchKernelGGL(k_step_mfma, dim3(c.n), dim3(256), step_mfma_lds(), s, E->d_probs, L.items + c.off, c.n,
+                       mode);
+  if (d.n) hipLaunchKernelGGL(k_step, dim3(d.n), dim3(W90_NT), 0, s, E->d_probs, L.items + d.off, d.n, mode);
   HIPCHK(E->ctx, hipGetLastError());
   return W90GPU_OK;
 }
@@ -774,14 +790,29 @@ int launch_domega(Engine* E, const Lists& L) {
       if (g.n == 0) continue;
       const Item* it = L.items + g.off;
       if (c <= 1) {
-        if (q == 0) hipLaunchKernelGGL((k_domega_k<16, cplx>), dim3(g.n), dim3(W90_NT), 0, s, P, it, g.n);
-        else hipLaunchKernelGGL((k_domega_k<16, float2>), dim3(g.n), dim3(W90_NT), 0, s, P, it, g.n);
+        if (q == 0)
+          hipLaunchKernelGGL((k_domega_k<16, cplx>), dim3(g.n), dim3(domega_nt<16>()), (domega_lds<16, cplx>()), s, P, it,
+                             g.n);
+        else
+          hipLaunchKernelGGL((k_domega_k<16, float2>), dim3(g.n), dim3(domega_nt<16>()), (domega_lds<16, float2>()), s, P,
+                             it, g.n);
       } else if (c == 2) {
-        if (q == 0) hipLaunchKernelGGL((k_domega_k<kLdsMaxWann, cplx>), dim3(g.n), dim3(W90_NT), 0, s, P, it, g.n);
-        else hipLaunchKernelGGL((k_domega_k<kLdsMaxWann, float2>), dim3(g.n), dim3(W90_NT), 0, s, P, it, g.n);
+        if (q == 0)
+          hipLaunchKernelGGL((k_domega_k<kLdsMaxWann, cplx>), dim3(g.n), dim3(domega_nt<kLdsMaxWann>()),
+                             (domega_lds<kLdsMaxWann, cplx>()), s, P, it, g.n);
+        else
+          hipLaunchKernelGGL((k_domega_k<kLdsMaxWann, float2>), dim3(g.n), dim3(domega_nt<kLdsMaxWann>()),
+                             (domega_lds<kLdsMaxWann, float2>()), s, P, it, g.n);
+      } else if (c == 3) {
+        if (q == 0)
+          hipLaunchKernelGGL((k_domega_k<W90_STEP_N, cplx>), dim3(g.n), dim3(domega_nt<W90_STEP_N>()),
+                             (domega_lds<W90_STEP_N, cplx>()), s, P, it, g.n);
+        else
+          hipLaunchKernelGGL((k_domega_k<W90_STEP_N, float2>), dim3(g.n), dim3(domega_nt<W90_STEP_N>()),
+                             (domega_lds<W90_STEP_N, float2>()), s, P, it, g.n);
       } else {
-        if (q == 0) hipLaunchKernelGGL(k_domega<cplx>, dim3(g.n), dim3(W90_NT), 0, s, P, it, g.n);
-        else hipLaunchKernelGGL(k_domega<float2>, dim3(g.n), dim3(W90_NT), 0, s, P, it, g.n);
+        if (q == 0) hipLaunchKernelGGL(k_domega<cplx>, dim3(g.n), dim3(1024), 0, s, P, it, g.n);
+        else hipLaunchKernelGGL(k_domega<float2>, dim3(g.n), dim3(1024), 0, s, P, it, g.n);
       }
       HIPCHK(E->ctx, hipGetLastError());
     }
@@ -1141,6 +1172,9 @@ int w90gpu_create(int device, w90gpu_ctx** ctx) {
       {(const void*)k_rotate_kb<16, cplx>, rotate_kb_lds<16, cplx>(W90GPU_NUM_NNMAX)},
       {(const void*)k_rotate_kb<8, float2>, rotate_kb_lds<8, float2>(W90GPU_NUM_NNMAX)},
       {(const void*)k_rotate_kb<16, float2>, rotate_kb_lds<16, float2>(W90GPU_NUM_NNMAX)},
+      {(const void*)k_step_mfma, step_mfma_lds()},
+      {(const void*)k_domega_k<W90_STEP_N, cplx>, domega_lds<W90_STEP_N, cplx>()},
+      {(const void*)k_domega_k<W90_STEP_N, float2>, domega_lds<W90_STEP_N, float2>()},
   };
   for (const auto& l : lds) {
     if (hipFuncSetAttribute(l.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.bytes) != hipSuccess) {

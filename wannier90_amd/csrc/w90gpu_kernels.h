@@ -42,7 +42,6 @@
 typedef double2 cplx;
 
 #define W90_NT 256          // threads per workgroup of the per-k kernels
-#define W90_PANEL 16        // output rows per panel-rotation workgroup
 #define W90_FINAL_NT 1024   // threads of the per-problem omega kernel
 #define W90_MAX_GROUPS 32   // k-ranges of the first omega reduction level
 #define W90_MIXED_RTOL 1e-6 // relative FP32 noise floor of Omega in the bulk phase
@@ -165,7 +164,8 @@ struct State {
   int phase;       // 0: mixed bulk (FP32 M), 1: FP64
   int hold;        // bulk phase ended, waiting for the FP64 rebuild of M
   int bulk_iter;   // iterations done in the bulk phase
-  int pad0, pad1;
+  int stall;       // consecutive rejected bulk steps
+  int pad0;
   double gcnorm0, gcnorm1, gcfac, doda0, alphamin, falphamin;
   double beta, dir_sign, scale;
   double omega_invariant;
@@ -177,6 +177,7 @@ struct State {
 struct Prob {
   int nw, nk, nn, npan;
   int psz, nw2, ngroup, pad;  // psz: per-k partial vector length; ngroup: first-level k-ranges
+  // npan: tiles of the MFMA rotation per block M(k,b) (num_wann > 32)
   const int* nnl;     // [k][b] 0-based neighbour
   const double* bk;   // [k][b][3]
   const double* wb;   // [b]
@@ -191,9 +192,10 @@ struct Prob {
   cplx* W1;           // workspace per k
   cplx* W2;           // workspace per k
   double* ln[2];      // [k][b][n] Im ln M_nn of each buffer
-  double* d2;         // [k][b][n] |M_nn|^2 (panel path)
-  double* part_od;    // [k][b][panel] (panel path)
-  double* part_all;   // [k][b][panel] (panel path)
+  double* d2;         // [k][b][n] |M_nn|^2 (MFMA path)
+  double* part_od;    // [k][b][tile] (MFMA path)
+  double* part_all;   // [k][b][tile] (MFMA path)
+  void* Tm;           // [k][b][nw*nw] E_k^+ M(k,b) of the MFMA path, storage precision
   double* kpart;      // [k][psz] per-k partial sums of wann_omega (DiagAcc layout)
   double* gpart;      // [ngroup][psz] sums over contiguous k-ranges
   double* part_gg;    // [k]
@@ -429,89 +431,9 @@ __device__ __forceinline__ const cplx* rot_factor(const Prob& p, int, int) { ret
 
 // --------------------------------------------------------- M rotation
 
-// M rotation M(k,b) <- E_k^+ M(k,b) E_{k+b} for one (problem, k, b, row
-// panel) -- gridDim.y covers b and the panels -- with the per-block partial
-// sums of wann_omega: Im ln M_nn, |M_nn|^2, sum_{m!=n}|M_mn|^2 and
-// sum_mn |M_mn|^2.  Used for num_wann above the LDS kernels.
-// MODE_INIT only evaluates the partial sums of the current buffer.
-// (internal_new_u_and_m :1737-1748; wann_omega :2034-2043, :2076-2090,
-//  :2214-2258)
-template <class T2>
-__global__ void __launch_bounds__(W90_NT) k_rotate(const Prob* probs, const Item* items, int nitems, int mode) {
-  if (blockIdx.x >= (unsigned)nitems) return;
-  const Item it = items[blockIdx.x];
-  const Prob& p = probs[it.prob];
-  const State* st = p.st;
-  if (!gate(st, mode)) return;
-  const int nw = p.nw, nw2 = p.nw2;
-  const int bp = blockIdx.y;
-  if (bp >= p.nn * p.npan) return;
-  const int panel = bp % p.npan;
-  const int kb = it.k * p.nn + bp / p.npan;
-  const int k = it.k;
-  const int cur = st->cur;
-  const int dst = dst_buf(mode, cur);
-  const int i0 = panel * W90_PANEL;
-  const int prows = min(W90_PANEL, nw - i0);
-  const T2* Ms = rot_src<T2>(p, mode, cur) + (size_t)kb * nw2;
-  T2* Md = mbuf<T2>(p, dst) + (size_t)kb * nw2;
-  __shared__ T2 T[W90_PANEL * W90GPU_MAX_WANN];
-  __shared__ double red[W90_NT];
-
-  const int nout = prows * nw;
-  double od = 0.0, all = 0.0;
-  if (mode != MODE_INIT) {
-    const cplx* F = rot_factor(p, mode, cur);
-    const cplx* Ek = F + (size_t)k * nw2;
-    const cplx* Ekb = F + (size_t)p.nnl[kb] * nw2;
-    for (int e = threadIdx.x; e < nout; e += blockDim.x) {
-      const int i = e % prows, j = e / prows;
-      const cplx* ec = Ek + (size_t)(i0 + i) * nw;
-      const T2* mc = Ms + (size_t)j * nw;
-      T2 acc = c_zero<T2>();
-      for (int l = 0; l < nw; ++l) acc = c_add(acc, c_mul(c_narrow<T2>(c_conj(ec[l])), mc[l]));
-      T[i + j * W90_PANEL] = acc;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < nout; e += blockDim.x) {
-      const int i = e % prows, j = e / prows;
-      const cplx* ec = Ekb + (size_t)j * nw;
-      T2 acc = c_zero<T2>();
-      for (int l = 0; l < nw; ++l) acc = c_add(acc, c_mul(T[i + l * W90_PANEL], c_narrow<T2>(ec[l])));
-      Md[(i0 + i) + (size_t)j * nw] = acc;
-      const cplx a = c_wide(acc);
-      const double a2 = c_abs2(a);
-      all += a2;
-      if (i0 + i == j) {
-        p.ln[dst][(size_t)kb * nw + j] = atan2(a.y, a.x);
-        p.d2[(size_t)kb * nw + j] = a2;
-      } else {
-        od += a2;
-      }
-    }
-  } else {
-    for (int e = threadIdx.x; e < nout; e += blockDim.x) {
-      const int i = e % prows, j = e / prows;
-      const cplx v = c_wide(Ms[(i0 + i) + (size_t)j * nw]);
-      const double a2 = c_abs2(v);
-      all += a2;
-      if (i0 + i == j) {
-        p.ln[dst][(size_t)kb * nw + j] = atan2(v.y, v.x);
-        p.d2[(size_t)kb * nw + j] = a2;
-      } else {
-        od += a2;
-      }
-    }
-  }
-  od = block_sum(od, red);
-  all = block_sum(all, red);
-  if (threadIdx.x == 0) {
-    p.part_od[(size_t)kb * p.npan + panel] = od;
-    p.part_all[(size_t)kb * p.npan + panel] = all;
-  }
-}
-
-// Per-k partial vector from the outputs of the panel rotation kernel.
+// Per-k partial vector from the per-tile outputs of the MFMA rotation
+// (num_wann above the LDS kernels): diagonal phases and |M_nn|^2, and the
+// tile sums of |M_mn|^2.
 __global__ void __launch_bounds__(W90_NT) k_kpart(const Prob* probs, const Item* items, int nitems, int mode) {
   if (blockIdx.x >= (unsigned)nitems) return;
   const Item it = items[blockIdx.x];
@@ -867,10 +789,12 @@ __device__ void conv_step(State* st, double* logrow, double gc) {
 // Acceptance of a step with spread sp into buffer dst (wannierise.F90
 // :660-701), then the convergence test.  In the mixed bulk phase a step is
 // taken only if it lowers Omega by more than the FP32 noise floor of Omega
-// (W90_MIXED_RTOL relative, at least conv_tol); the first step that does
-// not -- or the end of the iteration budget -- ends the bulk phase: the
+// (W90_MIXED_RTOL relative, at least conv_tol), so a noise-level line
+// search is never followed.  A step that does not is rejected and the
+// conjugate gradient restarts from steepest descent; a second rejection in
+// a row -- or the end of the iteration budget -- ends the bulk phase: the
 // problem holds at its current point for the FP64 rebuild of M, from which
-// the FP64 loop continues.  A noise-level line search is never followed.
+// the FP64 loop continues.
 __device__ void accept_step(State* st, const Spread& sp, int dst, double* logrow) {
   if (st->phase == 0) {
     const double floor_tol = fmax(st->conv_tol, W90_MIXED_RTOL * fabs(st->wann.om_tot));
@@ -879,6 +803,10 @@ __device__ void accept_step(State* st, const Spread& sp, int dst, double* logrow
       st->old = st->wann;
       st->wann = sp;
       st->cur = dst;
+      st->stall = 0;
+    } else {
+      st->stall += 1;
+      st->ncg = st->num_cg_steps;
     }
     st->iter += 1;
     st->bulk_iter = st->iter;
@@ -888,7 +816,7 @@ __device__ void accept_step(State* st, const Spread& sp, int dst, double* logrow
       logrow[2] = st->wann.om_od;
       logrow[3] = st->gcnorm1;
     }
-    if (!progress || st->iter >= st->num_iter) st->hold = 1;
+    if (st->stall >= 2 || st->iter >= st->num_iter) st->hold = 1;
     return;
   }
   st->old = st->wann;
@@ -1104,7 +1032,7 @@ __device__ __forceinline__ cplx domega_term(cplx acc, double w, cplx mmn, cplx m
 // workgroup per (problem, k), fused with the per-k partial sums Re<G,G> and
 // Re<G,D_old> needed by internal_search_direction.
 template <class T2>
-__global__ void __launch_bounds__(W90_NT) k_domega(const Prob* probs, const Item* items, int nitems) {
+__global__ void __launch_bounds__(1024) k_domega(const Prob* probs, const Item* items, int nitems) {
   if (blockIdx.x >= (unsigned)nitems) return;
   const Item it = items[blockIdx.x];
   const Prob& p = probs[it.prob];
@@ -1116,7 +1044,7 @@ __global__ void __launch_bounds__(W90_NT) k_domega(const Prob* probs, const Item
   const double* ln = p.ln[cur];
   const double* rave = p.rave[cur];
   const T2* Mk = mbuf<T2>(p, cur) + (size_t)k * nn * nw2;
-  __shared__ double red[W90_NT];
+  __shared__ double red[1024];
   const double rnk = (double)p.nk;
   double gg = 0.0, gd = 0.0;
   for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
@@ -1146,29 +1074,44 @@ __global__ void __launch_bounds__(W90_NT) k_domega(const Prob* probs, const Item
   }
 }
 
+// Threads of k_domega_k: enough wavefronts per k to keep the block loads
+// of the larger num_wann in flight.
+template <int NWM>
+__host__ __device__ constexpr int domega_nt() {
+  return NWM > 32 ? 1024 : W90_NT;
+}
+
+// LDS bytes of k_domega_k (dynamic shared memory).
+template <int NWM, class T2>
+__host__ __device__ constexpr size_t domega_lds() {
+  return sizeof(T2) * (size_t)NWM * (NWM + 1) + sizeof(double) * (2 * W90GPU_NUM_NNMAX * NWM + domega_nt<NWM>());
+}
+
 // wann_domega with each block M(k,b) staged in LDS (num_wann <= NWM): the
 // transposed and diagonal reads of the gradient formula hit LDS (padded
 // leading dimension), and the scaled phases wb Im ln M_nn and b.rave_n of
 // the k are precomputed.
 template <int NWM, class T2>
-__global__ void __launch_bounds__(W90_NT) k_domega_k(const Prob* probs, const Item* items, int nitems) {
+__global__ void __launch_bounds__(domega_nt<NWM>()) k_domega_k(const Prob* probs, const Item* items, int nitems) {
   if (blockIdx.x >= (unsigned)nitems) return;
   const Item it = items[blockIdx.x];
   const Prob& p = probs[it.prob];
   const State* st = p.st;
   if (st->done || st->hold) return;
-  constexpr int SL = (NWM * NWM + W90_NT - 1) / W90_NT;
+  constexpr int NT = domega_nt<NWM>();
+  constexpr int SL = (NWM * NWM + NT - 1) / NT;
   const int nw = p.nw, nn = p.nn, nw2 = p.nw2;
   const int ld = nw + 1;
   const int k = it.k;
   const int tid = threadIdx.x;
   const int cur = st->cur;
-  __shared__ T2 Ms[NWM * (NWM + 1)];
-  __shared__ double q[W90GPU_NUM_NNMAX * NWM];
-  __shared__ double rb[W90GPU_NUM_NNMAX * NWM];
-  __shared__ double red[W90_NT];
+  extern __shared__ double lds_domega[];
+  double* q = lds_domega;                   // [b][n] wb Im ln M_nn
+  double* rb = q + W90GPU_NUM_NNMAX * NWM;  // [b][n] b.rave_n
+  double* red = rb + W90GPU_NUM_NNMAX * NWM;
+  T2* Ms = reinterpret_cast<T2*>(red + NT);
   const double* rave = p.rave[cur];
-  for (int t = tid; t < nn * nw; t += W90_NT) {
+  for (int t = tid; t < nn * nw; t += NT) {
     const int b = t / nw, n = t % nw;
     const size_t kb = (size_t)k * nn + b;
     const double* bv = p.bk + kb * 3;
@@ -1182,12 +1125,12 @@ __global__ void __launch_bounds__(W90_NT) k_domega_k(const Prob* probs, const It
   for (int b = 0; b < nn; ++b) {
     const T2* src = Mk + (size_t)b * nw2;
     __syncthreads();
-    for (int e = tid; e < nw2; e += W90_NT) Ms[(e % nw) + (e / nw) * ld] = src[e];
+    for (int e = tid; e < nw2; e += NT) Ms[(e % nw) + (e / nw) * ld] = src[e];
     __syncthreads();
     const double w = p.wb[b];
 #pragma unroll
     for (int s = 0; s < SL; ++s) {
-      const int e = tid + s * W90_NT;
+      const int e = tid + s * NT;
       if (e < nw2) {
         const int m = e % nw, n = e / nw;
         acc[s] = domega_term(acc[s], w, c_wide(Ms[m + n * ld]), c_wide(Ms[n + m * ld]), c_wide(Ms[n + n * ld]),
@@ -1200,7 +1143,7 @@ __global__ void __launch_bounds__(W90_NT) k_domega_k(const Prob* probs, const It
   double gg = 0.0, gd = 0.0;
 #pragma unroll
   for (int s = 0; s < SL; ++s) {
-    const int e = tid + s * W90_NT;
+    const int e = tid + s * NT;
     if (e < nw2) {
       const cplx g = c_scale(make_double2(acc[s].x / rnk, acc[s].y / rnk), 4.0);
       p.G[(size_t)k * nw2 + e] = g;
@@ -1346,8 +1289,11 @@ __global__ void __launch_bounds__(W90_NT) k_delta_u(const Prob* probs, const Ite
   wg_gemm_cn(p.E + o, p.U0 + o, p.U[st->cur] + o, p.nw);
 }
 
-// Retraction of one (problem, k) with workspaces in global memory (any
-// num_wann): forms the search direction D (trial step only), X = scale * D,
+// C = A * B on FP64 MFMA for matrices in global memory (w90gpu_mfma.h).
+__device__ void wg_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n);
+
+// Retraction of one (problem, k) with workspaces in global memory (num_wann
+// above the LDS kernels): forms the search direction D (trial step only), X = scale * D,
 // E = exp(X) by scaling and squaring of a Taylor polynomial, and U_new = U E.
 // X is anti-Hermitian, so exp(X) is the same unitary that
 // internal_new_u_and_m builds from the eigenvectors of i X
@@ -1400,7 +1346,7 @@ __global__ void __launch_bounds__(W90_NT) k_step(const Prob* probs, const Item* 
   }
   __syncthreads();
   for (int j = m - 1; j >= 1; --j) {
-    wg_gemm(Ek, X, P, nw);
+    wg_gemm_mfma(Ek, X, P, nw);
     const double invj = 1.0 / (double)j;
     for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
       cplx v = c_scale(Ek[e], invj);
@@ -1412,14 +1358,14 @@ __global__ void __launch_bounds__(W90_NT) k_step(const Prob* probs, const Item* 
   cplx* a = P;
   cplx* b = X;
   for (int s = 0; s < sq; ++s) {
-    wg_gemm(b, a, a, nw);
+    wg_gemm_mfma(b, a, a, nw);
     cplx* t = a;
     a = b;
     b = t;
   }
   for (int e = threadIdx.x; e < nw2; e += blockDim.x) Ek[e] = a[e];
   __syncthreads();
-  wg_gemm(p.U[1 - cur] + (size_t)k * nw2, p.U[cur] + (size_t)k * nw2, Ek, nw);
+  wg_gemm_mfma(p.U[1 - cur] + (size_t)k * nw2, p.U[cur] + (size_t)k * nw2, Ek, nw);
 }
 
 // Retraction of one (problem, k) with every matrix in LDS (num_wann <= NWM):

@@ -107,7 +107,7 @@ def make_problem(seed, kind="sc", a=5.0, nw=4, mp=(4, 4, 4), decay=1.5, c_over_a
     rvec = np.array(list(hr.keys()), dtype=float)
     hmat = np.array(list(hr.values()))
     phase = np.exp(2j * math.pi * (kpts @ rvec.T))           # (nk, nR)
-    hk = np.einsum("kr,rab->kab", phase, hmat)
+    hk = (phase @ hmat.reshape(len(hmat), nw * nw)).reshape(nk, nw, nw)
     hk = 0.5 * (hk + np.conj(np.transpose(hk, (0, 2, 1))))
     evals, cvec = np.linalg.eigh(hk)                           # cvec[k][:, n]
     gauge = np.exp(2j * math.pi * rng.uniform(size=(nk, nw)))
@@ -134,7 +134,8 @@ def overlaps(prob):
         bvec = km.bk[:, b, :].T                                # (nk,3)
         ph = np.exp(-1j * (bvec @ prob.tau_cart.T))            # (nk, nw): e^{-i b.tau_a}
         k2 = km.nnlist[:, b] - 1
-        blk = np.einsum("kam,ka,kan->kmn", c.conj(), ph, c[k2])
+        # C(k)^+ diag(e^{-i b.tau}) C(k+b), one batched product per b
+        blk = np.matmul(np.conj(np.transpose(c, (0, 2, 1))), ph[:, :, None] * c[k2])
         m[:, :, b, :] = np.transpose(blk, (1, 2, 0))
     return m
 

@@ -1,0 +1,339 @@
+"""Ragged-batch and serving workloads over the batch C ABI (w90gpu_batch_*).
+
+batch: a ragged batch of independent problems (lattice uniform over sc, fcc,
+bcc, hexagonal with c/a in [1.2, 1.8]; num_wann uniform in [4, 48]; k-mesh
+uniform over 4^3, 6^3, 8^3; each from its own sub-seed), solved
+  - one problem per launch: a device-resident session per problem, FP64 (the
+    plain drop-in path), problems one after another;
+  - as one batch in FP64 (the results must equal the plain ones bit for bit);
+  - as one batch in mixed precision, whose Omega and centres are compared
+    with the plain FP64 result of each problem.
+
+serve: open-loop Poisson arrivals of problems drawn from the same
+distribution extended to num_wann <= 96 and k-meshes up to 12^3; requests are
+admitted into a running batch as they arrive and as device memory allows
+(continuous batching), their M/U/centres stay on the device until fetched;
+p50/p99 time-to-converged from arrival and the throughput are reported next
+to a serial plain FP64 server (one request at a time, service times measured
+on the same device).
+"""
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import synth
+
+METRIC = "MLWF solves/s at ΔΩ≤1e-8 Å² vs FP64; p50/p99 latency; mixed vs plain GPU"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------ problem sets
+
+def batch_params(i):
+    """Item i of the ragged batch (256 items)."""
+    return synth.batch_params(i)
+
+
+def serve_params(i):
+    """Request i of the serving workload: the batch distribution extended to
+    num_wann <= 96 and k-meshes up to 12^3."""
+    return synth.batch_params(i, seed=synth.BASE_SEED + 5, nw_range=(4, 96), meshes=(4, 6, 8, 10, 12))
+
+
+def _generate(params):
+    p = dict(params)
+    seed = p.pop("seed")
+    prob = synth.make_problem(seed, **p)
+    u0, m0 = synth.initial_state(prob)
+    km = prob.kmesh
+    return dict(nw=prob.num_wann, nk=prob.num_kpts, nn=km.nntot, kind=p["kind"], mp=p["mp"],
+                real_lattice=prob.real_lattice, nnlist=km.nnlist, nncell=km.nncell, bk=km.bk, wb=km.wb,
+                wbtot=km.wbtot, m0=m0, u0=u0)
+
+
+def generate(plist, workers=None):
+    """Problems for a list of parameter sets, generated in worker processes
+    (spawned, so no GPU state is shared)."""
+    workers = workers or min(16, os.cpu_count() or 1)
+    t = time.perf_counter()
+    if workers <= 1 or len(plist) <= 1:
+        out = [_generate(p) for p in plist]
+    else:
+        # one BLAS thread per worker; largest problems first
+        saved = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "MKL_NUM_THREADS", "OPENBLAS_NUM_THREADS")}
+        for k in saved:
+            os.environ[k] = "1"
+        order = sorted(range(len(plist)), key=lambda i: -plist[i]["nw"] ** 2 * plist[i]["mp"][0] ** 3)
+        try:
+            with mp.get_context("spawn").Pool(workers) as pool:
+                res = pool.map(_generate, [plist[i] for i in order], chunksize=1)
+        finally:
+            for k, v in saved.items():
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = v
+        out = [None] * len(plist)
+        for j, i in enumerate(order):
+            out[i] = res[j]
+    log(f"[workload] generated {len(plist)} problems in {time.perf_counter() - t:.1f}s ({workers} workers)")
+    return out
+
+
+def kmesh_of(p):
+    from . import capi
+
+    return capi.KMesh(p["nn"], np.asfortranarray(p["nnlist"]), np.asfortranarray(p["nncell"]),
+                      np.asfortranarray(p["bk"]), np.ascontiguousarray(p["wb"]), p["wbtot"])
+
+
+def control(num_iter, mixed):
+    from . import capi
+
+    return capi.control(num_iter=num_iter, conv_window=3, conv_tol=1e-10, trial_step=2.0, num_cg_steps=5,
+                        precision=capi.PREC_MIXED if mixed else capi.PREC_FP64)
+
+
+def solve_alone(ctx, p, ctl):
+    """One problem per launch (device-resident session): (result, centres)."""
+    s = ctx.session(kmesh_of(p), p["nw"], p["nk"], ctl)
+    s.load(p["m0"], p["u0"])
+    r = s.solve()
+    _, _, cen, spr = s.fetch(want_m=False)
+    s.close()
+    return r, cen, spr
+
+
+def same_set(cen_a, spr_a, cen_b, spr_b, lattice, tol_c=1e-6, tol_s=1e-6):
+    """Centres/spreads equal up to a relabelling of the Wannier functions and
+    lattice translations of their centres."""
+    lat = np.asarray(lattice)
+    inv = np.linalg.inv(lat.T)
+    used = set()
+    for n in range(cen_a.shape[1]):
+        hit = None
+        for m in range(cen_b.shape[1]):
+            if m in used or abs(spr_a[n] - spr_b[m]) > tol_s:
+                continue
+            f = inv @ (cen_a[:, n] - cen_b[:, m])
+            if np.abs(lat.T @ (f - np.round(f))).max() <= tol_c:
+                hit = m
+                break
+        if hit is None:
+            return False
+        used.add(hit)
+    return True
+
+
+def pct(v, q):
+    v = sorted(v)
+    if not v:
+        return None
+    return v[min(len(v) - 1, int(round(q * (len(v) - 1))))]
+
+
+# ------------------------------------------------------------ batch
+
+def run_batch(args):
+    from . import capi
+
+    probs = generate([batch_params(i) for i in range(args.problems)])
+    n = len(probs)
+    ctx = capi.Context(0)
+    warm = probs[0]
+    solve_alone(ctx, warm, control(20, False))
+
+    # one problem per launch, FP64
+    plain = []
+    t0 = time.perf_counter()
+    for i, p in enumerate(probs):
+        r, cen, spr = solve_alone(ctx, p, control(args.num_iter, False))
+        plain.append((r, cen, spr))
+    t_plain_wall = time.perf_counter() - t0
+    t_plain = sum(r.solve_seconds for r, _, _ in plain)
+    log(f"[batch] plain: {t_plain:.2f}s of solves ({t_plain_wall:.2f}s wall with set-up)")
+
+    def batched(mixed):
+        ctl = control(args.num_iter, mixed)
+        pool = sum(capi.problem_bytes(p["nw"], p["nk"], p["nn"], args.num_iter) for p in probs)
+        b = ctx.batch(pool)
+        ids = [b.add(kmesh_of(p), p["m0"], p["u0"], ctl) for p in probs]
+        assert None not in ids
+        t = time.perf_counter()
+        b.run()
+        dt = time.perf_counter() - t
+        out = []
+        for i in ids:
+            r = b.result(i)
+            _, _, cen, spr = b.fetch(i)
+            out.append((r, cen, spr))
+        b.close()
+        return dt, out
+
+    t_b64, res64 = batched(False)
+    exact = sum(1 for (a, ca, _), (b, cb, _) in zip(plain, res64)
+                if a.omega_total == b.omega_total and a.iterations == b.iterations and np.array_equal(ca, cb))
+    log(f"[batch] batched FP64: {t_b64:.2f}s, {exact}/{n} bit-identical to one-per-launch")
+    t_mix, resm = batched(True)
+    d_om = [abs(a.omega_total - b.omega_total) for (a, _, _), (b, _, _) in zip(plain, resm)]
+    d_c = [float(np.abs(ca - cb).max()) for (_, ca, _), (_, cb, _) in zip(plain, resm)]
+    ok = [do <= 1e-8 and dc <= 1e-6 for do, dc in zip(d_om, d_c)]
+    ok_set = [do <= 1e-8 and same_set(cb, sb, ca, sa, p["real_lattice"])
+              for do, (_, ca, sa), (_, cb, sb), p in zip(d_om, plain, resm, probs)]
+    conv = sum(1 for r, _, _ in plain if r.converged)
+    log(f"[batch] batched mixed: {t_mix:.2f}s; within 1e-8/1e-6: {sum(ok)}/{n}, "
+        f"same WF set: {sum(ok_set)}/{n}; plain converged {conv}/{n}")
+    its = [r.iterations for r, _, _ in plain]
+    return {
+        "metric": METRIC,
+        "value": sum(ok) / t_mix,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": 1e3 * t_mix,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"ragged batch of {n} problems: sc/fcc/bcc/hex, num_wann U[4,48], k-mesh "
+                        f"4^3/6^3/8^3, num_iter={args.num_iter}, conv_window=3, conv_tol=1e-10",
+            "plain_fp64_one_per_launch_s": t_plain,
+            "batched_fp64_s": t_b64,
+            "batched_mixed_s": t_mix,
+            "speedup_batched_mixed_vs_plain": t_plain / t_mix,
+            "speedup_batched_fp64_vs_plain": t_plain / t_b64,
+            "batched_fp64_bit_identical": exact,
+            "mixed_within_1e8_omega_1e6_centres": sum(ok),
+            "mixed_same_wannier_set": sum(ok_set),
+            "max_delta_omega": max(d_om),
+            "median_delta_omega": pct(d_om, 0.5),
+            "plain_converged": conv,
+            "plain_iterations_median": pct(its, 0.5),
+            "mixed_bulk_iterations_median": pct([r.bulk_iterations for r, _, _ in resm], 0.5),
+            "problems": n,
+        },
+    }
+
+
+# ------------------------------------------------------------ serving
+
+def run_serve(args):
+    from . import capi
+
+    probs = generate([serve_params(i) for i in range(args.requests)])
+    n = len(probs)
+    ctx = capi.Context(0)
+    solve_alone(ctx, probs[0], control(20, False))
+    # serial plain FP64 server: service time of each request alone
+    service, plain = [], []
+    for p in probs:
+        r, cen, spr = solve_alone(ctx, p, control(args.num_iter, False))
+        service.append(r.solve_seconds)
+        plain.append((r, cen, spr))
+    cap = n / sum(service)
+    rate = args.rate if args.rate > 0 else 0.7 * cap
+    rng = np.random.default_rng(synth.BASE_SEED + 6)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    arrivals -= arrivals[0]
+    # serial FCFS queue with the measured service times
+    fin, lat_serial = 0.0, []
+    for a, s in zip(arrivals, service):
+        fin = max(fin, a) + s
+        lat_serial.append(fin - a)
+    log(f"[serve] {n} requests, serial capacity {cap:.2f}/s, arrival rate {rate:.2f}/s")
+
+    # continuous batching, mixed precision, admission by device bytes
+    ctl = control(args.num_iter, True)
+    need = [capi.problem_bytes(p["nw"], p["nk"], p["nn"], args.num_iter) for p in probs]
+    pool = max(max(need), int(args.pool_gb * (1 << 30)))
+    b = ctx.batch(pool)
+    queue, nxt, done = [], 0, 0
+    owner, lat, res = {}, [None] * n, [None] * n
+    max_running = 0
+    start = time.perf_counter()
+    while done < n:
+        t = time.perf_counter() - start
+        while nxt < n and arrivals[nxt] <= t:
+            queue.append(nxt)
+            nxt += 1
+        while queue:
+            i = queue[0]
+            p = probs[i]
+            h = b.add(kmesh_of(p), p["m0"], p["u0"], ctl)
+            if h is None:
+                break
+            owner[h] = i
+            queue.pop(0)
+        info = b.info()
+        max_running = max(max_running, info["running"] + info["queued"])
+        if info["running"] + info["queued"] == 0:
+            if nxt < n:
+                time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - start)))
+            continue
+        finished = b.step(1)
+        t = time.perf_counter() - start
+        for h in finished:
+            i = owner.pop(h)
+            lat[i] = t - arrivals[i]
+            r = b.result(h)
+            _, _, cen, spr = b.fetch(h)
+            res[i] = (r, cen, spr)
+            b.remove(h)
+            done += 1
+    makespan = time.perf_counter() - start
+    b.close()
+    d_om = [abs(a[0].omega_total - r[0].omega_total) for a, r in zip(plain, res)]
+    ok = [do <= 1e-8 and float(np.abs(a[1] - r[1]).max()) <= 1e-6 for do, a, r in zip(d_om, plain, res)]
+    ok_set = [do <= 1e-8 and same_set(r[1], r[2], a[1], a[2], p["real_lattice"])
+              for do, a, r, p in zip(d_om, plain, res, probs)]
+    thr = n / makespan
+    log(f"[serve] continuous batching: p50 {pct(lat, 0.5):.3f}s p99 {pct(lat, 0.99):.3f}s, {thr:.2f} solves/s; "
+        f"serial plain p50 {pct(lat_serial, 0.5):.3f}s p99 {pct(lat_serial, 0.99):.3f}s; "
+        f"within 1e-8: {sum(ok)}/{n}, same set {sum(ok_set)}/{n}")
+    return {
+        "metric": METRIC,
+        "value": sum(ok) / makespan,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": n,
+        "warmup": 1,
+        "ms_per_step": 1e3 * makespan / n,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"open-loop Poisson serving, {n} requests at {rate:.3f}/s: sc/fcc/bcc/hex, num_wann "
+                        f"U[4,96], k-mesh 4^3..12^3, num_iter={args.num_iter}, conv_window=3, conv_tol=1e-10, "
+                        f"mixed precision, continuous batching, admission by device bytes (pool "
+                        f"{pool / (1 << 30):.0f} GiB)",
+            "p50_s": pct(lat, 0.5),
+            "p99_s": pct(lat, 0.99),
+            "throughput_solves_s": thr,
+            "serial_plain_p50_s": pct(lat_serial, 0.5),
+            "serial_plain_p99_s": pct(lat_serial, 0.99),
+            "serial_plain_capacity_solves_s": cap,
+            "arrival_rate": rate,
+            "max_in_flight": max_running,
+            "mixed_within_1e8_omega_1e6_centres": sum(ok),
+            "mixed_same_wannier_set": sum(ok_set),
+            "max_delta_omega": max(d_om),
+            "requests": n,
+        },
+    }
+
+
+def run(args):
+    if args.workload == "batch":
+        return run_batch(args)
+    return run_serve(args)
