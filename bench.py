@@ -47,7 +47,14 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-iters", type=int, default=100)
+    ap.add_argument("--no-mixed", action="store_true", help="single: skip the mixed-precision comparison")
+    ap.add_argument("--cpu-sample-iters", type=int, default=0,
+                    help="iterations of the CPU baseline sample (0: 100 for configs 1-2, 3 for config 3)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device once (a kernel of known traffic that "
+                         "calibrates FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--config", type=int, choices=[1, 2, 3], default=2,
+                    help="single: benchmark configuration (2: the metric's; 3: num_wann 64 triclinic)")
     ap.add_argument("--workload", choices=["single", "batch", "serve"], default="single")
     ap.add_argument("--problems", type=int, default=256, help="batch: number of problems")
     ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
@@ -108,6 +115,25 @@ def cpu_baseline(prob, params, sample_iters):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+WORKLOADS = {
+    1: "sc a=5.0 A, num_wann=4, 4x4x4 MP grid (64 k)",
+    2: "fcc a=5.43 A, num_wann=16, 8x8x8 MP grid (512 k)",
+    3: "triclinic a,b,c~U[4,7] A, num_wann=64, 10x10x8 MP grid (800 k)",
+}
+
+
+def measured_traffic(config, precision):
+    """HBM bytes per launch of the rotation kernel from the committed PMC
+    summary of the same workload (profiles/*_pmc.json: FETCH_SIZE and
+    WRITE_SIZE per dispatch, scaled by the same counters on a copy kernel
+    of known size), or None."""
+    f = os.path.join(ROOT, "profiles", f"rotate_cfg{config}_{precision}_pmc.json")
+    if not os.path.exists(f):
+        return None
+    with open(f) as fh:
+        return json.load(fh).get("traffic_bytes_per_launch")
+
+
 def mixed_comparison(ctx, prob, m0, u0, sess_fp64, r64, args):
     """The same problem in mixed precision: solves/s, and Omega / centres
     against the FP64 solve of the same inputs (sess_fp64 holds its state)."""
@@ -136,6 +162,25 @@ def mixed_comparison(ctx, prob, m0, u0, sess_fp64, r64, args):
                 within_1e8=bool(d_om <= 1e-8 and d_c <= 1e-6))
 
 
+def copy_probe(mib):
+    """One device-to-device copy of `mib` MiB through the HIP runtime's copy
+    kernel: FETCH_SIZE and WRITE_SIZE of that dispatch should each read
+    mib * 1024 KiB, which gives the scale of the counters."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = C.c_size_t(mib << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), n) or hip.hipMalloc(C.byref(b), n):
+        raise RuntimeError("hipMalloc failed in the copy probe")
+    hip.hipMemset(a, 1, n)
+    hip.hipMemcpy(b, a, n, 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"[bench] copy probe: {mib} MiB device to device")
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -150,11 +195,15 @@ def main():
         return
     from wannier90_amd import capi, synth
 
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
     t = time.perf_counter()
-    prob = synth.config(2)
+    prob = synth.config(args.config)
     u0, m0 = synth.initial_state(prob)
-    log(f"[bench] generated cfg2 problem in {time.perf_counter() - t:.1f}s: nk={prob.num_kpts} "
+    log(f"[bench] generated cfg{args.config} problem in {time.perf_counter() - t:.1f}s: nk={prob.num_kpts} "
         f"nn={prob.kmesh.nntot} nw={prob.num_wann}")
+    if args.cpu_sample_iters <= 0:
+        args.cpu_sample_iters = 3 if args.config == 3 else 100
     params = dict(num_iter=500, conv_tol="1.0e-10", conv_window=3, trial_step="2.0",
                   num_cg_steps=5, num_print_cycles=100)
     prec = capi.PREC_FP64 if args.precision == "fp64" else capi.PREC_MIXED
@@ -185,7 +234,7 @@ def main():
     avg_ms, launches = sess.kernel_time()
     sess.profile(False)
     nk, nn, nw = prob.num_kpts, prob.kmesh.nntot, prob.num_wann
-    rot_bytes = 16.0 * nk * nw * nw * (2 * nn + 1)     # M read + write, E read once
+    rot_bytes = (16.0 if prec == capi.PREC_FP64 else 8.0) * nk * nw * nw * (2 * nn + 1)  # M read + write, E once
     rot_flops = 16.0 * nk * nn * nw ** 3               # two complex GEMMs per (k,b)
     ai = rot_flops / rot_bytes
     peak_flops = FP64_PEAK_TFLOPS if prec == capi.PREC_FP64 else FP32_PEAK_TFLOPS
@@ -198,8 +247,10 @@ def main():
         achieved = rot_flops / (avg_ms * 1e-3) / 1e12
         roof = dict(bound="mfma", achieved=achieved, peak=peak_flops, unit="TFLOP/s",
                     frac=achieved / peak_flops, traffic=None)
-    roof.update(kernel="k_rotate", avg_launch_ms=avg_ms, launches=launches,
+    kname = ("k_rotate_w16" if nw <= 16 else "k_rotate_k<32>" if nw <= 32 else "k_rot_gemm stage 1+2")
+    roof.update(kernel=kname, avg_launch_ms=avg_ms, launches=launches,
                 algorithmic_bytes=rot_bytes, algorithmic_flops=rot_flops)
+    roof["traffic"] = measured_traffic(args.config, args.precision)
     log(f"[bench] k_rotate: {launches} launches, avg {avg_ms * 1e3:.1f} us, AI={ai:.2f} flop/B")
 
     out = {
@@ -216,8 +267,8 @@ def main():
         "dtype": "f64" if prec == capi.PREC_FP64 else "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": "fcc a=5.43 A, num_wann=16, 8x8x8 MP grid (512 k), nntot=8, "
-                        "num_iter=500, conv_tol=1e-10, conv_window=3, one problem per launch",
+            "workload": WORKLOADS[args.config] + f", nntot={nn}, num_iter=500, conv_tol=1e-10, "
+                        "conv_window=3, one problem per launch",
             "precision": args.precision,
             "iterations": its,
             "omega_total": omega,
@@ -226,7 +277,7 @@ def main():
         },
         "roofline": roof,
     }
-    if prec == capi.PREC_FP64:
+    if prec == capi.PREC_FP64 and not args.no_mixed:
         out["mixed"] = mixed_comparison(ctx, prob, m0, u0, sess, results[-1], args)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(prob, params, args.cpu_sample_iters)

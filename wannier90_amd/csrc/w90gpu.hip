@@ -63,18 +63,18 @@ struct Member {
   double t_start = 0.0, t_end = 0.0;
 };
 
-// kernel classes by num_wann: b-parallel LDS rotation up to 8 and 16, the
-// per-k LDS kernels up to 32, the MFMA rotation with the MFMA retraction
-// and LDS gradient up to 64, the MFMA rotation with the global-memory
-// retraction and gradient up to W90GPU_MAX_WANN
-constexpr int kClasses = 5;
+// kernel classes by num_wann: the wavefront-per-neighbour MFMA rotation up
+// to 16, the per-k LDS kernels up to 32, the MFMA GEMM rotation with the
+// MFMA retraction and LDS gradient up to 64, the MFMA GEMM rotation with
+// the global-memory retraction and gradient up to W90GPU_MAX_WANN
+constexpr int kClasses = 4;
 constexpr int kLdsMaxWann = 32;
 // iterations per captured block; the solve is polled once per block
 constexpr int kGraphBlock = 8;
 
-int wann_class(int nw) { return nw <= 8 ? 0 : nw <= 16 ? 1 : nw <= kLdsMaxWann ? 2 : nw <= W90_STEP_N ? 3 : 4; }
+int wann_class(int nw) { return nw <= 16 ? 0 : nw <= kLdsMaxWann ? 1 : nw <= W90_STEP_N ? 2 : 3; }
 // per-(k,b) partial sums of the rotation: one per MFMA tile
-int npanels(int nw) { return wann_class(nw) >= 3 ? gemm_tiles(nw) : 1; }
+int npanels(int nw) { return wann_class(nw) >= 2 ? gemm_tiles(nw) : 1; }
 
 // A contiguous range of an item list with the launch shape its kernels need.
 struct Seg {
@@ -89,6 +89,7 @@ struct Lists {
   int np = 0;
   Seg seg[kClasses][2];  // [class][0: FP64, 1: FP32]
   bool any_second = false;
+  bool any_group = false;  // some problem uses the two-level omega reduction
 };
 
 struct Pool {
@@ -242,7 +243,7 @@ size_t problem_layout(int nw, int nk, int nn, int num_iter, Layout& lay) {
   lay.add(lnb);  // ln[0]
   lay.add(lnb);  // ln[1]
   lay.add(lnb);  // d2
-  lay.add(wann_class(nw) >= 3 ? mb : 256);                  // Tm (MFMA rotation scratch)
+  lay.add(wann_class(nw) >= 2 ? mb : 256);                  // Tm (MFMA rotation scratch)
   lay.add((size_t)nk * nn * npanels(nw) * sizeof(double));  // part_od
   lay.add((size_t)nk * nn * npanels(nw) * sizeof(double));  // part_all
   lay.add((size_t)nk * psz * sizeof(double));               // kpart
@@ -451,7 +452,9 @@ void member_bind(Member* m, char* a, const Layout& lay) {
   p.nw2 = m->nw * m->nw;
   p.npan = npanels(m->nw);
   p.psz = 8 * m->nw + 8;
-  p.ngroup = std::min(m->nk, W90_MAX_GROUPS);
+  // two reduction levels except for a handful of k-points (one workgroup
+  // summing all per-k vectors is slower than the group level above that)
+  p.ngroup = (m->nk <= 8) ? 0 : std::min(m->nk, W90_MAX_GROUPS);
   p.M[0] = (cplx*)(a + lay.off[i++]);
   p.M[1] = (cplx*)(a + lay.off[i++]);
   m->M0 = (cplx*)(a + lay.off[i++]);
@@ -632,6 +635,7 @@ void build_lists(const std::vector<Member*>& ms, F prec_of, std::vector<Item>& i
   for (Member* m : ms) {
     plist.push_back(m->slot);
     if (!m->ctl.lfixstep) L.any_second = true;
+    if (m->p.ngroup > 0) L.any_group = true;
   }
   L.np = (int)plist.size();
 }
@@ -709,20 +713,11 @@ int launch_rotate(Engine* E, const Lists& L, int mode) {
       if (c == 0) {
         const dim3 blk(64 * g.nnmax);
         if (q == 0)
-          hipLaunchKernelGGL((k_rotate_kb<8, cplx>), grid, blk, (rotate_kb_lds<8, cplx>(g.nnmax)), s, P, it, g.n,
+          hipLaunchKernelGGL(k_rotate_w16<cplx>, grid, blk, rotate_w16_lds<cplx>(g.nnmax), s, P, it, g.n, mode);
+        else
+          hipLaunchKernelGGL(k_rotate_w16<float2>, grid, blk, rotate_w16_lds<float2>(g.nnmax), s, P, it, g.n,
                              mode);
-        else
-          hipLaunchKernelGGL((k_rotate_kb<8, float2>), grid, blk, (rotate_kb_lds<8, float2>(g.nnmax)), s, P, it,
-                             g.n, mode);
       } else if (c == 1) {
-        const dim3 blk(64 * g.nnmax);
-        if (q == 0)
-          hipLaunchKernelGGL((k_rotate_kb<16, cplx>), grid, blk, (rotate_kb_lds<16, cplx>(g.nnmax)), s, P, it,
-                             g.n, mode);
-        else
-          hipLaunchKernelGGL((k_rotate_kb<16, float2>), grid, blk, (rotate_kb_lds<16, float2>(g.nnmax)), s, P,
-                             it, g.n, mode);
-      } else if (c == 2) {
         if (q == 0)
           hipLaunchKernelGGL((k_rotate_k<kLdsMaxWann, cplx>), grid, dim3(W90_NT), 0, s, P, it, g.n, mode);
         else
@@ -745,7 +740,7 @@ int launch_rotate(Engine* E, const Lists& L, int mode) {
     }
   if (pe) HIPCHK(E->ctx, hipEventRecord(pe, s));
   // per-k partial vectors of the MFMA classes
-  const Seg gk = class_range(L, 3, kClasses - 1);
+  const Seg gk = class_range(L, 2, kClasses - 1);
   if (gk.n) {
     hipLaunchKernelGGL(k_kpart, dim3(gk.n), dim3(W90_NT), 0, s, P, L.items + gk.off, gk.n, mode);
     HIPCHK(E->ctx, hipGetLastError());
@@ -756,9 +751,11 @@ int launch_rotate(Engine* E, const Lists& L, int mode) {
 int launch_omega(Engine* E, const Lists& L, int mode) {
   int rc = launch_rotate(E, L, mode);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_omega_group, dim3(L.np * W90_MAX_GROUPS), dim3(W90_NT), 0, E->stream, E->d_probs, L.plist,
-                     L.np, mode);
-  HIPCHK(E->ctx, hipGetLastError());
+  if (L.any_group) {
+    hipLaunchKernelGGL(k_omega_group, dim3(L.np * W90_MAX_GROUPS), dim3(W90_NT), 0, E->stream, E->d_probs,
+                       L.plist, L.np, mode);
+    HIPCHK(E->ctx, hipGetLastError());
+  }
   hipLaunchKernelGGL(k_omega_final, dim3(L.np), dim3(W90_FINAL_NT), 0, E->stream, E->d_probs, L.plist, L.np, mode);
   HIPCHK(E->ctx, hipGetLastError());
   return W90GPU_OK;
@@ -766,8 +763,8 @@ int launch_omega(Engine* E, const Lists& L, int mode) {
 
 int launch_step(Engine* E, const Lists& L, int mode) {
   hipStream_t s = E->stream;
-  const Seg a = class_range(L, 0, 1), b = class_range(L, 2, 2), c = class_range(L, 3, 3),
-            d = class_range(L, 4, 4);
+  const Seg a = class_range(L, 0, 0), b = class_range(L, 1, 1), c = class_range(L, 2, 2),
+            d = class_range(L, 3, 3);
   if (a.n)
     hipLaunchKernelGGL(k_step_k<16>, dim3(a.n), dim3(W90_NT), 0, s, E->d_probs, L.items + a.off, a.n, mode);
   if (b.n)
@@ -789,21 +786,21 @@ int launch_domega(Engine* E, const Lists& L) {
       const Seg& g = L.seg[c][q];
       if (g.n == 0) continue;
       const Item* it = L.items + g.off;
-      if (c <= 1) {
+      if (c == 0) {
         if (q == 0)
           hipLaunchKernelGGL((k_domega_k<16, cplx>), dim3(g.n), dim3(domega_nt<16>()), (domega_lds<16, cplx>()), s, P, it,
                              g.n);
         else
           hipLaunchKernelGGL((k_domega_k<16, float2>), dim3(g.n), dim3(domega_nt<16>()), (domega_lds<16, float2>()), s, P,
                              it, g.n);
-      } else if (c == 2) {
+      } else if (c == 1) {
         if (q == 0)
           hipLaunchKernelGGL((k_domega_k<kLdsMaxWann, cplx>), dim3(g.n), dim3(domega_nt<kLdsMaxWann>()),
                              (domega_lds<kLdsMaxWann, cplx>()), s, P, it, g.n);
         else
           hipLaunchKernelGGL((k_domega_k<kLdsMaxWann, float2>), dim3(g.n), dim3(domega_nt<kLdsMaxWann>()),
                              (domega_lds<kLdsMaxWann, float2>()), s, P, it, g.n);
-      } else if (c == 3) {
+      } else if (c == 2) {
         if (q == 0)
           hipLaunchKernelGGL((k_domega_k<W90_STEP_N, cplx>), dim3(g.n), dim3(domega_nt<W90_STEP_N>()),
                              (domega_lds<W90_STEP_N, cplx>()), s, P, it, g.n);
@@ -1168,10 +1165,8 @@ int w90gpu_create(int device, w90gpu_ctx** ctx) {
     const void* f;
     size_t bytes;
   } lds[] = {
-      {(const void*)k_rotate_kb<8, cplx>, rotate_kb_lds<8, cplx>(W90GPU_NUM_NNMAX)},
-      {(const void*)k_rotate_kb<16, cplx>, rotate_kb_lds<16, cplx>(W90GPU_NUM_NNMAX)},
-      {(const void*)k_rotate_kb<8, float2>, rotate_kb_lds<8, float2>(W90GPU_NUM_NNMAX)},
-      {(const void*)k_rotate_kb<16, float2>, rotate_kb_lds<16, float2>(W90GPU_NUM_NNMAX)},
+      {(const void*)k_rotate_w16<cplx>, rotate_w16_lds<cplx>(W90GPU_NUM_NNMAX)},
+      {(const void*)k_rotate_w16<float2>, rotate_w16_lds<float2>(W90GPU_NUM_NNMAX)},
       {(const void*)k_step_mfma, step_mfma_lds()},
       {(const void*)k_domega_k<W90_STEP_N, cplx>, domega_lds<W90_STEP_N, cplx>()},
       {(const void*)k_domega_k<W90_STEP_N, float2>, domega_lds<W90_STEP_N, float2>()},

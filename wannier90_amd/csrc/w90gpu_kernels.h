@@ -177,6 +177,7 @@ struct State {
 struct Prob {
   int nw, nk, nn, npan;
   int psz, nw2, ngroup, pad;  // psz: per-k partial vector length; ngroup: first-level k-ranges
+                              // (0: the final reduction sums the per-k vectors itself)
   // npan: tiles of the MFMA rotation per block M(k,b) (num_wann > 32)
   const int* nnl;     // [k][b] 0-based neighbour
   const double* bk;   // [k][b][3]
@@ -607,152 +608,6 @@ __global__ void __launch_bounds__(W90_NT) k_rotate_k(const Prob* probs, const It
   }
 }
 
-// LDS bytes of k_rotate_kb for up to nn neighbours (dynamic shared memory).
-template <int NWM, class T2>
-__host__ __device__ constexpr size_t rotate_kb_lds(int nn) {
-  return sizeof(T2) * (size_t)NWM * (NWM + 1) * (1 + 2 * (size_t)nn) +
-         sizeof(double) * (2 * (size_t)nn * NWM + 2 * (size_t)nn);
-}
-
-// M rotation with the omega partial sums for small blocks (num_wann <= NWM
-// <= 16): one workgroup per (problem, k) with one wavefront per neighbour b,
-// so the nn rotations of a k proceed concurrently.  E_k, the nn blocks
-// M(k,b) and the nn E_{k+b} are staged in LDS by one round of loads; T =
-// E_k^+ M(k,b) replaces M(k,b) in LDS; the diagonal of each rotated block
-// goes through LDS to threads n < num_wann, which sum it over b in order.
-// blockDim.x = 64 * (largest nntot of the launch); wavefronts beyond the
-// problem's nntot only take part in the barriers, and every sum is formed
-// per wavefront and then over b in order, so the result of a problem does
-// not depend on the other problems of the launch.
-template <int NWM, class T2>
-__global__ void __launch_bounds__(64 * W90GPU_NUM_NNMAX) k_rotate_kb(const Prob* probs, const Item* items,
-                                                                     int nitems, int mode) {
-  if (blockIdx.x >= (unsigned)nitems) return;
-  const Item it = items[blockIdx.x];
-  const Prob& p = probs[it.prob];
-  const State* st = p.st;
-  if (!gate(st, mode)) return;
-  constexpr int B2 = NWM * NWM;
-  constexpr int BP = NWM * (NWM + 1);
-  constexpr int SL = (B2 + 63) / 64;
-  const int nw = p.nw, nw2 = p.nw2, nn = p.nn;
-  const int nnl = blockDim.x >> 6;  // wavefronts of this launch (>= nn)
-  const int ld = nw + 1;
-  const int k = it.k;
-  const int tid = threadIdx.x;
-  const int b = tid >> 6;
-  const bool active = b < nn;
-  const int lane = tid & 63;
-  const int cur = st->cur;
-  const bool rot = (mode != MODE_INIT);
-  const int dst = dst_buf(mode, cur);
-  const cplx* F = rot_factor(p, mode, cur);
-  extern __shared__ double lds_raw[];
-  T2* EkH = reinterpret_cast<T2*>(lds_raw);
-  T2* Ms = EkH + BP + (size_t)b * 2 * BP;  // M(k,b), then T = E_k^+ M(k,b)
-  T2* Eb = Ms + BP;                        // E_{k+b}
-  double* lnL = reinterpret_cast<double*>(EkH + BP + (size_t)nnl * 2 * BP);  // [b][NWM]
-  double* d2L = lnL + nnl * NWM;
-  double* wod = d2L + nnl * NWM;  // [b] per-wave sum_{m!=n} |M_mn|^2
-  double* wall = wod + nnl;       // [b] per-wave sum_mn |M_mn|^2
-  const size_t kb = (size_t)k * nn + b;
-  if (rot)
-    for (int e = tid; e < nw2; e += blockDim.x) {
-      const int l = e % nw, i = e / nw;
-      EkH[i + l * ld] = c_narrow<T2>(c_conj(F[(size_t)k * nw2 + e]));
-    }
-  if (active) {
-    const T2* src = rot_src<T2>(p, mode, cur) + kb * nw2;
-    for (int e = lane; e < nw2; e += 64) Ms[(e % nw) + (e / nw) * ld] = src[e];
-    if (rot) {
-      const cplx* eb = F + (size_t)p.nnl[kb] * nw2;
-      for (int e = lane; e < nw2; e += 64) Eb[(e % nw) + (e / nw) * ld] = c_narrow<T2>(eb[e]);
-    }
-  }
-  __syncthreads();
-  if (rot) {
-    T2 tr[SL];
-#pragma unroll
-    for (int s = 0; s < SL; ++s) {
-      const int e = lane + 64 * s;
-      tr[s] = c_zero<T2>();
-      if (active && e < nw2) {
-        const int i = e % nw, j = e / nw;
-        T2 a = c_zero<T2>();
-        for (int l = 0; l < nw; ++l) a = c_add(a, c_mul(EkH[i + l * ld], Ms[l + j * ld]));
-        tr[s] = a;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < SL; ++s) {
-      const int e = lane + 64 * s;
-      if (active && e < nw2) Ms[(e % nw) + (e / nw) * ld] = tr[s];
-    }
-    __syncthreads();
-  }
-  double s_od = 0.0, s_all = 0.0;
-  if (active) {
-    T2* out = mbuf<T2>(p, dst) + kb * nw2;
-#pragma unroll
-    for (int s = 0; s < SL; ++s) {
-      const int e = lane + 64 * s;
-      if (e < nw2) {
-        const int i = e % nw, j = e / nw;
-        T2 r;
-        if (rot) {
-          r = c_zero<T2>();
-          for (int l = 0; l < nw; ++l) r = c_add(r, c_mul(Ms[i + l * ld], Eb[l + j * ld]));
-          out[e] = r;
-        } else {
-          r = Ms[i + j * ld];
-        }
-        const cplx rw = c_wide(r);
-        const double a2 = c_abs2(rw);
-        s_all += a2;
-        if (i != j) {
-          s_od += a2;
-        } else {
-          const double l = atan2(rw.y, rw.x);
-          p.ln[dst][kb * nw + i] = l;
-          lnL[b * NWM + i] = l;
-          d2L[b * NWM + i] = a2;
-        }
-      }
-    }
-  }
-  s_od = wave_sum(s_od);
-  s_all = wave_sum(s_all);
-  if (active && lane == 0) {
-    wod[b] = s_od;
-    wall[b] = s_all;
-  }
-  __syncthreads();
-  double* outk = p.kpart + (size_t)k * p.psz;
-  if (tid < nw) {
-    const int n = tid;
-    double r0[3];
-    ref_centre(p, mode, cur, n, r0);
-    DiagAcc d = {};
-    for (int bb = 0; bb < nn; ++bb)
-      diag_add(d, p.wb[bb], p.bk + ((size_t)k * nn + bb) * 3, lnL[bb * NWM + n], d2L[bb * NWM + n], r0[0],
-               r0[1], r0[2]);
-    diag_store(outk, nw, n, d);
-  }
-  if (tid == 0) {
-    double sod = 0.0, sall = 0.0, wsum = 0.0;
-    for (int bb = 0; bb < nn; ++bb) {
-      const double w = p.wb[bb];
-      sod += w * wod[bb];
-      sall += w * wall[bb];
-      wsum += w;
-    }
-    outk[8 * nw + 0] = sod;
-    outk[8 * nw + 1] = (double)nw * wsum - sall;
-    kpart_bmat(p, k, outk + 8 * nw);
-  }
-}
-
 // ---------------------------------------------------------- omega / control
 
 // internal_test_convergence (wannierise.F90:1076-1164) with the noise branch
@@ -847,76 +702,19 @@ __global__ void __launch_bounds__(W90_NT) k_omega_group(const Prob* probs, const
 // state (:452-470), internal_optimal_step (:1528-1596) after the trial step
 // and the acceptance of the trial or parabolic step (:660-701).  MODE_REMIX
 // installs the FP64 rebuild of M at the end of the mixed bulk phase.
-__global__ void __launch_bounds__(W90_FINAL_NT) k_omega_final(const Prob* probs, const int* plist, int np,
-                                                              int mode) {
-  if (blockIdx.x >= (unsigned)np) return;
-  const Prob& p = probs[plist[blockIdx.x]];
-  State* st = p.st;
-  if (!gate(st, mode)) return;
-  const int nw = p.nw, nk = p.nk, psz = p.psz;
-  const int cur = st->cur;
-  const int dst = dst_buf(mode, cur);
-  const double rnk = (double)nk;
-  const int tid = threadIdx.x;
-  __shared__ double red[W90_FINAL_NT];
-  __shared__ double sums[8 * W90GPU_MAX_WANN + 8];
-  __shared__ double sc[8];
-
-  // sum of the group vectors: VB values per pass (coalesced along a row),
-  // J lanes over groups, lanes combined in fixed order
-  const int VB = min(psz, W90_FINAL_NT);
-  const int J = W90_FINAL_NT / VB;
-  const int G = p.ngroup;
-  for (int v0 = 0; v0 < psz; v0 += VB) {
-    const int v = v0 + tid % VB;
-    const int j = tid / VB;
-    double s = 0.0;
-    if (j < J && v < psz)
-      for (int g = j; g < G; g += J) s += p.gpart[(size_t)g * psz + v];
-    red[tid] = s;
-    __syncthreads();
-    if (tid < VB && v0 + tid < psz) {
-      double t = 0.0;
-      for (int jj = 0; jj < J; ++jj) t += red[tid + jj * VB];
-      sums[v0 + tid] = t;
-    }
-    __syncthreads();
-  }
-
-  double omd_n = 0.0;
-  for (int n = tid; n < nw; n += blockDim.x) {
-    double q[8];
-    for (int u = 0; u < 8; ++u) q[u] = sums[u * nw + n];
-    double bb[6];
-    for (int u = 0; u < 6; ++u) bb[u] = sums[8 * nw + 2 + u];
-    // reference centres are read before rave[dst] is written (dst == cur
-    // for the rebuild)
-    double r0[3];
-    ref_centre(p, mode, cur, n, r0);
-    const double rx = -q[0] / rnk, ry = -q[1] / rnk, rz = -q[2] / rnk;
-    p.rave[dst][0 + 3 * n] = rx;
-    p.rave[dst][1 + 3 * n] = ry;
-    p.rave[dst][2 + 3 * n] = rz;
-    p.r2ave[dst][n] = q[3] / rnk;
-    const double dx = rx - r0[0], dy = ry - r0[1], dz = rz - r0[2];
-    const double dbd = bb[0] * dx * dx + bb[1] * dy * dy + bb[2] * dz * dz +
-                       2.0 * (bb[3] * dx * dy + bb[4] * dy * dz + bb[5] * dz * dx);
-    omd_n += q[4] + 2.0 * (dx * q[5] + dy * q[6] + dz * q[7]) + dbd;
-  }
-  // Omega_D summed over n in a fixed tree
-  const double omd = block_sum(omd_n, red);
-  if (threadIdx.x == 0) {
-    sc[0] = sums[8 * nw + 0];
-    sc[1] = sums[8 * nw + 1];
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-
+// Control logic of wann_main after a wann_omega evaluation, run by one
+// thread on the workgroup's copy of the state: the initial state
+// (:452-470), internal_optimal_step (:1528-1596) after the trial step, the
+// acceptance of the trial or parabolic step (:660-701), and the FP64
+// rebuild at the end of the mixed bulk phase.  sod, si, omd: the summed
+// off-diagonal, invariant and Omega_D numerators.
+__device__ void omega_control(State* st, const Prob& p, double sod, double si, double omd, int mode, int dst) {
+  const double rnk = (double)p.nk;
   Spread sp;
-  sp.om_od = sc[0] / rnk;
+  sp.om_od = sod / rnk;
   sp.om_d = omd / rnk;
   if (mode == MODE_INIT && st->first_pass) {
-    sp.om_i = sc[1] / rnk;
+    sp.om_i = si / rnk;
   } else {
     sp.om_i = st->omega_invariant;
   }
@@ -1006,6 +804,86 @@ __global__ void __launch_bounds__(W90_FINAL_NT) k_omega_final(const Prob* probs,
   st->second = 0;
   accept_step(st, sp, dst, logrow);
 }
+
+__global__ void __launch_bounds__(W90_FINAL_NT) k_omega_final(const Prob* probs, const int* plist, int np,
+                                                              int mode) {
+  if (blockIdx.x >= (unsigned)np) return;
+  const Prob& p = probs[plist[blockIdx.x]];
+  // the state is worked on in LDS: the control logic below is a chain of
+  // dependent reads and writes that would each pay a global-memory latency
+  __shared__ State sst;
+  static_assert(sizeof(State) % sizeof(double) == 0, "State is copied as doubles");
+  constexpr int NSD = sizeof(State) / sizeof(double);
+  for (int i = threadIdx.x; i < NSD; i += blockDim.x)
+    reinterpret_cast<double*>(&sst)[i] = reinterpret_cast<const double*>(p.st)[i];
+  __syncthreads();
+  State* st = &sst;
+  if (!gate(st, mode)) return;
+  const int nw = p.nw, nk = p.nk, psz = p.psz;
+  const int cur = st->cur;
+  const int dst = dst_buf(mode, cur);
+  const double rnk = (double)nk;
+  const int tid = threadIdx.x;
+  __shared__ double red[W90_FINAL_NT];
+  __shared__ double sums[8 * W90GPU_MAX_WANN + 8];
+  __shared__ double sc[8];
+
+  // sum of the group vectors (or, for problems without the group level,
+  // of the per-k vectors): VB values per pass (coalesced along a row), J
+  // lanes over rows, lanes combined in fixed order
+  const int VB = min(psz, W90_FINAL_NT);
+  const int J = W90_FINAL_NT / VB;
+  const int G = p.ngroup > 0 ? p.ngroup : nk;
+  const double* rows = p.ngroup > 0 ? p.gpart : p.kpart;
+  for (int v0 = 0; v0 < psz; v0 += VB) {
+    const int v = v0 + tid % VB;
+    const int j = tid / VB;
+    double s = 0.0;
+    if (j < J && v < psz)
+      for (int g = j; g < G; g += J) s += rows[(size_t)g * psz + v];
+    red[tid] = s;
+    __syncthreads();
+    if (tid < VB && v0 + tid < psz) {
+      double t = 0.0;
+      for (int jj = 0; jj < J; ++jj) t += red[tid + jj * VB];
+      sums[v0 + tid] = t;
+    }
+    __syncthreads();
+  }
+
+  double omd_n = 0.0;
+  for (int n = tid; n < nw; n += blockDim.x) {
+    double q[8];
+    for (int u = 0; u < 8; ++u) q[u] = sums[u * nw + n];
+    double bb[6];
+    for (int u = 0; u < 6; ++u) bb[u] = sums[8 * nw + 2 + u];
+    // reference centres are read before rave[dst] is written (dst == cur
+    // for the rebuild)
+    double r0[3];
+    ref_centre(p, mode, cur, n, r0);
+    const double rx = -q[0] / rnk, ry = -q[1] / rnk, rz = -q[2] / rnk;
+    p.rave[dst][0 + 3 * n] = rx;
+    p.rave[dst][1 + 3 * n] = ry;
+    p.rave[dst][2 + 3 * n] = rz;
+    p.r2ave[dst][n] = q[3] / rnk;
+    const double dx = rx - r0[0], dy = ry - r0[1], dz = rz - r0[2];
+    const double dbd = bb[0] * dx * dx + bb[1] * dy * dy + bb[2] * dz * dz +
+                       2.0 * (bb[3] * dx * dy + bb[4] * dy * dz + bb[5] * dz * dx);
+    omd_n += q[4] + 2.0 * (dx * q[5] + dy * q[6] + dz * q[7]) + dbd;
+  }
+  // Omega_D summed over n in a fixed tree
+  const double omd = block_sum(omd_n, red);
+  if (threadIdx.x == 0) {
+    sc[0] = sums[8 * nw + 0];
+    sc[1] = sums[8 * nw + 1];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) omega_control(st, p, sc[0], sc[1], omd, mode, dst);
+  __syncthreads();
+  for (int i = threadIdx.x; i < NSD; i += blockDim.x)
+    reinterpret_cast<double*>(p.st)[i] = reinterpret_cast<const double*>(&sst)[i];
+}
+
 
 // ---------------------------------------------------------- gradient
 
@@ -1160,27 +1038,9 @@ __global__ void __launch_bounds__(domega_nt<NWM>()) k_domega_k(const Prob* probs
   }
 }
 
-// internal_search_direction (wannierise.F90:1371-1525) without
-// preconditioning or noise, one workgroup per problem: Fletcher-Reeves
-// coefficient with its resets, slope along D and the uphill handling.  D
-// itself is formed by the step kernel.
-__global__ void __launch_bounds__(W90_NT) k_search(const Prob* probs, const int* plist, int np) {
-  if (blockIdx.x >= (unsigned)np) return;
-  const Prob& p = probs[plist[blockIdx.x]];
-  State* st = p.st;
-  if (st->done || st->hold) return;
-  __shared__ double red[W90_NT];
-  double gg = 0.0, gd = 0.0;
-  // fixed-order chunked sums over k
-  const int per = (p.nk + blockDim.x - 1) / blockDim.x;
-  const int kb = threadIdx.x * per, ke = min(p.nk, kb + per);
-  for (int k = kb; k < ke; ++k) {
-    gg += p.part_gg[k];
-    gd += p.part_gd[k];
-  }
-  gg = block_sum(gg, red);
-  gd = block_sum(gd, red);
-  if (threadIdx.x != 0) return;
+// Scalar part of internal_search_direction (wannierise.F90:1371-1525) on
+// the state copy: gg = Re<G,G>, gd = Re<G,D_old>.
+__device__ void search_control(State* st, double gg, double gd) {
   const int iter = st->iter + 1;
   st->second = 0;
   st->gcnorm1 = gg;
@@ -1227,6 +1087,40 @@ __global__ void __launch_bounds__(W90_NT) k_search(const Prob* probs, const int*
   st->doda0 = doda0;
   st->scale = (st->lfixstep ? st->fixed_step : st->trial_step) / w4;
 }
+
+// internal_search_direction (wannierise.F90:1371-1525) without
+// preconditioning or noise, one workgroup per problem: Fletcher-Reeves
+// coefficient with its resets, slope along D and the uphill handling.  D
+// itself is formed by the step kernel.
+__global__ void __launch_bounds__(W90_NT) k_search(const Prob* probs, const int* plist, int np) {
+  if (blockIdx.x >= (unsigned)np) return;
+  const Prob& p = probs[plist[blockIdx.x]];
+  // state in LDS, as in k_omega_final
+  __shared__ State sst;
+  constexpr int NSD = sizeof(State) / sizeof(double);
+  for (int i = threadIdx.x; i < NSD; i += blockDim.x)
+    reinterpret_cast<double*>(&sst)[i] = reinterpret_cast<const double*>(p.st)[i];
+  __syncthreads();
+  State* st = &sst;
+  if (st->done || st->hold) return;
+  __shared__ double red[W90_NT];
+  double gg = 0.0, gd = 0.0;
+  // fixed-order chunked sums over k
+  const int per = (p.nk + blockDim.x - 1) / blockDim.x;
+  const int kb = threadIdx.x * per, ke = min(p.nk, kb + per);
+  for (int k = kb; k < ke; ++k) {
+    gg += p.part_gg[k];
+    gd += p.part_gd[k];
+  }
+  gg = block_sum(gg, red);
+  gd = block_sum(gd, red);
+  if (threadIdx.x == 0) search_control(st, gg, gd);
+  __syncthreads();
+  for (int i = threadIdx.x; i < NSD; i += blockDim.x)
+    reinterpret_cast<double*>(p.st)[i] = reinterpret_cast<const double*>(&sst)[i];
+}
+
+
 
 // ---------------------------------------------------------- retraction
 

@@ -497,3 +497,174 @@ __device__ void wg_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n) {
   }
   __syncthreads();
 }
+
+// ---------------------------------------------------------- small blocks
+
+// LDS bytes of k_rotate_w16 for up to nn neighbours: one padded 16 x 17
+// block per wavefront, the diagonal of each rotated block and the per-wave
+// sums.
+template <class T2>
+__host__ __device__ constexpr size_t rotate_w16_lds(int nn) {
+  return sizeof(T2) * 16 * 17 * (size_t)nn + sizeof(double) * (2 * 16 * (size_t)nn + 2 * (size_t)nn);
+}
+
+// M rotation with the omega partial sums for num_wann <= 16: one workgroup
+// per (problem, k), one wavefront per neighbour b, each block a single
+// 16 x 16 MFMA tile (zero-padded).  T = E_k^+ M(k,b) is formed from operand
+// fragments loaded straight from memory, passed through the wavefront's LDS
+// block to become the A operand of R = T E_{k+b}; R is written once, its
+// diagonal goes through LDS to threads n < num_wann, which sum it over b in
+// order.  blockDim.x = 64 * (largest nntot of the launch); wavefronts beyond
+// the problem's nntot only take part in the barriers, and every sum is
+// formed per wavefront and then over b in order, so a problem's result does
+// not depend on the other problems of the launch.  MODE_INIT only reduces.
+// (internal_new_u_and_m :1737-1748; wann_omega :2034-2090, :2214-2275)
+template <class T2>
+__global__ void __launch_bounds__(64 * W90GPU_NUM_NNMAX) k_rotate_w16(const Prob* probs, const Item* items,
+                                                                      int nitems, int mode) {
+  typedef typename Real<T2>::type R;
+  typedef typename Mfma<R>::acc Acc;
+  if (blockIdx.x >= (unsigned)nitems) return;
+  const Item it = items[blockIdx.x];
+  const Prob& p = probs[it.prob];
+  const State* st = p.st;
+  if (!gate(st, mode)) return;
+  const int nw = p.nw, nw2 = p.nw2, nn = p.nn;
+  const int nnl = blockDim.x >> 6;
+  const int k = it.k;
+  const int tid = threadIdx.x;
+  const int b = tid >> 6;
+  const bool active = b < nn;
+  const int lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  const int cur = st->cur;
+  const bool rot = (mode != MODE_INIT);
+  const int dst = dst_buf(mode, cur);
+  extern __shared__ double lds_w16[];
+  T2* Tl = reinterpret_cast<T2*>(lds_w16) + (size_t)b * 16 * 17;  // this wave's block, column-major
+  double* lnL = reinterpret_cast<double*>(reinterpret_cast<T2*>(lds_w16) + (size_t)nnl * 16 * 17);
+  double* d2L = lnL + nnl * 16;
+  double* wod = d2L + nnl * 16;
+  double* wall = wod + nnl;
+  const size_t kb = (size_t)k * nn + (active ? b : 0);
+  const T2* src = rot_src<T2>(p, mode, cur) + kb * nw2;
+  const cplx* Ek = p.E + (size_t)k * nw2;
+  const cplx* Ekb = p.E + (size_t)p.nnl[kb] * nw2;
+  Acc Rr, Ri;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    Rr[i] = 0;
+    Ri[i] = 0;
+  }
+  // every operand fragment of both products is loaded up front, so the
+  // global-memory latencies overlap
+  cplx fa[4], fm[4], fb[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int l = 4 * s + lk;
+    fa[s] = fm[s] = fb[s] = c_make(0.0, 0.0);
+    if (active && rot && l < nw && lr < nw) {
+      fa[s] = Ek[l + lr * nw];
+      fm[s] = c_wide(src[l + lr * nw]);
+      fb[s] = Ekb[l + lr * nw];
+    }
+  }
+  if (active && rot) {
+    // T = E_k^+ M: A[i][l] = conj(E_k[l + i nw]), B[l][j] = M[l + j nw]
+    Acc Tr, Ti;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      Tr[i] = 0;
+      Ti[i] = 0;
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const cplx a = fa[s], v = fm[s];
+      const R ar = (R)a.x, ai = (R)(-a.y), br = (R)v.x, bi = (R)v.y;
+      Tr = Mfma<R>::mma(ar, br, Tr);
+      Tr = Mfma<R>::mma(-ai, bi, Tr);
+      Ti = Mfma<R>::mma(ar, bi, Ti);
+      Ti = Mfma<R>::mma(ai, br, Ti);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      T2 t;
+      t.x = Tr[i];
+      t.y = Ti[i];
+      Tl[Mfma<R>::row(lk, i) + lr * 17] = t;
+    }
+  }
+  __syncthreads();
+  double s_od = 0.0, s_all = 0.0;
+  if (active) {
+    T2* out = mbuf<T2>(p, dst) + kb * nw2;
+    if (rot) {
+      // R = T E_{k+b}: A[i][l] = T[i][l] (LDS), B[l][j] = E_{k+b}[l + j nw]
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int l = 4 * s + lk;
+        const T2 a = Tl[lr + l * 17];
+        const R br = (R)fb[s].x, bi = (R)fb[s].y;
+        Rr = Mfma<R>::mma(a.x, br, Rr);
+        Rr = Mfma<R>::mma(-a.y, bi, Rr);
+        Ri = Mfma<R>::mma(a.x, bi, Ri);
+        Ri = Mfma<R>::mma(a.y, br, Ri);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = rot ? Mfma<R>::row(lk, i) : (lane + 64 * i) % 16;
+      const int c = rot ? lr : (lane + 64 * i) / 16;
+      if (r < nw && c < nw) {
+        T2 v;
+        if (rot) {
+          v.x = Rr[i];
+          v.y = Ri[i];
+          out[r + c * nw] = v;
+        } else {
+          v = src[r + c * nw];
+        }
+        const cplx vw = c_wide(v);
+        const double a2 = c_abs2(vw);
+        s_all += a2;
+        if (r != c) {
+          s_od += a2;
+        } else {
+          const double l = atan2(vw.y, vw.x);
+          p.ln[dst][kb * nw + r] = l;
+          lnL[b * 16 + r] = l;
+          d2L[b * 16 + r] = a2;
+        }
+      }
+    }
+  }
+  s_od = wave_sum(s_od);
+  s_all = wave_sum(s_all);
+  if (active && lane == 0) {
+    wod[b] = s_od;
+    wall[b] = s_all;
+  }
+  __syncthreads();
+  double* outk = p.kpart + (size_t)k * p.psz;
+  if (tid < nw) {
+    const int n = tid;
+    double r0[3];
+    ref_centre(p, mode, cur, n, r0);
+    DiagAcc d = {};
+    for (int bb = 0; bb < nn; ++bb)
+      diag_add(d, p.wb[bb], p.bk + ((size_t)k * nn + bb) * 3, lnL[bb * 16 + n], d2L[bb * 16 + n], r0[0], r0[1],
+               r0[2]);
+    diag_store(outk, nw, n, d);
+  }
+  if (tid == 0) {
+    double sod = 0.0, sall = 0.0, wsum = 0.0;
+    for (int bb = 0; bb < nn; ++bb) {
+      const double w = p.wb[bb];
+      sod += w * wod[bb];
+      sall += w * wall[bb];
+      wsum += w;
+    }
+    outk[8 * nw + 0] = sod;
+    outk[8 * nw + 1] = (double)nw * wsum - sall;
+    kpart_bmat(p, k, outk + 8 * nw);
+  }
+}
