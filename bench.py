@@ -68,16 +68,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(prob, params, sample_iters):
+def cpu_baseline(prob, params, sample_iters, use_reference=True):
     """Reference wannier90.x (oracle/_ref, serial, MKL sequential) on the same
     inputs, num_iter cut to `sample_iters`; scaled to the full solve by the
-    iteration count of the GPU solve.  Falls back to the C restatement."""
+    iteration count of the GPU solve.  Falls back to the C restatement (also
+    used for num_wann 64, whose text .mmn would take minutes to write and
+    read)."""
     from wannier90_amd import w90io
 
     exe = os.path.join(ROOT, "oracle", "_ref", "wannier90.x")
     tmp = tempfile.mkdtemp(prefix="w90bench_")
     try:
-        if os.path.exists(exe):
+        if use_reference and os.path.exists(exe):
             p = dict(params)
             p["num_iter"] = sample_iters
             w90io.write_win(os.path.join(tmp, "w.win"), prob.real_lattice, prob.mp, prob.kpts,
@@ -228,11 +230,10 @@ def main():
     omega = results[-1].omega_total
     log(f"[bench] {args.steps} solves in {total:.3f}s; iterations={its} Omega={omega:.10f}")
 
-    # kernel timing of the dominant kernel (M rotation) with HIP events
-    sess.profile(True)
-    r = sess.solve()
-    avg_ms, launches = sess.kernel_time()
-    sess.profile(False)
+    # the dominant kernel (M rotation with the fused omega partial sums):
+    # HIP events around back-to-back launches on the final state
+    launches = 200 if prob.num_wann <= 32 else 20
+    avg_ms = sess.rotation_time(launches)
     nk, nn, nw = prob.num_kpts, prob.kmesh.nntot, prob.num_wann
     rot_bytes = (16.0 if prec == capi.PREC_FP64 else 8.0) * nk * nw * nw * (2 * nn + 1)  # M read + write, E once
     rot_flops = 16.0 * nk * nn * nw ** 3               # two complex GEMMs per (k,b)
@@ -280,7 +281,7 @@ def main():
     if prec == capi.PREC_FP64 and not args.no_mixed:
         out["mixed"] = mixed_comparison(ctx, prob, m0, u0, sess, results[-1], args)
     if not args.no_cpu_baseline:
-        cb = cpu_baseline(prob, params, args.cpu_sample_iters)
+        cb = cpu_baseline(prob, params, args.cpu_sample_iters, use_reference=args.config != 3)
         per_solve = cb["seconds"] * its / max(cb["iters"], 1)
         out["cpu_baseline"] = dict(value=1.0 / per_solve, unit="solves/s", cores=1, kind=cb["kind"],
                                    sample=cb["sample"] + f"; scaled to {its} iterations")

@@ -196,11 +196,13 @@ int w90gpu_session_fetch(w90gpu_session* sess, double* m_matrix, double* u_matri
 int w90gpu_session_history(w90gpu_session* sess, double* out, int rows);
 /* Device bytes held by a session of this problem shape. */
 size_t w90gpu_session_bytes(int num_wann, int num_kpts, int nntot);
-/* Kernel timing: when enabled, HIP events bracket every launch of the
- * M-rotation kernel on the session's stream; *avg_ms and *launches report
- * the average duration and count since the last reset. */
-int w90gpu_session_profile(w90gpu_session* sess, int enable);
-int w90gpu_session_kernel_time(w90gpu_session* sess, double* avg_ms, int* launches);
+/* Kernel timing: the M-rotation kernel of this problem (the fused rotation
+ * and omega partial sums of one trial step) launched `reps` times back to
+ * back on the final state of the last solve, replayed from a HIP graph as
+ * in the solve and bracketed by HIP events on the session's stream;
+ * *avg_ms = elapsed / reps.  The solver state is not changed (each launch
+ * rewrites the same scratch buffer). */
+int w90gpu_session_rotation_time(w90gpu_session* sess, int reps, double* avg_ms);
 
 /* ------------------------------------------------ batches and serving
  *

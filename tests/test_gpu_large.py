@@ -1,4 +1,4 @@
-"""num_wann above the LDS kernels (33-128): the MFMA GEMM rotation, checked
+"""num_wann above 16 (17-128): the MFMA GEMM rotation, checked
 against the oracle on seeded synthetic problems (sc, bcc, fcc, triclinic
 lattices; nntot 6-12) small enough in k for the oracle to finish in seconds.
 
@@ -17,9 +17,11 @@ pytestmark = pytest.mark.gpu
 
 po = pytest.importorskip("oracle.pyoracle")
 
-# (lattice, num_wann, MP grid)
+# (lattice, num_wann, MP grid): 32x32 MFMA tiles (17-32), 64x64 tiles with
+# the LDS retraction (33-64) and with the global-memory retraction (65-128)
 LARGE = [("sc", 40, (3, 3, 3)), ("bcc", 48, (3, 3, 3)), ("fcc", 33, (2, 2, 2)),
-         ("triclinic", 64, (2, 2, 3)), ("sc", 96, (2, 2, 2)), ("hex", 70, (2, 2, 2))]
+         ("triclinic", 64, (2, 2, 3)), ("fcc", 24, (3, 3, 3)), ("hex", 17, (3, 3, 3)),
+         ("bcc", 32, (3, 3, 3)), ("sc", 96, (2, 2, 2)), ("hex", 70, (2, 2, 2))]
 
 
 @pytest.fixture(scope="module")
@@ -81,7 +83,7 @@ def test_large_phases_match_oracle(ctx, case):
     assert np.abs(g - g_ref).max() <= 1e-12 * max(1.0, np.abs(g_ref).max())
 
 
-@pytest.mark.parametrize("case", LARGE[:4], ids=ids)
+@pytest.mark.parametrize("case", LARGE[:7], ids=ids)
 def test_large_trajectory_matches_oracle(ctx, case):
     from wannier90_amd import capi
 

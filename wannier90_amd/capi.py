@@ -92,8 +92,7 @@ SIGNATURES = [
     ("w90gpu_session_fetch", C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     ("w90gpu_session_bytes", C.c_size_t, [C.c_int, C.c_int, C.c_int]),
     ("w90gpu_session_history", C.c_int, [_vp, _dp, C.c_int]),
-    ("w90gpu_session_profile", C.c_int, [_vp, C.c_int]),
-    ("w90gpu_session_kernel_time", C.c_int, [_vp, _dp, _ip]),
+    ("w90gpu_session_rotation_time", C.c_int, [_vp, C.c_int, _dp]),
     ("w90gpu_problem_bytes", C.c_size_t, [C.c_int, C.c_int, C.c_int, C.c_int]),
     ("w90gpu_batch_create", C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
     ("w90gpu_batch_destroy", C.c_int, [_vp]),
@@ -357,14 +356,12 @@ class Session:
             raise W90Error(-n, self.ctx.error())
         return out[:n]
 
-    def profile(self, enable=True):
-        self.ctx._check(self.lib.w90gpu_session_profile(self.h, 1 if enable else 0))
-
-    def kernel_time(self):
+    def rotation_time(self, reps=50):
+        """Average duration (ms) of the M-rotation kernel, `reps` launches
+        back to back on the final state of the last solve."""
         ms = C.c_double(0.0)
-        n = C.c_int(0)
-        self.ctx._check(self.lib.w90gpu_session_kernel_time(self.h, C.byref(ms), C.byref(n)))
-        return ms.value, n.value
+        self.ctx._check(self.lib.w90gpu_session_rotation_time(self.h, reps, C.byref(ms)))
+        return ms.value
 
 
 class Batch:

@@ -64,9 +64,10 @@ struct Member {
 };
 
 // kernel classes by num_wann: the wavefront-per-neighbour MFMA rotation up
-// to 16, the per-k LDS kernels up to 32, the MFMA GEMM rotation with the
-// MFMA retraction and LDS gradient up to 64, the MFMA GEMM rotation with
-// the global-memory retraction and gradient up to W90GPU_MAX_WANN
+// to 16; the MFMA GEMM rotation (32 x 32 tiles) with the per-k LDS
+// retraction and gradient up to 32; the MFMA GEMM rotation (64 x 64 tiles)
+// with the MFMA retraction and LDS gradient up to 64, and with the
+// global-memory retraction and gradient up to W90GPU_MAX_WANN
 constexpr int kClasses = 4;
 constexpr int kLdsMaxWann = 32;
 // iterations per captured block; the solve is polled once per block
@@ -74,7 +75,7 @@ constexpr int kGraphBlock = 8;
 
 int wann_class(int nw) { return nw <= 16 ? 0 : nw <= kLdsMaxWann ? 1 : nw <= W90_STEP_N ? 2 : 3; }
 // per-(k,b) partial sums of the rotation: one per MFMA tile
-int npanels(int nw) { return wann_class(nw) >= 2 ? gemm_tiles(nw) : 1; }
+int npanels(int nw) { return wann_class(nw) >= 1 ? gemm_tiles(nw) : 1; }
 
 // A contiguous range of an item list with the launch shape its kernels need.
 struct Seg {
@@ -161,12 +162,6 @@ struct Engine {
   bool use_graph = true;
   hipEvent_t blk_ev[2] = {nullptr, nullptr};
   std::vector<int> blk_slots[2];
-  // profiling of the rotation kernel
-  bool prof = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
-  size_t ev_used = 0;
-  double prof_ms = 0.0;
-  int prof_launches = 0;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   std::vector<int> finished_ids;  // retired since the last report
 };
@@ -243,7 +238,7 @@ size_t problem_layout(int nw, int nk, int nn, int num_iter, Layout& lay) {
   lay.add(lnb);  // ln[0]
   lay.add(lnb);  // ln[1]
   lay.add(lnb);  // d2
-  lay.add(wann_class(nw) >= 2 ? mb : 256);                  // Tm (MFMA rotation scratch)
+  lay.add(wann_class(nw) >= 1 ? mb : 256);                  // Tm (MFMA rotation scratch)
   lay.add((size_t)nk * nn * npanels(nw) * sizeof(double));  // part_od
   lay.add((size_t)nk * nn * npanels(nw) * sizeof(double));  // part_all
   lay.add((size_t)nk * psz * sizeof(double));               // kpart
@@ -426,10 +421,6 @@ void engine_destroy(Engine* E) {
   drop_graph(E);
   reap_graphs(E);
   for (auto& kv : E->by_id) delete kv.second;
-  for (auto& ev : E->evs) {
-    (void)hipEventDestroy(ev.first);
-    (void)hipEventDestroy(ev.second);
-  }
   for (hipEvent_t ev : {E->t0, E->t1, E->blk_ev[0], E->blk_ev[1]})
     if (ev) (void)hipEventDestroy(ev);
   for (void* q : {(void*)E->d_probs, (void*)E->d_items, (void*)E->d_plist, (void*)E->d_aux_items,
@@ -452,9 +443,9 @@ void member_bind(Member* m, char* a, const Layout& lay) {
   p.nw2 = m->nw * m->nw;
   p.npan = npanels(m->nw);
   p.psz = 8 * m->nw + 8;
-  // two reduction levels except for a handful of k-points (one workgroup
-  // summing all per-k vectors is slower than the group level above that)
-  p.ngroup = (m->nk <= 8) ? 0 : std::min(m->nk, W90_MAX_GROUPS);
+  // two reduction levels only where one workgroup summing all per-k
+  // vectors would be slow (over 128 Ki values)
+  p.ngroup = ((size_t)m->nk * p.psz <= (1u << 17)) ? 0 : std::min(m->nk, W90_MAX_GROUPS);
   p.M[0] = (cplx*)(a + lay.off[i++]);
   p.M[1] = (cplx*)(a + lay.off[i++]);
   m->M0 = (cplx*)(a + lay.off[i++]);
@@ -673,20 +664,6 @@ int aux_lists(Engine* E, const std::vector<Member*>& ms, Lists& L) {
 
 // ---------------------------------------------------------- launches
 
-int prof_begin(Engine* E, hipEvent_t* b) {
-  if (!E->prof) return W90GPU_OK;
-  if (E->ev_used >= E->evs.size()) {
-    hipEvent_t e0, e1;
-    HIPCHK(E->ctx, hipEventCreate(&e0));
-    HIPCHK(E->ctx, hipEventCreate(&e1));
-    E->evs.emplace_back(e0, e1);
-  }
-  HIPCHK(E->ctx, hipEventRecord(E->evs[E->ev_used].first, E->stream));
-  *b = E->evs[E->ev_used].second;
-  ++E->ev_used;
-  return W90GPU_OK;
-}
-
 // A range of whole classes [c0, c1] of the item list (both precisions).
 Seg class_range(const Lists& L, int c0, int c1) {
   Seg r;
@@ -701,9 +678,6 @@ Seg class_range(const Lists& L, int c0, int c1) {
 int launch_rotate(Engine* E, const Lists& L, int mode) {
   hipStream_t s = E->stream;
   const Prob* P = E->d_probs;
-  hipEvent_t pe = nullptr;
-  int rc = prof_begin(E, &pe);
-  if (rc) return rc;
   for (int c = 0; c < kClasses; ++c)
     for (int q = 0; q < 2; ++q) {
       const Seg& g = L.seg[c][q];
@@ -717,30 +691,37 @@ int launch_rotate(Engine* E, const Lists& L, int mode) {
         else
           hipLaunchKernelGGL(k_rotate_w16<float2>, grid, blk, rotate_w16_lds<float2>(g.nnmax), s, P, it, g.n,
                              mode);
-      } else if (c == 1) {
-        if (q == 0)
-          hipLaunchKernelGGL((k_rotate_k<kLdsMaxWann, cplx>), grid, dim3(W90_NT), 0, s, P, it, g.n, mode);
-        else
-          hipLaunchKernelGGL((k_rotate_k<kLdsMaxWann, float2>), grid, dim3(W90_NT), 0, s, P, it, g.n, mode);
       } else {
         // MFMA GEMM tiles; gridDim.y covers b and the tiles of a block
         const dim3 g2(g.n, g.nnmax * g.npanmax);
-        if (mode == MODE_INIT) {
-          if (q == 0) hipLaunchKernelGGL(k_reduce_tiles<cplx>, g2, dim3(256), 0, s, P, it, g.n, mode);
-          else hipLaunchKernelGGL(k_reduce_tiles<float2>, g2, dim3(256), 0, s, P, it, g.n, mode);
-        } else if (q == 0) {
-          hipLaunchKernelGGL((k_rot_gemm<cplx, 1>), g2, dim3(256), 0, s, P, it, g.n, mode);
-          hipLaunchKernelGGL((k_rot_gemm<cplx, 2>), g2, dim3(256), 0, s, P, it, g.n, mode);
+        if (c == 1) {
+          if (mode == MODE_INIT) {
+            if (q == 0) hipLaunchKernelGGL((k_reduce_tiles<cplx, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
+            else hipLaunchKernelGGL((k_reduce_tiles<float2, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
+          } else if (q == 0) {
+            hipLaunchKernelGGL((k_rot_gemm<cplx, 1, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
+            hipLaunchKernelGGL((k_rot_gemm<cplx, 2, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
+          } else {
+            hipLaunchKernelGGL((k_rot_gemm<float2, 1, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
+            hipLaunchKernelGGL((k_rot_gemm<float2, 2, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
+          }
         } else {
-          hipLaunchKernelGGL((k_rot_gemm<float2, 1>), g2, dim3(256), 0, s, P, it, g.n, mode);
-          hipLaunchKernelGGL((k_rot_gemm<float2, 2>), g2, dim3(256), 0, s, P, it, g.n, mode);
+          if (mode == MODE_INIT) {
+            if (q == 0) hipLaunchKernelGGL((k_reduce_tiles<cplx, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
+            else hipLaunchKernelGGL((k_reduce_tiles<float2, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
+          } else if (q == 0) {
+            hipLaunchKernelGGL((k_rot_gemm<cplx, 1, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
+            hipLaunchKernelGGL((k_rot_gemm<cplx, 2, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
+          } else {
+            hipLaunchKernelGGL((k_rot_gemm<float2, 1, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
+            hipLaunchKernelGGL((k_rot_gemm<float2, 2, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
+          }
         }
       }
       HIPCHK(E->ctx, hipGetLastError());
     }
-  if (pe) HIPCHK(E->ctx, hipEventRecord(pe, s));
   // per-k partial vectors of the MFMA classes
-  const Seg gk = class_range(L, 2, kClasses - 1);
+  const Seg gk = class_range(L, 1, kClasses - 1);
   if (gk.n) {
     hipLaunchKernelGGL(k_kpart, dim3(gk.n), dim3(W90_NT), 0, s, P, L.items + gk.off, gk.n, mode);
     HIPCHK(E->ctx, hipGetLastError());
@@ -825,8 +806,7 @@ int all_items(const Lists& L) {
 int enqueue_iteration(Engine* E, const Lists& L) {
   int rc;
   if ((rc = launch_domega(E, L))) return rc;
-  hipLaunchKernelGGL(k_search, dim3(L.np), dim3(W90_NT), 0, E->stream, E->d_probs, L.plist, L.np);
-  HIPCHK(E->ctx, hipGetLastError());
+  // the search direction is formed inside the trial-step kernels
   if ((rc = launch_step(E, L, MODE_TRIAL))) return rc;
   if ((rc = launch_omega(E, L, MODE_TRIAL))) return rc;
   if (L.any_second) {
@@ -981,7 +961,7 @@ int enqueue_block(Engine* E, int slot) {
   int rc;
   // the first block of an epoch is launched directly; the graph is
   // captured when the membership has stayed the same for a block
-  const bool graph = E->use_graph && !E->prof && !E->graph_failed && E->epoch_blocks >= 1;
+  const bool graph = E->use_graph && !E->graph_failed && E->epoch_blocks >= 1;
   if (graph && !E->gexec) {
     if ((rc = capture_block(E))) return rc;
   }
@@ -1008,6 +988,7 @@ int enqueue_block(Engine* E, int slot) {
 int process_block(Engine* E, int slot, std::vector<Member*>& remix) {
   HIPCHK(E->ctx, hipEventSynchronize(E->blk_ev[slot]));
   const Status* st = E->h_status + (size_t)slot * E->cap_plist;
+
   std::vector<Member*> done;
   const std::vector<int>& sl = E->blk_slots[slot];
   for (size_t i = 0; i < sl.size(); ++i) {
@@ -1266,15 +1247,6 @@ int w90gpu_session_wait(w90gpu_session* s, w90gpu_result* r) {
   HIPCHK(E->ctx, hipEventSynchronize(E->t1));
   float ms = 0.f;
   HIPCHK(E->ctx, hipEventElapsedTime(&ms, E->t0, E->t1));
-  if (E->prof) {
-    for (size_t i = 0; i < E->ev_used; ++i) {
-      float t = 0.f;
-      HIPCHK(E->ctx, hipEventElapsedTime(&t, E->evs[i].first, E->evs[i].second));
-      E->prof_ms += t;
-      E->prof_launches += 1;
-    }
-    E->ev_used = 0;
-  }
   E->finished_ids.clear();
   if (s->m->stage != 3) {
     set_error(E->ctx, "solve did not finish");
@@ -1311,22 +1283,48 @@ int w90gpu_session_history(w90gpu_session* s, double* out, int rows) {
   return n;
 }
 
-int w90gpu_session_profile(w90gpu_session* s, int enable) {
-  if (!s) return W90GPU_ERR_INPUT;
+int w90gpu_session_rotation_time(w90gpu_session* s, int reps, double* avg_ms) {
+  if (!s || reps < 1 || !avg_ms) return W90GPU_ERR_INPUT;
   Engine* E = s->eng;
-  E->prof = enable != 0;
-  E->prof_ms = 0.0;
-  E->prof_launches = 0;
-  E->ev_used = 0;
-  return W90GPU_OK;
-}
-
-int w90gpu_session_kernel_time(w90gpu_session* s, double* avg_ms, int* launches) {
-  if (!s) return W90GPU_ERR_INPUT;
-  const Engine* E = s->eng;
-  if (avg_ms) *avg_ms = E->prof_launches ? E->prof_ms / E->prof_launches : 0.0;
-  if (launches) *launches = E->prof_launches;
-  return W90GPU_OK;
+  if (s->m->stage == 1 || !s->loaded) {
+    set_error(E->ctx, "rotation timing needs a solved session");
+    return W90GPU_ERR_INPUT;
+  }
+  Lists L;
+  int rc = single_lists(E, s->m, L);
+  if (rc) return rc;
+  // the rotation of the final state, repeated: each launch reads M(cur) and
+  // E and rewrites the other buffer, so every repetition does the same work.
+  // The launches are replayed from a graph, as in the solve.
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  HIPCHK(E->ctx, hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < reps && rc == W90GPU_OK; ++i) rc = launch_rotate(E, L, MODE_BENCH);
+  const hipError_t ec = hipStreamEndCapture(E->stream, &g);
+  if (rc || ec != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    set_error(E->ctx, "capture of the rotation timing failed");
+    return W90GPU_ERR_FATAL;
+  }
+  auto done = [&](int code) {
+    if (ge) (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
+    return code;
+  };
+  if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+    set_error(E->ctx, "instantiation of the rotation timing graph failed");
+    return done(W90GPU_ERR_FATAL);
+  }
+  float ms = 0.f;
+  if (hipGraphLaunch(ge, E->stream) != hipSuccess ||  // warm-up
+      hipEventRecord(E->t0, E->stream) != hipSuccess || hipGraphLaunch(ge, E->stream) != hipSuccess ||
+      hipEventRecord(E->t1, E->stream) != hipSuccess || hipEventSynchronize(E->t1) != hipSuccess ||
+      hipEventElapsedTime(&ms, E->t0, E->t1) != hipSuccess) {
+    set_error(E->ctx, "rotation timing failed");
+    return done(W90GPU_ERR_FATAL);
+  }
+  *avg_ms = ms / reps;
+  return done(W90GPU_OK);
 }
 
 int w90gpu_wann_main(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90gpu_control* ctl, double* m_matrix,

@@ -44,7 +44,7 @@ typedef double2 cplx;
 #define W90_NT 256          // threads per workgroup of the per-k kernels
 #define W90_FINAL_NT 1024   // threads of the per-problem omega kernel
 #define W90_MAX_GROUPS 32   // k-ranges of the first omega reduction level
-#define W90_MIXED_RTOL 1e-6 // relative FP32 noise floor of Omega in the bulk phase
+#define W90_MIXED_RTOL 1e-8 // relative FP32 noise floor of Omega differences in the bulk phase
 
 // --------------------------------------------------------------- helpers
 
@@ -144,7 +144,9 @@ struct Spread {
   double om_i, om_d, om_od, om_tot;
 };
 
-enum Mode { MODE_INIT = 0, MODE_TRIAL = 1, MODE_SECOND = 2, MODE_REMIX = 3 };
+// MODE_BENCH: a trial-step rotation that runs whatever the loop state
+// (timing of the rotation kernel on a solved problem)
+enum Mode { MODE_INIT = 0, MODE_TRIAL = 1, MODE_SECOND = 2, MODE_REMIX = 3, MODE_BENCH = 4 };
 
 // Per-problem solver state, resident in device memory.
 struct State {
@@ -153,7 +155,7 @@ struct State {
   double conv_tol, trial_step, fixed_step, wbtot;
   // loop state of wann_main
   int iter;        // completed iterations
-  int ncg;
+  int ncg_s[2];    // CG step counter, double-buffered by iteration parity (see search_control)
   int cur;         // buffer holding the current U, M
   int done;        // loop finished (converged or num_iter)
   int converged;
@@ -165,8 +167,8 @@ struct State {
   int hold;        // bulk phase ended, waiting for the FP64 rebuild of M
   int bulk_iter;   // iterations done in the bulk phase
   int stall;       // consecutive rejected bulk steps
-  int pad0;
-  double gcnorm0, gcnorm1, gcfac, doda0, alphamin, falphamin;
+  double gcnorm0_s[2];  // previous squared gradient norm, double-buffered like ncg_s
+  double gcnorm1, gcfac, doda0, alphamin, falphamin;
   double beta, dir_sign, scale;
   double omega_invariant;
   Spread wann, old, trial;
@@ -239,6 +241,7 @@ __device__ __forceinline__ float2* mbuf<float2>(const Prob& p, int i) {
 }
 
 __device__ __forceinline__ bool gate(const State* st, int mode) {
+  if (mode == MODE_BENCH) return true;
   if (mode == MODE_REMIX) return st->hold != 0 && !st->done;
   if (st->done || st->hold) return false;
   if (mode == MODE_SECOND) return st->second != 0;
@@ -494,120 +497,6 @@ __device__ __forceinline__ void lds_gemm(cplx* C, const cplx* A, const cplx* B, 
   __syncthreads();
 }
 
-// M rotation with the omega partial sums, one workgroup per (problem, k),
-// all operands in LDS (num_wann <= NWM): E_k is loaded once for the nn
-// blocks of k; each block M(k,b) and E_{k+b} are staged in LDS, T = E_k^+ M
-// and R = T E_{k+b} are formed there, R is written once and reduced in
-// registers into the per-k partial vector.  LDS matrices are column-major
-// with leading dimension nw + 1 (conflict-free rows and broadcast columns),
-// E_k stored conjugate-transposed.  MODE_INIT only reduces.
-// (internal_new_u_and_m :1737-1748; wann_omega :2034-2090, :2214-2275)
-template <int NWM, class T2>
-__global__ void __launch_bounds__(W90_NT) k_rotate_k(const Prob* probs, const Item* items, int nitems, int mode) {
-  if (blockIdx.x >= (unsigned)nitems) return;
-  const Item it = items[blockIdx.x];
-  const Prob& p = probs[it.prob];
-  const State* st = p.st;
-  if (!gate(st, mode)) return;
-  constexpr int SL = (NWM * NWM + W90_NT - 1) / W90_NT;
-  const int nw = p.nw, nw2 = p.nw2, nn = p.nn;
-  const int ld = nw + 1;
-  const int k = it.k;
-  const int tid = threadIdx.x;
-  const int cur = st->cur;
-  const bool rot = (mode != MODE_INIT);
-  const int dst = dst_buf(mode, cur);
-  const T2* src0 = rot_src<T2>(p, mode, cur);
-  const cplx* F = rot_factor(p, mode, cur);
-  __shared__ T2 EkH[NWM * (NWM + 1)];
-  __shared__ T2 Eb[NWM * (NWM + 1)];
-  __shared__ T2 Ms[NWM * (NWM + 1)];
-  __shared__ T2 T[NWM * (NWM + 1)];
-  __shared__ double red[W90_NT];
-
-  DiagAcc acc[SL];
-  double r0[SL][3];
-#pragma unroll
-  for (int s = 0; s < SL; ++s) {
-    acc[s] = DiagAcc{};
-    r0[s][0] = r0[s][1] = r0[s][2] = 0.0;
-    const int e = tid + s * W90_NT;
-    if (e < nw2 && e % nw == e / nw) ref_centre(p, mode, cur, e % nw, r0[s]);
-  }
-  if (rot) {
-    const cplx* ek = F + (size_t)k * nw2;
-    for (int e = tid; e < nw2; e += W90_NT) EkH[(e / nw) + (e % nw) * ld] = c_narrow<T2>(c_conj(ek[e]));
-  }
-  double s_od = 0.0, s_all = 0.0;
-  for (int b = 0; b < nn; ++b) {
-    const size_t kb = (size_t)k * nn + b;
-    const T2* src = src0 + kb * nw2;
-    __syncthreads();
-    for (int e = tid; e < nw2; e += W90_NT) Ms[(e % nw) + (e / nw) * ld] = src[e];
-    if (rot) {
-      const cplx* eb = F + (size_t)p.nnl[kb] * nw2;
-      for (int e = tid; e < nw2; e += W90_NT) Eb[(e % nw) + (e / nw) * ld] = c_narrow<T2>(eb[e]);
-    }
-    __syncthreads();
-    if (rot) {
-#pragma unroll
-      for (int s = 0; s < SL; ++s) {
-        const int e = tid + s * W90_NT;
-        if (e < nw2) {
-          const int i = e % nw, j = e / nw;
-          T2 a = c_zero<T2>();
-          for (int l = 0; l < nw; ++l) a = c_add(a, c_mul(EkH[i + l * ld], Ms[l + j * ld]));
-          T[i + j * ld] = a;
-        }
-      }
-      __syncthreads();
-    }
-    const double w = p.wb[b];
-    const double* bv = p.bk + kb * 3;
-    T2* out = mbuf<T2>(p, dst) + kb * nw2;
-#pragma unroll
-    for (int s = 0; s < SL; ++s) {
-      const int e = tid + s * W90_NT;
-      if (e < nw2) {
-        const int i = e % nw, j = e / nw;
-        T2 r;
-        if (rot) {
-          r = c_zero<T2>();
-          for (int l = 0; l < nw; ++l) r = c_add(r, c_mul(T[i + l * ld], Eb[l + j * ld]));
-          out[e] = r;
-        } else {
-          r = Ms[i + j * ld];
-        }
-        const cplx rw = c_wide(r);
-        const double a2 = c_abs2(rw);
-        s_all += w * a2;
-        if (i != j) {
-          s_od += w * a2;
-        } else {
-          const double l = atan2(rw.y, rw.x);
-          p.ln[dst][kb * nw + i] = l;
-          diag_add(acc[s], w, bv, l, a2, r0[s][0], r0[s][1], r0[s][2]);
-        }
-      }
-    }
-  }
-  double* outk = p.kpart + (size_t)k * p.psz;
-#pragma unroll
-  for (int s = 0; s < SL; ++s) {
-    const int e = tid + s * W90_NT;
-    if (e < nw2 && e % nw == e / nw) diag_store(outk, nw, e % nw, acc[s]);
-  }
-  s_od = block_sum(s_od, red);
-  s_all = block_sum(s_all, red);
-  if (tid == 0) {
-    double wsum = 0.0;
-    for (int b = 0; b < nn; ++b) wsum += p.wb[b];
-    outk[8 * nw + 0] = s_od;
-    outk[8 * nw + 1] = (double)nw * wsum - s_all;
-    kpart_bmat(p, k, outk + 8 * nw);
-  }
-}
-
 // ---------------------------------------------------------- omega / control
 
 // internal_test_convergence (wannierise.F90:1076-1164) with the noise branch
@@ -661,10 +550,10 @@ __device__ void accept_step(State* st, const Spread& sp, int dst, double* logrow
       st->stall = 0;
     } else {
       st->stall += 1;
-      st->ncg = st->num_cg_steps;
     }
     st->iter += 1;
     st->bulk_iter = st->iter;
+    if (!progress) st->ncg_s[st->iter & 1] = st->num_cg_steps;  // next search restarts CG
     if (logrow) {
       logrow[0] = st->wann.om_tot;
       logrow[1] = st->wann.om_d;
@@ -741,7 +630,7 @@ __device__ void omega_control(State* st, const Prob& p, double sod, double si, d
     st->cur = dst;
     st->phase = 1;
     st->hold = 0;
-    st->ncg = st->num_cg_steps;
+    st->ncg_s[st->iter & 1] = st->num_cg_steps;
     st->conv_count = 0;
     for (int j = 0; j < W90GPU_MAX_CONV_WINDOW; ++j) st->history[j] = 0.0;
     if (p.log) {
@@ -839,8 +728,18 @@ __global__ void __launch_bounds__(W90_FINAL_NT) k_omega_final(const Prob* probs,
     const int v = v0 + tid % VB;
     const int j = tid / VB;
     double s = 0.0;
-    if (j < J && v < psz)
-      for (int g = j; g < G; g += J) s += rows[(size_t)g * psz + v];
+    if (j < J && v < psz) {
+      // rows j, j+J, ...: eight independent partial sums keep eight loads
+      // in flight, combined in a fixed order
+      double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      int g = j;
+      for (; g + 7 * J < G; g += 8 * J) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] += rows[(size_t)(g + u * J) * psz + v];
+      }
+      for (int u = 0; g < G; g += J, ++u) a[u] += rows[(size_t)g * psz + v];
+      s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    }
     red[tid] = s;
     __syncthreads();
     if (tid < VB && v0 + tid < psz) {
@@ -1000,10 +899,28 @@ __global__ void __launch_bounds__(domega_nt<NWM>()) k_domega_k(const Prob* probs
 #pragma unroll
   for (int s = 0; s < SL; ++s) acc[s] = c_make(0.0, 0.0);
   const T2* Mk = mbuf<T2>(p, cur) + (size_t)k * nn * nw2;
+  // block b+1 is loaded into registers while block b is used from LDS
+  T2 nxt[SL];
+#pragma unroll
+  for (int s = 0; s < SL; ++s) {
+    const int e = tid + s * NT;
+    if (e < nw2) nxt[s] = Mk[e];
+  }
   for (int b = 0; b < nn; ++b) {
-    const T2* src = Mk + (size_t)b * nw2;
     __syncthreads();
-    for (int e = tid; e < nw2; e += NT) Ms[(e % nw) + (e / nw) * ld] = src[e];
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int e = tid + s * NT;
+      if (e < nw2) Ms[(e % nw) + (e / nw) * ld] = nxt[s];
+    }
+    if (b + 1 < nn) {
+      const T2* src = Mk + (size_t)(b + 1) * nw2;
+#pragma unroll
+      for (int s = 0; s < SL; ++s) {
+        const int e = tid + s * NT;
+        if (e < nw2) nxt[s] = src[e];
+      }
+    }
     __syncthreads();
     const double w = p.wb[b];
 #pragma unroll
@@ -1039,37 +956,43 @@ __global__ void __launch_bounds__(domega_nt<NWM>()) k_domega_k(const Prob* probs
 }
 
 // Scalar part of internal_search_direction (wannierise.F90:1371-1525) on
-// the state copy: gg = Re<G,G>, gd = Re<G,D_old>.
+// a state copy: gg = Re<G,G>, gd = Re<G,D_old>.  Every workgroup of the
+// trial-step kernel runs it on its own copy; the CG counter and the previous
+// gradient norm are read from the slot of this iteration's parity and
+// written to the other one, so the workgroup that stores the results never
+// changes what the others read.
 __device__ void search_control(State* st, double gg, double gd) {
   const int iter = st->iter + 1;
+  const int sl = st->iter & 1, nx = sl ^ 1;
+  int ncg = st->ncg_s[sl];
   st->second = 0;
   st->gcnorm1 = gg;
   double gcfac;
-  if (iter == 1 || st->ncg >= st->num_cg_steps) {
+  if (iter == 1 || ncg >= st->num_cg_steps) {
     gcfac = 0.0;
-    st->ncg = 0;
+    ncg = 0;
   } else {
-    if (st->gcnorm0 > 2.220446049250313e-16) {
-      gcfac = gg / st->gcnorm0;
+    if (st->gcnorm0_s[sl] > 2.220446049250313e-16) {
+      gcfac = gg / st->gcnorm0_s[sl];
       if (gcfac > 3.0) {
         gcfac = 0.0;
-        st->ncg = 0;
+        ncg = 0;
       } else {
-        st->ncg += 1;
+        ncg += 1;
       }
     } else {
       gcfac = 0.0;
-      st->ncg = 0;
+      ncg = 0;
     }
   }
-  st->gcnorm0 = gg;
+  st->gcnorm0_s[nx] = gg;
   const double w4 = 4.0 * st->wbtot;
   double doda0 = -(gg + gcfac * gd) / w4;
   double beta = gcfac, sign = 1.0;
   if (doda0 > 0.0) {
-    if (st->ncg > 0) {
+    if (ncg > 0) {
       beta = 0.0;
-      st->ncg = 0;
+      ncg = 0;
       gcfac = 0.0;
       doda0 = -gg / w4;
       if (doda0 > 0.0) {
@@ -1081,6 +1004,7 @@ __device__ void search_control(State* st, double gg, double gd) {
       doda0 = -doda0;
     }
   }
+  st->ncg_s[nx] = ncg;
   st->gcfac = gcfac;
   st->beta = beta;
   st->dir_sign = sign;
@@ -1089,38 +1013,43 @@ __device__ void search_control(State* st, double gg, double gd) {
 }
 
 // internal_search_direction (wannierise.F90:1371-1525) without
-// preconditioning or noise, one workgroup per problem: Fletcher-Reeves
-// coefficient with its resets, slope along D and the uphill handling.  D
-// itself is formed by the step kernel.
-__global__ void __launch_bounds__(W90_NT) k_search(const Prob* probs, const int* plist, int np) {
-  if (blockIdx.x >= (unsigned)np) return;
-  const Prob& p = probs[plist[blockIdx.x]];
-  // state in LDS, as in k_omega_final
-  __shared__ State sst;
+// preconditioning or noise, fused into the trial-step kernels: each
+// workgroup sums Re<G,G> and Re<G,D_old> over k in a fixed order (every
+// workgroup of a problem gets the same sums), runs the scalar part on a copy
+// of the state in LDS and forms its D from the result; the workgroup of
+// k = 0 stores the results.  Called by all threads of the block; blockDim
+// must be W90_NT (the summation order depends on it).
+__device__ void search_in_step(const Prob& p, int k, State* lst, double* red) {
   constexpr int NSD = sizeof(State) / sizeof(double);
   for (int i = threadIdx.x; i < NSD; i += blockDim.x)
-    reinterpret_cast<double*>(&sst)[i] = reinterpret_cast<const double*>(p.st)[i];
-  __syncthreads();
-  State* st = &sst;
-  if (st->done || st->hold) return;
-  __shared__ double red[W90_NT];
+    reinterpret_cast<double*>(lst)[i] = reinterpret_cast<const double*>(p.st)[i];
   double gg = 0.0, gd = 0.0;
-  // fixed-order chunked sums over k
   const int per = (p.nk + blockDim.x - 1) / blockDim.x;
   const int kb = threadIdx.x * per, ke = min(p.nk, kb + per);
-  for (int k = kb; k < ke; ++k) {
-    gg += p.part_gg[k];
-    gd += p.part_gd[k];
+  for (int q = kb; q < ke; ++q) {
+    gg += p.part_gg[q];
+    gd += p.part_gd[q];
   }
-  gg = block_sum(gg, red);
+  gg = block_sum(gg, red);  // (its barriers also order the copy above)
   gd = block_sum(gd, red);
-  if (threadIdx.x == 0) search_control(st, gg, gd);
+  if (threadIdx.x == 0) {
+    search_control(lst, gg, gd);
+    if (k == 0) {
+      State* st = p.st;
+      const int nx = (lst->iter & 1) ^ 1;
+      st->second = lst->second;
+      st->gcnorm1 = lst->gcnorm1;
+      st->ncg_s[nx] = lst->ncg_s[nx];
+      st->gcnorm0_s[nx] = lst->gcnorm0_s[nx];
+      st->gcfac = lst->gcfac;
+      st->beta = lst->beta;
+      st->dir_sign = lst->dir_sign;
+      st->doda0 = lst->doda0;
+      st->scale = lst->scale;
+    }
+  }
   __syncthreads();
-  for (int i = threadIdx.x; i < NSD; i += blockDim.x)
-    reinterpret_cast<double*>(p.st)[i] = reinterpret_cast<const double*>(&sst)[i];
 }
-
-
 
 // ---------------------------------------------------------- retraction
 
@@ -1206,10 +1135,16 @@ __global__ void __launch_bounds__(W90_NT) k_step(const Prob* probs, const Item* 
   cplx* X = p.W1 + (size_t)k * nw2;
   cplx* P = p.W2 + (size_t)k * nw2;
   cplx* Ek = p.E + (size_t)k * nw2;
-  const double scale = st->scale;
+  __shared__ State lst;
+  const State* ss = st;
+  if (mode == MODE_TRIAL) {
+    search_in_step(p, k, &lst, red);
+    ss = &lst;
+  }
+  const double scale = ss->scale;
   if (mode == MODE_TRIAL) {
     const cplx* Gk = p.G + (size_t)k * nw2;
-    const double beta = st->beta, sign = st->dir_sign;
+    const double beta = ss->beta, sign = ss->dir_sign;
     for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
       cplx d = c_scale(c_add(Gk[e], c_scale(Dk[e], beta)), sign);
       Dk[e] = d;
@@ -1282,10 +1217,16 @@ __global__ void __launch_bounds__(W90_NT) k_step_k(const Prob* probs, const Item
   __shared__ cplx T[NWM * (NWM + 1)];
   __shared__ double red[W90_NT];
   cplx* Dk = p.D + (size_t)k * nw2;
-  const double scale = st->scale;
+  __shared__ State lst;
+  const State* ss = st;
+  if (mode == MODE_TRIAL) {
+    search_in_step(p, k, &lst, red);
+    ss = &lst;
+  }
+  const double scale = ss->scale;
   if (mode == MODE_TRIAL) {
     const cplx* Gk = p.G + (size_t)k * nw2;
-    const double beta = st->beta, sign = st->dir_sign;
+    const double beta = ss->beta, sign = ss->dir_sign;
     for (int e = tid; e < nw2; e += W90_NT) {
       cplx d = c_scale(c_add(Gk[e], c_scale(Dk[e], beta)), sign);
       Dk[e] = d;

@@ -4,9 +4,10 @@
 // path and FP32 with FP32 accumulation for the mixed bulk phase.
 //
 // A workgroup computes a GT x GT tile of one product for one (problem, k,
-// b): gridDim.y covers b and the tiles.  Four wavefronts hold 32 x 32 of the
-// tile each, as 2 x 2 fragments of 16 x 16 with the real and imaginary parts
-// in separate accumulators: a complex product is four real MFMA products.
+// b): gridDim.y covers b and the tiles; GT = 64, or 32 for num_wann <= 32.
+// Four wavefronts hold a GT/2 x GT/2 quadrant of the tile each, as 16 x 16
+// fragments with the real and imaginary parts in separate accumulators: a
+// complex product is four real MFMA products.
 // K runs in chunks of GK staged in LDS with coalesced global loads (planar
 // re/im, padded rows).  Fragments wholly outside num_wann are skipped.
 //
@@ -24,7 +25,6 @@
 
 #include "w90gpu_kernels.h"
 
-#define W90_GT 64  // output tile edge of a workgroup
 #define W90_GK 16  // K chunk staged in LDS
 
 typedef double w90_d4 __attribute__((ext_vector_type(4)));
@@ -61,14 +61,18 @@ struct Real<float2> {
   typedef float type;
 };
 
-// tiles of W90_GT per problem
+// output tile edge of a workgroup: 32 up to num_wann 32, 64 above
+__host__ __device__ __forceinline__ int gemm_edge(int nw) { return nw <= 32 ? 32 : 64; }
+
+// GEMM tiles per block M(k,b)
 __host__ __device__ __forceinline__ int gemm_tiles(int nw) {
-  const int t = (nw + W90_GT - 1) / W90_GT;
+  const int t = (nw + gemm_edge(nw) - 1) / gemm_edge(nw);
   return t * t;
 }
 
-template <class T2, int STAGE>
+template <class T2, int STAGE, int GT>
 __global__ void __launch_bounds__(256) k_rot_gemm(const Prob* probs, const Item* items, int nitems, int mode) {
+  constexpr int FR = GT / 32;  // fragments per wavefront along rows and columns
   typedef typename Real<T2>::type R;
   typedef typename Mfma<R>::acc Acc;
   if (blockIdx.x >= (unsigned)nitems) return;
@@ -77,12 +81,12 @@ __global__ void __launch_bounds__(256) k_rot_gemm(const Prob* probs, const Item*
   const State* st = p.st;
   if (!gate(st, mode)) return;
   const int nw = p.nw, nn = p.nn;
-  const int nt1 = (nw + W90_GT - 1) / W90_GT;
+  const int nt1 = (nw + GT - 1) / GT;
   const int ntl = nt1 * nt1;
   const int by = blockIdx.y;
   if (by >= nn * ntl) return;
   const int b = by / ntl, tile = by % ntl;
-  const int row0 = (tile % nt1) * W90_GT, col0 = (tile / nt1) * W90_GT;
+  const int row0 = (tile % nt1) * GT, col0 = (tile / nt1) * GT;
   const int k = it.k;
   const size_t kb = (size_t)k * nn + b;
   const size_t nw2 = p.nw2;
@@ -93,25 +97,25 @@ __global__ void __launch_bounds__(256) k_rot_gemm(const Prob* probs, const Item*
   const cplx* Ekb = p.E + (size_t)p.nnl[kb] * nw2;
   const T2* src = rot_src<T2>(p, mode, cur) + kb * nw2;
 
-  __shared__ R As[2][W90_GK][W90_GT + 1];  // [re/im][k][row]
-  __shared__ R Bs[2][W90_GK][W90_GT + 1];  // [re/im][k][col]
+  __shared__ R As[2][W90_GK][GT + 1];  // [re/im][k][row]
+  __shared__ R Bs[2][W90_GK][GT + 1];  // [re/im][k][col]
   __shared__ double red[256];
 
   const int tid = threadIdx.x;
   const int w = tid >> 6, lane = tid & 63;
-  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  const int wr = (w >> 1) * (GT / 2), wc = (w & 1) * (GT / 2);
   const int lr = lane & 15, lk = lane >> 4;
-  bool fv_r[2], fv_c[2];
+  bool fv_r[FR], fv_c[FR];
 #pragma unroll
-  for (int f = 0; f < 2; ++f) {
+  for (int f = 0; f < FR; ++f) {
     fv_r[f] = row0 + wr + f * 16 < nw;
     fv_c[f] = col0 + wc + f * 16 < nw;
   }
-  Acc Cr[2][2], Ci[2][2];
+  Acc Cr[FR][FR], Ci[FR][FR];
 #pragma unroll
-  for (int fi = 0; fi < 2; ++fi)
+  for (int fi = 0; fi < FR; ++fi)
 #pragma unroll
-    for (int fj = 0; fj < 2; ++fj)
+    for (int fj = 0; fj < FR; ++fj)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         Cr[fi][fj][i] = 0;
@@ -121,7 +125,7 @@ __global__ void __launch_bounds__(256) k_rot_gemm(const Prob* probs, const Item*
   for (int kk = 0; kk < nw; kk += W90_GK) {
     __syncthreads();
     // A chunk: rows row0.., k = kk..kk+GK
-    for (int e = tid; e < W90_GK * W90_GT; e += 256) {
+    for (int e = tid; e < W90_GK * GT; e += 256) {
       cplx v;
       int l, i;
       if (STAGE == 1) {
@@ -133,8 +137,8 @@ __global__ void __launch_bounds__(256) k_rot_gemm(const Prob* probs, const Item*
         v.y = -v.y;
       } else {
         // A = T: A[i][l] = T[i + l nw]; threads run along i
-        i = e % W90_GT;
-        l = e / W90_GT;
+        i = e % GT;
+        l = e / GT;
         const int gi = row0 + i, gl = kk + l;
         v = (gi < nw && gl < nw) ? c_wide(Tm[gi + (size_t)gl * nw]) : c_make(0.0, 0.0);
       }
@@ -142,7 +146,7 @@ __global__ void __launch_bounds__(256) k_rot_gemm(const Prob* probs, const Item*
       As[1][l][i] = (R)v.y;
     }
     // B chunk: B[l][j] = Bsrc[l + j nw]; threads run along l
-    for (int e = tid; e < W90_GK * W90_GT; e += 256) {
+    for (int e = tid; e < W90_GK * GT; e += 256) {
       const int l = e % W90_GK, j = e / W90_GK;
       const int gl = kk + l, gj = col0 + j;
       cplx v = c_make(0.0, 0.0);
@@ -153,19 +157,19 @@ __global__ void __launch_bounds__(256) k_rot_gemm(const Prob* probs, const Item*
     __syncthreads();
 #pragma unroll
     for (int ks = 0; ks < W90_GK; ks += 4) {
-      R ar[2], ai[2], br[2], bi[2];
+      R ar[FR], ai[FR], br[FR], bi[FR];
 #pragma unroll
-      for (int f = 0; f < 2; ++f) {
+      for (int f = 0; f < FR; ++f) {
         ar[f] = As[0][ks + lk][wr + f * 16 + lr];
         ai[f] = As[1][ks + lk][wr + f * 16 + lr];
         br[f] = Bs[0][ks + lk][wc + f * 16 + lr];
         bi[f] = Bs[1][ks + lk][wc + f * 16 + lr];
       }
 #pragma unroll
-      for (int fi = 0; fi < 2; ++fi) {
+      for (int fi = 0; fi < FR; ++fi) {
         if (!fv_r[fi]) continue;
 #pragma unroll
-        for (int fj = 0; fj < 2; ++fj) {
+        for (int fj = 0; fj < FR; ++fj) {
           if (!fv_c[fj]) continue;
           Cr[fi][fj] = Mfma<R>::mma(ar[fi], br[fj], Cr[fi][fj]);
           Cr[fi][fj] = Mfma<R>::mma(-ai[fi], bi[fj], Cr[fi][fj]);
@@ -179,9 +183,9 @@ __global__ void __launch_bounds__(256) k_rot_gemm(const Prob* probs, const Item*
   double od = 0.0, all = 0.0;
   T2* out = (STAGE == 1) ? Tm : mbuf<T2>(p, dst) + kb * nw2;
 #pragma unroll
-  for (int fi = 0; fi < 2; ++fi)
+  for (int fi = 0; fi < FR; ++fi)
 #pragma unroll
-    for (int fj = 0; fj < 2; ++fj)
+    for (int fj = 0; fj < FR; ++fj)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = row0 + wr + fi * 16 + Mfma<R>::row(lk, i);
@@ -216,7 +220,7 @@ __global__ void __launch_bounds__(256) k_rot_gemm(const Prob* probs, const Item*
 
 // Omega partial sums of the current M by GEMM tile (MODE_INIT of the MFMA
 // class): the layout the stage-2 epilogue writes, without a rotation.
-template <class T2>
+template <class T2, int GT>
 __global__ void __launch_bounds__(256) k_reduce_tiles(const Prob* probs, const Item* items, int nitems, int mode) {
   if (blockIdx.x >= (unsigned)nitems) return;
   const Item it = items[blockIdx.x];
@@ -224,20 +228,20 @@ __global__ void __launch_bounds__(256) k_reduce_tiles(const Prob* probs, const I
   const State* st = p.st;
   if (!gate(st, mode)) return;
   const int nw = p.nw, nn = p.nn;
-  const int nt1 = (nw + W90_GT - 1) / W90_GT;
+  const int nt1 = (nw + GT - 1) / GT;
   const int ntl = nt1 * nt1;
   const int by = blockIdx.y;
   if (by >= nn * ntl) return;
   const int b = by / ntl, tile = by % ntl;
-  const int row0 = (tile % nt1) * W90_GT, col0 = (tile / nt1) * W90_GT;
+  const int row0 = (tile % nt1) * GT, col0 = (tile / nt1) * GT;
   const size_t kb = (size_t)it.k * nn + b;
   const int cur = st->cur;
   const int dst = dst_buf(mode, cur);
   const T2* m = mbuf<T2>(p, cur) + kb * p.nw2;
   __shared__ double red[256];
   double od = 0.0, all = 0.0;
-  for (int e = threadIdx.x; e < W90_GT * W90_GT; e += blockDim.x) {
-    const int r = row0 + e % W90_GT, c = col0 + e / W90_GT;
+  for (int e = threadIdx.x; e < GT * GT; e += blockDim.x) {
+    const int r = row0 + e % GT, c = col0 + e / GT;
     if (r >= nw || c >= nw) continue;
     const cplx v = c_wide(m[r + (size_t)c * nw]);
     const double a2 = c_abs2(v);
@@ -344,7 +348,13 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
   double* Pi = Pr + W90_STEP_MAT;
   double* red = Pi + W90_STEP_MAT;
   cplx* Dk = p.D + (size_t)k * nw2;
-  const double scale = st->scale;
+  __shared__ State lst;
+  const State* ss = st;
+  if (mode == MODE_TRIAL) {
+    search_in_step(p, k, &lst, red);
+    ss = &lst;
+  }
+  const double scale = ss->scale;
   // X = scale D (zero outside nw), D formed on the trial step
   for (int e = tid; e < W90_STEP_N * W90_STEP_N; e += 256) {
     const int r = e % W90_STEP_N, c = e / W90_STEP_N;
@@ -353,7 +363,7 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
       const int g = r + c * nw;
       cplx d = Dk[g];
       if (mode == MODE_TRIAL) {
-        d = c_scale(c_add(p.G[(size_t)k * nw2 + g], c_scale(d, st->beta)), st->dir_sign);
+        d = c_scale(c_add(p.G[(size_t)k * nw2 + g], c_scale(d, ss->beta)), ss->dir_sign);
         Dk[g] = d;
       }
       x = c_scale(d, scale);
