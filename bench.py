@@ -61,6 +61,7 @@ def parse():
     ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second (0: from a probe)")
     ap.add_argument("--num-iter", type=int, default=500)
     ap.add_argument("--pool-gb", type=float, default=160.0, help="serve: device pool for admission")
+    ap.add_argument("--max-running", type=int, default=16, help="serve: problems in flight at most")
     return ap.parse_args()
 
 

@@ -179,6 +179,8 @@ def run_batch(args):
     t_b64, res64 = batched(False)
     exact = sum(1 for (a, ca, _), (b, cb, _) in zip(plain, res64)
                 if a.omega_total == b.omega_total and a.iterations == b.iterations and np.array_equal(ca, cb))
+    ok64 = sum(1 for (a, ca, _), (b, cb, _) in zip(plain, res64)
+               if abs(a.omega_total - b.omega_total) <= 1e-8 and float(np.abs(ca - cb).max()) <= 1e-6)
     log(f"[batch] batched FP64: {t_b64:.2f}s, {exact}/{n} bit-identical to one-per-launch")
     t_mix, resm = batched(True)
     d_om = [abs(a.omega_total - b.omega_total) for (a, _, _), (b, _, _) in zip(plain, resm)]
@@ -190,9 +192,12 @@ def run_batch(args):
     log(f"[batch] batched mixed: {t_mix:.2f}s; within 1e-8/1e-6: {sum(ok)}/{n}, "
         f"same WF set: {sum(ok_set)}/{n}; plain converged {conv}/{n}")
     its = [r.iterations for r, _, _ in plain]
+    # the value is the faster batched form counting only solves within
+    # 1e-8 Ang^2 (and centres 1e-6 Ang) of the one-per-launch FP64 solve
+    best = max((ok64 / t_b64, "batched fp64"), (sum(ok) / t_mix, "batched mixed"))
     return {
         "metric": METRIC,
-        "value": sum(ok) / t_mix,
+        "value": best[0],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": 1,
@@ -201,9 +206,10 @@ def run_batch(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f64" if best[1] == "batched fp64" else "f32+f64",
         "data": "synthetic",
         "config": {
+            "value_from": best[1],
             "workload": f"ragged batch of {n} problems: sc/fcc/bcc/hex, num_wann U[4,48], k-mesh "
                         f"4^3/6^3/8^3, num_iter={args.num_iter}, conv_window=3, conv_tol=1e-10",
             "plain_fp64_one_per_launch_s": t_plain,
@@ -212,6 +218,7 @@ def run_batch(args):
             "speedup_batched_mixed_vs_plain": t_plain / t_mix,
             "speedup_batched_fp64_vs_plain": t_plain / t_b64,
             "batched_fp64_bit_identical": exact,
+            "batched_fp64_within_1e8": ok64,
             "mixed_within_1e8_omega_1e6_centres": sum(ok),
             "mixed_same_wannier_set": sum(ok_set),
             "max_delta_omega": max(d_om),
@@ -251,8 +258,12 @@ def run_serve(args):
         lat_serial.append(fin - a)
     log(f"[serve] {n} requests, serial capacity {cap:.2f}/s, arrival rate {rate:.2f}/s")
 
-    # continuous batching, mixed precision, admission by device bytes
-    ctl = control(args.num_iter, True)
+    # continuous batching, admission by device bytes and a cap on the
+    # problems in flight; FP64 unless --precision mixed (FP64 results are
+    # bit-identical to the same request solved alone)
+    mixed = getattr(args, "precision", "fp64") == "mixed"
+    ctl = control(args.num_iter, mixed)
+    cap_running = max(1, int(getattr(args, "max_running", 16)))
     need = [capi.problem_bytes(p["nw"], p["nk"], p["nn"], args.num_iter) for p in probs]
     pool = max(max(need), int(args.pool_gb * (1 << 30)))
     b = ctx.batch(pool)
@@ -265,7 +276,7 @@ def run_serve(args):
         while nxt < n and arrivals[nxt] <= t:
             queue.append(nxt)
             nxt += 1
-        while queue:
+        while queue and len(owner) < cap_running:
             i = queue[0]
             p = probs[i]
             h = b.add(kmesh_of(p), p["m0"], p["u0"], ctl)
@@ -279,7 +290,8 @@ def run_serve(args):
             if nxt < n:
                 time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - start)))
             continue
-        finished = b.step(1)
+        # a few blocks per step while nobody is waiting to be admitted
+        finished = b.step(1 if queue or (nxt < n and arrivals[nxt] <= time.perf_counter() - start) else 2)
         t = time.perf_counter() - start
         for h in finished:
             i = owner.pop(h)
@@ -310,13 +322,13 @@ def run_serve(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic",
         "config": {
             "workload": f"open-loop Poisson serving, {n} requests at {rate:.3f}/s: sc/fcc/bcc/hex, num_wann "
                         f"U[4,96], k-mesh 4^3..12^3, num_iter={args.num_iter}, conv_window=3, conv_tol=1e-10, "
-                        f"mixed precision, continuous batching, admission by device bytes (pool "
-                        f"{pool / (1 << 30):.0f} GiB)",
+                        f"{'mixed precision' if mixed else 'FP64'}, continuous batching of up to {cap_running} "
+                        f"problems, admission by device bytes (pool {pool / (1 << 30):.0f} GiB)",
             "p50_s": pct(lat, 0.5),
             "p99_s": pct(lat, 0.99),
             "throughput_solves_s": thr,
