@@ -249,11 +249,18 @@ def main():
         achieved = rot_flops / (avg_ms * 1e-3) / 1e12
         roof = dict(bound="mfma", achieved=achieved, peak=peak_flops, unit="TFLOP/s",
                     frac=achieved / peak_flops, traffic=None)
-    kname = ("k_rotate_w16" if nw <= 16 else "k_rotate_k<32>" if nw <= 32 else "k_rot_gemm stage 1+2")
+    kname = "k_rotate_w16" if nw <= 16 else "k_rot_gemm stage 1 + stage 2 + k_kpart"
     roof.update(kernel=kname, avg_launch_ms=avg_ms, launches=launches,
                 algorithmic_bytes=rot_bytes, algorithmic_flops=rot_flops)
     roof["traffic"] = measured_traffic(args.config, args.precision)
     log(f"[bench] k_rotate: {launches} launches, avg {avg_ms * 1e3:.1f} us, AI={ai:.2f} flop/B")
+    # the other phases of an iteration, timed the same way (the omega
+    # reductions and control are the remainder of the per-iteration time)
+    phase_us = {"rotation": avg_ms * 1e3,
+                "gradient": sess.phase_time("gradient", launches) * 1e3,
+                "retraction": sess.phase_time("retraction", launches) * 1e3,
+                "iteration": 1e6 * total / args.steps / max(1, its)}
+    log("[bench] phases (us): " + ", ".join(f"{k} {v:.1f}" for k, v in phase_us.items()))
 
     out = {
         "metric": METRIC,
@@ -273,6 +280,7 @@ def main():
                         "conv_window=3, one problem per launch",
             "precision": args.precision,
             "iterations": its,
+            "phase_us": phase_us,
             "omega_total": omega,
             "p50_ms": 1e3 * lat[len(lat) // 2],
             "p99_ms": 1e3 * lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))],

@@ -93,6 +93,7 @@ SIGNATURES = [
     ("w90gpu_session_bytes", C.c_size_t, [C.c_int, C.c_int, C.c_int]),
     ("w90gpu_session_history", C.c_int, [_vp, _dp, C.c_int]),
     ("w90gpu_session_rotation_time", C.c_int, [_vp, C.c_int, _dp]),
+    ("w90gpu_session_phase_time", C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     ("w90gpu_problem_bytes", C.c_size_t, [C.c_int, C.c_int, C.c_int, C.c_int]),
     ("w90gpu_batch_create", C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
     ("w90gpu_batch_destroy", C.c_int, [_vp]),
@@ -361,6 +362,16 @@ class Session:
         back to back on the final state of the last solve."""
         ms = C.c_double(0.0)
         self.ctx._check(self.lib.w90gpu_session_rotation_time(self.h, reps, C.byref(ms)))
+        return ms.value
+
+    PHASES = {"rotation": 0, "gradient": 1, "retraction": 2}
+
+    def phase_time(self, phase, reps=50):
+        """Average duration (ms) of one phase of an iteration ("rotation",
+        "gradient" or "retraction"), `reps` launches back to back on the
+        final state of the last solve; the solve's results are unchanged."""
+        ms = C.c_double(0.0)
+        self.ctx._check(self.lib.w90gpu_session_phase_time(self.h, self.PHASES[phase], reps, C.byref(ms)))
         return ms.value
 
 

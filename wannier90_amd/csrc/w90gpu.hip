@@ -759,7 +759,7 @@ int launch_step(Engine* E, const Lists& L, int mode) {
   return W90GPU_OK;
 }
 
-int launch_domega(Engine* E, const Lists& L) {
+int launch_domega(Engine* E, const Lists& L, int mode) {
   hipStream_t s = E->stream;
   const Prob* P = E->d_probs;
   for (int c = 0; c < kClasses; ++c)
@@ -770,27 +770,31 @@ int launch_domega(Engine* E, const Lists& L) {
       if (c == 0) {
         if (q == 0)
           hipLaunchKernelGGL((k_domega_k<16, cplx>), dim3(g.n), dim3(domega_nt<16>()), (domega_lds<16, cplx>()), s, P, it,
-                             g.n);
+                             g.n, mode);
         else
           hipLaunchKernelGGL((k_domega_k<16, float2>), dim3(g.n), dim3(domega_nt<16>()), (domega_lds<16, float2>()), s, P,
-                             it, g.n);
+                             it, g.n, mode);
       } else if (c == 1) {
         if (q == 0)
           hipLaunchKernelGGL((k_domega_k<kLdsMaxWann, cplx>), dim3(g.n), dim3(domega_nt<kLdsMaxWann>()),
-                             (domega_lds<kLdsMaxWann, cplx>()), s, P, it, g.n);
+                             (domega_lds<kLdsMaxWann, cplx>()), s, P, it,
+                             g.n, mode);
         else
           hipLaunchKernelGGL((k_domega_k<kLdsMaxWann, float2>), dim3(g.n), dim3(domega_nt<kLdsMaxWann>()),
-                             (domega_lds<kLdsMaxWann, float2>()), s, P, it, g.n);
+                             (domega_lds<kLdsMaxWann, float2>()), s, P, it,
+                             g.n, mode);
       } else if (c == 2) {
         if (q == 0)
           hipLaunchKernelGGL((k_domega_k<W90_STEP_N, cplx>), dim3(g.n), dim3(domega_nt<W90_STEP_N>()),
-                             (domega_lds<W90_STEP_N, cplx>()), s, P, it, g.n);
+                             (domega_lds<W90_STEP_N, cplx>()), s, P, it,
+                             g.n, mode);
         else
           hipLaunchKernelGGL((k_domega_k<W90_STEP_N, float2>), dim3(g.n), dim3(domega_nt<W90_STEP_N>()),
-                             (domega_lds<W90_STEP_N, float2>()), s, P, it, g.n);
+                             (domega_lds<W90_STEP_N, float2>()), s, P, it,
+                             g.n, mode);
       } else {
-        if (q == 0) hipLaunchKernelGGL(k_domega<cplx>, dim3(g.n), dim3(1024), 0, s, P, it, g.n);
-        else hipLaunchKernelGGL(k_domega<float2>, dim3(g.n), dim3(1024), 0, s, P, it, g.n);
+        if (q == 0) hipLaunchKernelGGL(k_domega<cplx>, dim3(g.n), dim3(1024), 0, s, P, it, g.n, mode);
+        else hipLaunchKernelGGL(k_domega<float2>, dim3(g.n), dim3(1024), 0, s, P, it, g.n, mode);
       }
       HIPCHK(E->ctx, hipGetLastError());
     }
@@ -805,7 +809,7 @@ int all_items(const Lists& L) {
 // One iteration of wann_main over the lists (wannierise.F90:555-817).
 int enqueue_iteration(Engine* E, const Lists& L) {
   int rc;
-  if ((rc = launch_domega(E, L))) return rc;
+  if ((rc = launch_domega(E, L, MODE_TRIAL))) return rc;
   // the search direction is formed inside the trial-step kernels
   if ((rc = launch_step(E, L, MODE_TRIAL))) return rc;
   if ((rc = launch_omega(E, L, MODE_TRIAL))) return rc;
@@ -1149,6 +1153,10 @@ int w90gpu_create(int device, w90gpu_ctx** ctx) {
       {(const void*)k_rotate_w16<cplx>, rotate_w16_lds<cplx>(W90GPU_NUM_NNMAX)},
       {(const void*)k_rotate_w16<float2>, rotate_w16_lds<float2>(W90GPU_NUM_NNMAX)},
       {(const void*)k_step_mfma, step_mfma_lds()},
+      {(const void*)k_domega_k<16, cplx>, domega_lds<16, cplx>()},
+      {(const void*)k_domega_k<16, float2>, domega_lds<16, float2>()},
+      {(const void*)k_domega_k<kLdsMaxWann, cplx>, domega_lds<kLdsMaxWann, cplx>()},
+      {(const void*)k_domega_k<kLdsMaxWann, float2>, domega_lds<kLdsMaxWann, float2>()},
       {(const void*)k_domega_k<W90_STEP_N, cplx>, domega_lds<W90_STEP_N, cplx>()},
       {(const void*)k_domega_k<W90_STEP_N, float2>, domega_lds<W90_STEP_N, float2>()},
   };
@@ -1283,27 +1291,34 @@ int w90gpu_session_history(w90gpu_session* s, double* out, int rows) {
   return n;
 }
 
-int w90gpu_session_rotation_time(w90gpu_session* s, int reps, double* avg_ms) {
+int w90gpu_session_phase_time(w90gpu_session* s, int phase, int reps, double* avg_ms) {
   if (!s || reps < 1 || !avg_ms) return W90GPU_ERR_INPUT;
+  if (phase < W90GPU_PHASE_ROTATION || phase > W90GPU_PHASE_RETRACTION) return W90GPU_ERR_INPUT;
   Engine* E = s->eng;
   if (s->m->stage == 1 || !s->loaded) {
-    set_error(E->ctx, "rotation timing needs a solved session");
+    set_error(E->ctx, "phase timing needs a solved session");
     return W90GPU_ERR_INPUT;
   }
   Lists L;
   int rc = single_lists(E, s->m, L);
   if (rc) return rc;
-  // the rotation of the final state, repeated: each launch reads M(cur) and
-  // E and rewrites the other buffer, so every repetition does the same work.
-  // The launches are replayed from a graph, as in the solve.
+  // one phase on the final state, repeated: a rotation reads M(cur) and E
+  // and rewrites the other M buffer, a gradient rewrites G and its norms, a
+  // retraction reads D and U(cur) and rewrites E and the other U buffer, so
+  // every repetition does the same work.  The launches are replayed from a
+  // graph, as in the solve.
   hipGraph_t g = nullptr;
   hipGraphExec_t ge = nullptr;
   HIPCHK(E->ctx, hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
-  for (int i = 0; i < reps && rc == W90GPU_OK; ++i) rc = launch_rotate(E, L, MODE_BENCH);
+  for (int i = 0; i < reps && rc == W90GPU_OK; ++i) {
+    if (phase == W90GPU_PHASE_ROTATION) rc = launch_rotate(E, L, MODE_BENCH);
+    else if (phase == W90GPU_PHASE_GRADIENT) rc = launch_domega(E, L, MODE_BENCH);
+    else rc = launch_step(E, L, MODE_BENCH);
+  }
   const hipError_t ec = hipStreamEndCapture(E->stream, &g);
   if (rc || ec != hipSuccess || !g) {
     if (g) (void)hipGraphDestroy(g);
-    set_error(E->ctx, "capture of the rotation timing failed");
+    set_error(E->ctx, "capture of the phase timing failed");
     return W90GPU_ERR_FATAL;
   }
   auto done = [&](int code) {
@@ -1312,7 +1327,7 @@ int w90gpu_session_rotation_time(w90gpu_session* s, int reps, double* avg_ms) {
     return code;
   };
   if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
-    set_error(E->ctx, "instantiation of the rotation timing graph failed");
+    set_error(E->ctx, "instantiation of the phase timing graph failed");
     return done(W90GPU_ERR_FATAL);
   }
   float ms = 0.f;
@@ -1320,11 +1335,15 @@ int w90gpu_session_rotation_time(w90gpu_session* s, int reps, double* avg_ms) {
       hipEventRecord(E->t0, E->stream) != hipSuccess || hipGraphLaunch(ge, E->stream) != hipSuccess ||
       hipEventRecord(E->t1, E->stream) != hipSuccess || hipEventSynchronize(E->t1) != hipSuccess ||
       hipEventElapsedTime(&ms, E->t0, E->t1) != hipSuccess) {
-    set_error(E->ctx, "rotation timing failed");
+    set_error(E->ctx, "phase timing failed");
     return done(W90GPU_ERR_FATAL);
   }
   *avg_ms = ms / reps;
   return done(W90GPU_OK);
+}
+
+int w90gpu_session_rotation_time(w90gpu_session* s, int reps, double* avg_ms) {
+  return w90gpu_session_phase_time(s, W90GPU_PHASE_ROTATION, reps, avg_ms);
 }
 
 int w90gpu_wann_main(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90gpu_control* ctl, double* m_matrix,
@@ -1540,7 +1559,7 @@ int w90gpu_domega(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* m_m
     hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(1024, (nd + 255) / 256)), dim3(256), 0, E->stream,
                        m->p.D, nd);
     if ((r = launch_omega(E, L, MODE_INIT))) return r;
-    if ((r = launch_domega(E, L))) return r;
+    if ((r = launch_domega(E, L, MODE_TRIAL))) return r;
     HIPCHK(ctx, hipMemcpyAsync(cdodq, m->p.G, ub, hipMemcpyDeviceToHost, E->stream));
     HIPCHK(ctx, hipStreamSynchronize(E->stream));
     return W90GPU_OK;

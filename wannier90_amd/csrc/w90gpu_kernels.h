@@ -393,12 +393,13 @@ __device__ __forceinline__ void diag_store(double* out, int nw, int n, const Dia
   out[7 * nw + n] = d.v[2];
 }
 
-// Scalars of the per-k vector that depend only on the k-mesh: B = sum wb b b^T.
-__device__ void kpart_bmat(const Prob& p, int k, double* out) {
+// Scalars of the per-k vector that depend only on the k-mesh: B = sum wb b b^T
+// from the weights wb[b] and the vectors bk[3 b + i] of the k.
+__device__ void kpart_bmat(const double* wb, const double* bk, int nn, double* out) {
   double b6[6] = {0, 0, 0, 0, 0, 0};
-  for (int b = 0; b < p.nn; ++b) {
-    const double w = p.wb[b];
-    const double* bv = p.bk + ((size_t)k * p.nn + b) * 3;
+  for (int b = 0; b < nn; ++b) {
+    const double w = wb[b];
+    const double* bv = bk + b * 3;
     b6[0] += w * bv[0] * bv[0];
     b6[1] += w * bv[1] * bv[1];
     b6[2] += w * bv[2] * bv[2];
@@ -407,6 +408,15 @@ __device__ void kpart_bmat(const Prob& p, int k, double* out) {
     b6[5] += w * bv[2] * bv[0];
   }
   for (int u = 0; u < 6; ++u) out[2 + u] = b6[u];
+}
+
+// The weights wb and the vectors bk(:,:,k) of one k staged in LDS by
+// threads 0 .. 4 nn - 1 (one round of loads instead of a chain of dependent
+// ones inside the loops over b); visible after the next barrier.
+__device__ __forceinline__ void kmesh_stage(const Prob& p, int k, double* wbL, double* bkL) {
+  const int t = threadIdx.x, nn = p.nn;
+  if (t < nn) wbL[t] = p.wb[t];
+  else if (t < 4 * nn) bkL[t - nn] = p.bk[(size_t)k * nn * 3 + (t - nn)];
 }
 
 // Reference centres r0 of the partial vectors: the centres of the source
@@ -450,31 +460,51 @@ __global__ void __launch_bounds__(W90_NT) k_kpart(const Prob* probs, const Item*
   const int dst = dst_buf(mode, cur);
   const double* ln = p.ln[dst];
   double* out = p.kpart + (size_t)k * p.psz;
-  for (int n = threadIdx.x; n < nw; n += blockDim.x) {
-    double r0[3];
-    ref_centre(p, mode, cur, n, r0);
-    DiagAcc d = {};
-    for (int b = 0; b < nn; ++b) {
-      const size_t kb = (size_t)k * nn + b;
-      diag_add(d, p.wb[b], p.bk + kb * 3, ln[kb * nw + n], p.d2[kb * nw + n], r0[0], r0[1], r0[2]);
+  const int npan = p.npan;
+  // every input of the k is loaded in one round: the mesh tables and the
+  // tile sums into LDS, the diagonal phases and moduli into registers
+  __shared__ double wbL[W90GPU_NUM_NNMAX], bkL[3 * W90GPU_NUM_NNMAX];
+  __shared__ double podL[W90GPU_NUM_NNMAX * 4], pallL[W90GPU_NUM_NNMAX * 4];
+  kmesh_stage(p, k, wbL, bkL);
+  for (int t = threadIdx.x; t < nn * npan; t += blockDim.x) {
+    podL[t] = p.part_od[(size_t)k * nn * npan + t];
+    pallL[t] = p.part_all[(size_t)k * nn * npan + t];
+  }
+  double lnv[W90GPU_NUM_NNMAX], d2v[W90GPU_NUM_NNMAX], r0[3];
+  const int n0 = threadIdx.x;
+  if (n0 < nw) {
+    ref_centre(p, mode, cur, n0, r0);
+#pragma unroll
+    for (int b = 0; b < W90GPU_NUM_NNMAX; ++b) {
+      if (b < nn) {
+        const size_t kb = (size_t)k * nn + b;
+        lnv[b] = ln[kb * nw + n0];
+        d2v[b] = p.d2[kb * nw + n0];
+      }
     }
-    diag_store(out, nw, n, d);
+  }
+  __syncthreads();
+  if (n0 < nw) {
+    DiagAcc d = {};
+#pragma unroll
+    for (int b = 0; b < W90GPU_NUM_NNMAX; ++b)
+      if (b < nn) diag_add(d, wbL[b], bkL + b * 3, lnv[b], d2v[b], r0[0], r0[1], r0[2]);
+    diag_store(out, nw, n0, d);
   }
   if (threadIdx.x == 0) {
     double sod = 0.0, si = 0.0;
     for (int b = 0; b < nn; ++b) {
-      const size_t kb = (size_t)k * nn + b;
       double od = 0.0, al = 0.0;
-      for (int q = 0; q < p.npan; ++q) {
-        od += p.part_od[kb * p.npan + q];
-        al += p.part_all[kb * p.npan + q];
+      for (int q = 0; q < npan; ++q) {
+        od += podL[b * npan + q];
+        al += pallL[b * npan + q];
       }
-      sod += p.wb[b] * od;
-      si += p.wb[b] * ((double)nw - al);
+      sod += wbL[b] * od;
+      si += wbL[b] * ((double)nw - al);
     }
     out[8 * nw + 0] = sod;
     out[8 * nw + 1] = si;
-    kpart_bmat(p, k, out + 8 * nw);
+    kpart_bmat(wbL, bkL, nn, out + 8 * nw);
   }
 }
 
@@ -729,10 +759,19 @@ __global__ void __launch_bounds__(W90_FINAL_NT) k_omega_final(const Prob* probs,
     const int j = tid / VB;
     double s = 0.0;
     if (j < J && v < psz) {
-      // rows j, j+J, ...: eight independent partial sums keep eight loads
-      // in flight, combined in a fixed order
+      // rows j, j+J, ...: eight independent partial sums, combined in a
+      // fixed order; four rounds of eight rows are loaded at once (32 loads
+      // in flight) and added round by round, the same order as one round
+      // at a time
       double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       int g = j;
+      for (; g + 31 * J < G; g += 32 * J) {
+        double x[32];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) x[u] = rows[(size_t)(g + u * J) * psz + v];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) a[u % 8] += x[u];
+      }
       for (; g + 7 * J < G; g += 8 * J) {
 #pragma unroll
         for (int u = 0; u < 8; ++u) a[u] += rows[(size_t)(g + u * J) * psz + v];
@@ -809,12 +848,12 @@ __device__ __forceinline__ cplx domega_term(cplx acc, double w, cplx mmn, cplx m
 // workgroup per (problem, k), fused with the per-k partial sums Re<G,G> and
 // Re<G,D_old> needed by internal_search_direction.
 template <class T2>
-__global__ void __launch_bounds__(1024) k_domega(const Prob* probs, const Item* items, int nitems) {
+__global__ void __launch_bounds__(1024) k_domega(const Prob* probs, const Item* items, int nitems, int mode) {
   if (blockIdx.x >= (unsigned)nitems) return;
   const Item it = items[blockIdx.x];
   const Prob& p = probs[it.prob];
   const State* st = p.st;
-  if (st->done || st->hold) return;
+  if (mode != MODE_BENCH && (st->done || st->hold)) return;
   const int nw = p.nw, nn = p.nn, nw2 = p.nw2;
   const int k = it.k;
   const int cur = st->cur;
@@ -858,25 +897,38 @@ __host__ __device__ constexpr int domega_nt() {
   return NWM > 32 ? 1024 : W90_NT;
 }
 
+// Blocks M(k,b) that k_domega_k holds in LDS at once: all of them for
+// num_wann <= 16, four for <= 32, one above.
+template <int NWM>
+__host__ __device__ constexpr int domega_chunk() {
+  return NWM <= 16 ? W90GPU_NUM_NNMAX : (NWM <= 32 ? 4 : 1);
+}
+
 // LDS bytes of k_domega_k (dynamic shared memory).
 template <int NWM, class T2>
 __host__ __device__ constexpr size_t domega_lds() {
-  return sizeof(T2) * (size_t)NWM * (NWM + 1) + sizeof(double) * (2 * W90GPU_NUM_NNMAX * NWM + domega_nt<NWM>());
+  return sizeof(T2) * (size_t)NWM * (NWM + 1) * domega_chunk<NWM>() +
+         sizeof(double) * (2 * W90GPU_NUM_NNMAX * NWM + domega_nt<NWM>());
 }
 
-// wann_domega with each block M(k,b) staged in LDS (num_wann <= NWM): the
+// wann_domega with the blocks M(k,b) staged in LDS (num_wann <= NWM): the
 // transposed and diagonal reads of the gradient formula hit LDS (padded
 // leading dimension), and the scaled phases wb Im ln M_nn and b.rave_n of
-// the k are precomputed.
+// the k are precomputed.  The blocks are loaded in chunks of
+// domega_chunk<NWM>() (every load of a chunk in flight at once), the next
+// chunk into registers while the current one is used; the sum over b runs
+// in b order whatever the chunking.
 template <int NWM, class T2>
-__global__ void __launch_bounds__(domega_nt<NWM>()) k_domega_k(const Prob* probs, const Item* items, int nitems) {
+__global__ void __launch_bounds__(domega_nt<NWM>()) k_domega_k(const Prob* probs, const Item* items, int nitems,
+                                                                int mode) {
   if (blockIdx.x >= (unsigned)nitems) return;
   const Item it = items[blockIdx.x];
   const Prob& p = probs[it.prob];
   const State* st = p.st;
-  if (st->done || st->hold) return;
+  if (mode != MODE_BENCH && (st->done || st->hold)) return;
   constexpr int NT = domega_nt<NWM>();
   constexpr int SL = (NWM * NWM + NT - 1) / NT;
+  constexpr int CB = domega_chunk<NWM>();
   const int nw = p.nw, nn = p.nn, nw2 = p.nw2;
   const int ld = nw + 1;
   const int k = it.k;
@@ -898,39 +950,55 @@ __global__ void __launch_bounds__(domega_nt<NWM>()) k_domega_k(const Prob* probs
   cplx acc[SL];
 #pragma unroll
   for (int s = 0; s < SL; ++s) acc[s] = c_make(0.0, 0.0);
+  __shared__ double wL[W90GPU_NUM_NNMAX];
+  if (tid < nn) wL[tid] = p.wb[tid];
   const T2* Mk = mbuf<T2>(p, cur) + (size_t)k * nn * nw2;
-  // block b+1 is loaded into registers while block b is used from LDS
-  T2 nxt[SL];
+  const int bs = NWM * (NWM + 1);  // LDS stride of the chunk's blocks
+  T2 nxt[CB][SL];
 #pragma unroll
-  for (int s = 0; s < SL; ++s) {
-    const int e = tid + s * NT;
-    if (e < nw2) nxt[s] = Mk[e];
-  }
-  for (int b = 0; b < nn; ++b) {
-    __syncthreads();
+  for (int j = 0; j < CB; ++j)
 #pragma unroll
     for (int s = 0; s < SL; ++s) {
       const int e = tid + s * NT;
-      if (e < nw2) Ms[(e % nw) + (e / nw) * ld] = nxt[s];
+      T2 v = {};
+      if (j < nn && e < nw2) v = Mk[(size_t)j * nw2 + e];
+      nxt[j][s] = v;
     }
-    if (b + 1 < nn) {
-      const T2* src = Mk + (size_t)(b + 1) * nw2;
+  for (int b0 = 0; b0 < nn; b0 += CB) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < CB; ++j)
 #pragma unroll
       for (int s = 0; s < SL; ++s) {
         const int e = tid + s * NT;
-        if (e < nw2) nxt[s] = src[e];
+        if (b0 + j < nn && e < nw2) Ms[j * bs + (e % nw) + (e / nw) * ld] = nxt[j][s];
       }
+    if (b0 + CB < nn) {
+      const T2* src = Mk + (size_t)(b0 + CB) * nw2;
+#pragma unroll
+      for (int j = 0; j < CB; ++j)
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          const int e = tid + s * NT;
+          T2 v = {};
+          if (b0 + CB + j < nn && e < nw2) v = src[(size_t)j * nw2 + e];
+          nxt[j][s] = v;
+        }
     }
     __syncthreads();
-    const double w = p.wb[b];
+    for (int j = 0; j < CB && b0 + j < nn; ++j) {
+      const int b = b0 + j;
+      const T2* Mb = Ms + j * bs;
+      const double w = wL[b];
 #pragma unroll
-    for (int s = 0; s < SL; ++s) {
-      const int e = tid + s * NT;
-      if (e < nw2) {
-        const int m = e % nw, n = e / nw;
-        acc[s] = domega_term(acc[s], w, c_wide(Ms[m + n * ld]), c_wide(Ms[n + m * ld]), c_wide(Ms[n + n * ld]),
-                             c_wide(Ms[m + m * ld]), q[b * nw + n], q[b * nw + m], rb[b * nw + n],
-                             rb[b * nw + m]);
+      for (int s = 0; s < SL; ++s) {
+        const int e = tid + s * NT;
+        if (e < nw2) {
+          const int m = e % nw, n = e / nw;
+          acc[s] = domega_term(acc[s], w, c_wide(Mb[m + n * ld]), c_wide(Mb[n + m * ld]), c_wide(Mb[n + n * ld]),
+                               c_wide(Mb[m + m * ld]), q[b * nw + n], q[b * nw + m], rb[b * nw + n],
+                               rb[b * nw + m]);
+        }
       }
     }
   }
@@ -1217,6 +1285,27 @@ __global__ void __launch_bounds__(W90_NT) k_step_k(const Prob* probs, const Item
   __shared__ cplx T[NWM * (NWM + 1)];
   __shared__ double red[W90_NT];
   cplx* Dk = p.D + (size_t)k * nw2;
+  // G, D and U of the k are loaded before the search and the exponential,
+  // so their latency overlaps that work
+  constexpr int SL = (NWM * NWM + W90_NT - 1) / W90_NT;
+  cplx gpre[SL], dpre[SL], upre[SL];
+  {
+    const cplx* Gk = p.G + (size_t)k * nw2;
+    const cplx* Uc = p.U[cur] + (size_t)k * nw2;
+#pragma unroll
+    for (int i = 0; i < SL; ++i) {
+      const int e = tid + i * W90_NT;
+      cplx g = c_make(0.0, 0.0), d = g, u = g;
+      if (e < nw2) {
+        if (mode == MODE_TRIAL) g = Gk[e];
+        d = Dk[e];
+        u = Uc[e];
+      }
+      gpre[i] = g;
+      dpre[i] = d;
+      upre[i] = u;
+    }
+  }
   __shared__ State lst;
   const State* ss = st;
   if (mode == MODE_TRIAL) {
@@ -1225,15 +1314,22 @@ __global__ void __launch_bounds__(W90_NT) k_step_k(const Prob* probs, const Item
   }
   const double scale = ss->scale;
   if (mode == MODE_TRIAL) {
-    const cplx* Gk = p.G + (size_t)k * nw2;
     const double beta = ss->beta, sign = ss->dir_sign;
-    for (int e = tid; e < nw2; e += W90_NT) {
-      cplx d = c_scale(c_add(Gk[e], c_scale(Dk[e], beta)), sign);
-      Dk[e] = d;
-      X[(e % nw) + (e / nw) * ld] = c_scale(d, scale);
+#pragma unroll
+    for (int i = 0; i < SL; ++i) {
+      const int e = tid + i * W90_NT;
+      if (e < nw2) {
+        cplx d = c_scale(c_add(gpre[i], c_scale(dpre[i], beta)), sign);
+        Dk[e] = d;
+        X[(e % nw) + (e / nw) * ld] = c_scale(d, scale);
+      }
     }
   } else {
-    for (int e = tid; e < nw2; e += W90_NT) X[(e % nw) + (e / nw) * ld] = c_scale(Dk[e], scale);
+#pragma unroll
+    for (int i = 0; i < SL; ++i) {
+      const int e = tid + i * W90_NT;
+      if (e < nw2) X[(e % nw) + (e / nw) * ld] = c_scale(dpre[i], scale);
+    }
   }
   __syncthreads();
   double cs = 0.0;
@@ -1275,11 +1371,14 @@ __global__ void __launch_bounds__(W90_NT) k_step_k(const Prob* probs, const Item
     b = t;
   }
   cplx* Ek = p.E + (size_t)k * nw2;
-  const cplx* Uc = p.U[cur] + (size_t)k * nw2;
-  for (int e = tid; e < nw2; e += W90_NT) {
-    const int q = (e % nw) + (e / nw) * ld;
-    Ek[e] = a[q];
-    X[q] = Uc[e];
+#pragma unroll
+  for (int i = 0; i < SL; ++i) {
+    const int e = tid + i * W90_NT;
+    if (e < nw2) {
+      const int q = (e % nw) + (e / nw) * ld;
+      Ek[e] = a[q];
+      X[q] = upre[i];
+    }
   }
   __syncthreads();
   lds_gemm<NWM>(b, X, a, nw);

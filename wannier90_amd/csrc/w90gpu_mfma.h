@@ -511,20 +511,22 @@ __device__ void wg_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n) {
 // ---------------------------------------------------------- small blocks
 
 // LDS bytes of k_rotate_w16 for up to nn neighbours: one padded 16 x 17
-// block per wavefront, the diagonal of each rotated block and the per-wave
-// sums.
+// block per wavefront, the diagonal of each rotated block (complex value,
+// phase, modulus), the per-wave sums and the k's mesh tables (wb, bk).
 template <class T2>
 __host__ __device__ constexpr size_t rotate_w16_lds(int nn) {
-  return sizeof(T2) * 16 * 17 * (size_t)nn + sizeof(double) * (2 * 16 * (size_t)nn + 2 * (size_t)nn);
+  return sizeof(T2) * 16 * 17 * (size_t)nn + sizeof(double) * (4 * 16 * (size_t)nn + 6 * (size_t)nn);
 }
 
 // M rotation with the omega partial sums for num_wann <= 16: one workgroup
 // per (problem, k), one wavefront per neighbour b, each block a single
 // 16 x 16 MFMA tile (zero-padded).  T = E_k^+ M(k,b) is formed from operand
 // fragments loaded straight from memory, passed through the wavefront's LDS
-// block to become the A operand of R = T E_{k+b}; R is written once, its
-// diagonal goes through LDS to threads n < num_wann, which sum it over b in
-// order.  blockDim.x = 64 * (largest nntot of the launch); wavefronts beyond
+// block to become the A operand of R = T E_{k+b}; R goes back through the
+// same LDS block so that it is stored in column order (coalesced) and its
+// diagonal is collected per b; the phases Im ln M_nn of all b are then
+// formed in one pass over the workgroup, and threads n < num_wann sum them
+// over b in order.  blockDim.x = 64 * (largest nntot of the launch); wavefronts beyond
 // the problem's nntot only take part in the barriers, and every sum is
 // formed per wavefront and then over b in order, so a problem's result does
 // not depend on the other problems of the launch.  MODE_INIT only reduces.
@@ -555,10 +557,24 @@ __global__ void __launch_bounds__(64 * W90GPU_NUM_NNMAX) k_rotate_w16(const Prob
   double* d2L = lnL + nnl * 16;
   double* wod = d2L + nnl * 16;
   double* wall = wod + nnl;
+  double* wbL = wall + nnl;
+  double* bkL = wbL + nnl;
+  double* dRe = bkL + 3 * nnl;  // [b][16] diagonal of the rotated block
+  double* dIm = dRe + nnl * 16;
+  // mesh tables and the reference centres are loaded with the operands,
+  // not inside the loops over b of the epilogue
+  kmesh_stage(p, k, wbL, bkL);
+  double r0[3];
+  if (tid < nw) ref_centre(p, mode, cur, tid, r0);
   const size_t kb = (size_t)k * nn + (active ? b : 0);
   const T2* src = rot_src<T2>(p, mode, cur) + kb * nw2;
   const cplx* Ek = p.E + (size_t)k * nw2;
   const cplx* Ekb = p.E + (size_t)p.nnl[kb] * nw2;
+  // output addresses are formed before the first store (the compiler
+  // would otherwise reload the problem's pointers after each store)
+  T2* out = mbuf<T2>(p, dst) + kb * nw2;
+  double* lnd = p.ln[dst] + (size_t)k * nn * nw;
+  double* outk = p.kpart + (size_t)k * p.psz;
   Acc Rr, Ri;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -604,31 +620,41 @@ __global__ void __launch_bounds__(64 * W90GPU_NUM_NNMAX) k_rotate_w16(const Prob
     }
   }
   __syncthreads();
-  double s_od = 0.0, s_all = 0.0;
-  if (active) {
-    T2* out = mbuf<T2>(p, dst) + kb * nw2;
-    if (rot) {
-      // R = T E_{k+b}: A[i][l] = T[i][l] (LDS), B[l][j] = E_{k+b}[l + j nw]
+  if (active && rot) {
+    // R = T E_{k+b}: A[i][l] = T[i][l] (LDS), B[l][j] = E_{k+b}[l + j nw]
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int l = 4 * s + lk;
-        const T2 a = Tl[lr + l * 17];
-        const R br = (R)fb[s].x, bi = (R)fb[s].y;
-        Rr = Mfma<R>::mma(a.x, br, Rr);
-        Rr = Mfma<R>::mma(-a.y, bi, Rr);
-        Ri = Mfma<R>::mma(a.x, bi, Ri);
-        Ri = Mfma<R>::mma(a.y, br, Ri);
-      }
+    for (int s = 0; s < 4; ++s) {
+      const int l = 4 * s + lk;
+      const T2 a = Tl[lr + l * 17];
+      const R br = (R)fb[s].x, bi = (R)fb[s].y;
+      Rr = Mfma<R>::mma(a.x, br, Rr);
+      Rr = Mfma<R>::mma(-a.y, bi, Rr);
+      Ri = Mfma<R>::mma(a.x, bi, Ri);
+      Ri = Mfma<R>::mma(a.y, br, Ri);
     }
+    // R replaces T in the wavefront's block (every read of T above feeds
+    // the products that produce R)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int r = rot ? Mfma<R>::row(lk, i) : (lane + 64 * i) % 16;
-      const int c = rot ? lr : (lane + 64 * i) / 16;
+      T2 t;
+      t.x = Rr[i];
+      t.y = Ri[i];
+      Tl[Mfma<R>::row(lk, i) + lr * 17] = t;
+    }
+  }
+  __syncthreads();
+  double s_od = 0.0, s_all = 0.0;
+  if (active) {
+    // element e = lane + 64 i of the block: consecutive lanes hold
+    // consecutive rows of a column, so the stores are contiguous
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = lane + 64 * i;
+      const int r = e & 15, c = e >> 4;
       if (r < nw && c < nw) {
         T2 v;
         if (rot) {
-          v.x = Rr[i];
-          v.y = Ri[i];
+          v = Tl[r + c * 17];
           out[r + c * nw] = v;
         } else {
           v = src[r + c * nw];
@@ -639,10 +665,8 @@ __global__ void __launch_bounds__(64 * W90GPU_NUM_NNMAX) k_rotate_w16(const Prob
         if (r != c) {
           s_od += a2;
         } else {
-          const double l = atan2(vw.y, vw.x);
-          p.ln[dst][kb * nw + r] = l;
-          lnL[b * 16 + r] = l;
-          d2L[b * 16 + r] = a2;
+          dRe[b * 16 + r] = vw.x;
+          dIm[b * 16 + r] = vw.y;
         }
       }
     }
@@ -654,27 +678,35 @@ __global__ void __launch_bounds__(64 * W90GPU_NUM_NNMAX) k_rotate_w16(const Prob
     wall[b] = s_all;
   }
   __syncthreads();
-  double* outk = p.kpart + (size_t)k * p.psz;
+  // Im ln M_nn and |M_nn|^2 of every b in one pass
+  for (int t = tid; t < nn * 16; t += blockDim.x) {
+    const int bb = t >> 4, n = t & 15;
+    if (n < nw) {
+      const cplx vw = c_make(dRe[t], dIm[t]);
+      const double l = atan2(vw.y, vw.x);
+      lnd[bb * nw + n] = l;
+      lnL[t] = l;
+      d2L[t] = c_abs2(vw);
+    }
+  }
+  __syncthreads();
   if (tid < nw) {
     const int n = tid;
-    double r0[3];
-    ref_centre(p, mode, cur, n, r0);
     DiagAcc d = {};
     for (int bb = 0; bb < nn; ++bb)
-      diag_add(d, p.wb[bb], p.bk + ((size_t)k * nn + bb) * 3, lnL[bb * 16 + n], d2L[bb * 16 + n], r0[0], r0[1],
-               r0[2]);
+      diag_add(d, wbL[bb], bkL + bb * 3, lnL[bb * 16 + n], d2L[bb * 16 + n], r0[0], r0[1], r0[2]);
     diag_store(outk, nw, n, d);
   }
   if (tid == 0) {
     double sod = 0.0, sall = 0.0, wsum = 0.0;
     for (int bb = 0; bb < nn; ++bb) {
-      const double w = p.wb[bb];
+      const double w = wbL[bb];
       sod += w * wod[bb];
       sall += w * wall[bb];
       wsum += w;
     }
     outk[8 * nw + 0] = sod;
     outk[8 * nw + 1] = (double)nw * wsum - sall;
-    kpart_bmat(p, k, outk + 8 * nw);
+    kpart_bmat(wbL, bkL, nn, outk + 8 * nw);
   }
 }
