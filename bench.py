@@ -259,6 +259,7 @@ def main():
     phase_us = {"rotation": avg_ms * 1e3,
                 "gradient": sess.phase_time("gradient", launches) * 1e3,
                 "retraction": sess.phase_time("retraction", launches) * 1e3,
+                "reduction": sess.phase_time("reduction", launches) * 1e3,
                 "iteration": 1e6 * total / args.steps / max(1, its)}
     log("[bench] phases (us): " + ", ".join(f"{k} {v:.1f}" for k, v in phase_us.items()))
 

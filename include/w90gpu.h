@@ -204,14 +204,17 @@ size_t w90gpu_session_bytes(int num_wann, int num_kpts, int nntot);
  * rewrites the same scratch buffer). */
 int w90gpu_session_rotation_time(w90gpu_session* sess, int reps, double* avg_ms);
 /* The same for one phase of an iteration: W90GPU_PHASE_ROTATION (as
- * above), W90GPU_PHASE_GRADIENT (wann_domega: G and its per-k norms) or
+ * above), W90GPU_PHASE_GRADIENT (wann_domega: G and its per-k norms),
  * W90GPU_PHASE_RETRACTION (exp of the scaled search direction and the U
- * update of the parabolic step).  Each launch rewrites scratch buffers only
- * (the other M or U buffer, G, exp(dW)), so the returned M, U and centres
+ * update of the parabolic step) or W90GPU_PHASE_REDUCTION (the final
+ * reduction of wann_omega over k, without the control logic that follows
+ * it).  Each launch rewrites scratch buffers only (the other M or U buffer,
+ * G, exp(dW), the other centres buffer), so the returned M, U and centres
  * of the solve are unchanged. */
 #define W90GPU_PHASE_ROTATION 0
 #define W90GPU_PHASE_GRADIENT 1
 #define W90GPU_PHASE_RETRACTION 2
+#define W90GPU_PHASE_REDUCTION 3
 int w90gpu_session_phase_time(w90gpu_session* sess, int phase, int reps, double* avg_ms);
 
 /* ------------------------------------------------ batches and serving

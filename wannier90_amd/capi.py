@@ -364,12 +364,13 @@ class Session:
         self.ctx._check(self.lib.w90gpu_session_rotation_time(self.h, reps, C.byref(ms)))
         return ms.value
 
-    PHASES = {"rotation": 0, "gradient": 1, "retraction": 2}
+    PHASES = {"rotation": 0, "gradient": 1, "retraction": 2, "reduction": 3}
 
     def phase_time(self, phase, reps=50):
         """Average duration (ms) of one phase of an iteration ("rotation",
-        "gradient" or "retraction"), `reps` launches back to back on the
-        final state of the last solve; the solve's results are unchanged."""
+        "gradient", "retraction" or "reduction"), `reps` launches back to
+        back on the final state of the last solve; the solve's results are
+        unchanged."""
         ms = C.c_double(0.0)
         self.ctx._check(self.lib.w90gpu_session_phase_time(self.h, self.PHASES[phase], reps, C.byref(ms)))
         return ms.value

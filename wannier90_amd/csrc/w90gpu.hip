@@ -729,9 +729,9 @@ int launch_rotate(Engine* E, const Lists& L, int mode) {
   return W90GPU_OK;
 }
 
-int launch_omega(Engine* E, const Lists& L, int mode) {
-  int rc = launch_rotate(E, L, mode);
-  if (rc) return rc;
+// The reduction of the per-k partial vectors over k, with the control
+// logic that follows each wann_omega.
+int launch_reduce(Engine* E, const Lists& L, int mode) {
   if (L.any_group) {
     hipLaunchKernelGGL(k_omega_group, dim3(L.np * W90_MAX_GROUPS), dim3(W90_NT), 0, E->stream, E->d_probs,
                        L.plist, L.np, mode);
@@ -740,6 +740,12 @@ int launch_omega(Engine* E, const Lists& L, int mode) {
   hipLaunchKernelGGL(k_omega_final, dim3(L.np), dim3(W90_FINAL_NT), 0, E->stream, E->d_probs, L.plist, L.np, mode);
   HIPCHK(E->ctx, hipGetLastError());
   return W90GPU_OK;
+}
+
+int launch_omega(Engine* E, const Lists& L, int mode) {
+  int rc = launch_rotate(E, L, mode);
+  if (rc) return rc;
+  return launch_reduce(E, L, mode);
 }
 
 int launch_step(Engine* E, const Lists& L, int mode) {
@@ -1293,7 +1299,7 @@ int w90gpu_session_history(w90gpu_session* s, double* out, int rows) {
 
 int w90gpu_session_phase_time(w90gpu_session* s, int phase, int reps, double* avg_ms) {
   if (!s || reps < 1 || !avg_ms) return W90GPU_ERR_INPUT;
-  if (phase < W90GPU_PHASE_ROTATION || phase > W90GPU_PHASE_RETRACTION) return W90GPU_ERR_INPUT;
+  if (phase < W90GPU_PHASE_ROTATION || phase > W90GPU_PHASE_REDUCTION) return W90GPU_ERR_INPUT;
   Engine* E = s->eng;
   if (s->m->stage == 1 || !s->loaded) {
     set_error(E->ctx, "phase timing needs a solved session");
@@ -1313,7 +1319,8 @@ int w90gpu_session_phase_time(w90gpu_session* s, int phase, int reps, double* av
   for (int i = 0; i < reps && rc == W90GPU_OK; ++i) {
     if (phase == W90GPU_PHASE_ROTATION) rc = launch_rotate(E, L, MODE_BENCH);
     else if (phase == W90GPU_PHASE_GRADIENT) rc = launch_domega(E, L, MODE_BENCH);
-    else rc = launch_step(E, L, MODE_BENCH);
+    else if (phase == W90GPU_PHASE_RETRACTION) rc = launch_step(E, L, MODE_BENCH);
+    else rc = launch_reduce(E, L, MODE_BENCH);
   }
   const hipError_t ec = hipStreamEndCapture(E->stream, &g);
   if (rc || ec != hipSuccess || !g) {

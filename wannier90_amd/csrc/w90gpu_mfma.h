@@ -61,6 +61,52 @@ struct Real<float2> {
   typedef float type;
 };
 
+// C = A B for the LDS retraction (k_step_k): n x n column-major complex
+// matrices in LDS, leading dimension n + 1, n <= NWM.  The product is
+// (NWM/16)^2 tiles of 16 x 16; each (tile, real or imaginary part) is one
+// job of a wavefront: K in steps of 4 on FP64 MFMA 16x16x4, operand
+// fragments read from LDS (zero outside n), the result part written to its
+// half of the complex elements of C.  C must not overlap A or B; the
+// workgroup is synchronised before return.
+template <int NWM>
+__device__ void lds_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n) {
+  static_assert(NWM % 16 == 0, "whole 16 x 16 tiles");
+  constexpr int TN = NWM / 16;
+  constexpr int JOBS = 2 * TN * TN;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int ld = n + 1;
+  const int nwaves = blockDim.x >> 6;
+  double* Cd = reinterpret_cast<double*>(C);
+  for (int job = wave; job < JOBS; job += nwaves) {
+    const int part = job & 1;  // 0: real part, 1: imaginary part
+    const int t = job >> 1;
+    const int ti = t % TN, tj = t / TN;
+    const int i = 16 * ti + lr, j = 16 * tj + lr;
+    Mfma<double>::acc acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < NWM / 4; ++s) {
+      const int l = 4 * s + lk;
+      cplx a = c_make(0.0, 0.0), bv = c_make(0.0, 0.0);
+      if (i < n && l < n) a = A[i + l * ld];
+      if (l < n && j < n) bv = B[l + j * ld];
+      if (part == 0) {
+        acc = Mfma<double>::mma(a.x, bv.x, acc);
+        acc = Mfma<double>::mma(-a.y, bv.y, acc);
+      } else {
+        acc = Mfma<double>::mma(a.x, bv.y, acc);
+        acc = Mfma<double>::mma(a.y, bv.x, acc);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = 16 * ti + Mfma<double>::row(lk, q), c = 16 * tj + lr;
+      if (r < n && c < n) Cd[2 * (r + c * ld) + part] = acc[q];
+    }
+  }
+  __syncthreads();
+}
+
 // output tile edge of a workgroup: 32 up to num_wann 32, 64 above
 __host__ __device__ __forceinline__ int gemm_edge(int nw) { return nw <= 32 ? 32 : 64; }
 
