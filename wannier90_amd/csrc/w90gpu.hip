@@ -694,21 +694,24 @@ int launch_rotate(Engine* E, const Lists& L, int mode) {
       } else {
         // MFMA GEMM tiles; gridDim.y covers b and the tiles of a block
         const dim3 g2(g.n, g.nnmax * g.npanmax);
+        // classes 1 and 2 (one tile per block): both products in one launch
+        const dim3 gb(g.n, g.nnmax);
         if (c == 1) {
           if (mode == MODE_INIT) {
             if (q == 0) hipLaunchKernelGGL((k_reduce_tiles<cplx, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
             else hipLaunchKernelGGL((k_reduce_tiles<float2, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
           } else if (q == 0) {
-            hipLaunchKernelGGL((k_rot_gemm<cplx, 1, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
-            hipLaunchKernelGGL((k_rot_gemm<cplx, 2, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
+            hipLaunchKernelGGL((k_rot_fused<cplx, 32>), gb, dim3(256), 0, s, P, it, g.n, mode);
           } else {
-            hipLaunchKernelGGL((k_rot_gemm<float2, 1, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
-            hipLaunchKernelGGL((k_rot_gemm<float2, 2, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
+            hipLaunchKernelGGL((k_rot_fused<float2, 32>), gb, dim3(256), 0, s, P, it, g.n, mode);
           }
         } else {
           if (mode == MODE_INIT) {
             if (q == 0) hipLaunchKernelGGL((k_reduce_tiles<cplx, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
             else hipLaunchKernelGGL((k_reduce_tiles<float2, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
+          } else if (c == 2) {
+            if (q == 0) hipLaunchKernelGGL((k_rot_fused<cplx, 64>), gb, dim3(256), 0, s, P, it, g.n, mode);
+            else hipLaunchKernelGGL((k_rot_fused<float2, 64>), gb, dim3(256), 0, s, P, it, g.n, mode);
           } else if (q == 0) {
             hipLaunchKernelGGL((k_rot_gemm<cplx, 1, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
             hipLaunchKernelGGL((k_rot_gemm<cplx, 2, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);

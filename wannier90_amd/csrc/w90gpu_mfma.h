@@ -25,7 +25,9 @@
 
 #include "w90gpu_kernels.h"
 
+#ifndef W90_GK
 #define W90_GK 16  // K chunk staged in LDS
+#endif
 
 typedef double w90_d4 __attribute__((ext_vector_type(4)));
 typedef float w90_f4 __attribute__((ext_vector_type(4)));
@@ -116,36 +118,24 @@ __host__ __device__ __forceinline__ int gemm_tiles(int nw) {
   return t * t;
 }
 
+// One GT x GT output tile (row0, col0) of one stage of the rotation of
+// block (k, b) by the workgroup (256 threads): operands and scratch as in
+// k_rot_gemm, LDS passed in.
 template <class T2, int STAGE, int GT>
-__global__ void __launch_bounds__(256) k_rot_gemm(const Prob* probs, const Item* items, int nitems, int mode) {
+__device__ __forceinline__ void rot_gemm_tile(const Prob& p, int k, int b, int tile, int row0, int col0, int mode,
+                                              int cur, typename Real<T2>::type (*As)[W90_GK][GT + 1],
+                                              typename Real<T2>::type (*Bs)[W90_GK][GT + 1], double* red) {
   constexpr int FR = GT / 32;  // fragments per wavefront along rows and columns
   typedef typename Real<T2>::type R;
   typedef typename Mfma<R>::acc Acc;
-  if (blockIdx.x >= (unsigned)nitems) return;
-  const Item it = items[blockIdx.x];
-  const Prob& p = probs[it.prob];
-  const State* st = p.st;
-  if (!gate(st, mode)) return;
   const int nw = p.nw, nn = p.nn;
-  const int nt1 = (nw + GT - 1) / GT;
-  const int ntl = nt1 * nt1;
-  const int by = blockIdx.y;
-  if (by >= nn * ntl) return;
-  const int b = by / ntl, tile = by % ntl;
-  const int row0 = (tile % nt1) * GT, col0 = (tile / nt1) * GT;
-  const int k = it.k;
   const size_t kb = (size_t)k * nn + b;
   const size_t nw2 = p.nw2;
-  const int cur = st->cur;
   const int dst = dst_buf(mode, cur);
   T2* Tm = reinterpret_cast<T2*>(p.Tm) + kb * nw2;
   const cplx* Ek = p.E + (size_t)k * nw2;
   const cplx* Ekb = p.E + (size_t)p.nnl[kb] * nw2;
   const T2* src = rot_src<T2>(p, mode, cur) + kb * nw2;
-
-  __shared__ R As[2][W90_GK][GT + 1];  // [re/im][k][row]
-  __shared__ R Bs[2][W90_GK][GT + 1];  // [re/im][k][col]
-  __shared__ double red[256];
 
   const int tid = threadIdx.x;
   const int w = tid >> 6, lane = tid & 63;
@@ -168,38 +158,49 @@ __global__ void __launch_bounds__(256) k_rot_gemm(const Prob* probs, const Item*
         Ci[fi][fj][i] = 0;
       }
 
-  for (int kk = 0; kk < nw; kk += W90_GK) {
-    __syncthreads();
-    // A chunk: rows row0.., k = kk..kk+GK
-    for (int e = tid; e < W90_GK * GT; e += 256) {
-      cplx v;
-      int l, i;
+  // K chunks are software-pipelined: the operands of chunk kk + GK are
+  // loaded into registers before the products of chunk kk, and stored to
+  // LDS after them.  Thread slot e = tid + 256 u of a chunk.
+  constexpr int NE = W90_GK * GT / 256;
+  static_assert(NE * 256 == W90_GK * GT, "whole chunk per thread slot");
+  cplx pa[NE], pb[NE];
+  auto load_chunk = [&](int kk) {
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      const int e = tid + 256 * u;
+      cplx v = c_make(0.0, 0.0);
       if (STAGE == 1) {
         // A = E_k^+: A[i][l] = conj(E_k[l + i nw]); threads run along l
-        l = e % W90_GK;
-        i = e / W90_GK;
-        const int gi = row0 + i, gl = kk + l;
-        v = (gi < nw && gl < nw) ? Ek[gl + (size_t)gi * nw] : c_make(0.0, 0.0);
+        const int gi = row0 + e / W90_GK, gl = kk + e % W90_GK;
+        if (gi < nw && gl < nw) v = Ek[gl + (size_t)gi * nw];
         v.y = -v.y;
       } else {
         // A = T: A[i][l] = T[i + l nw]; threads run along i
-        i = e % GT;
-        l = e / GT;
-        const int gi = row0 + i, gl = kk + l;
-        v = (gi < nw && gl < nw) ? c_wide(Tm[gi + (size_t)gl * nw]) : c_make(0.0, 0.0);
+        const int gi = row0 + e % GT, gl = kk + e / GT;
+        if (gi < nw && gl < nw) v = c_wide(Tm[gi + (size_t)gl * nw]);
       }
-      As[0][l][i] = (R)v.x;
-      As[1][l][i] = (R)v.y;
+      pa[u] = v;
+      // B[l][j] = Bsrc[l + j nw]; threads run along l
+      const int gl = kk + e % W90_GK, gj = col0 + e / W90_GK;
+      cplx w2 = c_make(0.0, 0.0);
+      if (gl < nw && gj < nw) w2 = (STAGE == 1) ? c_wide(src[gl + (size_t)gj * nw]) : Ekb[gl + (size_t)gj * nw];
+      pb[u] = w2;
     }
-    // B chunk: B[l][j] = Bsrc[l + j nw]; threads run along l
-    for (int e = tid; e < W90_GK * GT; e += 256) {
-      const int l = e % W90_GK, j = e / W90_GK;
-      const int gl = kk + l, gj = col0 + j;
-      cplx v = c_make(0.0, 0.0);
-      if (gl < nw && gj < nw) v = (STAGE == 1) ? c_wide(src[gl + (size_t)gj * nw]) : Ekb[gl + (size_t)gj * nw];
-      Bs[0][l][j] = (R)v.x;
-      Bs[1][l][j] = (R)v.y;
+  };
+  load_chunk(0);
+  for (int kk = 0; kk < nw; kk += W90_GK) {
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      const int e = tid + 256 * u;
+      const int la = (STAGE == 1) ? e % W90_GK : e / GT, ia = (STAGE == 1) ? e / W90_GK : e % GT;
+      As[0][la][ia] = (R)pa[u].x;
+      As[1][la][ia] = (R)pa[u].y;
+      const int lb = e % W90_GK, jb = e / W90_GK;
+      Bs[0][lb][jb] = (R)pb[u].x;
+      Bs[1][lb][jb] = (R)pb[u].y;
     }
+    if (kk + W90_GK < nw) load_chunk(kk + W90_GK);
     __syncthreads();
 #pragma unroll
     for (int ks = 0; ks < W90_GK; ks += 4) {
@@ -262,6 +263,51 @@ __global__ void __launch_bounds__(256) k_rot_gemm(const Prob* probs, const Item*
       p.part_all[kb * p.npan + tile] = all;
     }
   }
+}
+
+// Two-stage rotation, one launch per stage: workgroup (item, b, tile).
+template <class T2, int STAGE, int GT>
+__global__ void __launch_bounds__(256) k_rot_gemm(const Prob* probs, const Item* items, int nitems, int mode) {
+  typedef typename Real<T2>::type R;
+  if (blockIdx.x >= (unsigned)nitems) return;
+  const Item it = items[blockIdx.x];
+  const Prob& p = probs[it.prob];
+  const State* st = p.st;
+  if (!gate(st, mode)) return;
+  const int nw = p.nw, nn = p.nn;
+  const int nt1 = (nw + GT - 1) / GT;
+  const int ntl = nt1 * nt1;
+  const int by = blockIdx.y;
+  if (by >= nn * ntl) return;
+  const int b = by / ntl, tile = by % ntl;
+  const int row0 = (tile % nt1) * GT, col0 = (tile / nt1) * GT;
+  __shared__ R As[2][W90_GK][GT + 1];  // [re/im][k][row]
+  __shared__ R Bs[2][W90_GK][GT + 1];  // [re/im][k][col]
+  __shared__ double red[256];
+  rot_gemm_tile<T2, STAGE, GT>(p, it.k, b, tile, row0, col0, mode, st->cur, As, Bs, red);
+}
+
+// Both stages in one workgroup (item, b) when one tile covers the block
+// (num_wann <= GT): T = E_k^+ src goes to the block's scratch slot and is
+// read back by the same workgroup after a barrier (it stays in the cache),
+// with no kernel boundary between the stages.
+template <class T2, int GT>
+__global__ void __launch_bounds__(256) k_rot_fused(const Prob* probs, const Item* items, int nitems, int mode) {
+  typedef typename Real<T2>::type R;
+  if (blockIdx.x >= (unsigned)nitems) return;
+  const Item it = items[blockIdx.x];
+  const Prob& p = probs[it.prob];
+  const State* st = p.st;
+  if (!gate(st, mode)) return;
+  const int b = blockIdx.y;
+  if (b >= p.nn) return;
+  const int cur = st->cur;
+  __shared__ R As[2][W90_GK][GT + 1];
+  __shared__ R Bs[2][W90_GK][GT + 1];
+  __shared__ double red[256];
+  rot_gemm_tile<T2, 1, GT>(p, it.k, b, 0, 0, 0, mode, cur, As, Bs, red);
+  __syncthreads();
+  rot_gemm_tile<T2, 2, GT>(p, it.k, b, 0, 0, 0, mode, cur, As, Bs, red);
 }
 
 // Omega partial sums of the current M by GEMM tile (MODE_INIT of the MFMA
