@@ -841,6 +841,22 @@ __device__ __forceinline__ cplx domega_term(cplx acc, double w, cplx mmn, cplx m
   return acc;
 }
 
+// The same term with the divisions taken out: rn = 1/M_nn and
+// tn = wb Im ln M_nn + wb b.rave_n precomputed per (b, n), so that
+// T_mn = M_mn rn tn.
+__device__ __forceinline__ cplx domega_term_r(cplx acc, double w, cplx mmn, cplx mnm, cplx mnn, cplx mmm, cplx rn,
+                                              cplx rm, double tn, double tm) {
+  const cplx mhalf_i = c_make(0.0, -0.5);
+  const cplx cr_mn = c_mul(mmn, c_conj(mnn));
+  const cplx cr_nm = c_mul(mnm, c_conj(mmm));
+  // A[R] = (R - R^+)/2
+  acc = c_add(acc, c_scale(c_sub(cr_mn, c_conj(cr_nm)), w * 0.5));
+  // -S[T] = -(T + T^+)/2i
+  const cplx t_mn = c_scale(c_mul(mmn, rn), tn);
+  const cplx t_nm = c_scale(c_mul(mnm, rm), tm);
+  return c_sub(acc, c_mul(c_add(t_mn, c_conj(t_nm)), mhalf_i));
+}
+
 // wann_domega (wannierise.F90:2287-2569), non-SLWF gradient, one
 // workgroup per (problem, k), fused with the per-k partial sums Re<G,G> and
 // Re<G,D_old> needed by internal_search_direction.
@@ -905,13 +921,13 @@ __host__ __device__ constexpr int domega_chunk() {
 template <int NWM, class T2>
 __host__ __device__ constexpr size_t domega_lds() {
   return sizeof(T2) * (size_t)NWM * (NWM + 1) * domega_chunk<NWM>() +
-         sizeof(double) * (2 * W90GPU_NUM_NNMAX * NWM + domega_nt<NWM>());
+         sizeof(double) * (3 * W90GPU_NUM_NNMAX * NWM + domega_nt<NWM>());
 }
 
 // wann_domega with the blocks M(k,b) staged in LDS (num_wann <= NWM): the
 // transposed and diagonal reads of the gradient formula hit LDS (padded
-// leading dimension), and the scaled phases wb Im ln M_nn and b.rave_n of
-// the k are precomputed.  The blocks are loaded in chunks of
+// leading dimension), and per (b, n) the phase wb (Im ln M_nn + b.rave_n)
+// and the reciprocal 1/M_nn are precomputed (no division per element).  The blocks are loaded in chunks of
 // domega_chunk<NWM>() (every load of a chunk in flight at once), the next
 // chunk into registers while the current one is used; the sum over b runs
 // in b order whatever the chunking.
@@ -932,24 +948,28 @@ __global__ void __launch_bounds__(domega_nt<NWM>()) k_domega_k(const Prob* probs
   const int tid = threadIdx.x;
   const int cur = st->cur;
   extern __shared__ double lds_domega[];
-  double* q = lds_domega;                   // [b][n] wb Im ln M_nn
-  double* rb = q + W90GPU_NUM_NNMAX * NWM;  // [b][n] b.rave_n
-  double* red = rb + W90GPU_NUM_NNMAX * NWM;
+  double* qt = lds_domega;                    // [b][n] wb (Im ln M_nn + b.rave_n)
+  cplx* rec = reinterpret_cast<cplx*>(qt + W90GPU_NUM_NNMAX * NWM);  // [b][n] 1/M_nn
+  double* red = qt + 3 * W90GPU_NUM_NNMAX * NWM;
   T2* Ms = reinterpret_cast<T2*>(red + NT);
   const double* rave = p.rave[cur];
+  const T2* Mk = mbuf<T2>(p, cur) + (size_t)k * nn * nw2;
   for (int t = tid; t < nn * nw; t += NT) {
     const int b = t / nw, n = t % nw;
     const size_t kb = (size_t)k * nn + b;
     const double* bv = p.bk + kb * 3;
-    q[t] = p.wb[b] * p.ln[cur][kb * nw + n];
-    rb[t] = bv[0] * rave[0 + 3 * n] + bv[1] * rave[1 + 3 * n] + bv[2] * rave[2 + 3 * n];
+    const double w = p.wb[b];
+    const double rbn = bv[0] * rave[0 + 3 * n] + bv[1] * rave[1 + 3 * n] + bv[2] * rave[2 + 3 * n];
+    qt[t] = w * p.ln[cur][kb * nw + n] + w * rbn;
+    const cplx d = c_wide(Mk[(size_t)b * nw2 + n + (size_t)n * nw]);
+    const double inv = 1.0 / c_abs2(d);
+    rec[t] = make_double2(d.x * inv, -d.y * inv);
   }
   cplx acc[SL];
 #pragma unroll
   for (int s = 0; s < SL; ++s) acc[s] = c_make(0.0, 0.0);
   __shared__ double wL[W90GPU_NUM_NNMAX];
   if (tid < nn) wL[tid] = p.wb[tid];
-  const T2* Mk = mbuf<T2>(p, cur) + (size_t)k * nn * nw2;
   const int bs = NWM * (NWM + 1);  // LDS stride of the chunk's blocks
   T2 nxt[CB][SL];
 #pragma unroll
@@ -992,9 +1012,9 @@ __global__ void __launch_bounds__(domega_nt<NWM>()) k_domega_k(const Prob* probs
         const int e = tid + s * NT;
         if (e < nw2) {
           const int m = e % nw, n = e / nw;
-          acc[s] = domega_term(acc[s], w, c_wide(Mb[m + n * ld]), c_wide(Mb[n + m * ld]), c_wide(Mb[n + n * ld]),
-                               c_wide(Mb[m + m * ld]), q[b * nw + n], q[b * nw + m], rb[b * nw + n],
-                               rb[b * nw + m]);
+          acc[s] = domega_term_r(acc[s], w, c_wide(Mb[m + n * ld]), c_wide(Mb[n + m * ld]), c_wide(Mb[n + n * ld]),
+                                 c_wide(Mb[m + m * ld]), rec[b * nw + n], rec[b * nw + m], qt[b * nw + n],
+                                 qt[b * nw + m]);
         }
       }
     }

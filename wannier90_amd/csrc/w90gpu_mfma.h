@@ -267,7 +267,7 @@ __device__ __forceinline__ void rot_gemm_tile(const Prob& p, int k, int b, int t
 
 // Two-stage rotation, one launch per stage: workgroup (item, b, tile).
 template <class T2, int STAGE, int GT>
-__global__ void __launch_bounds__(256) k_rot_gemm(const Prob* probs, const Item* items, int nitems, int mode) {
+__global__ void __launch_bounds__(256, 3) k_rot_gemm(const Prob* probs, const Item* items, int nitems, int mode) {
   typedef typename Real<T2>::type R;
   if (blockIdx.x >= (unsigned)nitems) return;
   const Item it = items[blockIdx.x];
@@ -292,7 +292,7 @@ __global__ void __launch_bounds__(256) k_rot_gemm(const Prob* probs, const Item*
 // read back by the same workgroup after a barrier (it stays in the cache),
 // with no kernel boundary between the stages.
 template <class T2, int GT>
-__global__ void __launch_bounds__(256) k_rot_fused(const Prob* probs, const Item* items, int nitems, int mode) {
+__global__ void __launch_bounds__(256, 3) k_rot_fused(const Prob* probs, const Item* items, int nitems, int mode) {
   typedef typename Real<T2>::type R;
   if (blockIdx.x >= (unsigned)nitems) return;
   const Item it = items[blockIdx.x];
