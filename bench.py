@@ -125,12 +125,16 @@ WORKLOADS = {
 }
 
 
+# file stem of each workload's kernel profiles under profiles/
+PROFILE_STEM = {1: "sc4_64k", 2: "fcc16_512k", 3: "triclinic64_800k"}
+
+
 def measured_traffic(config, precision):
     """HBM bytes per launch of the rotation kernel from the committed PMC
     summary of the same workload (profiles/*_pmc.json: FETCH_SIZE and
     WRITE_SIZE per dispatch, scaled by the same counters on a copy kernel
     of known size), or None."""
-    f = os.path.join(ROOT, "profiles", f"rotate_cfg{config}_{precision}_pmc.json")
+    f = os.path.join(ROOT, "profiles", f"rotate_{PROFILE_STEM[config]}_{precision}_pmc.json")
     if not os.path.exists(f):
         return None
     with open(f) as fh:
@@ -249,7 +253,8 @@ def main():
         achieved = rot_flops / (avg_ms * 1e-3) / 1e12
         roof = dict(bound="mfma", achieved=achieved, peak=peak_flops, unit="TFLOP/s",
                     frac=achieved / peak_flops, traffic=None)
-    kname = "k_rotate_w16" if nw <= 16 else "k_rot_gemm stage 1 + stage 2 + k_kpart"
+    kname = ("k_rotate_w16" if nw <= 16 else "k_rot_fused + k_kpart" if nw <= 64
+             else "k_rot_gemm stage 1 + stage 2 + k_kpart")
     roof.update(kernel=kname, avg_launch_ms=avg_ms, launches=launches,
                 algorithmic_bytes=rot_bytes, algorithmic_flops=rot_flops)
     roof["traffic"] = measured_traffic(args.config, args.precision)

@@ -337,8 +337,8 @@ def run_serve(args):
             "serial_plain_capacity_solves_s": cap,
             "arrival_rate": rate,
             "max_in_flight": max_running,
-            "mixed_within_1e8_omega_1e6_centres": sum(ok),
-            "mixed_same_wannier_set": sum(ok_set),
+            "within_1e8_omega_1e6_centres_vs_alone": sum(ok),
+            "same_wannier_set_vs_alone": sum(ok_set),
             "max_delta_omega": max(d_om),
             "requests": n,
         },
