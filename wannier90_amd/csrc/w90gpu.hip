@@ -238,7 +238,7 @@ size_t problem_layout(int nw, int nk, int nn, int num_iter, Layout& lay) {
   lay.add(lnb);  // ln[0]
   lay.add(lnb);  // ln[1]
   lay.add(lnb);  // d2
-  lay.add(wann_class(nw) >= 1 ? mb : 256);                  // Tm (MFMA rotation scratch)
+  lay.add(wann_class(nw) >= 3 ? mb : 256);                  // Tm (two-stage rotation scratch)
   lay.add((size_t)nk * nn * npanels(nw) * sizeof(double));  // part_od
   lay.add((size_t)nk * nn * npanels(nw) * sizeof(double));  // part_all
   lay.add((size_t)nk * psz * sizeof(double));               // kpart
@@ -694,24 +694,25 @@ int launch_rotate(Engine* E, const Lists& L, int mode) {
       } else {
         // MFMA GEMM tiles; gridDim.y covers b and the tiles of a block
         const dim3 g2(g.n, g.nnmax * g.npanmax);
-        // classes 1 and 2 (one tile per block): both products in one launch
-        const dim3 gb(g.n, g.nnmax);
+        // classes 1 and 2 (one tile per block): both products in one launch,
+        // XCD-aware one-dimensional grid (see k_rot_fused)
+        const dim3 gb((unsigned)(8 * ((g.n + 7) / 8) * g.nnmax));
         if (c == 1) {
           if (mode == MODE_INIT) {
             if (q == 0) hipLaunchKernelGGL((k_reduce_tiles<cplx, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
             else hipLaunchKernelGGL((k_reduce_tiles<float2, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
           } else if (q == 0) {
-            hipLaunchKernelGGL((k_rot_fused<cplx, 32>), gb, dim3(256), 0, s, P, it, g.n, mode);
+            hipLaunchKernelGGL((k_rot_fused<cplx, 32>), gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
           } else {
-            hipLaunchKernelGGL((k_rot_fused<float2, 32>), gb, dim3(256), 0, s, P, it, g.n, mode);
+            hipLaunchKernelGGL((k_rot_fused<float2, 32>), gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
           }
         } else {
           if (mode == MODE_INIT) {
             if (q == 0) hipLaunchKernelGGL((k_reduce_tiles<cplx, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
             else hipLaunchKernelGGL((k_reduce_tiles<float2, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
           } else if (c == 2) {
-            if (q == 0) hipLaunchKernelGGL((k_rot_fused<cplx, 64>), gb, dim3(256), 0, s, P, it, g.n, mode);
-            else hipLaunchKernelGGL((k_rot_fused<float2, 64>), gb, dim3(256), 0, s, P, it, g.n, mode);
+            if (q == 0) hipLaunchKernelGGL((k_rot_fused<cplx, 64>), gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
+            else hipLaunchKernelGGL((k_rot_fused<float2, 64>), gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
           } else if (q == 0) {
             hipLaunchKernelGGL((k_rot_gemm<cplx, 1, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
             hipLaunchKernelGGL((k_rot_gemm<cplx, 2, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);

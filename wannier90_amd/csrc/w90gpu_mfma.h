@@ -287,27 +287,255 @@ __global__ void __launch_bounds__(256, 3) k_rot_gemm(const Prob* probs, const It
   rot_gemm_tile<T2, STAGE, GT>(p, it.k, b, tile, row0, col0, mode, st->cur, As, Bs, red);
 }
 
-// Both stages in one workgroup (item, b) when one tile covers the block
-// (num_wann <= GT): T = E_k^+ src goes to the block's scratch slot and is
-// read back by the same workgroup after a barrier (it stays in the cache),
-// with no kernel boundary between the stages.
+// K chunk of the second product of k_rot_fused (T held in LDS).
+#define W90_GK2 8
+
+// LDS elements (storage-precision reals) of k_rot_fused: the staging of the
+// first product (A and B chunks) and, after it, T with the B chunks of the
+// second product share one buffer.
+template <class R, int GT>
+__host__ __device__ constexpr int rot_fused_elems() {
+  return (4 * W90_GK * (GT + 1)) > (2 * GT * (GT + 1) + 2 * W90_GK2 * (GT + 1))
+             ? 4 * W90_GK * (GT + 1)
+             : 2 * GT * (GT + 1) + 2 * W90_GK2 * (GT + 1);
+}
+
+// Both products of the rotation of block (k, b) by one workgroup, when one
+// GT x GT tile covers the block (num_wann <= GT).  T = E_k^+ src is formed
+// as in k_rot_gemm stage 1, then written from the accumulators into LDS
+// (planar, k-major: T[i][l] at Ts[p][l][i], zero outside num_wann) over the
+// staging buffers of the first product; R = T E_{k+b} takes its A
+// fragments from there and its B operand in chunks of W90_GK2 rows of
+// E_{k+b}, loaded one chunk ahead.  The epilogue is that of stage 2.
 template <class T2, int GT>
-__global__ void __launch_bounds__(256, 3) k_rot_fused(const Prob* probs, const Item* items, int nitems, int mode) {
+__device__ __forceinline__ void rot_fused_tile(const Prob& p, int k, int b, int mode, int cur,
+                                               typename Real<T2>::type* lds, double* red) {
+  constexpr int FR = GT / 32;
+  constexpr int LDA = GT + 1;
   typedef typename Real<T2>::type R;
-  if (blockIdx.x >= (unsigned)nitems) return;
-  const Item it = items[blockIdx.x];
+  typedef typename Mfma<R>::acc Acc;
+  const int nw = p.nw, nn = p.nn;
+  const size_t kb = (size_t)k * nn + b;
+  const size_t nw2 = p.nw2;
+  const int dst = dst_buf(mode, cur);
+  const cplx* Ek = p.E + (size_t)k * nw2;
+  const cplx* Ekb = p.E + (size_t)p.nnl[kb] * nw2;
+  const T2* src = rot_src<T2>(p, mode, cur) + kb * nw2;
+  T2* out = mbuf<T2>(p, dst) + kb * nw2;
+  double* lnd = p.ln[dst] + kb * nw;
+  double* d2d = p.d2 + kb * nw;
+  double* pod = p.part_od + kb * p.npan;
+  double* pall = p.part_all + kb * p.npan;
+  R* As = lds;                     // [2][GK][LDA]   first product
+  R* Bs = lds + 2 * W90_GK * LDA;  // [2][GK][LDA]   first product
+  R* Ts = lds;                     // [2][GT][LDA]   T, k-major
+  R* B2 = lds + 2 * GT * LDA;      // [2][GK2][LDA]  second product
+
+  const int tid = threadIdx.x;
+  const int w = tid >> 6, lane = tid & 63;
+  const int wr = (w >> 1) * (GT / 2), wc = (w & 1) * (GT / 2);
+  const int lr = lane & 15, lk = lane >> 4;
+  bool fv_r[FR], fv_c[FR];
+#pragma unroll
+  for (int f = 0; f < FR; ++f) {
+    fv_r[f] = wr + f * 16 < nw;
+    fv_c[f] = wc + f * 16 < nw;
+  }
+  Acc Cr[FR][FR], Ci[FR][FR];
+#pragma unroll
+  for (int fi = 0; fi < FR; ++fi)
+#pragma unroll
+    for (int fj = 0; fj < FR; ++fj)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        Cr[fi][fj][i] = 0;
+        Ci[fi][fj][i] = 0;
+      }
+
+  // ---- T = E_k^+ src, K chunks of GK loaded one chunk ahead
+  constexpr int NE = W90_GK * GT / 256;
+  static_assert(NE * 256 == W90_GK * GT, "whole chunk per thread slot");
+  cplx pa[NE], pb[NE];
+  auto load1 = [&](int kk) {
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      const int e = tid + 256 * u;
+      const int l = e % W90_GK, x = e / W90_GK;
+      const int gl = kk + l;
+      cplx v = c_make(0.0, 0.0), v2 = c_make(0.0, 0.0);
+      if (x < nw && gl < nw) {
+        v = Ek[gl + (size_t)x * nw];  // A[x][l] = conj(E_k[l + x nw])
+        v2 = c_wide(src[gl + (size_t)x * nw]);  // B[l][x] = src[l + x nw]
+      }
+      v.y = -v.y;
+      pa[u] = v;
+      pb[u] = v2;
+    }
+  };
+  load1(0);
+  for (int kk = 0; kk < nw; kk += W90_GK) {
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      const int e = tid + 256 * u;
+      const int l = e % W90_GK, x = e / W90_GK;
+      As[(0 * W90_GK + l) * LDA + x] = (R)pa[u].x;
+      As[(1 * W90_GK + l) * LDA + x] = (R)pa[u].y;
+      Bs[(0 * W90_GK + l) * LDA + x] = (R)pb[u].x;
+      Bs[(1 * W90_GK + l) * LDA + x] = (R)pb[u].y;
+    }
+    if (kk + W90_GK < nw) load1(kk + W90_GK);
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < W90_GK; ks += 4) {
+      R ar[FR], ai[FR], br[FR], bi[FR];
+#pragma unroll
+      for (int f = 0; f < FR; ++f) {
+        ar[f] = As[(0 * W90_GK + ks + lk) * LDA + wr + f * 16 + lr];
+        ai[f] = As[(1 * W90_GK + ks + lk) * LDA + wr + f * 16 + lr];
+        br[f] = Bs[(0 * W90_GK + ks + lk) * LDA + wc + f * 16 + lr];
+        bi[f] = Bs[(1 * W90_GK + ks + lk) * LDA + wc + f * 16 + lr];
+      }
+#pragma unroll
+      for (int fi = 0; fi < FR; ++fi) {
+        if (!fv_r[fi]) continue;
+#pragma unroll
+        for (int fj = 0; fj < FR; ++fj) {
+          if (!fv_c[fj]) continue;
+          Cr[fi][fj] = Mfma<R>::mma(ar[fi], br[fj], Cr[fi][fj]);
+          Cr[fi][fj] = Mfma<R>::mma(-ai[fi], bi[fj], Cr[fi][fj]);
+          Ci[fi][fj] = Mfma<R>::mma(ar[fi], bi[fj], Ci[fi][fj]);
+          Ci[fi][fj] = Mfma<R>::mma(ai[fi], br[fj], Ci[fi][fj]);
+        }
+      }
+    }
+  }
+  // every wavefront is done with the staging buffers before T overwrites them
+  __syncthreads();
+#pragma unroll
+  for (int fi = 0; fi < FR; ++fi)
+#pragma unroll
+    for (int fj = 0; fj < FR; ++fj)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = wr + fi * 16 + Mfma<R>::row(lk, i);
+        const int c = wc + fj * 16 + lr;
+        Ts[(0 * GT + c) * LDA + r] = Cr[fi][fj][i];
+        Ts[(1 * GT + c) * LDA + r] = Ci[fi][fj][i];
+        Cr[fi][fj][i] = 0;
+        Ci[fi][fj][i] = 0;
+      }
+
+  // ---- R = T E_{k+b}, K chunks of GK2 rows of E_{k+b} loaded one ahead
+  constexpr int NE2 = W90_GK2 * GT / 256;
+  static_assert(NE2 * 256 == W90_GK2 * GT, "whole chunk per thread slot");
+  cplx pc[NE2];
+  auto load2 = [&](int kk) {
+#pragma unroll
+    for (int u = 0; u < NE2; ++u) {
+      const int e = tid + 256 * u;
+      const int l = e % W90_GK2, j = e / W90_GK2;
+      const int gl = kk + l;
+      cplx v = c_make(0.0, 0.0);
+      if (j < nw && gl < nw) v = Ekb[gl + (size_t)j * nw];
+      pc[u] = v;
+    }
+  };
+  load2(0);
+  for (int kk = 0; kk < nw; kk += W90_GK2) {
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < NE2; ++u) {
+      const int e = tid + 256 * u;
+      const int l = e % W90_GK2, j = e / W90_GK2;
+      B2[(0 * W90_GK2 + l) * LDA + j] = (R)pc[u].x;
+      B2[(1 * W90_GK2 + l) * LDA + j] = (R)pc[u].y;
+    }
+    if (kk + W90_GK2 < nw) load2(kk + W90_GK2);
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < W90_GK2; ks += 4) {
+      R ar[FR], ai[FR], br[FR], bi[FR];
+#pragma unroll
+      for (int f = 0; f < FR; ++f) {
+        ar[f] = Ts[(0 * GT + kk + ks + lk) * LDA + wr + f * 16 + lr];
+        ai[f] = Ts[(1 * GT + kk + ks + lk) * LDA + wr + f * 16 + lr];
+        br[f] = B2[(0 * W90_GK2 + ks + lk) * LDA + wc + f * 16 + lr];
+        bi[f] = B2[(1 * W90_GK2 + ks + lk) * LDA + wc + f * 16 + lr];
+      }
+#pragma unroll
+      for (int fi = 0; fi < FR; ++fi) {
+        if (!fv_r[fi]) continue;
+#pragma unroll
+        for (int fj = 0; fj < FR; ++fj) {
+          if (!fv_c[fj]) continue;
+          Cr[fi][fj] = Mfma<R>::mma(ar[fi], br[fj], Cr[fi][fj]);
+          Cr[fi][fj] = Mfma<R>::mma(-ai[fi], bi[fj], Cr[fi][fj]);
+          Ci[fi][fj] = Mfma<R>::mma(ar[fi], bi[fj], Ci[fi][fj]);
+          Ci[fi][fj] = Mfma<R>::mma(ai[fi], br[fj], Ci[fi][fj]);
+        }
+      }
+    }
+  }
+
+  // ---- epilogue: R to the other buffer, diagonal phases and tile sums
+  double od = 0.0, all = 0.0;
+#pragma unroll
+  for (int fi = 0; fi < FR; ++fi)
+#pragma unroll
+    for (int fj = 0; fj < FR; ++fj)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = wr + fi * 16 + Mfma<R>::row(lk, i);
+        const int c = wc + fj * 16 + lr;
+        if (r < nw && c < nw) {
+          T2 v;
+          v.x = Cr[fi][fj][i];
+          v.y = Ci[fi][fj][i];
+          out[r + (size_t)c * nw] = v;
+          const cplx vw = c_wide(v);
+          const double a2 = c_abs2(vw);
+          all += a2;
+          if (r != c) {
+            od += a2;
+          } else {
+            lnd[r] = atan2(vw.y, vw.x);
+            d2d[r] = a2;
+          }
+        }
+      }
+  od = block_sum(od, red);
+  all = block_sum(all, red);
+  if (tid == 0) {
+    pod[0] = od;
+    pall[0] = all;
+  }
+}
+
+// The rotation for num_wann <= GT (one tile per block), both products in
+// one workgroup per (item, b) with T in LDS (rot_fused_tile).  Workgroups
+// are placed XCD-aware: a one-dimensional grid of 8 * ceil(nitems / 8) *
+// nnmax workgroups, where workgroup w runs on XCD w % 8 (round-robin
+// dispatch) and takes item 8 t + w % 8 and neighbour b with
+// (t, b) = divmod(w / 8, nnmax): the nnmax workgroups of one k run back to
+// back on one XCD and share E_k through its L2.
+template <class T2, int GT>
+__global__ void __launch_bounds__(256, 2) k_rot_fused(const Prob* probs, const Item* items, int nitems, int nnmax,
+                                                      int mode) {
+  typedef typename Real<T2>::type R;
+  const int w = blockIdx.x;
+  const int r = w >> 3;
+  const int item = 8 * (r / nnmax) + (w & 7);
+  const int b = r % nnmax;
+  if (item >= nitems) return;
+  const Item it = items[item];
   const Prob& p = probs[it.prob];
   const State* st = p.st;
   if (!gate(st, mode)) return;
-  const int b = blockIdx.y;
   if (b >= p.nn) return;
-  const int cur = st->cur;
-  __shared__ R As[2][W90_GK][GT + 1];
-  __shared__ R Bs[2][W90_GK][GT + 1];
+  __shared__ R lds[rot_fused_elems<R, GT>()];
   __shared__ double red[256];
-  rot_gemm_tile<T2, 1, GT>(p, it.k, b, 0, 0, 0, mode, cur, As, Bs, red);
-  __syncthreads();
-  rot_gemm_tile<T2, 2, GT>(p, it.k, b, 0, 0, 0, mode, cur, As, Bs, red);
+  rot_fused_tile<T2, GT>(p, it.k, b, mode, st->cur, lds, red);
 }
 
 // Omega partial sums of the current M by GEMM tile (MODE_INIT of the MFMA
