@@ -1,19 +1,22 @@
-// Matrix-core (MFMA) kernels of libw90gpu for num_wann above the LDS
-// kernels (num_wann > 32): the M rotation M(k,b) <- E_k^+ M(k,b) E_{k+b} as
-// two batched complex GEMMs on v_mfma_*_16x16x4 tiles, FP64 for the plain
-// path and FP32 with FP32 accumulation for the mixed bulk phase.
+// Matrix-core (MFMA) kernels of libw90gpu: the M rotation
+// M(k,b) <- E_k^+ M(k,b) E_{k+b} as two complex products on
+// v_mfma_*_16x16x4 tiles (FP64 for the plain path, FP32 with FP32
+// accumulation for the mixed bulk phase), and the products of the
+// retraction above num_wann 16.
 //
-// A workgroup computes a GT x GT tile of one product for one (problem, k,
-// b): gridDim.y covers b and the tiles; GT = 64, or 32 for num_wann <= 32.
-// Four wavefronts hold a GT/2 x GT/2 quadrant of the tile each, as 16 x 16
-// fragments with the real and imaginary parts in separate accumulators: a
-// complex product is four real MFMA products.
-// K runs in chunks of GK staged in LDS with coalesced global loads (planar
-// re/im, padded rows).  Fragments wholly outside num_wann are skipped.
-//
-//   stage 1: T(k,b) = E_k^+ src(k,b)      (scratch T, storage precision)
-//   stage 2: dst(k,b) = T(k,b) E_{k+b}    with the omega partial sums of the
-//            tile (Im ln M_nn, |M_nn|^2, sum |M_mn|^2, off-diagonal sum)
+// num_wann <= 16 (k_rotate_w16): one 16 x 16 tile per block, one wavefront
+// per neighbour b.  num_wann 17-64 (k_rot_fused): a workgroup computes both
+// products of one (k, b) block as one GT x GT tile (GT = 32 or 64), T kept
+// in LDS between them.  num_wann 65-128 (k_rot_gemm): each product is its
+// own launch over 2 x 2 tiles of 64, T through a scratch buffer.  Four
+// wavefronts hold a GT/2 x GT/2 quadrant of a tile each, as 16 x 16
+// fragments; a complex fragment product is three real MFMA products per K
+// step (Cfrag, Gauss's form).  K runs in chunks staged in LDS with
+// coalesced global loads (planar re/im, padded rows), the next chunk
+// loaded into registers while the current one is multiplied.  Fragments
+// wholly outside num_wann are skipped.  The second product's epilogue
+// forms the omega partial sums of the tile (Im ln M_nn, |M_nn|^2,
+// sum |M_mn|^2, off-diagonal sum).
 //
 // MFMA 16x16x4 lane layout (measured on gfx950): lane l holds A[l%16][l/16]
 // and B[l/16][l%16]; element i of its result is D[r][l%16] with
@@ -50,6 +53,30 @@ struct Mfma<float> {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
   static __device__ __forceinline__ int row(int lk, int i) { return 4 * lk + i; }
+};
+
+// Complex 16 x 16 fragment product accumulated with three real MFMA
+// products per K step (Gauss): P1 += ar br, P2 += ai bi,
+// P3 += (ar + ai)(br + bi); the product is Re = P1 - P2,
+// Im = P3 - P1 - P2.  Three MFMA instructions instead of four.
+template <class R>
+struct Cfrag {
+  typename Mfma<R>::acc p1, p2, p3;
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      p1[i] = 0;
+      p2[i] = 0;
+      p3[i] = 0;
+    }
+  }
+  __device__ __forceinline__ void mma(R ar, R ai, R br, R bi) {
+    p1 = Mfma<R>::mma(ar, br, p1);
+    p2 = Mfma<R>::mma(ai, bi, p2);
+    p3 = Mfma<R>::mma(ar + ai, br + bi, p3);
+  }
+  __device__ __forceinline__ R re(int i) const { return p1[i] - p2[i]; }
+  __device__ __forceinline__ R im(int i) const { return p3[i] - p1[i] - p2[i]; }
 };
 
 template <class T2>
@@ -147,16 +174,11 @@ __device__ __forceinline__ void rot_gemm_tile(const Prob& p, int k, int b, int t
     fv_r[f] = row0 + wr + f * 16 < nw;
     fv_c[f] = col0 + wc + f * 16 < nw;
   }
-  Acc Cr[FR][FR], Ci[FR][FR];
+  Cfrag<R> C[FR][FR];
 #pragma unroll
   for (int fi = 0; fi < FR; ++fi)
 #pragma unroll
-    for (int fj = 0; fj < FR; ++fj)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        Cr[fi][fj][i] = 0;
-        Ci[fi][fj][i] = 0;
-      }
+    for (int fj = 0; fj < FR; ++fj) C[fi][fj].zero();
 
   // K chunks are software-pipelined: the operands of chunk kk + GK are
   // loaded into registers before the products of chunk kk, and stored to
@@ -218,10 +240,7 @@ __device__ __forceinline__ void rot_gemm_tile(const Prob& p, int k, int b, int t
 #pragma unroll
         for (int fj = 0; fj < FR; ++fj) {
           if (!fv_c[fj]) continue;
-          Cr[fi][fj] = Mfma<R>::mma(ar[fi], br[fj], Cr[fi][fj]);
-          Cr[fi][fj] = Mfma<R>::mma(-ai[fi], bi[fj], Cr[fi][fj]);
-          Ci[fi][fj] = Mfma<R>::mma(ar[fi], bi[fj], Ci[fi][fj]);
-          Ci[fi][fj] = Mfma<R>::mma(ai[fi], br[fj], Ci[fi][fj]);
+          C[fi][fj].mma(ar[fi], ai[fi], br[fj], bi[fj]);
         }
       }
     }
@@ -239,8 +258,8 @@ __device__ __forceinline__ void rot_gemm_tile(const Prob& p, int k, int b, int t
         const int c = col0 + wc + fj * 16 + lr;
         if (r < nw && c < nw) {
           T2 v;
-          v.x = Cr[fi][fj][i];
-          v.y = Ci[fi][fj][i];
+          v.x = C[fi][fj].re(i);
+          v.y = C[fi][fj].im(i);
           out[r + (size_t)c * nw] = v;
           if (STAGE == 2) {
             const cplx vw = c_wide(v);
@@ -267,7 +286,7 @@ __device__ __forceinline__ void rot_gemm_tile(const Prob& p, int k, int b, int t
 
 // Two-stage rotation, one launch per stage: workgroup (item, b, tile).
 template <class T2, int STAGE, int GT>
-__global__ void __launch_bounds__(256, 3) k_rot_gemm(const Prob* probs, const Item* items, int nitems, int mode) {
+__global__ void __launch_bounds__(256, 2) k_rot_gemm(const Prob* probs, const Item* items, int nitems, int mode) {
   typedef typename Real<T2>::type R;
   if (blockIdx.x >= (unsigned)nitems) return;
   const Item it = items[blockIdx.x];
@@ -341,16 +360,11 @@ __device__ __forceinline__ void rot_fused_tile(const Prob& p, int k, int b, int 
     fv_r[f] = wr + f * 16 < nw;
     fv_c[f] = wc + f * 16 < nw;
   }
-  Acc Cr[FR][FR], Ci[FR][FR];
+  Cfrag<R> C[FR][FR];
 #pragma unroll
   for (int fi = 0; fi < FR; ++fi)
 #pragma unroll
-    for (int fj = 0; fj < FR; ++fj)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        Cr[fi][fj][i] = 0;
-        Ci[fi][fj][i] = 0;
-      }
+    for (int fj = 0; fj < FR; ++fj) C[fi][fj].zero();
 
   // ---- T = E_k^+ src, K chunks of GK loaded one chunk ahead
   constexpr int NE = W90_GK * GT / 256;
@@ -402,10 +416,7 @@ __device__ __forceinline__ void rot_fused_tile(const Prob& p, int k, int b, int 
 #pragma unroll
         for (int fj = 0; fj < FR; ++fj) {
           if (!fv_c[fj]) continue;
-          Cr[fi][fj] = Mfma<R>::mma(ar[fi], br[fj], Cr[fi][fj]);
-          Cr[fi][fj] = Mfma<R>::mma(-ai[fi], bi[fj], Cr[fi][fj]);
-          Ci[fi][fj] = Mfma<R>::mma(ar[fi], bi[fj], Ci[fi][fj]);
-          Ci[fi][fj] = Mfma<R>::mma(ai[fi], br[fj], Ci[fi][fj]);
+          C[fi][fj].mma(ar[fi], ai[fi], br[fj], bi[fj]);
         }
       }
     }
@@ -420,11 +431,13 @@ __device__ __forceinline__ void rot_fused_tile(const Prob& p, int k, int b, int 
       for (int i = 0; i < 4; ++i) {
         const int r = wr + fi * 16 + Mfma<R>::row(lk, i);
         const int c = wc + fj * 16 + lr;
-        Ts[(0 * GT + c) * LDA + r] = Cr[fi][fj][i];
-        Ts[(1 * GT + c) * LDA + r] = Ci[fi][fj][i];
-        Cr[fi][fj][i] = 0;
-        Ci[fi][fj][i] = 0;
+        Ts[(0 * GT + c) * LDA + r] = C[fi][fj].re(i);
+        Ts[(1 * GT + c) * LDA + r] = C[fi][fj].im(i);
       }
+#pragma unroll
+  for (int fi = 0; fi < FR; ++fi)
+#pragma unroll
+    for (int fj = 0; fj < FR; ++fj) C[fi][fj].zero();
 
   // ---- R = T E_{k+b}, K chunks of GK2 rows of E_{k+b} loaded one ahead
   constexpr int NE2 = W90_GK2 * GT / 256;
@@ -469,10 +482,7 @@ __device__ __forceinline__ void rot_fused_tile(const Prob& p, int k, int b, int 
 #pragma unroll
         for (int fj = 0; fj < FR; ++fj) {
           if (!fv_c[fj]) continue;
-          Cr[fi][fj] = Mfma<R>::mma(ar[fi], br[fj], Cr[fi][fj]);
-          Cr[fi][fj] = Mfma<R>::mma(-ai[fi], bi[fj], Cr[fi][fj]);
-          Ci[fi][fj] = Mfma<R>::mma(ar[fi], bi[fj], Ci[fi][fj]);
-          Ci[fi][fj] = Mfma<R>::mma(ai[fi], br[fj], Ci[fi][fj]);
+          C[fi][fj].mma(ar[fi], ai[fi], br[fj], bi[fj]);
         }
       }
     }
@@ -490,8 +500,8 @@ __device__ __forceinline__ void rot_fused_tile(const Prob& p, int k, int b, int 
         const int c = wc + fj * 16 + lr;
         if (r < nw && c < nw) {
           T2 v;
-          v.x = Cr[fi][fj][i];
-          v.y = Ci[fi][fj][i];
+          v.x = C[fi][fj].re(i);
+          v.y = C[fi][fj].im(i);
           out[r + (size_t)c * nw] = v;
           const cplx vw = c_wide(v);
           const double a2 = c_abs2(vw);
@@ -595,9 +605,10 @@ __global__ void __launch_bounds__(256) k_reduce_tiles(const Prob* probs, const I
 __host__ __device__ constexpr size_t step_mfma_lds() { return sizeof(double) * (4 * W90_STEP_MAT + 256); }
 
 // C = A B for the 64 x 64 LDS matrices (A, B planar at Ar/Ai, Br/Bi): each
-// wavefront accumulates its 32 x 32 quadrant as 2 x 2 FP64 MFMA fragments;
-// fragments wholly outside n are skipped (their rows and columns are zero).
-__device__ __forceinline__ void step_gemm(w90_d4 (&Cr)[2][2], w90_d4 (&Ci)[2][2], const double* Ar,
+// wavefront accumulates its 32 x 32 quadrant as 2 x 2 complex FP64 MFMA
+// fragments (three real products per K step, Cfrag); fragments wholly
+// outside n are skipped (their rows and columns are zero).
+__device__ __forceinline__ void step_gemm(Cfrag<double> (&C)[2][2], const double* Ar,
                                           const double* Ai, const double* Br, const double* Bi, int n) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
@@ -608,8 +619,7 @@ __device__ __forceinline__ void step_gemm(w90_d4 (&Cr)[2][2], w90_d4 (&Ci)[2][2]
     for (int fj = 0; fj < 2; ++fj)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        Cr[fi][fj][i] = 0.0;
-        Ci[fi][fj][i] = 0.0;
+        C[fi][fj].p1[i] = 0.0, C[fi][fj].p2[i] = 0.0, C[fi][fj].p3[i] = 0.0;
       }
   const bool vr0 = wr < n, vr1 = wr + 16 < n, vc0 = wc < n, vc1 = wc + 16 < n;
   for (int ks = 0; ks < n; ks += 4) {
@@ -629,10 +639,7 @@ __device__ __forceinline__ void step_gemm(w90_d4 (&Cr)[2][2], w90_d4 (&Ci)[2][2]
 #pragma unroll
       for (int fj = 0; fj < 2; ++fj) {
         if (!(fj ? vc1 : vc0)) continue;
-        Cr[fi][fj] = Mfma<double>::mma(ar[fi], br[fj], Cr[fi][fj]);
-        Cr[fi][fj] = Mfma<double>::mma(-ai[fi], bi[fj], Cr[fi][fj]);
-        Ci[fi][fj] = Mfma<double>::mma(ar[fi], bi[fj], Ci[fi][fj]);
-        Ci[fi][fj] = Mfma<double>::mma(ai[fi], br[fj], Ci[fi][fj]);
+        C[fi][fj].mma(ar[fi], ai[fi], br[fj], bi[fj]);
       }
     }
   }
@@ -712,10 +719,10 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
     Pi[q] = xi * invm;
   }
   __syncthreads();
-  w90_d4 Cr[2][2], Ci[2][2];
+  Cfrag<double> C[2][2];
   // Horner: P <- I + X P / j
   for (int j = m - 1; j >= 1; --j) {
-    step_gemm(Cr, Ci, Xr, Xi, Pr, Pi, nw);
+    step_gemm(C, Xr, Xi, Pr, Pi, nw);
     __syncthreads();
     const double invj = 1.0 / (double)j;
 #pragma unroll
@@ -727,14 +734,14 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
           int r, c;
           step_pos(fi, fj, i, &r, &c);
           const int q = c * W90_STEP_LD + r;
-          Pr[q] = Cr[fi][fj][i] * invj + (r == c ? 1.0 : 0.0);
-          Pi[q] = Ci[fi][fj][i] * invj;
+          Pr[q] = C[fi][fj].re(i) * invj + (r == c ? 1.0 : 0.0);
+          Pi[q] = C[fi][fj].im(i) * invj;
         }
     __syncthreads();
   }
   // squarings: P <- P P
   for (int s = 0; s < sq; ++s) {
-    step_gemm(Cr, Ci, Pr, Pi, Pr, Pi, nw);
+    step_gemm(C, Pr, Pi, Pr, Pi, nw);
     __syncthreads();
 #pragma unroll
     for (int fi = 0; fi < 2; ++fi)
@@ -745,8 +752,8 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
           int r, c;
           step_pos(fi, fj, i, &r, &c);
           const int q = c * W90_STEP_LD + r;
-          Pr[q] = Cr[fi][fj][i];
-          Pi[q] = Ci[fi][fj][i];
+          Pr[q] = C[fi][fj].re(i);
+          Pi[q] = C[fi][fj].im(i);
         }
     __syncthreads();
   }
@@ -762,7 +769,7 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
     Xi[q] = u.y;
   }
   __syncthreads();
-  step_gemm(Cr, Ci, Xr, Xi, Pr, Pi, nw);
+  step_gemm(C, Xr, Xi, Pr, Pi, nw);
   cplx* Un = p.U[1 - cur] + (size_t)k * nw2;
 #pragma unroll
   for (int fi = 0; fi < 2; ++fi)
@@ -772,7 +779,7 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
       for (int i = 0; i < 4; ++i) {
         int r, c;
         step_pos(fi, fj, i, &r, &c);
-        if (r < nw && c < nw) Un[r + c * nw] = c_make(Cr[fi][fj][i], Ci[fi][fj][i]);
+        if (r < nw && c < nw) Un[r + c * nw] = c_make(C[fi][fj].re(i), C[fi][fj].im(i));
       }
 }
 
@@ -790,17 +797,13 @@ __device__ void wg_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n) {
   for (int t0 = 2 * w; t0 < ntt; t0 += 2 * nwv) {
     const bool two = t0 + 1 < ntt;
     int i0[2], j0[2];
-    w90_d4 cr[2], ci[2];
+    Cfrag<double> cc[2];
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       const int t = two ? t0 + u : t0;
       i0[u] = (t % nt) * 16;
       j0[u] = (t / nt) * 16;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        cr[u][i] = 0.0;
-        ci[u][i] = 0.0;
-      }
+      cc[u].zero();
     }
     for (int ks = 0; ks < n; ks += 4) {
       const int kk = ks + lk;
@@ -809,10 +812,7 @@ __device__ void wg_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n) {
         const int r = i0[u] + lr, c = j0[u] + lr;
         const cplx a = (r < n && kk < n) ? A[r + (size_t)kk * n] : c_make(0.0, 0.0);
         const cplx b = (kk < n && c < n) ? B[kk + (size_t)c * n] : c_make(0.0, 0.0);
-        cr[u] = Mfma<double>::mma(a.x, b.x, cr[u]);
-        cr[u] = Mfma<double>::mma(-a.y, b.y, cr[u]);
-        ci[u] = Mfma<double>::mma(a.x, b.y, ci[u]);
-        ci[u] = Mfma<double>::mma(a.y, b.x, ci[u]);
+        cc[u].mma(a.x, a.y, b.x, b.y);
       }
     }
 #pragma unroll
@@ -821,7 +821,7 @@ __device__ void wg_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = i0[u] + Mfma<double>::row(lk, i), c = j0[u] + lr;
-        if (r < n && c < n) C[r + (size_t)c * n] = c_make(cr[u][i], ci[u][i]);
+        if (r < n && c < n) C[r + (size_t)c * n] = c_make(cc[u].re(i), cc[u].im(i));
       }
     }
   }
@@ -895,12 +895,8 @@ __global__ void __launch_bounds__(64 * W90GPU_NUM_NNMAX) k_rotate_w16(const Prob
   T2* out = mbuf<T2>(p, dst) + kb * nw2;
   double* lnd = p.ln[dst] + (size_t)k * nn * nw;
   double* outk = p.kpart + (size_t)k * p.psz;
-  Acc Rr, Ri;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    Rr[i] = 0;
-    Ri[i] = 0;
-  }
+  Cfrag<R> Racc;
+  Racc.zero();
   // every operand fragment of both products is loaded up front, so the
   // global-memory latencies overlap
   cplx fa[4], fm[4], fb[4];
@@ -916,26 +912,18 @@ __global__ void __launch_bounds__(64 * W90GPU_NUM_NNMAX) k_rotate_w16(const Prob
   }
   if (active && rot) {
     // T = E_k^+ M: A[i][l] = conj(E_k[l + i nw]), B[l][j] = M[l + j nw]
-    Acc Tr, Ti;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      Tr[i] = 0;
-      Ti[i] = 0;
-    }
+    Cfrag<R> Tacc;
+    Tacc.zero();
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const cplx a = fa[s], v = fm[s];
-      const R ar = (R)a.x, ai = (R)(-a.y), br = (R)v.x, bi = (R)v.y;
-      Tr = Mfma<R>::mma(ar, br, Tr);
-      Tr = Mfma<R>::mma(-ai, bi, Tr);
-      Ti = Mfma<R>::mma(ar, bi, Ti);
-      Ti = Mfma<R>::mma(ai, br, Ti);
+      Tacc.mma((R)a.x, (R)(-a.y), (R)v.x, (R)v.y);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       T2 t;
-      t.x = Tr[i];
-      t.y = Ti[i];
+      t.x = Tacc.re(i);
+      t.y = Tacc.im(i);
       Tl[Mfma<R>::row(lk, i) + lr * 17] = t;
     }
   }
@@ -946,19 +934,15 @@ __global__ void __launch_bounds__(64 * W90GPU_NUM_NNMAX) k_rotate_w16(const Prob
     for (int s = 0; s < 4; ++s) {
       const int l = 4 * s + lk;
       const T2 a = Tl[lr + l * 17];
-      const R br = (R)fb[s].x, bi = (R)fb[s].y;
-      Rr = Mfma<R>::mma(a.x, br, Rr);
-      Rr = Mfma<R>::mma(-a.y, bi, Rr);
-      Ri = Mfma<R>::mma(a.x, bi, Ri);
-      Ri = Mfma<R>::mma(a.y, br, Ri);
+      Racc.mma(a.x, a.y, (R)fb[s].x, (R)fb[s].y);
     }
     // R replaces T in the wavefront's block (every read of T above feeds
     // the products that produce R)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       T2 t;
-      t.x = Rr[i];
-      t.y = Ri[i];
+      t.x = Racc.re(i);
+      t.y = Racc.im(i);
       Tl[Mfma<R>::row(lk, i) + lr * 17] = t;
     }
   }
