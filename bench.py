@@ -62,6 +62,10 @@ def parse():
     ap.add_argument("--num-iter", type=int, default=500)
     ap.add_argument("--pool-gb", type=float, default=160.0, help="serve: device pool for admission")
     ap.add_argument("--max-running", type=int, default=16, help="serve: problems in flight at most")
+    ap.add_argument("--large-running", type=int, default=2,
+                    help="serve: problems in flight at most in the lane of num_wann > 32")
+    ap.add_argument("--lanes", type=int, choices=[1, 2], default=2,
+                    help="serve: 2 runs num_wann <= 32 and larger requests in separate batches")
     return ap.parse_args()
 
 

@@ -20,6 +20,7 @@ on the same device).
 import multiprocessing as mp
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -260,49 +261,87 @@ def run_serve(args):
 
     # continuous batching, admission by device bytes and a cap on the
     # problems in flight; FP64 unless --precision mixed (FP64 results are
-    # bit-identical to the same request solved alone)
+    # bit-identical to the same request solved alone).  With two lanes the
+    # requests of num_wann <= 32 and the larger ones run in separate batches
+    # (own context, stream and device pool share) from two host threads, so
+    # a small request does not advance in lockstep with the large problems
+    # of its batch; the library calls release the interpreter lock.
     mixed = getattr(args, "precision", "fp64") == "mixed"
     ctl = control(args.num_iter, mixed)
     cap_running = max(1, int(getattr(args, "max_running", 16)))
+    # the large problems fill the device on their own: sharing it between
+    # many of them only stretches each one, so their lane admits fewer
+    cap_large = max(1, int(getattr(args, "large_running", 2)))
+    nlanes = 2 if int(getattr(args, "lanes", 2)) >= 2 else 1
     need = [capi.problem_bytes(p["nw"], p["nk"], p["nn"], args.num_iter) for p in probs]
-    pool = max(max(need), int(args.pool_gb * (1 << 30)))
-    b = ctx.batch(pool)
-    queue, nxt, done = [], 0, 0
-    owner, lat, res = {}, [None] * n, [None] * n
-    max_running = 0
+    pool_total = max(max(need), int(args.pool_gb * (1 << 30)))
+    lane_of = [0 if (nlanes == 1 or p["nw"] <= 32) else 1 for p in probs]
+    lanes = [[i for i in range(n) if lane_of[i] == ln] for ln in range(nlanes)]
+    # device pool per lane in proportion to the bytes its requests need
+    lane_bytes = [sum(need[i] for i in ids) for ids in lanes]
+    pools = [max([need[i] for i in ids] + [1 << 20], default=1 << 20) for ids in lanes]
+    pools = [max(pl, int(pool_total * lb / max(1, sum(lane_bytes)))) for pl, lb in zip(pools, lane_bytes)]
+    lat, res = [None] * n, [None] * n
+    peak = [0] * nlanes
+    errors = []
+
+    def lane(ln, start):
+        ids = lanes[ln]
+        if not ids:
+            return
+        lctx = capi.Context(0)
+        b = lctx.batch(pools[ln])
+        queue, nxt, done = [], 0, 0
+        owner = {}
+        try:
+            while done < len(ids):
+                t = time.perf_counter() - start
+                while nxt < len(ids) and arrivals[ids[nxt]] <= t:
+                    queue.append(ids[nxt])
+                    nxt += 1
+                while queue and len(owner) < (cap_running if ln == 0 else cap_large):
+                    i = queue[0]
+                    p = probs[i]
+                    h = b.add(kmesh_of(p), p["m0"], p["u0"], ctl)
+                    if h is None:
+                        break
+                    owner[h] = i
+                    queue.pop(0)
+                info = b.info()
+                peak[ln] = max(peak[ln], info["running"] + info["queued"])
+                if info["running"] + info["queued"] == 0:
+                    if nxt < len(ids):
+                        time.sleep(max(0.0, arrivals[ids[nxt]] - (time.perf_counter() - start)))
+                    continue
+                # a few blocks per step while nobody is waiting to be admitted
+                due = nxt < len(ids) and arrivals[ids[nxt]] <= time.perf_counter() - start
+                finished = b.step(1 if queue or due else 2)
+                t = time.perf_counter() - start
+                for h in finished:
+                    i = owner.pop(h)
+                    lat[i] = t - arrivals[i]
+                    r = b.result(h)
+                    _, _, cen, spr = b.fetch(h)
+                    res[i] = (r, cen, spr)
+                    b.remove(h)
+                    done += 1
+        except Exception as ex:  # reported after the lanes have joined
+            errors.append(ex)
+        finally:
+            b.close()
+            lctx.close()
+
     start = time.perf_counter()
-    while done < n:
-        t = time.perf_counter() - start
-        while nxt < n and arrivals[nxt] <= t:
-            queue.append(nxt)
-            nxt += 1
-        while queue and len(owner) < cap_running:
-            i = queue[0]
-            p = probs[i]
-            h = b.add(kmesh_of(p), p["m0"], p["u0"], ctl)
-            if h is None:
-                break
-            owner[h] = i
-            queue.pop(0)
-        info = b.info()
-        max_running = max(max_running, info["running"] + info["queued"])
-        if info["running"] + info["queued"] == 0:
-            if nxt < n:
-                time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - start)))
-            continue
-        # a few blocks per step while nobody is waiting to be admitted
-        finished = b.step(1 if queue or (nxt < n and arrivals[nxt] <= time.perf_counter() - start) else 2)
-        t = time.perf_counter() - start
-        for h in finished:
-            i = owner.pop(h)
-            lat[i] = t - arrivals[i]
-            r = b.result(h)
-            _, _, cen, spr = b.fetch(h)
-            res[i] = (r, cen, spr)
-            b.remove(h)
-            done += 1
+    threads = [threading.Thread(target=lane, args=(ln, start)) for ln in range(nlanes)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    if errors:
+        raise errors[0]
+    max_running = sum(peak)
+    pool = sum(pools)
     makespan = time.perf_counter() - start
-    b.close()
     d_om = [abs(a[0].omega_total - r[0].omega_total) for a, r in zip(plain, res)]
     ok = [do <= 1e-8 and float(np.abs(a[1] - r[1]).max()) <= 1e-6 for do, a, r in zip(d_om, plain, res)]
     ok_set = [do <= 1e-8 and same_set(r[1], r[2], a[1], a[2], p["real_lattice"])
@@ -328,9 +367,14 @@ def run_serve(args):
             "workload": f"open-loop Poisson serving, {n} requests at {rate:.3f}/s: sc/fcc/bcc/hex, num_wann "
                         f"U[4,96], k-mesh 4^3..12^3, num_iter={args.num_iter}, conv_window=3, conv_tol=1e-10, "
                         f"{'mixed precision' if mixed else 'FP64'}, continuous batching of up to {cap_running} "
-                        f"problems, admission by device bytes (pool {pool / (1 << 30):.0f} GiB)",
+                        f"problems ({cap_large} in the lane of num_wann > 32), {nlanes} lane(s), admission by device "
+                        f"bytes (pool {pool / (1 << 30):.0f} GiB)",
             "p50_s": pct(lat, 0.5),
             "p99_s": pct(lat, 0.99),
+            "lanes": nlanes,
+            "lane_p50_s": [pct([lat[i] for i in ids], 0.5) if ids else None for ids in lanes],
+            "lane_p99_s": [pct([lat[i] for i in ids], 0.99) if ids else None for ids in lanes],
+            "lane_requests": [len(ids) for ids in lanes],
             "throughput_solves_s": thr,
             "serial_plain_p50_s": pct(lat_serial, 0.5),
             "serial_plain_p99_s": pct(lat_serial, 0.99),
