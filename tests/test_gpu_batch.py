@@ -116,6 +116,59 @@ def test_continuous_batching_equals_alone(ctx):
     b.close()
 
 
+def test_two_batches_in_two_threads_equal_alone(ctx):
+    """Two batches (own context and stream each) stepped concurrently from
+    two host threads, as the serving lanes run, give every problem the
+    result it gives alone."""
+    import threading
+
+    from wannier90_amd import capi
+
+    print("[batch] two batches in two threads", flush=True)
+    groups = [["ref_basic1", "hex_nw5", "sc_nw1"], ["bcc_nw6", "fcc_nw8", "ref_example05"]]
+    cases = [[Case(n) for n in g] for g in groups]
+    ctls = [[c.gpu_control() for c in g] for g in cases]
+    alone = [[solve_alone(ctx, c, t) for c, t in zip(g, tg)] for g, tg in zip(cases, ctls)]
+    out = [dict(), dict()]
+    errors = []
+
+    def lane(ln):
+        try:
+            lctx = capi.Context(0)
+            b = lctx.batch(pool_for(cases[ln], ctls[ln]))
+            ids = {}
+            for k, c in enumerate(cases[ln]):
+                ids[b.add(c.kmesh, np.array(c.m0, order="F"), np.array(c.u0, order="F"), ctls[ln][k])] = k
+            guard = 0
+            while len(out[ln]) < len(cases[ln]):
+                guard += 1
+                assert guard < 10000
+                for i in b.step(1):
+                    r = b.result(i)
+                    _, u, cen, spr = b.fetch(i)
+                    out[ln][ids[i]] = (r, u, cen, spr)
+                    b.remove(i)
+            b.close()
+            lctx.close()
+        except Exception as ex:
+            errors.append(ex)
+
+    threads = [threading.Thread(target=lane, args=(ln,)) for ln in range(2)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors
+    for ln in range(2):
+        for k, c in enumerate(cases[ln]):
+            r, u, cen, spr = out[ln][k]
+            a = alone[ln][k]
+            assert r.iterations == a["res"].iterations, c.name
+            assert r.omega_total == a["res"].omega_total, c.name
+            assert np.array_equal(u, a["u"]) and np.array_equal(cen, a["centres"]), c.name
+            assert np.array_equal(spr, a["spreads"]), c.name
+
+
 def test_admission_by_bytes(ctx):
     """A pool that holds one problem admits a second only after the first
     is removed; a refused add leaves the batch unchanged."""
