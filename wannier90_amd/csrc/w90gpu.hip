@@ -445,7 +445,10 @@ void member_bind(Member* m, char* a, const Layout& lay) {
   p.psz = 8 * m->nw + 8;
   // two reduction levels only where one workgroup summing all per-k
   // vectors would be slow (over 128 Ki values)
-  p.ngroup = ((size_t)m->nk * p.psz <= (1u << 17)) ? 0 : std::min(m->nk, W90_MAX_GROUPS);
+  // meshes above 64 k-points reduce in two levels (32 k-ranges, then their
+  // sums): one workgroup alone reads the nk vectors at the bandwidth of one
+  // CU
+  p.ngroup = (m->nk <= 64) ? 0 : std::min(m->nk, W90_MAX_GROUPS);
   p.M[0] = (cplx*)(a + lay.off[i++]);
   p.M[1] = (cplx*)(a + lay.off[i++]);
   m->M0 = (cplx*)(a + lay.off[i++]);

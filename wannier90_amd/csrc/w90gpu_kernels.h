@@ -587,8 +587,8 @@ __device__ void accept_step(State* st, const Spread& sp, int dst, double* logrow
 }
 
 // First level of the omega reduction: workgroup (problem, g) sums the per-k
-// vectors of a contiguous k-range, in k order (rows read coalesced, up to
-// 32 rows loaded at once and then added in order).
+// vectors of a contiguous k-range, in k order (rows read coalesced, 16 rows
+// loaded at once and then added in order).
 __global__ void __launch_bounds__(W90_NT) k_omega_group(const Prob* probs, const int* plist, int np, int mode) {
   const int pi = blockIdx.x / W90_MAX_GROUPS;
   const int g = blockIdx.x % W90_MAX_GROUPS;
@@ -599,15 +599,16 @@ __global__ void __launch_bounds__(W90_NT) k_omega_group(const Prob* probs, const
   const int k0 = (int)(((long)g * p.nk) / G), k1 = (int)(((long)(g + 1) * p.nk) / G);
   for (int v = threadIdx.x; v < psz; v += blockDim.x) {
     double s = 0.0;
-    int k = k0;
-    for (; k + 32 <= k1; k += 32) {
-      double x[32];
+    // chunks of 16 rows: the loads of a chunk are issued together (rows
+    // past k1 read as zero), then added in row order
+    for (int k = k0; k < k1; k += 16) {
+      double x[16];
 #pragma unroll
-      for (int u = 0; u < 32; ++u) x[u] = p.kpart[(size_t)(k + u) * psz + v];
+      for (int u = 0; u < 16; ++u) x[u] = (k + u < k1) ? p.kpart[(size_t)(k + u) * psz + v] : 0.0;
 #pragma unroll
-      for (int u = 0; u < 32; ++u) s += x[u];
+      for (int u = 0; u < 16; ++u)
+        if (k + u < k1) s += x[u];
     }
-    for (; k < k1; ++k) s += p.kpart[(size_t)k * psz + v];
     p.gpart[(size_t)g * psz + v] = s;
   }
 }
