@@ -91,7 +91,8 @@ typedef struct {
   int unitarity_ok;       /* 1 if the final U passed the 1e-5 check      */
   int status;             /* status of the solve                         */
   int bulk_iterations;    /* iterations of the FP32 bulk phase (mixed)   */
-  int reserved;
+  int held_second_steps; /* parabolic steps launched by the host after a block
+                            without second-step kernels (engine diagnostics) */
 } w90gpu_result;
 
 /* ---------------------------------------------------------------- setup */

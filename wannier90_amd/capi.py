@@ -56,7 +56,7 @@ class Result(C.Structure):
         ("omega_od", C.c_double), ("omega_tilde", C.c_double), ("gcnorm1", C.c_double),
         ("delta_omega", C.c_double), ("solve_seconds", C.c_double), ("iterations", C.c_int),
         ("converged", C.c_int), ("unitarity_ok", C.c_int), ("status", C.c_int),
-        ("bulk_iterations", C.c_int), ("reserved", C.c_int),
+        ("bulk_iterations", C.c_int), ("held_second_steps", C.c_int),
     ]
 
     def as_dict(self):

@@ -155,11 +155,17 @@ struct Engine {
   Lists run;
   std::vector<int> run_slots;
   int epoch_blocks = 0;
+  // graphs of one block of the epoch: with the second-step kernels (full)
+  // and without them (lean, used while no problem takes parabolic steps)
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec_lean = nullptr;
+  hipGraph_t graph_lean = nullptr;
+  int clean_blocks = 0;           // processed blocks in a row without a parabolic step
   std::vector<std::pair<hipGraph_t, hipGraphExec_t>> retired;
   bool graph_failed = false;
   bool use_graph = true;
+  bool use_lean = true;           // lean blocks allowed (W90GPU_NO_LEAN unset)
   hipEvent_t blk_ev[2] = {nullptr, nullptr};
   std::vector<int> blk_slots[2];
   hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -354,8 +360,11 @@ int grow_tables(Engine* E, int nprob, int nitems) {
 
 void drop_graph(Engine* E) {
   if (E->gexec || E->graph) E->retired.emplace_back(E->graph, E->gexec);
+  if (E->gexec_lean || E->graph_lean) E->retired.emplace_back(E->graph_lean, E->gexec_lean);
   E->graph = nullptr;
   E->gexec = nullptr;
+  E->graph_lean = nullptr;
+  E->gexec_lean = nullptr;
   E->epoch_blocks = 0;
 }
 
@@ -406,6 +415,7 @@ int engine_create(w90gpu_ctx* ctx, size_t pool_bytes, Engine** out) {
     return fail(W90GPU_ERR_FATAL);
   }
   E->use_graph = std::getenv("W90GPU_NO_GRAPH") == nullptr;
+  E->use_lean = std::getenv("W90GPU_NO_LEAN") == nullptr;
   if (grow_tables(E, 4, 64)) {
     engine_destroy(E);
     return W90GPU_ERR_FATAL;
@@ -647,16 +657,21 @@ int upload_lists(Engine* E, const std::vector<Item>& items, const std::vector<in
   return W90GPU_OK;
 }
 
-// Aux lists (start, rebuild, retire) for a set of members, uploaded in
-// stream order.  Every member in FP64 precision unless fp32 is set.
-int aux_lists(Engine* E, const std::vector<Member*>& ms, Lists& L) {
+// Aux lists (start, rebuild, retire, host-launched second steps) for a set
+// of members, uploaded in stream order.  Every member in FP64 storage, or
+// with by_phase in the storage of its current phase (FP32 in the mixed
+// bulk phase), as in the run lists.
+int aux_lists(Engine* E, const std::vector<Member*>& ms, Lists& L, bool by_phase = false) {
   int nitems = 0;
   for (Member* m : ms) nitems += m->nk;
   int rc = grow_tables(E, (int)E->slots.size(), nitems);
   if (rc) return rc;
   std::vector<Item> items;
   std::vector<int> plist;
-  build_lists(ms, [](const Member*) { return 0; }, items, plist, L);
+  if (by_phase)
+    build_lists(ms, [](const Member* m) { return m->phase == 0 ? 1 : 0; }, items, plist, L);
+  else
+    build_lists(ms, [](const Member*) { return 0; }, items, plist, L);
   // pageable sources are staged by the copy call itself; keep them alive
   // until the stream reaches the copy
   rc = upload_lists(E, items, plist, E->d_aux_items, E->d_aux_plist, L);
@@ -738,21 +753,22 @@ int launch_rotate(Engine* E, const Lists& L, int mode) {
 
 // The reduction of the per-k partial vectors over k, with the control
 // logic that follows each wann_omega.
-int launch_reduce(Engine* E, const Lists& L, int mode) {
+int launch_reduce(Engine* E, const Lists& L, int mode, int lean = 0) {
   if (L.any_group) {
     hipLaunchKernelGGL(k_omega_group, dim3(L.np * W90_MAX_GROUPS), dim3(W90_NT), 0, E->stream, E->d_probs,
                        L.plist, L.np, mode);
     HIPCHK(E->ctx, hipGetLastError());
   }
-  hipLaunchKernelGGL(k_omega_final, dim3(L.np), dim3(W90_FINAL_NT), 0, E->stream, E->d_probs, L.plist, L.np, mode);
+  hipLaunchKernelGGL(k_omega_final, dim3(L.np), dim3(W90_FINAL_NT), 0, E->stream, E->d_probs, L.plist, L.np, mode,
+                     lean);
   HIPCHK(E->ctx, hipGetLastError());
   return W90GPU_OK;
 }
 
-int launch_omega(Engine* E, const Lists& L, int mode) {
+int launch_omega(Engine* E, const Lists& L, int mode, int lean = 0) {
   int rc = launch_rotate(E, L, mode);
   if (rc) return rc;
-  return launch_reduce(E, L, mode);
+  return launch_reduce(E, L, mode, lean);
 }
 
 int launch_step(Engine* E, const Lists& L, int mode) {
@@ -820,13 +836,16 @@ int all_items(const Lists& L) {
 }
 
 // One iteration of wann_main over the lists (wannierise.F90:555-817).
-int enqueue_iteration(Engine* E, const Lists& L) {
+// lean: without the second-step kernels; a problem whose line search asks
+// for the parabolic step then holds until the host launches it
+// (enqueue_seconds).
+int enqueue_iteration(Engine* E, const Lists& L, bool lean = false) {
   int rc;
   if ((rc = launch_domega(E, L, MODE_TRIAL))) return rc;
   // the search direction is formed inside the trial-step kernels
   if ((rc = launch_step(E, L, MODE_TRIAL))) return rc;
-  if ((rc = launch_omega(E, L, MODE_TRIAL))) return rc;
-  if (L.any_second) {
+  if ((rc = launch_omega(E, L, MODE_TRIAL, lean ? 1 : 0))) return rc;
+  if (L.any_second && !lean) {
     if ((rc = launch_step(E, L, MODE_SECOND))) return rc;
     if ((rc = launch_omega(E, L, MODE_SECOND))) return rc;
   }
@@ -907,6 +926,7 @@ int retire(Engine* E, const std::vector<Member*>& ms) {
     r.iterations = st.iter;
     r.converged = st.converged;
     r.bulk_iterations = (m->ctl.precision == W90GPU_PREC_MIXED) ? st.bulk_iter : 0;
+    r.held_second_steps = st.nheld;
     const double mx = E->h_final[i].unit_max;
     r.unitarity_ok = (mx <= 1e-5) ? 1 : 0;
     r.status = r.unitarity_ok ? W90GPU_OK : W90GPU_ERR_FATAL;
@@ -947,7 +967,7 @@ int build_run_lists(Engine* E) {
 // Capture kGraphBlock iterations over the run lists.  The kernels read the
 // loop state from device memory, so one graph serves every block of the
 // epoch.  On a capture failure the engine keeps launching directly.
-int capture_block(Engine* E) {
+int capture_block(Engine* E, bool lean) {
   hipGraph_t g = nullptr;
   if (hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
     (void)hipGetLastError();
@@ -955,7 +975,7 @@ int capture_block(Engine* E) {
     return W90GPU_OK;
   }
   int rc = W90GPU_OK;
-  for (int i = 0; i < kGraphBlock && rc == W90GPU_OK; ++i) rc = enqueue_iteration(E, E->run);
+  for (int i = 0; i < kGraphBlock && rc == W90GPU_OK; ++i) rc = enqueue_iteration(E, E->run, lean);
   const hipError_t e = hipStreamEndCapture(E->stream, &g);
   if (rc != W90GPU_OK || e != hipSuccess || !g) {
     (void)hipGetLastError();
@@ -963,14 +983,20 @@ int capture_block(Engine* E) {
     E->graph_failed = true;
     return W90GPU_OK;
   }
-  if (hipGraphInstantiate(&E->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+  hipGraphExec_t ge = nullptr;
+  if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
     (void)hipGetLastError();
     (void)hipGraphDestroy(g);
-    E->gexec = nullptr;
     E->graph_failed = true;
     return W90GPU_OK;
   }
-  E->graph = g;
+  if (lean) {
+    E->graph_lean = g;
+    E->gexec_lean = ge;
+  } else {
+    E->graph = g;
+    E->gexec = ge;
+  }
   return W90GPU_OK;
 }
 
@@ -979,14 +1005,19 @@ int enqueue_block(Engine* E, int slot) {
   // the first block of an epoch is launched directly; the graph is
   // captured when the membership has stayed the same for a block
   const bool graph = E->use_graph && !E->graph_failed && E->epoch_blocks >= 1;
-  if (graph && !E->gexec) {
-    if ((rc = capture_block(E))) return rc;
+  // without second-step kernels after a processed block in which no problem
+  // took a parabolic step (a problem that asks for one then waits for the
+  // host, which launches it and returns to full blocks)
+  const bool lean = E->use_lean && E->run.any_second && E->clean_blocks >= 1;
+  hipGraphExec_t& ge = lean ? E->gexec_lean : E->gexec;
+  if (graph && !ge) {
+    if ((rc = capture_block(E, lean))) return rc;
   }
-  if (graph && E->gexec) {
-    HIPCHK(E->ctx, hipGraphLaunch(E->gexec, E->stream));
+  if (graph && ge) {
+    HIPCHK(E->ctx, hipGraphLaunch(ge, E->stream));
   } else {
     for (int i = 0; i < kGraphBlock; ++i)
-      if ((rc = enqueue_iteration(E, E->run))) return rc;
+      if ((rc = enqueue_iteration(E, E->run, lean))) return rc;
   }
   E->epoch_blocks++;
   const int np = E->run.np;
@@ -1002,22 +1033,43 @@ int enqueue_block(Engine* E, int slot) {
 
 // Read the status of a completed block: finished members are retired,
 // members whose bulk phase ended get their FP64 rebuild queued.
+// The parabolic steps found in blocks without second-step kernels: release
+// the held problems and launch the step, its rotation and wann_omega for
+// them (stream order puts this after the blocks already in flight, in which
+// these problems only wait).
+int enqueue_seconds(Engine* E, const std::vector<Member*>& ms) {
+  Lists L;
+  int rc = aux_lists(E, ms, L, true);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_release, dim3((L.np + 255) / 256), dim3(256), 0, E->stream, E->d_probs, L.plist, L.np);
+  HIPCHK(E->ctx, hipGetLastError());
+  if ((rc = launch_step(E, L, MODE_SECOND))) return rc;
+  return launch_omega(E, L, MODE_SECOND);
+}
+
 int process_block(Engine* E, int slot, std::vector<Member*>& remix) {
   HIPCHK(E->ctx, hipEventSynchronize(E->blk_ev[slot]));
   const Status* st = E->h_status + (size_t)slot * E->cap_plist;
 
-  std::vector<Member*> done;
+  std::vector<Member*> done, seconds;
+  bool any_second = false;
   const std::vector<int>& sl = E->blk_slots[slot];
   for (size_t i = 0; i < sl.size(); ++i) {
     Member* m = E->slots[sl[i]];
     if (!m || m->stage != 2) continue;
+    if (st[i].nsecond > 0) any_second = true;
     if (st[i].done) {
       done.push_back(m);
+    } else if (st[i].second) {
+      seconds.push_back(m);
     } else if (st[i].hold && m->phase == 0 &&
                std::find(remix.begin(), remix.end(), m) == remix.end()) {
       remix.push_back(m);
     }
   }
+  E->clean_blocks = any_second ? 0 : E->clean_blocks + 1;
+  int rc;
+  if (!seconds.empty() && (rc = enqueue_seconds(E, seconds))) return rc;
   if (!done.empty()) return retire(E, done);
   return W90GPU_OK;
 }

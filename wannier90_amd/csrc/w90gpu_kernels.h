@@ -164,9 +164,13 @@ struct State {
   int conv_count;
   int first_pass;
   int phase;       // 0: mixed bulk (FP32 M), 1: FP64
-  int hold;        // bulk phase ended, waiting for the FP64 rebuild of M
+  int hold;        // waiting for the host: the FP64 rebuild of M after the
+                   // mixed bulk phase, or (with second set) a parabolic step
+                   // found in a block launched without second-step kernels
   int bulk_iter;   // iterations done in the bulk phase
   int stall;       // consecutive rejected bulk steps
+  int nsecond;     // parabolic steps taken since the host last read the status
+  int nheld;       // parabolic steps released to the host (k_release)
   double gcnorm0_s[2];  // previous squared gradient norm, double-buffered like ncg_s
   double gcnorm1, gcfac, doda0, alphamin, falphamin;
   double beta, dir_sign, scale;
@@ -220,6 +224,9 @@ struct Item {
 // Per-problem status the host reads after each block of iterations.
 struct Status {
   int done, hold, iter, converged;
+  int second;  // a parabolic step waits for the host (hold set with it)
+  int nsecond; // parabolic steps taken during the block
+  int pad[2];
 };
 
 // Final state of a problem gathered for the host.
@@ -317,16 +324,36 @@ __global__ void __launch_bounds__(W90_NT) k_to_f32(const Prob* probs, const Item
 __global__ void __launch_bounds__(W90_NT) k_delta_u(const Prob* probs, const Item* items, int nitems);
 
 // Status of each listed problem (one thread per problem).
+// (one thread per problem; clears the count of parabolic steps).
 __global__ void k_status(const Prob* probs, const int* plist, int np, Status* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  const State* st = probs[plist[i]].st;
+  State* st = probs[plist[i]].st;
   Status s;
   s.done = st->done;
   s.hold = st->hold;
   s.iter = st->iter;
   s.converged = st->converged;
+  s.second = (st->hold && st->second) ? 1 : 0;
+  s.nsecond = st->nsecond;
+  s.pad[0] = s.pad[1] = 0;
   out[i] = s;
+  st->nsecond = 0;
+}
+
+// Release the problems listed whose hold is a pending parabolic step (found
+// in a block without second-step kernels), so that the step can be
+// launched.  Any other hold (the end of the mixed bulk phase) stays, and a
+// problem whose step was already taken is left as it is, so a request
+// repeated from a stale status does nothing.
+__global__ void k_release(const Prob* probs, const int* plist, int np) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  State* st = probs[plist[i]].st;
+  if (st->hold && st->second && !st->done) {
+    st->hold = 0;
+    st->nheld += 1;
+  }
 }
 
 // Final state and largest unitarity deviation of each listed problem (one
@@ -623,8 +650,11 @@ __global__ void __launch_bounds__(W90_NT) k_omega_group(const Prob* probs, const
 // (:452-470), internal_optimal_step (:1528-1596) after the trial step, the
 // acceptance of the trial or parabolic step (:660-701), and the FP64
 // rebuild at the end of the mixed bulk phase.  sod, si, omd: the summed
-// off-diagonal, invariant and Omega_D numerators.
-__device__ void omega_control(State* st, const Prob& p, double sod, double si, double omd, int mode, int dst) {
+// off-diagonal, invariant and Omega_D numerators.  lean: the launch
+// sequence has no second-step kernels after this one, so a parabolic step
+// holds the problem for the host.
+__device__ void omega_control(State* st, const Prob& p, double sod, double si, double omd, int mode, int dst,
+                              int lean) {
   const double rnk = (double)p.nk;
   Spread sp;
   sp.om_od = sod / rnk;
@@ -710,6 +740,8 @@ __device__ void omega_control(State* st, const Prob& p, double sod, double si, d
     st->lquad = lquad;
     if (lquad) {
       st->second = 1;
+      st->nsecond += 1;
+      if (lean) st->hold = 1;
       st->scale = alphamin / (4.0 * st->wbtot);
     } else {
       accept_step(st, st->trial, dst, logrow);
@@ -722,7 +754,7 @@ __device__ void omega_control(State* st, const Prob& p, double sod, double si, d
 }
 
 __global__ void __launch_bounds__(W90_FINAL_NT) k_omega_final(const Prob* probs, const int* plist, int np,
-                                                              int mode) {
+                                                              int mode, int lean) {
   if (blockIdx.x >= (unsigned)np) return;
   const Prob& p = probs[plist[blockIdx.x]];
   // the state is worked on in LDS: the control logic below is a chain of
@@ -814,7 +846,7 @@ __global__ void __launch_bounds__(W90_FINAL_NT) k_omega_final(const Prob* probs,
   }
   __syncthreads();
   if (mode == MODE_BENCH) return;  // timing of the reduction alone: the state stays as it is
-  if (threadIdx.x == 0) omega_control(st, p, sc[0], sc[1], omd, mode, dst);
+  if (threadIdx.x == 0) omega_control(st, p, sc[0], sc[1], omd, mode, dst, lean);
   __syncthreads();
   for (int i = threadIdx.x; i < NSD; i += blockDim.x)
     reinterpret_cast<double*>(p.st)[i] = reinterpret_cast<const double*>(&sst)[i];

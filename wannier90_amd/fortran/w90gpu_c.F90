@@ -57,7 +57,7 @@ module w90gpu_c
     integer(c_int) :: unitarity_ok
     integer(c_int) :: status
     integer(c_int) :: bulk_iterations
-    integer(c_int) :: reserved
+    integer(c_int) :: held_second_steps
   end type w90gpu_result
 
   interface
