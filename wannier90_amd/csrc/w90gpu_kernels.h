@@ -1382,14 +1382,30 @@ __global__ void __launch_bounds__(W90_NT) k_step_k(const Prob* probs, const Item
     }
   }
   __syncthreads();
-  double cs = 0.0;
-  for (int j = tid; j < nw; j += W90_NT) {
-    double s = 0.0;
-    for (int i = 0; i < nw; ++i) s += sqrt(c_abs2(X[i + j * ld]));
-    cs = fmax(cs, s);
+  // 1-norm of X: the moduli in parallel (into T, free until the Taylor
+  // products), then each column summed in row order
+  double* absx = reinterpret_cast<double*>(T);
+  for (int e = tid; e < nw2; e += W90_NT) {
+    const int q = (e % nw) + (e / nw) * ld;
+    absx[q] = sqrt(c_abs2(X[q]));
   }
+  __syncthreads();
+  // column j < nw <= 64 is summed by lane j of wavefront 0, whose maximum
+  // is formed by lane exchange and passed on through LDS
+  double cs = 0.0;
+  if (tid < 64) {
+    if (tid < nw) {
+#pragma unroll 4
+      for (int i = 0; i < nw; ++i) cs += absx[i + tid * ld];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) cs = fmax(cs, __shfl_xor(cs, o));
+    if (tid == 0) red[0] = cs;
+  }
+  __syncthreads();
+  cs = red[0];
   int sq, m;
-  taylor_plan(block_max(cs, red), &sq, &m);
+  taylor_plan(cs, &sq, &m);
   const double f = ldexp(1.0, -sq);
   const double invm = 1.0 / (double)m;
   for (int e = tid; e < nw2; e += W90_NT) {
