@@ -699,14 +699,29 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
     Xi[c * W90_STEP_LD + r] = x.y;
   }
   __syncthreads();
-  double cs = 0.0;
-  for (int c = tid; c < nw; c += 256) {
-    double s = 0.0;
-    for (int r = 0; r < nw; ++r) s += sqrt(c_abs2(c_make(Xr[c * W90_STEP_LD + r], Xi[c * W90_STEP_LD + r])));
-    cs = fmax(cs, s);
+  // 1-norm of X: the moduli in parallel (into the P slot, free until the
+  // Taylor polynomial), each column summed in row order by lane c of
+  // wavefront 0, the maximum by lane exchange
+  for (int e = tid; e < W90_STEP_N * W90_STEP_N; e += 256) {
+    const int r = e % W90_STEP_N, c = e / W90_STEP_N;
+    const int q = c * W90_STEP_LD + r;
+    Pr[q] = sqrt(c_abs2(c_make(Xr[q], Xi[q])));
   }
+  __syncthreads();
+  double cs = 0.0;
+  if (tid < 64) {
+    if (tid < nw) {
+#pragma unroll 4
+      for (int r = 0; r < nw; ++r) cs += Pr[tid * W90_STEP_LD + r];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) cs = fmax(cs, __shfl_xor(cs, o));
+    if (tid == 0) red[0] = cs;
+  }
+  __syncthreads();
+  cs = red[0];
   int sq, m;
-  taylor_plan(block_max(cs, red), &sq, &m);
+  taylor_plan(cs, &sq, &m);
   const double f = ldexp(1.0, -sq);
   const double invm = 1.0 / (double)m;
   for (int e = tid; e < W90_STEP_N * W90_STEP_N; e += 256) {
