@@ -38,12 +38,14 @@ struct w90gpu_ctx {
 // kernel class and storage precision, and one problem list.  An iteration
 // is the fixed launch sequence of wann_main over these lists; a block of
 // kGraphBlock iterations is captured once per epoch into a HIP graph and
-// replayed.  After each block a status kernel copies done/hold flags to
-// pinned host memory; two blocks are kept in flight, so the host reads the
-// flags of block j while block j+1 runs.  Finished problems are retired
-// (final state gathered, unitarity checked) and held problems get their
-// FP64 rebuild of M; either starts a new epoch.  A session is an engine
-// with one problem.
+// replayed, with or without the second-step kernels (lean blocks while no
+// problem takes parabolic steps).  After each block a status kernel copies
+// done/hold flags to pinned host memory; two blocks are kept in flight, so
+// the host reads the flags of block j while block j+1 runs.  Finished
+// problems are retired (final state gathered, unitarity checked), held
+// problems get their FP64 rebuild of M (a new epoch) or, if a lean block
+// held them for a parabolic step, that step.  A session is an engine with
+// one problem.
 
 struct Member {
   int id = -1;
@@ -64,9 +66,10 @@ struct Member {
 };
 
 // kernel classes by num_wann: the wavefront-per-neighbour MFMA rotation up
-// to 16; the MFMA GEMM rotation (32 x 32 tiles) with the per-k LDS
-// retraction and gradient up to 32; the MFMA GEMM rotation (64 x 64 tiles)
-// with the MFMA retraction and LDS gradient up to 64, and with the
+// to 16; the one-workgroup-per-block MFMA rotation (32 x 32 tile, T in LDS)
+// with the per-k LDS retraction and gradient up to 32; the same rotation
+// with a 64 x 64 tile, the MFMA retraction and the LDS gradient up to 64;
+// the two-launch MFMA rotation (64 x 64 tiles over panels) with the
 // global-memory retraction and gradient up to W90GPU_MAX_WANN
 constexpr int kClasses = 4;
 constexpr int kLdsMaxWann = 32;

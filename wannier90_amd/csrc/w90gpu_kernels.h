@@ -184,7 +184,7 @@ struct Prob {
   int nw, nk, nn, npan;
   int psz, nw2, ngroup, pad;  // psz: per-k partial vector length; ngroup: first-level k-ranges
                               // (0: the final reduction sums the per-k vectors itself)
-  // npan: tiles of the MFMA rotation per block M(k,b) (num_wann > 32)
+  // npan: tiles of the MFMA rotation per block M(k,b) (num_wann > 16)
   const int* nnl;     // [k][b] 0-based neighbour
   const double* bk;   // [k][b][3]
   const double* wb;   // [b]
