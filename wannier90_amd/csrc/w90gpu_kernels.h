@@ -93,49 +93,40 @@ __device__ __forceinline__ float2 c_narrow<float2>(cplx a) {
   return make_float2((float)a.x, (float)a.y);
 }
 
-// Block-wide sum of one double per thread; deterministic (fixed tree), any
-// block size: the part above the largest power of two is folded in first.
-__device__ double block_sum(double v, double* red) {
-  const int t = threadIdx.x;
-  const int n = blockDim.x;
-  int p2 = 1;
-  while (p2 * 2 <= n) p2 *= 2;
-  red[t] = v;
-  __syncthreads();
-  if (t < n - p2) red[t] += red[t + p2];
-  __syncthreads();
-  for (int s = p2 / 2; s > 0; s >>= 1) {
-    if (t < s) red[t] += red[t + s];
-    __syncthreads();
-  }
-  const double r = red[0];
-  __syncthreads();
-  return r;
-}
-
-__device__ double block_max(double v, double* red) {
-  const int t = threadIdx.x;
-  const int n = blockDim.x;
-  int p2 = 1;
-  while (p2 * 2 <= n) p2 *= 2;
-  red[t] = v;
-  __syncthreads();
-  if (t < n - p2) red[t] = fmax(red[t], red[t + p2]);
-  __syncthreads();
-  for (int s = p2 / 2; s > 0; s >>= 1) {
-    if (t < s) red[t] = fmax(red[t], red[t + s]);
-    __syncthreads();
-  }
-  const double r = red[0];
-  __syncthreads();
-  return r;
-}
-
 // Sum over the 64 lanes of a wavefront (butterfly; lane 0's value is used).
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+// Block-wide sum of one double per thread (blockDim a multiple of 64):
+// each wavefront's butterfly sum (lane 0), then the wavefront sums in
+// wavefront order by every thread; deterministic, two barriers.
+__device__ double block_sum(double v, double* red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nwv = blockDim.x >> 6;
+  v = wave_sum(v);
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int i = 0; i < nwv; ++i) r += red[i];
+  __syncthreads();
+  return r;
+}
+
+// Block-wide maximum (blockDim a multiple of 64), as block_sum.
+__device__ double block_max(double v, double* red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nwv = blockDim.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double r = red[0];
+  for (int i = 1; i < nwv; ++i) r = fmax(r, red[i]);
+  __syncthreads();
+  return r;
 }
 
 // --------------------------------------------------------------- state
