@@ -978,6 +978,18 @@ __global__ void __launch_bounds__(domega_nt<NWM>()) k_domega_k(const Prob* probs
   T2* Ms = reinterpret_cast<T2*>(red + NT);
   const double* rave = p.rave[cur];
   const T2* Mk = mbuf<T2>(p, cur) + (size_t)k * nn * nw2;
+  // the first chunk of blocks is requested before the per-(b, n) inputs
+  // below, so both rounds of loads are in flight together
+  T2 nxt[CB][SL];
+#pragma unroll
+  for (int j = 0; j < CB; ++j)
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int e = tid + s * NT;
+      T2 v = {};
+      if (j < nn && e < nw2) v = Mk[(size_t)j * nw2 + e];
+      nxt[j][s] = v;
+    }
   for (int t = tid; t < nn * nw; t += NT) {
     const int b = t / nw, n = t % nw;
     const size_t kb = (size_t)k * nn + b;
@@ -995,16 +1007,6 @@ __global__ void __launch_bounds__(domega_nt<NWM>()) k_domega_k(const Prob* probs
   __shared__ double wL[W90GPU_NUM_NNMAX];
   if (tid < nn) wL[tid] = p.wb[tid];
   const int bs = NWM * (NWM + 1);  // LDS stride of the chunk's blocks
-  T2 nxt[CB][SL];
-#pragma unroll
-  for (int j = 0; j < CB; ++j)
-#pragma unroll
-    for (int s = 0; s < SL; ++s) {
-      const int e = tid + s * NT;
-      T2 v = {};
-      if (j < nn && e < nw2) v = Mk[(size_t)j * nw2 + e];
-      nxt[j][s] = v;
-    }
   for (int b0 = 0; b0 < nn; b0 += CB) {
     __syncthreads();
 #pragma unroll
@@ -1130,8 +1132,7 @@ __device__ void search_control(State* st, double gg, double gd) {
 // must be W90_NT (the summation order depends on it).
 __device__ void search_in_step(const Prob& p, int k, State* lst, double* red) {
   constexpr int NSD = sizeof(State) / sizeof(double);
-  for (int i = threadIdx.x; i < NSD; i += blockDim.x)
-    reinterpret_cast<double*>(lst)[i] = reinterpret_cast<const double*>(p.st)[i];
+  // the partial sums are requested first, the state copy overlaps them
   double gg = 0.0, gd = 0.0;
   const int per = (p.nk + blockDim.x - 1) / blockDim.x;
   const int kb = threadIdx.x * per, ke = min(p.nk, kb + per);
@@ -1139,6 +1140,8 @@ __device__ void search_in_step(const Prob& p, int k, State* lst, double* red) {
     gg += p.part_gg[q];
     gd += p.part_gd[q];
   }
+  for (int i = threadIdx.x; i < NSD; i += blockDim.x)
+    reinterpret_cast<double*>(lst)[i] = reinterpret_cast<const double*>(p.st)[i];
   gg = block_sum(gg, red);  // (its barriers also order the copy above)
   gd = block_sum(gd, red);
   if (threadIdx.x == 0) {
