@@ -37,7 +37,7 @@ struct w90gpu_ctx {
 // epoch, for which the host builds one item list (problem, k) sorted by
 // kernel class and storage precision, and one problem list.  An iteration
 // is the fixed launch sequence of wann_main over these lists; a block of
-// kGraphBlock iterations is captured once per epoch into a HIP graph and
+// graph_block iterations is captured once per epoch into a HIP graph and
 // replayed, with or without the second-step kernels (lean blocks while no
 // problem takes parabolic steps).  After each block a status kernel copies
 // done/hold flags to pinned host memory; two blocks are kept in flight, so
@@ -73,8 +73,9 @@ struct Member {
 // global-memory retraction and gradient up to W90GPU_MAX_WANN
 constexpr int kClasses = 4;
 constexpr int kLdsMaxWann = 32;
-// iterations per captured block; the solve is polled once per block
-constexpr int kGraphBlock = 8;
+// iterations per captured block (W90GPU_GRAPH_BLOCK overrides it, 1-64);
+// the solve is polled once per block
+constexpr int kGraphBlock = 16;
 
 int wann_class(int nw) { return nw <= 16 ? 0 : nw <= kLdsMaxWann ? 1 : nw <= W90_STEP_N ? 2 : 3; }
 // per-(k,b) partial sums of the rotation: one per MFMA tile
@@ -149,8 +150,8 @@ struct Engine {
   Item* d_aux_items = nullptr;    // start / rebuild / retire lists
   int* d_aux_plist = nullptr;
   int cap_items = 0, cap_plist = 0;
-  Status* d_status = nullptr;     // [2][cap_plist]
-  Status* h_status = nullptr;     // pinned
+  Status* h_status = nullptr;     // [2][cap_plist] pinned
+  Status* hd_status = nullptr;    // device address of h_status: k_status writes it directly
   Final* d_final = nullptr;       // [cap_plist]
   Final* h_final = nullptr;       // pinned
   // run lists of the current epoch
@@ -169,6 +170,7 @@ struct Engine {
   bool graph_failed = false;
   bool use_graph = true;
   bool use_lean = true;           // lean blocks allowed (W90GPU_NO_LEAN unset)
+  int graph_block = kGraphBlock;  // iterations per block
   hipEvent_t blk_ev[2] = {nullptr, nullptr};
   std::vector<int> blk_slots[2];
   hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -329,20 +331,20 @@ int grow_tables(Engine* E, int nprob, int nitems) {
   if (nprob > E->cap_plist) {
     const int cap = std::max(nprob, 2 * E->cap_plist);
     HIPCHK(ctx, hipStreamSynchronize(E->stream));
-    for (void* q : {(void*)E->d_plist, (void*)E->d_aux_plist, (void*)E->d_status, (void*)E->d_final})
+    for (void* q : {(void*)E->d_plist, (void*)E->d_aux_plist, (void*)E->d_final})
       if (q) HIPCHK(ctx, hipFree(q));
     if (E->h_status) HIPCHK(ctx, hipHostFree(E->h_status));
     if (E->h_final) HIPCHK(ctx, hipHostFree(E->h_final));
     E->d_plist = E->d_aux_plist = nullptr;
-    E->d_status = nullptr;
+    E->hd_status = nullptr;
     E->d_final = nullptr;
     E->h_status = nullptr;
     E->h_final = nullptr;
     HIPCHK(ctx, hipMalloc(&E->d_plist, sizeof(int) * cap));
     HIPCHK(ctx, hipMalloc(&E->d_aux_plist, sizeof(int) * cap));
-    HIPCHK(ctx, hipMalloc(&E->d_status, sizeof(Status) * 2 * cap));
     HIPCHK(ctx, hipMalloc(&E->d_final, sizeof(Final) * cap));
     HIPCHK(ctx, hipHostMalloc((void**)&E->h_status, sizeof(Status) * 2 * cap, hipHostMallocDefault));
+    HIPCHK(ctx, hipHostGetDevicePointer((void**)&E->hd_status, E->h_status, 0));
     HIPCHK(ctx, hipHostMalloc((void**)&E->h_final, sizeof(Final) * cap, hipHostMallocDefault));
     E->cap_plist = cap;
     E->epoch++;  // run lists must be uploaded again
@@ -419,6 +421,7 @@ int engine_create(w90gpu_ctx* ctx, size_t pool_bytes, Engine** out) {
   }
   E->use_graph = std::getenv("W90GPU_NO_GRAPH") == nullptr;
   E->use_lean = std::getenv("W90GPU_NO_LEAN") == nullptr;
+  if (const char* gb = std::getenv("W90GPU_GRAPH_BLOCK")) E->graph_block = std::min(64, std::max(1, std::atoi(gb)));
   if (grow_tables(E, 4, 64)) {
     engine_destroy(E);
     return W90GPU_ERR_FATAL;
@@ -437,7 +440,7 @@ void engine_destroy(Engine* E) {
   for (hipEvent_t ev : {E->t0, E->t1, E->blk_ev[0], E->blk_ev[1]})
     if (ev) (void)hipEventDestroy(ev);
   for (void* q : {(void*)E->d_probs, (void*)E->d_items, (void*)E->d_plist, (void*)E->d_aux_items,
-                  (void*)E->d_aux_plist, (void*)E->d_status, (void*)E->d_final, (void*)E->pool.base})
+                  (void*)E->d_aux_plist, (void*)E->d_final, (void*)E->pool.base})
     if (q) (void)hipFree(q);
   if (E->h_status) (void)hipHostFree(E->h_status);
   if (E->h_final) (void)hipHostFree(E->h_final);
@@ -967,7 +970,7 @@ int build_run_lists(Engine* E) {
   return W90GPU_OK;
 }
 
-// Capture kGraphBlock iterations over the run lists.  The kernels read the
+// Capture graph_block iterations over the run lists.  The kernels read the
 // loop state from device memory, so one graph serves every block of the
 // epoch.  On a capture failure the engine keeps launching directly.
 int capture_block(Engine* E, bool lean) {
@@ -978,7 +981,7 @@ int capture_block(Engine* E, bool lean) {
     return W90GPU_OK;
   }
   int rc = W90GPU_OK;
-  for (int i = 0; i < kGraphBlock && rc == W90GPU_OK; ++i) rc = enqueue_iteration(E, E->run, lean);
+  for (int i = 0; i < E->graph_block && rc == W90GPU_OK; ++i) rc = enqueue_iteration(E, E->run, lean);
   const hipError_t e = hipStreamEndCapture(E->stream, &g);
   if (rc != W90GPU_OK || e != hipSuccess || !g) {
     (void)hipGetLastError();
@@ -1019,16 +1022,16 @@ int enqueue_block(Engine* E, int slot) {
   if (graph && ge) {
     HIPCHK(E->ctx, hipGraphLaunch(ge, E->stream));
   } else {
-    for (int i = 0; i < kGraphBlock; ++i)
+    for (int i = 0; i < E->graph_block; ++i)
       if ((rc = enqueue_iteration(E, E->run, lean))) return rc;
   }
   E->epoch_blocks++;
   const int np = E->run.np;
+  // the status rows go straight to pinned host memory; the event after the
+  // kernel makes them visible to the host
   hipLaunchKernelGGL(k_status, dim3((np + 255) / 256), dim3(256), 0, E->stream, E->d_probs, E->run.plist, np,
-                     E->d_status + (size_t)slot * E->cap_plist);
+                     E->hd_status + (size_t)slot * E->cap_plist);
   HIPCHK(E->ctx, hipGetLastError());
-  HIPCHK(E->ctx, hipMemcpyAsync(E->h_status + (size_t)slot * E->cap_plist, E->d_status + (size_t)slot * E->cap_plist,
-                                sizeof(Status) * np, hipMemcpyDeviceToHost, E->stream));
   HIPCHK(E->ctx, hipEventRecord(E->blk_ev[slot], E->stream));
   E->blk_slots[slot] = E->run_slots;
   return W90GPU_OK;

@@ -82,6 +82,18 @@ __device__ __forceinline__ T2 c_zero() {
 // storage -> FP64 and FP64 -> storage
 __device__ __forceinline__ cplx c_wide(cplx a) { return a; }
 __device__ __forceinline__ cplx c_wide(float2 a) { return make_double2((double)a.x, (double)a.y); }
+// Streaming (non-temporal) store of a complex element.  Used for per-k
+// blocks whose next readers are later kernels: the XCD's L2 does not keep
+// them across the kernel boundary, so holding them there only lengthens the
+// write-back at the end of the kernel.  Measured on the 16-band rotation
+// (17.2 -> 14.9 us per launch, fcc 512-k mesh); the 17-64 band rotation
+// keeps ordinary stores (the streaming form was slower there, 707 -> 726 us
+// on the triclinic 64-band 800-k mesh).
+template <class T2>
+__device__ __forceinline__ void st_stream(T2* dst, T2 v) {
+  __builtin_nontemporal_store(v.x, &dst->x);
+  __builtin_nontemporal_store(v.y, &dst->y);
+}
 template <class T2>
 __device__ __forceinline__ T2 c_narrow(cplx a);
 template <>
@@ -1052,7 +1064,7 @@ __global__ void __launch_bounds__(domega_nt<NWM>()) k_domega_k(const Prob* probs
     const int e = tid + s * NT;
     if (e < nw2) {
       const cplx g = c_scale(make_double2(acc[s].x / rnk, acc[s].y / rnk), 4.0);
-      p.G[(size_t)k * nw2 + e] = g;
+      st_stream(&p.G[(size_t)k * nw2 + e], g);
       gg += c_abs2(g);
       const cplx dold = p.D[(size_t)k * nw2 + e];
       gd += g.x * dold.x + g.y * dold.y;
@@ -1136,9 +1148,21 @@ __device__ void search_in_step(const Prob& p, int k, State* lst, double* red) {
   double gg = 0.0, gd = 0.0;
   const int per = (p.nk + blockDim.x - 1) / blockDim.x;
   const int kb = threadIdx.x * per, ke = min(p.nk, kb + per);
-  for (int q = kb; q < ke; ++q) {
-    gg += p.part_gg[q];
-    gd += p.part_gd[q];
+  // chunks of 4: the loads of a chunk are issued together, then added in
+  // k order
+  for (int q = kb; q < ke; q += 4) {
+    double xg[4], xd[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      xg[u] = (q + u < ke) ? p.part_gg[q + u] : 0.0;
+      xd[u] = (q + u < ke) ? p.part_gd[q + u] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (q + u < ke) {
+        gg += xg[u];
+        gd += xd[u];
+      }
   }
   for (int i = threadIdx.x; i < NSD; i += blockDim.x)
     reinterpret_cast<double*>(lst)[i] = reinterpret_cast<const double*>(p.st)[i];
@@ -1364,7 +1388,7 @@ __global__ void __launch_bounds__(W90_NT) k_step_k(const Prob* probs, const Item
       const int e = tid + i * W90_NT;
       if (e < nw2) {
         cplx d = c_scale(c_add(gpre[i], c_scale(dpre[i], beta)), sign);
-        Dk[e] = d;
+        st_stream(&Dk[e], d);
         X[(e % nw) + (e / nw) * ld] = c_scale(d, scale);
       }
     }
@@ -1436,14 +1460,14 @@ __global__ void __launch_bounds__(W90_NT) k_step_k(const Prob* probs, const Item
     const int e = tid + i * W90_NT;
     if (e < nw2) {
       const int q = (e % nw) + (e / nw) * ld;
-      Ek[e] = a[q];
+      st_stream(&Ek[e], a[q]);
       X[q] = upre[i];
     }
   }
   __syncthreads();
   lds_gemm_mfma<NWM>(b, X, a, nw);
   cplx* Un = p.U[1 - cur] + (size_t)k * nw2;
-  for (int e = tid; e < nw2; e += W90_NT) Un[e] = b[(e % nw) + (e / nw) * ld];
+  for (int e = tid; e < nw2; e += W90_NT) st_stream(&Un[e], b[(e % nw) + (e / nw) * ld]);
 }
 
 // ---------------------------------------------------------- projection

@@ -90,18 +90,6 @@ struct Real<float2> {
   typedef float type;
 };
 
-// Streaming (non-temporal) store of a rotated overlap element, for the
-// 16-band rotation: its blocks are read next by the gradient and reduction
-// kernels on other XCDs, so keeping them in this XCD's L2 gains nothing
-// (measured: 17.2 -> 14.9 us per launch on the fcc 16-band 512-k mesh).
-// The fused 17-64 band rotation keeps ordinary stores: its blocks are
-// re-read on the same XCD and the streaming form was slower (707 -> 726 us
-// on the triclinic 64-band 800-k mesh).
-template <class T2>
-__device__ __forceinline__ void st_stream(T2* dst, T2 v) {
-  __builtin_nontemporal_store(v.x, &dst->x);
-  __builtin_nontemporal_store(v.y, &dst->y);
-}
 
 // C = A B for the LDS retraction (k_step_k): n x n column-major complex
 // matrices in LDS, leading dimension n + 1, n <= NWM.  The product is
@@ -987,6 +975,7 @@ __global__ void __launch_bounds__(64 * W90GPU_NUM_NNMAX) k_rotate_w16(const Prob
         T2 v;
         if (rot) {
           v = Tl[r + c * 17];
+          // streamed: the next readers of the block run on other XCDs
           st_stream(&out[r + c * nw], v);
         } else {
           v = src[r + c * nw];
