@@ -1,7 +1,7 @@
 """Benchmark of the MI355X wannierisation path.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
-                    [--no-cpu-baseline] [--workload single|batch|serve]
+                    [--no-cpu-baseline] [--workload single|batch|serve|mmn]
 
 single (default): the benchmark configuration of the metric, a seeded fcc
 tight-binding problem (a = 5.43 Ang, num_wann = num_bands = 16, decay 2 Ang,
@@ -18,7 +18,8 @@ stream) and the CPU baseline (the reference wannier90.x on a bounded sample,
 timed on this host).
 
 batch and serve (wannier90_amd/workloads.py): the ragged batch of 256
-problems and the open-loop serving loop.
+problems and the open-loop serving loop.  mmn: the parse of the benchmark
+problem's .mmn file by the library's parallel reader (host only).
 """
 import argparse
 import json
@@ -55,7 +56,7 @@ def parse():
                          "calibrates FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--config", type=int, choices=[1, 2, 3], default=2,
                     help="single: benchmark configuration (2: the metric's; 3: num_wann 64 triclinic)")
-    ap.add_argument("--workload", choices=["single", "batch", "serve"], default="single")
+    ap.add_argument("--workload", choices=["single", "batch", "serve", "mmn"], default="single")
     ap.add_argument("--problems", type=int, default=256, help="batch: number of problems")
     ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second (0: from a probe)")

@@ -126,6 +126,31 @@ int w90gpu_kmesh_get(const double* real_lattice, int num_kpts, const double* kpt
                      int skip_b1_tests, int* nntot, int* nnlist, int* nncell, double* bk,
                      double* wb, double* wbtot, char* errmsg, int errlen);
 
+/* -------------------------------------------------------- file formats */
+
+/* Reads path (a .mmn file) into m_matrix(num_bands,num_bands,nntot,num_kpts)
+ * complex; replaces the .mmn part of overlap_read (src/overlap.F90:234-308).
+ * Each block "k k' G" goes to the b with nnlist(k,b) = k' and
+ * nncell(:,k,b) = G (nnlist(num_kpts,nntot) 1-based, nncell(3,num_kpts,nntot),
+ * the layout of w90gpu_kmesh_get); the later of two blocks with one
+ * destination wins, destinations no block names are zero.  The file is
+ * parsed by nthreads host threads (<= 0: OMP_NUM_THREADS, else all cores);
+ * values are converted with correct rounding, as strtod does.
+ * W90GPU_ERR_FILE with the reference's message in errmsg when the file
+ * cannot be opened or read, its sizes differ, or a block has no (or more
+ * than one) matching neighbour. */
+int w90gpu_read_mmn(const char* path, int num_bands, int num_kpts, int nntot, const int* nnlist,
+                    const int* nncell, double* m_matrix, int nthreads, char* errmsg, int errlen);
+
+/* Reads path (a .amn file) into a_matrix(num_bands,num_proj,num_kpts)
+ * complex; replaces the .amn part of overlap_read (src/overlap.F90:323-366)
+ * for num_proj = num_wann projections, all of them used.  Entries no record
+ * names are zero; the later of two records with one destination wins.
+ * Threads, conversion and errors as w90gpu_read_mmn (a record whose
+ * indices are out of range is a read error). */
+int w90gpu_read_amn(const char* path, int num_bands, int num_kpts, int num_proj, double* a_matrix,
+                    int nthreads, char* errmsg, int errlen);
+
 /* ------------------------------------------------ one-shot drop-in solve */
 
 /* The spread minimisation loop of wann_main (src/wannierise.F90:48-1072)

@@ -31,7 +31,7 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["w90gpu.hip"]
-CPP_SOURCES = ["kmesh.cpp"]
+CPP_SOURCES = ["kmesh.cpp", "w90io.cpp"]
 
 LIB_NAME = "libw90gpu.so"
 
@@ -52,12 +52,12 @@ def build_hip(verbose=True):
         objs.append(obj)
     for src in CPP_SOURCES:
         obj = os.path.join(objdir, src.replace(".cpp", ".o"))
-        _run(["g++", "-O2", "-fPIC", "-std=c++17", "-Wall", f"-I{INCLUDE}", "-c",
+        _run(["g++", "-O2", "-fPIC", "-std=c++17", "-pthread", "-Wall", f"-I{INCLUDE}", "-c",
               os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     out = os.path.join(LIBDIR, LIB_NAME)
     tmp = out + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-pthread", "-o", tmp])
     os.replace(tmp, out)
     return out
 

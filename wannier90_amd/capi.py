@@ -77,6 +77,10 @@ SIGNATURES = [
     ("w90gpu_version", C.c_char_p, []),
     ("w90gpu_kmesh_get", C.c_int, [_dp, C.c_int, _dp, C.c_int, C.c_double, C.c_int, _ip, C.c_int,
                                    _ip, _ip, _ip, _dp, _dp, _dp, C.c_char_p, C.c_int]),
+    ("w90gpu_read_mmn", C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, C.c_int,
+                                  C.c_char_p, C.c_int]),
+    ("w90gpu_read_amn", C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, _dp, C.c_int, C.c_char_p,
+                                  C.c_int]),
     ("w90gpu_wann_main", C.c_int, [_vp, C.POINTER(Problem), C.POINTER(Control), _dp, _dp, _dp, _dp,
                                    C.POINTER(Result)]),
     ("w90gpu_omega", C.c_int, [_vp, C.POINTER(Problem), _dp, C.c_int, C.c_double, _dp, _dp, _dp]),
@@ -194,6 +198,37 @@ def kmesh_get(real_lattice, kpt_latt, search_shells=36, tol=1e-6, shell_list=Non
         wb[:nn].copy(),
         wbtot.value,
     )
+
+
+def read_mmn(path, num_bands, kmesh, nthreads=0):
+    """.mmn file -> M (num_bands, num_bands, nntot, nk) complex, order='F'
+    (w90gpu_read_mmn; overlap_read, src/overlap.F90:234-308): each block
+    goes to the b of the k-mesh with the block's k' and G.  Host threads
+    parse the file; nthreads <= 0 takes OMP_NUM_THREADS or all cores."""
+    lib = load()
+    nnl = np.asfortranarray(kmesh.nnlist, dtype=np.int32)
+    nnc = np.asfortranarray(kmesh.nncell, dtype=np.int32)
+    nk, nn = nnl.shape
+    m = np.empty((num_bands, num_bands, nn, nk), dtype=np.complex128, order="F")
+    err = C.create_string_buffer(512)
+    rc = lib.w90gpu_read_mmn(os.fsencode(path), num_bands, nk, nn, _i(nnl), _i(nnc), m.ctypes.data_as(_dp),
+                             nthreads, err, 512)
+    if rc != OK:
+        raise W90Error(rc, err.value.decode())
+    return m
+
+
+def read_amn(path, num_bands, num_kpts, num_proj, nthreads=0):
+    """.amn file -> A (num_bands, num_proj, nk) complex, order='F'
+    (w90gpu_read_amn; overlap_read, src/overlap.F90:323-366)."""
+    lib = load()
+    a = np.empty((num_bands, num_proj, num_kpts), dtype=np.complex128, order="F")
+    err = C.create_string_buffer(512)
+    rc = lib.w90gpu_read_amn(os.fsencode(path), num_bands, num_kpts, num_proj, a.ctypes.data_as(_dp), nthreads,
+                             err, 512)
+    if rc != OK:
+        raise W90Error(rc, err.value.decode())
+    return a
 
 
 class Context:

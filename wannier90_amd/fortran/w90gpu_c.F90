@@ -108,6 +108,31 @@ module w90gpu_c
       integer(c_int) :: rc
     end function
 
+    function w90gpu_read_mmn(path, num_bands, num_kpts, nntot, nnlist, nncell, m_matrix, nthreads, &
+                             errmsg, errlen) bind(c, name='w90gpu_read_mmn') result(rc)
+      import :: c_int, c_double_complex, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: num_bands, num_kpts, nntot
+      integer(c_int), intent(in) :: nnlist(*), nncell(*)
+      complex(c_double_complex), intent(out) :: m_matrix(*)
+      integer(c_int), value :: nthreads
+      character(kind=c_char), intent(out) :: errmsg(*)
+      integer(c_int), value :: errlen
+      integer(c_int) :: rc
+    end function
+
+    function w90gpu_read_amn(path, num_bands, num_kpts, num_proj, a_matrix, nthreads, errmsg, errlen) &
+      bind(c, name='w90gpu_read_amn') result(rc)
+      import :: c_int, c_double_complex, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: num_bands, num_kpts, num_proj
+      complex(c_double_complex), intent(out) :: a_matrix(*)
+      integer(c_int), value :: nthreads
+      character(kind=c_char), intent(out) :: errmsg(*)
+      integer(c_int), value :: errlen
+      integer(c_int) :: rc
+    end function
+
     function w90gpu_wann_main(ctx, prob, ctl, m_matrix, u_matrix, centres, spreads, res) &
       bind(c, name='w90gpu_wann_main') result(rc)
       import :: c_int, c_ptr, c_double, c_double_complex, w90gpu_problem, w90gpu_control, w90gpu_result

@@ -6,6 +6,8 @@
 !> _u.mat (src/plot.F90:2224-2288), _centres.xyz (src/plot.F90:2790-2862).
 module w90gpu_io
   use iso_fortran_env, only: int32, real64
+  use iso_c_binding, only: c_int, c_char, c_null_char
+  use w90gpu_c, only: w90gpu_read_mmn, w90gpu_read_amn
   implicit none
   private
 
@@ -377,7 +379,8 @@ contains
   end function get_block
 
   !> .mmn: each (k, k', G) block is stored at the b of the k-mesh with
-  !> nnlist(k,b) = k' and nncell(:,k,b) = G (overlap_read).
+  !> nnlist(k,b) = k' and nncell(:,k,b) = G (overlap_read), by the
+  !> library's parallel reader (w90gpu_read_mmn)
   subroutine read_mmn(path, nb, nk, nntot, nnlist, nncell, m, ierr, msg)
     character(len=*), intent(in) :: path
     integer, intent(in) :: nb, nk, nntot
@@ -385,113 +388,51 @@ contains
     complex(dp), intent(out) :: m(nb, nb, nntot, nk)
     integer, intent(out) :: ierr
     character(len=*), intent(out) :: msg
-    integer :: u, ios, nbt, nkt, nnt, ic, k, k2, g(3), b, inn, i, j
-    real(dp) :: re, im
-    character(len=linelen) :: dummy
-    ierr = 0
-    msg = ''
-    open (newunit=u, file=path, status='old', action='read', iostat=ios)
-    if (ios /= 0) then
-      ierr = 6
-      msg = 'Error: Problem opening input file '//trim(path)
-      return
-    end if
-    read (u, '(a)', iostat=ios) dummy
-    read (u, *, iostat=ios) nbt, nkt, nnt
-    if (ios /= 0) goto 100
-    if (nbt /= nb) then
-      ierr = 6
-      msg = trim(path)//' has not the right number of bands'
-      close (u)
-      return
-    end if
-    if (nkt /= nk) then
-      ierr = 6
-      msg = trim(path)//' has not the right number of k-points'
-      close (u)
-      return
-    end if
-    if (nnt /= nntot) then
-      ierr = 6
-      msg = trim(path)//' has not the right number of nearest neighbours'
-      close (u)
-      return
-    end if
-    do ic = 1, nk * nntot
-      read (u, *, iostat=ios) k, k2, g
-      if (ios /= 0) goto 100
-      b = 0
-      do inn = 1, nntot
-        if (k < 1 .or. k > nk) exit
-        if (nnlist(k, inn) == k2 .and. all(nncell(:, k, inn) == g)) then
-          if (b /= 0) then
-            ierr = 6
-            msg = 'Error reading '//trim(path)//'. More than one matching nearest neighbour found'
-            close (u)
-            return
-          end if
-          b = inn
-        end if
-      end do
-      if (b == 0) then
-        ierr = 6
-        msg = 'Neighbour not found'
-        close (u)
-        return
-      end if
-      do j = 1, nb
-        do i = 1, nb
-          read (u, *, iostat=ios) re, im
-          if (ios /= 0) goto 100
-          m(i, j, b, k) = cmplx(re, im, kind=dp)
-        end do
-      end do
-    end do
-    close (u)
-    return
-100 ierr = 6
-    msg = 'Error: Problem reading input file '//trim(path)
-    close (u)
+    character(kind=c_char) :: err(512)
+    integer(c_int), allocatable :: nnl(:, :), nnc(:, :, :)
+    allocate (nnl(nk, nntot), nnc(3, nk, nntot))
+    nnl = int(nnlist, c_int)
+    nnc = int(nncell, c_int)
+    ierr = int(w90gpu_read_mmn(c_path(path), int(nb, c_int), int(nk, c_int), int(nntot, c_int), nnl, nnc, &
+                               m, 0_c_int, err, 512_c_int))
+    msg = c_message(err)
   end subroutine read_mmn
 
+  !> .amn (overlap_read), by the library's parallel reader (w90gpu_read_amn)
   subroutine read_amn(path, nb, nk, nproj, a, ierr, msg)
     character(len=*), intent(in) :: path
     integer, intent(in) :: nb, nk, nproj
     complex(dp), intent(out) :: a(nb, nproj, nk)
     integer, intent(out) :: ierr
     character(len=*), intent(out) :: msg
-    integer :: u, ios, nbt, nkt, npt, ic, m, n, k
-    real(dp) :: re, im
-    character(len=linelen) :: dummy
-    ierr = 0
-    msg = ''
-    a = (0.0_dp, 0.0_dp)
-    open (newunit=u, file=path, status='old', action='read', iostat=ios)
-    if (ios /= 0) then
-      ierr = 6
-      msg = 'Error: Problem opening input file '//trim(path)
-      return
-    end if
-    read (u, '(a)', iostat=ios) dummy
-    read (u, *, iostat=ios) nbt, nkt, npt
-    if (ios /= 0 .or. nbt /= nb .or. nkt /= nk .or. npt /= nproj) then
-      ierr = 6
-      msg = trim(path)//' has not the right number of bands, k-points or projections'
-      close (u)
-      return
-    end if
-    do ic = 1, nb * nproj * nk
-      read (u, *, iostat=ios) m, n, k, re, im
-      if (ios /= 0) then
-        ierr = 6
-        msg = 'Error: Problem reading input file '//trim(path)
-        close (u)
-        return
-      end if
-      a(m, n, k) = cmplx(re, im, kind=dp)
-    end do
-    close (u)
+    character(kind=c_char) :: err(512)
+    ierr = int(w90gpu_read_amn(c_path(path), int(nb, c_int), int(nk, c_int), int(nproj, c_int), a, 0_c_int, &
+                               err, 512_c_int))
+    msg = c_message(err)
   end subroutine read_amn
+
+  !> NUL-terminated copy of a Fortran file name
+  function c_path(path) result(cp)
+    character(len=*), intent(in) :: path
+    character(kind=c_char) :: cp(len_trim(path) + 1)
+    integer :: i
+    do i = 1, len_trim(path)
+      cp(i) = path(i:i)
+    end do
+    cp(len_trim(path) + 1) = c_null_char
+  end function c_path
+
+  !> Fortran text of a NUL-terminated message
+  function c_message(err) result(msg)
+    character(kind=c_char), intent(in) :: err(:)
+    character(len=size(err)) :: msg
+    integer :: i
+    msg = ''
+    do i = 1, size(err)
+      if (err(i) == c_null_char) exit
+      msg(i:i) = err(i)
+    end do
+  end function c_message
 
   !> unformatted checkpoint, record by record as the reference writes it
   !> without disentanglement

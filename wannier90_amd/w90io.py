@@ -88,42 +88,31 @@ def write_eig(path, evals_kn):
                 f.write(f"{n + 1:5d}{k + 1:5d}{evals_kn[k, n]:22.14f}\n")
 
 
-def read_mmn(path, kmesh):
-    """Reads .mmn into (nb,nb,nn,nk) with b ordered as kmesh (overlap_read)."""
+def _sizes(path):
+    """The three sizes on the second line of a .mmn or .amn file."""
     with open(path) as f:
         f.readline()
-        nb, nk, nn = (int(x) for x in f.readline().split())
-        if nn != kmesh.nntot:
-            raise ValueError(f"{path} has nntot={nn}, k-mesh has {kmesh.nntot}")
-        data = np.loadtxt(f, dtype=str, delimiter="\n") if False else f.read().split("\n")
-    m = np.zeros((nb, nb, nn, nk), dtype=np.complex128, order="F")
-    pos = 0
-    lookup = {}
-    for k in range(nk):
-        for b in range(nn):
-            lookup[(k + 1, int(kmesh.nnlist[k, b]), *[int(x) for x in kmesh.nncell[:, k, b]])] = b
-    for _ in range(nk * nn):
-        while not data[pos].strip():
-            pos += 1
-        hdr = tuple(int(x) for x in data[pos].split())
-        pos += 1
-        vals = np.array([l.split() for l in data[pos:pos + nb * nb]], dtype=float)
-        pos += nb * nb
-        if hdr not in lookup:
-            raise ValueError(f"neighbour {hdr} not found in the k-mesh")
-        m[:, :, lookup[hdr], hdr[0] - 1] = (vals[:, 0] + 1j * vals[:, 1]).reshape((nb, nb), order="F")
-    return m
+        for line in f:
+            if line.strip():
+                return tuple(int(x) for x in line.split()[:3])
+    raise ValueError(f"{path}: no sizes line")
 
 
-def read_amn(path):
-    with open(path) as f:
-        f.readline()
-        nb, nk, npj = (int(x) for x in f.readline().split())
-        raw = np.array(f.read().split(), dtype=float).reshape(-1, 5)
-    a = np.zeros((nk, nb, npj), dtype=np.complex128)
-    idx = raw[:, :3].astype(int) - 1
-    a[idx[:, 2], idx[:, 0], idx[:, 1]] = raw[:, 3] + 1j * raw[:, 4]
-    return a
+def read_mmn(path, kmesh, nthreads=0):
+    """Reads .mmn into (nb,nb,nn,nk) with b ordered as kmesh (overlap_read),
+    through the library's parallel reader (w90gpu_read_mmn)."""
+    from . import capi
+    nb = _sizes(path)[0]
+    return capi.read_mmn(path, nb, kmesh, nthreads)
+
+
+def read_amn(path, nthreads=0):
+    """Reads .amn into (nk, nb, nproj) with A_mn(k) (overlap_read), through
+    the library's parallel reader (w90gpu_read_amn)."""
+    from . import capi
+    nb, nk, npj = _sizes(path)
+    a = capi.read_amn(path, nb, nk, npj, nthreads)
+    return np.ascontiguousarray(a.transpose(2, 0, 1))
 
 
 # ------------------------------------------------------------ checkpoint

@@ -389,7 +389,93 @@ def run_serve(args):
     }
 
 
+def write_mmn_f18(path, nb, kmesh, seed=0):
+    """A .mmn with uniform random entries in pw2wannier90's record format
+    (2f18.12), blocks in the k-mesh's (k, b) order."""
+    nk, nn = kmesh.nnlist.shape
+    rng = np.random.default_rng(seed)
+    with open(path, "w") as f:
+        f.write(" Created by wannier90_amd.workloads\n")
+        f.write(f"{nb:12d}{nk:12d}{nn:12d}\n")
+        for k in range(nk):
+            for b in range(nn):
+                g = kmesh.nncell[:, k, b]
+                f.write(f"{k + 1:5d}{kmesh.nnlist[k, b]:5d}{g[0]:5d}{g[1]:5d}{g[2]:5d}\n")
+                v = rng.uniform(-1.0, 1.0, (nb * nb, 2))
+                f.write("".join("%18.12f%18.12f\n" % (x, y) for x, y in v))
+
+
+def run_mmn(args):
+    """Parse of the benchmark problem's .mmn (w90gpu_read_mmn) with all
+    host threads and with one; the two results must agree bit for bit, and
+    sampled entries must equal float() of their text."""
+    import tempfile
+
+    from . import capi
+
+    prob = synth.config(args.config)
+    mesh = prob.kmesh
+    nb = prob.num_wann
+    nk, nn = mesh.nnlist.shape
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "bench.mmn")
+        t = time.perf_counter()
+        write_mmn_f18(path, nb, mesh)
+        log(f"[mmn] wrote {os.path.getsize(path) / 1e6:.1f} MB in {time.perf_counter() - t:.1f}s")
+        size = os.path.getsize(path)
+
+        def timed(nt, reps):
+            best, m = 1e30, None
+            for _ in range(reps):
+                t = time.perf_counter()
+                m = capi.read_mmn(path, nb, mesh, nthreads=nt)
+                best = min(best, time.perf_counter() - t)
+            return best, m
+
+        capi.read_mmn(path, nb, mesh)  # page cache warm
+        t_all, m_all = timed(0, max(args.steps, 1))
+        t_one, m_one = timed(1, max(min(args.steps, 3), 1))
+        same = np.array_equal(m_all.ravel(order="F").view(np.uint64), m_one.ravel(order="F").view(np.uint64))
+        # sampled records against float() of their text
+        rng = np.random.default_rng(1)
+        lines = open(path).read().split("\n")
+        ok = 0
+        picks = rng.integers(0, nk * nn * nb * nb, 2000)
+        for q in picks:
+            blk, e = divmod(int(q), nb * nb)
+            k, b = divmod(blk, nn)
+            re, im = (float(x) for x in lines[2 + blk * (nb * nb + 1) + 1 + e].split())
+            z = m_all[e % nb, e // nb, b, k]
+            ok += int(z.real == re and z.imag == im)
+    nt = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    return {
+        "metric": ".mmn parse throughput (w90gpu_read_mmn)",
+        "value": size / t_all / 1e6,
+        "unit": "MB/s",
+        "n_gpus": 1,
+        "steps": max(args.steps, 1),
+        "warmup": 1,
+        "ms_per_step": t_all * 1e3,
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f".mmn of num_bands={nb}, num_kpts={nk}, nntot={nn} (2f18.12 records, "
+                        f"{nk * nn * nb * nb} value pairs, {size / 1e6:.1f} MB), host threads {nt}",
+            "seconds_all_threads": t_all,
+            "seconds_one_thread": t_one,
+            "one_thread_mb_s": size / t_one / 1e6,
+            "bit_identical_1_vs_all_threads": bool(same),
+            "sampled_equal_float_of_text": f"{ok}/{len(picks)}",
+        },
+    }
+
+
 def run(args):
     if args.workload == "batch":
         return run_batch(args)
+    if args.workload == "mmn":
+        return run_mmn(args)
     return run_serve(args)
