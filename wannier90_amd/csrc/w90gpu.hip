@@ -820,13 +820,11 @@ int launch_domega(Engine* E, const Lists& L, int mode) {
                              g.n, mode);
       } else if (c == 2) {
         if (q == 0)
-          hipLaunchKernelGGL((k_domega_k<W90_STEP_N, cplx>), dim3(g.n), dim3(domega_nt<W90_STEP_N>()),
-                             (domega_lds<W90_STEP_N, cplx>()), s, P, it,
-                             g.n, mode);
+          hipLaunchKernelGGL((k_domega_w64<cplx>), dim3(g.n), dim3(domega_nt<W90_STEP_N>()),
+                             (domega_lds<W90_STEP_N, cplx>()), s, P, it, g.n, mode);
         else
-          hipLaunchKernelGGL((k_domega_k<W90_STEP_N, float2>), dim3(g.n), dim3(domega_nt<W90_STEP_N>()),
-                             (domega_lds<W90_STEP_N, float2>()), s, P, it,
-                             g.n, mode);
+          hipLaunchKernelGGL((k_domega_w64<float2>), dim3(g.n), dim3(domega_nt<W90_STEP_N>()),
+                             (domega_lds<W90_STEP_N, float2>()), s, P, it, g.n, mode);
       } else {
         if (q == 0) hipLaunchKernelGGL(k_domega<cplx>, dim3(g.n), dim3(1024), 0, s, P, it, g.n, mode);
         else hipLaunchKernelGGL(k_domega<float2>, dim3(g.n), dim3(1024), 0, s, P, it, g.n, mode);
@@ -1228,8 +1226,8 @@ int w90gpu_create(int device, w90gpu_ctx** ctx) {
       {(const void*)k_domega_k<16, float2>, domega_lds<16, float2>()},
       {(const void*)k_domega_k<kLdsMaxWann, cplx>, domega_lds<kLdsMaxWann, cplx>()},
       {(const void*)k_domega_k<kLdsMaxWann, float2>, domega_lds<kLdsMaxWann, float2>()},
-      {(const void*)k_domega_k<W90_STEP_N, cplx>, domega_lds<W90_STEP_N, cplx>()},
-      {(const void*)k_domega_k<W90_STEP_N, float2>, domega_lds<W90_STEP_N, float2>()},
+      {(const void*)k_domega_w64<cplx>, domega_lds<W90_STEP_N, cplx>()},
+      {(const void*)k_domega_w64<float2>, domega_lds<W90_STEP_N, float2>()},
   };
   for (const auto& l : lds) {
     if (hipFuncSetAttribute(l.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.bytes) != hipSuccess) {

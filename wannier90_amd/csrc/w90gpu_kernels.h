@@ -1078,6 +1078,124 @@ __global__ void __launch_bounds__(domega_nt<NWM>()) k_domega_k(const Prob* probs
   }
 }
 
+// wann_domega for 32 < num_wann <= 64: the formula and the sum over b of
+// k_domega_k, with one block in LDS and the next two requested ahead into
+// registers (two block loads in flight while a block is reduced).  Thread
+// t takes row m = t % 64 and columns n = t / 64 + 16 s (s < 4) of the block
+// padded to 64 x 64, so its row terms (M_mm, 1/M_mm, the phase of row m)
+// are read once per b and its element offsets are one base and a uniform
+// stride; with this layout the two blocks fit the register budget of 1024
+// threads (three do not).  For num_wann = 64 the thread's elements are
+// those of k_domega_k, so the per-k sums Re<G,G> and Re<G,D_old> are formed
+// in the same order too.
+constexpr int kDw64Ahead = 2;
+template <class T2>
+__global__ void __launch_bounds__(1024) k_domega_w64(const Prob* probs, const Item* items, int nitems, int mode) {
+  if (blockIdx.x >= (unsigned)nitems) return;
+  const Item it = items[blockIdx.x];
+  const Prob& p = probs[it.prob];
+  const State* st = p.st;
+  if (mode != MODE_BENCH && (st->done || st->hold)) return;
+  constexpr int NWM = 64, NT = 1024, SL = NWM * NWM / NT, RS = NT / NWM, LD = NWM + 1;
+  constexpr int AH = kDw64Ahead;
+  static_assert(domega_nt<NWM>() == NT && domega_chunk<NWM>() == 1, "LDS layout of domega_lds<64>");
+  const int nw = p.nw, nn = p.nn, nw2 = p.nw2;
+  const int k = it.k;
+  const int tid = threadIdx.x;
+  const int cur = st->cur;
+  const int m = tid % NWM, n0 = tid / NWM;
+  extern __shared__ double lds_domega[];
+  double* qt = lds_domega;                                           // [b][n]
+  cplx* rec = reinterpret_cast<cplx*>(qt + W90GPU_NUM_NNMAX * NWM);  // [b][n]
+  double* red = qt + 3 * W90GPU_NUM_NNMAX * NWM;
+  T2* Ms = reinterpret_cast<T2*>(red + NT);
+  const double* rave = p.rave[cur];
+  const T2* Mk = mbuf<T2>(p, cur) + (size_t)k * nn * nw2;
+  // element s: (m, n0 + RS s), at g0 + s RS nw in the block
+  const int g0 = m + n0 * nw, gs = RS * nw;
+  bool ev[SL];
+#pragma unroll
+  for (int s = 0; s < SL; ++s) ev[s] = m < nw && n0 + RS * s < nw;
+  T2 nxt[AH][SL];
+#pragma unroll
+  for (int j = 0; j < AH; ++j)
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      T2 v = {};
+      if (j < nn && ev[s]) v = Mk[(size_t)j * nw2 + g0 + s * gs];
+      nxt[j][s] = v;
+    }
+  for (int t = tid; t < nn * nw; t += NT) {
+    const int b = t / nw, n = t % nw;
+    const size_t kb = (size_t)k * nn + b;
+    const double* bv = p.bk + kb * 3;
+    const double w = p.wb[b];
+    const double rbn = bv[0] * rave[0 + 3 * n] + bv[1] * rave[1 + 3 * n] + bv[2] * rave[2 + 3 * n];
+    qt[t] = w * p.ln[cur][kb * nw + n] + w * rbn;
+    const cplx d = c_wide(Mk[(size_t)b * nw2 + n + (size_t)n * nw]);
+    const double inv = 1.0 / c_abs2(d);
+    rec[t] = make_double2(d.x * inv, -d.y * inv);
+  }
+  cplx acc[SL];
+#pragma unroll
+  for (int s = 0; s < SL; ++s) acc[s] = c_make(0.0, 0.0);
+  __shared__ double wL[W90GPU_NUM_NNMAX];
+  if (tid < nn) wL[tid] = p.wb[tid];
+  for (int b0 = 0; b0 < nn; b0 += AH) {
+#pragma unroll
+    for (int j = 0; j < AH; ++j) {
+      const int b = b0 + j;
+      if (b < nn) {
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < SL; ++s)
+          if (ev[s]) Ms[m + (n0 + RS * s) * LD] = nxt[j][s];
+        if (b + AH < nn) {
+          const T2* src = Mk + (size_t)(b + AH) * nw2;
+#pragma unroll
+          for (int s = 0; s < SL; ++s) {
+            T2 v = {};
+            if (ev[s]) v = src[g0 + s * gs];
+            nxt[j][s] = v;
+          }
+        }
+        __syncthreads();
+        if (m < nw) {
+          const double w = wL[b];
+          const cplx mmm = c_wide(Ms[m * (LD + 1)]);
+          const cplx rm = rec[b * nw + m];
+          const double tm = qt[b * nw + m];
+#pragma unroll
+          for (int s = 0; s < SL; ++s) {
+            const int n = n0 + RS * s;
+            if (n < nw)
+              acc[s] = domega_term_r(acc[s], w, c_wide(Ms[m + n * LD]), c_wide(Ms[n + m * LD]),
+                                     c_wide(Ms[n * (LD + 1)]), mmm, rec[b * nw + n], rm, qt[b * nw + n], tm);
+          }
+        }
+      }
+    }
+  }
+  const double rnk = (double)p.nk;
+  double gg = 0.0, gd = 0.0;
+#pragma unroll
+  for (int s = 0; s < SL; ++s) {
+    if (ev[s]) {
+      const cplx g = c_scale(make_double2(acc[s].x / rnk, acc[s].y / rnk), 4.0);
+      st_stream(&p.G[(size_t)k * nw2 + g0 + s * gs], g);
+      gg += c_abs2(g);
+      const cplx dold = p.D[(size_t)k * nw2 + g0 + s * gs];
+      gd += g.x * dold.x + g.y * dold.y;
+    }
+  }
+  gg = block_sum(gg, red);
+  gd = block_sum(gd, red);
+  if (tid == 0) {
+    p.part_gg[k] = gg;
+    p.part_gd[k] = gd;
+  }
+}
+
 // Scalar part of internal_search_direction (wannierise.F90:1371-1525) on
 // a state copy: gg = Re<G,G>, gd = Re<G,D_old>.  Every workgroup of the
 // trial-step kernel runs it on its own copy; the CG counter and the previous

@@ -676,6 +676,31 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
   double* Pi = Pr + W90_STEP_MAT;
   double* red = Pi + W90_STEP_MAT;
   cplx* Dk = p.D + (size_t)k * nw2;
+  // every G, D and U element of the thread is requested before the search
+  // and the exponential (one workgroup fills a CU's LDS, so there are no
+  // other wavefronts to cover the latency of loads issued one loop trip
+  // at a time)
+  constexpr int SL = W90_STEP_N * W90_STEP_N / 256;
+  cplx gpre[SL], dpre[SL], upre[SL];
+  {
+    const cplx* Gk = p.G + (size_t)k * nw2;
+    const cplx* Uc = p.U[cur] + (size_t)k * nw2;
+#pragma unroll
+    for (int i = 0; i < SL; ++i) {
+      const int e = tid + 256 * i;
+      const int r = e % W90_STEP_N, c = e / W90_STEP_N;
+      cplx g = c_make(0.0, 0.0), d = g, u = g;
+      if (r < nw && c < nw) {
+        const int q = r + c * nw;
+        if (mode == MODE_TRIAL) g = Gk[q];
+        d = Dk[q];
+        u = Uc[q];
+      }
+      gpre[i] = g;
+      dpre[i] = d;
+      upre[i] = u;
+    }
+  }
   __shared__ State lst;
   const State* ss = st;
   if (mode == MODE_TRIAL) {
@@ -684,15 +709,16 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
   }
   const double scale = ss->scale;
   // X = scale D (zero outside nw), D formed on the trial step
-  for (int e = tid; e < W90_STEP_N * W90_STEP_N; e += 256) {
+#pragma unroll
+  for (int i = 0; i < SL; ++i) {
+    const int e = tid + 256 * i;
     const int r = e % W90_STEP_N, c = e / W90_STEP_N;
     cplx x = c_make(0.0, 0.0);
     if (r < nw && c < nw) {
-      const int g = r + c * nw;
-      cplx d = Dk[g];
+      cplx d = dpre[i];
       if (mode == MODE_TRIAL) {
-        d = c_scale(c_add(p.G[(size_t)k * nw2 + g], c_scale(d, ss->beta)), ss->dir_sign);
-        Dk[g] = d;
+        d = c_scale(c_add(gpre[i], c_scale(d, ss->beta)), ss->dir_sign);
+        st_stream(&Dk[r + c * nw], d);
       }
       x = c_scale(d, scale);
     }
@@ -773,16 +799,16 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
         }
     __syncthreads();
   }
-  // E = P to global; U into the X slot
+  // E = P to global; U into the X slot (zero outside nw)
   cplx* Ek = p.E + (size_t)k * nw2;
-  const cplx* Uc = p.U[cur] + (size_t)k * nw2;
-  for (int e = tid; e < nw2; e += 256) {
-    const int r = e % nw, c = e / nw;
+#pragma unroll
+  for (int i = 0; i < SL; ++i) {
+    const int e = tid + 256 * i;
+    const int r = e % W90_STEP_N, c = e / W90_STEP_N;
     const int q = c * W90_STEP_LD + r;
-    Ek[e] = c_make(Pr[q], Pi[q]);
-    const cplx u = Uc[e];
-    Xr[q] = u.x;
-    Xi[q] = u.y;
+    if (r < nw && c < nw) st_stream(&Ek[r + c * nw], c_make(Pr[q], Pi[q]));
+    Xr[q] = upre[i].x;
+    Xi[q] = upre[i].y;
   }
   __syncthreads();
   step_gemm(C, Xr, Xi, Pr, Pi, nw);
@@ -795,7 +821,7 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
       for (int i = 0; i < 4; ++i) {
         int r, c;
         step_pos(fi, fj, i, &r, &c);
-        if (r < nw && c < nw) Un[r + c * nw] = c_make(C[fi][fj].re(i), C[fi][fj].im(i));
+        if (r < nw && c < nw) st_stream(&Un[r + c * nw], c_make(C[fi][fj].re(i), C[fi][fj].im(i)));
       }
 }
 
