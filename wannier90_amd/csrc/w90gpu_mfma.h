@@ -605,45 +605,53 @@ __global__ void __launch_bounds__(256) k_reduce_tiles(const Prob* probs, const I
 // LDS bytes of k_step_mfma: X and P (re, im each) and a reduction buffer.
 __host__ __device__ constexpr size_t step_mfma_lds() { return sizeof(double) * (4 * W90_STEP_MAT + 256); }
 
-// C = A B for the 64 x 64 LDS matrices (A, B planar at Ar/Ai, Br/Bi): each
-// wavefront accumulates its 32 x 32 quadrant as 2 x 2 complex FP64 MFMA
-// fragments (three real products per K step, Cfrag); fragments wholly
-// outside n are skipped (their rows and columns are zero).
-__device__ __forceinline__ void step_gemm(Cfrag<double> (&C)[2][2], const double* Ar,
-                                          const double* Ai, const double* Br, const double* Bi, int n) {
+// C = A B for the 64 x 64 LDS matrices (planar at Ar/Ai, Br/Bi) over the
+// first 4 KS columns of A (rows of B): each wavefront accumulates its 32 x 32 quadrant as 2 x 2 complex
+// FP64 MFMA fragments (three real products per K step, Cfrag).  The rows
+// and columns beyond num_wann are zero in every operand, so all fragments
+// are formed and the K steps past num_wann (up to KS) add zeros.  The K
+// steps are unrolled without branches and the operands of step ks + 1 are
+// read from LDS before the products of step ks are issued: with one
+// wavefront per SIMD nothing else covers the LDS latency.
+template <int KS>
+__device__ __forceinline__ void step_gemm(Cfrag<double> (&C)[2][2], const double* Ar, const double* Ai,
+                                          const double* Br, const double* Bi) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
 #pragma unroll
   for (int fi = 0; fi < 2; ++fi)
 #pragma unroll
-    for (int fj = 0; fj < 2; ++fj)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        C[fi][fj].p1[i] = 0.0, C[fi][fj].p2[i] = 0.0, C[fi][fj].p3[i] = 0.0;
-      }
-  const bool vr0 = wr < n, vr1 = wr + 16 < n, vc0 = wc < n, vc1 = wc + 16 < n;
-  for (int ks = 0; ks < n; ks += 4) {
-    double ar[2], ai[2], br[2], bi[2];
+    for (int fj = 0; fj < 2; ++fj) C[fi][fj].zero();
+  double ar[2][2], ai[2][2], br[2][2], bi[2][2];  // [buffer][fragment]
+  auto load = [&](int ks, int buf) {
 #pragma unroll
     for (int f = 0; f < 2; ++f) {
-      const int ia = (ks + lk) * W90_STEP_LD + wr + f * 16 + lr;  // A[row][ks+lk]
-      const int ib = (wc + f * 16 + lr) * W90_STEP_LD + ks + lk;  // B[ks+lk][col]
-      ar[f] = Ar[ia];
-      ai[f] = Ai[ia];
-      br[f] = Br[ib];
-      bi[f] = Bi[ib];
+      const int ia = (4 * ks + lk) * W90_STEP_LD + wr + f * 16 + lr;  // A[row][4 ks + lk]
+      ar[buf][f] = Ar[ia];
+      ai[buf][f] = Ai[ia];
+      const int ib = (wc + f * 16 + lr) * W90_STEP_LD + 4 * ks + lk;  // B[4 ks + lk][col]
+      br[buf][f] = Br[ib];
+      bi[buf][f] = Bi[ib];
     }
+  };
+  load(0, 0);
 #pragma unroll
-    for (int fi = 0; fi < 2; ++fi) {
-      if (!(fi ? vr1 : vr0)) continue;
+  for (int ks = 0; ks < KS; ++ks) {
+    const int cb = ks & 1;
+    if (ks + 1 < KS) load(ks + 1, cb ^ 1);
 #pragma unroll
-      for (int fj = 0; fj < 2; ++fj) {
-        if (!(fj ? vc1 : vc0)) continue;
-        C[fi][fj].mma(ar[fi], ai[fi], br[fj], bi[fj]);
-      }
-    }
+    for (int fi = 0; fi < 2; ++fi)
+#pragma unroll
+      for (int fj = 0; fj < 2; ++fj) C[fi][fj].mma(ar[cb][fi], ai[cb][fi], br[cb][fj], bi[cb][fj]);
   }
+}
+
+// step_gemm over the K steps num_wann needs: 12 up to 48 bands, else 16.
+__device__ __forceinline__ void step_gemm_n(Cfrag<double> (&C)[2][2], const double* Ar, const double* Ai,
+                                            const double* Br, const double* Bi, int n) {
+  if (n > 48) step_gemm<16>(C, Ar, Ai, Br, Bi);
+  else step_gemm<12>(C, Ar, Ai, Br, Bi);
 }
 
 // Position (row, column) of accumulator element i of fragment (fi, fj).
@@ -764,7 +772,7 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
   Cfrag<double> C[2][2];
   // Horner: P <- I + X P / j
   for (int j = m - 1; j >= 1; --j) {
-    step_gemm(C, Xr, Xi, Pr, Pi, nw);
+    step_gemm_n(C, Xr, Xi, Pr, Pi, nw);
     __syncthreads();
     const double invj = 1.0 / (double)j;
 #pragma unroll
@@ -783,7 +791,7 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
   }
   // squarings: P <- P P
   for (int s = 0; s < sq; ++s) {
-    step_gemm(C, Pr, Pi, Pr, Pi, nw);
+    step_gemm_n(C, Pr, Pi, Pr, Pi, nw);
     __syncthreads();
 #pragma unroll
     for (int fi = 0; fi < 2; ++fi)
@@ -811,7 +819,7 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
     Xi[q] = upre[i].y;
   }
   __syncthreads();
-  step_gemm(C, Xr, Xi, Pr, Pi, nw);
+  step_gemm_n(C, Xr, Xi, Pr, Pi, nw);
   cplx* Un = p.U[1 - cur] + (size_t)k * nw2;
 #pragma unroll
   for (int fi = 0; fi < 2; ++fi)
