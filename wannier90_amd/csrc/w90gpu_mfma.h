@@ -355,12 +355,6 @@ __device__ __forceinline__ void rot_fused_tile(const Prob& p, int k, int b, int 
   const int w = tid >> 6, lane = tid & 63;
   const int wr = (w >> 1) * (GT / 2), wc = (w & 1) * (GT / 2);
   const int lr = lane & 15, lk = lane >> 4;
-  bool fv_r[FR], fv_c[FR];
-#pragma unroll
-  for (int f = 0; f < FR; ++f) {
-    fv_r[f] = wr + f * 16 < nw;
-    fv_c[f] = wc + f * 16 < nw;
-  }
   Cfrag<R> C[FR][FR];
 #pragma unroll
   for (int fi = 0; fi < FR; ++fi)
@@ -412,14 +406,9 @@ __device__ __forceinline__ void rot_fused_tile(const Prob& p, int k, int b, int 
         bi[f] = Bs[(1 * W90_GK + ks + lk) * LDA + wc + f * 16 + lr];
       }
 #pragma unroll
-      for (int fi = 0; fi < FR; ++fi) {
-        if (!fv_r[fi]) continue;
+      for (int fi = 0; fi < FR; ++fi)
 #pragma unroll
-        for (int fj = 0; fj < FR; ++fj) {
-          if (!fv_c[fj]) continue;
-          C[fi][fj].mma(ar[fi], ai[fi], br[fj], bi[fj]);
-        }
-      }
+        for (int fj = 0; fj < FR; ++fj) C[fi][fj].mma(ar[fi], ai[fi], br[fj], bi[fj]);
     }
   }
   // every wavefront is done with the staging buffers before T overwrites them
@@ -478,14 +467,9 @@ __device__ __forceinline__ void rot_fused_tile(const Prob& p, int k, int b, int 
         bi[f] = B2[(1 * W90_GK2 + ks + lk) * LDA + wc + f * 16 + lr];
       }
 #pragma unroll
-      for (int fi = 0; fi < FR; ++fi) {
-        if (!fv_r[fi]) continue;
+      for (int fi = 0; fi < FR; ++fi)
 #pragma unroll
-        for (int fj = 0; fj < FR; ++fj) {
-          if (!fv_c[fj]) continue;
-          C[fi][fj].mma(ar[fi], ai[fi], br[fj], bi[fj]);
-        }
-      }
+        for (int fj = 0; fj < FR; ++fj) C[fi][fj].mma(ar[fi], ai[fi], br[fj], bi[fj]);
     }
   }
 
