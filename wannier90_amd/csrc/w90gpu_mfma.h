@@ -137,6 +137,55 @@ __device__ void lds_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n) {
   __syncthreads();
 }
 
+// The 16-band case of lds_gemm_mfma: one tile, wavefront 0 forms the real
+// part and wavefront 1 the imaginary part (same MFMA sequence and order as
+// the general form, so the same result bit for bit).  The four K steps'
+// operands are read at once, outside n through a clamped index and a
+// select rather than a lane branch, so the reads are in flight together
+// and the products follow a single wait.
+template <>
+__device__ void lds_gemm_mfma<16>(cplx* C, const cplx* A, const cplx* B, int n) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int ld = n + 1;
+  if (wave < 2) {
+    double ax[4], ay[4], bx[4], by[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int l = 4 * s + lk;
+      const bool ok = lr < n && l < n;  // A[lr][l] and B[l][lr] inside n
+      const cplx a = A[ok ? lr + l * ld : 0];
+      const cplx bv = B[ok ? l + lr * ld : 0];
+      ax[s] = ok ? a.x : 0.0;
+      ay[s] = ok ? a.y : 0.0;
+      bx[s] = ok ? bv.x : 0.0;
+      by[s] = ok ? bv.y : 0.0;
+    }
+    Mfma<double>::acc acc = {0.0, 0.0, 0.0, 0.0};
+    if (wave == 0) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        acc = Mfma<double>::mma(ax[s], bx[s], acc);
+        acc = Mfma<double>::mma(-ay[s], by[s], acc);
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        acc = Mfma<double>::mma(ax[s], by[s], acc);
+        acc = Mfma<double>::mma(ay[s], bx[s], acc);
+      }
+    }
+    double* Cd = reinterpret_cast<double*>(C);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = Mfma<double>::row(lk, q), c = lr;
+      if (r < n && c < n) Cd[2 * (r + c * ld) + wave] = acc[q];
+    }
+  }
+  __syncthreads();
+}
+
 // output tile edge of a workgroup: 32 up to num_wann 32, 64 above
 __host__ __device__ __forceinline__ int gemm_edge(int nw) { return nw <= 32 ? 32 : 64; }
 
