@@ -103,7 +103,11 @@ __device__ void lds_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n) {
   static_assert(NWM % 16 == 0, "whole 16 x 16 tiles");
   constexpr int TN = NWM / 16;
   constexpr int JOBS = 2 * TN * TN;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the job index lives in a scalar register, and operands outside n are
+  // read through a clamped index and a select: no lane branches in the K
+  // loop, so its LDS reads are issued together
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   const int lr = lane & 15, lk = lane >> 4;
   const int ld = n + 1;
   const int nwaves = blockDim.x >> 6;
@@ -113,74 +117,36 @@ __device__ void lds_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n) {
     const int t = job >> 1;
     const int ti = t % TN, tj = t / TN;
     const int i = 16 * ti + lr, j = 16 * tj + lr;
-    Mfma<double>::acc acc = {0.0, 0.0, 0.0, 0.0};
+    double ax[NWM / 4], ay[NWM / 4], bx[NWM / 4], by[NWM / 4];
 #pragma unroll
     for (int s = 0; s < NWM / 4; ++s) {
       const int l = 4 * s + lk;
-      cplx a = c_make(0.0, 0.0), bv = c_make(0.0, 0.0);
-      if (i < n && l < n) a = A[i + l * ld];
-      if (l < n && j < n) bv = B[l + j * ld];
-      if (part == 0) {
-        acc = Mfma<double>::mma(a.x, bv.x, acc);
-        acc = Mfma<double>::mma(-a.y, bv.y, acc);
-      } else {
-        acc = Mfma<double>::mma(a.x, bv.y, acc);
-        acc = Mfma<double>::mma(a.y, bv.x, acc);
+      const bool oka = i < n && l < n, okb = l < n && j < n;
+      const cplx a = A[oka ? i + l * ld : 0];
+      const cplx bv = B[okb ? l + j * ld : 0];
+      ax[s] = oka ? a.x : 0.0;
+      ay[s] = oka ? a.y : 0.0;
+      bx[s] = okb ? bv.x : 0.0;
+      by[s] = okb ? bv.y : 0.0;
+    }
+    Mfma<double>::acc acc = {0.0, 0.0, 0.0, 0.0};
+    if (part == 0) {
+#pragma unroll
+      for (int s = 0; s < NWM / 4; ++s) {
+        acc = Mfma<double>::mma(ax[s], bx[s], acc);
+        acc = Mfma<double>::mma(-ay[s], by[s], acc);
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < NWM / 4; ++s) {
+        acc = Mfma<double>::mma(ax[s], by[s], acc);
+        acc = Mfma<double>::mma(ay[s], bx[s], acc);
       }
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int r = 16 * ti + Mfma<double>::row(lk, q), c = 16 * tj + lr;
       if (r < n && c < n) Cd[2 * (r + c * ld) + part] = acc[q];
-    }
-  }
-  __syncthreads();
-}
-
-// The 16-band case of lds_gemm_mfma: one tile, wavefront 0 forms the real
-// part and wavefront 1 the imaginary part (same MFMA sequence and order as
-// the general form, so the same result bit for bit).  The four K steps'
-// operands are read at once, outside n through a clamped index and a
-// select rather than a lane branch, so the reads are in flight together
-// and the products follow a single wait.
-template <>
-__device__ void lds_gemm_mfma<16>(cplx* C, const cplx* A, const cplx* B, int n) {
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const int lr = lane & 15, lk = lane >> 4;
-  const int ld = n + 1;
-  if (wave < 2) {
-    double ax[4], ay[4], bx[4], by[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int l = 4 * s + lk;
-      const bool ok = lr < n && l < n;  // A[lr][l] and B[l][lr] inside n
-      const cplx a = A[ok ? lr + l * ld : 0];
-      const cplx bv = B[ok ? l + lr * ld : 0];
-      ax[s] = ok ? a.x : 0.0;
-      ay[s] = ok ? a.y : 0.0;
-      bx[s] = ok ? bv.x : 0.0;
-      by[s] = ok ? bv.y : 0.0;
-    }
-    Mfma<double>::acc acc = {0.0, 0.0, 0.0, 0.0};
-    if (wave == 0) {
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        acc = Mfma<double>::mma(ax[s], bx[s], acc);
-        acc = Mfma<double>::mma(-ay[s], by[s], acc);
-      }
-    } else {
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        acc = Mfma<double>::mma(ax[s], by[s], acc);
-        acc = Mfma<double>::mma(ay[s], bx[s], acc);
-      }
-    }
-    double* Cd = reinterpret_cast<double*>(C);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int r = Mfma<double>::row(lk, q), c = lr;
-      if (r < n && c < n) Cd[2 * (r + c * ld) + wave] = acc[q];
     }
   }
   __syncthreads();
