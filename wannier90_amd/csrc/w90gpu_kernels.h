@@ -250,12 +250,25 @@ __device__ __forceinline__ float2* mbuf<float2>(const Prob& p, int i) {
   return p.Mf[i];
 }
 
+// Whether a kernel of this mode works on the problem.  The flags are read
+// unconditionally, so their loads are in flight together: the state was
+// written by the kernel before, and a read behind each test would pay a
+// memory latency of its own.
 __device__ __forceinline__ bool gate(const State* st, int mode) {
+  const int done = st->done, hold = st->hold, second = st->second;
   if (mode == MODE_BENCH) return true;
-  if (mode == MODE_REMIX) return st->hold != 0 && !st->done;
-  if (st->done || st->hold) return false;
-  if (mode == MODE_SECOND) return st->second != 0;
+  if (mode == MODE_REMIX) return (hold != 0) & (done == 0);
+  if (done | hold) return false;
+  if (mode == MODE_SECOND) return second != 0;
   return true;
+}
+
+// gate() and the current buffer in one round of loads: the buffer, or -1
+// when the kernel skips the problem (the early return depends on the
+// buffer's value, so its load is issued with the flags').
+__device__ __forceinline__ int gate_cur(const State* st, int mode) {
+  const int cur = st->cur;
+  return gate(st, mode) ? cur : -1;
 }
 
 // buffer written by a rotation of this mode: the next buffer for a step;
@@ -483,10 +496,10 @@ __global__ void __launch_bounds__(W90_NT) k_kpart(const Prob* probs, const Item*
   const Item it = items[blockIdx.x];
   const Prob& p = probs[it.prob];
   const State* st = p.st;
-  if (!gate(st, mode)) return;
+  const int cur = gate_cur(st, mode);
+  if (cur < 0) return;
   const int nw = p.nw, nn = p.nn;
   const int k = it.k;
-  const int cur = st->cur;
   const int dst = dst_buf(mode, cur);
   const double* ln = p.ln[dst];
   double* out = p.kpart + (size_t)k * p.psz;
@@ -769,9 +782,9 @@ __global__ void __launch_bounds__(W90_FINAL_NT) k_omega_final(const Prob* probs,
     reinterpret_cast<double*>(&sst)[i] = reinterpret_cast<const double*>(p.st)[i];
   __syncthreads();
   State* st = &sst;
-  if (!gate(st, mode)) return;
+  const int cur = gate_cur(st, mode);
+  if (cur < 0) return;
   const int nw = p.nw, nk = p.nk, psz = p.psz;
-  const int cur = st->cur;
   const int dst = dst_buf(mode, cur);
   const double rnk = (double)nk;
   const int tid = threadIdx.x;
@@ -902,10 +915,10 @@ __global__ void __launch_bounds__(1024) k_domega(const Prob* probs, const Item* 
   const Item it = items[blockIdx.x];
   const Prob& p = probs[it.prob];
   const State* st = p.st;
-  if (mode != MODE_BENCH && (st->done || st->hold)) return;
+  const int cur = gate_cur(st, (mode == MODE_BENCH ? MODE_BENCH : MODE_TRIAL));
+  if (cur < 0) return;
   const int nw = p.nw, nn = p.nn, nw2 = p.nw2;
   const int k = it.k;
-  const int cur = st->cur;
   const double* ln = p.ln[cur];
   const double* rave = p.rave[cur];
   const T2* Mk = mbuf<T2>(p, cur) + (size_t)k * nn * nw2;
@@ -974,7 +987,8 @@ __global__ void __launch_bounds__(domega_nt<NWM>()) k_domega_k(const Prob* probs
   const Item it = items[blockIdx.x];
   const Prob& p = probs[it.prob];
   const State* st = p.st;
-  if (mode != MODE_BENCH && (st->done || st->hold)) return;
+  const int cur = gate_cur(st, (mode == MODE_BENCH ? MODE_BENCH : MODE_TRIAL));
+  if (cur < 0) return;
   constexpr int NT = domega_nt<NWM>();
   constexpr int SL = (NWM * NWM + NT - 1) / NT;
   constexpr int CB = domega_chunk<NWM>();
@@ -982,7 +996,6 @@ __global__ void __launch_bounds__(domega_nt<NWM>()) k_domega_k(const Prob* probs
   const int ld = nw + 1;
   const int k = it.k;
   const int tid = threadIdx.x;
-  const int cur = st->cur;
   extern __shared__ double lds_domega[];
   double* qt = lds_domega;                    // [b][n] wb (Im ln M_nn + b.rave_n)
   cplx* rec = reinterpret_cast<cplx*>(qt + W90GPU_NUM_NNMAX * NWM);  // [b][n] 1/M_nn
@@ -1095,14 +1108,14 @@ __global__ void __launch_bounds__(1024) k_domega_w64(const Prob* probs, const It
   const Item it = items[blockIdx.x];
   const Prob& p = probs[it.prob];
   const State* st = p.st;
-  if (mode != MODE_BENCH && (st->done || st->hold)) return;
+  const int cur = gate_cur(st, (mode == MODE_BENCH ? MODE_BENCH : MODE_TRIAL));
+  if (cur < 0) return;
   constexpr int NWM = 64, NT = 1024, SL = NWM * NWM / NT, RS = NT / NWM, LD = NWM + 1;
   constexpr int AH = kDw64Ahead;
   static_assert(domega_nt<NWM>() == NT && domega_chunk<NWM>() == 1, "LDS layout of domega_lds<64>");
   const int nw = p.nw, nn = p.nn, nw2 = p.nw2;
   const int k = it.k;
   const int tid = threadIdx.x;
-  const int cur = st->cur;
   const int m = tid % NWM, n0 = tid / NWM;
   extern __shared__ double lds_domega[];
   double* qt = lds_domega;                                           // [b][n]
@@ -1380,10 +1393,10 @@ __global__ void __launch_bounds__(W90_NT) k_step(const Prob* probs, const Item* 
   const Item it = items[blockIdx.x];
   const Prob& p = probs[it.prob];
   const State* st = p.st;
-  if (!gate(st, mode)) return;
+  const int cur = gate_cur(st, mode);
+  if (cur < 0) return;
   const int nw = p.nw, nw2 = p.nw2;
   const int k = it.k;
-  const int cur = st->cur;
   __shared__ double red[W90_NT];
   cplx* Dk = p.D + (size_t)k * nw2;
   cplx* X = p.W1 + (size_t)k * nw2;
@@ -1460,12 +1473,12 @@ __global__ void __launch_bounds__(W90_NT) k_step_k(const Prob* probs, const Item
   const Item it = items[blockIdx.x];
   const Prob& p = probs[it.prob];
   const State* st = p.st;
-  if (!gate(st, mode)) return;
+  const int cur = gate_cur(st, mode);
+  if (cur < 0) return;
   const int nw = p.nw, nw2 = p.nw2;
   const int ld = nw + 1;
   const int k = it.k;
   const int tid = threadIdx.x;
-  const int cur = st->cur;
   __shared__ cplx X[NWM * (NWM + 1)];
   __shared__ cplx P[NWM * (NWM + 1)];
   __shared__ cplx T[NWM * (NWM + 1)];

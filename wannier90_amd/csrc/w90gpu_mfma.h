@@ -308,7 +308,8 @@ __global__ void __launch_bounds__(256, 2) k_rot_gemm(const Prob* probs, const It
   const Item it = items[blockIdx.x];
   const Prob& p = probs[it.prob];
   const State* st = p.st;
-  if (!gate(st, mode)) return;
+  const int cur = gate_cur(st, mode);
+  if (cur < 0) return;
   const int nw = p.nw, nn = p.nn;
   const int nt1 = (nw + GT - 1) / GT;
   const int ntl = nt1 * nt1;
@@ -319,7 +320,7 @@ __global__ void __launch_bounds__(256, 2) k_rot_gemm(const Prob* probs, const It
   __shared__ R As[2][W90_GK][GT + 1];  // [re/im][k][row]
   __shared__ R Bs[2][W90_GK][GT + 1];  // [re/im][k][col]
   __shared__ double red[256];
-  rot_gemm_tile<T2, STAGE, GT>(p, it.k, b, tile, row0, col0, mode, st->cur, As, Bs, red);
+  rot_gemm_tile<T2, STAGE, GT>(p, it.k, b, tile, row0, col0, mode, cur, As, Bs, red);
 }
 
 // K chunk of the second product of k_rot_fused (T held in LDS).
@@ -541,11 +542,12 @@ __global__ void __launch_bounds__(256, 2) k_rot_fused(const Prob* probs, const I
   const Item it = items[item];
   const Prob& p = probs[it.prob];
   const State* st = p.st;
-  if (!gate(st, mode)) return;
+  const int cur = gate_cur(st, mode);
+  if (cur < 0) return;
   if (b >= p.nn) return;
   __shared__ R lds[rot_fused_elems<R, GT>()];
   __shared__ double red[256];
-  rot_fused_tile<T2, GT>(p, it.k, b, mode, st->cur, lds, red);
+  rot_fused_tile<T2, GT>(p, it.k, b, mode, cur, lds, red);
 }
 
 // Omega partial sums of the current M by GEMM tile (MODE_INIT of the MFMA
@@ -556,7 +558,8 @@ __global__ void __launch_bounds__(256) k_reduce_tiles(const Prob* probs, const I
   const Item it = items[blockIdx.x];
   const Prob& p = probs[it.prob];
   const State* st = p.st;
-  if (!gate(st, mode)) return;
+  const int cur = gate_cur(st, mode);
+  if (cur < 0) return;
   const int nw = p.nw, nn = p.nn;
   const int nt1 = (nw + GT - 1) / GT;
   const int ntl = nt1 * nt1;
@@ -565,7 +568,6 @@ __global__ void __launch_bounds__(256) k_reduce_tiles(const Prob* probs, const I
   const int b = by / ntl, tile = by % ntl;
   const int row0 = (tile % nt1) * GT, col0 = (tile / nt1) * GT;
   const size_t kb = (size_t)it.k * nn + b;
-  const int cur = st->cur;
   const int dst = dst_buf(mode, cur);
   const T2* m = mbuf<T2>(p, cur) + kb * p.nw2;
   __shared__ double red[256];
@@ -671,11 +673,11 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
   const Item it = items[blockIdx.x];
   const Prob& p = probs[it.prob];
   const State* st = p.st;
-  if (!gate(st, mode)) return;
+  const int cur = gate_cur(st, mode);
+  if (cur < 0) return;
   const int nw = p.nw, nw2 = p.nw2;
   const int k = it.k;
   const int tid = threadIdx.x;
-  const int cur = st->cur;
   extern __shared__ double lds_step[];
   double* Xr = lds_step;
   double* Xi = Xr + W90_STEP_MAT;
@@ -909,7 +911,8 @@ __global__ void __launch_bounds__(64 * W90GPU_NUM_NNMAX) k_rotate_w16(const Prob
   const Item it = items[blockIdx.x];
   const Prob& p = probs[it.prob];
   const State* st = p.st;
-  if (!gate(st, mode)) return;
+  const int cur = gate_cur(st, mode);
+  if (cur < 0) return;
   const int nw = p.nw, nw2 = p.nw2, nn = p.nn;
   const int nnl = blockDim.x >> 6;
   const int k = it.k;
@@ -917,7 +920,6 @@ __global__ void __launch_bounds__(64 * W90GPU_NUM_NNMAX) k_rotate_w16(const Prob
   const int b = tid >> 6;
   const bool active = b < nn;
   const int lane = tid & 63, lr = lane & 15, lk = lane >> 4;
-  const int cur = st->cur;
   const bool rot = (mode != MODE_INIT);
   const int dst = dst_buf(mode, cur);
   extern __shared__ double lds_w16[];
