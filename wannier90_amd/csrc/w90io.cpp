@@ -251,15 +251,24 @@ std::vector<const char*> cut_ranges(const char* b, const char* e, int nt) {
   return cut;
 }
 
+// f(i) for every range i, one thread per range; ranges whose thread cannot
+// be started run on the calling thread (no exception leaves the C ABI).
 template <class F>
 void run_threads(int nt, F f) {
-  if (nt == 1) {
-    f(0);
-    return;
-  }
   std::vector<std::thread> th;
-  th.reserve(nt);
-  for (int i = 0; i < nt; ++i) th.emplace_back(f, i);
+  int started = 0;
+  if (nt > 1) {
+    try {
+      th.reserve(nt - 1);
+      for (int i = 1; i < nt; ++i) {
+        th.emplace_back(f, i);
+        ++started;
+      }
+    } catch (...) {
+    }
+  }
+  f(0);
+  for (int i = 1 + started; i < nt; ++i) f(i);
   for (auto& x : th) x.join();
 }
 
@@ -314,10 +323,8 @@ const char* read_sizes(const Mapped& f, long sz[3], bool* ok) {
   return le < e ? le + 1 : e;
 }
 
-}  // namespace
-
-extern "C" int w90gpu_read_mmn(const char* path, int num_bands, int num_kpts, int nntot, const int* nnlist,
-                               const int* nncell, double* m_matrix, int nthreads, char* errmsg, int errlen) {
+int read_mmn(const char* path, int num_bands, int num_kpts, int nntot, const int* nnlist, const int* nncell,
+             double* m_matrix, int nthreads, char* errmsg, int errlen) {
   set_msg(errmsg, errlen, "");
   if (!path || !nnlist || !nncell || !m_matrix || num_bands <= 0 || num_kpts <= 0 || nntot <= 0) {
     set_msg(errmsg, errlen, "w90gpu_read_mmn: invalid arguments");
@@ -490,8 +497,8 @@ extern "C" int w90gpu_read_mmn(const char* path, int num_bands, int num_kpts, in
   return W90GPU_OK;
 }
 
-extern "C" int w90gpu_read_amn(const char* path, int num_bands, int num_kpts, int num_proj, double* a_matrix,
-                               int nthreads, char* errmsg, int errlen) {
+int read_amn(const char* path, int num_bands, int num_kpts, int num_proj, double* a_matrix, int nthreads,
+             char* errmsg, int errlen) {
   set_msg(errmsg, errlen, "");
   if (!path || !a_matrix || num_bands <= 0 || num_kpts <= 0 || num_proj <= 0) {
     set_msg(errmsg, errlen, "w90gpu_read_amn: invalid arguments");
@@ -587,4 +594,28 @@ extern "C" int w90gpu_read_amn(const char* path, int num_bands, int num_kpts, in
     a_matrix[2 * dst[r] + 1] = val[2 * r + 1];
   }
   return W90GPU_OK;
+}
+
+}  // namespace
+
+// The entry points: no exception leaves the C ABI (an allocation that
+// fails is W90GPU_ERR_ALLOC).
+extern "C" int w90gpu_read_mmn(const char* path, int num_bands, int num_kpts, int nntot, const int* nnlist,
+                               const int* nncell, double* m_matrix, int nthreads, char* errmsg, int errlen) {
+  try {
+    return read_mmn(path, num_bands, num_kpts, nntot, nnlist, nncell, m_matrix, nthreads, errmsg, errlen);
+  } catch (...) {
+    set_msg(errmsg, errlen, "w90gpu_read_mmn: out of host memory");
+    return W90GPU_ERR_ALLOC;
+  }
+}
+
+extern "C" int w90gpu_read_amn(const char* path, int num_bands, int num_kpts, int num_proj, double* a_matrix,
+                               int nthreads, char* errmsg, int errlen) {
+  try {
+    return read_amn(path, num_bands, num_kpts, num_proj, a_matrix, nthreads, errmsg, errlen);
+  } catch (...) {
+    set_msg(errmsg, errlen, "w90gpu_read_amn: out of host memory");
+    return W90GPU_ERR_ALLOC;
+  }
 }
