@@ -164,6 +164,10 @@ def initial_state(prob):
 
 # ---------------------------------------------------------------- configs
 
+# The three single-problem workloads: 1 = sc a = 5 Ang, 4 bands, 4^3 mesh
+# (64 k, the small CPU-sized case); 2 = fcc a = 5.43 Ang, 16 bands, 8^3 mesh
+# (512 k, the benchmark problem); 3 = triclinic, 64 bands, 10x10x8 mesh
+# (800 k, nntot 12, the MFMA-bound case).
 CONFIGS = {
     1: dict(kind="sc", a=5.0, nw=4, mp=(4, 4, 4), decay=1.5),
     2: dict(kind="fcc", a=5.43, nw=16, mp=(8, 8, 8), decay=2.0),
