@@ -774,20 +774,18 @@ __global__ void __launch_bounds__(W90_FINAL_NT) k_omega_final(const Prob* probs,
   if (blockIdx.x >= (unsigned)np) return;
   const Prob& p = probs[plist[blockIdx.x]];
   // the state is worked on in LDS: the control logic below is a chain of
-  // dependent reads and writes that would each pay a global-memory latency
+  // dependent reads and writes that would each pay a global-memory latency.
+  // Its words are requested first and stored to LDS after the row sums
+  // below have issued their loads, so the two rounds of loads overlap (the
+  // sums of a problem that the gate then skips are not used).
   __shared__ State sst;
   static_assert(sizeof(State) % sizeof(double) == 0, "State is copied as doubles");
   constexpr int NSD = sizeof(State) / sizeof(double);
-  for (int i = threadIdx.x; i < NSD; i += blockDim.x)
-    reinterpret_cast<double*>(&sst)[i] = reinterpret_cast<const double*>(p.st)[i];
-  __syncthreads();
-  State* st = &sst;
-  const int cur = gate_cur(st, mode);
-  if (cur < 0) return;
-  const int nw = p.nw, nk = p.nk, psz = p.psz;
-  const int dst = dst_buf(mode, cur);
-  const double rnk = (double)nk;
+  static_assert(NSD <= W90_FINAL_NT, "one state word per thread");
   const int tid = threadIdx.x;
+  const double sword = tid < NSD ? reinterpret_cast<const double*>(p.st)[tid] : 0.0;
+  const int nw = p.nw, nk = p.nk, psz = p.psz;
+  const double rnk = (double)nk;
   __shared__ double red[W90_FINAL_NT];
   __shared__ double sums[8 * W90GPU_MAX_WANN + 8];
   __shared__ double sc[8];
@@ -833,6 +831,12 @@ __global__ void __launch_bounds__(W90_FINAL_NT) k_omega_final(const Prob* probs,
     }
     __syncthreads();
   }
+  if (tid < NSD) reinterpret_cast<double*>(&sst)[tid] = sword;
+  __syncthreads();
+  State* st = &sst;
+  const int cur = gate_cur(st, mode);
+  if (cur < 0) return;
+  const int dst = dst_buf(mode, cur);
 
   double omd_n = 0.0;
   for (int n = tid; n < nw; n += blockDim.x) {
