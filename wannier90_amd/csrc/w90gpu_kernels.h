@@ -127,6 +127,28 @@ __device__ double block_sum(double v, double* red) {
   return r;
 }
 
+// Two block sums in one pass (red holds 2 blockDim / 64 doubles): each value
+// is summed in block_sum's order, with two barriers for both.
+__device__ void block_sum2(double& a, double& b, double* red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nwv = blockDim.x >> 6;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if (lane == 0) {
+    red[w] = a;
+    red[nwv + w] = b;
+  }
+  __syncthreads();
+  double ra = 0.0, rb = 0.0;
+  for (int i = 0; i < nwv; ++i) {
+    ra += red[i];
+    rb += red[nwv + i];
+  }
+  __syncthreads();
+  a = ra;
+  b = rb;
+}
+
 // Block-wide maximum (blockDim a multiple of 64), as block_sum.
 __device__ double block_max(double v, double* red) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -948,8 +970,7 @@ __global__ void __launch_bounds__(1024) k_domega(const Prob* probs, const Item* 
     const cplx dold = p.D[(size_t)k * nw2 + e];
     gd += g.x * dold.x + g.y * dold.y;
   }
-  gg = block_sum(gg, red);
-  gd = block_sum(gd, red);
+  block_sum2(gg, gd, red);
   if (threadIdx.x == 0) {
     p.part_gg[k] = gg;
     p.part_gd[k] = gd;
@@ -1087,8 +1108,7 @@ __global__ void __launch_bounds__(domega_nt<NWM>()) k_domega_k(const Prob* probs
       gd += g.x * dold.x + g.y * dold.y;
     }
   }
-  gg = block_sum(gg, red);
-  gd = block_sum(gd, red);
+  block_sum2(gg, gd, red);
   if (tid == 0) {
     p.part_gg[k] = gg;
     p.part_gd[k] = gd;
@@ -1205,8 +1225,7 @@ __global__ void __launch_bounds__(1024) k_domega_w64(const Prob* probs, const It
       gd += g.x * dold.x + g.y * dold.y;
     }
   }
-  gg = block_sum(gg, red);
-  gd = block_sum(gd, red);
+  block_sum2(gg, gd, red);
   if (tid == 0) {
     p.part_gg[k] = gg;
     p.part_gd[k] = gd;
@@ -1301,8 +1320,7 @@ __device__ void search_in_step(const Prob& p, int k, State* lst, double* red) {
   }
   for (int i = threadIdx.x; i < NSD; i += blockDim.x)
     reinterpret_cast<double*>(lst)[i] = reinterpret_cast<const double*>(p.st)[i];
-  gg = block_sum(gg, red);  // (its barriers also order the copy above)
-  gd = block_sum(gd, red);
+  block_sum2(gg, gd, red);  // (its barriers also order the copy above)
   if (threadIdx.x == 0) {
     search_control(lst, gg, gd);
     if (k == 0) {

@@ -291,8 +291,7 @@ __device__ __forceinline__ void rot_gemm_tile(const Prob& p, int k, int b, int t
         }
       }
   if (STAGE == 2) {
-    od = block_sum(od, red);
-    all = block_sum(all, red);
+    block_sum2(od, all, red);
     if (tid == 0) {
       p.part_od[kb * p.npan + tile] = od;
       p.part_all[kb * p.npan + tile] = all;
@@ -515,8 +514,7 @@ __device__ __forceinline__ void rot_fused_tile(const Prob& p, int k, int b, int 
           }
         }
       }
-  od = block_sum(od, red);
-  all = block_sum(all, red);
+  block_sum2(od, all, red);
   if (tid == 0) {
     pod[0] = od;
     pall[0] = all;
@@ -585,8 +583,7 @@ __global__ void __launch_bounds__(256) k_reduce_tiles(const Prob* probs, const I
       p.d2[kb * nw + r] = a2;
     }
   }
-  od = block_sum(od, red);
-  all = block_sum(all, red);
+  block_sum2(od, all, red);
   if (threadIdx.x == 0) {
     p.part_od[kb * p.npan + tile] = od;
     p.part_all[kb * p.npan + tile] = all;
