@@ -573,11 +573,12 @@ __global__ void __launch_bounds__(W90_NT) k_kpart(const Prob* probs, const Item*
   }
 }
 
-// C = A B of n x n column-major complex matrices in LDS with leading
-// dimension n + 1 (n <= NWM, NWM a multiple of 16), on FP64 MFMA tiles;
-// defined with the other matrix-core code.
+// C = A B (with ident: I + scale A B) of n x n column-major complex
+// matrices in LDS with leading dimension n + 1 (n <= NWM, NWM a multiple of
+// 16), on FP64 MFMA tiles; defined with the other matrix-core code.
 template <int NWM>
-__device__ void lds_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n);
+__device__ void lds_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n, double scale = 1.0,
+                              bool ident = false);
 
 // ---------------------------------------------------------- omega / control
 
@@ -1588,19 +1589,18 @@ __global__ void __launch_bounds__(W90_NT) k_step_k(const Prob* probs, const Item
     P[q] = v;
   }
   __syncthreads();
+  // Horner: P <- I + X P / j, the scale and the identity applied as each
+  // product is stored; P and T take turns holding the polynomial
+  cplx* pc = P;
+  cplx* pn = T;
   for (int j = m - 1; j >= 1; --j) {
-    lds_gemm_mfma<NWM>(T, X, P, nw);
-    const double invj = 1.0 / (double)j;
-    for (int e = tid; e < nw2; e += W90_NT) {
-      const int q = (e % nw) + (e / nw) * ld;
-      cplx v = c_scale(T[q], invj);
-      if (e % nw == e / nw) v.x += 1.0;
-      P[q] = v;
-    }
-    __syncthreads();
+    lds_gemm_mfma<NWM>(pn, X, pc, nw, 1.0 / (double)j, true);
+    cplx* t = pc;
+    pc = pn;
+    pn = t;
   }
-  cplx* a = P;
-  cplx* b = T;
+  cplx* a = pc;
+  cplx* b = pn;
   for (int s = 0; s < sq; ++s) {
     lds_gemm_mfma<NWM>(b, a, a, nw);
     cplx* t = a;

@@ -96,10 +96,11 @@ struct Real<float2> {
 // (NWM/16)^2 tiles of 16 x 16; each (tile, real or imaginary part) is one
 // job of a wavefront: K in steps of 4 on FP64 MFMA 16x16x4, operand
 // fragments read from LDS (zero outside n), the result part written to its
-// half of the complex elements of C.  C must not overlap A or B; the
-// workgroup is synchronised before return.
+// half of the complex elements of C.  With ident, C = I + scale A B (one
+// Taylor step, applied as the product is stored).  C must not overlap A or
+// B; the workgroup is synchronised before return.
 template <int NWM>
-__device__ void lds_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n) {
+__device__ void lds_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n, double scale, bool ident) {
   static_assert(NWM % 16 == 0, "whole 16 x 16 tiles");
   constexpr int TN = NWM / 16;
   constexpr int JOBS = 2 * TN * TN;
@@ -146,7 +147,14 @@ __device__ void lds_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int r = 16 * ti + Mfma<double>::row(lk, q), c = 16 * tj + lr;
-      if (r < n && c < n) Cd[2 * (r + c * ld) + part] = acc[q];
+      if (r < n && c < n) {
+        double v = acc[q];
+        if (ident) {
+          v = v * scale;
+          if (part == 0 && r == c) v += 1.0;
+        }
+        Cd[2 * (r + c * ld) + part] = v;
+      }
     }
   }
   __syncthreads();
