@@ -1402,8 +1402,10 @@ __global__ void __launch_bounds__(W90_NT) k_delta_u(const Prob* probs, const Ite
   wg_gemm_cn(p.E + o, p.U0 + o, p.U[st->cur] + o, p.nw);
 }
 
-// C = A * B on FP64 MFMA for matrices in global memory (w90gpu_mfma.h).
-__device__ void wg_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n);
+// C = A * B (with ident: I + scale A B) on FP64 MFMA for matrices in global
+// memory (w90gpu_mfma.h).
+__device__ void wg_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n, double scale = 1.0,
+                             bool ident = false);
 
 // Retraction of one (problem, k) with workspaces in global memory (num_wann
 // above the LDS kernels): forms the search direction D (trial step only), X = scale * D,
@@ -1464,17 +1466,17 @@ __global__ void __launch_bounds__(W90_NT) k_step(const Prob* probs, const Item* 
     P[e] = v;
   }
   __syncthreads();
+  // Horner: P <- I + X P / j, the scale and the identity applied as each
+  // product is stored; P and E take turns holding the polynomial
+  cplx* pc = P;
+  cplx* pn = Ek;
   for (int j = m - 1; j >= 1; --j) {
-    wg_gemm_mfma(Ek, X, P, nw);
-    const double invj = 1.0 / (double)j;
-    for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
-      cplx v = c_scale(Ek[e], invj);
-      if (e % nw == e / nw) v.x += 1.0;
-      P[e] = v;
-    }
-    __syncthreads();
+    wg_gemm_mfma(pn, X, pc, nw, 1.0 / (double)j, true);
+    cplx* t = pc;
+    pc = pn;
+    pn = t;
   }
-  cplx* a = P;
+  cplx* a = pc;
   cplx* b = X;
   for (int s = 0; s < sq; ++s) {
     wg_gemm_mfma(b, a, a, nw);

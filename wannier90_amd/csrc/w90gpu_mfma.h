@@ -844,8 +844,10 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
 // a time (independent accumulator chains), on FP64 MFMA with the operand
 // fragments loaded straight from memory (the matrices of one k stay in the
 // caches: A fragments read along a column, B fragments reuse cache lines
-// over consecutive K steps).  Used by the retraction above num_wann 64.
-__device__ void wg_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n) {
+// over consecutive K steps).  With ident, C = I + scale A B (one Taylor
+// step, applied as the product is stored).  Used by the retraction above
+// num_wann 64.
+__device__ void wg_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n, double scale, bool ident) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nwv = blockDim.x >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int nt = (n + 15) / 16;
@@ -877,7 +879,14 @@ __device__ void wg_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = i0[u] + Mfma<double>::row(lk, i), c = j0[u] + lr;
-        if (r < n && c < n) C[r + (size_t)c * n] = c_make(cc[u].re(i), cc[u].im(i));
+        if (r < n && c < n) {
+          cplx v = c_make(cc[u].re(i), cc[u].im(i));
+          if (ident) {
+            v = c_scale(v, scale);
+            if (r == c) v.x += 1.0;
+          }
+          C[r + (size_t)c * n] = v;
+        }
       }
     }
   }
