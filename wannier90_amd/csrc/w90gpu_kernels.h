@@ -1537,6 +1537,9 @@ __global__ void __launch_bounds__(W90_NT) k_step_k(const Prob* probs, const Item
     ss = &lst;
   }
   const double scale = ss->scale;
+  // X, and the moduli of its elements for the 1-norm (into T, free until
+  // the Taylor products)
+  double* absx = reinterpret_cast<double*>(T);
   if (mode == MODE_TRIAL) {
     const double beta = ss->beta, sign = ss->dir_sign;
 #pragma unroll
@@ -1545,23 +1548,23 @@ __global__ void __launch_bounds__(W90_NT) k_step_k(const Prob* probs, const Item
       if (e < nw2) {
         cplx d = c_scale(c_add(gpre[i], c_scale(dpre[i], beta)), sign);
         st_stream(&Dk[e], d);
-        X[(e % nw) + (e / nw) * ld] = c_scale(d, scale);
+        const int q = (e % nw) + (e / nw) * ld;
+        const cplx x = c_scale(d, scale);
+        X[q] = x;
+        absx[q] = sqrt(c_abs2(x));
       }
     }
   } else {
 #pragma unroll
     for (int i = 0; i < SL; ++i) {
       const int e = tid + i * W90_NT;
-      if (e < nw2) X[(e % nw) + (e / nw) * ld] = c_scale(dpre[i], scale);
+      if (e < nw2) {
+        const int q = (e % nw) + (e / nw) * ld;
+        const cplx x = c_scale(dpre[i], scale);
+        X[q] = x;
+        absx[q] = sqrt(c_abs2(x));
+      }
     }
-  }
-  __syncthreads();
-  // 1-norm of X: the moduli in parallel (into T, free until the Taylor
-  // products), then each column summed in row order
-  double* absx = reinterpret_cast<double*>(T);
-  for (int e = tid; e < nw2; e += W90_NT) {
-    const int q = (e % nw) + (e / nw) * ld;
-    absx[q] = sqrt(c_abs2(X[q]));
   }
   __syncthreads();
   // column j < nw <= 64 is summed by lane j of wavefront 0, whose maximum

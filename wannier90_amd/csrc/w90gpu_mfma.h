@@ -722,7 +722,10 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
     ss = &lst;
   }
   const double scale = ss->scale;
-  // X = scale D (zero outside nw), D formed on the trial step
+  // X = scale D (zero outside nw), D formed on the trial step, and the
+  // moduli of X for its 1-norm (into the P slot, free until the Taylor
+  // polynomial); each column is then summed in row order by lane c of
+  // wavefront 0, the maximum by lane exchange
 #pragma unroll
   for (int i = 0; i < SL; ++i) {
     const int e = tid + 256 * i;
@@ -738,15 +741,7 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
     }
     Xr[c * W90_STEP_LD + r] = x.x;
     Xi[c * W90_STEP_LD + r] = x.y;
-  }
-  __syncthreads();
-  // 1-norm of X: the moduli in parallel (into the P slot, free until the
-  // Taylor polynomial), each column summed in row order by lane c of
-  // wavefront 0, the maximum by lane exchange
-  for (int e = tid; e < W90_STEP_N * W90_STEP_N; e += 256) {
-    const int r = e % W90_STEP_N, c = e / W90_STEP_N;
-    const int q = c * W90_STEP_LD + r;
-    Pr[q] = sqrt(c_abs2(c_make(Xr[q], Xi[q])));
+    Pr[c * W90_STEP_LD + r] = sqrt(c_abs2(x));
   }
   __syncthreads();
   double cs = 0.0;
