@@ -1,252 +1,308 @@
-!> C interface of the wannierisation library with the names and signatures
-!> of wannier90's src/c_interface.F90 (module w90_library_c), so that C and
-!> C++ callers written against src/wannier90.hh link unchanged.  Strings
-!> are F2018 assumed-length character dummies (CFI descriptors from C).
-!> Added here: cproject_overlap (w90_project_overlap, which the reference
-!> does not export to C) and cget_omega (Omega decomposition).
+!> C entry points of the host library.  The exported names and C
+!> signatures are those of wannier90's C interface (src/c_interface.F90,
+!> prototypes in src/wannier90.hh), declared for this build in
+!> include/w90_library_c.h, so a C or C++ caller written against the
+!> reference links unchanged.  Every entry point turns the opaque handle
+!> into the library state (state_of), views caller memory through
+!> c_f_pointer with the shape the state knows (the caller's array is
+!> associated, never copied), and forwards to module w90_library.  The
+!> steps (reading the input, projection, disentanglement, minimisation) go
+!> through one dispatcher that fetches the Fortran output units.
+!>
+!> Strings (keywords, seedname) are F2018 assumed-length character dummies:
+!> the C side passes CFI descriptors.  Added here: cproject_overlap and
+!> cget_omega, which the reference does not export to C.
 module w90_library_c
   use iso_c_binding
   use w90_library
   implicit none
-  public
+  private
+
+  public :: w90_create, w90_delete
+  public :: cinput_setopt, cinput_reader
+  public :: cset_eigval, cset_m_local, cset_u_matrix, cset_u_opt
+  public :: cget_nn, cget_nnkp
+  public :: cdisentangle, cproject_overlap, cwannierise
+  public :: cget_centres, cget_spreads, cget_omega
+  public :: cset_option_int, cset_option_int3, cset_option_intx
+  public :: cset_option_float, cset_option_float33, cset_option_floatxy
+
+  ! the steps of a calculation, run by do_step
+  enum, bind(c)
+    enumerator :: read_input = 1, project, disentangle, wannierise
+  end enum
+
 contains
+
+  ! ------------------------------------------------------------- helpers
+
+  !> The library state behind a handle made by w90_create.
+  function state_of(handle) result(lib)
+    type(c_ptr), intent(in) :: handle
+    type(lib_common_type), pointer :: lib
+    call c_f_pointer(handle, lib)
+  end function state_of
+
+  !> Run one step on the state behind `handle`; its status goes to ierr.
+  subroutine do_step(handle, which, ierr)
+    type(c_ptr), intent(in) :: handle
+    integer, intent(in) :: which
+    integer(c_int), intent(out) :: ierr
+    type(lib_common_type), pointer :: lib
+    integer :: unit_out, unit_err, status
+    lib => state_of(handle)
+    call w90_get_fortran_stdout(unit_out)
+    call w90_get_fortran_stderr(unit_err)
+    status = 0
+    select case (which)
+    case (read_input)
+      call w90_input_reader(lib, unit_out, unit_err, status)
+    case (project)
+      call w90_project_overlap(lib, unit_out, unit_err, status)
+    case (disentangle)
+      call w90_disentangle(lib, unit_out, unit_err, status)
+    case (wannierise)
+      call w90_wannierise(lib, unit_out, unit_err, status)
+    end select
+    ierr = int(status, c_int)
+  end subroutine do_step
+
+  !> The k-mesh tables exist once the options are applied; the array
+  !> setters and getters that need nntot build them on first use.
+  subroutine need_kmesh(lib)
+    type(lib_common_type), intent(inout) :: lib
+    integer :: unit_out, unit_err, status
+    if (lib%setup_complete) return
+    call w90_get_fortran_stdout(unit_out)
+    call w90_get_fortran_stderr(unit_err)
+    call w90_create_kmesh(lib, unit_out, unit_err, status)
+  end subroutine need_kmesh
+
+  ! ----------------------------------------------------- state and steps
+
   type(c_ptr) function w90_create() bind(c)
-    type(lib_common_type), pointer :: common_data
-    allocate (common_data)
-    w90_create = c_loc(common_data)
+    type(lib_common_type), pointer :: lib
+    allocate (lib)
+    w90_create = c_loc(lib)
   end function w90_create
 
-  subroutine w90_delete(common_cptr) bind(c)
-    type(c_ptr), value :: common_cptr
-    type(lib_common_type), pointer :: common_fptr
-    call c_f_pointer(common_cptr, common_fptr)
-    deallocate (common_fptr)
+  subroutine w90_delete(handle) bind(c)
+    type(c_ptr), value :: handle
+    type(lib_common_type), pointer :: lib
+    lib => state_of(handle)
+    if (associated(lib)) deallocate (lib)
   end subroutine w90_delete
 
-  subroutine cdisentangle(common_cptr, ierr) bind(c)
-    type(c_ptr), value :: common_cptr
-    integer(kind=c_int) :: ierr
-    type(lib_common_type), pointer :: common_fptr
-    integer :: istdout, istderr, ie
-    call w90_get_fortran_stderr(istderr)
-    call w90_get_fortran_stdout(istdout)
-    call c_f_pointer(common_cptr, common_fptr)
-    call w90_disentangle(common_fptr, istdout, istderr, ie)
-    ierr = ie
-  end subroutine cdisentangle
-
-  subroutine cproject_overlap(common_cptr, ierr) bind(c)
-    type(c_ptr), value :: common_cptr
-    integer(kind=c_int) :: ierr
-    type(lib_common_type), pointer :: common_fptr
-    integer :: istdout, istderr, ie
-    call w90_get_fortran_stderr(istderr)
-    call w90_get_fortran_stdout(istdout)
-    call c_f_pointer(common_cptr, common_fptr)
-    call w90_project_overlap(common_fptr, istdout, istderr, ie)
-    ierr = ie
-  end subroutine cproject_overlap
-
-  subroutine cwannierise(common_cptr, ierr) bind(c)
-    type(c_ptr), value :: common_cptr
-    integer(kind=c_int) :: ierr
-    type(lib_common_type), pointer :: common_fptr
-    integer :: istdout, istderr, ie
-    call w90_get_fortran_stderr(istderr)
-    call w90_get_fortran_stdout(istdout)
-    call c_f_pointer(common_cptr, common_fptr)
-    call w90_wannierise(common_fptr, istdout, istderr, ie)
-    ierr = ie
-  end subroutine cwannierise
-
-  subroutine cinput_setopt(common_cptr, seedname, ierr, comm) bind(c)
-    type(c_ptr), value :: common_cptr
+  subroutine cinput_setopt(handle, seedname, ierr, comm) bind(c)
+    type(c_ptr), value :: handle
     character(*, kind=c_char) :: seedname
-    integer(kind=c_int) :: ierr
-    integer(kind=c_int), value :: comm
-    type(lib_common_type), pointer :: common_fptr
-    integer :: istdout, istderr, ie
-    call w90_get_fortran_stderr(istderr)
-    call w90_get_fortran_stdout(istdout)
-    call c_f_pointer(common_cptr, common_fptr)
-    call w90_input_setopt(common_fptr, seedname, int(comm), istdout, istderr, ie)
-    ierr = ie
+    integer(c_int) :: ierr
+    integer(c_int), value :: comm
+    type(lib_common_type), pointer :: lib
+    integer :: unit_out, unit_err, status
+    lib => state_of(handle)
+    call w90_get_fortran_stdout(unit_out)
+    call w90_get_fortran_stderr(unit_err)
+    call w90_input_setopt(lib, seedname, int(comm), unit_out, unit_err, status)
+    ierr = int(status, c_int)
   end subroutine cinput_setopt
 
-  subroutine cinput_reader(common_cptr, ierr) bind(c)
-    type(c_ptr), value :: common_cptr
-    integer(kind=c_int) :: ierr
-    type(lib_common_type), pointer :: common_fptr
-    integer :: istdout, istderr, ie
-    call w90_get_fortran_stderr(istderr)
-    call w90_get_fortran_stdout(istdout)
-    call c_f_pointer(common_cptr, common_fptr)
-    call w90_input_reader(common_fptr, istdout, istderr, ie)
-    ierr = ie
+  subroutine cinput_reader(handle, ierr) bind(c)
+    type(c_ptr), value :: handle
+    integer(c_int) :: ierr
+    call do_step(handle, read_input, ierr)
   end subroutine cinput_reader
 
-  subroutine cset_eigval(common_cptr, eigval_cptr) bind(c)
-    type(c_ptr), value :: common_cptr, eigval_cptr
-    real(8), pointer :: eigval_fptr(:, :)
-    type(lib_common_type), pointer :: common_fptr
-    call c_f_pointer(common_cptr, common_fptr)
-    call c_f_pointer(eigval_cptr, eigval_fptr, [common_fptr%num_bands, common_fptr%num_kpts])
-    call w90_set_eigval(common_fptr, eigval_fptr)
+  subroutine cproject_overlap(handle, ierr) bind(c)
+    type(c_ptr), value :: handle
+    integer(c_int) :: ierr
+    call do_step(handle, project, ierr)
+  end subroutine cproject_overlap
+
+  subroutine cdisentangle(handle, ierr) bind(c)
+    type(c_ptr), value :: handle
+    integer(c_int) :: ierr
+    call do_step(handle, disentangle, ierr)
+  end subroutine cdisentangle
+
+  subroutine cwannierise(handle, ierr) bind(c)
+    type(c_ptr), value :: handle
+    integer(c_int) :: ierr
+    call do_step(handle, wannierise, ierr)
+  end subroutine cwannierise
+
+  ! ------------------------------------------------ caller-owned arrays
+
+  !> eigval(num_bands, num_kpts)
+  subroutine cset_eigval(handle, data) bind(c)
+    type(c_ptr), value :: handle, data
+    type(lib_common_type), pointer :: lib
+    real(c_double), pointer :: eig(:, :)
+    lib => state_of(handle)
+    call c_f_pointer(data, eig, [lib%num_bands, lib%num_kpts])
+    call w90_set_eigval(lib, eig)
   end subroutine cset_eigval
 
-  subroutine cset_m_local(common_cptr, m_cptr) bind(c)
-    type(c_ptr), value :: common_cptr, m_cptr
-    complex(8), pointer :: m_fptr(:, :, :, :)
-    type(lib_common_type), pointer :: common_fptr
-    integer :: istdout, istderr, ie
-    call c_f_pointer(common_cptr, common_fptr)
-    if (.not. common_fptr%setup_complete) then
-      call w90_get_fortran_stderr(istderr)
-      call w90_get_fortran_stdout(istdout)
-      call w90_create_kmesh(common_fptr, istdout, istderr, ie)
-    end if
-    call c_f_pointer(m_cptr, m_fptr, [common_fptr%num_bands, common_fptr%num_bands, &
-                                      common_fptr%nntot, common_fptr%num_kpts])
-    call w90_set_m_local(common_fptr, m_fptr)
+  !> m(num_bands, num_bands, nntot, num_kpts)
+  subroutine cset_m_local(handle, data) bind(c)
+    type(c_ptr), value :: handle, data
+    type(lib_common_type), pointer :: lib
+    complex(c_double_complex), pointer :: overlaps(:, :, :, :)
+    lib => state_of(handle)
+    call need_kmesh(lib)
+    call c_f_pointer(data, overlaps, [lib%num_bands, lib%num_bands, lib%nntot, lib%num_kpts])
+    call w90_set_m_local(lib, overlaps)
   end subroutine cset_m_local
 
-  subroutine cset_u_matrix(common_cptr, a_cptr) bind(c)
-    type(c_ptr), value :: common_cptr, a_cptr
-    complex(8), pointer :: a_fptr(:, :, :)
-    type(lib_common_type), pointer :: common_fptr
-    call c_f_pointer(common_cptr, common_fptr)
-    call c_f_pointer(a_cptr, a_fptr, [common_fptr%num_wann, common_fptr%num_wann, common_fptr%num_kpts])
-    call w90_set_u_matrix(common_fptr, a_fptr)
+  !> u(num_wann, num_wann, num_kpts)
+  subroutine cset_u_matrix(handle, data) bind(c)
+    type(c_ptr), value :: handle, data
+    type(lib_common_type), pointer :: lib
+    complex(c_double_complex), pointer :: rot(:, :, :)
+    lib => state_of(handle)
+    call c_f_pointer(data, rot, [lib%num_wann, lib%num_wann, lib%num_kpts])
+    call w90_set_u_matrix(lib, rot)
   end subroutine cset_u_matrix
 
-  subroutine cset_u_opt(common_cptr, a_cptr) bind(c)
-    type(c_ptr), value :: common_cptr, a_cptr
-    complex(8), pointer :: a_fptr(:, :, :)
-    type(lib_common_type), pointer :: common_fptr
-    call c_f_pointer(common_cptr, common_fptr)
-    call c_f_pointer(a_cptr, a_fptr, [common_fptr%num_bands, common_fptr%num_wann, common_fptr%num_kpts])
-    call w90_set_u_opt(common_fptr, a_fptr)
+  !> u_opt(num_bands, num_wann, num_kpts)
+  subroutine cset_u_opt(handle, data) bind(c)
+    type(c_ptr), value :: handle, data
+    type(lib_common_type), pointer :: lib
+    complex(c_double_complex), pointer :: rot(:, :, :)
+    lib => state_of(handle)
+    call c_f_pointer(data, rot, [lib%num_bands, lib%num_wann, lib%num_kpts])
+    call w90_set_u_opt(lib, rot)
   end subroutine cset_u_opt
 
-  subroutine cget_nn(common_cptr, n) bind(c)
-    type(c_ptr), value :: common_cptr, n
-    type(lib_common_type), pointer :: common_fptr
-    integer(kind=c_int), pointer :: ndat
-    integer :: istdout, istderr, ierr, nn
-    call w90_get_fortran_stderr(istderr)
-    call w90_get_fortran_stdout(istdout)
-    call c_f_pointer(common_cptr, common_fptr)
-    call c_f_pointer(n, ndat)
-    call w90_get_nn(common_fptr, nn, istdout, istderr, ierr)
-    ndat = nn
+  ! ------------------------------------------------------------- k-mesh
+
+  subroutine cget_nn(handle, nntot) bind(c)
+    type(c_ptr), value :: handle
+    integer(c_int), intent(out) :: nntot
+    type(lib_common_type), pointer :: lib
+    integer :: unit_out, unit_err, status, n
+    lib => state_of(handle)
+    call w90_get_fortran_stdout(unit_out)
+    call w90_get_fortran_stderr(unit_err)
+    call w90_get_nn(lib, n, unit_out, unit_err, status)
+    nntot = int(n, c_int)
   end subroutine cget_nn
 
-  subroutine cget_nnkp(common_cptr, nnkp) bind(c)
-    type(c_ptr), value :: common_cptr, nnkp
-    type(lib_common_type), pointer :: common_fptr
-    integer(kind=c_int), pointer :: nfptr(:, :)
-    integer :: istdout, istderr, ierr
-    integer, allocatable :: tmp(:, :)
-    call w90_get_fortran_stderr(istderr)
-    call w90_get_fortran_stdout(istdout)
-    call c_f_pointer(common_cptr, common_fptr)
-    if (.not. common_fptr%setup_complete) call w90_create_kmesh(common_fptr, istdout, istderr, ierr)
-    call c_f_pointer(nnkp, nfptr, [common_fptr%num_kpts, common_fptr%nntot])
-    allocate (tmp(common_fptr%num_kpts, common_fptr%nntot))
-    call w90_get_nnkp(common_fptr, tmp, istdout, istderr, ierr)
-    nfptr = tmp
+  !> nnlist(num_kpts, nntot), 1-based
+  subroutine cget_nnkp(handle, data) bind(c)
+    type(c_ptr), value :: handle, data
+    type(lib_common_type), pointer :: lib
+    integer(c_int), pointer :: table(:, :)
+    integer, allocatable :: nnlist(:, :)
+    integer :: unit_out, unit_err, status
+    lib => state_of(handle)
+    call need_kmesh(lib)
+    call w90_get_fortran_stdout(unit_out)
+    call w90_get_fortran_stderr(unit_err)
+    allocate (nnlist(lib%num_kpts, lib%nntot))
+    call w90_get_nnkp(lib, nnlist, unit_out, unit_err, status)
+    call c_f_pointer(data, table, [lib%num_kpts, lib%nntot])
+    table = int(nnlist, c_int)
   end subroutine cget_nnkp
 
-  subroutine cget_spreads(common_cptr, spreads) bind(c)
-    type(c_ptr), value :: common_cptr, spreads
-    type(lib_common_type), pointer :: common_fptr
-    real(kind=8), pointer :: fspreads(:)
-    call c_f_pointer(common_cptr, common_fptr)
-    call c_f_pointer(spreads, fspreads, [common_fptr%num_wann])
-    call w90_get_spreads(common_fptr, fspreads)
-  end subroutine cget_spreads
+  ! ------------------------------------------------------------ results
 
-  subroutine cget_centres(common_cptr, centres) bind(c)
-    type(c_ptr), value :: common_cptr, centres
-    type(lib_common_type), pointer :: common_fptr
-    real(kind=8), pointer :: fcentres(:, :)
-    call c_f_pointer(common_cptr, common_fptr)
-    call c_f_pointer(centres, fcentres, [3, common_fptr%num_wann])
-    call w90_get_centres(common_fptr, fcentres)
+  !> centres(3, num_wann), Ang
+  subroutine cget_centres(handle, data) bind(c)
+    type(c_ptr), value :: handle, data
+    type(lib_common_type), pointer :: lib
+    real(c_double), pointer :: xyz(:, :)
+    lib => state_of(handle)
+    call c_f_pointer(data, xyz, [3, lib%num_wann])
+    call w90_get_centres(lib, xyz)
   end subroutine cget_centres
 
+  !> spreads(num_wann), Ang^2
+  subroutine cget_spreads(handle, data) bind(c)
+    type(c_ptr), value :: handle, data
+    type(lib_common_type), pointer :: lib
+    real(c_double), pointer :: sprd(:)
+    lib => state_of(handle)
+    call c_f_pointer(data, sprd, [lib%num_wann])
+    call w90_get_spreads(lib, sprd)
+  end subroutine cget_spreads
+
   !> omega(4) = Omega_I, Omega_D, Omega_OD, Omega (Ang^2)
-  subroutine cget_omega(common_cptr, omega) bind(c)
-    type(c_ptr), value :: common_cptr, omega
-    type(lib_common_type), pointer :: common_fptr
-    real(kind=8), pointer :: fomega(:)
-    call c_f_pointer(common_cptr, common_fptr)
-    call c_f_pointer(omega, fomega, [4])
-    call w90_get_omega(common_fptr, fomega)
+  subroutine cget_omega(handle, data) bind(c)
+    type(c_ptr), value :: handle, data
+    type(lib_common_type), pointer :: lib
+    real(c_double), pointer :: parts(:)
+    lib => state_of(handle)
+    call c_f_pointer(data, parts, [4])
+    call w90_get_omega(lib, parts)
   end subroutine cget_omega
 
-  subroutine cset_option_int(common_cptr, keyword, ival) bind(c)
-    type(c_ptr), value :: common_cptr
+  ! ------------------------------------------------------------ options
+
+  subroutine cset_option_int(handle, keyword, value) bind(c)
+    type(c_ptr), value :: handle
     character(*, kind=c_char) :: keyword
-    integer(kind=c_int), value :: ival
-    type(lib_common_type), pointer :: common_fptr
-    call c_f_pointer(common_cptr, common_fptr)
-    call w90_set_option(common_fptr, keyword, int(ival))
+    integer(c_int), value :: value
+    type(lib_common_type), pointer :: lib
+    lib => state_of(handle)
+    call w90_set_option(lib, keyword, int(value))
   end subroutine cset_option_int
 
-  subroutine cset_option_int3(common_cptr, keyword, ival) bind(c)
-    type(c_ptr), value :: common_cptr
-    type(c_ptr) :: ival
+  !> `where` holds the address of three ints (the caller passes int**)
+  subroutine cset_option_int3(handle, keyword, where) bind(c)
+    type(c_ptr), value :: handle
     character(*, kind=c_char) :: keyword
-    integer(kind=c_int), pointer :: m_fptr(:)
-    type(lib_common_type), pointer :: common_fptr
-    call c_f_pointer(common_cptr, common_fptr)
-    call c_f_pointer(ival, m_fptr, [3])
-    call w90_set_option(common_fptr, keyword, int(m_fptr))
+    type(c_ptr), intent(in) :: where
+    integer(c_int), pointer :: triple(:)
+    type(lib_common_type), pointer :: lib
+    lib => state_of(handle)
+    call c_f_pointer(where, triple, [3])
+    call w90_set_option(lib, keyword, int(triple))
   end subroutine cset_option_int3
 
-  subroutine cset_option_intx(common_cptr, keyword, ival, ict) bind(c)
-    type(c_ptr), value :: common_cptr
-    type(c_ptr), value :: ival
+  subroutine cset_option_intx(handle, keyword, values, count) bind(c)
+    type(c_ptr), value :: handle, values
     character(*, kind=c_char) :: keyword
-    integer(kind=c_int), value :: ict
-    integer(kind=c_int), pointer :: m_fptr(:)
-    type(lib_common_type), pointer :: common_fptr
-    call c_f_pointer(common_cptr, common_fptr)
-    call c_f_pointer(ival, m_fptr, [ict])
-    call w90_set_option(common_fptr, keyword, int(m_fptr))
+    integer(c_int), value :: count
+    integer(c_int), pointer :: list(:)
+    type(lib_common_type), pointer :: lib
+    lib => state_of(handle)
+    call c_f_pointer(values, list, [count])
+    call w90_set_option(lib, keyword, int(list))
   end subroutine cset_option_intx
 
-  subroutine cset_option_float(common_cptr, keyword, ival) bind(c)
-    type(c_ptr), value :: common_cptr
+  subroutine cset_option_float(handle, keyword, value) bind(c)
+    type(c_ptr), value :: handle
     character(*, kind=c_char) :: keyword
-    real(kind=c_double), value :: ival
-    type(lib_common_type), pointer :: common_fptr
-    call c_f_pointer(common_cptr, common_fptr)
-    call w90_set_option(common_fptr, keyword, real(ival, 8))
+    real(c_double), value :: value
+    type(lib_common_type), pointer :: lib
+    lib => state_of(handle)
+    call w90_set_option(lib, keyword, real(value, kind(1.0d0)))
   end subroutine cset_option_float
 
-  subroutine cset_option_float33(common_cptr, keyword, ival) bind(c)
-    type(c_ptr), value :: common_cptr
+  !> `where` holds the address of a C double[3][3] (row i = vector i),
+  !> which Fortran sees column by column as (3,3)
+  subroutine cset_option_float33(handle, keyword, where) bind(c)
+    type(c_ptr), value :: handle
     character(*, kind=c_char) :: keyword
-    type(c_ptr) :: ival
-    type(lib_common_type), pointer :: common_fptr
-    real(kind=8), pointer :: m_fptr(:, :)
-    call c_f_pointer(ival, m_fptr, [3, 3])
-    call c_f_pointer(common_cptr, common_fptr)
-    call w90_set_option(common_fptr, keyword, m_fptr)
+    type(c_ptr), intent(in) :: where
+    real(c_double), pointer :: block(:, :)
+    type(lib_common_type), pointer :: lib
+    lib => state_of(handle)
+    call c_f_pointer(where, block, [3, 3])
+    call w90_set_option(lib, keyword, block)
   end subroutine cset_option_float33
 
-  subroutine cset_option_floatxy(common_cptr, keyword, ival, x, y) bind(c)
-    type(c_ptr), value :: common_cptr
+  !> a C double[x][y]: y values per row, seen by Fortran as (y, x)
+  subroutine cset_option_floatxy(handle, keyword, values, x, y) bind(c)
+    type(c_ptr), value :: handle, values
     character(*, kind=c_char) :: keyword
-    type(c_ptr), value :: ival
-    integer(kind=c_int), value :: x, y
-    type(lib_common_type), pointer :: common_fptr
-    real(kind=8), pointer :: m_fptr(:, :)
-    call c_f_pointer(ival, m_fptr, [y, x])
-    call c_f_pointer(common_cptr, common_fptr)
-    call w90_set_option(common_fptr, keyword, m_fptr)
+    integer(c_int), value :: x, y
+    real(c_double), pointer :: rows(:, :)
+    type(lib_common_type), pointer :: lib
+    lib => state_of(handle)
+    call c_f_pointer(values, rows, [y, x])
+    call w90_set_option(lib, keyword, rows)
   end subroutine cset_option_floatxy
 
 end module w90_library_c

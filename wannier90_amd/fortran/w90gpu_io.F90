@@ -313,7 +313,7 @@ contains
   !> real block r(columns, rows): a r2d entry, or the numeric lines of a
   !> block whose optional first line gives the unit (bohr/ang); values in
   !> Bohr are converted to Angstrom when length_block is set
-  subroutine get_r2d(s, key, r, cols, rows, length_block, found, ierr)
+  subroutine get_r2d(s, key, r, cols, rows, length_block, found, ierr, bohr)
     type(w90_settings), intent(in) :: s
     character(len=*), intent(in) :: key
     integer, intent(in) :: cols, rows
@@ -321,6 +321,7 @@ contains
     logical, intent(in) :: length_block
     logical, intent(out) :: found
     integer, intent(inout) :: ierr
+    real(dp), intent(in), optional :: bohr  ! Angstrom per Bohr (default bohr_ang)
     integer :: i, j, first, ios
     real(dp) :: scale
     character(len=linelen) :: t
@@ -341,7 +342,10 @@ contains
         t = lower(adjustl(s%e(i)%lines(1)))
         if (t(1:4) == 'bohr') then
           first = 2
-          if (length_block) scale = bohr_ang
+          if (length_block) then
+            scale = bohr_ang
+            if (present(bohr)) scale = bohr
+          end if
         else if (t(1:3) == 'ang') then
           first = 2
         end if

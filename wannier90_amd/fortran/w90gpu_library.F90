@@ -17,7 +17,8 @@ module w90_library
   private
 
   public :: lib_common_type
-  public :: w90_get_fortran_stdout, w90_get_fortran_stderr
+  public :: w90_get_fortran_stdout, w90_get_fortran_stderr, w90_get_fortran_file
+  public :: w90_set_constant_bohr_to_ang, w90_get_proj
   public :: w90_set_option, w90_input_setopt, w90_input_reader
   public :: w90_create_kmesh, w90_get_nn, w90_get_nnkp, w90_get_gkpb
   public :: w90_set_m_local, w90_set_u_matrix, w90_set_u_opt, w90_set_eigval
@@ -42,6 +43,8 @@ module w90_library
     logical :: setup_complete = .false.
     real(dp), allocatable :: kpt_latt(:, :)
     real(dp) :: real_lattice(3, 3) = 0.0_dp
+    ! Angstrom per Bohr for inputs in Bohr (w90_set_constant_bohr_to_ang)
+    real(dp) :: bohr = bohr_ang
     ! kmesh_info_type / kmesh_input_type
     integer :: nntot = 0
     integer(c_int), allocatable :: nnlist(:, :)
@@ -103,6 +106,24 @@ contains
     integer, intent(out) :: istderr
     istderr = error_unit
   end subroutine w90_get_fortran_stderr
+
+  !> A new formatted unit on file `name` (w90_get_fortran_file,
+  !> src/library_interface.F90:154): callers route the library's output to
+  !> it in place of the standard units.
+  subroutine w90_get_fortran_file(output, name)
+    integer, intent(out) :: output
+    character(len=*), intent(in) :: name
+    open (newunit=output, file=name, form='formatted', status='unknown')
+  end subroutine w90_get_fortran_file
+
+  !> The length of the Bohr radius in Angstrom of the calling code
+  !> (w90_set_constant_bohr_to_ang, src/library_interface.F90:655): used
+  !> for blocks given in bohr by the options applied after this call.
+  subroutine w90_set_constant_bohr_to_ang(common_data, bohr_to_angstrom)
+    type(lib_common_type), intent(inout) :: common_data
+    real(dp), intent(in) :: bohr_to_angstrom
+    common_data%bohr = bohr_to_angstrom
+  end subroutine w90_set_constant_bohr_to_ang
 
   !> report an error like prterr (src/io.F90:395-452): message on stderr
   subroutine report(istdout, istderr, code, msg, ierr)
@@ -296,7 +317,7 @@ contains
       msg = 'Error: Did not find the kpoint grid (mp_grid) in input file'
       return
     end if
-    call get_r2d(cd%settings, 'unit_cell_cart', uc, 3, 3, .true., found, ierr)
+    call get_r2d(cd%settings, 'unit_cell_cart', uc, 3, 3, .true., found, ierr, cd%bohr)
     if (.not. found .or. ierr /= 0) then
       ierr = 5
       msg = 'Error: Did not find the cell information (unit_cell_cart) in the input file'
@@ -436,7 +457,7 @@ contains
         do nk = 1, n
           line = cd%settings%e(i)%lines(nk)
           if (lower(adjustl(line(1:4))) == 'bohr') then
-            rval = bohr_ang
+            rval = cd%bohr
             cycle
           end if
           if (lower(adjustl(line(1:3))) == 'ang') cycle
@@ -752,6 +773,366 @@ contains
       spreads = 0.0_dp
     end if
   end subroutine w90_get_spreads
+
+  !> The projections of the input (w90_get_proj,
+  !> src/library_interface.F90:902): n functions with their site(3,:) in
+  !> fractional coordinates, l, m (the mr index) and s, and where asked
+  !> the spin axis sqa, the axes z and x, the radial index rad and zona.
+  !> The projections block is expanded as the reference reads it
+  !> (src/readwrite.F90:3390-4076) for inputs without spinors: a site is an
+  !> atom label of the atoms block (every atom of it), f= fractional or c=
+  !> Cartesian (Ang, or Bohr when the block starts with bohr); the angular
+  !> part is a ;-list of names (s, p, pz, ..., sp3d2-6) or l=[,mr=]; the
+  !> options z=, x=, r=, zona= follow after further colons.  Per line the
+  !> functions go out by site, then l from -5 to 3, then mr.  Errors as
+  !> the reference's: no projections block, arrays too short.
+  subroutine w90_get_proj(common_data, n, site, l, m, s, istdout, istderr, ierr, sqa, z, x, rad, zona)
+    type(lib_common_type), intent(in) :: common_data
+    integer, intent(inout) :: n, l(:), m(:), s(:)
+    real(dp), intent(inout) :: site(:, :)
+    integer, intent(in) :: istdout, istderr
+    integer, intent(out) :: ierr
+    integer, intent(inout), optional :: rad(:)
+    real(dp), intent(inout), optional :: sqa(:, :), z(:, :), x(:, :), zona(:)
+    integer, allocatable :: pl(:), pm(:), prad(:)
+    real(dp), allocatable :: psite(:, :), pz(:, :), px(:, :), pzona(:)
+    character(len=256) :: msg
+    integer :: np, i
+    ierr = 0
+    call expand_projections(common_data, np, pl, pm, psite, pz, px, prad, pzona, ierr, msg)
+    if (ierr /= 0) then
+      call report(istdout, istderr, ierr, msg, ierr)
+      return
+    end if
+    n = np
+    if (size(l) < n .or. size(m) < n .or. size(s) < n .or. size(site, 2) < n) then
+      call report(istdout, istderr, 1, 'Array argument in get_proj() call is insufficiently sized', ierr)
+      return
+    end if
+    if (present(sqa)) then
+      if (size(sqa, 2) < n) ierr = 1
+    end if
+    if (present(z)) then
+      if (size(z, 2) < n) ierr = 1
+    end if
+    if (present(x)) then
+      if (size(x, 2) < n) ierr = 1
+    end if
+    if (present(rad)) then
+      if (size(rad) < n) ierr = 1
+    end if
+    if (present(zona)) then
+      if (size(zona) < n) ierr = 1
+    end if
+    if (ierr /= 0) then
+      call report(istdout, istderr, 1, 'Optional array argument in get_proj() call is insufficiently sized', ierr)
+      return
+    end if
+    do i = 1, n
+      l(i) = pl(i)
+      m(i) = pm(i)
+      s(i) = 0
+      site(:, i) = psite(:, i)
+      if (present(sqa)) sqa(:, i) = [0.0_dp, 0.0_dp, 1.0_dp]
+      if (present(z)) z(:, i) = pz(:, i)
+      if (present(x)) x(:, i) = px(:, i)
+      if (present(rad)) rad(i) = prad(i)
+      if (present(zona)) zona(i) = pzona(i)
+    end do
+  end subroutine w90_get_proj
+
+  !> The (l, first mr, last mr) of a projection name, or l = -99.
+  subroutine proj_name(name, pl, m0, m1)
+    character(len=*), intent(in) :: name
+    integer, intent(out) :: pl, m0, m1
+    character(len=12), parameter :: one(0:3, 7) = reshape([character(len=12) :: &
+      's', 'pz', 'dz2', 'fz3', &
+      '', 'px', 'dxz', 'fxz2', &
+      '', 'py', 'dyz', 'fyz2', &
+      '', '', 'dx2-y2', 'fxyz', &
+      '', '', 'dxy', 'fz(x2-y2)', &
+      '', '', '', 'fx(x2-3y2)', &
+      '', '', '', 'fy(3x2-y2)'], [4, 7])
+    character(len=5), parameter :: hyb(5) = [character(len=5) :: 'sp', 'sp2', 'sp3', 'sp3d', 'sp3d2']
+    character(len=1), parameter :: whole(0:3) = ['s', 'p', 'd', 'f']
+    integer :: i, j, dash, k, ios
+    pl = -99
+    do i = 0, 3
+      if (trim(name) == whole(i)) then
+        pl = i
+        m0 = 1
+        m1 = 2 * i + 1
+        return
+      end if
+      do j = 1, 7
+        if (len_trim(one(i, j)) > 0 .and. trim(name) == trim(one(i, j))) then
+          pl = i
+          m0 = j
+          m1 = j
+          return
+        end if
+      end do
+    end do
+    ! hybrids: sp..sp3d2 (l = -1..-5, mr 1..|l|+1), or one of them as name-k
+    dash = index(name, '-')
+    do i = 1, 5
+      if (dash == 0) then
+        if (trim(name) /= trim(hyb(i))) cycle
+        pl = -i
+        m0 = 1
+        m1 = i + 1
+        return
+      else if (name(1:dash - 1) == trim(hyb(i))) then
+        read (name(dash + 1:), *, iostat=ios) k
+        if (ios /= 0 .or. k < 1 .or. k > i + 1) return
+        pl = -i
+        m0 = k
+        m1 = k
+        return
+      end if
+    end do
+  end subroutine proj_name
+
+  !> The functions of the projections block (see w90_get_proj).
+  subroutine expand_projections(cd, np, pl, pm, psite, pz, px, prad, pzona, ierr, msg)
+    type(lib_common_type), intent(in) :: cd
+    integer, intent(out) :: np
+    integer, allocatable, intent(out) :: pl(:), pm(:), prad(:)
+    real(dp), allocatable, intent(out) :: psite(:, :), pz(:, :), px(:, :), pzona(:)
+    integer, intent(out) :: ierr
+    character(len=*), intent(out) :: msg
+    integer, parameter :: cap = 4096
+    character(len=512) :: line, field, part, rest
+    logical :: states(1:7, -5:3)
+    real(dp) :: inv(3, 3), frac(3), zax(3), xax(3), zn, scale
+    real(dp), allocatable :: sites(:, :)
+    integer :: ib, nl, j, c, nsite, ia, ll, mm, m0, m1, ios, rad, first, q
+    ierr = 0
+    msg = ''
+    np = 0
+    ib = settings_find(cd%settings, 'projections')
+    if (ib == 0) then
+      ierr = 1
+      msg = 'Projectors are not setup in Wannier90 library when requested via get_proj()'
+      return
+    end if
+    if (cd%settings%e(ib)%kind /= K_BLOCK) then
+      ierr = 1
+      msg = 'Projectors are not setup in Wannier90 library when requested via get_proj()'
+      return
+    end if
+    allocate (pl(cap), pm(cap), prad(cap), psite(3, cap), pz(3, cap), px(3, cap), pzona(cap))
+    inv = inverse3(cd%real_lattice)
+    nl = size(cd%settings%e(ib)%lines)
+    scale = 1.0_dp
+    first = 1
+    if (nl > 0) then
+      line = lower(adjustl(cd%settings%e(ib)%lines(1)))
+      if (line(1:4) == 'bohr') then
+        scale = cd%bohr
+        first = 2
+      else if (line(1:3) == 'ang') then
+        first = 2
+      else if (line(1:6) == 'random') then
+        ierr = 5
+        msg = 'Error: random projections are not supported by this build'
+        return
+      end if
+    end if
+    do j = first, nl
+      line = strip_spaces(lower(cd%settings%e(ib)%lines(j)))
+      if (len_trim(line) == 0) cycle
+      c = index(line, ':')
+      if (c == 0) then
+        ierr = 5
+        msg = 'Error: malformed projection definition: '//trim(line)
+        return
+      end if
+      field = line(:c - 1)
+      rest = line(c + 1:)
+      ! the site(s)
+      if (field(1:2) == 'f=' .or. field(1:2) == 'c=') then
+        read (field(3:), *, iostat=ios) frac
+        if (ios /= 0) then
+          ierr = 5
+          msg = 'Error: malformed projection site: '//trim(field)
+          return
+        end if
+        if (field(1:1) == 'c') frac = matmul(frac * scale, inv)
+        nsite = 1
+        allocate (sites(3, 1))
+        sites(:, 1) = frac
+      else
+        nsite = 0
+        do ia = 1, cd%num_atoms
+          if (lower(trim(cd%atom_symbol(ia))) == trim(field)) nsite = nsite + 1
+        end do
+        if (nsite == 0) then
+          ierr = 5
+          msg = 'Error: Atom site not recognised '//trim(field)
+          return
+        end if
+        allocate (sites(3, nsite))
+        nsite = 0
+        do ia = 1, cd%num_atoms
+          if (lower(trim(cd%atom_symbol(ia))) /= trim(field)) cycle
+          nsite = nsite + 1
+          sites(:, nsite) = matmul(cd%atom_cart(:, ia), inv)
+        end do
+      end if
+      if (index(rest, '[') > 0 .or. index(rest, '(u') > 0 .or. index(rest, '(d') > 0) then
+        ierr = 5
+        msg = 'Error: spinor projections are not supported by this build'
+        return
+      end if
+      ! angular part: up to the next colon, ;-separated
+      c = index(rest, ':')
+      if (c > 0) then
+        part = rest(:c - 1)
+        rest = rest(c + 1:)
+      else
+        part = rest
+        rest = ''
+      end if
+      states = .false.
+      do while (len_trim(part) > 0)
+        q = index(part, ';')
+        if (q > 0) then
+          field = part(:q - 1)
+          part = part(q + 1:)
+        else
+          field = part
+          part = ''
+        end if
+        if (field(1:2) == 'l=') then
+          q = index(field, ',mr=')
+          if (q > 0) then
+            read (field(3:q - 1), *, iostat=ios) ll
+          else
+            read (field(3:), *, iostat=ios) ll
+          end if
+          if (ios /= 0 .or. ll < -5 .or. ll > 3) then
+            ierr = 5
+            msg = 'Error: malformed angular momentum: '//trim(field)
+            return
+          end if
+          m1 = merge(2 * ll + 1, 1 - ll, ll >= 0)
+          if (q == 0) then
+            states(1:m1, ll) = .true.
+          else
+            field = field(q + 4:)
+            do while (len_trim(field) > 0)
+              ia = index(field, ',')
+              if (ia == 0) ia = len_trim(field) + 1
+              read (field(:ia - 1), *, iostat=ios) mm
+              if (ios /= 0 .or. mm < 1 .or. mm > m1) then
+                ierr = 5
+                msg = 'Error: malformed mr in projection: '//trim(line)
+                return
+              end if
+              states(mm, ll) = .true.
+              field = field(ia + 1:)
+            end do
+          end if
+        else
+          call proj_name(trim(field), ll, m0, m1)
+          if (ll == -99) then
+            ierr = 5
+            msg = 'Error: unknown projection '//trim(field)
+            return
+          end if
+          states(m0:m1, ll) = .true.
+        end if
+      end do
+      ! options z=, x=, r=, zona=
+      zax = [0.0_dp, 0.0_dp, 1.0_dp]
+      xax = [1.0_dp, 0.0_dp, 0.0_dp]
+      zn = 1.0_dp
+      rad = 1
+      do while (len_trim(rest) > 0)
+        c = index(rest, ':')
+        if (c > 0) then
+          field = rest(:c - 1)
+          rest = rest(c + 1:)
+        else
+          field = rest
+          rest = ''
+        end if
+        ios = 0
+        if (field(1:2) == 'z=') then
+          read (field(3:), *, iostat=ios) zax
+        else if (field(1:2) == 'x=') then
+          read (field(3:), *, iostat=ios) xax
+        else if (field(1:5) == 'zona=') then
+          read (field(6:), *, iostat=ios) zn
+        else if (field(1:2) == 'r=') then
+          read (field(3:), *, iostat=ios) rad
+        else
+          ios = 1
+        end if
+        if (ios /= 0) then
+          ierr = 5
+          msg = 'Error: malformed projection option: '//trim(field)
+          return
+        end if
+      end do
+      do ia = 1, nsite
+        do ll = -5, 3
+          do mm = 1, 7
+            if (.not. states(mm, ll)) cycle
+            if (np == cap) then
+              ierr = 5
+              msg = 'Error: too many projection functions'
+              return
+            end if
+            np = np + 1
+            pl(np) = ll
+            pm(np) = mm
+            psite(:, np) = sites(:, ia)
+            pz(:, np) = zax
+            px(:, np) = xax
+            prad(np) = rad
+            pzona(np) = zn
+          end do
+        end do
+      end do
+      deallocate (sites)
+    end do
+    if (np < cd%num_wann) then
+      ierr = 5
+      msg = 'Error: too few projection functions defined'
+    end if
+  end subroutine expand_projections
+
+  !> `s` without blanks
+  pure function strip_spaces(s) result(r)
+    character(len=*), intent(in) :: s
+    character(len=len(s)) :: r
+    integer :: i, k
+    r = ''
+    k = 0
+    do i = 1, len_trim(s)
+      if (s(i:i) == ' ' .or. s(i:i) == achar(9)) cycle
+      k = k + 1
+      r(k:k) = s(i:i)
+    end do
+  end function strip_spaces
+
+  pure function inverse3(a) result(b)
+    real(dp), intent(in) :: a(3, 3)
+    real(dp) :: b(3, 3), det
+    b(1, 1) = a(2, 2) * a(3, 3) - a(2, 3) * a(3, 2)
+    b(1, 2) = a(1, 3) * a(3, 2) - a(1, 2) * a(3, 3)
+    b(1, 3) = a(1, 2) * a(2, 3) - a(1, 3) * a(2, 2)
+    b(2, 1) = a(2, 3) * a(3, 1) - a(2, 1) * a(3, 3)
+    b(2, 2) = a(1, 1) * a(3, 3) - a(1, 3) * a(3, 1)
+    b(2, 3) = a(1, 3) * a(2, 1) - a(1, 1) * a(2, 3)
+    b(3, 1) = a(2, 1) * a(3, 2) - a(2, 2) * a(3, 1)
+    b(3, 2) = a(1, 2) * a(3, 1) - a(1, 1) * a(3, 2)
+    b(3, 3) = a(1, 1) * a(2, 2) - a(1, 2) * a(2, 1)
+    det = a(1, 1) * b(1, 1) + a(1, 2) * b(2, 1) + a(1, 3) * b(3, 1)
+    b = b / det
+  end function inverse3
 
   !> Omega decomposition: omega(1:4) = Omega_I, Omega_D, Omega_OD, Omega;
   !> the reference keeps these in common_data%omega without a getter.
