@@ -16,7 +16,7 @@ import re
 import numpy as np
 import pytest
 
-from conftest import ROOT, Case, golden_names, tol
+from conftest import ROOT, Case, golden_names, stored_golden_names, tol
 
 po = pytest.importorskip("oracle.pyoracle")
 
@@ -30,7 +30,7 @@ def _oracle_ok():
 
 
 needs_oracle = pytest.mark.skipif(not _oracle_ok(), reason="oracle not built (make -C oracle)")
-FULL = [n for n in golden_names() if n != "cfg2"]
+FULL = stored_golden_names()
 
 
 def oracle_problem(c):
@@ -59,16 +59,39 @@ def test_oracle_trajectory_matches_reference(name):
 @needs_oracle
 @pytest.mark.parametrize("name", FULL)
 def test_oracle_final_state_matches_reference(name):
+    """The final state equals the reference's to the plain-path bounds, or
+    to 20x the trajectory's measured sensitivity to rounding where that is
+    larger.  On a path that passes saddles (rounding moves the final U by
+    more than 1e-3), the reference converged to the closed-form global
+    minimum and the oracle, given the iterations to converge, reaches the
+    same minimum."""
     c = Case(name)
-    if not c.well_conditioned():
-        pytest.skip("trajectory not reproducible by any FP64 implementation (sensitivity %.1e)" % c.sens_u)
     r = po.wann_main(oracle_problem(c), c.ctl, c.m0, c.u0)
-    assert np.abs(r["u"] - c.z["u_final"]).max() <= tol(1e-7, c.sens_u)
-    assert np.abs(r["m"] - c.z["m_final"]).max() <= tol(1e-7, c.sens_u)
-    assert np.abs(r["centres"] - c.z["centres"]).max() <= tol(1e-6, c.sens_centres)
-    # .wout prints Omega with 9 decimals and the .chk keeps spreads in full
-    assert abs(r["omega"][3] - c.z["omega"][3]) <= 1e-9 + tol(1e-8, c.sens_omega)
-    assert np.abs(r["spreads"] - c.z["spreads"]).max() <= tol(1e-8, c.sens_omega)
+    if c.sens_u <= 1e-3:
+        assert np.abs(r["u"] - c.z["u_final"]).max() <= tol(1e-7, c.sens_u)
+        assert np.abs(r["m"] - c.z["m_final"]).max() <= tol(1e-7, c.sens_u)
+        assert np.abs(r["centres"] - c.z["centres"]).max() <= tol(1e-6, c.sens_centres)
+        # .wout prints Omega with 9 decimals and the .chk keeps spreads in full
+        assert abs(r["omega"][3] - c.z["omega"][3]) <= 1e-9 + tol(1e-8, c.sens_omega)
+        assert np.abs(r["spreads"] - c.z["spreads"]).max() <= tol(1e-8, c.sens_omega)
+        return
+    if int(c.z["iterations"]) >= c.ctl["num_iter"]:
+        pytest.skip("the reference run ends unconverged on a path that rounding changes "
+                    "(sensitivity of Omega %.1e); no final state to compare" % c.sens_omega)
+    assert abs(c.z["omega"][3]) <= 1e-8
+    r = po.wann_main(oracle_problem(c), dict(c.ctl, num_iter=6000), c.m0, c.u0)
+    assert r["converged"] and abs(r["omega"][3] - c.z["omega"][3]) <= 1e-8
+    tau, lat = c.z["tau_cart"], c.z["real_lattice"]
+    inv = np.linalg.inv(lat)
+    for cen in (r["centres"], c.z["centres"]):
+        hit = []
+        for x in cen.T:
+            d = (x - tau) @ inv
+            d -= np.round(d)
+            dist = np.linalg.norm(d @ lat, axis=1)
+            assert dist.min() < 1e-6
+            hit.append(int(np.argmin(dist)))
+        assert sorted(hit) == list(range(c.nw))
 
 
 @needs_oracle
@@ -151,6 +174,34 @@ def test_rotation_keeps_unitarity_and_symmetry(name):
                     and np.allclose(bk[:, bb, k2], -bk[:, b, k], atol=1e-8)][0]
             for m in (c.m0, m1):
                 assert np.abs(np.conj(m[:, :, b, k].T) - m[:, :, bneg, k2]).max() < 1e-10
+
+
+@needs_oracle
+def test_check_unitarity_closed_forms():
+    """wann_check_unitarity's deviation max |UU^+ - 1|, |U^+U - 1| on
+    non-unitary U with known deviations: a block scaled by (1 + d) deviates
+    by 2d + d^2 on the diagonal, and the identity with one off-diagonal entry
+    e deviates by e (off-diagonal) when e < 1; the largest block decides."""
+    c = Case("ref_example05")
+    pr = oracle_problem(c)
+    u = np.array(c.u0, order="F")
+    assert po.unitarity(pr, u) < 1e-13
+    for d in (1e-6, 3e-4, 0.25):
+        v = np.array(u, order="F")
+        v[:, :, 5] *= 1.0 + d
+        assert abs(po.unitarity(pr, v) - (2 * d + d * d)) <= 1e-12 * max(1.0, d)
+    for e in (2e-5, 0.1, 0.6):
+        v = np.array(u, order="F")
+        v[:, :, 7] = np.eye(c.nw)
+        v[2, 0, 7] = e
+        assert abs(po.unitarity(pr, v) - e) <= 1e-15
+        # a second, smaller deviation elsewhere does not change the maximum
+        v[:, :, 9] *= 1.0 + 0.1 * e
+        assert abs(po.unitarity(pr, v) - e) <= 1e-15
+    # a phase change of one column keeps U unitary
+    v = np.array(u, order="F")
+    v[:, 1, 3] *= np.exp(0.7j)
+    assert po.unitarity(pr, v) < 1e-13
 
 
 @needs_oracle
@@ -307,3 +358,23 @@ def test_oracle_lowdin_matches_reference(name):
     u, m1 = po.lowdin(oracle_problem(c), a, m)
     assert np.abs(u - c.u0).max() < 1e-12
     assert np.abs(m1 - c.m0).max() < 1e-12
+
+
+def test_generator_workers_do_not_load_the_gpu_runtime():
+    """Problem generation (k-mesh search included) runs in worker processes
+    that load only the host library: no HIP runtime in the process."""
+    import subprocess
+    import sys
+
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from wannier90_amd import workloads\n"
+            "p = workloads._generate(workloads.batch_params(3))\n"
+            "maps = open('/proc/self/maps').read()\n"
+            "print(p['nw'], p['nn'], 'amdhip64' in maps, 'libw90gpu' in maps, 'libw90io' in maps)\n" % ROOT)
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    nw, nn, hip, gpu, host = out.stdout.split()
+    assert (hip, gpu, host) == ("False", "False", "True")
+    from wannier90_amd import workloads
+
+    assert workloads.MAX_WORKERS + 1 <= 15

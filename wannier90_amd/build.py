@@ -34,6 +34,10 @@ HIP_SOURCES = ["w90gpu.hip"]
 CPP_SOURCES = ["kmesh.cpp", "w90io.cpp"]
 
 LIB_NAME = "libw90gpu.so"
+# the host-only part (k-mesh search, .mmn/.amn readers) as a library of its
+# own without HIP code, for processes that must not start the GPU runtime
+# (problem generators in worker processes)
+HOST_LIB_NAME = "libw90io.so"
 
 
 def _run(cmd, cwd=None):
@@ -59,6 +63,9 @@ def build_hip(verbose=True):
     tmp = out + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-pthread", "-o", tmp])
     os.replace(tmp, out)
+    host = os.path.join(LIBDIR, HOST_LIB_NAME)
+    _run(["g++", "-shared", "-fPIC", *objs[len(HIP_SOURCES):], "-pthread", "-o", host + ".tmp"])
+    os.replace(host + ".tmp", host)
     return out
 
 

@@ -5,7 +5,10 @@ order='F' with shapes m(nw,nw,nn,nk), u(nw,nw,nk), cdq(nw,nw,nk);
 nnlist int32 (nk,nn) 1-based; bk float64 (3,nn,nk); wb float64 (nn).
 
 The library is loaded from wannier90_amd/lib; there is no fallback: if the
-library is missing or has no GPU, calls raise.
+library is missing or has no GPU, calls raise.  The host-side entry points
+(k-mesh search, .mmn/.amn readers) are also built without any HIP code into
+libw90io.so, which kmesh_get/read_mmn/read_amn use: a process that only
+generates or reads problems never starts the GPU runtime.
 """
 import ctypes as C
 import os
@@ -14,6 +17,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libw90gpu.so")
+HOST_LIB_PATH = os.path.join(HERE, "lib", "libw90io.so")
 
 OK = 0
 ERR_FATAL = 1
@@ -64,6 +68,7 @@ class Result(C.Structure):
 
 
 _lib = None
+_host_lib = None
 
 # (name, restype, argtypes) of every entry point of include/w90gpu.h
 _dp = C.POINTER(C.c_double)
@@ -110,6 +115,25 @@ SIGNATURES = [
     ("w90gpu_batch_remove", C.c_int, [_vp, C.c_int]),
     ("w90gpu_batch_info", C.c_int, [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), _ip, _ip]),
 ]
+
+
+HOST_ENTRY_POINTS = ("w90gpu_kmesh_get", "w90gpu_read_mmn", "w90gpu_read_amn")
+
+
+def load_host():
+    """Load libw90io.so (host code only: no HIP runtime is loaded)."""
+    global _host_lib
+    if _host_lib is None:
+        if not os.path.exists(HOST_LIB_PATH):
+            raise OSError(f"{HOST_LIB_PATH} not built: run `python -m wannier90_amd.build`")
+        lib = C.CDLL(HOST_LIB_PATH)
+        for name, res, args in SIGNATURES:
+            if name in HOST_ENTRY_POINTS:
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+        _host_lib = lib
+    return _host_lib
 
 
 def load(path=None):
@@ -167,7 +191,7 @@ def kmesh_get(real_lattice, kpt_latt, search_shells=36, tol=1e-6, shell_list=Non
     real_lattice: (3,3) rows are lattice vectors in Angstrom.
     kpt_latt: (nk,3) fractional k-points.
     """
-    lib = load()
+    lib = load_host()
     rl = np.asfortranarray(np.asarray(real_lattice, dtype=np.float64))
     kp = np.ascontiguousarray(np.asarray(kpt_latt, dtype=np.float64))  # (nk,3) C == (3,nk) F
     nk = kp.shape[0]
@@ -205,7 +229,7 @@ def read_mmn(path, num_bands, kmesh, nthreads=0):
     (w90gpu_read_mmn; overlap_read, src/overlap.F90:234-308): each block
     goes to the b of the k-mesh with the block's k' and G.  Host threads
     parse the file; nthreads <= 0 takes OMP_NUM_THREADS or all cores."""
-    lib = load()
+    lib = load_host()
     nnl = np.asfortranarray(kmesh.nnlist, dtype=np.int32)
     nnc = np.asfortranarray(kmesh.nncell, dtype=np.int32)
     nk, nn = nnl.shape
@@ -221,7 +245,7 @@ def read_mmn(path, num_bands, kmesh, nthreads=0):
 def read_amn(path, num_bands, num_kpts, num_proj, nthreads=0):
     """.amn file -> A (num_bands, num_proj, nk) complex, order='F'
     (w90gpu_read_amn; overlap_read, src/overlap.F90:323-366)."""
-    lib = load()
+    lib = load_host()
     a = np.empty((num_bands, num_proj, num_kpts), dtype=np.complex128, order="F")
     err = C.create_string_buffer(512)
     rc = lib.w90gpu_read_amn(os.fsencode(path), num_bands, num_kpts, num_proj, a.ctypes.data_as(_dp), nthreads,

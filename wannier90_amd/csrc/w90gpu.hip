@@ -327,6 +327,7 @@ int grow_tables(Engine* E, int nprob, int nitems) {
     }
     E->d_probs = np_;
     E->cap_probs = cap;
+    E->epoch++;  // captured graphs hold the old table address
   }
   if (nprob > E->cap_plist) {
     const int cap = std::max(nprob, 2 * E->cap_plist);
