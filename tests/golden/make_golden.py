@@ -69,7 +69,12 @@ def control_from_win(text):
     return c
 
 
-def golden_case(name, srcdir, seed, store_m=True, overrides=None):
+def golden_case(name, srcdir, seed, store_m=True, overrides=None, tail_from=None):
+    """One fixture.  With tail_from = K the fixture is the tail of the run:
+    its initial state is the reference's own checkpoint after K iterations
+    (a run with num_iter = K), and its final state that of the reference
+    restarted from that checkpoint (restart = wannierise) with the case's
+    num_iter."""
     tmp = tempfile.mkdtemp(prefix="w90gold_")
     try:
         for f in os.listdir(srcdir):
@@ -81,11 +86,12 @@ def golden_case(name, srcdir, seed, store_m=True, overrides=None):
             text = set_keyword(text, k, v)
         open(win, "w").write(text)
         ctl = control_from_win(text)
-        # initial state: num_iter = 0 (Lowdin U and rotated M, wannierise.F90 loop skipped)
-        open(win, "w").write(set_keyword(text, "num_iter", 0))
+        # initial state: num_iter = 0 (Lowdin U and rotated M, wannierise.F90
+        # loop skipped), or the checkpoint after tail_from iterations
+        open(win, "w").write(set_keyword(text, "num_iter", tail_from or 0))
         run_ref(tmp, seed)
         chk0 = w90io.read_chk(os.path.join(tmp, seed + ".chk"))
-        open(win, "w").write(text)
+        open(win, "w").write(set_keyword(text, "restart", "wannierise") if tail_from else text)
         run_ref(tmp, seed)
         chk = w90io.read_chk(os.path.join(tmp, seed + ".chk"))
         cen, spr, om = w90io.read_wout_final(os.path.join(tmp, seed + ".wout"))
@@ -108,6 +114,7 @@ def golden_case(name, srcdir, seed, store_m=True, overrides=None):
         conv_iter=np.array(om["conv_iter"], dtype=np.int64),
         conv_omega=np.array(om["conv_omega"]),
         control=json.dumps(ctl),
+        tail_from=np.int64(tail_from or 0),
     )
     if store_m:
         data.update(u0=chk0["u_matrix"], m0=chk0["m_matrix"], m_final=chk["m_matrix"])
@@ -153,6 +160,38 @@ def sensitivity(km, ctl, m0, u0, eps=1e-15, seeds=(1, 2)):
                 pinned_iters=np.int64(max(pinned, 0)))
 
 
+def golden_driver_case(name, prob, params, num_iter):
+    """A problem too large for text .mmn files: its initial state (the
+    Lowdin U0 and rotated M0 of wannier90_amd.synth) goes in binary to the
+    reference's library through oracle/ref_driver.F90 (oracle/refrun.py),
+    which minimises for num_iter iterations.  Only scalars, the CONV
+    trajectory, the centres, spreads and a few U blocks are stored; the
+    tests regenerate U0 and M0 from the seed."""
+    from oracle import refrun
+
+    u0, m0 = synth.initial_state(prob)
+    ctl = dict(num_iter=num_iter, num_cg_steps=5, conv_window=3, conv_tol=1e-10, trial_step=2.0,
+               fixed_step=-999.0, lfixstep=0)
+    ctl.update(params)
+    km = prob.kmesh
+    r = refrun.run(prob.real_lattice, prob.mp, prob.kpts, km, m0, u0, ctl)
+    om_i, om_tilde, om_tot = r["omega"]
+    data = dict(
+        real_lattice=prob.real_lattice, kpt_latt=prob.kpts, mp_grid=np.array(prob.mp),
+        nnlist=km.nnlist, nncell=km.nncell, bk=km.bk, wb=km.wb, wbtot=np.float64(km.wbtot),
+        u_final=r["u"][:, :, :8].copy(), centres=r["centres"], spreads=r["spreads"],
+        # the library returns Omega_I, Omega~ and Omega; Omega_D is not split off
+        omega=np.array([om_i, np.nan, np.nan, om_tot]), omega_tilde=np.float64(om_tilde),
+        iterations=np.int64(r["iterations"]), conv_iter=r["conv_iter"], conv_omega=r["conv_omega"],
+        control=json.dumps(ctl), tail_from=np.int64(0), tau_cart=prob.tau_cart,
+        reference_seconds=np.float64(r["seconds"]),
+    )
+    data.update(sensitivity(km, ctl, m0, u0, seeds=(1,)))
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **data)
+    print(f"{name}: nk={prob.num_kpts} nn={km.nntot} nw={prob.num_wann} iters={r['iterations']} "
+          f"Omega={om_tot} ({r['seconds']:.1f} s in the reference)", flush=True)
+
+
 def synth_dir(tag, prob, params):
     d = tempfile.mkdtemp(prefix="w90syn_" + tag)
     base = dict(num_iter=500, conv_tol="1.0e-10", conv_window=3, trial_step="2.0", num_cg_steps=5,
@@ -195,14 +234,26 @@ def main(which=None):
     for name, prob, params in syn:
         d = synth_dir(name, prob, params)
         cases.append((name, d, "w", True, None))
+    # cfg1's full run passes near saddles where rounding decides the path
+    # (its final state moves by Omega ~ 7 under a 1e-15 perturbation of M0);
+    # the reference restarted from its own iteration-300 checkpoint converges
+    # to the global minimum on a path that rounding does not change
+    tails = [("cfg1_tail", synth.config(1), {}, 300)]
+    for name, prob, params, k in tails:
+        d = synth_dir(name, prob, params)
+        cases.append((name, d, "w", True, None, k))
     # the benchmark workload (fcc, 16 WFs, 8^3 mesh): inputs too large to store
     if which and "cfg2" in which:
         d = synth_dir("cfg2", synth.config(2), {})
         cases.append(("cfg2", d, "w", False, None))
-    for name, d, seed, store_m, ov in cases:
+    # the triclinic 64-band problem on its 10x10x8 mesh (800 k, nntot 12):
+    # 40 iterations of the reference take ~2 min here
+    if which and "cfg3" in which:
+        golden_driver_case("cfg3", synth.config(3), {}, 40)
+    for name, d, seed, store_m, ov, *tail in cases:
         if which and name not in which:
             continue
-        golden_case(name, d, seed, store_m, ov)
+        golden_case(name, d, seed, store_m, ov, tail_from=tail[0] if tail else None)
         tau = os.path.join(d, "tau_cart.npy")
         if os.path.exists(tau):
             z = dict(np.load(os.path.join(OUT, name + ".npz")))

@@ -13,11 +13,11 @@ within 1e-8 Ang^2 and centres within 1e-6 Ang.
 import numpy as np
 import pytest
 
-from conftest import Case, golden_names
+from conftest import Case, golden_names, stored_golden_names
 
 pytestmark = pytest.mark.gpu
 
-SMALL = [n for n in golden_names() if n != "cfg2"]
+SMALL = stored_golden_names()
 
 
 @pytest.fixture(scope="module")

@@ -10,12 +10,12 @@ iterations every FP64 implementation reproduces (`pinned_iters`).
 import numpy as np
 import pytest
 
-from conftest import Case, golden_names, tol
+from conftest import Case, golden_names, stored_golden_names, tol
 
 pytestmark = pytest.mark.gpu
 
 po = pytest.importorskip("oracle.pyoracle")
-FULL = [n for n in golden_names() if n != "cfg2"]
+FULL = stored_golden_names()
 
 
 @pytest.fixture(scope="module")
@@ -115,16 +115,71 @@ def test_trajectory_matches_reference(ctx, name):
         assert r["res"].iterations == int(c.z["iterations"])
 
 
+def centres_at_orbitals(cen, tau, lat, tol_c=1e-6):
+    """Every centre sits on an orbital position (Cartesian), up to lattice
+    vectors, and each orbital holds one centre."""
+    inv = np.linalg.inv(lat)
+    hit = []
+    for x in cen.T:
+        d = (x - tau) @ inv
+        d -= np.round(d)
+        dist = np.linalg.norm(d @ lat, axis=1)
+        if dist.min() >= tol_c:
+            return False
+        hit.append(int(np.argmin(dist)))
+    return sorted(hit) == list(range(tau.shape[0]))
+
+
+def basin_invariants(c, r):
+    """What holds for the final state whatever path rounding selects: Omega_I
+    stays 0 (unitary overlaps), U stays unitary, the spreads sum to Omega."""
+    res = r["res"]
+    assert res.unitarity_ok == 1
+    # the fixture's Omega_I comes from the .wout, printed with 9 decimals
+    assert abs(res.omega_invariant - c.z["omega"][0]) <= 1e-9
+    assert abs(r["spreads"].sum() - res.omega_total) <= 1e-9 * max(1.0, res.omega_total)
+
+
+def fixed_point_of_oracle(c, r, iters=20):
+    """The converged GPU state is a fixed point of the reference algorithm:
+    the oracle restarted from it moves Omega by <= 1e-8 and the centres by
+    <= 1e-6 over `iters` iterations."""
+    o = po.wann_main(oracle_problem(c), c.oracle_control(num_iter=iters, conv_window=-1), r["m"], r["u"])
+    assert abs(o["omega"][3] - r["res"].omega_total) <= 1e-8
+    assert np.abs(o["centres"] - r["centres"]).max() <= 1e-6
+
+
 @pytest.mark.parametrize("name", FULL)
 def test_final_state_matches_reference(ctx, name):
     print(f"[final] {name}", flush=True)
     c = Case(name)
     r = gpu_solve(ctx, c)
     res = r["res"]
-    assert res.unitarity_ok == 1
+    basin_invariants(c, r)
     if c.sens_u > 1e-3:
-        pytest.skip("final U not reproducible by any FP64 implementation (sensitivity %.1e); "
-                    "pinned by test_pinned_prefix_matches_oracle" % c.sens_u)
+        # the path passes saddles where rounding decides the way down: a
+        # 1e-15 change of M0 moves the final Omega by sens_omega, and the
+        # oracle (same algorithm, same LAPACK) does not reproduce the
+        # reference's final state either.  What every FP64 implementation
+        # shares is the set of minima: with a budget that lets the solve
+        # converge, the GPU state is a minimum (a fixed point of the
+        # oracle's iteration); where the reference itself converged to the
+        # closed-form global minimum (Omega = 0, centres at the orbitals),
+        # the GPU reaches that same minimum.
+        r2 = gpu_solve(ctx, c, num_iter=6000)
+        basin_invariants(c, r2)
+        print(f"[final] {name}: {r2['res'].iterations} iterations to Omega {r2['res'].omega_total:.3e} "
+              f"(reference {int(c.z['iterations'])} to {c.z['omega'][3]:.3e})", flush=True)
+        assert r2["res"].converged == 1
+        fixed_point_of_oracle(c, r2)
+        if int(c.z["iterations"]) < c.ctl["num_iter"] and "tau_cart" in c.z:
+            # the reference converged, to the global minimum
+            assert abs(c.z["omega"][3]) <= 1e-8
+            assert abs(r2["res"].omega_total - c.z["omega"][3]) <= 1e-8
+            tau, lat = c.z["tau_cart"], c.z["real_lattice"]
+            assert centres_at_orbitals(c.z["centres"], tau, lat)
+            assert centres_at_orbitals(r2["centres"], tau, lat)
+        return
     if c.well_conditioned():
         assert res.iterations == int(c.z["iterations"])
     else:
@@ -146,13 +201,21 @@ def test_pinned_prefix_matches_oracle(ctx, name):
     c = Case(name)
     n = max(1, min(int(c.z["pinned_iters"]), c.ctl["num_iter"]) - 2)
     r = gpu_solve(ctx, c, num_iter=n)
-    o = po.wann_main(oracle_problem(c), c.oracle_control(num_iter=n), c.m0, c.u0)
+    op = oracle_problem(c)
+    o = po.wann_main(op, c.oracle_control(num_iter=n), c.m0, c.u0)
     assert r["res"].iterations == o["iterations"]
-    # pinned_iters pins the Omega trajectory; U and the centres may still
-    # carry the case's measured sensitivity to rounding (near-degenerate
-    # gauge directions), which bounds them as in the full-run tests
-    assert np.abs(r["u"] - o["u"]).max() <= tol(1e-7, c.sens_u)
-    assert np.abs(r["centres"] - o["centres"]).max() <= tol(1e-6, c.sens_centres)
+    # the sensitivity of this prefix to rounding (not of the full run): the
+    # oracle rerun over the same n iterations from M0 perturbed by 1e-15
+    s_u = s_c = 0.0
+    for seed in (1, 2):
+        rng = np.random.default_rng(seed)
+        o2 = po.wann_main(op, c.oracle_control(num_iter=n), c.m0 * (1.0 + 1e-15 * rng.standard_normal(c.m0.shape)),
+                          c.u0)
+        s_u = max(s_u, float(np.abs(o2["u"] - o["u"]).max()))
+        s_c = max(s_c, float(np.abs(o2["centres"] - o["centres"]).max()))
+    print(f"[prefix] {name}: {n} iterations, sensitivity U {s_u:.1e} centres {s_c:.1e}", flush=True)
+    assert np.abs(r["u"] - o["u"]).max() <= tol(1e-7, s_u)
+    assert np.abs(r["centres"] - o["centres"]).max() <= tol(1e-6, s_c)
     assert abs(r["res"].omega_total - o["omega"][3]) <= 1e-8
     assert np.abs(r["history"][:, 0] - o["history"][:, 0]).max() <= 1e-8
 
