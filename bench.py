@@ -1,25 +1,38 @@
 """Benchmark of the MI355X wannierisation path.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
-                    [--no-cpu-baseline] [--workload single|batch|serve|mmn]
+                    [--config 1|2|3] [--extras batch,serve|none] [--no-cpu-baseline]
+                    [--workload single|batch|serve|converge|mmn]
 
-single (default): the benchmark configuration of the metric, a seeded fcc
-tight-binding problem (a = 5.43 Ang, num_wann = num_bands = 16, decay 2 Ang,
-8x8x8 MP grid = 512 k, nntot = 8), solved with num_iter = 500,
-conv_tol = 1e-10, conv_window = 3, trial_step = 2, num_cg_steps = 5.
-One step = one complete minimisation (wann_main) from the Lowdin-projected
-initial state, with M and U resident in HBM before the timed region.  The
-same problem is then solved in mixed precision and its Omega and centres
-compared with the FP64 solve.
+single (default), --config 3 (default): the largest single-GPU
+configuration, a seeded triclinic tight-binding problem (a, b, c ~ U[4,7]
+Ang, angles ~ U[75,105] deg, num_wann = num_bands = 64, decay 2.5 Ang,
+10x10x8 MP grid = 800 k, nntot 12), solved with num_iter = 500,
+conv_tol = 1e-10, conv_window = 3, trial_step = 2, num_cg_steps = 5 by the
+plain FP64 drop-in path, one problem per launch.  One step = one complete
+minimisation (wann_main) from the Lowdin-projected initial state, with M
+and U resident in HBM before the timed region.  --config 2 is the fcc
+16-band 512-k problem, --config 1 the 4-band 64-k one.
 
-Prints one JSON line (value = FP64 solves/s on one GPU) with the roofline
-of the dominant kernel (the M rotation, timed with HIP events on its own
-stream) and the CPU baseline (the reference wannier90.x on a bounded sample,
-timed on this host).
+Prints one JSON line: value = FP64 solves/s on one GPU, with
+  roofline      the dominant kernel (the M rotation with the fused omega
+                partial sums), its duration from HIP events on its own
+                stream, against the FP64 MFMA or HBM peak;
+  mixed         the same problem in mixed precision (FP32 MFMA bulk, FP64
+                phases and polish): solves/s and Omega / centres against the
+                FP64 solve;
+  ragged_batch  (extras) the ragged batch of 256 problems: one problem per
+                launch in FP64, batched FP64, batched mixed;
+  serving       (extras) open-loop Poisson serving with continuous
+                batching: p50/p99 time to converged;
+  cpu_baseline  the reference (its w90_library built from its sources,
+                driven by oracle/ref_driver.F90) with MPI over k-points on
+                the host's cores, on a bounded sample of the same workload.
 
-batch and serve (wannier90_amd/workloads.py): the ragged batch of 256
-problems and the open-loop serving loop.  mmn: the parse of the benchmark
-problem's .mmn file by the library's parallel reader (host only).
+batch, serve, converge (wannier90_amd/workloads.py): the ragged batch of 256
+problems, the serving loop, and the batch run to convergence.  mmn: the
+parse of a benchmark problem's .mmn file by the library's parallel reader
+(host only).
 """
 import argparse
 import json
@@ -44,20 +57,27 @@ FP32_PEAK_TFLOPS = 157.3
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="single: skip the mixed-precision comparison")
     ap.add_argument("--cpu-sample-iters", type=int, default=0,
-                    help="iterations of the CPU baseline sample (0: 100 for configs 1-2, 3 for config 3)")
+                    help="iterations of the CPU baseline sample (0: 100 for configs 1-2, 20 for config 3)")
+    ap.add_argument("--cpu-ranks", type=int, default=0,
+                    help="MPI ranks of the CPU baseline (0: the host's cores, at most 16)")
+    ap.add_argument("--extras", default="batch,serve",
+                    help="single: comma list of extra workloads reported beside the line (batch, serve) or none")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device once (a kernel of known traffic that "
                          "calibrates FETCH_SIZE/WRITE_SIZE counters)")
-    ap.add_argument("--config", type=int, choices=[1, 2, 3], default=2,
-                    help="single: benchmark configuration (2: the metric's; 3: num_wann 64 triclinic)")
-    ap.add_argument("--workload", choices=["single", "batch", "serve", "mmn"], default="single")
+    ap.add_argument("--config", type=int, choices=[1, 2, 3], default=3,
+                    help="single: configuration (3: num_wann 64 triclinic 800 k, the largest; 2: fcc 16 "
+                         "bands 512 k; 1: sc 4 bands 64 k)")
+    ap.add_argument("--workload", choices=["single", "batch", "serve", "mmn", "converge"], default="single")
     ap.add_argument("--problems", type=int, default=256, help="batch: number of problems")
+    ap.add_argument("--first", type=int, default=0, help="converge: first batch item")
+    ap.add_argument("--no-plain", action="store_true", help="converge: skip the one-per-launch FP64 solves")
     ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second (0: from a probe)")
     ap.add_argument("--num-iter", type=int, default=500)
@@ -74,53 +94,80 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(prob, params, sample_iters, use_reference=True):
-    """Reference wannier90.x (oracle/_ref, serial, MKL sequential) on the same
-    inputs, num_iter cut to `sample_iters`; scaled to the full solve by the
-    iteration count of the GPU solve.  Falls back to the C restatement (also
-    used for num_wann 64, whose text .mmn would take minutes to write and
-    read)."""
-    from wannier90_amd import w90io
-
-    exe = os.path.join(ROOT, "oracle", "_ref", "wannier90.x")
-    tmp = tempfile.mkdtemp(prefix="w90bench_")
+def cpu_ranks(requested):
+    """MPI ranks of the CPU baseline: the requested count, or the host's
+    cores (at most 16, the CPU share of one GPU on the benchmark host)."""
+    if requested > 0:
+        return requested
     try:
-        if use_reference and os.path.exists(exe):
-            p = dict(params)
-            p["num_iter"] = sample_iters
-            w90io.write_win(os.path.join(tmp, "w.win"), prob.real_lattice, prob.mp, prob.kpts,
-                            prob.num_wann, params=p)
-            w90io.write_mmn(os.path.join(tmp, "w.mmn"), prob.m_orig, prob.kmesh)
-            w90io.write_amn(os.path.join(tmp, "w.amn"), prob.amat)
-            w90io.write_eig(os.path.join(tmp, "w.eig"), prob.evals)
-            env = dict(os.environ, OMP_NUM_THREADS="1", MKL_NUM_THREADS="1")
-            t0 = time.perf_counter()
-            subprocess.run([exe, "w"], cwd=tmp, check=True, env=env, stdout=subprocess.DEVNULL,
-                           stderr=subprocess.DEVNULL)
-            dt = time.perf_counter() - t0
-            _, _, om = w90io.read_wout_final(os.path.join(tmp, "w.wout"))
-            # loop time from the CONV wall-clock column (iteration 0 is printed
-            # when wann_main starts), so file parsing is not counted
-            loop = om["conv_time"][-1] - om["conv_time"][0]
-            return dict(seconds=loop, iters=om["iterations"], kind="reference",
-                        sample=f"reference wannier90.x (serial, MKL sequential) on the benchmark "
-                               f"inputs with num_iter={sample_iters}; loop wall time {loop:.2f} s "
-                               f"from its CONV lines ({dt:.1f} s for the whole process)")
-        from oracle import pyoracle as po
-        from wannier90_amd import synth
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
-        u0, m0 = synth.initial_state(prob)
-        pr = po.Problem(prob.num_wann, prob.num_kpts, prob.kmesh)
-        ctl = dict(num_iter=sample_iters, num_cg_steps=5, conv_window=3, conv_tol=1e-10,
-                   trial_step=2.0)
-        t0 = time.perf_counter()
-        r = po.wann_main(pr, ctl, m0, u0)
-        dt = time.perf_counter() - t0
-        return dict(seconds=dt, iters=r["iterations"], kind="port",
-                    sample=f"C restatement of wann_main (1 thread, MKL sequential), "
-                           f"num_iter={sample_iters}")
-    finally:
-        shutil.rmtree(tmp, ignore_errors=True)
+
+def cpu_baseline(prob, m0, u0, sample_iters, ranks):
+    """The reference's wann_main (its w90_library compiled from its sources
+    by oracle/Makefile.ref, driven by oracle/ref_driver.F90) on the same
+    initial state with num_iter cut to `sample_iters`, with MPI over
+    k-points on `ranks` ranks (MKL sequential, one core per rank).  The loop
+    time comes from the CONV wall-clock column (iteration 0 is printed when
+    wann_main starts).  Falls back to the serial reference, then to the C
+    restatement (kind "port")."""
+    from oracle import refrun
+
+    ctl = dict(num_iter=sample_iters, num_cg_steps=5, conv_window=3, conv_tol=1e-10, trial_step=2.0,
+               num_print_cycles=1)
+    for p in ([ranks, 1] if ranks > 1 else [1]):
+        if not refrun.available(p):
+            continue
+        try:
+            r = refrun.run(prob.real_lattice, prob.mp, prob.kpts, prob.kmesh, m0, u0, ctl, ranks=p)
+        except (RuntimeError, OSError, subprocess.SubprocessError) as e:
+            log(f"[bench] cpu baseline with {p} ranks failed: {e}")
+            continue
+        loop = float(r["conv_wall"][-1] - r["conv_wall"][0])
+        return dict(seconds=loop, iters=r["iterations"], kind="reference", cores=p,
+                    sample=f"reference w90_library (its sources, amdflang, MKL sequential) driven by "
+                           f"oracle/ref_driver.F90, {'MPI over k-points on %d ranks' % p if p > 1 else 'serial'}, "
+                           f"num_iter={sample_iters} from the same initial state; loop wall time {loop:.2f} s "
+                           f"from its CONV lines ({r['seconds']:.1f} s for w90_wannierise)")
+    from oracle import pyoracle as po
+
+    pr = po.Problem(prob.num_wann, prob.num_kpts, prob.kmesh)
+    t0 = time.perf_counter()
+    r = po.wann_main(pr, ctl, m0, u0)
+    dt = time.perf_counter() - t0
+    return dict(seconds=dt, iters=r["iterations"], kind="port", cores=1,
+                sample=f"C restatement of wann_main (1 thread, MKL sequential), num_iter={sample_iters}")
+
+
+def batch_summary(args):
+    """The ragged batch of 256 problems (workloads.run_batch) as an extra key."""
+    from types import SimpleNamespace
+
+    from wannier90_amd import workloads
+
+    a = SimpleNamespace(problems=256, num_iter=500)
+    r = workloads.run_batch(a)
+    c = dict(r["config"])
+    c["value_solves_s"] = r["value"]
+    return c
+
+
+def serve_summary(args):
+    """The serving loop (workloads.run_serve) as an extra key."""
+    from types import SimpleNamespace
+
+    from wannier90_amd import workloads
+
+    a = SimpleNamespace(requests=args.requests, num_iter=500, rate=args.rate, precision="fp64",
+                        max_running=args.max_running, large_running=args.large_running, lanes=args.lanes,
+                        pool_gb=args.pool_gb)
+    r = workloads.run_serve(a)
+    c = dict(r["config"])
+    c["value_solves_s"] = r["value"]
+    return c
 
 
 WORKLOADS = {
@@ -215,9 +262,7 @@ def main():
     log(f"[bench] generated cfg{args.config} problem in {time.perf_counter() - t:.1f}s: nk={prob.num_kpts} "
         f"nn={prob.kmesh.nntot} nw={prob.num_wann}")
     if args.cpu_sample_iters <= 0:
-        args.cpu_sample_iters = 3 if args.config == 3 else 100
-    params = dict(num_iter=500, conv_tol="1.0e-10", conv_window=3, trial_step="2.0",
-                  num_cg_steps=5, num_print_cycles=100)
+        args.cpu_sample_iters = 20 if args.config == 3 else 100
     prec = capi.PREC_FP64 if args.precision == "fp64" else capi.PREC_MIXED
     ctl = capi.control(num_iter=500, conv_tol=1e-10, conv_window=3, trial_step=2.0,
                        num_cg_steps=5, precision=prec)
@@ -300,14 +345,19 @@ def main():
     }
     if prec == capi.PREC_FP64 and not args.no_mixed:
         out["mixed"] = mixed_comparison(ctx, prob, m0, u0, sess, results[-1], args)
-    if not args.no_cpu_baseline:
-        cb = cpu_baseline(prob, params, args.cpu_sample_iters, use_reference=args.config != 3)
-        per_solve = cb["seconds"] * its / max(cb["iters"], 1)
-        out["cpu_baseline"] = dict(value=1.0 / per_solve, unit="solves/s", cores=1, kind=cb["kind"],
-                                   sample=cb["sample"] + f"; scaled to {its} iterations")
-        log(f"[bench] cpu baseline: {cb['seconds']:.1f}s for {cb['iters']} iterations")
     sess.close()
     ctx.close()
+    extras = [] if args.extras in ("", "none") else args.extras.split(",")
+    if "batch" in extras:
+        out["ragged_batch"] = batch_summary(args)
+    if "serve" in extras:
+        out["serving"] = serve_summary(args)
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(prob, m0, u0, args.cpu_sample_iters, cpu_ranks(args.cpu_ranks))
+        per_solve = cb["seconds"] * its / max(cb["iters"], 1)
+        out["cpu_baseline"] = dict(value=1.0 / per_solve, unit="solves/s", cores=cb["cores"], kind=cb["kind"],
+                                   sample=cb["sample"] + f"; scaled to the GPU solve's {its} iterations")
+        log(f"[bench] cpu baseline: {cb['seconds']:.1f}s for {cb['iters']} iterations on {cb['cores']} cores")
     print(json.dumps(out), flush=True)
 
 
