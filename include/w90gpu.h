@@ -176,6 +176,20 @@ int w90gpu_omega(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* m_ma
                  int first_pass, double omega_invariant, double* rave, double* r2ave,
                  double* omega);
 
+/* No counterpart in the reference.  w90gpu_omega for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same result, computed with every
+ * device buffer in a static array of W90GPU_NOALLOC_SCRATCH_BYTES per device
+ * (taken when the library's code is loaded on a device, held until the
+ * process ends), on the null stream; it calls no hipMalloc, hipFree or other
+ * allocating routine.  A problem whose buffers exceed the array is rejected
+ * with W90GPU_ERR_INPUT.  Calls on one device share the array: calls that
+ * overlap on one GPU race. */
+#define W90GPU_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int w90gpu_omega_noalloc(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* m_matrix,
+                         int first_pass, double omega_invariant, double* rave, double* r2ave,
+                         double* omega);
+
 /* wann_domega (src/wannierise.F90:2287-2569), non-SLWF gradient:
  * cdodq(num_wann,num_wann,num_kpts) complex. */
 int w90gpu_domega(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* m_matrix,

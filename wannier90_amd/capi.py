@@ -89,6 +89,7 @@ SIGNATURES = [
     ("w90gpu_wann_main", C.c_int, [_vp, C.POINTER(Problem), C.POINTER(Control), _dp, _dp, _dp, _dp,
                                    C.POINTER(Result)]),
     ("w90gpu_omega", C.c_int, [_vp, C.POINTER(Problem), _dp, C.c_int, C.c_double, _dp, _dp, _dp]),
+    ("w90gpu_omega_noalloc", C.c_int, [_vp, C.POINTER(Problem), _dp, C.c_int, C.c_double, _dp, _dp, _dp]),
     ("w90gpu_domega", C.c_int, [_vp, C.POINTER(Problem), _dp, _dp]),
     ("w90gpu_new_u_and_m", C.c_int, [_vp, C.POINTER(Problem), _dp, _dp, _dp]),
     ("w90gpu_project", C.c_int, [_vp, C.POINTER(Problem), _dp, _dp, _dp]),
@@ -320,6 +321,19 @@ class Context:
         om = np.zeros(4)
         self._check(self.lib.w90gpu_omega(self.h, C.byref(prob), _d(m), 1 if first_pass else 0,
                                           omega_invariant, _d(rave), _d(r2ave), _d(om)))
+        return rave, r2ave, om
+
+    def omega_noalloc(self, kmesh, m, first_pass=True, omega_invariant=0.0):
+        """omega() through the allocation-free entry point (static device
+        scratch)."""
+        _check_fortran(m, 4)
+        nw, nk = m.shape[0], m.shape[3]
+        prob = self.problem(nw, kmesh, nk)
+        rave = np.zeros((3, nw), order="F")
+        r2ave = np.zeros(nw)
+        om = np.zeros(4)
+        self._check(self.lib.w90gpu_omega_noalloc(self.h, C.byref(prob), _d(m), 1 if first_pass else 0,
+                                                  omega_invariant, _d(rave), _d(r2ave), _d(om)))
         return rave, r2ave, om
 
     def domega(self, kmesh, m):

@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -26,7 +27,16 @@
 struct w90gpu_ctx {
   int device = 0;
   std::string err;
+  char* scratch = nullptr;  // this device's copy of `scratch` (w90gpu_omega_noalloc)
 };
+
+// Device memory of the allocation-free entry point w90gpu_omega_noalloc:
+// one static array, which every GPU the library runs on holds until the
+// process ends.  Host code never uses the name itself (on the host it is a
+// stand-in, not a device address): w90gpu_create looks up the address of
+// each device's copy once (scratch_address) and the entry point uses that.
+constexpr size_t kScratchBytes = W90GPU_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) char scratch[kScratchBytes];
 
 // Host side of libw90gpu: the engine that runs a set of problems on one
 // device, and the C ABI of include/w90gpu.h on top of it.
@@ -1176,6 +1186,58 @@ int member_fetch(Engine* E, Member* m, double* m_matrix, double* u_matrix, doubl
 // A one-member list of a member for the per-phase entry points.
 int single_lists(Engine* E, Member* m, Lists& L) { return aux_lists(E, std::vector<Member*>{m}, L); }
 
+// Address of `scratch` on `device`, looked up once per device (with the
+// device current) and kept for the life of the process.
+int scratch_address(int device, char** out) {
+  static std::mutex mu;
+  static std::map<int, char*> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = addr.find(device);
+  if (it != addr.end()) {
+    *out = it->second;
+    return W90GPU_OK;
+  }
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    set_error(nullptr, "hipGetSymbolAddress(scratch) failed on device %d: %s", device, hipGetErrorString(e));
+    return W90GPU_ERR_FATAL;
+  }
+  addr[device] = (char*)p;
+  *out = (char*)p;
+  return W90GPU_OK;
+}
+
+// Tables of the allocation-free path, built on the device: the work items
+// (problem 0, k), the problem list {0} and the neighbour table nnl[k][b]
+// (0-based) from the caller's nnlist(nk, nn), 1-based, copied raw.
+__global__ void k_noalloc_tables(Item* items, int* plist, int* nnl, const int* nnlist_raw, int nk, int nn) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) plist[0] = 0;
+  if (i < nk) items[i] = Item{0, i};
+  if (i < nk * nn) {
+    const int k = i / nn, b = i % nn;
+    nnl[i] = nnlist_raw[k + (size_t)nk * b] - 1;
+  }
+}
+
+// Offsets of the allocation-free path in the scratch array: the problem
+// record, the items, the problem list, the raw nnlist, then the arena of
+// problem_layout (num_iter 0, as w90gpu_omega uses).
+struct ScratchLayout {
+  size_t items, plist, raw, arena, total;
+};
+
+ScratchLayout scratch_layout(int nw, int nk, int nn, Layout& lay) {
+  ScratchLayout s;
+  s.items = align_up(sizeof(Prob));
+  s.plist = s.items + align_up(sizeof(Item) * (size_t)nk);
+  s.raw = s.plist + align_up(sizeof(int));
+  s.arena = s.raw + align_up(sizeof(int) * (size_t)nk * nn);
+  s.total = s.arena + problem_layout(nw, nk, nn, 0, lay);
+  return s;
+}
+
 }  // namespace
 
 // =============================================================== C ABI
@@ -1236,9 +1298,12 @@ int w90gpu_create(int device, w90gpu_ctx** ctx) {
       return W90GPU_ERR_FATAL;
     }
   }
+  char* sp = nullptr;
+  if (scratch_address(device, &sp) != W90GPU_OK) return W90GPU_ERR_FATAL;
   w90gpu_ctx* c = new (std::nothrow) w90gpu_ctx();
   if (!c) return W90GPU_ERR_ALLOC;
   c->device = device;
+  c->scratch = sp;
   *ctx = c;
   return W90GPU_OK;
 }
@@ -1606,6 +1671,91 @@ int w90gpu_omega(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* m_ma
   rc = run();
   w90gpu_session_destroy(s);
   return rc;
+}
+
+// w90gpu_omega with every device buffer in the static scratch array: the
+// same kernels and launch sequence (initial wann_omega, MODE_INIT), on the
+// null stream, with no allocation.  Calls on one device share the array,
+// so calls that overlap on one GPU race; every call rewrites what it reads
+// (the loop state from its template, U[0] cleared as in w90gpu_omega).
+int w90gpu_omega_noalloc(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* m_matrix, int first_pass,
+                         double omega_invariant, double* rave, double* r2ave, double* omega) {
+  if (!ctx || !m_matrix) return W90GPU_ERR_INPUT;
+  if (!ctx->scratch) {
+    set_error(ctx, "w90gpu_omega_noalloc: no device scratch on this context");
+    return W90GPU_ERR_FATAL;
+  }
+  w90gpu_control ctl;
+  w90gpu_control_defaults(&ctl);
+  ctl.num_iter = 0;
+  int rc = validate(ctx, prob, &ctl);
+  if (rc) return rc;
+  const int nw = prob->num_wann, nk = prob->num_kpts, nn = prob->nntot;
+  Layout lay;
+  const ScratchLayout sl = scratch_layout(nw, nk, nn, lay);
+  if (sl.total > kScratchBytes) {
+    set_error(ctx,
+              "w90gpu_omega_noalloc: num_wann=%d, num_kpts=%d, nntot=%d needs %zu bytes of device scratch; the "
+              "static scratch holds %zu (use w90gpu_omega)",
+              nw, nk, nn, sl.total, kScratchBytes);
+    return W90GPU_ERR_INPUT;
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  char* base = ctx->scratch;
+  Member m;
+  m.nw = nw;
+  m.nk = nk;
+  m.nn = nn;
+  m.cls = wann_class(nw);
+  m.ctl = ctl;
+  m.wbtot = prob->wbtot;
+  member_bind(&m, base + sl.arena, lay);
+  const State st0 = state_template(&m, first_pass ? 1 : 0, omega_invariant);
+  const size_t mb = (size_t)nw * nw * nn * nk * sizeof(cplx);
+  const size_t nd = (size_t)nw * nw * nk;
+  Item* items = (Item*)(base + sl.items);
+  int* plist = (int*)(base + sl.plist);
+  int* raw = (int*)(base + sl.raw);
+  HIPCHK(ctx, hipMemcpy(base, &m.p, sizeof(Prob), hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(raw, prob->nnlist, sizeof(int) * (size_t)nk * nn, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy((void*)m.p.bk, prob->bk, sizeof(double) * 3 * (size_t)nk * nn, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy((void*)m.p.wb, prob->wb, sizeof(double) * (size_t)nn, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(m.st_init, &st0, sizeof(State), hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(m.p.M[0], m_matrix, mb, hipMemcpyHostToDevice));
+  const int nt = std::max(nk * nn, nk);
+  hipLaunchKernelGGL(k_noalloc_tables, dim3((nt + 255) / 256), dim3(256), 0, nullptr, items, plist, (int*)m.p.nnl,
+                     raw, nk, nn);
+  HIPCHK(ctx, hipGetLastError());
+  Engine E;
+  E.ctx = ctx;
+  E.stream = nullptr;
+  E.d_probs = (Prob*)base;
+  Lists L;
+  L.items = items;
+  L.plist = plist;
+  L.np = 1;
+  // the segments build_lists gives a single member: empty before and after
+  // its class, all nk items in (class, FP64)
+  for (int c = 0; c < kClasses; ++c)
+    for (int q = 0; q < 2; ++q) L.seg[c][q].off = (c > m.cls || (c == m.cls && q == 1)) ? nk : 0;
+  L.seg[m.cls][0] = Seg{0, nk, nn, npanels(nw)};
+  L.any_group = m.p.ngroup > 0;
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, nullptr, E.d_probs, L.plist, 1);
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(1024, (nd + 255) / 256)), dim3(256), 0, nullptr,
+                     m.p.U[0], nd);
+  HIPCHK(ctx, hipGetLastError());
+  if ((rc = launch_omega(&E, L, MODE_INIT))) return rc;
+  State st;
+  HIPCHK(ctx, hipMemcpy(&st, m.p.st, sizeof(State), hipMemcpyDeviceToHost));
+  if (rave) HIPCHK(ctx, hipMemcpy(rave, m.p.rave[0], 3 * (size_t)nw * sizeof(double), hipMemcpyDeviceToHost));
+  if (r2ave) HIPCHK(ctx, hipMemcpy(r2ave, m.p.r2ave[0], (size_t)nw * sizeof(double), hipMemcpyDeviceToHost));
+  if (omega) {
+    omega[0] = st.wann.om_i;
+    omega[1] = st.wann.om_d;
+    omega[2] = st.wann.om_od;
+    omega[3] = st.wann.om_tot;
+  }
+  return W90GPU_OK;
 }
 
 int w90gpu_domega(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* m_matrix, double* cdodq) {
