@@ -36,7 +36,6 @@ extern "C" {
 
 #define W90GPU_NUM_NNMAX 12      /* types.F90:116 num_nnmax */
 #define W90GPU_MAX_WANN 128      /* largest num_wann the kernels accept */
-#define W90GPU_MAX_CONV_WINDOW 64
 
 /* arithmetic of the rotation / gradient pass */
 #define W90GPU_PREC_FP64 0       /* plain drop-in: everything in FP64 */

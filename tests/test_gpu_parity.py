@@ -220,6 +220,23 @@ def test_pinned_prefix_matches_oracle(ctx, name):
     assert np.abs(r["history"][:, 0] - o["history"][:, 0]).max() <= 1e-8
 
 
+@pytest.mark.parametrize("name", ["ref_basic1", "ref_example05"])
+def test_convergence_window_matches_oracle(ctx, name):
+    """The convergence test of internal_test_convergence for windows of 3 to
+    150 iterations (no cap on conv_window): the solve stops at the oracle's
+    iteration, which keeps the reference's window of Omega changes, with its
+    Omega."""
+    c = Case(name)
+    op = oracle_problem(c)
+    for cw, tol in ((3, 1e-6), (65, 1e-7), (80, 1e-8), (150, 1e-5)):
+        print(f"[conv_window] {name} window {cw} tol {tol:.0e}", flush=True)
+        over = dict(num_iter=600, conv_window=cw, conv_tol=tol)
+        r = gpu_solve(ctx, c, **over)
+        o = po.wann_main(op, c.oracle_control(**over), c.m0, c.u0)
+        assert (r["res"].iterations, bool(r["res"].converged)) == (o["iterations"], o["converged"])
+        assert abs(r["res"].omega_total - o["omega"][3]) <= 1e-8
+
+
 def test_benchmark_workload_matches_reference(ctx):
     """The benchmark problem (fcc, 16 WFs, 512 k, nntot 8, 500 iterations),
     regenerated from its seed: Omega after each of the 500 iterations, the

@@ -360,9 +360,10 @@ def test_oracle_lowdin_matches_reference(name):
     assert np.abs(m1 - c.m0).max() < 1e-12
 
 
-def test_generator_workers_do_not_load_the_gpu_runtime():
-    """Problem generation (k-mesh search included) runs in worker processes
-    that load only the host library: no HIP runtime in the process."""
+def test_generator_does_not_load_the_gpu_runtime():
+    """Problem generation (k-mesh search included) loads only the host
+    library: no HIP runtime in the process; threads give the results of a
+    sequential run."""
     import subprocess
     import sys
 
@@ -375,6 +376,12 @@ def test_generator_workers_do_not_load_the_gpu_runtime():
     assert out.returncode == 0, out.stderr
     nw, nn, hip, gpu, host = out.stdout.split()
     assert (hip, gpu, host) == ("False", "False", "True")
+    import numpy as np
+
     from wannier90_amd import workloads
 
-    assert workloads.MAX_WORKERS + 1 <= 15
+    plist = [workloads.batch_params(i) for i in (15, 19, 28, 29)]
+    a = workloads.generate(plist, workers=1)
+    b = workloads.generate(plist, workers=4)
+    for x, y in zip(a, b):
+        assert np.array_equal(x["m0"], y["m0"]) and np.array_equal(x["u0"], y["u0"])

@@ -180,6 +180,7 @@ struct Engine {
   bool graph_failed = false;
   bool use_graph = true;
   bool use_lean = true;           // lean blocks allowed (W90GPU_NO_LEAN unset)
+  bool rot_direct64 = false;      // 33-64 bands: direct-operand rotation instead of the LDS-staged one
   int graph_block = kGraphBlock;  // iterations per block
   hipEvent_t blk_ev[2] = {nullptr, nullptr};
   std::vector<int> blk_slots[2];
@@ -307,10 +308,6 @@ int validate(w90gpu_ctx* ctx, const w90gpu_problem* prob, const w90gpu_control* 
     set_error(ctx, "precond, guiding centres, selective localisation and conv_noise are not on this path");
     return W90GPU_ERR_INPUT;
   }
-  if (ctl->conv_window > W90GPU_MAX_CONV_WINDOW) {
-    set_error(ctx, "conv_window=%d exceeds %d", ctl->conv_window, W90GPU_MAX_CONV_WINDOW);
-    return W90GPU_ERR_INPUT;
-  }
   if (ctl->num_iter < 0) {
     set_error(ctx, "num_iter < 0");
     return W90GPU_ERR_INPUT;
@@ -432,6 +429,7 @@ int engine_create(w90gpu_ctx* ctx, size_t pool_bytes, Engine** out) {
   }
   E->use_graph = std::getenv("W90GPU_NO_GRAPH") == nullptr;
   E->use_lean = std::getenv("W90GPU_NO_LEAN") == nullptr;
+  E->rot_direct64 = std::getenv("W90GPU_ROT_DIRECT64") != nullptr;
   if (const char* gb = std::getenv("W90GPU_GRAPH_BLOCK")) E->graph_block = std::min(64, std::max(1, std::atoi(gb)));
   if (grow_tables(E, 4, 64)) {
     engine_destroy(E);
@@ -492,6 +490,7 @@ void member_bind(Member* m, char* a, const Layout& lay) {
   p.E = (cplx*)(a + lay.off[i++]);
   p.W1 = (cplx*)(a + lay.off[i++]);
   p.W2 = (cplx*)(a + lay.off[i++]);
+  p.ET = p.W1;
   p.ln[0] = (double*)(a + lay.off[i++]);
   p.ln[1] = (double*)(a + lay.off[i++]);
   p.d2 = (double*)(a + lay.off[i++]);
@@ -746,8 +745,14 @@ int launch_rotate(Engine* E, const Lists& L, int mode) {
             if (q == 0) hipLaunchKernelGGL((k_reduce_tiles<cplx, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
             else hipLaunchKernelGGL((k_reduce_tiles<float2, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
           } else if (c == 2) {
-            if (q == 0) hipLaunchKernelGGL((k_rot_fused<cplx, 64>), gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
-            else hipLaunchKernelGGL((k_rot_fused<float2, 64>), gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
+            if (!E->rot_direct64) {
+              if (q == 0) hipLaunchKernelGGL((k_rot_fused<cplx, 64>), gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
+              else hipLaunchKernelGGL((k_rot_fused<float2, 64>), gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
+            } else if (q == 0) {
+              hipLaunchKernelGGL(k_rot_direct<cplx>, gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
+            } else {
+              hipLaunchKernelGGL(k_rot_direct<float2>, gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
+            }
           } else if (q == 0) {
             hipLaunchKernelGGL((k_rot_gemm<cplx, 1, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
             hipLaunchKernelGGL((k_rot_gemm<cplx, 2, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
@@ -1847,6 +1852,8 @@ int w90gpu_project(w90gpu_ctx* ctx, const w90gpu_problem* prob, const double* a_
     HIPCHK(ctx, hipMemcpyAsync(m->p.M[0], m_matrix, mb, hipMemcpyHostToDevice, E->stream));
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, E->stream, E->d_probs, L.plist, 1);
     hipLaunchKernelGGL(k_polar, dim3(n), dim3(W90_NT), 0, E->stream, E->d_probs, L.items, n);
+    HIPCHK(ctx, hipGetLastError());
+    hipLaunchKernelGGL(k_transpose_e, dim3(n), dim3(W90_NT), 0, E->stream, E->d_probs, L.items, n);
     HIPCHK(ctx, hipGetLastError());
     hipLaunchKernelGGL(k_set_direct, dim3(1), dim3(64), 0, E->stream, E->d_probs + m->slot, 1.0);
     if ((r = launch_rotate(E, L, MODE_SECOND))) return r;

@@ -201,7 +201,8 @@ struct State {
   double beta, dir_sign, scale;
   double omega_invariant;
   Spread wann, old, trial;
-  double history[W90GPU_MAX_CONV_WINDOW];
+  int conv_run;    // iterations in a row, up to this one, with |dOmega| <= conv_tol
+  int pad_run;
 };
 
 // Device view of one problem: sizes and pointers into its HBM arena.
@@ -223,6 +224,9 @@ struct Prob {
   cplx* E;            // exp(dW) per k
   cplx* W1;           // workspace per k
   cplx* W2;           // workspace per k
+  cplx* ET;           // [k][nw*nw] exp(dW) transposed (row-major E), the operand layout of
+                      // k_rot_direct; shares W1's memory (the retraction above 64 bands
+                      // uses W1 as a workspace and its rotation reads E)
   double* ln[2];      // [k][b][n] Im ln M_nn of each buffer
   double* d2;         // [k][b][n] |M_nn|^2 (MFMA path)
   double* part_od;    // [k][b][tile] (MFMA path)
@@ -594,21 +598,15 @@ __device__ void conv_step(State* st, double* logrow, double gc) {
     logrow[2] = st->wann.om_od;
     logrow[3] = gc;
   }
+  // internal_test_convergence (wannierise.F90:1076-1164) keeps the last
+  // conv_window changes of Omega and stops when none exceeds conv_tol after
+  // conv_window tests: the same as a run of conv_window small changes in a
+  // row, which needs no history of any length ("not greater", so a NaN
+  // change counts as small, as it does there)
   if (st->conv_window > 1) {
-    const int cw = st->conv_window;
-    if (st->conv_count < cw) {
-      st->history[st->conv_count] = delta;
-    } else {
-      for (int j = 0; j < cw - 1; ++j) st->history[j] = st->history[j + 1];
-      st->history[cw - 1] = delta;
-    }
     st->conv_count += 1;
-    if (st->conv_count >= cw) {
-      bool ok = true;
-      for (int j = 0; j < cw; ++j)
-        if (fabs(st->history[j]) > st->conv_tol) ok = false;
-      if (ok) st->converged = 1;
-    }
+    st->conv_run = (fabs(delta) > st->conv_tol) ? 0 : st->conv_run + 1;
+    if (st->conv_count >= st->conv_window && st->conv_run >= st->conv_window) st->converged = 1;
   }
   if (st->converged || iter >= st->num_iter) st->done = 1;
 }
@@ -728,7 +726,7 @@ __device__ void omega_control(State* st, const Prob& p, double sod, double si, d
     st->hold = 0;
     st->ncg_s[st->iter & 1] = st->num_cg_steps;
     st->conv_count = 0;
-    for (int j = 0; j < W90GPU_MAX_CONV_WINDOW; ++j) st->history[j] = 0.0;
+    st->conv_run = 0;
     if (p.log) {
       double* row = p.log + 4 * (size_t)st->iter;
       row[0] = sp.om_tot;
@@ -1392,6 +1390,17 @@ __device__ void wg_gemm_cn(cplx* C, const cplx* A, const cplx* B, int n) {
   __syncthreads();
 }
 
+// ET(k) = E(k)^T for one k by one workgroup (E written by this workgroup
+// before; the caller synchronised it): coalesced writes, reads along rows.
+__device__ void wg_transpose_e(const Prob& p, int k) {
+  const int nw = p.nw;
+  const size_t o = (size_t)k * p.nw2;
+  for (int e = threadIdx.x; e < p.nw2; e += blockDim.x) {
+    const int c = e % nw, r = e / nw;
+    p.ET[o + c + (size_t)r * nw] = p.E[o + r + (size_t)c * nw];
+  }
+}
+
 __global__ void __launch_bounds__(W90_NT) k_delta_u(const Prob* probs, const Item* items, int nitems) {
   if (blockIdx.x >= (unsigned)nitems) return;
   const Item it = items[blockIdx.x];
@@ -1400,6 +1409,15 @@ __global__ void __launch_bounds__(W90_NT) k_delta_u(const Prob* probs, const Ite
   if (!gate(st, MODE_REMIX)) return;
   const size_t o = (size_t)it.k * p.nw2;
   wg_gemm_cn(p.E + o, p.U0 + o, p.U[st->cur] + o, p.nw);
+  wg_transpose_e(p, it.k);
+}
+
+// ET = E^T for every item (E formed outside the retraction kernels: the
+// polar factor of the projection).
+__global__ void __launch_bounds__(W90_NT) k_transpose_e(const Prob* probs, const Item* items, int nitems) {
+  if (blockIdx.x >= (unsigned)nitems) return;
+  const Item it = items[blockIdx.x];
+  wg_transpose_e(probs[it.prob], it.k);
 }
 
 // C = A * B (with ident: I + scale A B) on FP64 MFMA for matrices in global

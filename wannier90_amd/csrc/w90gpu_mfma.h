@@ -556,6 +556,243 @@ __global__ void __launch_bounds__(256, 2) k_rot_fused(const Prob* probs, const I
   rot_fused_tile<T2, GT>(p, it.k, b, mode, cur, lds, red);
 }
 
+// ------------------------------------------------ rotation, direct operands
+
+// K steps of global operands in flight per wavefront in k_rot_direct.
+#ifndef W90_RD_PF
+#define W90_RD_PF 4
+#endif
+
+// The rotation of block (k, b) for num_wann <= 64 by one workgroup, with
+// every global operand loaded straight into MFMA fragments, coalesced:
+// the products are taken in the order P = M(k,b) E_{k+b}, then
+// R = E_k^+ P, so that M (column-major) is the A operand, whose fragment
+// rows are consecutive in memory, and E enters through its transpose ET
+// (rows of E consecutive: the B operand of the first product, the
+// conjugated A operand of the second).  Operands of W90_RD_PF K steps are
+// in flight ahead of the MFMA products; P goes to LDS once (planar,
+// row-major), the only barrier between the products; R goes through the
+// same LDS for column-major stores of whole blocks.  Rows and columns
+// outside num_wann are read at clamped (valid) addresses and zeroed.
+// Epilogue as k_rot_fused: Im ln M_nn, |M_nn|^2, sum |M_mn|^2 and the
+// off-diagonal sum of the block.
+template <class T2>
+__device__ __forceinline__ void rot_direct_tile(const Prob& p, int k, int b, int mode, int cur,
+                                                typename Real<T2>::type* Ps, double* red) {
+  constexpr int GT = 64, LD = GT + 1, FR = 2, PF = W90_RD_PF;
+  typedef typename Real<T2>::type R;
+  const int nw = p.nw, nn = p.nn;
+  const size_t kb = (size_t)k * nn + b;
+  const size_t nw2 = p.nw2;
+  const int dst = dst_buf(mode, cur);
+  const cplx* ETk = p.ET + (size_t)k * nw2;
+  const cplx* ETkb = p.ET + (size_t)p.nnl[kb] * nw2;
+  const T2* src = rot_src<T2>(p, mode, cur) + kb * nw2;
+  T2* out = mbuf<T2>(p, dst) + kb * nw2;
+  double* lnd = p.ln[dst] + kb * nw;
+  double* d2d = p.d2 + kb * nw;
+
+  const int tid = threadIdx.x;
+  const int w = tid >> 6, lane = tid & 63;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int nks = (nw + 3) >> 2;
+  // this lane's A rows and B columns, clamped into the block
+  int ri[FR], ci[FR];
+  bool rv[FR], cv[FR];
+#pragma unroll
+  for (int f = 0; f < FR; ++f) {
+    const int r = wr + 16 * f + lr, c = wc + 16 * f + lr;
+    rv[f] = r < nw;
+    cv[f] = c < nw;
+    ri[f] = rv[f] ? r : 0;
+    ci[f] = cv[f] ? c : 0;
+  }
+  Cfrag<R> C[FR][FR];
+#pragma unroll
+  for (int fi = 0; fi < FR; ++fi)
+#pragma unroll
+    for (int fj = 0; fj < FR; ++fj) C[fi][fj].zero();
+
+  // ---- P = M E_{k+b}: A[r][l] = src[r + l nw], B[l][c] = ETkb[c + l nw]
+  T2 qa[PF][FR];
+  cplx qb[PF][FR];
+#pragma unroll
+  for (int j = 0; j < PF; ++j) {
+    const int l = 4 * j + lk;
+    const size_t lo = (size_t)(l < nw ? l : 0) * nw;
+#pragma unroll
+    for (int f = 0; f < FR; ++f) {
+      qa[j][f] = src[ri[f] + lo];
+      qb[j][f] = ETkb[ci[f] + lo];
+    }
+  }
+  for (int s0 = 0; s0 < nks; s0 += PF) {
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+      const int s = s0 + j;
+      if (s < nks) {
+        const bool lv = 4 * s + lk < nw;
+        R ar[FR], ai[FR], br[FR], bi[FR];
+#pragma unroll
+        for (int f = 0; f < FR; ++f) {
+          const bool va = lv && rv[f], vb = lv && cv[f];
+          ar[f] = va ? (R)qa[j][f].x : (R)0;
+          ai[f] = va ? (R)qa[j][f].y : (R)0;
+          br[f] = vb ? (R)qb[j][f].x : (R)0;
+          bi[f] = vb ? (R)qb[j][f].y : (R)0;
+        }
+        const int sn = s + PF;
+        if (sn < nks) {
+          const int l = 4 * sn + lk;
+          const size_t lo = (size_t)(l < nw ? l : 0) * nw;
+#pragma unroll
+          for (int f = 0; f < FR; ++f) {
+            qa[j][f] = src[ri[f] + lo];
+            qb[j][f] = ETkb[ci[f] + lo];
+          }
+        }
+#pragma unroll
+        for (int fi = 0; fi < FR; ++fi)
+#pragma unroll
+          for (int fj = 0; fj < FR; ++fj) C[fi][fj].mma(ar[fi], ai[fi], br[fj], bi[fj]);
+      }
+    }
+  }
+  // P to LDS, row-major: P[r][c] at Ps[(part * GT + r) * LD + c]
+#pragma unroll
+  for (int fi = 0; fi < FR; ++fi)
+#pragma unroll
+    for (int fj = 0; fj < FR; ++fj)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = wr + fi * 16 + Mfma<R>::row(lk, i);
+        const int c = wc + fj * 16 + lr;
+        Ps[(0 * GT + r) * LD + c] = C[fi][fj].re(i);
+        Ps[(1 * GT + r) * LD + c] = C[fi][fj].im(i);
+      }
+#pragma unroll
+  for (int fi = 0; fi < FR; ++fi)
+#pragma unroll
+    for (int fj = 0; fj < FR; ++fj) C[fi][fj].zero();
+
+  // ---- R = E_k^+ P: A[r][l] = conj(ETk[r + l nw]), B[l][c] = P[l][c]
+  cplx qe[PF][FR];
+#pragma unroll
+  for (int j = 0; j < PF; ++j) {
+    const int l = 4 * j + lk;
+    const size_t lo = (size_t)(l < nw ? l : 0) * nw;
+#pragma unroll
+    for (int f = 0; f < FR; ++f) qe[j][f] = ETk[ri[f] + lo];
+  }
+  __syncthreads();
+  for (int s0 = 0; s0 < nks; s0 += PF) {
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+      const int s = s0 + j;
+      if (s < nks) {
+        const int l = 4 * s + lk;
+        const bool lv = l < nw;
+        R ar[FR], ai[FR], br[FR], bi[FR];
+#pragma unroll
+        for (int f = 0; f < FR; ++f) {
+          const bool va = lv && rv[f];
+          ar[f] = va ? (R)qe[j][f].x : (R)0;
+          ai[f] = va ? -(R)qe[j][f].y : (R)0;
+          br[f] = Ps[(0 * GT + l) * LD + wc + 16 * f + lr];
+          bi[f] = Ps[(1 * GT + l) * LD + wc + 16 * f + lr];
+        }
+        const int sn = s + PF;
+        if (sn < nks) {
+          const int ln = 4 * sn + lk;
+          const size_t lo = (size_t)(ln < nw ? ln : 0) * nw;
+#pragma unroll
+          for (int f = 0; f < FR; ++f) qe[j][f] = ETk[ri[f] + lo];
+        }
+#pragma unroll
+        for (int fi = 0; fi < FR; ++fi)
+#pragma unroll
+          for (int fj = 0; fj < FR; ++fj) C[fi][fj].mma(ar[fi], ai[fi], br[fj], bi[fj]);
+      }
+    }
+  }
+
+  // ---- epilogue: phases, moduli and sums from the accumulators; R through
+  // LDS (column-major) for whole-block stores
+  double od = 0.0, all = 0.0;
+#pragma unroll
+  for (int fi = 0; fi < FR; ++fi)
+#pragma unroll
+    for (int fj = 0; fj < FR; ++fj)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = wr + fi * 16 + Mfma<R>::row(lk, i);
+        const int c = wc + fj * 16 + lr;
+        if (r < nw && c < nw) {
+          T2 v;
+          v.x = C[fi][fj].re(i);
+          v.y = C[fi][fj].im(i);
+          const cplx vw = c_wide(v);
+          const double a2 = c_abs2(vw);
+          all += a2;
+          if (r != c) {
+            od += a2;
+          } else {
+            lnd[r] = atan2(vw.y, vw.x);
+            d2d[r] = a2;
+          }
+        }
+      }
+  __syncthreads();  // every wavefront has read its last B fragment of P
+#pragma unroll
+  for (int fi = 0; fi < FR; ++fi)
+#pragma unroll
+    for (int fj = 0; fj < FR; ++fj)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = wr + fi * 16 + Mfma<R>::row(lk, i);
+        const int c = wc + fj * 16 + lr;
+        Ps[(0 * GT + c) * LD + r] = C[fi][fj].re(i);
+        Ps[(1 * GT + c) * LD + r] = C[fi][fj].im(i);
+      }
+  __syncthreads();
+  for (int e = tid; e < (int)nw2; e += 256) {
+    const int r = e % nw, c = e / nw;
+    T2 v;
+    v.x = Ps[(0 * GT + c) * LD + r];
+    v.y = Ps[(1 * GT + c) * LD + r];
+    out[e] = v;
+  }
+  block_sum2(od, all, red);
+  if (tid == 0) {
+    p.part_od[kb * p.npan] = od;
+    p.part_all[kb * p.npan] = all;
+  }
+}
+
+// The rotation for 32 < num_wann <= 64 with direct operand loads
+// (rot_direct_tile); workgroup placement as k_rot_fused (the nnmax
+// workgroups of one k back to back on one XCD, sharing ET_k in its L2).
+template <class T2>
+__global__ void __launch_bounds__(256, 2) k_rot_direct(const Prob* probs, const Item* items, int nitems,
+                                                       int nnmax, int mode) {
+  typedef typename Real<T2>::type R;
+  const int w = blockIdx.x;
+  const int r = w >> 3;
+  const int item = 8 * (r / nnmax) + (w & 7);
+  const int b = r % nnmax;
+  if (item >= nitems) return;
+  const Item it = items[item];
+  const Prob& p = probs[it.prob];
+  const State* st = p.st;
+  const int cur = gate_cur(st, mode);
+  if (cur < 0) return;
+  if (b >= p.nn) return;
+  __shared__ R Ps[2 * 64 * 65];
+  __shared__ double red[256];
+  rot_direct_tile<T2>(p, it.k, b, mode, cur, Ps, red);
+}
+
 // Omega partial sums of the current M by GEMM tile (MODE_INIT of the MFMA
 // class): the layout the stage-2 epilogue writes, without a rotation.
 template <class T2, int GT>
@@ -808,14 +1045,20 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
         }
     __syncthreads();
   }
-  // E = P to global; U into the X slot (zero outside nw)
+  // E = P to global, and its transpose ET (rows of E, for the rotation's
+  // operand loads); U into the X slot (zero outside nw)
   cplx* Ek = p.E + (size_t)k * nw2;
+  cplx* ETk = p.ET + (size_t)k * nw2;
 #pragma unroll
   for (int i = 0; i < SL; ++i) {
     const int e = tid + 256 * i;
     const int r = e % W90_STEP_N, c = e / W90_STEP_N;
     const int q = c * W90_STEP_LD + r;
-    if (r < nw && c < nw) st_stream(&Ek[r + c * nw], c_make(Pr[q], Pi[q]));
+    if (r < nw && c < nw) {
+      st_stream(&Ek[r + c * nw], c_make(Pr[q], Pi[q]));
+      const int qt = r * W90_STEP_LD + c;  // E[c][r]
+      st_stream(&ETk[r + c * nw], c_make(Pr[qt], Pi[qt]));
+    }
     Xr[q] = upre[i].x;
     Xi[q] = upre[i].y;
   }
