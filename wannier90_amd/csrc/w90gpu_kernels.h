@@ -45,6 +45,9 @@ typedef double2 cplx;
 #define W90_FINAL_NT 1024   // threads of the per-problem omega kernel
 #define W90_MAX_GROUPS 32   // k-ranges of the first omega reduction level
 #define W90_MIXED_RTOL 1e-8 // relative FP32 noise floor of Omega differences in the bulk phase
+#ifndef W90_BULK_STALLS
+#define W90_BULK_STALLS 2   // rejected bulk steps in a row that end the bulk phase
+#endif
 
 // --------------------------------------------------------------- helpers
 
@@ -641,7 +644,7 @@ __device__ void accept_step(State* st, const Spread& sp, int dst, double* logrow
       logrow[2] = st->wann.om_od;
       logrow[3] = st->gcnorm1;
     }
-    if (st->stall >= 2 || st->iter >= st->num_iter) st->hold = 1;
+    if (st->stall >= W90_BULK_STALLS || st->iter >= st->num_iter) st->hold = 1;
     return;
   }
   st->old = st->wann;
