@@ -298,6 +298,29 @@ int w90gpu_batch_fetch(w90gpu_batch* batch, int id, double* m_matrix, double* u_
 int w90gpu_batch_remove(w90gpu_batch* batch, int id);
 int w90gpu_batch_info(w90gpu_batch* batch, size_t* pool_bytes, size_t* used_bytes, int* running, int* queued);
 
+/* ------------------------------------------------ rotation tile shapes
+ *
+ * No counterpart in the reference.  The M rotation has several kernels, by
+ * tile shape, per num_wann class: "w16" (up to 16 bands), "fused32" or
+ * "fused64" (17-32), "fused64" or "direct64" (33-64), "gemm64" or "gemm32"
+ * (65-128).  A problem's kernel is fixed by (num_wann, nntot) when it is
+ * admitted: a choice made earlier in the process (w90gpu_tune_set, or a
+ * measurement of the online tuner), else the table measured on MI355X that
+ * is compiled into the library, else the class default.  With the
+ * environment variable W90GPU_TUNE=1 when a context or batch is created, a
+ * key without a choice is measured at the start of its first solve: every
+ * candidate is timed on the problem's own data, and one replaces the
+ * default only if it is at least 3% faster.  W90GPU_TUNE_FILE=<path> reads
+ * choices from, and appends measurements to, a text file.  Results depend
+ * on the kernel only through rounding. */
+/* Name of the kernel that a problem of this shape admitted now would use
+ * (NUL-terminated, at most len bytes). */
+int w90gpu_tune_get(int num_wann, int nntot, char* name, size_t len);
+/* Fix the kernel of (num_wann, nntot) for problems admitted from now on;
+ * name NULL or "" drops a fixed choice.  W90GPU_ERR_INPUT if the kernel
+ * does not serve num_wann's class. */
+int w90gpu_tune_set(int num_wann, int nntot, const char* name);
+
 #ifdef __cplusplus
 }
 #endif

@@ -115,6 +115,8 @@ SIGNATURES = [
     ("w90gpu_batch_fetch", C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
     ("w90gpu_batch_remove", C.c_int, [_vp, C.c_int]),
     ("w90gpu_batch_info", C.c_int, [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), _ip, _ip]),
+    ("w90gpu_tune_get", C.c_int, [C.c_int, C.c_int, C.c_char_p, C.c_size_t]),
+    ("w90gpu_tune_set", C.c_int, [C.c_int, C.c_int, C.c_char_p]),
 ]
 
 
@@ -183,6 +185,28 @@ class KMesh:
         self.bk = bk          # (3, nn, nk) float64, order='F'
         self.wb = wb          # (nn,)
         self.wbtot = wbtot
+
+
+ROT_KERNELS = ("w16", "fused32", "fused64", "direct64", "gemm64", "gemm32")
+
+
+def tune_get(num_wann, nntot):
+    """Rotation kernel a problem of this shape admitted now would use."""
+    lib = load()
+    buf = C.create_string_buffer(32)
+    rc = lib.w90gpu_tune_get(num_wann, nntot, buf, len(buf))
+    if rc:
+        raise W90Error(rc, lib.w90gpu_last_error(None).decode())
+    return buf.value.decode()
+
+
+def tune_set(num_wann, nntot, name):
+    """Fix the rotation kernel of (num_wann, nntot) for problems admitted
+    from now on; None drops a fixed choice."""
+    lib = load()
+    rc = lib.w90gpu_tune_set(num_wann, nntot, name.encode() if name else None)
+    if rc:
+        raise W90Error(rc, lib.w90gpu_last_error(None).decode())
 
 
 def kmesh_get(real_lattice, kpt_latt, search_shells=36, tol=1e-6, shell_list=None,

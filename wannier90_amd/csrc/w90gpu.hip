@@ -62,6 +62,8 @@ struct Member {
   int slot = -1;
   int nw = 0, nk = 0, nn = 0;
   int cls = 0;     // kernel class by num_wann
+  int rk = 0;      // rotation kernel (RotKernel)
+  bool tune_pending = false;  // rk to be measured at the start of the first solve
   int phase = 1;   // 0: mixed bulk, 1: FP64 (host view)
   int stage = 0;   // 0 idle, 1 queued, 2 running, 3 finished
   w90gpu_control ctl{};
@@ -88,24 +90,92 @@ constexpr int kLdsMaxWann = 32;
 constexpr int kGraphBlock = 16;
 
 int wann_class(int nw) { return nw <= 16 ? 0 : nw <= kLdsMaxWann ? 1 : nw <= W90_STEP_N ? 2 : 3; }
-// per-(k,b) partial sums of the rotation: one per MFMA tile
-int npanels(int nw) { return wann_class(nw) >= 1 ? gemm_tiles(nw) : 1; }
+
+// Rotation kernels by tile shape.  Each class above 16 bands has a default
+// and candidates of other tile shapes:
+//   17-32: fused32 (32 x 32 tile, T in LDS) | fused64 (64 x 64 tile)
+//   33-64: fused64 | direct64 (operands straight into MFMA fragments)
+//   65-128: gemm64 (two launches over 64 x 64 panels) | gemm32 (32 x 32
+//           panels: less padding for 65-96 bands, lower reuse per load)
+// The choice is made per (num_wann, nntot) by the autotuner (rot_choice,
+// tune_member) and fixed for the life of the process, so a problem is
+// rotated by the same kernel alone and in a batch.
+enum RotKernel { RK_W16 = 0, RK_FUSED32, RK_FUSED64, RK_DIRECT64, RK_GEMM64, RK_GEMM32, kRotKernels };
+const char* const kRotName[kRotKernels] = {"w16", "fused32", "fused64", "direct64", "gemm64", "gemm32"};
+// output tile edge (the initial reduction k_reduce_tiles uses the same tiles)
+constexpr int kRotEdge[kRotKernels] = {16, 32, 64, 64, 64, 32};
+
+// Launch groups: (class, rotation kernel) pairs in class order.  Item lists
+// are sorted by class, then storage precision, then group, so that one
+// (class, precision) is a contiguous range for the gradient kernels and one
+// (group, precision) a contiguous range for the rotation.
+constexpr int kGroups = 7;
+constexpr int kGroupClass[kGroups] = {0, 1, 1, 2, 2, 3, 3};
+constexpr int kGroupRot[kGroups] = {RK_W16, RK_FUSED32, RK_FUSED64, RK_FUSED64, RK_DIRECT64, RK_GEMM64, RK_GEMM32};
+
+int group_of(int cls, int rk) {
+  for (int g = 0; g < kGroups; ++g)
+    if (kGroupClass[g] == cls && kGroupRot[g] == rk) return g;
+  return -1;
+}
+int default_rot(int cls) { return cls == 0 ? RK_W16 : cls == 1 ? RK_FUSED32 : cls == 2 ? RK_FUSED64 : RK_GEMM64; }
+int first_group(int cls) {
+  int g = 0;
+  while (kGroupClass[g] != cls) ++g;
+  return g;
+}
+int last_group(int cls) {
+  int g = kGroups - 1;
+  while (kGroupClass[g] != cls) --g;
+  return g;
+}
+
+// Tiles per block M(k,b) of a rotation kernel: the per-(k,b) partial sums
+// of its epilogue.
+int rot_panels(int rk, int nw) {
+  if (rk == RK_W16) return 1;
+  const int t = (nw + kRotEdge[rk] - 1) / kRotEdge[rk];
+  return t * t;
+}
+static_assert(((W90GPU_MAX_WANN + 31) / 32) * ((W90GPU_MAX_WANN + 31) / 32) <= W90_MAX_PANELS,
+              "k_kpart stages every tile sum of a block");
+
+// partial-sum slots per block in a problem's arena: the most that any
+// candidate of its class writes
+int max_panels(int nw) {
+  const int c = wann_class(nw);
+  int mx = 1;
+  for (int g = first_group(c); g <= last_group(c); ++g) mx = std::max(mx, rot_panels(kGroupRot[g], nw));
+  return mx;
+}
 
 // A contiguous range of an item list with the launch shape its kernels need.
 struct Seg {
   int off = 0, n = 0, nnmax = 0, npanmax = 0;
 };
 
-// Device lists of one launch sequence: items sorted by (class, precision),
-// problem indices for the per-problem kernels.
+// Device lists of one launch sequence: items sorted by (class, precision,
+// group), problem indices for the per-problem kernels.
 struct Lists {
   const Item* items = nullptr;
   const int* plist = nullptr;
   int np = 0;
-  Seg seg[kClasses][2];  // [class][0: FP64, 1: FP32]
+  Seg seg[kGroups][2];  // [group][0: FP64, 1: FP32]
   bool any_second = false;
   bool any_group = false;  // some problem uses the two-level omega reduction
 };
+
+// Lists over n items of one group and precision (the segments before it
+// empty at 0, those after it empty at n).
+void one_group_lists(Lists& L, int g0, int q0, int n, int nn, int npan) {
+  for (int c = 0; c < kClasses; ++c)
+    for (int q = 0; q < 2; ++q)
+      for (int g = first_group(c); g <= last_group(c); ++g) {
+        const bool after = c > kGroupClass[g0] || (c == kGroupClass[g0] && (q > q0 || (q == q0 && g > g0)));
+        L.seg[g][q] = Seg{after ? n : 0, 0, 0, 0};
+      }
+  L.seg[g0][q0] = Seg{0, n, nn, npan};
+}
 
 struct Pool {
   char* base = nullptr;
@@ -180,7 +250,6 @@ struct Engine {
   bool graph_failed = false;
   bool use_graph = true;
   bool use_lean = true;           // lean blocks allowed (W90GPU_NO_LEAN unset)
-  bool rot_direct64 = false;      // 33-64 bands: direct-operand rotation instead of the LDS-staged one
   int graph_block = kGraphBlock;  // iterations per block
   hipEvent_t blk_ev[2] = {nullptr, nullptr};
   std::vector<int> blk_slots[2];
@@ -261,8 +330,8 @@ size_t problem_layout(int nw, int nk, int nn, int num_iter, Layout& lay) {
   lay.add(lnb);  // ln[1]
   lay.add(lnb);  // d2
   lay.add(wann_class(nw) >= 3 ? mb : 256);                  // Tm (two-stage rotation scratch)
-  lay.add((size_t)nk * nn * npanels(nw) * sizeof(double));  // part_od
-  lay.add((size_t)nk * nn * npanels(nw) * sizeof(double));  // part_all
+  lay.add((size_t)nk * nn * max_panels(nw) * sizeof(double));  // part_od
+  lay.add((size_t)nk * nn * max_panels(nw) * sizeof(double));  // part_all
   lay.add((size_t)nk * psz * sizeof(double));               // kpart
   lay.add((size_t)W90_MAX_GROUPS * psz * sizeof(double));   // gpart
   lay.add((size_t)nk * sizeof(double));                     // part_gg
@@ -392,7 +461,10 @@ void reap_graphs(Engine* E) {
 
 void engine_destroy(Engine* E);
 
+void tune_refresh();
+
 int engine_create(w90gpu_ctx* ctx, size_t pool_bytes, Engine** out) {
+  tune_refresh();
   *out = nullptr;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   Engine* E = new (std::nothrow) Engine();
@@ -429,7 +501,6 @@ int engine_create(w90gpu_ctx* ctx, size_t pool_bytes, Engine** out) {
   }
   E->use_graph = std::getenv("W90GPU_NO_GRAPH") == nullptr;
   E->use_lean = std::getenv("W90GPU_NO_LEAN") == nullptr;
-  E->rot_direct64 = std::getenv("W90GPU_ROT_DIRECT64") != nullptr;
   if (const char* gb = std::getenv("W90GPU_GRAPH_BLOCK")) E->graph_block = std::min(64, std::max(1, std::atoi(gb)));
   if (grow_tables(E, 4, 64)) {
     engine_destroy(E);
@@ -466,7 +537,7 @@ void member_bind(Member* m, char* a, const Layout& lay) {
   p.nk = m->nk;
   p.nn = m->nn;
   p.nw2 = m->nw * m->nw;
-  p.npan = npanels(m->nw);
+  p.npan = rot_panels(m->rk, m->nw);
   p.psz = 8 * m->nw + 8;
   // two reduction levels only where one workgroup summing all per-k
   // vectors would be slow (over 128 Ki values)
@@ -540,6 +611,90 @@ int upload_state_template(Engine* E, Member* m, int first_pass, double omega_inv
   return W90GPU_OK;
 }
 
+// ------------------------------------------------------------- autotuner
+//
+// Rotation tile shape per (num_wann, nntot).  Sources, first match wins:
+//   W90GPU_ROT_KERNEL=<name>  forces a kernel wherever it serves the class;
+//   choices already made in this process (w90gpu_tune_set, measurements),
+//     or read from W90GPU_TUNE_FILE;
+//   with W90GPU_TUNE=1, a measurement at the start of the first solve of
+//     the key (tune_member), appended to W90GPU_TUNE_FILE if set;
+//   otherwise the table measured on MI355X (w90gpu_tune_table.h), and the
+//     class default where the table has no row.
+// A measured choice is not revised within a process (w90gpu_tune_set may
+// replace any choice); a member keeps the kernel it was admitted with.
+
+#include "w90gpu_tune_table.h"
+
+struct TuneCache {
+  std::mutex mu;
+  bool online = false;
+  int forced = -1;
+  std::string file;
+  std::map<std::pair<int, int>, int> chosen;  // (num_wann, nntot) -> RotKernel
+};
+
+TuneCache& tune_cache() {
+  static TuneCache c;
+  return c;
+}
+
+int rot_by_name(const char* name) {
+  for (int r = 0; r < kRotKernels; ++r)
+    if (std::strcmp(name, kRotName[r]) == 0) return r;
+  return -1;
+}
+
+// Environment (read at every context and engine creation, so that a
+// process may turn the online tuner on and off) and the tuning file (read
+// when its path changes; its choices do not replace ones already made).
+void tune_refresh() {
+  TuneCache& tc = tune_cache();
+  std::lock_guard<std::mutex> lock(tc.mu);
+  const char* on = std::getenv("W90GPU_TUNE");
+  tc.online = on && std::strcmp(on, "0") != 0;
+  const char* fk = std::getenv("W90GPU_ROT_KERNEL");
+  tc.forced = fk ? rot_by_name(fk) : -1;
+  const char* f = std::getenv("W90GPU_TUNE_FILE");
+  const std::string path = f ? f : "";
+  if (path == tc.file) return;
+  tc.file = path;
+  if (path.empty()) return;
+  // lines "num_wann nntot kernel [name=microseconds ...]"; '#' starts a comment
+  if (FILE* fp = std::fopen(path.c_str(), "r")) {
+    char line[512];
+    while (std::fgets(line, sizeof(line), fp)) {
+      int nw = 0, nn = 0;
+      char name[32] = {0};
+      if (line[0] == '#' || std::sscanf(line, "%d %d %31s", &nw, &nn, name) != 3) continue;
+      const int r = rot_by_name(name);
+      if (r >= 0 && nw >= 1 && nw <= W90GPU_MAX_WANN && group_of(wann_class(nw), r) >= 0)
+        tc.chosen.emplace(std::make_pair(nw, nn), r);
+    }
+    std::fclose(fp);
+  }
+}
+
+// The rotation kernel of a problem.  *pending (when given) is set if the
+// key is to be measured at the start of its first solve; until then the
+// class default stands.
+int rot_choice(int nw, int nn, bool* pending) {
+  TuneCache& tc = tune_cache();
+  std::lock_guard<std::mutex> lock(tc.mu);
+  const int c = wann_class(nw);
+  if (pending) *pending = false;
+  if (tc.forced >= 0 && group_of(c, tc.forced) >= 0) return tc.forced;
+  auto it = tc.chosen.find({nw, nn});
+  if (it != tc.chosen.end()) return it->second;
+  if (tc.online) {
+    if (pending && first_group(c) != last_group(c)) *pending = true;
+    return default_rot(c);
+  }
+  for (const TuneRow& r : kTuneTable)
+    if (nw >= r.nw_lo && nw <= r.nw_hi && nn >= r.nn_lo && nn <= r.nn_hi && group_of(c, r.rk) >= 0) return r.rk;
+  return default_rot(c);
+}
+
 // Admit a problem: pool bytes, a slot, its k-mesh tables and state template.
 // M0/U0 are uploaded separately (member_upload).
 int engine_add(Engine* E, const w90gpu_problem* prob, const w90gpu_control* ctl, Member** out) {
@@ -563,6 +718,7 @@ int engine_add(Engine* E, const w90gpu_problem* prob, const w90gpu_control* ctl,
   m->nk = prob->num_kpts;
   m->nn = prob->nntot;
   m->cls = wann_class(m->nw);
+  m->rk = rot_choice(m->nw, m->nn, &m->tune_pending);
   m->ctl = *ctl;
   m->wbtot = prob->wbtot;
   m->off = off;
@@ -632,8 +788,8 @@ void engine_remove(Engine* E, Member* m) {
 
 // ------------------------------------------------------------- lists
 
-// Build lists for a set of members: items sorted by (class, precision).
-// prec_of(m) gives 0 (FP64) or 1 (FP32) for each member.
+// Build lists for a set of members: items sorted by (class, precision,
+// group).  prec_of(m) gives 0 (FP64) or 1 (FP32) for each member.
 template <class F>
 void build_lists(const std::vector<Member*>& ms, F prec_of, std::vector<Item>& items, std::vector<int>& plist,
                  Lists& L) {
@@ -641,17 +797,18 @@ void build_lists(const std::vector<Member*>& ms, F prec_of, std::vector<Item>& i
   plist.clear();
   L = Lists{};
   for (int c = 0; c < kClasses; ++c)
-    for (int q = 0; q < 2; ++q) {
-      Seg& s = L.seg[c][q];
-      s.off = (int)items.size();
-      for (Member* m : ms) {
-        if (m->cls != c || prec_of(m) != q) continue;
-        for (int k = 0; k < m->nk; ++k) items.push_back(Item{m->slot, k});
-        s.nnmax = std::max(s.nnmax, m->nn);
-        s.npanmax = std::max(s.npanmax, npanels(m->nw));
+    for (int q = 0; q < 2; ++q)
+      for (int g = first_group(c); g <= last_group(c); ++g) {
+        Seg& s = L.seg[g][q];
+        s.off = (int)items.size();
+        for (Member* m : ms) {
+          if (m->cls != c || m->rk != kGroupRot[g] || prec_of(m) != q) continue;
+          for (int k = 0; k < m->nk; ++k) items.push_back(Item{m->slot, k});
+          s.nnmax = std::max(s.nnmax, m->nn);
+          s.npanmax = std::max(s.npanmax, rot_panels(m->rk, m->nw));
+        }
+        s.n = (int)items.size() - s.off;
       }
-      s.n = (int)items.size() - s.off;
-    }
   for (Member* m : ms) {
     plist.push_back(m->slot);
     if (!m->ctl.lfixstep) L.any_second = true;
@@ -701,67 +858,78 @@ int aux_lists(Engine* E, const std::vector<Member*>& ms, Lists& L, bool by_phase
 // A range of whole classes [c0, c1] of the item list (both precisions).
 Seg class_range(const Lists& L, int c0, int c1) {
   Seg r;
-  r.off = L.seg[c0][0].off;
-  const Seg& last = L.seg[c1][1];
+  r.off = L.seg[first_group(c0)][0].off;
+  const Seg& last = L.seg[last_group(c1)][1];
   r.n = last.off + last.n - r.off;
   return r;
 }
 
+// The items of one class in one storage precision (every group), with the
+// widest launch shape among them.
+Seg class_prec_range(const Lists& L, int c, int q) {
+  Seg r;
+  r.off = L.seg[first_group(c)][q].off;
+  const Seg& last = L.seg[last_group(c)][q];
+  r.n = last.off + last.n - r.off;
+  for (int g = first_group(c); g <= last_group(c); ++g) {
+    r.nnmax = std::max(r.nnmax, L.seg[g][q].nnmax);
+    r.npanmax = std::max(r.npanmax, L.seg[g][q].npanmax);
+  }
+  return r;
+}
+
+// One rotation kernel over a segment of items in storage precision T2; for
+// MODE_INIT the tile reduction of the current M with the kernel's tiles.
+template <class T2>
+void launch_rot_kernel(hipStream_t s, const Prob* P, const Item* it, const Seg& g, int rk, int mode) {
+  // MFMA GEMM tiles; gridDim.y covers b and the tiles of a block
+  const dim3 g2(g.n, g.nnmax * g.npanmax);
+  // one tile per block: both products in one launch, XCD-aware
+  // one-dimensional grid (see k_rot_fused)
+  const dim3 gb((unsigned)(8 * ((g.n + 7) / 8) * g.nnmax));
+  if (rk == RK_W16) {
+    hipLaunchKernelGGL(k_rotate_w16<T2>, dim3(g.n), dim3(64 * g.nnmax), rotate_w16_lds<T2>(g.nnmax), s, P, it, g.n,
+                       mode);
+    return;
+  }
+  if (mode == MODE_INIT) {
+    if (kRotEdge[rk] == 32) hipLaunchKernelGGL((k_reduce_tiles<T2, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
+    else hipLaunchKernelGGL((k_reduce_tiles<T2, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
+    return;
+  }
+  switch (rk) {
+    case RK_FUSED32:
+      hipLaunchKernelGGL((k_rot_fused<T2, 32>), gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
+      break;
+    case RK_FUSED64:
+      hipLaunchKernelGGL((k_rot_fused<T2, 64>), gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
+      break;
+    case RK_DIRECT64:
+      hipLaunchKernelGGL(k_rot_direct<T2>, gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
+      break;
+    case RK_GEMM64:
+      hipLaunchKernelGGL((k_rot_gemm<T2, 1, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
+      hipLaunchKernelGGL((k_rot_gemm<T2, 2, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
+      break;
+    default:  // RK_GEMM32
+      hipLaunchKernelGGL((k_rot_gemm<T2, 1, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
+      hipLaunchKernelGGL((k_rot_gemm<T2, 2, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
+      break;
+  }
+}
+
 // The rotation of M and the reduction of its diagonal into the per-k
-// partial sums of wann_omega, over every class and precision of the lists.
+// partial sums of wann_omega, over every group and precision of the lists.
 int launch_rotate(Engine* E, const Lists& L, int mode) {
   hipStream_t s = E->stream;
   const Prob* P = E->d_probs;
-  for (int c = 0; c < kClasses; ++c)
+  for (int g = 0; g < kGroups; ++g)
     for (int q = 0; q < 2; ++q) {
-      const Seg& g = L.seg[c][q];
-      if (g.n == 0) continue;
-      const Item* it = L.items + g.off;
-      const dim3 grid(g.n);
-      if (c == 0) {
-        const dim3 blk(64 * g.nnmax);
-        if (q == 0)
-          hipLaunchKernelGGL(k_rotate_w16<cplx>, grid, blk, rotate_w16_lds<cplx>(g.nnmax), s, P, it, g.n, mode);
-        else
-          hipLaunchKernelGGL(k_rotate_w16<float2>, grid, blk, rotate_w16_lds<float2>(g.nnmax), s, P, it, g.n,
-                             mode);
-      } else {
-        // MFMA GEMM tiles; gridDim.y covers b and the tiles of a block
-        const dim3 g2(g.n, g.nnmax * g.npanmax);
-        // classes 1 and 2 (one tile per block): both products in one launch,
-        // XCD-aware one-dimensional grid (see k_rot_fused)
-        const dim3 gb((unsigned)(8 * ((g.n + 7) / 8) * g.nnmax));
-        if (c == 1) {
-          if (mode == MODE_INIT) {
-            if (q == 0) hipLaunchKernelGGL((k_reduce_tiles<cplx, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
-            else hipLaunchKernelGGL((k_reduce_tiles<float2, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
-          } else if (q == 0) {
-            hipLaunchKernelGGL((k_rot_fused<cplx, 32>), gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
-          } else {
-            hipLaunchKernelGGL((k_rot_fused<float2, 32>), gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
-          }
-        } else {
-          if (mode == MODE_INIT) {
-            if (q == 0) hipLaunchKernelGGL((k_reduce_tiles<cplx, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
-            else hipLaunchKernelGGL((k_reduce_tiles<float2, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
-          } else if (c == 2) {
-            if (!E->rot_direct64) {
-              if (q == 0) hipLaunchKernelGGL((k_rot_fused<cplx, 64>), gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
-              else hipLaunchKernelGGL((k_rot_fused<float2, 64>), gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
-            } else if (q == 0) {
-              hipLaunchKernelGGL(k_rot_direct<cplx>, gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
-            } else {
-              hipLaunchKernelGGL(k_rot_direct<float2>, gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
-            }
-          } else if (q == 0) {
-            hipLaunchKernelGGL((k_rot_gemm<cplx, 1, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
-            hipLaunchKernelGGL((k_rot_gemm<cplx, 2, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
-          } else {
-            hipLaunchKernelGGL((k_rot_gemm<float2, 1, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
-            hipLaunchKernelGGL((k_rot_gemm<float2, 2, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
-          }
-        }
-      }
+      const Seg& sg = L.seg[g][q];
+      if (sg.n == 0) continue;
+      const Item* it = L.items + sg.off;
+      if (q == 0) launch_rot_kernel<cplx>(s, P, it, sg, kGroupRot[g], mode);
+      else launch_rot_kernel<float2>(s, P, it, sg, kGroupRot[g], mode);
       HIPCHK(E->ctx, hipGetLastError());
     }
   // per-k partial vectors of the MFMA classes
@@ -815,7 +983,7 @@ int launch_domega(Engine* E, const Lists& L, int mode) {
   const Prob* P = E->d_probs;
   for (int c = 0; c < kClasses; ++c)
     for (int q = 0; q < 2; ++q) {
-      const Seg& g = L.seg[c][q];
+      const Seg g = class_prec_range(L, c, q);
       if (g.n == 0) continue;
       const Item* it = L.items + g.off;
       if (c == 0) {
@@ -872,12 +1040,107 @@ int enqueue_iteration(Engine* E, const Lists& L, bool lean = false) {
   return W90GPU_OK;
 }
 
+// Measure the rotation candidates of a member's class on its own data and
+// fix the choice for its (num_wann, nntot).  Each candidate runs in
+// MODE_BENCH (a trial rotation of the current buffer that writes only the
+// other buffer, the per-(k,b) outputs and the partial sums, all rebuilt by
+// the initial evaluation that follows) over the member's k items, repeated
+// so that one launch holds at least kTuneItems items, as in a batch; five
+// timed launches after one untimed, twice in alternating order, the
+// smaller time kept.  A candidate replaces the class default only when it
+// is at least 3% faster.  The member's state must be valid (cur 0 or 1):
+// engine_add uploads the template, so it is from admission on.
+constexpr int kTuneItems = 2048;
+
+int tune_member(Engine* E, Member* m) {
+  const int c = m->cls, nw = m->nw, nn = m->nn;
+  const int reps = std::max(1, (kTuneItems + m->nk - 1) / m->nk);
+  const int n = reps * m->nk;
+  int rc = grow_tables(E, (int)E->slots.size(), n);
+  if (rc) return rc;
+  std::vector<Item> items((size_t)n);
+  for (int r = 0; r < reps; ++r)
+    for (int k = 0; k < m->nk; ++k) items[(size_t)r * m->nk + k] = Item{m->slot, k};
+  HIPCHK(E->ctx, hipMemcpyAsync(E->d_aux_items, items.data(), sizeof(Item) * n, hipMemcpyHostToDevice, E->stream));
+  std::vector<int> cand;
+  for (int g = first_group(c); g <= last_group(c); ++g) cand.push_back(kGroupRot[g]);
+  double t[kRotKernels];
+  for (double& x : t) x = 1e30;
+  auto time_one = [&](int rk) -> int {
+    m->p.npan = rot_panels(rk, nw);
+    HIPCHK(E->ctx, hipMemcpyAsync(E->d_probs + m->slot, &m->p, sizeof(Prob), hipMemcpyHostToDevice, E->stream));
+    Lists L;
+    one_group_lists(L, group_of(c, rk), 0, n, nn, m->p.npan);
+    L.items = E->d_aux_items;
+    int r = launch_rotate(E, L, MODE_BENCH);
+    if (r) return r;
+    HIPCHK(E->ctx, hipEventRecord(E->t0, E->stream));
+    for (int i = 0; i < 5 && !r; ++i) r = launch_rotate(E, L, MODE_BENCH);
+    if (r) return r;
+    HIPCHK(E->ctx, hipEventRecord(E->t1, E->stream));
+    HIPCHK(E->ctx, hipEventSynchronize(E->t1));
+    float ms = 0.f;
+    HIPCHK(E->ctx, hipEventElapsedTime(&ms, E->t0, E->t1));
+    t[rk] = std::min(t[rk], 1e3 * ms / 5.0);
+    return W90GPU_OK;
+  };
+  for (int pass = 0; pass < 2; ++pass)
+    for (size_t i = 0; i < cand.size(); ++i)
+      if ((rc = time_one(cand[pass ? cand.size() - 1 - i : i]))) return rc;
+  const int d = default_rot(c);
+  int pick = d;
+  for (int rk : cand)
+    if (t[rk] < t[pick]) pick = rk;
+  if (pick != d && !(t[pick] <= 0.97 * t[d])) pick = d;
+  m->rk = pick;
+  m->p.npan = rot_panels(pick, nw);
+  HIPCHK(E->ctx, hipMemcpyAsync(E->d_probs + m->slot, &m->p, sizeof(Prob), hipMemcpyHostToDevice, E->stream));
+  HIPCHK(E->ctx, hipStreamSynchronize(E->stream));
+  TuneCache& tc = tune_cache();
+  std::lock_guard<std::mutex> lock(tc.mu);
+  if (!tc.chosen.emplace(std::make_pair(nw, nn), pick).second) return W90GPU_OK;
+  if (!tc.file.empty()) {
+    if (FILE* fp = std::fopen(tc.file.c_str(), "a")) {
+      std::fprintf(fp, "%d %d %s", nw, nn, kRotName[pick]);
+      for (int rk : cand) std::fprintf(fp, " %s=%.2f", kRotName[rk], t[rk]);
+      std::fprintf(fp, " items=%d\n", n);
+      std::fclose(fp);
+    }
+  }
+  return W90GPU_OK;
+}
+
+// Members whose key is still to be measured: the first member of each key
+// is measured, the others take its choice.
+int tune_pending(Engine* E, const std::vector<Member*>& ms) {
+  for (Member* m : ms) {
+    if (!m->tune_pending) continue;
+    m->tune_pending = false;
+    bool again = false;
+    const int rk = rot_choice(m->nw, m->nn, &again);
+    if (!again) {
+      if (rk != m->rk) {
+        m->rk = rk;
+        m->p.npan = rot_panels(rk, m->nw);
+        HIPCHK(E->ctx,
+               hipMemcpyAsync(E->d_probs + m->slot, &m->p, sizeof(Prob), hipMemcpyHostToDevice, E->stream));
+      }
+      continue;
+    }
+    const int rc = tune_member(E, m);
+    if (rc) return rc;
+  }
+  return W90GPU_OK;
+}
+
 // Start the queued members: state from the template, M and U from M0/U0,
 // the initial wann_omega in FP64, and the FP32 copy of M for the mixed
 // bulk phase.
 int enqueue_start(Engine* E, const std::vector<Member*>& ms) {
+  int rc = tune_pending(E, ms);
+  if (rc) return rc;
   Lists L;
-  int rc = aux_lists(E, ms, L);
+  rc = aux_lists(E, ms, L);
   if (rc) return rc;
   hipStream_t s = E->stream;
   hipLaunchKernelGGL(k_reset, dim3((L.np + 255) / 256), dim3(256), 0, s, E->d_probs, L.plist, L.np);
@@ -1282,6 +1545,7 @@ int w90gpu_create(int device, w90gpu_ctx** ctx) {
     set_error(nullptr, "hipSetDevice(%d) failed", device);
     return W90GPU_ERR_FATAL;
   }
+  tune_refresh();
   // dynamic LDS above 64 KiB must be enabled per kernel
   const struct {
     const void* f;
@@ -1328,6 +1592,37 @@ size_t w90gpu_problem_bytes(int num_wann, int num_kpts, int nntot, int num_iter)
 
 size_t w90gpu_session_bytes(int num_wann, int num_kpts, int nntot) {
   return w90gpu_problem_bytes(num_wann, num_kpts, nntot, 0);
+}
+
+// ------------------------------------------------- rotation tile shapes
+
+int w90gpu_tune_get(int num_wann, int nntot, char* name, size_t len) {
+  if (num_wann < 1 || num_wann > W90GPU_MAX_WANN || nntot < 1 || nntot > W90GPU_NUM_NNMAX || !name || len == 0) {
+    set_error(nullptr, "w90gpu_tune_get: invalid num_wann=%d, nntot=%d or buffer", num_wann, nntot);
+    return W90GPU_ERR_INPUT;
+  }
+  std::snprintf(name, len, "%s", kRotName[rot_choice(num_wann, nntot, nullptr)]);
+  return W90GPU_OK;
+}
+
+int w90gpu_tune_set(int num_wann, int nntot, const char* name) {
+  if (num_wann < 1 || num_wann > W90GPU_MAX_WANN || nntot < 1 || nntot > W90GPU_NUM_NNMAX) {
+    set_error(nullptr, "w90gpu_tune_set: invalid num_wann=%d or nntot=%d", num_wann, nntot);
+    return W90GPU_ERR_INPUT;
+  }
+  TuneCache& tc = tune_cache();
+  std::lock_guard<std::mutex> lock(tc.mu);
+  if (!name || !name[0]) {
+    tc.chosen.erase({num_wann, nntot});
+    return W90GPU_OK;
+  }
+  const int r = rot_by_name(name);
+  if (r < 0 || group_of(wann_class(num_wann), r) < 0) {
+    set_error(nullptr, "w90gpu_tune_set: kernel '%s' does not rotate num_wann=%d", name, num_wann);
+    return W90GPU_ERR_INPUT;
+  }
+  tc.chosen[{num_wann, nntot}] = r;
+  return W90GPU_OK;
 }
 
 // ------------------------------------------------------------ sessions
@@ -1712,6 +2007,7 @@ int w90gpu_omega_noalloc(w90gpu_ctx* ctx, const w90gpu_problem* prob, const doub
   m.nk = nk;
   m.nn = nn;
   m.cls = wann_class(nw);
+  m.rk = rot_choice(nw, nn, nullptr);
   m.ctl = ctl;
   m.wbtot = prob->wbtot;
   member_bind(&m, base + sl.arena, lay);
@@ -1740,10 +2036,8 @@ int w90gpu_omega_noalloc(w90gpu_ctx* ctx, const w90gpu_problem* prob, const doub
   L.plist = plist;
   L.np = 1;
   // the segments build_lists gives a single member: empty before and after
-  // its class, all nk items in (class, FP64)
-  for (int c = 0; c < kClasses; ++c)
-    for (int q = 0; q < 2; ++q) L.seg[c][q].off = (c > m.cls || (c == m.cls && q == 1)) ? nk : 0;
-  L.seg[m.cls][0] = Seg{0, nk, nn, npanels(nw)};
+  // its group, all nk items in (group, FP64)
+  one_group_lists(L, group_of(m.cls, m.rk), 0, nk, nn, rot_panels(m.rk, nw));
   L.any_group = m.p.ngroup > 0;
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, nullptr, E.d_probs, L.plist, 1);
   hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(1024, (nd + 255) / 256)), dim3(256), 0, nullptr,

@@ -517,6 +517,10 @@ __device__ __forceinline__ const cplx* rot_factor(const Prob& p, int, int) { ret
 
 // --------------------------------------------------------- M rotation
 
+// Tiles per block M(k,b) of the MFMA rotation at most: 32 x 32 panels over
+// 128 bands.
+#define W90_MAX_PANELS 16
+
 // Per-k partial vector from the per-tile outputs of the MFMA rotation
 // (num_wann above the LDS kernels): diagonal phases and |M_nn|^2, and the
 // tile sums of |M_mn|^2.
@@ -536,7 +540,7 @@ __global__ void __launch_bounds__(W90_NT) k_kpart(const Prob* probs, const Item*
   // every input of the k is loaded in one round: the mesh tables and the
   // tile sums into LDS, the diagonal phases and moduli into registers
   __shared__ double wbL[W90GPU_NUM_NNMAX], bkL[3 * W90GPU_NUM_NNMAX];
-  __shared__ double podL[W90GPU_NUM_NNMAX * 4], pallL[W90GPU_NUM_NNMAX * 4];
+  __shared__ double podL[W90GPU_NUM_NNMAX * W90_MAX_PANELS], pallL[W90GPU_NUM_NNMAX * W90_MAX_PANELS];
   kmesh_stage(p, k, wbL, bkL);
   for (int t = threadIdx.x; t < nn * npan; t += blockDim.x) {
     podL[t] = p.part_od[(size_t)k * nn * npan + t];
