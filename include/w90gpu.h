@@ -298,6 +298,62 @@ int w90gpu_batch_fetch(w90gpu_batch* batch, int id, double* m_matrix, double* u_
 int w90gpu_batch_remove(w90gpu_batch* batch, int id);
 int w90gpu_batch_info(w90gpu_batch* batch, size_t* pool_bytes, size_t* used_bytes, int* running, int* queued);
 
+/* ------------------------------------------------------ disentanglement
+ *
+ * dis_main (src/disentangle.F90:31-284) for k-point meshes, followed by
+ * setup_m_loc (:286-370): the optimally connected num_wann-dimensional
+ * subspace of the num_bands bands inside the outer energy window, with the
+ * bands of the inner (frozen) window kept, found by the iteration of
+ * dis_extract (:1893-2864).  Not covered: gamma_only, site symmetry and
+ * dis_spheres (W90GPU_ERR_INPUT).  Limits: num_bands <= W90GPU_DIS_MAX_BANDS
+ * (each k's subspace problems run in one workgroup's LDS). */
+#define W90GPU_DIS_MAX_BANDS 64
+
+/* dis_control_type and dis_manifold_type (src/wannier90_types.F90:127-136,
+ * src/types.F90:179-195): field meaning and defaults as there. */
+typedef struct {
+  double win_min, win_max;   /* outer window, eV (default: every band)   */
+  double froz_min, froz_max; /* inner window, eV                          */
+  int frozen_states;         /* 1 if dis_froz_max is given (default 0)   */
+  int num_iter;              /* dis_num_iter    (200)                     */
+  double mix_ratio;          /* dis_mix_ratio   (0.5), in (0, 1]          */
+  double conv_tol;           /* dis_conv_tol    (1e-10)                   */
+  int conv_window;           /* dis_conv_window (3)                       */
+} w90gpu_dis_control;
+
+typedef struct {
+  double omega_invariant; /* final Omega_I of the optimal subspace (Ang^2) */
+  double seconds;         /* wall time of the call on the device            */
+  int iterations;         /* dis_extract iterations performed               */
+  int converged;          /* 1 if the dis_conv_window test was satisfied    */
+  int status;             /* status of the call                             */
+} w90gpu_dis_result;
+
+void w90gpu_dis_control_defaults(w90gpu_dis_control* ctl);
+
+/* Disentangle one problem.  prob gives num_wann, num_kpts and the k-mesh
+ * tables (bk is not used).  Inputs, Fortran layout: eigval (num_bands,
+ * num_kpts) in eV, ascending per k; a_matrix (num_bands, num_wann,
+ * num_kpts), the projections that the reference passes in u_matrix_opt;
+ * m_matrix_orig (num_bands, num_bands, nntot, num_kpts).  Outputs:
+ * u_matrix_opt (num_bands, num_wann, num_kpts), rows relative to each
+ * k's window, zero below it; u_matrix (num_wann, num_wann, num_kpts), the
+ * initial U of wannierise (internal_find_u); m_matrix (num_wann, num_wann,
+ * nntot, num_kpts), the overlaps in the gauge of u_matrix_opt and
+ * u_matrix (the input of w90gpu_wann_main); eigval_opt (num_bands,
+ * num_kpts), the eigenvalues of the Hamiltonian in the optimal subspace
+ * in rows 1..num_wann and the window's eigenvalues above; nfirstwin and
+ * ndimwin (num_kpts), the 1-based first band and size of each window;
+ * history (3, rows) per iteration Omega_I(i-1) (the estimate from the new
+ * eigenvalues), Omega_I(i) and the fractional change, as on the DIS lines
+ * of the .wout (may be NULL; rows beyond history_rows are not written).
+ * The error messages of dis_windows, dis_project and dis_proj_froz are
+ * kept. */
+int w90gpu_disentangle(w90gpu_ctx* ctx, const w90gpu_problem* prob, int num_bands, const w90gpu_dis_control* ctl,
+                       const double* eigval, const double* a_matrix, const double* m_matrix_orig,
+                       double* u_matrix_opt, double* u_matrix, double* m_matrix, double* eigval_opt,
+                       int* nfirstwin, int* ndimwin, double* history, int history_rows, w90gpu_dis_result* result);
+
 /* ------------------------------------------------ rotation tile shapes
  *
  * No counterpart in the reference.  The M rotation has several kernels, by

@@ -139,3 +139,69 @@ def wann_main(prob, ctl, m, u):
     it = int(stats[0])
     return dict(m=m, u=u, centres=cen, spreads=spr, omega=om, history=hist[: it + 1],
                 iterations=it, converged=bool(stats[1]))
+
+
+# --------------------------------------------------------- disentanglement
+
+class ODisProblem(C.Structure):
+    _fields_ = [("nb", C.c_int), ("nw", C.c_int), ("nk", C.c_int), ("nn", C.c_int), ("nnlist", _ip),
+                ("wb", _dp), ("wbtot", C.c_double)]
+
+
+class ODisControl(C.Structure):
+    _fields_ = [("win_min", C.c_double), ("win_max", C.c_double), ("froz_min", C.c_double),
+                ("froz_max", C.c_double), ("frozen_states", C.c_int), ("num_iter", C.c_int),
+                ("mix_ratio", C.c_double), ("conv_tol", C.c_double), ("conv_window", C.c_int)]
+
+
+# dis_control / dis_manifold defaults (src/wannier90_types.F90:129-135,
+# src/types.F90:183-189)
+DIS_DEFAULTS = dict(win_min=-1.7976931348623157e308, win_max=1.7976931348623157e308,
+                    froz_min=-1.7976931348623157e308, froz_max=1.7976931348623157e308, frozen_states=0,
+                    num_iter=200, mix_ratio=0.5, conv_tol=1e-10, conv_window=3)
+DIS_ERRORS = {1: "outer window contains no eigenvalues", 2: "window has fewer states than num_wann",
+              3: "more frozen states than num_wann", 4: "unitarity of initial U in dis_project",
+              5: "LAPACK failure", 6: "dis_proj_froz error", 7: "orthonormality of the trial orbitals"}
+
+
+def dis_main(num_bands, num_wann, kmesh, ctl, eig, a, m):
+    """dis_main + setup_m_loc.  eig (nb,nk), a (nb,nw,nk), m (nb,nb,nn,nk).
+    Returns a dict: u_opt (nb,nw,nk), u (nw,nw,nk), m (nw,nw,nn,nk) in the
+    gauge of u (the wannierise input), eigval_opt (nb,nk), nfirstwin and
+    ndimwin (1-based first band, size), ndimfroz, omega_invariant, history
+    (iterations,3): Omega_I estimate, Omega_I, delta; iterations, converged."""
+    lib = load()
+    if not hasattr(lib, "_dis"):
+        lib.oracle_dis_main.argtypes = [C.POINTER(ODisProblem), C.POINTER(ODisControl), _dp, _dp, _dp, _dp, _dp,
+                                        _dp, _dp, _ip, _ip, _ip, _dp, _dp, _ip]
+        lib.oracle_dis_main.restype = C.c_int
+        lib._dis = True
+    nb, nw = num_bands, num_wann
+    nk = eig.shape[1]
+    nn = kmesh.nntot
+    nnl = np.asfortranarray(kmesh.nnlist, dtype=np.int32)
+    wb = np.ascontiguousarray(kmesh.wb, dtype=np.float64)
+    p = ODisProblem(nb, nw, nk, nn, nnl.ctypes.data_as(_ip), _d(wb), float(kmesh.wbtot))
+    c = dict(DIS_DEFAULTS)
+    c.update(ctl)
+    oc = ODisControl(*(c[f] for f, _ in ODisControl._fields_))
+    eig = np.asfortranarray(eig, dtype=np.float64)
+    a = np.array(a, dtype=np.complex128, order="F")
+    m = np.array(m, dtype=np.complex128, order="F")
+    uopt = np.zeros((nb, nw, nk), dtype=np.complex128, order="F")
+    u = np.zeros((nw, nw, nk), dtype=np.complex128, order="F")
+    mw = np.zeros((nw, nw, nn, nk), dtype=np.complex128, order="F")
+    eigopt = np.zeros((nb, nk), order="F")
+    nfirst = np.zeros(nk, dtype=np.int32)
+    ndim = np.zeros(nk, dtype=np.int32)
+    nfroz = np.zeros(nk, dtype=np.int32)
+    om = np.zeros(1)
+    hist = np.zeros((max(int(c["num_iter"]), 1), 3))
+    conv = np.zeros(1, dtype=np.int32)
+    ip = lambda v: v.ctypes.data_as(_ip)  # noqa: E731
+    rc = lib.oracle_dis_main(C.byref(p), C.byref(oc), _d(eig), _d(a), _d(m), _d(uopt), _d(u), _d(mw), _d(eigopt),
+                             ip(nfirst), ip(ndim), ip(nfroz), _d(om), _d(hist), ip(conv))
+    if rc < 0:
+        raise RuntimeError(f"oracle_dis_main: {DIS_ERRORS.get(-rc, -rc)}")
+    return dict(u_opt=uopt, u=u, m=mw, eigval_opt=eigopt, nfirstwin=nfirst, ndimwin=ndim, ndimfroz=nfroz,
+                omega_invariant=float(om[0]), history=hist[:rc], iterations=rc, converged=bool(conv[0]))

@@ -14,6 +14,9 @@ the full run.  Cases too large to store keep only their final scalars and
 centres; the tests regenerate their inputs with wannier90_amd.synth.
 
     python tests/golden/make_golden.py            (needs /root/reference)
+    python tests/golden/make_golden.py dis        (the disentanglement
+        fixtures under tests/golden/dis/: test-suite inputs with
+        num_bands > num_wann, and the reference's postdis and final states)
 """
 import json
 import os
@@ -208,6 +211,102 @@ def synth_dir(tag, prob, params):
     return d
 
 
+DIS_OUT = os.path.join(OUT, "dis")
+CHECKPOINTS = "/root/reference/test-suite/checkpoints"
+
+
+def dis_control_from_win(text):
+    """w90gpu_dis_control fields from the .win keywords (their defaults
+    otherwise, src/wannier90_types.F90:127-136, src/types.F90:179-195)."""
+    big = 1.7976931348623157e308
+    c = dict(win_min=-big, win_max=big, froz_min=-big, froz_max=big, frozen_states=0,
+             num_iter=get_keyword(text, "dis_num_iter", 200), mix_ratio=get_keyword(text, "dis_mix_ratio", 0.5),
+             conv_tol=get_keyword(text, "dis_conv_tol", 1e-10), conv_window=get_keyword(text, "dis_conv_window", 3))
+    for key in ("win_min", "win_max", "froz_min", "froz_max"):
+        if re.search(r"^\s*dis_" + key + r"\s*[=:]", text, re.M | re.I):
+            c[key] = get_keyword(text, "dis_" + key, 0.0)
+    if re.search(r"^\s*dis_froz_max\s*[=:]", text, re.M | re.I):
+        c["frozen_states"] = 1
+    return c
+
+
+def golden_dis_case(name, srcdir, seed, mmn_bz2=None):
+    """One disentanglement fixture from a test-suite input: the raw
+    .eig/.amn/.mmn, the DIS history and postdis state (a num_iter = 0 run:
+    Omega_I, ndimwin, lwindow, U_opt, U, M), and the final state of the full
+    run (wannierise after disentanglement)."""
+    tmp = tempfile.mkdtemp(prefix="w90dis_")
+    for ext in ("win", "amn", "eig", "mmn"):
+        f = os.path.join(srcdir, seed + "." + ext)
+        if os.path.exists(f):
+            shutil.copy(f, tmp)
+    if mmn_bz2:
+        import bz2
+
+        with bz2.open(mmn_bz2) as src, open(os.path.join(tmp, seed + ".mmn"), "wb") as dst:
+            shutil.copyfileobj(src, dst)
+    for f in os.listdir(tmp):
+        os.chmod(os.path.join(tmp, f), 0o644)
+    win = os.path.join(tmp, seed + ".win")
+    text = open(win).read()
+    ctl = control_from_win(text)
+    dctl = dis_control_from_win(text)
+    open(win, "w").write(set_keyword(text, "num_iter", 0))
+    run_ref(tmp, seed)
+    chk0 = w90io.read_chk(os.path.join(tmp, seed + ".chk"))
+    hist, om_final = w90io.read_wout_dis(os.path.join(tmp, seed + ".wout"))
+    open(win, "w").write(text)
+    run_ref(tmp, seed)
+    chk = w90io.read_chk(os.path.join(tmp, seed + ".chk"))
+    cen, spr, om = (None, None, {})
+    if ctl["num_iter"] > 0:
+        cen, spr, om = w90io.read_wout_final(os.path.join(tmp, seed + ".wout"))
+    subprocess.run([WANNIER, "-pp", seed], cwd=tmp, check=True, stdout=subprocess.DEVNULL,
+                   stderr=subprocess.DEVNULL)
+    nn, nnl, nnc = w90io.read_nnkp(os.path.join(tmp, seed + ".nnkp"))
+    lat = chk0["real_lattice"]
+    kpts = chk0["kpt_latt"].T.copy()
+    km = capi.kmesh_get(lat, kpts)
+    assert km.nntot == nn and np.array_equal(km.nnlist, nnl) and np.array_equal(km.nncell, nnc), name
+    nb, nw, nk = chk0["num_bands"], chk0["num_wann"], chk0["num_kpts"]
+    m = capi.read_mmn(os.path.join(tmp, seed + ".mmn"), nb, km)
+    a = capi.read_amn(os.path.join(tmp, seed + ".amn"), nb, nk, nw)
+    eig = np.loadtxt(os.path.join(tmp, seed + ".eig"))[:, 2].reshape(nk, nb).T.copy(order="F")
+    data = dict(
+        real_lattice=lat, kpt_latt=kpts, mp_grid=np.array(chk0["mp_grid"]),
+        nnlist=km.nnlist, nncell=km.nncell, bk=km.bk, wb=km.wb, wbtot=np.float64(km.wbtot),
+        num_bands=np.int64(nb), num_wann=np.int64(nw), eig=eig, a=a, m=m,
+        dis_control=json.dumps(dctl), control=json.dumps(ctl),
+        dis_history=hist, dis_iterations=np.int64(len(hist)), omega_invariant=np.float64(chk0["omega_invariant"]),
+        ndimwin=chk0["ndimwin"], lwindow=chk0["lwindow"], u_opt=chk0["u_matrix_opt"], u_dis=chk0["u_matrix"],
+        m_dis=chk0["m_matrix"],
+        u_final=chk["u_matrix"], m_final=chk["m_matrix"],
+    )
+    if ctl["num_iter"] > 0:
+        data.update(centres=cen, spreads=spr,
+                    omega=np.array([om.get("omega_i", np.nan), om.get("omega_d", np.nan),
+                                    om.get("omega_od", np.nan), om.get("omega_total", np.nan)]),
+                    iterations=np.int64(om["iterations"]), conv_omega=np.array(om["conv_omega"]))
+    os.makedirs(DIS_OUT, exist_ok=True)
+    np.savez_compressed(os.path.join(DIS_OUT, name + ".npz"), **data)
+    shutil.rmtree(tmp, ignore_errors=True)
+    print(f"{name}: nb={nb} nw={nw} nk={nk} nn={nn} dis iterations={len(hist)} "
+          f"Omega_I={chk0['omega_invariant']}", flush=True)
+
+
+def main_dis(which=None):
+    cases = [
+        ("dis_silicon", os.path.join(REF, "testw90_example03"), "silicon",
+         os.path.join(CHECKPOINTS, "si_geninterp", "silicon.mmn.bz2")),
+        ("dis_copper", os.path.join(REF, "testw90_example04"), "copper", None),
+        ("dis_li_window", os.path.join(REF, "testw90_write_u_matrices_disent"), "wannier", None),
+    ]
+    for name, d, seed, bz in cases:
+        if which and name not in which:
+            continue
+        golden_dis_case(name, d, seed, bz)
+
+
 def main(which=None):
     cases = []
     # the reference's own regression inputs on this path
@@ -264,4 +363,8 @@ def main(which=None):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or None)
+    args = sys.argv[1:]
+    if args and args[0] == "dis":
+        main_dis(args[1:] or None)
+    else:
+        main(args or None)

@@ -30,7 +30,7 @@ HIP_FLAGS = [
     f"-I{INCLUDE}",
 ]
 
-HIP_SOURCES = ["w90gpu.hip"]
+HIP_SOURCES = ["w90gpu.hip", "w90dis.hip"]
 CPP_SOURCES = ["kmesh.cpp", "w90io.cpp"]
 
 LIB_NAME = "libw90gpu.so"

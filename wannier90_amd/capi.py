@@ -54,6 +54,18 @@ class Problem(C.Structure):
     ]
 
 
+class DisControl(C.Structure):
+    """w90gpu_dis_control (dis_control_type / dis_manifold_type)."""
+    _fields_ = [("win_min", C.c_double), ("win_max", C.c_double), ("froz_min", C.c_double),
+                ("froz_max", C.c_double), ("frozen_states", C.c_int), ("num_iter", C.c_int),
+                ("mix_ratio", C.c_double), ("conv_tol", C.c_double), ("conv_window", C.c_int)]
+
+
+class DisResult(C.Structure):
+    _fields_ = [("omega_invariant", C.c_double), ("seconds", C.c_double), ("iterations", C.c_int),
+                ("converged", C.c_int), ("status", C.c_int)]
+
+
 class Result(C.Structure):
     _fields_ = [
         ("omega_total", C.c_double), ("omega_invariant", C.c_double), ("omega_d", C.c_double),
@@ -90,6 +102,9 @@ SIGNATURES = [
                                    C.POINTER(Result)]),
     ("w90gpu_omega", C.c_int, [_vp, C.POINTER(Problem), _dp, C.c_int, C.c_double, _dp, _dp, _dp]),
     ("w90gpu_omega_noalloc", C.c_int, [_vp, C.POINTER(Problem), _dp, C.c_int, C.c_double, _dp, _dp, _dp]),
+    ("w90gpu_dis_control_defaults", None, [C.POINTER(DisControl)]),
+    ("w90gpu_disentangle", C.c_int, [_vp, C.POINTER(Problem), C.c_int, C.POINTER(DisControl), _dp, _dp, _dp, _dp,
+                                     _dp, _dp, _dp, _ip, _ip, _dp, C.c_int, C.POINTER(DisResult)]),
     ("w90gpu_domega", C.c_int, [_vp, C.POINTER(Problem), _dp, _dp]),
     ("w90gpu_new_u_and_m", C.c_int, [_vp, C.POINTER(Problem), _dp, _dp, _dp]),
     ("w90gpu_project", C.c_int, [_vp, C.POINTER(Problem), _dp, _dp, _dp]),
@@ -163,6 +178,16 @@ def _d(a):
 
 def _i(a):
     return a.ctypes.data_as(_ip)
+
+
+def dis_control(**kw):
+    c = DisControl()
+    load().w90gpu_dis_control_defaults(C.byref(c))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise KeyError(k)
+        setattr(c, k, v)
+    return c
 
 
 def control(**kw):
@@ -346,6 +371,35 @@ class Context:
         self._check(self.lib.w90gpu_omega(self.h, C.byref(prob), _d(m), 1 if first_pass else 0,
                                           omega_invariant, _d(rave), _d(r2ave), _d(om)))
         return rave, r2ave, om
+
+    def disentangle(self, kmesh, num_bands, ctl, eig, a, m):
+        """w90gpu_disentangle: eig (nb,nk), a (nb,nw,nk), m (nb,nb,nn,nk), ctl
+        a dict of w90gpu_dis_control fields (the rest at their defaults).
+        Returns u_opt, u, m (nw,nw,nn,nk), eigval_opt, nfirstwin, ndimwin,
+        history (iterations,3), omega_invariant, iterations, converged,
+        seconds."""
+        _check_fortran(m, 4)
+        _check_fortran(a, 3)
+        nb, nw, nk = num_bands, a.shape[1], a.shape[2]
+        nn = kmesh.nntot
+        prob = self.problem(nw, kmesh, nk)
+        c = dis_control(**ctl)
+        eig = np.asfortranarray(eig, dtype=np.float64)
+        uopt = np.zeros((nb, nw, nk), dtype=np.complex128, order="F")
+        u = np.zeros((nw, nw, nk), dtype=np.complex128, order="F")
+        mw = np.zeros((nw, nw, nn, nk), dtype=np.complex128, order="F")
+        eigopt = np.zeros((nb, nk), order="F")
+        nfirst = np.zeros(nk, dtype=np.int32)
+        ndim = np.zeros(nk, dtype=np.int32)
+        rows = max(int(c.num_iter), 1)
+        hist = np.zeros((rows, 3))
+        res = DisResult()
+        self._check(self.lib.w90gpu_disentangle(self.h, C.byref(prob), nb, C.byref(c), _d(eig), _d(a), _d(m),
+                                                _d(uopt), _d(u), _d(mw), _d(eigopt), _i(nfirst), _i(ndim),
+                                                _d(hist), rows, C.byref(res)))
+        return dict(u_opt=uopt, u=u, m=mw, eigval_opt=eigopt, nfirstwin=nfirst, ndimwin=ndim,
+                    history=hist[: res.iterations], omega_invariant=res.omega_invariant,
+                    iterations=res.iterations, converged=bool(res.converged), seconds=res.seconds)
 
     def omega_noalloc(self, kmesh, m, first_pass=True, omega_invariant=0.0):
         """omega() through the allocation-free entry point (static device

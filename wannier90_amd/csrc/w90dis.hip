@@ -1,0 +1,1129 @@
+// libw90gpu: disentanglement on MI355X (gfx950) -- dis_main for k-point
+// meshes (src/disentangle.F90:31-284) and setup_m_loc (:286-370), behind
+// w90gpu_disentangle of include/w90gpu.h.
+//
+// Every per-k step is one workgroup per k-point:
+//   k_dis_project     dis_project (:1173-1429): U_opt(k) = the polar factor
+//                     of the projections A restricted to the outer window;
+//   k_dis_proj_froz   dis_proj_froz (:1431-1891): the frozen bands, and the
+//                     leading eigenvectors of Q P Q for the rest;
+//   k_dis_orthonorm   internal_check_orthonorm (:372-451);
+//   k_dis_zmat        internal_zmatrix (:2922-2988) with the Omega_I terms
+//                     of dis_extract (:2069-2104, :2229-2256): for every
+//                     neighbour b, C = M(k,b) U_opt(k+b) once, from which
+//                     Z(k) += w_b C C^+ (non-frozen rows), the spread term
+//                     w_b ||U_opt(k)^+ C||^2 and its frozen rows;
+//   k_dis_update      one extraction step (:2029-2068, :2115-2160): Z mixed
+//                     with the previous one, its leading eigenvectors as the
+//                     new non-frozen columns of U_opt(k);
+//   k_dis_reduce      one workgroup: Omega_I and its estimate over k, the
+//                     change, and the convergence window (:2866-2920);
+//   k_dis_ham         the Hamiltonian in the optimal subspace and U_opt in
+//                     its eigenbasis (:2574-2640);
+//   k_dis_find_u      internal_find_u (:531-698): U(k) = the polar factor of
+//                     U_opt(k)^+ A(k); then W(k) = U_opt(k) U(k);
+//   k_dis_rotate      one workgroup per (k, b): M(k,b) <- W(k)^+ M(k,b)
+//                     W(k+b), the dis_main rotation (:229-241) and
+//                     setup_m_loc in one product.
+// Hermitian eigenproblems (of Z, Q P Q and the subspace Hamiltonian, n <=
+// 64) are solved in LDS by a two-sided cyclic Jacobi method with round-robin
+// (tournament) ordering: n/2 disjoint complex rotations per round, applied
+// to columns and then to rows in two parallel passes; the polar factors (of
+// A and of U_opt^+ A) by the one-sided Jacobi SVD with the same ordering.  The
+// extraction only uses the spanned subspaces and the eigenvalue sums, which
+// do not depend on the eigenvector phases; the final U_opt and U do, so they
+// agree with the reference up to the phase of each Hamiltonian eigenvector
+// (their product U_opt U, and M, agree to rounding).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/w90gpu.h"
+#include "w90gpu_ctx.h"
+
+namespace {
+
+typedef double2 zd;
+
+__device__ __forceinline__ zd zmk(double r, double i) { return make_double2(r, i); }
+__device__ __forceinline__ zd zadd(zd a, zd b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ zd zmul(zd a, zd b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+// conj(a) * b
+__device__ __forceinline__ zd zcmul(zd a, zd b) { return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x); }
+// a * conj(b)
+__device__ __forceinline__ zd zmulc(zd a, zd b) { return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y); }
+__device__ __forceinline__ zd zconj(zd a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ zd zscale(zd a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ double zabs2(zd a) { return a.x * a.x + a.y * a.y; }
+// a += b * c
+__device__ __forceinline__ void zfma(zd& a, zd b, zd c) {
+  a.x = fma(b.x, c.x, fma(-b.y, c.y, a.x));
+  a.y = fma(b.x, c.y, fma(b.y, c.x, a.y));
+}
+// a += conj(b) * c
+__device__ __forceinline__ void zcfma(zd& a, zd b, zd c) {
+  a.x = fma(b.x, c.x, fma(b.y, c.y, a.x));
+  a.y = fma(b.x, c.y, fma(-b.y, c.x, a.y));
+}
+
+constexpr int kNT = 256;             // threads per workgroup
+constexpr int kMaxN = W90GPU_DIS_MAX_BANDS;
+
+// Window of one k: first band (0-based), size, frozen count and the
+// window row of the first frozen band (the frozen bands are contiguous).
+struct Win {
+  int first, ndim, nfroz, fmin;
+};
+
+struct DisState {
+  int iter;        // iterations done
+  int done;        // loop finished (converged or num_iter)
+  int converged;
+  int num_iter;
+  int conv_window;
+  int run;         // trailing iterations with |delta| < conv_tol
+  double mix, conv_tol;
+  double womegai, womegai1, delta;
+};
+
+// error and diagnostic bits (DisProb::flags)
+enum : int {
+  F_PROJECT_UNIT = 1,    // dis_project: U_opt not unitary within 1e-5
+  F_FROZ_EIG = 2,        // dis_proj_froz: eigenvalues of QPQ outside [0, 1]
+  F_FROZ_FIX = 4,        // dis_proj_froz: ortho-fix found too few vectors
+  F_ORTHONORM = 8,       // internal_check_orthonorm
+  F_JACOBI = 16,         // an eigenproblem did not converge in kMaxSweeps
+};
+
+struct DisProb {
+  int nb, nw, nk, nn, ld;  // ld: leading dimension of the LDS matrices (nb + 1)
+  double wbtot;
+  const int* nnl;      // [k * nn + b], 0-based
+  const double* wb;    // [nn]
+  const Win* win;      // [nk]
+  const double* eig;   // (nb, nk)
+  const zd* M;         // (nb, nb, nn, nk), all bands
+  const zd* A;         // (nb, nw, nk), all bands
+  zd* Uopt;            // (nb, nw, nk), rows relative to the window
+  zd* Zin;             // (nb, nb, nk)
+  zd* Zout;            // (nb, nb, nk)
+  zd* U;               // (nw, nw, nk)
+  zd* W;               // (nb, nw, nk): U_opt U
+  zd* Mw;              // (nw, nw, nn, nk)
+  double* wk1;         // [nk] Omega_I estimate terms
+  double* wkom;        // [nk] Omega_I terms
+  double* frz;         // [nk] frozen-row spread terms
+  double* eigopt;      // (nb, nk)
+  double* log;         // (3, num_iter)
+  DisState* st;
+  int* flags;
+};
+
+// window row of the i-th non-frozen state
+__device__ __forceinline__ int nf_row(const Win& w, int i) { return i < w.fmin ? i : i + w.nfroz; }
+
+// Deterministic block reductions (wavefront butterflies, then the four
+// wavefront results in order); red holds kNT / 64 doubles.
+__device__ double block_sum(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int i = 0; i < kNT / 64; ++i) r += red[i];
+  __syncthreads();
+  return r;
+}
+
+// ------------------------------------------------------------ Jacobi
+
+constexpr int kMaxSweeps = 40;
+
+// A complex Jacobi rotation J on (p, q): J = [[c, s], [-conj(s), c]].
+struct JRot {
+  int p, q;
+  double c;
+  zd s;
+};
+
+struct JacobiLds {
+  JRot rot[kMaxN / 2];
+  int count;
+  double red[kNT / 64];
+  double lam[kMaxN];
+  int perm[kMaxN];
+};
+
+// Eigen-decomposition of the Hermitian n x n matrix A (LDS, column-major,
+// leading dimension ld; the upper triangle and the real diagonal are
+// read, A must be stored whole): on return A is diagonal to the Jacobi
+// tolerance and V (LDS) holds the eigenvectors in its columns.  For each
+// pair, D = diag(1, b/|b|) makes the 2 x 2 block real, and the real
+// rotation that diagonalises it (tan 2t = 2|b| / (a_qq - a_pp), the smaller
+// angle) gives J = D R D^+.  A rotation is skipped when |a_pq| <= 1e-14
+// sqrt(|a_pp a_qq|) (relative to the diagonal) or <= 1e-14 ||A||_F (the
+// eigenvalues are then exact to that much of the norm, as LAPACK's; without
+// this floor the rounding noise among the zero eigenvalues of a
+// rank-deficient matrix would be rotated forever); the sweeps end when a
+// whole sweep skips every pair.  Returns the
+// number of sweeps (kMaxSweeps: not converged).
+__device__ int jacobi_eigh(zd* A, zd* V, int n, int ld, JacobiLds& J) {
+  const int t = threadIdx.x;
+  double fro = 0.0;
+  for (int e = t; e < n * n; e += kNT) {
+    const int r = e % n, c = e / n;
+    V[r + c * ld] = zmk(r == c ? 1.0 : 0.0, 0.0);
+    fro += zabs2(A[r + c * ld]);
+  }
+  fro = block_sum(fro, J.red);
+  const double floor2 = fro * 1e-28;
+  const int m = n + (n & 1), half = m / 2;
+  int sweep = 0;
+  for (; sweep < kMaxSweeps; ++sweep) {
+    if (t == 0) J.count = 0;
+    __syncthreads();
+    for (int r = 0; r < m - 1; ++r) {
+      if (t < half) {
+        const int a = (t == 0) ? 0 : 1 + ((t - 1 + r) % (m - 1));
+        const int b = 1 + ((m - 2 - t + r) % (m - 1));
+        JRot R;
+        R.p = min(a, b);
+        R.q = max(a, b);
+        R.c = 1.0;
+        R.s = zmk(0.0, 0.0);
+        if (R.q < n) {
+          const zd apq = A[R.p + R.q * ld];
+          const double b2 = zabs2(apq);
+          const double app = A[R.p + R.p * ld].x, aqq = A[R.q + R.q * ld].x;
+          if (b2 > floor2 && b2 > 1e-28 * fabs(app * aqq)) {
+            const double bn = sqrt(b2);
+            const double tau = (aqq - app) / (2.0 * bn);
+            double tt;
+            if (fabs(tau) > 1e150) tt = 0.5 / tau;
+            else tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            const double c = 1.0 / sqrt(1.0 + tt * tt), s = tt * c;
+            R.c = c;
+            R.s = zmk(s * apq.x / bn, s * apq.y / bn);
+            atomicAdd(&J.count, 1);
+          }
+        }
+        J.rot[t] = R;
+      }
+      __syncthreads();
+      // columns: A J and V J
+      for (int e = t; e < half * n; e += kNT) {
+        const JRot R = J.rot[e / n];
+        if (R.s.x == 0.0 && R.s.y == 0.0) continue;
+        const int row = e % n;
+        const zd sc = zconj(R.s);
+        zd ap = A[row + R.p * ld], aq = A[row + R.q * ld];
+        A[row + R.p * ld] = zadd(zscale(ap, R.c), zscale(zmul(sc, aq), -1.0));
+        A[row + R.q * ld] = zadd(zmul(R.s, ap), zscale(aq, R.c));
+        ap = V[row + R.p * ld];
+        aq = V[row + R.q * ld];
+        V[row + R.p * ld] = zadd(zscale(ap, R.c), zscale(zmul(sc, aq), -1.0));
+        V[row + R.q * ld] = zadd(zmul(R.s, ap), zscale(aq, R.c));
+      }
+      __syncthreads();
+      // rows: J^+ A
+      for (int e = t; e < half * n; e += kNT) {
+        const JRot R = J.rot[e / n];
+        if (R.s.x == 0.0 && R.s.y == 0.0) continue;
+        const int col = e % n;
+        const zd ap = A[R.p + col * ld], aq = A[R.q + col * ld];
+        A[R.p + col * ld] = zadd(zscale(ap, R.c), zscale(zmul(R.s, aq), -1.0));
+        A[R.q + col * ld] = zadd(zmul(zconj(R.s), ap), zscale(aq, R.c));
+      }
+      __syncthreads();
+    }
+    const int cnt = J.count;
+    __syncthreads();
+    if (cnt == 0) break;
+  }
+  return sweep;
+}
+
+// Eigenvalues (the diagonal of A after jacobi_eigh) into J.lam and their
+// ascending order into J.perm (ties by index).
+__device__ void eig_order(const zd* A, int n, int ld, JacobiLds& J) {
+  const int t = threadIdx.x;
+  if (t < n) J.lam[t] = A[t + t * ld].x;
+  __syncthreads();
+  if (t < n) {
+    const double l = J.lam[t];
+    int rank = 0;
+    for (int j = 0; j < n; ++j) rank += (J.lam[j] < l || (J.lam[j] == l && j < t)) ? 1 : 0;
+    J.perm[rank] = t;
+  }
+  __syncthreads();
+}
+
+// Polar factor U = Z V^+ of the rows x nw matrix X (global, leading
+// dimension ldx; rows >= nw), from its singular value decomposition by the
+// one-sided (Hestenes) Jacobi method: the columns of G = X are rotated in
+// pairs until they are orthogonal (|g_p^+ g_q| <= 1e-13 ||g_p|| ||g_q||, or
+// one of the two below 1e-15 ||X||_F),
+// G = X V, and Z = G S^-1 with S the column norms.  Working on X itself
+// (not on X^+ X) keeps small singular values to relative accuracy.  Columns
+// with a singular value at or below 1e-12 of the largest (a rank-deficient
+// X, where the polar factor is not unique, as with the SVD) are completed
+// by Gram-Schmidt from unit vectors.  U goes to global memory (leading
+// dimension ldu).  LDS: G (rows x nw, leading dimension ld), V.
+__device__ void polar_factor(const zd* X, int ldx, int rows, int nw, zd* U, int ldu, zd* G, zd* V, int ld,
+                             JacobiLds& J, int* flags) {
+  const int t = threadIdx.x;
+  double fro = 0.0;
+  for (int e = t; e < rows * nw; e += kNT) {
+    const int r = e % rows, c = e / rows;
+    const zd x = X[r + (size_t)c * ldx];
+    G[r + c * ld] = x;
+    fro += zabs2(x);
+  }
+  // columns below 1e-15 ||X||_F are numerically zero: rotating their
+  // rounding noise against the others would never end
+  const double zero2 = 1e-30 * block_sum(fro, J.red);
+  for (int e = t; e < nw * nw; e += kNT) {
+    const int r = e % nw, c = e / nw;
+    V[r + c * ld] = zmk(r == c ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  const int m = nw + (nw & 1), half = m / 2;
+  int sweep = 0;
+  for (; sweep < kMaxSweeps; ++sweep) {
+    if (t == 0) J.count = 0;
+    __syncthreads();
+    for (int r = 0; r < m - 1; ++r) {
+      // Gram entries of each pair, 8 lanes per pair
+      const int pi = t >> 3, lane = t & 7;
+      double al = 0.0, be = 0.0;
+      zd ga = zmk(0.0, 0.0);
+      int P = 0, Q = 0;
+      bool valid = false;
+      if (pi < half) {
+        const int a = (pi == 0) ? 0 : 1 + ((pi - 1 + r) % (m - 1));
+        const int b = 1 + ((m - 2 - pi + r) % (m - 1));
+        P = min(a, b);
+        Q = max(a, b);
+        valid = Q < nw;
+        if (valid)
+          for (int row = lane; row < rows; row += 8) {
+            const zd gp = G[row + P * ld], gq = G[row + Q * ld];
+            al += zabs2(gp);
+            be += zabs2(gq);
+            zcfma(ga, gp, gq);
+          }
+      }
+      for (int off = 1; off < 8; off <<= 1) {
+        al += __shfl_xor(al, off, 64);
+        be += __shfl_xor(be, off, 64);
+        ga.x += __shfl_xor(ga.x, off, 64);
+        ga.y += __shfl_xor(ga.y, off, 64);
+      }
+      if (pi < half && lane == 0) {
+        JRot R;
+        R.p = P;
+        R.q = Q;
+        R.c = 1.0;
+        R.s = zmk(0.0, 0.0);
+        const double b2 = zabs2(ga);
+        if (valid && b2 > 0.0 && b2 > 1e-26 * al * be && fmin(al, be) > zero2) {
+          const double bn = sqrt(b2);
+          const double tau = (be - al) / (2.0 * bn);
+          double tt;
+          if (fabs(tau) > 1e150) tt = 0.5 / tau;
+          else tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+          const double c = 1.0 / sqrt(1.0 + tt * tt), s = tt * c;
+          R.c = c;
+          R.s = zmk(s * ga.x / bn, s * ga.y / bn);
+          atomicAdd(&J.count, 1);
+        }
+        J.rot[pi] = R;
+      }
+      __syncthreads();
+      for (int e = t; e < half * (rows + nw); e += kNT) {
+        const int i = e / (rows + nw), row = e % (rows + nw);
+        const JRot R = J.rot[i];
+        if (R.s.x == 0.0 && R.s.y == 0.0) continue;
+        zd* M = row < rows ? G : V;
+        const int rr = row < rows ? row : row - rows;
+        const zd ap = M[rr + R.p * ld], aq = M[rr + R.q * ld];
+        M[rr + R.p * ld] = zadd(zscale(ap, R.c), zscale(zmul(zconj(R.s), aq), -1.0));
+        M[rr + R.q * ld] = zadd(zmul(R.s, ap), zscale(aq, R.c));
+      }
+      __syncthreads();
+    }
+    const int cnt = J.count;
+    __syncthreads();
+    if (cnt == 0) break;
+  }
+  if (sweep >= kMaxSweeps && t == 0) atomicOr(flags, F_JACOBI);
+  if (t < nw) {
+    double s2 = 0.0;
+    for (int r = 0; r < rows; ++r) s2 += zabs2(G[r + t * ld]);
+    J.lam[t] = sqrt(s2);
+  }
+  __syncthreads();
+  double smax = 0.0;
+  for (int i = 0; i < nw; ++i) smax = fmax(smax, J.lam[i]);
+  auto good = [&](int i) { return J.lam[i] > 1e-12 * smax && J.lam[i] > 0.0; };
+  for (int e = t; e < rows * nw; e += kNT) {
+    const int r = e % rows, i = e / rows;
+    G[r + i * ld] = good(i) ? zscale(G[r + i * ld], 1.0 / J.lam[i]) : zmk(0.0, 0.0);
+  }
+  __syncthreads();
+  // completion of rank-deficient columns (sequential, rare)
+  if (t == 0) {
+    for (int i = 0; i < nw; ++i) {
+      if (good(i)) continue;
+      for (int e0 = 0; e0 < rows; ++e0) {
+        for (int r = 0; r < rows; ++r) G[r + i * ld] = zmk(r == e0 ? 1.0 : 0.0, 0.0);
+        for (int pass = 0; pass < 2; ++pass)
+          for (int j = 0; j < nw; ++j) {
+            if (j == i || !(good(j) || j < i)) continue;
+            zd d = zmk(0.0, 0.0);
+            for (int r = 0; r < rows; ++r) zcfma(d, G[r + j * ld], G[r + i * ld]);
+            for (int r = 0; r < rows; ++r) zfma(G[r + i * ld], zscale(d, -1.0), G[r + j * ld]);
+          }
+        double nrm = 0.0;
+        for (int r = 0; r < rows; ++r) nrm += zabs2(G[r + i * ld]);
+        if (nrm > 0.25) {
+          const double inv = 1.0 / sqrt(nrm);
+          for (int r = 0; r < rows; ++r) G[r + i * ld] = zscale(G[r + i * ld], inv);
+          break;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // U = Z V^+
+  for (int e = t; e < rows * nw; e += kNT) {
+    const int r = e % rows, j = e / rows;
+    zd u = zmk(0.0, 0.0);
+    for (int i = 0; i < nw; ++i) zfma(u, G[r + i * ld], zconj(V[j + i * ld]));
+    U[r + (size_t)j * ldu] = u;
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------ kernels
+
+extern __shared__ zd dis_lds[];
+
+// dis_project: U_opt(k) = polar factor of A(window rows, :); its unitarity
+// checked to 1e-5; rows below the window zero.
+__global__ void __launch_bounds__(kNT) k_dis_project(DisProb p) {
+  __shared__ JacobiLds J;
+  const int k = blockIdx.x;
+  const Win w = p.win[k];
+  const int nb = p.nb, nw = p.nw, ld = p.ld;
+  zd* S = dis_lds;
+  zd* V = dis_lds + nb * ld;
+  zd* U = p.Uopt + (size_t)k * nb * nw;
+  const zd* X = p.A + (size_t)k * nb * nw + w.first;
+  polar_factor(X, nb, w.ndim, nw, U, nb, S, V, ld, J, p.flags);
+  for (int e = threadIdx.x; e < nb * nw; e += kNT)
+    if (e % nb >= w.ndim) U[e] = zmk(0.0, 0.0);
+  // unitarity of U_opt (dis_project, :1384-1410)
+  bool bad = false;
+  for (int e = threadIdx.x; e < nw * nw; e += kNT) {
+    const int i = e % nw, j = e / nw;
+    zd s = zmk(0.0, 0.0);
+    for (int m = 0; m < w.ndim; ++m) zcfma(s, U[m + (size_t)i * nb], U[m + (size_t)j * nb]);
+    if (i == j) s.x -= 1.0;
+    if (sqrt(zabs2(s)) > 1e-5) bad = true;
+  }
+  if (bad) atomicOr(p.flags, F_PROJECT_UNIT);
+}
+
+// dis_proj_froz for one k: frozen columns = unit vectors at the frozen
+// bands; the others = the leading nw - nfroz eigenvectors of Q P Q, with
+// P = U_opt U_opt^+ and Q the projector off the frozen bands, in ascending
+// order of eigenvalue; when some of those eigenvalues are below 1e-8 the
+// reference's ortho-fix picks, from the top down, eigenvectors orthogonal
+// to the frozen columns.
+__global__ void __launch_bounds__(kNT) k_dis_proj_froz(DisProb p) {
+  __shared__ JacobiLds J;
+  __shared__ int vmap[kMaxN];
+  __shared__ int fix_ok;
+  const int k = blockIdx.x;
+  const Win w = p.win[k];
+  const int nb = p.nb, nw = p.nw, ld = p.ld, nd = w.ndim, t = threadIdx.x;
+  zd* Aq = dis_lds;
+  zd* V = dis_lds + nb * ld;
+  zd* U = p.Uopt + (size_t)k * nb * nw;
+  const int r = nw - w.nfroz;
+  auto frozen = [&](int i) { return i >= w.fmin && i < w.fmin + w.nfroz; };
+  if (r > 0) {
+    for (int e = t; e < nd * nd; e += kNT) {
+      const int i = e % nd, j = e / nd;
+      if (i > j) continue;
+      zd s = zmk(0.0, 0.0);
+      if (!frozen(i) && !frozen(j))
+        for (int l = 0; l < nw; ++l) zfma(s, U[i + (size_t)l * nb], zconj(U[j + (size_t)l * nb]));
+      if (i == j) s.y = 0.0;
+      Aq[i + j * ld] = s;
+      Aq[j + i * ld] = zconj(s);
+    }
+    __syncthreads();
+    if (jacobi_eigh(Aq, V, nd, ld, J) >= kMaxSweeps && t == 0) atomicOr(p.flags, F_JACOBI);
+    eig_order(Aq, nd, ld, J);
+    if (t == 0) {
+      fix_ok = 1;
+      for (int j = 0; j < nd; ++j)
+        if (J.lam[j] < -1e-8 || J.lam[j] > 1.0 + 1e-8) atomicOr(p.flags, F_FROZ_EIG);
+      int nzero = 0, goods = 0;
+      for (int j = nd - 1; j >= nd - r; --j) {
+        if (J.lam[J.perm[j]] < 1e-8) nzero++;
+        else goods++;
+      }
+      if (nzero == 0) {
+        for (int l = 0; l < r; ++l) vmap[l] = J.perm[nd - r + l];
+      } else {
+        // ortho-fix (:1780-1850): the good eigenvectors from the top, then
+        // eigenvectors orthogonal to the frozen columns, searched from the top
+        int used[kMaxN];
+        for (int j = 0; j < nd; ++j) used[j] = 0;
+        int cnt = 0;
+        for (int j = nd - 1; j >= nd - goods; --j) {
+          vmap[cnt++] = J.perm[j];
+          used[j] = 1;
+        }
+        for (int f = 0; f < nzero; ++f) {
+          int found = -1;
+          for (int j = nd - 1; j >= 0 && found < 0; --j) {
+            if (used[j]) continue;
+            const int v = J.perm[j];
+            // orthogonal to the first nfroz columns of U_opt as dis_project
+            // left them (the unit frozen columns are set afterwards)
+            bool take = true;
+            for (int m = 0; m < w.nfroz && take; ++m) {
+              zd c = zmk(0.0, 0.0);
+              for (int j2 = 0; j2 < nd; ++j2) zcfma(c, U[j2 + (size_t)m * nb], V[j2 + v * ld]);
+              if (sqrt(zabs2(c)) > 1e-8) take = false;
+            }
+            if (take) found = j;
+          }
+          if (found < 0) {
+            fix_ok = 0;
+            break;
+          }
+          used[found] = 1;
+          vmap[cnt++] = J.perm[found];
+        }
+        // the fix-up columns follow the good ones in the reference's order:
+        // good ones from the top down, then the picked ones
+      }
+      if (!fix_ok) atomicOr(p.flags, F_FROZ_FIX);
+    }
+    __syncthreads();
+    if (fix_ok) {
+      for (int e = t; e < nd * r; e += kNT) {
+        const int i = e % nd, l = e / nd;
+        U[i + (size_t)(w.nfroz + l) * nb] = V[i + vmap[l] * ld];
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < nb * w.nfroz; e += kNT) {
+    const int i = e % nb, l = e / nb;
+    U[i + (size_t)l * nb] = zmk(i == w.fmin + l ? 1.0 : 0.0, 0.0);
+  }
+}
+
+// internal_check_orthonorm: the columns of U_opt(k) orthonormal to 1e-8.
+__global__ void __launch_bounds__(kNT) k_dis_orthonorm(DisProb p) {
+  const int k = blockIdx.x;
+  const Win w = p.win[k];
+  const int nb = p.nb, nw = p.nw;
+  const zd* U = p.Uopt + (size_t)k * nb * nw;
+  bool bad = false;
+  for (int e = threadIdx.x; e < nw * nw; e += kNT) {
+    const int l = e % nw, m = e / nw;
+    if (m > l) continue;
+    zd s = zmk(0.0, 0.0);
+    for (int j = 0; j < w.ndim; ++j) zcfma(s, U[j + (size_t)m * nb], U[j + (size_t)l * nb]);
+    if (l == m) s.x -= 1.0;
+    if (sqrt(zabs2(s)) > 1e-8) bad = true;
+  }
+  if (bad) atomicOr(p.flags, F_ORTHONORM);
+}
+
+// Upper-triangle entries of an n x n matrix per thread in k_dis_zmat.
+constexpr int kZPer = (kMaxN * (kMaxN + 1) / 2 + kNT - 1) / kNT;
+
+__device__ __forceinline__ void tri_index(int e, int& m, int& n) {
+  // e = n (n + 1) / 2 + m, m <= n
+  int c = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+  while (c * (c + 1) / 2 > e) --c;
+  while ((c + 1) * (c + 2) / 2 <= e) ++c;
+  n = c;
+  m = e - c * (c + 1) / 2;
+}
+
+// internal_zmatrix and the Omega_I terms for one k, with U_opt of every k
+// as it is now: Zout(k) (non-frozen rows, upper triangle mirrored),
+// wkom[k] = nw wbtot - sum_b w_b ||U_opt(k)^+ C_b||^2 and frz[k] = the same
+// sum over the frozen rows of U_opt(k)^+ C_b, with C_b = M(k,b) U_opt(k+b).
+__global__ void __launch_bounds__(kNT) k_dis_zmat(DisProb p) {
+  __shared__ double red[kNT / 64];
+  const DisState* st = p.st;
+  if (st->done) return;
+  const int k = blockIdx.x, t = threadIdx.x;
+  const Win w = p.win[k];
+  const int nb = p.nb, nw = p.nw, ld = p.ld, nn = p.nn;
+  const int nd = w.ndim - w.nfroz;
+  const bool want_z = nw > w.nfroz;
+  const int ntri = nd * (nd + 1) / 2;
+  zd* C = dis_lds;            // ndim(k) x nw
+  zd* U2 = dis_lds + nb * ld; // ndim(k+b) x nw
+  const zd* Uk = p.Uopt + (size_t)k * nb * nw;
+  zd z[kZPer];
+  for (int s = 0; s < kZPer; ++s) z[s] = zmk(0.0, 0.0);
+  double om = 0.0, fz = 0.0;
+  for (int b = 0; b < nn; ++b) {
+    const int k2 = p.nnl[k * nn + b];
+    const Win w2 = p.win[k2];
+    const zd* Ukb = p.Uopt + (size_t)k2 * nb * nw;
+    for (int e = t; e < w2.ndim * nw; e += kNT) {
+      const int l = e % w2.ndim, j = e / w2.ndim;
+      U2[l + j * ld] = Ukb[l + (size_t)j * nb];
+    }
+    __syncthreads();
+    const zd* Mkb = p.M + ((size_t)k * nn + b) * nb * nb + w.first + (size_t)w2.first * nb;
+    for (int e = t; e < w.ndim * nw; e += kNT) {
+      const int i = e % w.ndim, j = e / w.ndim;
+      zd c = zmk(0.0, 0.0);
+      for (int l = 0; l < w2.ndim; ++l) zfma(c, Mkb[i + (size_t)l * nb], U2[l + j * ld]);
+      C[i + j * ld] = c;
+    }
+    __syncthreads();
+    const double wbb = p.wb[b];
+    if (want_z) {
+      for (int s = 0; s < kZPer; ++s) {
+        const int e = t + s * kNT;
+        if (e >= ntri) break;
+        int m, n;
+        tri_index(e, m, n);
+        const int pr = nf_row(w, m), qr = nf_row(w, n);
+        zd cs = zmk(0.0, 0.0);
+        for (int l = 0; l < nw; ++l) zfma(cs, C[pr + l * ld], zconj(C[qr + l * ld]));
+        z[s] = zadd(z[s], zscale(cs, wbb));
+      }
+    }
+    double r = 0.0, rf = 0.0;
+    for (int e = t; e < nw * nw; e += kNT) {
+      const int i = e % nw, j = e / nw;
+      zd v = zmk(0.0, 0.0);
+      for (int l = 0; l < w.ndim; ++l) zcfma(v, Uk[l + (size_t)i * nb], C[l + j * ld]);
+      const double a2 = zabs2(v);
+      r += a2;
+      if (i < w.nfroz) rf += a2;
+    }
+    om += wbb * r;
+    fz += wbb * rf;
+    __syncthreads();
+  }
+  if (want_z) {
+    zd* Zo = p.Zout + (size_t)k * nb * nb;
+    for (int s = 0; s < kZPer; ++s) {
+      const int e = t + s * kNT;
+      if (e >= ntri) break;
+      int m, n;
+      tri_index(e, m, n);
+      Zo[m + (size_t)n * nb] = z[s];
+      Zo[n + (size_t)m * nb] = zconj(z[s]);
+    }
+  }
+  om = block_sum(om, red);
+  fz = block_sum(fz, red);
+  if (t == 0) {
+    p.wkom[k] = (double)nw * p.wbtot - om;
+    p.frz[k] = fz;
+  }
+}
+
+// One extraction step for one k: Zin mixed (the first iteration takes Zout
+// of the initial U_opt), its eigenvectors; the leading nw - nfroz replace
+// the non-frozen columns of U_opt(k) (in ascending order of eigenvalue) and
+// their eigenvalues are subtracted from the Omega_I estimate.
+__global__ void __launch_bounds__(kNT) k_dis_update(DisProb p) {
+  __shared__ JacobiLds J;
+  const DisState* st = p.st;
+  if (st->done) return;
+  const int k = blockIdx.x, t = threadIdx.x;
+  const Win w = p.win[k];
+  const int nb = p.nb, nw = p.nw, ld = p.ld;
+  const int nd = w.ndim - w.nfroz, r = nw - w.nfroz;
+  if (r <= 0) {
+    if (t == 0) p.wk1[k] = (double)nw * p.wbtot - p.frz[k];
+    return;
+  }
+  const bool first = st->iter == 0;
+  const double mix = st->mix;
+  zd* Zi = p.Zin + (size_t)k * nb * nb;
+  const zd* Zo = p.Zout + (size_t)k * nb * nb;
+  zd* A = dis_lds;
+  zd* V = dis_lds + nb * ld;
+  for (int e = t; e < nd * nd; e += kNT) {
+    const int i = e % nd, j = e / nd;  // row i, column j
+    if (i > j) continue;
+    zd v;
+    if (first) {
+      v = Zo[i + (size_t)j * nb];
+    } else {
+      const zd a = Zo[i + (size_t)j * nb], b = Zi[i + (size_t)j * nb];
+      v = zadd(zscale(a, mix), zscale(b, 1.0 - mix));
+    }
+    Zi[i + (size_t)j * nb] = v;
+    Zi[j + (size_t)i * nb] = zconj(v);
+    if (i == j) v.y = 0.0;
+    A[i + j * ld] = v;
+    A[j + i * ld] = zconj(v);
+  }
+  __syncthreads();
+  if (jacobi_eigh(A, V, nd, ld, J) >= kMaxSweeps && t == 0) atomicOr(p.flags, F_JACOBI);
+  eig_order(A, nd, ld, J);
+  zd* U = p.Uopt + (size_t)k * nb * nw;
+  for (int e = t; e < w.ndim * r; e += kNT) {
+    const int i = e % w.ndim, l = e / w.ndim;
+    U[i + (size_t)(w.nfroz + l) * nb] = zmk(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int e = t; e < nd * r; e += kNT) {
+    const int i = e % nd, l = e / nd;
+    U[nf_row(w, i) + (size_t)(w.nfroz + l) * nb] = V[i + J.perm[nd - r + l] * ld];
+  }
+  if (t == 0) {
+    double s = (double)nw * p.wbtot;
+    if (w.nfroz > 0) s -= p.frz[k];
+    for (int l = 0; l < r; ++l) s -= J.lam[J.perm[nd - r + l]];
+    p.wk1[k] = s;
+  }
+}
+
+// Omega_I estimate and Omega_I over k (summed in k order), the change and
+// the convergence window (internal_test_convergence: the last conv_window
+// changes all below conv_tol, once conv_window iterations are done).
+__global__ void __launch_bounds__(64) k_dis_reduce(DisProb p) {
+  DisState* st = p.st;
+  if (st->done || threadIdx.x != 0) return;
+  double s1 = 0.0, s2 = 0.0;
+  for (int k = 0; k < p.nk; ++k) s1 += p.wk1[k];
+  for (int k = 0; k < p.nk; ++k) s2 += p.wkom[k];
+  const double womegai1 = s1 / p.nk, womegai = s2 / p.nk;
+  const double delta = womegai1 / womegai - 1.0;
+  const int iter = st->iter + 1;
+  double* row = p.log + 3 * (size_t)(iter - 1);
+  row[0] = womegai1;
+  row[1] = womegai;
+  row[2] = delta;
+  st->iter = iter;
+  st->womegai1 = womegai1;
+  st->womegai = womegai;
+  st->delta = delta;
+  st->run = (fabs(delta) < st->conv_tol) ? st->run + 1 : 0;
+  if (iter >= st->conv_window && st->run >= st->conv_window) st->converged = 1;
+  if (st->converged || iter >= st->num_iter) st->done = 1;
+}
+
+// The Hamiltonian in the optimal subspace, H = U_opt^+ diag(e) U_opt, its
+// eigenvalues into eigopt(1:nw) and U_opt <- U_opt V (ascending order).
+__global__ void __launch_bounds__(kNT) k_dis_ham(DisProb p) {
+  __shared__ JacobiLds J;
+  const int k = blockIdx.x, t = threadIdx.x;
+  const Win w = p.win[k];
+  const int nb = p.nb, nw = p.nw, ld = p.ld;
+  zd* U = p.Uopt + (size_t)k * nb * nw;
+  const double* e = p.eig + (size_t)k * nb + w.first;
+  zd* H = dis_lds;
+  zd* V = dis_lds + nb * ld;
+  for (int q = t; q < nw * nw; q += kNT) {
+    const int i = q % nw, j = q / nw;
+    if (i > j) continue;
+    zd s = zmk(0.0, 0.0);
+    for (int l = 0; l < w.ndim; ++l) zcfma(s, U[l + (size_t)i * nb], zscale(U[l + (size_t)j * nb], e[l]));
+    if (i == j) s.y = 0.0;
+    H[i + j * ld] = s;
+    H[j + i * ld] = zconj(s);
+  }
+  __syncthreads();
+  if (jacobi_eigh(H, V, nw, ld, J) >= kMaxSweeps && t == 0) atomicOr(p.flags, F_JACOBI);
+  eig_order(H, nw, ld, J);
+  if (t < nw) p.eigopt[(size_t)k * nb + t] = J.lam[J.perm[t]];
+  // U_opt V into the H area, then back
+  for (int q = t; q < w.ndim * nw; q += kNT) {
+    const int i = q % w.ndim, j = q / w.ndim;
+    zd s = zmk(0.0, 0.0);
+    const int c = J.perm[j];
+    for (int l = 0; l < nw; ++l) zfma(s, U[i + (size_t)l * nb], V[l + c * ld]);
+    H[i + j * ld] = s;
+  }
+  __syncthreads();
+  for (int q = t; q < w.ndim * nw; q += kNT) {
+    const int i = q % w.ndim, j = q / w.ndim;
+    U[i + (size_t)j * nb] = H[i + j * ld];
+  }
+}
+
+// internal_find_u: U(k) = polar factor of X = U_opt(k)^+ A(k) (X staged in
+// the Zin area, free after the extraction), then W(k) = U_opt(k) U(k).
+__global__ void __launch_bounds__(kNT) k_dis_find_u(DisProb p) {
+  __shared__ JacobiLds J;
+  const int k = blockIdx.x, t = threadIdx.x;
+  const Win w = p.win[k];
+  const int nb = p.nb, nw = p.nw, ld = p.ld;
+  const zd* Uo = p.Uopt + (size_t)k * nb * nw;
+  const zd* A = p.A + (size_t)k * nb * nw + w.first;
+  zd* X = p.Zin + (size_t)k * nb * nb;  // nw x nw, leading dimension nw
+  for (int q = t; q < nw * nw; q += kNT) {
+    const int i = q % nw, j = q / nw;
+    zd s = zmk(0.0, 0.0);
+    for (int l = 0; l < w.ndim; ++l) zcfma(s, Uo[l + (size_t)i * nb], A[l + (size_t)j * nb]);
+    X[i + (size_t)j * nw] = s;
+  }
+  __threadfence_block();
+  __syncthreads();
+  zd* U = p.U + (size_t)k * nw * nw;
+  polar_factor(X, nw, nw, nw, U, nw, dis_lds, dis_lds + nb * ld, ld, J, p.flags);
+  __threadfence_block();
+  __syncthreads();
+  zd* W = p.W + (size_t)k * nb * nw;
+  for (int q = t; q < w.ndim * nw; q += kNT) {
+    const int i = q % w.ndim, j = q / w.ndim;
+    zd s = zmk(0.0, 0.0);
+    for (int l = 0; l < nw; ++l) zfma(s, Uo[i + (size_t)l * nb], U[l + (size_t)j * nw]);
+    W[i + (size_t)j * nb] = s;
+  }
+}
+
+// M(k,b) <- W(k)^+ M(k,b) W(k+b) (num_wann x num_wann), one workgroup per
+// (k, b).
+__global__ void __launch_bounds__(kNT) k_dis_rotate(DisProb p) {
+  const int k = blockIdx.x / p.nn, b = blockIdx.x % p.nn, t = threadIdx.x;
+  const int nb = p.nb, nw = p.nw, ld = p.ld, nn = p.nn;
+  const Win w = p.win[k];
+  const int k2 = p.nnl[k * nn + b];
+  const Win w2 = p.win[k2];
+  zd* P = dis_lds;            // ndim(k) x nw
+  zd* W2 = dis_lds + nb * ld; // ndim(k+b) x nw
+  const zd* Wk = p.W + (size_t)k * nb * nw;
+  const zd* Wkb = p.W + (size_t)k2 * nb * nw;
+  for (int e = t; e < w2.ndim * nw; e += kNT) {
+    const int l = e % w2.ndim, j = e / w2.ndim;
+    W2[l + j * ld] = Wkb[l + (size_t)j * nb];
+  }
+  __syncthreads();
+  const zd* Mkb = p.M + ((size_t)k * nn + b) * nb * nb + w.first + (size_t)w2.first * nb;
+  for (int e = t; e < w.ndim * nw; e += kNT) {
+    const int i = e % w.ndim, j = e / w.ndim;
+    zd c = zmk(0.0, 0.0);
+    for (int l = 0; l < w2.ndim; ++l) zfma(c, Mkb[i + (size_t)l * nb], W2[l + j * ld]);
+    P[i + j * ld] = c;
+  }
+  __syncthreads();
+  zd* out = p.Mw + ((size_t)k * nn + b) * nw * nw;
+  for (int e = t; e < nw * nw; e += kNT) {
+    const int i = e % nw, j = e / nw;
+    zd v = zmk(0.0, 0.0);
+    for (int l = 0; l < w.ndim; ++l) zcfma(v, Wk[l + (size_t)i * nb], P[l + j * ld]);
+    out[i + (size_t)j * nw] = v;
+  }
+}
+
+__global__ void k_dis_init_state(DisState* st, DisState s) { *st = s; }
+
+// ------------------------------------------------------------ host
+
+struct Arena {
+  std::vector<size_t> off;
+  size_t total = 0;
+  size_t add(size_t bytes) {
+    const size_t o = total;
+    total += (bytes + 255) & ~(size_t)255;
+    off.push_back(o);
+    return o;
+  }
+};
+
+#define DCHK(call)                                                                                      \
+  do {                                                                                                  \
+    hipError_t e_ = (call);                                                                             \
+    if (e_ != hipSuccess) {                                                                             \
+      w90gpu_set_error(ctx, "HIP error %s at %s:%d (%s)", hipGetErrorString(e_), __FILE__, __LINE__, #call); \
+      return W90GPU_ERR_FATAL;                                                                          \
+    }                                                                                                   \
+  } while (0)
+
+// dis_windows (:851-1171) without dis_spheres, on the host (each k is a
+// scan over its sorted eigenvalues).
+int windows(w90gpu_ctx* ctx, int nb, int nw, int nk, const w90gpu_dis_control* c, const double* eig,
+            std::vector<Win>& win, std::vector<double>& eigopt, bool* linner) {
+  win.assign(nk, Win{0, 0, 0, 0});
+  eigopt.assign(eig, eig + (size_t)nb * nk);
+  *linner = false;
+  for (int k = 0; k < nk; ++k) {
+    const double* e = eig + (size_t)k * nb;
+    if (e[0] > c->win_max || e[nb - 1] < c->win_min) {
+      w90gpu_set_error(ctx, "dis_windows: The outer energy window contains no eigenvalues (k-point %d)", k + 1);
+      return W90GPU_ERR_FATAL;
+    }
+    int imin = 0, imax = 0;
+    for (int i = 1; i <= nb; ++i) {
+      if (imin == 0) {
+        if (e[i - 1] >= c->win_min && e[i - 1] <= c->win_max) imin = i;
+        imax = i;
+      }
+      if (e[i - 1] <= c->win_max) imax = i;
+    }
+    const int ndim = imax - imin + 1;
+    if (ndim < nw) {
+      w90gpu_set_error(ctx,
+                       "dis_windows: Energy window contains fewer states than number of target WFs (k-point %d, "
+                       "ndimwin=%d, num_wann=%d)",
+                       k + 1, ndim, nw);
+      return W90GPU_ERR_FATAL;
+    }
+    int kmin = 0, kmax = -1;
+    if (c->frozen_states) {
+      for (int i = imin; i <= imax; ++i) {
+        if (kmin == 0) {
+          if (e[i - 1] >= c->froz_min && e[i - 1] <= c->froz_max) {
+            kmin = i - imin + 1;
+            kmax = i - imin + 1;
+          }
+        } else if (e[i - 1] <= c->froz_max) {
+          kmax += 1;
+        }
+      }
+    }
+    const int nfroz = kmax - kmin + 1;
+    if (nfroz > nw) {
+      w90gpu_set_error(ctx,
+                       "dis_windows: More states in the frozen window than target WFs (k-point %d, %d frozen, "
+                       "num_wann=%d)",
+                       k + 1, nfroz, nw);
+      return W90GPU_ERR_FATAL;
+    }
+    if (nfroz > 0) *linner = true;
+    win[k] = Win{imin - 1, ndim, nfroz, nfroz > 0 ? kmin - 1 : 0};
+    double* eo = eigopt.data() + (size_t)k * nb;
+    for (int q = 0; q < ndim; ++q) eo[q] = e[imin - 1 + q];
+    for (int q = ndim; q < nb; ++q) eo[q] = 0.0;
+  }
+  return W90GPU_OK;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+int set_lds_limits(w90gpu_ctx* ctx, size_t bytes) {
+  static bool done = false;
+  if (done) return W90GPU_OK;
+  const void* ks[] = {(const void*)k_dis_project, (const void*)k_dis_proj_froz, (const void*)k_dis_zmat,
+                      (const void*)k_dis_update,  (const void*)k_dis_ham,       (const void*)k_dis_find_u,
+                      (const void*)k_dis_rotate};
+  for (const void* f : ks)
+    DCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  done = true;
+  return W90GPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void w90gpu_dis_control_defaults(w90gpu_dis_control* c) {
+  if (!c) return;
+  c->win_min = -1.7976931348623157e308;
+  c->win_max = 1.7976931348623157e308;
+  c->froz_min = -1.7976931348623157e308;
+  c->froz_max = 1.7976931348623157e308;
+  c->frozen_states = 0;
+  c->num_iter = 200;
+  c->mix_ratio = 0.5;
+  c->conv_tol = 1e-10;
+  c->conv_window = 3;
+}
+
+int w90gpu_disentangle(w90gpu_ctx* ctx, const w90gpu_problem* prob, int num_bands, const w90gpu_dis_control* ctl,
+                       const double* eigval, const double* a_matrix, const double* m_matrix_orig,
+                       double* u_matrix_opt, double* u_matrix, double* m_matrix, double* eigval_opt,
+                       int* nfirstwin, int* ndimwin, double* history, int history_rows, w90gpu_dis_result* result) {
+  if (!ctx || !prob || !ctl || !eigval || !a_matrix || !m_matrix_orig || !u_matrix_opt || !u_matrix || !m_matrix) {
+    w90gpu_set_error(ctx, "w90gpu_disentangle: null argument");
+    return W90GPU_ERR_INPUT;
+  }
+  const int nb = num_bands, nw = prob->num_wann, nk = prob->num_kpts, nn = prob->nntot;
+  if (nw < 1 || nb < nw || nb > W90GPU_DIS_MAX_BANDS || nk < 1 || nn < 1 || nn > W90GPU_NUM_NNMAX ||
+      !prob->nnlist || !prob->wb) {
+    w90gpu_set_error(ctx, "w90gpu_disentangle: invalid sizes num_bands=%d num_wann=%d num_kpts=%d nntot=%d (num_bands "
+                          "<= %d)", nb, nw, nk, nn, W90GPU_DIS_MAX_BANDS);
+    return W90GPU_ERR_INPUT;
+  }
+  if (ctl->num_iter < 0 || !(ctl->mix_ratio > 0.0 && ctl->mix_ratio <= 1.0) || ctl->conv_tol < 0.0 ||
+      ctl->conv_window < 0) {
+    w90gpu_set_error(ctx, "w90gpu_disentangle: invalid dis_num_iter, dis_mix_ratio, dis_conv_tol or "
+                          "dis_conv_window");
+    return W90GPU_ERR_INPUT;
+  }
+  const auto t_start = std::chrono::steady_clock::now();
+  std::vector<Win> win;
+  std::vector<double> eigopt;
+  bool linner = false;
+  int rc = windows(ctx, nb, nw, nk, ctl, eigval, win, eigopt, &linner);
+  if (rc) return rc;
+  DCHK(hipSetDevice(ctx->device));
+  const int ld = nb + 1;
+  const size_t lds = 2 * (size_t)nb * ld * sizeof(zd);
+  if ((rc = set_lds_limits(ctx, 2 * (size_t)kMaxN * (kMaxN + 1) * sizeof(zd)))) return rc;
+
+  const size_t zb = sizeof(zd);
+  const int nlog = std::max(ctl->num_iter, 1);
+  Arena ar;
+  const size_t o_nnl = ar.add(sizeof(int) * nk * nn), o_wb = ar.add(sizeof(double) * nn),
+               o_win = ar.add(sizeof(Win) * nk), o_eig = ar.add(sizeof(double) * nb * nk),
+               o_M = ar.add(zb * nb * nb * nn * nk), o_A = ar.add(zb * nb * nw * nk),
+               o_Uopt = ar.add(zb * nb * nw * nk), o_Zin = ar.add(zb * nb * nb * nk),
+               o_Zout = ar.add(zb * nb * nb * nk), o_U = ar.add(zb * nw * nw * nk), o_W = ar.add(zb * nb * nw * nk),
+               o_Mw = ar.add(zb * nw * nw * nn * nk), o_wk1 = ar.add(sizeof(double) * nk),
+               o_wkom = ar.add(sizeof(double) * nk), o_frz = ar.add(sizeof(double) * nk),
+               o_eigopt = ar.add(sizeof(double) * nb * nk), o_log = ar.add(sizeof(double) * 3 * nlog),
+               o_st = ar.add(sizeof(DisState)), o_flags = ar.add(sizeof(int));
+  DevBuf buf;
+  DCHK(hipMalloc(&buf.p, ar.total));
+  char* base = (char*)buf.p;
+  DCHK(hipMemset(base, 0, ar.total));
+  std::vector<int> nnl((size_t)nk * nn);
+  for (int k = 0; k < nk; ++k)
+    for (int b = 0; b < nn; ++b) {
+      const int v = prob->nnlist[k + (size_t)nk * b];
+      if (v < 1 || v > nk) {
+        w90gpu_set_error(ctx, "w90gpu_disentangle: nnlist(%d,%d)=%d out of range", k + 1, b + 1, v);
+        return W90GPU_ERR_INPUT;
+      }
+      nnl[(size_t)k * nn + b] = v - 1;
+    }
+  DCHK(hipMemcpy(base + o_nnl, nnl.data(), sizeof(int) * nnl.size(), hipMemcpyHostToDevice));
+  DCHK(hipMemcpy(base + o_wb, prob->wb, sizeof(double) * nn, hipMemcpyHostToDevice));
+  DCHK(hipMemcpy(base + o_win, win.data(), sizeof(Win) * nk, hipMemcpyHostToDevice));
+  DCHK(hipMemcpy(base + o_eig, eigval, sizeof(double) * nb * nk, hipMemcpyHostToDevice));
+  DCHK(hipMemcpy(base + o_M, m_matrix_orig, zb * nb * nb * nn * nk, hipMemcpyHostToDevice));
+  DCHK(hipMemcpy(base + o_A, a_matrix, zb * nb * nw * nk, hipMemcpyHostToDevice));
+  DCHK(hipMemcpy(base + o_eigopt, eigopt.data(), sizeof(double) * nb * nk, hipMemcpyHostToDevice));
+
+  DisProb p;
+  p.nb = nb;
+  p.nw = nw;
+  p.nk = nk;
+  p.nn = nn;
+  p.ld = ld;
+  p.wbtot = prob->wbtot;
+  p.nnl = (const int*)(base + o_nnl);
+  p.wb = (const double*)(base + o_wb);
+  p.win = (const Win*)(base + o_win);
+  p.eig = (const double*)(base + o_eig);
+  p.M = (const zd*)(base + o_M);
+  p.A = (const zd*)(base + o_A);
+  p.Uopt = (zd*)(base + o_Uopt);
+  p.Zin = (zd*)(base + o_Zin);
+  p.Zout = (zd*)(base + o_Zout);
+  p.U = (zd*)(base + o_U);
+  p.W = (zd*)(base + o_W);
+  p.Mw = (zd*)(base + o_Mw);
+  p.wk1 = (double*)(base + o_wk1);
+  p.wkom = (double*)(base + o_wkom);
+  p.frz = (double*)(base + o_frz);
+  p.eigopt = (double*)(base + o_eigopt);
+  p.log = (double*)(base + o_log);
+  p.st = (DisState*)(base + o_st);
+  p.flags = (int*)(base + o_flags);
+
+  DisState s0;
+  std::memset(&s0, 0, sizeof(s0));
+  s0.num_iter = ctl->num_iter;
+  s0.conv_window = ctl->conv_window;
+  s0.mix = ctl->mix_ratio;
+  s0.conv_tol = ctl->conv_tol;
+  s0.done = ctl->num_iter == 0 ? 1 : 0;
+  hipStream_t sm = nullptr;
+  hipLaunchKernelGGL(k_dis_init_state, dim3(1), dim3(1), 0, sm, p.st, s0);
+  hipLaunchKernelGGL(k_dis_project, dim3(nk), dim3(kNT), lds, sm, p);
+  if (linner) hipLaunchKernelGGL(k_dis_proj_froz, dim3(nk), dim3(kNT), lds, sm, p);
+  hipLaunchKernelGGL(k_dis_orthonorm, dim3(nk), dim3(kNT), 0, sm, p);
+  DCHK(hipGetLastError());
+  int flags = 0;
+  DCHK(hipMemcpy(&flags, p.flags, sizeof(int), hipMemcpyDeviceToHost));
+  if (flags & F_PROJECT_UNIT) {
+    w90gpu_set_error(ctx, "dis_project: Error in unitarity of initial U in dis_project");
+    return W90GPU_ERR_FATAL;
+  }
+  if (flags & F_FROZ_EIG) {
+    w90gpu_set_error(ctx, "dis_proj_frozen: error - Eigenvalues not between 0 and 1");
+    return W90GPU_ERR_FATAL;
+  }
+  if (flags & F_FROZ_FIX) {
+    w90gpu_set_error(ctx, "dis_proj_froz: Ortho-fix failed to find enough vectors");
+    return W90GPU_ERR_FATAL;
+  }
+  if (flags & F_ORTHONORM) {
+    w90gpu_set_error(ctx, "Error in dis_main: orthonormal error (the trial orbitals for disentanglement are not "
+                          "orthonormal)");
+    return W90GPU_ERR_FATAL;
+  }
+  // the extraction: Z of the initial U_opt, then (update, Z, reduce) per
+  // iteration, polled every kPoll iterations
+  constexpr int kPoll = 16;
+  hipLaunchKernelGGL(k_dis_zmat, dim3(nk), dim3(kNT), lds, sm, p);
+  DisState st;
+  int launched = 0;
+  while (launched < ctl->num_iter) {
+    const int n = std::min(kPoll, ctl->num_iter - launched);
+    for (int i = 0; i < n; ++i) {
+      hipLaunchKernelGGL(k_dis_update, dim3(nk), dim3(kNT), lds, sm, p);
+      hipLaunchKernelGGL(k_dis_zmat, dim3(nk), dim3(kNT), lds, sm, p);
+      hipLaunchKernelGGL(k_dis_reduce, dim3(1), dim3(64), 0, sm, p);
+    }
+    DCHK(hipGetLastError());
+    launched += n;
+    DCHK(hipMemcpy(&st, p.st, sizeof(DisState), hipMemcpyDeviceToHost));
+    if (st.done) break;
+  }
+  hipLaunchKernelGGL(k_dis_ham, dim3(nk), dim3(kNT), lds, sm, p);
+  hipLaunchKernelGGL(k_dis_find_u, dim3(nk), dim3(kNT), lds, sm, p);
+  hipLaunchKernelGGL(k_dis_rotate, dim3(nk * nn), dim3(kNT), lds, sm, p);
+  DCHK(hipGetLastError());
+  DCHK(hipMemcpy(&st, p.st, sizeof(DisState), hipMemcpyDeviceToHost));
+  DCHK(hipMemcpy(&flags, p.flags, sizeof(int), hipMemcpyDeviceToHost));
+  if (flags & F_JACOBI) {
+    w90gpu_set_error(ctx, "dis_extract: Jacobi eigensolver did not converge in %d sweeps", kMaxSweeps);
+    return W90GPU_ERR_FATAL;
+  }
+  DCHK(hipMemcpy(u_matrix_opt, p.Uopt, zb * nb * nw * nk, hipMemcpyDeviceToHost));
+  DCHK(hipMemcpy(u_matrix, p.U, zb * nw * nw * nk, hipMemcpyDeviceToHost));
+  DCHK(hipMemcpy(m_matrix, p.Mw, zb * nw * nw * nn * nk, hipMemcpyDeviceToHost));
+  if (eigval_opt) DCHK(hipMemcpy(eigval_opt, p.eigopt, sizeof(double) * nb * nk, hipMemcpyDeviceToHost));
+  if (history && history_rows > 0 && st.iter > 0)
+    DCHK(hipMemcpy(history, p.log, sizeof(double) * 3 * std::min(history_rows, st.iter), hipMemcpyDeviceToHost));
+  for (int k = 0; k < nk; ++k) {
+    if (nfirstwin) nfirstwin[k] = win[k].first + 1;
+    if (ndimwin) ndimwin[k] = win[k].ndim;
+  }
+  if (result) {
+    result->omega_invariant = st.womegai;
+    result->iterations = st.iter;
+    result->converged = st.converged;
+    result->status = W90GPU_OK;
+    result->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  }
+  return W90GPU_OK;
+}
+
+}  // extern "C"

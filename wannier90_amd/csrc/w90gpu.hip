@@ -21,14 +21,10 @@
 #include <string>
 #include <vector>
 
+#include "w90gpu_ctx.h"
 #include "w90gpu_kernels.h"
 #include "w90gpu_mfma.h"
 
-struct w90gpu_ctx {
-  int device = 0;
-  std::string err;
-  char* scratch = nullptr;  // this device's copy of `scratch` (w90gpu_omega_noalloc)
-};
 
 // Device memory of the allocation-free entry point w90gpu_omega_noalloc:
 // one static array, which every GPU the library runs on holds until the
@@ -272,14 +268,18 @@ namespace {
 
 thread_local std::string g_noctx_err;
 
-void set_error(w90gpu_ctx* ctx, const char* fmt, ...) {
+void set_error_v(w90gpu_ctx* ctx, const char* fmt, va_list ap) {
   char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
   std::vsnprintf(buf, sizeof(buf), fmt, ap);
-  va_end(ap);
   if (ctx) ctx->err = buf;
   else g_noctx_err = buf;
+}
+
+void set_error(w90gpu_ctx* ctx, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  set_error_v(ctx, fmt, ap);
+  va_end(ap);
 }
 
 #define HIPCHK(ctx, call)                                                                         \
@@ -1583,6 +1583,13 @@ int w90gpu_destroy(w90gpu_ctx* ctx) {
 }
 
 const char* w90gpu_last_error(const w90gpu_ctx* ctx) { return ctx ? ctx->err.c_str() : g_noctx_err.c_str(); }
+
+void w90gpu_set_error(w90gpu_ctx* ctx, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  set_error_v(ctx, fmt, ap);
+  va_end(ap);
+}
 
 size_t w90gpu_problem_bytes(int num_wann, int num_kpts, int nntot, int num_iter) {
   if (num_wann < 1 || num_kpts < 1 || nntot < 1 || num_iter < 0) return 0;

@@ -162,6 +162,28 @@ def initial_state(prob):
     return uf, np.asfortranarray(m)
 
 
+def make_dis_problem(seed, kind="sc", num_bands=12, num_wann=8, mp=(3, 3, 3), decay=1.5, a=5.0, c_over_a=1.6,
+                     frozen=0, mix=0.3):
+    """A disentanglement problem: the tight-binding model of make_problem
+    with num_bands orbitals (its bands, eigenvalues and overlaps M(k,b) of
+    all num_bands), projections A(k) = C(k)^+ g onto num_wann trial
+    orbitals g = the first num_wann orbitals mixed with the others by
+    `mix`, and an inner window that freezes, where frozen > 0, the bands
+    below the lowest energy of band `frozen` over k (at most `frozen` bands
+    at any k).  Returns (Problem, dis control dict)."""
+    p = make_problem(seed, kind=kind, a=a, nw=num_bands, mp=mp, decay=decay, c_over_a=c_over_a)
+    rng = np.random.default_rng(seed + 1)
+    g = np.eye(num_bands)[:, :num_wann].astype(complex)
+    g += mix * (rng.normal(size=(num_bands, num_wann)) + 1j * rng.normal(size=(num_bands, num_wann)))
+    p.amat_dis = np.asfortranarray(np.transpose(np.einsum("kam,an->kmn", p.cvec.conj(), g), (1, 2, 0)))
+    p.eig = np.asfortranarray(p.evals.T)
+    p.num_bands, p.num_wann_dis = num_bands, num_wann
+    ctl = dict(num_iter=200, mix_ratio=0.5, conv_tol=1e-10, conv_window=3)
+    if frozen > 0:
+        ctl.update(frozen_states=1, froz_max=float(p.evals[:, frozen - 1].min()))
+    return p, ctl
+
+
 # ---------------------------------------------------------------- configs
 
 # The three single-problem workloads: 1 = sc a = 5 Ang, 4 bands, 4^3 mesh

@@ -153,7 +153,9 @@ def write_chk(path, real_lattice, kpts, mp, nntot, u, m, centres, spreads, num_b
 
 
 def read_chk(path):
-    """Reads an unformatted .chk written without disentanglement."""
+    """Reads an unformatted .chk (w90_wannier90_readwrite_write_chkpt,
+    src/wannier90_readwrite.F90:2336-2366), with the disentanglement records
+    (omega_invariant, lwindow, ndimwin, u_matrix_opt) when present."""
     with open(path, "rb") as f:
         buf = f.read()
     recs = []
@@ -181,7 +183,11 @@ def read_chk(path):
     dis = struct.unpack("<i", next(r))[0]
     out["have_disentangled"] = bool(dis)
     if dis:
-        raise ValueError("disentangled checkpoints are not on this path")
+        nb = out["num_bands"]
+        out["omega_invariant"] = struct.unpack("<d", next(r))[0]
+        out["lwindow"] = np.frombuffer(next(r), "<i4").reshape((nb, nk), order="F") != 0
+        out["ndimwin"] = np.frombuffer(next(r), "<i4").copy()
+        out["u_matrix_opt"] = np.frombuffer(next(r), "<c16").reshape((nb, nw, nk), order="F").copy(order="F")
     out["u_matrix"] = np.frombuffer(next(r), "<c16").reshape((nw, nw, nk), order="F").copy(order="F")
     out["m_matrix"] = np.frombuffer(next(r), "<c16").reshape((nw, nw, nn, nk), order="F").copy(order="F")
     out["centres"] = np.frombuffer(next(r), "<f8").reshape((3, nw), order="F").copy()
@@ -285,3 +291,16 @@ def read_wout_final(path):
     om["conv_omega"] = [float(r[3]) for r in conv]
     om["conv_time"] = [float(r[4]) for r in conv]  # io_wallclocktime, seconds
     return c, s, om
+
+
+def read_wout_dis(path):
+    """The DIS lines of dis_extract (src/disentangle.F90:2209-2212): per
+    iteration Omega_I(i-1), Omega_I(i) and the fractional change, as an
+    (iterations, 3) array, and the final Omega_I line's value (or None)."""
+    with open(path) as f:
+        txt = f.read()
+    rows = re.findall(r"^\s*(\d+)\s+(" + _FLOAT + r")\s+(" + _FLOAT + r")\s+(" + _FLOAT + r")\s+("
+                      + _FLOAT + r")\s+<-- DIS", txt, re.M)
+    hist = np.array([[float(r[1]), float(r[2]), float(r[3])] for r in rows]).reshape(-1, 3)
+    m = re.search(r"Final Omega_I\s+(" + _FLOAT + ")", txt)
+    return hist, (float(m.group(1)) if m else None)
