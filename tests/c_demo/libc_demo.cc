@@ -99,7 +99,8 @@ int main(int argc, char** argv) {
     cset_u_opt(w90, uopt.data());
     cproject_overlap(w90, &ierr);
   }
-  if (ierr == 0) cdisentangle(w90, &ierr);
+  // wannier_prog.F90 disentangles only when num_bands > num_wann; here
+  // num_bands == num_wann, and no band energies are set
   if (ierr == 0) cwannierise(w90, &ierr);
   if (ierr <= 0) {
     cget_centres(w90, centres.data());

@@ -205,3 +205,93 @@ def test_driver_reports_missing_inputs(tmp_path):
     r = run(seed, tmp_path)
     assert r.returncode != 0
     assert "mmn" in open(os.path.join(tmp_path, seed + ".werr")).read()
+
+
+def write_dis_inputs(name, dst, params=None):
+    """The .win/.mmn/.amn/.eig of a disentanglement fixture (the reference's
+    test-suite inputs, stored as data in tests/golden/dis)."""
+    import json
+
+    from wannier90_amd import capi, w90io
+
+    z = np.load(os.path.join(ROOT, "tests", "golden", "dis", name + ".npz"))
+    km = capi.KMesh(int(z["nnlist"].shape[1]), np.asfortranarray(z["nnlist"].astype(np.int32)),
+                    np.asfortranarray(z["nncell"]), np.asfortranarray(z["bk"]), np.ascontiguousarray(z["wb"]),
+                    float(z["wbtot"]))
+    dctl = json.loads(str(z["dis_control"]))
+    ctl = json.loads(str(z["control"]))
+    p = {"dis_num_iter": dctl["num_iter"], "dis_mix_ratio": dctl["mix_ratio"], "dis_conv_tol": dctl["conv_tol"],
+         "dis_conv_window": dctl["conv_window"], "num_iter": ctl["num_iter"], "num_cg_steps": ctl["num_cg_steps"],
+         "conv_window": ctl["conv_window"], "conv_tol": ctl["conv_tol"], "trial_step": ctl["trial_step"]}
+    for key in ("win_min", "win_max", "froz_max", "froz_min"):
+        if abs(dctl[key]) < 1e300 and (key != "froz_min" or dctl["frozen_states"]):
+            p["dis_" + key] = repr(float(dctl[key]))
+    p.update(params or {})
+    seed = "w"
+    w90io.write_win(os.path.join(dst, seed + ".win"), z["real_lattice"], tuple(z["mp_grid"]), z["kpt_latt"],
+                    int(z["num_wann"]), int(z["num_bands"]), p)
+    w90io.write_mmn(os.path.join(dst, seed + ".mmn"), z["m"], km)
+    w90io.write_amn(os.path.join(dst, seed + ".amn"), np.transpose(z["a"], (2, 0, 1)))
+    w90io.write_eig(os.path.join(dst, seed + ".eig"), z["eig"].T)
+    return seed, z
+
+
+@pytest.mark.gpu
+@needs_exe
+def test_driver_disentangles_like_the_reference(tmp_path):
+    """wannier90_gpu.x on the reference's silicon example (12 bands, 8 WFs,
+    outer and frozen windows): the DIS lines, the disentangled checkpoint
+    (Omega_I, ndimwin, the subspaces of U_opt, U_opt U) and the final Omega
+    of the reference's run; restart = wannierise reads the disentangled
+    checkpoint back."""
+    from wannier90_amd import w90io
+
+    seed, z = write_dis_inputs("dis_silicon", tmp_path)
+    r = run(seed, tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr + open(os.path.join(tmp_path, seed + ".werr")).read()
+    hist, om_i = w90io.read_wout_dis(os.path.join(tmp_path, seed + ".wout"))
+    ref = z["dis_history"]
+    assert hist.shape == ref.shape
+    assert np.abs(hist[:, :2] - ref[:, :2]).max() <= 2e-8  # both printed with 8 decimals
+    assert abs(om_i - float(z["omega_invariant"])) <= 2e-8
+    chk = w90io.read_chk(os.path.join(tmp_path, seed + ".chk"))
+    assert chk["have_disentangled"] and chk["checkpoint"] == "postwann"
+    assert np.array_equal(chk["ndimwin"], z["ndimwin"]) and np.array_equal(chk["lwindow"], z["lwindow"])
+    assert abs(chk["omega_invariant"] - float(z["omega_invariant"])) <= 1e-10 * float(z["omega_invariant"])
+    p = np.einsum("iwk,jwk->ijk", chk["u_matrix_opt"], chk["u_matrix_opt"].conj())
+    pr = np.einsum("iwk,jwk->ijk", z["u_opt"], z["u_opt"].conj())
+    assert np.abs(p - pr).max() <= 1e-8
+    w = np.einsum("bwk,wvk->bvk", chk["u_matrix_opt"], chk["u_matrix"])
+    wr = np.einsum("bwk,wvk->bvk", z["u_opt"], z["u_final"])
+    assert np.abs(w - wr).max() <= 1e-5
+    cen, spr, om = w90io.read_wout_final(os.path.join(tmp_path, seed + ".wout"))
+    assert abs(om["omega_total"] - float(z["omega"][3])) <= 2e-8
+    assert np.abs(np.sort(spr) - np.sort(z["spreads"])).max() <= 5e-6
+    # restart from the disentangled checkpoint: 5 more iterations
+    with open(os.path.join(tmp_path, seed + ".win"), "a") as f:
+        f.write("restart = wannierise\n")
+    r = run(seed, tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    chk2 = w90io.read_chk(os.path.join(tmp_path, seed + ".chk"))
+    assert chk2["have_disentangled"] and np.array_equal(chk2["u_matrix_opt"], chk["u_matrix_opt"])
+    _, _, om2 = w90io.read_wout_final(os.path.join(tmp_path, seed + ".wout"))
+    assert om2["omega_total"] <= om["omega_total"] + 1e-8
+
+
+def test_pp_accepts_disentanglement_inputs(tmp_path):
+    """-pp with num_bands > num_wann and the dis_* keywords: the k-mesh and
+    the keyword checks of the disentanglement run before any GPU work."""
+    if not os.path.exists(EXE):
+        pytest.skip("Fortran host not built")
+    seed, z = write_dis_inputs("dis_silicon", tmp_path)
+    r = run(seed, tmp_path, "-pp")
+    assert r.returncode == 0, r.stdout + r.stderr
+    win = os.path.join(tmp_path, seed + ".win")
+    txt = open(win).read()
+    for bad, msg in (("dis_mix_ratio = 0.0", "dis_mix_ratio must be greater than 0.0"),
+                     ("dis_froz_min = 1.0\ndis_froz_max = 0.5", "check disentanglement frozen windows")):
+        lines = [l for l in txt.splitlines() if not l.startswith(bad.split()[0])]
+        open(win, "w").write(bad + "\n" + "\n".join(lines) + "\n")
+        r = run(seed, tmp_path, "-pp")
+        assert r.returncode != 0
+        assert msg in open(os.path.join(tmp_path, seed + ".werr")).read()

@@ -14,6 +14,7 @@ module w90gpu_c
   integer(c_int), parameter :: W90GPU_NUM_NNMAX = 12
   integer(c_int), parameter :: W90GPU_PREC_FP64 = 0
   integer(c_int), parameter :: W90GPU_PREC_MIXED = 1
+  integer(c_int), parameter :: W90GPU_DIS_MAX_BANDS = 64
 
   !> wann_control_type fields on this path (w90gpu_control)
   type, bind(c) :: w90gpu_control
@@ -59,6 +60,28 @@ module w90gpu_c
     integer(c_int) :: bulk_iterations
     integer(c_int) :: held_second_steps
   end type w90gpu_result
+
+  !> dis_control_type and the windows of dis_manifold_type (w90gpu_dis_control)
+  type, bind(c) :: w90gpu_dis_control
+    real(c_double) :: win_min
+    real(c_double) :: win_max
+    real(c_double) :: froz_min
+    real(c_double) :: froz_max
+    integer(c_int) :: frozen_states
+    integer(c_int) :: num_iter
+    real(c_double) :: mix_ratio
+    real(c_double) :: conv_tol
+    integer(c_int) :: conv_window
+  end type w90gpu_dis_control
+
+  !> Omega_I and loop statistics of a disentanglement (w90gpu_dis_result)
+  type, bind(c) :: w90gpu_dis_result
+    real(c_double) :: omega_invariant
+    real(c_double) :: seconds
+    integer(c_int) :: iterations
+    integer(c_int) :: converged
+    integer(c_int) :: status
+  end type w90gpu_dis_result
 
   interface
     subroutine w90gpu_control_defaults(ctl) bind(c, name='w90gpu_control_defaults')
@@ -205,6 +228,34 @@ module w90gpu_c
       real(c_double), intent(out) :: out(*)
       integer(c_int), value :: rows
       integer(c_int) :: n
+    end function
+
+    subroutine w90gpu_dis_control_defaults(ctl) bind(c, name='w90gpu_dis_control_defaults')
+      import :: w90gpu_dis_control
+      type(w90gpu_dis_control), intent(out) :: ctl
+    end subroutine
+
+    function w90gpu_disentangle(ctx, prob, num_bands, ctl, eigval, a_matrix, m_matrix_orig, u_matrix_opt, &
+                                u_matrix, m_matrix, eigval_opt, nfirstwin, ndimwin, history, history_rows, res) &
+      bind(c, name='w90gpu_disentangle') result(rc)
+      import :: c_int, c_ptr, c_double, c_double_complex, w90gpu_problem, w90gpu_dis_control, w90gpu_dis_result
+      type(c_ptr), value :: ctx
+      type(w90gpu_problem), intent(in) :: prob
+      integer(c_int), value :: num_bands
+      type(w90gpu_dis_control), intent(in) :: ctl
+      real(c_double), intent(in) :: eigval(*)
+      complex(c_double_complex), intent(in) :: a_matrix(*)
+      complex(c_double_complex), intent(in) :: m_matrix_orig(*)
+      complex(c_double_complex), intent(out) :: u_matrix_opt(*)
+      complex(c_double_complex), intent(out) :: u_matrix(*)
+      complex(c_double_complex), intent(out) :: m_matrix(*)
+      real(c_double), intent(out) :: eigval_opt(*)
+      integer(c_int), intent(out) :: nfirstwin(*)
+      integer(c_int), intent(out) :: ndimwin(*)
+      real(c_double), intent(out) :: history(*)
+      integer(c_int), value :: history_rows
+      type(w90gpu_dis_result), intent(out) :: res
+      integer(c_int) :: rc
     end function
 
     function c_strlen(s) bind(c, name='strlen') result(n)

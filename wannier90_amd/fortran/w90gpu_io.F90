@@ -2,6 +2,7 @@
 !> .win file or from set_option calls, the .mmn/.amn readers, and the
 !> .chk/.nnkp/_centres.xyz/_u.mat writers.  File formats are wannier90's:
 !> .win (src/readwrite.F90:2151-2712), .mmn/.amn (src/overlap.F90:152-430),
+!> .eig (src/readwrite.F90:588-630),
 !> .chk (src/wannier90_readwrite.F90:2278-2373), .nnkp (src/kmesh.F90:743),
 !> _u.mat (src/plot.F90:2224-2288), _centres.xyz (src/plot.F90:2790-2862).
 module w90gpu_io
@@ -40,7 +41,7 @@ module w90gpu_io
 
   public :: settings_add, settings_find, settings_clear, read_win, lower
   public :: get_int, get_real, get_logical, get_text, get_i1d, get_r2d, get_block
-  public :: read_mmn, read_amn, write_chk, read_chk, write_nnkp, write_xyz, write_umat
+  public :: read_mmn, read_amn, read_eig, write_chk, read_chk, write_nnkp, write_xyz, write_umat
   public :: datetime
 
 contains
@@ -415,6 +416,51 @@ contains
     msg = c_message(err)
   end subroutine read_amn
 
+  !> .eig: lines "band k-point energy" in band-fastest order, each checked
+  !> against the expected indices (w90_readwrite_read_eigvals)
+  subroutine read_eig(path, nb, nk, eig, ierr, msg)
+    character(len=*), intent(in) :: path
+    integer, intent(in) :: nb, nk
+    real(dp), intent(out) :: eig(nb, nk)
+    integer, intent(out) :: ierr
+    character(len=*), intent(out) :: msg
+    integer :: unit, ios, i, j, n, k
+    logical :: there
+    ierr = 0
+    msg = ''
+    inquire (file=path, exist=there)
+    if (.not. there) then
+      ierr = 6
+      msg = 'No '//trim(path)//' file found. Needed for disentanglement'
+      return
+    end if
+    open (newunit=unit, file=path, form='formatted', status='old', iostat=ios)
+    if (ios /= 0) then
+      ierr = 6
+      msg = 'Error: Problem opening eigenvalue file '//trim(path)
+      return
+    end if
+    do k = 1, nk
+      do n = 1, nb
+        read (unit, *, iostat=ios) i, j, eig(n, k)
+        if (ios /= 0) then
+          ierr = 6
+          msg = 'Error: Problem reading eigenvalue file '//trim(path)
+          close (unit)
+          return
+        end if
+        if (i /= n .or. j /= k) then
+          ierr = 6
+          write (msg, '(a,i0,a,i0,a,i0,a,i0,a)') 'mismatch in '//trim(path)//' (wanted band ', n, &
+            ' k-point ', k, ', found band ', i, ' k-point ', j, ')'
+          close (unit)
+          return
+        end if
+      end do
+    end do
+    close (unit)
+  end subroutine read_eig
+
   !> NUL-terminated copy of a Fortran file name
   function c_path(path) result(cp)
     character(len=*), intent(in) :: path
@@ -438,16 +484,21 @@ contains
     end do
   end function c_message
 
-  !> unformatted checkpoint, record by record as the reference writes it
-  !> without disentanglement
+  !> unformatted checkpoint, record by record as the reference writes it;
+  !> after a disentanglement (u_opt present) with Omega_I, lwindow, ndimwin
+  !> and U_opt after have_disentangled
   subroutine write_chk(path, chkpt, num_bands, real_lattice, num_kpts, mp_grid, kpt_latt, nntot, &
-                       num_wann, u, m, centres, spreads)
+                       num_wann, u, m, centres, spreads, omega_invariant, lwindow, ndimwin, u_opt)
     character(len=*), intent(in) :: path, chkpt
     integer, intent(in) :: num_bands, num_kpts, mp_grid(3), nntot, num_wann
     real(dp), intent(in) :: real_lattice(3, 3), kpt_latt(3, num_kpts)
     complex(dp), intent(in) :: u(num_wann, num_wann, num_kpts)
     complex(dp), intent(in) :: m(num_wann, num_wann, nntot, num_kpts)
     real(dp), intent(in) :: centres(3, num_wann), spreads(num_wann)
+    real(dp), intent(in), optional :: omega_invariant
+    logical, intent(in), optional :: lwindow(num_bands, num_kpts)
+    integer, intent(in), optional :: ndimwin(num_kpts)
+    complex(dp), intent(in), optional :: u_opt(num_bands, num_wann, num_kpts)
     integer :: unit, i, j
     character(len=9) :: cdate, ctime
     character(len=33) :: header
@@ -470,7 +521,15 @@ contains
     write (unit) num_wann
     chkpt1 = adjustl(trim(chkpt))
     write (unit) chkpt1
-    write (unit) .false.
+    if (present(u_opt)) then
+      write (unit) .true.
+      write (unit) omega_invariant
+      write (unit) lwindow
+      write (unit) ndimwin
+      write (unit) u_opt
+    else
+      write (unit) .false.
+    end if
     write (unit) u
     write (unit) m
     write (unit) centres
@@ -478,14 +537,22 @@ contains
     close (unit)
   end subroutine write_chk
 
-  !> read U and M of a checkpoint written without disentanglement
-  subroutine read_chk(path, num_wann, num_kpts, nntot, u, m, ierr, msg)
+  !> read U and M of a checkpoint, and the disentanglement records into
+  !> the optional arguments (have_dis says whether there were any)
+  subroutine read_chk(path, num_wann, num_kpts, nntot, u, m, ierr, msg, num_bands, have_dis, omega_invariant, &
+                      lwindow, ndimwin, u_opt)
     character(len=*), intent(in) :: path
     integer, intent(in) :: num_wann, num_kpts, nntot
     complex(dp), intent(out) :: u(num_wann, num_wann, num_kpts)
     complex(dp), intent(out) :: m(num_wann, num_wann, nntot, num_kpts)
     integer, intent(out) :: ierr
     character(len=*), intent(out) :: msg
+    integer, intent(in), optional :: num_bands
+    logical, intent(out), optional :: have_dis
+    real(dp), intent(out), optional :: omega_invariant
+    logical, intent(out), optional :: lwindow(:, :)
+    integer, intent(out), optional :: ndimwin(:)
+    complex(dp), intent(out), optional :: u_opt(:, :, :)
     integer :: unit, ios, nb, nex, nk, mp(3), nn, nw
     character(len=33) :: header
     character(len=20) :: chkpt1
@@ -522,11 +589,24 @@ contains
       close (unit)
       return
     end if
+    if (present(have_dis)) have_dis = dis
     if (dis) then
-      ierr = 5
-      msg = 'Error: disentangled checkpoints are not on this path'
-      close (unit)
-      return
+      if (.not. present(u_opt)) then
+        ierr = 5
+        msg = 'Error: checkpoint '//trim(path)//' holds a disentanglement this caller does not read'
+        close (unit)
+        return
+      end if
+      if (nb /= num_bands) then
+        ierr = 6
+        msg = 'Error: checkpoint '//trim(path)//' does not match the input (num_bands)'
+        close (unit)
+        return
+      end if
+      read (unit, iostat=ios) omega_invariant
+      read (unit, iostat=ios) lwindow
+      read (unit, iostat=ios) ndimwin
+      read (unit, iostat=ios) u_opt
     end if
     read (unit, iostat=ios) u
     read (unit, iostat=ios) m

@@ -1,11 +1,12 @@
 !> Library interface of the wannierisation step, MI355X build.
 !>
 !> Keeps the procedure names, arguments and error behaviour of wannier90's
-!> module w90_library (src/library_interface.F90) for the path without
-!> disentanglement: options through w90_set_option + w90_input_setopt or a
-!> .win file through w90_input_reader, caller-owned M/U/A arrays associated
-!> by w90_set_*, k-mesh through w90_create_kmesh/w90_get_nn/w90_get_nnkp,
-!> the Lowdin projection in w90_project_overlap and the minimisation in
+!> module w90_library (src/library_interface.F90) for the wannierisation
+!> step: options through w90_set_option + w90_input_setopt or a .win file
+!> through w90_input_reader, caller-owned M/U/A/eigenvalue arrays
+!> associated by w90_set_*, k-mesh through w90_create_kmesh/w90_get_nn/
+!> w90_get_nnkp, the Lowdin projection in w90_project_overlap, the
+!> disentanglement in w90_disentangle and the minimisation in
 !> w90_wannierise.  The arithmetic runs in libw90gpu.so through
 !> ISO_C_BINDING (module w90gpu_c); nothing falls back to the CPU.
 !> Status: ierr = 0 ok, > 0 error (codes of src/error.F90:27-36), < 0 warning.
@@ -76,6 +77,15 @@ module w90_library
     real(dp), allocatable :: spreads(:)
     type(w90gpu_result) :: result
     real(dp), allocatable :: history(:, :)
+    ! dis_control_type / dis_manifold_type, and what w90_disentangle leaves
+    type(w90gpu_dis_control) :: dis_control
+    logical :: dis_set = .false.
+    type(w90gpu_dis_result) :: dis_result
+    real(dp) :: omega_invariant = 0.0_dp
+    integer, allocatable :: ndimwin(:), nfirstwin(:)
+    logical, allocatable :: lwindow(:, :)
+    real(dp), allocatable :: eigval_opt(:, :)
+    real(dp), allocatable :: dis_history(:, :)
     ! options (set_option entries or .win data)
     type(w90_settings) :: settings
     logical :: have_win = .false.
@@ -302,15 +312,18 @@ contains
     end if
     cd%num_bands = cd%num_wann
     call get_int(cd%settings, 'num_bands', cd%num_bands, found, ierr)
-    if (cd%num_bands > cd%num_wann) then
-      ierr = 5
-      msg = 'Error: num_bands > num_wann needs disentanglement, which this build does not carry'
-      return
-    else if (cd%num_bands < cd%num_wann) then
+    if (cd%num_bands < cd%num_wann) then
       ierr = 5
       msg = 'Error: num_bands must be greater than or equal to num_wann'
       return
+    else if (cd%num_bands > W90GPU_DIS_MAX_BANDS .and. cd%num_bands > cd%num_wann) then
+      ierr = 5
+      write (msg, '(a,i0,a)') 'Error: disentanglement in this build takes at most ', W90GPU_DIS_MAX_BANDS, &
+        ' bands (num_bands)'
+      return
     end if
+    call read_dis_settings(cd, ierr, msg)
+    if (ierr /= 0) return
     call get_i1d(cd%settings, 'mp_grid', cd%mp_grid, 3, found, ierr)
     if (.not. found .or. ierr /= 0) then
       ierr = 5
@@ -477,6 +490,73 @@ contains
   end subroutine apply_settings
 
   ! ------------------------------------------------------------ k-mesh
+
+  !> The disentanglement keywords (w90_readwrite_read_dis_manifold,
+  !> src/readwrite.F90:636-684, and the dis_control part of
+  !> src/wannier90_readwrite.F90:815-856), with the reference's checks.
+  subroutine read_dis_settings(cd, ierr, msg)
+    type(lib_common_type), intent(inout) :: cd
+    integer, intent(out) :: ierr
+    character(len=*), intent(out) :: msg
+    logical :: found, found2
+    integer :: n
+    ierr = 0
+    msg = ''
+    call w90gpu_dis_control_defaults(cd%dis_control)
+    cd%dis_set = .true.
+    call get_real(cd%settings, 'dis_win_min', cd%dis_control%win_min, found, ierr)
+    call get_real(cd%settings, 'dis_win_max', cd%dis_control%win_max, found, ierr)
+    if (cd%dis_control%win_max < cd%dis_control%win_min) then
+      ierr = 5
+      msg = 'Error: w90_readwrite_read_dis_manifold: check disentanglement windows (win_max < win_min !)'
+      return
+    end if
+    call get_real(cd%settings, 'dis_froz_max', cd%dis_control%froz_max, found, ierr)
+    if (found) cd%dis_control%frozen_states = 1
+    call get_real(cd%settings, 'dis_froz_min', cd%dis_control%froz_min, found2, ierr)
+    if (cd%dis_control%froz_max < cd%dis_control%froz_min) then
+      ierr = 5
+      msg = 'Error: w90_readwrite_read_dis_manifold: check disentanglement frozen windows'
+      return
+    end if
+    if (found2 .and. .not. found) then
+      ierr = 5
+      msg = 'Error: w90_readwrite_read_dis_manifold: found dis_froz_min but not dis_froz_max'
+      return
+    end if
+    call get_int(cd%settings, 'dis_num_iter', cd%dis_control%num_iter, found, ierr)
+    if (cd%dis_control%num_iter < 0) then
+      ierr = 5
+      msg = 'Error: dis_num_iter must be positive'
+      return
+    end if
+    call get_real(cd%settings, 'dis_mix_ratio', cd%dis_control%mix_ratio, found, ierr)
+    if (cd%dis_control%mix_ratio <= 0.0_dp .or. cd%dis_control%mix_ratio > 1.0_dp) then
+      ierr = 5
+      msg = 'Error: dis_mix_ratio must be greater than 0.0 but not greater than 1.0'
+      return
+    end if
+    call get_real(cd%settings, 'dis_conv_tol', cd%dis_control%conv_tol, found, ierr)
+    if (cd%dis_control%conv_tol < 0.0_dp) then
+      ierr = 5
+      msg = 'Error: dis_conv_tol must be positive'
+      return
+    end if
+    call get_int(cd%settings, 'dis_conv_window', cd%dis_control%conv_window, found, ierr)
+    if (cd%dis_control%conv_window < 0) then
+      ierr = 5
+      msg = 'Error: dis_conv_window must be positive'
+      return
+    end if
+    n = 0
+    call get_int(cd%settings, 'dis_spheres_num', n, found, ierr)
+    if (n > 0) then
+      ierr = 5
+      msg = 'Error: dis_spheres is not supported by this build'
+      return
+    end if
+    if (ierr /= 0) msg = 'Error: malformed disentanglement keyword'
+  end subroutine read_dis_settings
 
   subroutine ensure_context(cd, istdout, istderr, ierr)
     type(lib_common_type), intent(inout) :: cd
@@ -657,6 +737,7 @@ contains
     type(c_ptr) :: sess
     integer(c_int) :: rc, n
     real(c_double), allocatable :: hist(:)
+    complex(dp), allocatable :: mw(:, :, :, :)
     ierr = 0
     if (.not. associated(common_data%m_matrix_local)) then
       call report(istdout, istderr, 1, 'm_matrix_local not set for wannierise', ierr)
@@ -688,12 +769,16 @@ contains
       call report(istdout, istderr, int(rc), w90gpu_error_message(common_data%ctx), ierr)
       return
     end if
-    rc = w90gpu_session_load(sess, common_data%m_matrix_local, common_data%u_matrix)
+    ! after a disentanglement m_matrix_local is (num_bands, num_bands, ...)
+    ! with the overlaps in its num_wann x num_wann corner (setup_m_loc)
+    call pack_m(common_data, mw)
+    rc = w90gpu_session_load(sess, mw, common_data%u_matrix)
     if (rc == W90GPU_OK) rc = w90gpu_session_solve(sess, common_data%result)
     if (allocated(common_data%centres)) deallocate (common_data%centres, common_data%spreads)
     allocate (common_data%centres(3, common_data%num_wann), common_data%spreads(common_data%num_wann))
-    if (rc <= 0) rc = w90gpu_session_fetch(sess, common_data%m_matrix_local, common_data%u_matrix, &
-                                          common_data%centres, common_data%spreads)
+    if (rc <= 0) rc = w90gpu_session_fetch(sess, mw, common_data%u_matrix, common_data%centres, &
+                                          common_data%spreads)
+    if (rc <= 0) call unpack_m(common_data, mw)
     if (rc <= 0) then
       allocate (hist(4 * (common_data%control%num_iter + 1)))
       n = w90gpu_session_history(sess, hist, int(common_data%control%num_iter + 1, c_int))
@@ -705,16 +790,137 @@ contains
     rc = w90gpu_session_destroy(sess)
   end subroutine w90_wannierise
 
-  !> Disentanglement is not carried by this build: a no-op when
-  !> num_bands = num_wann (nothing to disentangle), an input error otherwise.
+  !> The num_wann x num_wann overlaps out of m_matrix_local (its corner when
+  !> it is num_bands x num_bands) into a contiguous array, and back.
+  subroutine pack_m(cd, mw)
+    type(lib_common_type), intent(in) :: cd
+    complex(dp), allocatable, intent(out) :: mw(:, :, :, :)
+    integer :: nw
+    nw = cd%num_wann
+    allocate (mw(nw, nw, cd%nntot, cd%num_kpts))
+    mw = cd%m_matrix_local(1:nw, 1:nw, 1:cd%nntot, 1:cd%num_kpts)
+  end subroutine pack_m
+
+  subroutine unpack_m(cd, mw)
+    type(lib_common_type), intent(inout) :: cd
+    complex(dp), intent(in) :: mw(:, :, :, :)
+    integer :: nw
+    nw = cd%num_wann
+    cd%m_matrix_local(1:nw, 1:nw, 1:cd%nntot, 1:cd%num_kpts) = mw
+  end subroutine unpack_m
+
+  !> w90_disentangle (src/library_interface.F90:362-420): dis_main and
+  !> setup_m_loc on the GPU.  On entry u_opt holds the projections A
+  !> (num_bands, num_wann, num_kpts), m_matrix_local the overlaps of all
+  !> bands and eigval the band energies; on exit u_opt is U_opt, u_matrix
+  !> the initial U of wannierise, and the num_wann x num_wann corner of each
+  !> block of m_matrix_local the overlaps in their gauge.  The DIS table of
+  !> dis_extract goes to istdout.
   subroutine w90_disentangle(common_data, istdout, istderr, ierr)
-    type(lib_common_type), intent(inout) :: common_data
+    type(lib_common_type), intent(inout), target :: common_data
     integer, intent(in) :: istdout, istderr
     integer, intent(out) :: ierr
+    type(w90gpu_problem) :: prob
+    integer(c_int) :: rc
+    integer :: nb, nw, nk, nn, k, j, rows
+    complex(dp), allocatable :: a(:, :, :), mw(:, :, :, :)
+    real(dp), allocatable :: hist(:, :)
+    integer(c_int), allocatable :: nfirst(:), ndim(:)
     ierr = 0
-    if (common_data%num_bands > common_data%num_wann) &
-      call report(istdout, istderr, 5, 'Error: disentanglement is not supported by this build', ierr)
+    if (.not. associated(common_data%m_matrix_local)) then
+      call report(istdout, istderr, 1, 'm_matrix_local not associated for disentangle call', ierr)
+      return
+    else if (.not. associated(common_data%u_matrix)) then
+      call report(istdout, istderr, 1, 'u_matrix not associated for disentangle call', ierr)
+      return
+    else if (.not. associated(common_data%u_opt)) then
+      call report(istdout, istderr, 1, 'u_opt not associated for disentangle call', ierr)
+      return
+    else if (.not. associated(common_data%eigval)) then
+      call report(istdout, istderr, 1, 'eigval not associated for disentangle call', ierr)
+      return
+    end if
+    if (common_data%gamma_only) then
+      call report(istdout, istderr, 5, 'Error: gamma_only disentanglement is not supported by this build', ierr)
+      return
+    end if
+    if (.not. common_data%setup_complete) then
+      call w90_create_kmesh(common_data, istdout, istderr, ierr)
+      if (ierr > 0) return
+    end if
+    if (.not. common_data%dis_set) then
+      call w90gpu_dis_control_defaults(common_data%dis_control)
+      common_data%dis_set = .true.
+    end if
+    call ensure_context(common_data, istdout, istderr, ierr)
+    if (ierr > 0) return
+    nb = common_data%num_bands
+    nw = common_data%num_wann
+    nk = common_data%num_kpts
+    nn = common_data%nntot
+    call make_problem(common_data, prob)
+    allocate (a(nb, nw, nk), mw(nw, nw, nn, nk), nfirst(nk), ndim(nk))
+    a = common_data%u_opt(1:nb, 1:nw, 1:nk)
+    rows = max(common_data%dis_control%num_iter, 1)
+    allocate (hist(3, rows))
+    if (allocated(common_data%eigval_opt)) deallocate (common_data%eigval_opt)
+    allocate (common_data%eigval_opt(nb, nk))
+    rc = w90gpu_disentangle(common_data%ctx, prob, int(nb, c_int), common_data%dis_control, common_data%eigval, a, &
+                            common_data%m_matrix_local, common_data%u_opt, common_data%u_matrix, mw, &
+                            common_data%eigval_opt, nfirst, ndim, hist, int(rows, c_int), common_data%dis_result)
+    if (rc /= W90GPU_OK) then
+      call report(istdout, istderr, int(rc), w90gpu_error_message(common_data%ctx), ierr)
+      return
+    end if
+    call unpack_m(common_data, mw)
+    if (allocated(common_data%ndimwin)) deallocate (common_data%ndimwin, common_data%nfirstwin, &
+                                                    common_data%lwindow)
+    allocate (common_data%ndimwin(nk), common_data%nfirstwin(nk), common_data%lwindow(nb, nk))
+    common_data%ndimwin = ndim
+    common_data%nfirstwin = nfirst
+    common_data%lwindow = .false.
+    do k = 1, nk
+      do j = nfirst(k), nfirst(k) + ndim(k) - 1
+        common_data%lwindow(j, k) = .true.
+      end do
+    end do
+    if (allocated(common_data%dis_history)) deallocate (common_data%dis_history)
+    allocate (common_data%dis_history(3, common_data%dis_result%iterations))
+    if (common_data%dis_result%iterations > 0) &
+      common_data%dis_history = hist(:, 1:common_data%dis_result%iterations)
+    common_data%omega_invariant = common_data%dis_result%omega_invariant
+    common_data%have_disentangled = .true.
+    if (common_data%iprint > 0) call print_dis(common_data, istdout)
   end subroutine w90_disentangle
+
+  !> The DIS table and the closing lines of dis_extract
+  !> (src/disentangle.F90:2017-2023, :2209-2212, :2545-2571), with the
+  !> device time spread evenly over the iterations in the time column.
+  subroutine print_dis(cd, istdout)
+    type(lib_common_type), intent(in) :: cd
+    integer, intent(in) :: istdout
+    integer :: it, nit
+    nit = cd%dis_result%iterations
+    write (istdout, '(/1x,a)') '                  Extraction of optimally-connected subspace                  '
+    write (istdout, '(1x,a)') '                  ------------------------------------------                  '
+    write (istdout, '(1x,a)') '+---------------------------------------------------------------------+<-- DIS'
+    write (istdout, '(1x,a)') '|  Iter     Omega_I(i-1)      Omega_I(i)      Delta (frac.)    Time   |<-- DIS'
+    write (istdout, '(1x,a)') '+---------------------------------------------------------------------+<-- DIS'
+    do it = 1, nit
+      write (istdout, '(2x,i6,3x,f14.8,3x,f14.8,6x,es10.3,2x,f8.2,4x,a)') it, cd%dis_history(1, it), &
+        cd%dis_history(2, it), cd%dis_history(3, it), cd%dis_result%seconds * it / max(nit, 1), '<-- DIS'
+    end do
+    if (cd%dis_result%converged == 1) then
+      write (istdout, '(/13x,a,es10.3,a,i2,a)') '<<<      Delta <', cd%dis_control%conv_tol, '  over ', &
+        cd%dis_control%conv_window, ' iterations     >>>'
+      write (istdout, '(13x,a)') '<<< Disentanglement convergence criteria satisfied >>>'
+    else
+      write (istdout, '(/5x,a)') '<<< Warning: Maximum number of disentanglement iterations reached >>>'
+      write (istdout, '(10x,a)') '<<< Disentanglement convergence criteria not satisfied >>>'
+    end if
+    write (istdout, '(/8x,a,f14.8,a/)') 'Final Omega_I ', cd%omega_invariant, ' (Ang^2)'
+    write (istdout, '(1x,a/)') '+----------------------------------------------------------------------------+'
+  end subroutine print_dis
 
   !> The output writers of plot_main on this path: _centres.xyz (write_xyz)
   !> and _u.mat (write_u_matrices).  Band/Fermi-surface/WF plotting is not
@@ -1143,23 +1349,81 @@ contains
              common_data%result%omega_od, common_data%result%omega_total]
   end subroutine w90_get_omega
 
+  !> .chk (w90_wannier90_readwrite_write_chkpt); with the disentanglement
+  !> records after w90_disentangle.  Centres and spreads not computed yet
+  !> (a 'postdis' checkpoint) are written as zeros.
   subroutine w90_write_chkpt(common_data, chkpt)
     type(lib_common_type), intent(in) :: common_data
     character(len=*), intent(in) :: chkpt
-    call write_chk(trim(common_data%seedname)//'.chk', chkpt, common_data%num_bands, &
-                   common_data%real_lattice, common_data%num_kpts, common_data%mp_grid, &
-                   common_data%kpt_latt, common_data%nntot, common_data%num_wann, common_data%u_matrix, &
-                   common_data%m_matrix_local, common_data%centres, common_data%spreads)
+    complex(dp), allocatable :: mw(:, :, :, :)
+    real(dp), allocatable :: cen(:, :), spr(:)
+    integer :: nw, nb
+    nw = common_data%num_wann
+    nb = common_data%num_bands
+    call pack_m(common_data, mw)
+    allocate (cen(3, nw), spr(nw))
+    cen = 0.0_dp
+    spr = 0.0_dp
+    if (allocated(common_data%centres)) then
+      cen = common_data%centres
+      spr = common_data%spreads
+    end if
+    if (common_data%have_disentangled) then
+      call write_chk(trim(common_data%seedname)//'.chk', chkpt, nb, common_data%real_lattice, &
+                     common_data%num_kpts, common_data%mp_grid, common_data%kpt_latt, common_data%nntot, nw, &
+                     common_data%u_matrix, mw, cen, spr, common_data%omega_invariant, common_data%lwindow, &
+                     common_data%ndimwin, common_data%u_opt(1:nb, 1:nw, 1:common_data%num_kpts))
+    else
+      call write_chk(trim(common_data%seedname)//'.chk', chkpt, nb, common_data%real_lattice, &
+                     common_data%num_kpts, common_data%mp_grid, common_data%kpt_latt, common_data%nntot, nw, &
+                     common_data%u_matrix, mw, cen, spr)
+    end if
   end subroutine w90_write_chkpt
 
+  !> U and M from the .chk, and after a disentanglement U_opt, lwindow,
+  !> ndimwin and Omega_I (the restart from 'postdis' or 'postwann').
   subroutine w90_read_chkpt(common_data, istdout, istderr, ierr)
     type(lib_common_type), intent(inout) :: common_data
     integer, intent(in) :: istdout, istderr
     integer, intent(out) :: ierr
     character(len=256) :: msg
-    call read_chk(trim(common_data%seedname)//'.chk', common_data%num_wann, common_data%num_kpts, &
-                  common_data%nntot, common_data%u_matrix, common_data%m_matrix_local, ierr, msg)
-    if (ierr > 0) call report(istdout, istderr, ierr, msg, ierr)
+    complex(dp), allocatable :: mw(:, :, :, :)
+    integer :: nb, nw, nk, k, j
+    logical :: dis
+    nb = common_data%num_bands
+    nw = common_data%num_wann
+    nk = common_data%num_kpts
+    allocate (mw(nw, nw, common_data%nntot, nk))
+    if (allocated(common_data%ndimwin)) deallocate (common_data%ndimwin, common_data%nfirstwin, &
+                                                    common_data%lwindow)
+    allocate (common_data%ndimwin(nk), common_data%nfirstwin(nk), common_data%lwindow(nb, nk))
+    if (nb > nw .and. .not. associated(common_data%u_opt)) then
+      call report(istdout, istderr, 1, 'u_opt not associated for reading a disentangled checkpoint', ierr)
+      return
+    end if
+    if (nb > nw) then
+      call read_chk(trim(common_data%seedname)//'.chk', nw, nk, common_data%nntot, common_data%u_matrix, mw, &
+                    ierr, msg, nb, dis, common_data%omega_invariant, common_data%lwindow, common_data%ndimwin, &
+                    common_data%u_opt)
+    else
+      call read_chk(trim(common_data%seedname)//'.chk', nw, nk, common_data%nntot, common_data%u_matrix, mw, &
+                    ierr, msg)
+      dis = .false.
+    end if
+    if (ierr > 0) then
+      call report(istdout, istderr, ierr, msg, ierr)
+      return
+    end if
+    call unpack_m(common_data, mw)
+    common_data%have_disentangled = dis
+    if (dis) then
+      do k = 1, nk
+        common_data%nfirstwin(k) = 0
+        do j = nb, 1, -1
+          if (common_data%lwindow(j, k)) common_data%nfirstwin(k) = j
+        end do
+      end do
+    end if
   end subroutine w90_read_chkpt
 
   subroutine w90_write_nnkp(common_data)

@@ -1,25 +1,27 @@
 !> wannier90_gpu.x [-pp] seedname
 !>
-!> The wannier90.x driver (src/wannier_prog.F90) for the wannierisation step
-!> without disentanglement, with the minimisation on an MI355X: reads
+!> The wannier90.x driver (src/wannier_prog.F90) for the wannierisation
+!> step, with the disentanglement and the minimisation on an MI355X: reads
 !> seedname.win, builds the k-mesh (-pp: writes seedname.nnkp and stops),
-!> reads seedname.mmn/.amn (or, with restart = wannierise, U and M from
-!> seedname.chk), projects (Lowdin), minimises the spread, and writes
-!> seedname.wout, seedname.chk ('postwann'), and, when asked for,
-!> seedname_centres.xyz and seedname_u.mat.
+!> reads seedname.mmn/.amn (and seedname.eig when num_bands > num_wann), or
+!> with restart = wannierise U, M (and U_opt) from seedname.chk; projects
+!> (Lowdin) or disentangles (then writes seedname.chk 'postdis'), minimises
+!> the spread, and writes seedname.wout, seedname.chk ('postwann'), and,
+!> when asked for, seedname_centres.xyz and seedname_u.mat.
 program wannier90_gpu
   use iso_c_binding
   use w90_library
-  use w90gpu_io, only: dp, read_mmn, read_amn, datetime
+  use w90gpu_io, only: dp, read_mmn, read_amn, read_eig, datetime
   implicit none
   type(lib_common_type), target :: cd
   character(len=256) :: arg, seed, msg
   character(len=9) :: cdate, ctime
   logical :: pp
-  integer :: i, nargs, stdout, stderr, ierr, nk, nw, nn, iw
+  integer :: i, nargs, stdout, stderr, ierr, nk, nw, nn, iw, nb
   integer(8) :: c0, c1, crate
   real(dp) :: wall
   complex(dp), allocatable, target :: m(:, :, :, :), u(:, :, :), uopt(:, :, :), a(:, :, :)
+  real(dp), allocatable, target :: eig(:, :)
 
   call system_clock(c0, crate)
   pp = .false.
@@ -42,7 +44,7 @@ program wannier90_gpu
   call datetime(cdate, ctime)
   write (stdout, '(1x,a)') '+---------------------------------------------------+'
   write (stdout, '(1x,a)') '|   wannier90_gpu: wannierisation on AMD MI355X     |'
-  write (stdout, '(1x,a)') '|   (spread minimisation without disentanglement)   |'
+  write (stdout, '(1x,a)') '|   (disentanglement and spread minimisation)       |'
   write (stdout, '(1x,a)') '+---------------------------------------------------+'
   write (stdout, '(1x,a)') 'Execution started on '//cdate//' at '//ctime
   write (stdout, *)
@@ -67,7 +69,8 @@ program wannier90_gpu
     stop
   end if
 
-  allocate (m(nw, nw, nn, nk), u(nw, nw, nk), uopt(nw, nw, nk))
+  nb = cd%num_bands
+  allocate (m(nb, nb, nn, nk), u(nw, nw, nk), uopt(nb, nw, nk))
   uopt = (0.0_dp, 0.0_dp)
   do i = 1, nk
     do iw = 1, nw
@@ -82,6 +85,21 @@ program wannier90_gpu
     write (stdout, '(1x,a)') 'Restarting the wannierisation from '//trim(seed)//'.chk'
     call w90_read_chkpt(cd, stdout, stderr, ierr)
     if (ierr > 0) call fail()
+  else if (nb > nw) then
+    ! disentanglement: the projections go in u_opt, the band energies from .eig
+    allocate (eig(nb, nk))
+    call read_eig(trim(seed)//'.eig', nb, nk, eig, ierr, msg)
+    if (ierr > 0) call fail_msg(msg)
+    call w90_set_eigval(cd, eig)
+    call read_mmn(trim(seed)//'.mmn', nb, nk, nn, cd%nnlist, cd%nncell, m, ierr, msg)
+    if (ierr > 0) call fail_msg(msg)
+    call read_amn(trim(seed)//'.amn', nb, nk, nw, uopt, ierr, msg)
+    if (ierr > 0) call fail_msg(msg)
+    call w90_disentangle(cd, stdout, stderr, ierr)
+    if (ierr > 0) call fail()
+    write (stdout, '(1x,a,f12.3,a,i6,a)') 'Time for disentangle (device) ', cd%dis_result%seconds, ' (sec), ', &
+      cd%dis_result%iterations, ' iterations'
+    call w90_write_chkpt(cd, 'postdis')
   else
     call read_mmn(trim(seed)//'.mmn', nw, nk, nn, cd%nnlist, cd%nncell, m, ierr, msg)
     if (ierr > 0) call fail_msg(msg)
