@@ -9,13 +9,16 @@
 //                     leading eigenvectors of Q P Q for the rest;
 //   k_dis_orthonorm   internal_check_orthonorm (:372-451);
 //   k_dis_zmat        internal_zmatrix (:2922-2988) with the Omega_I terms
-//                     of dis_extract (:2069-2104, :2229-2256): for every
-//                     neighbour b, C = M(k,b) U_opt(k+b) once, from which
-//                     Z(k) += w_b C C^+ (non-frozen rows), the spread term
-//                     w_b ||U_opt(k)^+ C||^2 and its frozen rows;
-//   k_dis_update      one extraction step (:2029-2068, :2115-2160): Z mixed
-//                     with the previous one, its leading eigenvectors as the
-//                     new non-frozen columns of U_opt(k);
+//                     of dis_extract (:2069-2104, :2229-2256), one
+//                     workgroup per (k, b): C = M(k,b) U_opt(k+b) once, from
+//                     which the term w_b C C^+ of Z(k) (non-frozen rows),
+//                     the spread term w_b ||U_opt(k)^+ C||^2 and its frozen
+//                     rows;
+//   k_dis_update      one extraction step (:2029-2068, :2115-2160): Z (the
+//                     terms summed over b) mixed with the previous one, its
+//                     leading eigenvectors as the new non-frozen columns of
+//                     U_opt(k), the eigenproblem warm-started from the
+//                     previous iteration's eigenvectors;
 //   k_dis_reduce      one workgroup: Omega_I and its estimate over k, the
 //                     change, and the convergence window (:2866-2920);
 //   k_dis_ham         the Hamiltonian in the optimal subspace and U_opt in
@@ -102,6 +105,7 @@ enum : int {
 
 struct DisProb {
   int nb, nw, nk, nn, ld;  // ld: leading dimension of the LDS matrices (nb + 1)
+  int ntri;                // nb (nb + 1) / 2
   double wbtot;
   const int* nnl;      // [k * nn + b], 0-based
   const double* wb;    // [nn]
@@ -111,13 +115,13 @@ struct DisProb {
   const zd* A;         // (nb, nw, nk), all bands
   zd* Uopt;            // (nb, nw, nk), rows relative to the window
   zd* Zin;             // (nb, nb, nk)
-  zd* Zout;            // (nb, nb, nk)
+  zd* Zpart;           // (ntri, nn, nk): w_b C C^+ per (k, b), packed upper triangle
+  zd* Vprev;           // (nb, nb, nk): eigenvectors of the last Z
   zd* U;               // (nw, nw, nk)
   zd* W;               // (nb, nw, nk): U_opt U
   zd* Mw;              // (nw, nw, nn, nk)
   double* wk1;         // [nk] Omega_I estimate terms
-  double* wkom;        // [nk] Omega_I terms
-  double* frz;         // [nk] frozen-row spread terms
+  double* part;        // (2, nn, nk): Omega_I terms and frozen-row terms per (k, b)
   double* eigopt;      // (nb, nk)
   double* log;         // (3, num_iter)
   DisState* st;
@@ -164,12 +168,14 @@ struct JacobiLds {
 // tolerance and V (LDS) holds the eigenvectors in its columns.  For each
 // pair, D = diag(1, b/|b|) makes the 2 x 2 block real, and the real
 // rotation that diagonalises it (tan 2t = 2|b| / (a_qq - a_pp), the smaller
-// angle) gives J = D R D^+.  A rotation is skipped when |a_pq| <= 1e-14
-// sqrt(|a_pp a_qq|) (relative to the diagonal) or <= 1e-14 ||A||_F (the
-// eigenvalues are then exact to that much of the norm, as LAPACK's; without
+// angle) gives J = D R D^+.  A rotation is skipped when |a_pq| <= 1e-15
+// sqrt(|a_pp a_qq|) (relative to the diagonal) or <= 1e-16 ||A||_F (without
 // this floor the rounding noise among the zero eigenvalues of a
 // rank-deficient matrix would be rotated forever); the sweeps end when a
-// whole sweep skips every pair.  Returns the
+// whole sweep skips every pair.  A warm start (a nearly diagonal A) leaves
+// off-diagonal entries just below these bounds, so they are kept tight:
+// the selected subspace moves by their size over the eigenvalue gap at the
+// selection.  Returns the
 // number of sweeps (kMaxSweeps: not converged).
 __device__ int jacobi_eigh(zd* A, zd* V, int n, int ld, JacobiLds& J) {
   const int t = threadIdx.x;
@@ -180,7 +186,7 @@ __device__ int jacobi_eigh(zd* A, zd* V, int n, int ld, JacobiLds& J) {
     fro += zabs2(A[r + c * ld]);
   }
   fro = block_sum(fro, J.red);
-  const double floor2 = fro * 1e-28;
+  const double floor2 = fro * 1e-32;
   const int m = n + (n & 1), half = m / 2;
   int sweep = 0;
   for (; sweep < kMaxSweeps; ++sweep) {
@@ -199,7 +205,7 @@ __device__ int jacobi_eigh(zd* A, zd* V, int n, int ld, JacobiLds& J) {
           const zd apq = A[R.p + R.q * ld];
           const double b2 = zabs2(apq);
           const double app = A[R.p + R.p * ld].x, aqq = A[R.q + R.q * ld].x;
-          if (b2 > floor2 && b2 > 1e-28 * fabs(app * aqq)) {
+          if (b2 > floor2 && b2 > 1e-30 * fabs(app * aqq)) {
             const double bn = sqrt(b2);
             const double tau = (aqq - app) / (2.0 * bn);
             double tt;
@@ -552,9 +558,7 @@ __global__ void __launch_bounds__(kNT) k_dis_orthonorm(DisProb p) {
   if (bad) atomicOr(p.flags, F_ORTHONORM);
 }
 
-// Upper-triangle entries of an n x n matrix per thread in k_dis_zmat.
-constexpr int kZPer = (kMaxN * (kMaxN + 1) / 2 + kNT - 1) / kNT;
-
+// Position (m, n), m <= n, of entry e of a packed upper triangle.
 __device__ __forceinline__ void tri_index(int e, int& m, int& n) {
   // e = n (n + 1) / 2 + m, m <= n
   int c = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
@@ -564,119 +568,114 @@ __device__ __forceinline__ void tri_index(int e, int& m, int& n) {
   m = e - c * (c + 1) / 2;
 }
 
-// internal_zmatrix and the Omega_I terms for one k, with U_opt of every k
-// as it is now: Zout(k) (non-frozen rows, upper triangle mirrored),
-// wkom[k] = nw wbtot - sum_b w_b ||U_opt(k)^+ C_b||^2 and frz[k] = the same
-// sum over the frozen rows of U_opt(k)^+ C_b, with C_b = M(k,b) U_opt(k+b).
+// internal_zmatrix and the Omega_I terms for one (k, b), one workgroup
+// each, with U_opt of every k as it is now: C = M(k,b) U_opt(k+b) and
+//   Zpart(k,b) = w_b C C^+ over the non-frozen rows (upper triangle,
+//                packed), summed over b by k_dis_update;
+//   part(0,k,b) = w_b ||U_opt(k)^+ C||^2, the Omega_I term, summed over b
+//                by k_dis_reduce;
+//   part(1,k,b) = the same over the frozen rows of U_opt(k)^+ C, the frozen
+//                term of the next extraction step.
 __global__ void __launch_bounds__(kNT) k_dis_zmat(DisProb p) {
   __shared__ double red[kNT / 64];
   const DisState* st = p.st;
   if (st->done) return;
-  const int k = blockIdx.x, t = threadIdx.x;
+  const int nn = p.nn;
+  const int k = blockIdx.x / nn, b = blockIdx.x % nn, t = threadIdx.x;
   const Win w = p.win[k];
-  const int nb = p.nb, nw = p.nw, ld = p.ld, nn = p.nn;
+  const int nb = p.nb, nw = p.nw, ld = p.ld;
   const int nd = w.ndim - w.nfroz;
-  const bool want_z = nw > w.nfroz;
   const int ntri = nd * (nd + 1) / 2;
   zd* C = dis_lds;            // ndim(k) x nw
   zd* U2 = dis_lds + nb * ld; // ndim(k+b) x nw
   const zd* Uk = p.Uopt + (size_t)k * nb * nw;
-  zd z[kZPer];
-  for (int s = 0; s < kZPer; ++s) z[s] = zmk(0.0, 0.0);
-  double om = 0.0, fz = 0.0;
-  for (int b = 0; b < nn; ++b) {
-    const int k2 = p.nnl[k * nn + b];
-    const Win w2 = p.win[k2];
-    const zd* Ukb = p.Uopt + (size_t)k2 * nb * nw;
-    for (int e = t; e < w2.ndim * nw; e += kNT) {
-      const int l = e % w2.ndim, j = e / w2.ndim;
-      U2[l + j * ld] = Ukb[l + (size_t)j * nb];
-    }
-    __syncthreads();
-    const zd* Mkb = p.M + ((size_t)k * nn + b) * nb * nb + w.first + (size_t)w2.first * nb;
-    for (int e = t; e < w.ndim * nw; e += kNT) {
-      const int i = e % w.ndim, j = e / w.ndim;
-      zd c = zmk(0.0, 0.0);
-      for (int l = 0; l < w2.ndim; ++l) zfma(c, Mkb[i + (size_t)l * nb], U2[l + j * ld]);
-      C[i + j * ld] = c;
-    }
-    __syncthreads();
-    const double wbb = p.wb[b];
-    if (want_z) {
-      for (int s = 0; s < kZPer; ++s) {
-        const int e = t + s * kNT;
-        if (e >= ntri) break;
-        int m, n;
-        tri_index(e, m, n);
-        const int pr = nf_row(w, m), qr = nf_row(w, n);
-        zd cs = zmk(0.0, 0.0);
-        for (int l = 0; l < nw; ++l) zfma(cs, C[pr + l * ld], zconj(C[qr + l * ld]));
-        z[s] = zadd(z[s], zscale(cs, wbb));
-      }
-    }
-    double r = 0.0, rf = 0.0;
-    for (int e = t; e < nw * nw; e += kNT) {
-      const int i = e % nw, j = e / nw;
-      zd v = zmk(0.0, 0.0);
-      for (int l = 0; l < w.ndim; ++l) zcfma(v, Uk[l + (size_t)i * nb], C[l + j * ld]);
-      const double a2 = zabs2(v);
-      r += a2;
-      if (i < w.nfroz) rf += a2;
-    }
-    om += wbb * r;
-    fz += wbb * rf;
-    __syncthreads();
+  const int k2 = p.nnl[k * nn + b];
+  const Win w2 = p.win[k2];
+  const zd* Ukb = p.Uopt + (size_t)k2 * nb * nw;
+  for (int e = t; e < w2.ndim * nw; e += kNT) {
+    const int l = e % w2.ndim, j = e / w2.ndim;
+    U2[l + j * ld] = Ukb[l + (size_t)j * nb];
   }
-  if (want_z) {
-    zd* Zo = p.Zout + (size_t)k * nb * nb;
-    for (int s = 0; s < kZPer; ++s) {
-      const int e = t + s * kNT;
-      if (e >= ntri) break;
+  __syncthreads();
+  const zd* Mkb = p.M + ((size_t)k * nn + b) * nb * nb + w.first + (size_t)w2.first * nb;
+  for (int e = t; e < w.ndim * nw; e += kNT) {
+    const int i = e % w.ndim, j = e / w.ndim;
+    zd c = zmk(0.0, 0.0);
+    for (int l = 0; l < w2.ndim; ++l) zfma(c, Mkb[i + (size_t)l * nb], U2[l + j * ld]);
+    C[i + j * ld] = c;
+  }
+  __syncthreads();
+  const double wbb = p.wb[b];
+  const size_t kb = (size_t)k * nn + b;
+  if (nw > w.nfroz) {
+    zd* Zp = p.Zpart + kb * p.ntri;
+    for (int e = t; e < ntri; e += kNT) {
       int m, n;
       tri_index(e, m, n);
-      Zo[m + (size_t)n * nb] = z[s];
-      Zo[n + (size_t)m * nb] = zconj(z[s]);
+      const int pr = nf_row(w, m), qr = nf_row(w, n);
+      zd cs = zmk(0.0, 0.0);
+      for (int l = 0; l < nw; ++l) zfma(cs, C[pr + l * ld], zconj(C[qr + l * ld]));
+      Zp[e] = zscale(cs, wbb);
     }
   }
-  om = block_sum(om, red);
-  fz = block_sum(fz, red);
+  double r = 0.0, rf = 0.0;
+  for (int e = t; e < nw * nw; e += kNT) {
+    const int i = e % nw, j = e / nw;
+    zd v = zmk(0.0, 0.0);
+    for (int l = 0; l < w.ndim; ++l) zcfma(v, Uk[l + (size_t)i * nb], C[l + j * ld]);
+    const double a2 = zabs2(v);
+    r += a2;
+    if (i < w.nfroz) rf += a2;
+  }
+  r = block_sum(r, red);
+  rf = block_sum(rf, red);
   if (t == 0) {
-    p.wkom[k] = (double)nw * p.wbtot - om;
-    p.frz[k] = fz;
+    p.part[2 * kb] = wbb * r;
+    p.part[2 * kb + 1] = wbb * rf;
   }
 }
 
-// One extraction step for one k: Zin mixed (the first iteration takes Zout
-// of the initial U_opt), its eigenvectors; the leading nw - nfroz replace
-// the non-frozen columns of U_opt(k) (in ascending order of eigenvalue) and
-// their eigenvalues are subtracted from the Omega_I estimate.
+// One extraction step for one k: Z of the last Z pass (its partial sums
+// over b) mixed with the previous Zin (the first iteration takes it
+// unmixed), its eigenvectors; the leading nw - nfroz replace the
+// non-frozen columns of U_opt(k) (in ascending order of eigenvalue) and
+// their eigenvalues are subtracted from the Omega_I estimate.  From the
+// second iteration on, the eigenproblem starts from the eigenvectors of the
+// previous one (Vprev): A = Vprev^+ Z Vprev is nearly diagonal and the
+// Jacobi sweeps end after one to three; the eigenvectors are Vprev V.
 __global__ void __launch_bounds__(kNT) k_dis_update(DisProb p) {
   __shared__ JacobiLds J;
   const DisState* st = p.st;
   if (st->done) return;
   const int k = blockIdx.x, t = threadIdx.x;
   const Win w = p.win[k];
-  const int nb = p.nb, nw = p.nw, ld = p.ld;
+  const int nb = p.nb, nw = p.nw, ld = p.ld, nn = p.nn;
   const int nd = w.ndim - w.nfroz, r = nw - w.nfroz;
+  const int ntri = nd * (nd + 1) / 2;
+  double fz = 0.0;
+  if (t == 0)
+    for (int b = 0; b < nn; ++b) fz += p.part[2 * ((size_t)k * nn + b) + 1];
   if (r <= 0) {
-    if (t == 0) p.wk1[k] = (double)nw * p.wbtot - p.frz[k];
+    if (t == 0) p.wk1[k] = (double)nw * p.wbtot - fz;
     return;
   }
   const bool first = st->iter == 0;
   const double mix = st->mix;
   zd* Zi = p.Zin + (size_t)k * nb * nb;
-  const zd* Zo = p.Zout + (size_t)k * nb * nb;
+  zd* Vp = p.Vprev + (size_t)k * nb * nb;
   zd* A = dis_lds;
   zd* V = dis_lds + nb * ld;
-  for (int e = t; e < nd * nd; e += kNT) {
-    const int i = e % nd, j = e / nd;  // row i, column j
-    if (i > j) continue;
+  for (int e = t; e < ntri; e += kNT) {
+    int i, j;  // row i <= column j
+    tri_index(e, i, j);
+    zd z = zmk(0.0, 0.0);
+    for (int b = 0; b < nn; ++b) z = zadd(z, p.Zpart[((size_t)k * nn + b) * p.ntri + e]);
     zd v;
     if (first) {
-      v = Zo[i + (size_t)j * nb];
+      v = z;
     } else {
-      const zd a = Zo[i + (size_t)j * nb], b = Zi[i + (size_t)j * nb];
-      v = zadd(zscale(a, mix), zscale(b, 1.0 - mix));
+      const zd zi = Zi[i + (size_t)j * nb];
+      v = zadd(zscale(z, mix), zscale(zi, 1.0 - mix));
     }
     Zi[i + (size_t)j * nb] = v;
     Zi[j + (size_t)i * nb] = zconj(v);
@@ -685,8 +684,62 @@ __global__ void __launch_bounds__(kNT) k_dis_update(DisProb p) {
     A[j + i * ld] = zconj(v);
   }
   __syncthreads();
+  if (!first) {
+    // A <- Vprev^+ A Vprev, through V
+    for (int e = t; e < nd * nd; e += kNT) {
+      const int i = e % nd, j = e / nd;
+      zd s = zmk(0.0, 0.0);
+      for (int l = 0; l < nd; ++l) zfma(s, A[i + l * ld], Vp[l + (size_t)j * nb]);
+      V[i + j * ld] = s;
+    }
+    __syncthreads();
+    for (int e = t; e < nd * nd; e += kNT) {
+      const int i = e % nd, j = e / nd;
+      if (i > j) continue;
+      zd s = zmk(0.0, 0.0);
+      for (int l = 0; l < nd; ++l) zcfma(s, Vp[l + (size_t)i * nb], V[l + j * ld]);
+      if (i == j) s.y = 0.0;
+      A[i + j * ld] = s;
+      A[j + i * ld] = zconj(s);
+    }
+    __syncthreads();
+  }
   if (jacobi_eigh(A, V, nd, ld, J) >= kMaxSweeps && t == 0) atomicOr(p.flags, F_JACOBI);
   eig_order(A, nd, ld, J);
+  // the eigenvectors of Z: E = Vprev V (or V) into the A area, then one
+  // Newton-Schulz step E <- E (3 I - E^+ E) / 2 to Vprev, which keeps the
+  // product of the iterations' rotations orthonormal to rounding (without
+  // it the basis drifts by ~n eps per iteration)
+  for (int e = t; e < nd * nd; e += kNT) {
+    const int i = e % nd, j = e / nd;
+    zd s;
+    if (first) {
+      s = V[i + j * ld];
+    } else {
+      s = zmk(0.0, 0.0);
+      for (int l = 0; l < nd; ++l) zfma(s, Vp[i + (size_t)l * nb], V[l + j * ld]);
+    }
+    A[i + j * ld] = s;
+  }
+  __syncthreads();
+  for (int e = t; e < nd * nd; e += kNT) {
+    const int i = e % nd, j = e / nd;
+    if (i > j) continue;
+    zd s = zmk(0.0, 0.0);
+    for (int l = 0; l < nd; ++l) zcfma(s, A[l + i * ld], A[l + j * ld]);
+    if (i == j) s.y = 0.0;
+    V[i + j * ld] = s;
+    V[j + i * ld] = zconj(s);
+  }
+  __syncthreads();
+  for (int e = t; e < nd * nd; e += kNT) {
+    const int i = e % nd, j = e / nd;
+    zd s = zscale(A[i + j * ld], 1.5);
+    for (int l = 0; l < nd; ++l) zfma(s, A[i + l * ld], zscale(V[l + j * ld], -0.5));
+    Vp[i + (size_t)j * nb] = s;
+  }
+  __threadfence_block();
+  __syncthreads();
   zd* U = p.Uopt + (size_t)k * nb * nw;
   for (int e = t; e < w.ndim * r; e += kNT) {
     const int i = e % w.ndim, l = e / w.ndim;
@@ -695,25 +748,35 @@ __global__ void __launch_bounds__(kNT) k_dis_update(DisProb p) {
   __syncthreads();
   for (int e = t; e < nd * r; e += kNT) {
     const int i = e % nd, l = e / nd;
-    U[nf_row(w, i) + (size_t)(w.nfroz + l) * nb] = V[i + J.perm[nd - r + l] * ld];
+    U[nf_row(w, i) + (size_t)(w.nfroz + l) * nb] = Vp[i + (size_t)J.perm[nd - r + l] * nb];
   }
   if (t == 0) {
     double s = (double)nw * p.wbtot;
-    if (w.nfroz > 0) s -= p.frz[k];
+    if (w.nfroz > 0) s -= fz;
     for (int l = 0; l < r; ++l) s -= J.lam[J.perm[nd - r + l]];
     p.wk1[k] = s;
   }
 }
 
-// Omega_I estimate and Omega_I over k (summed in k order), the change and
-// the convergence window (internal_test_convergence: the last conv_window
-// changes all below conv_tol, once conv_window iterations are done).
-__global__ void __launch_bounds__(64) k_dis_reduce(DisProb p) {
+// Omega_I estimate and Omega_I over k (per k its terms over b in order,
+// then a deterministic block sum), the change and the convergence window
+// (internal_test_convergence: the last conv_window changes all below
+// conv_tol, once conv_window iterations are done).
+__global__ void __launch_bounds__(kNT) k_dis_reduce(DisProb p) {
+  __shared__ double red[kNT / 64];
   DisState* st = p.st;
-  if (st->done || threadIdx.x != 0) return;
+  if (st->done) return;
+  const int t = threadIdx.x, nn = p.nn;
   double s1 = 0.0, s2 = 0.0;
-  for (int k = 0; k < p.nk; ++k) s1 += p.wk1[k];
-  for (int k = 0; k < p.nk; ++k) s2 += p.wkom[k];
+  for (int k = t; k < p.nk; k += kNT) {
+    s1 += p.wk1[k];
+    double om = 0.0;
+    for (int b = 0; b < nn; ++b) om += p.part[2 * ((size_t)k * nn + b)];
+    s2 += (double)p.nw * p.wbtot - om;
+  }
+  s1 = block_sum(s1, red);
+  s2 = block_sum(s2, red);
+  if (t != 0) return;
   const double womegai1 = s1 / p.nk, womegai = s2 / p.nk;
   const double delta = womegai1 / womegai - 1.0;
   const int iter = st->iter + 1;
@@ -992,9 +1055,10 @@ int w90gpu_disentangle(w90gpu_ctx* ctx, const w90gpu_problem* prob, int num_band
                o_win = ar.add(sizeof(Win) * nk), o_eig = ar.add(sizeof(double) * nb * nk),
                o_M = ar.add(zb * nb * nb * nn * nk), o_A = ar.add(zb * nb * nw * nk),
                o_Uopt = ar.add(zb * nb * nw * nk), o_Zin = ar.add(zb * nb * nb * nk),
-               o_Zout = ar.add(zb * nb * nb * nk), o_U = ar.add(zb * nw * nw * nk), o_W = ar.add(zb * nb * nw * nk),
+               o_Zpart = ar.add(zb * (nb * (nb + 1) / 2) * nn * nk), o_Vprev = ar.add(zb * nb * nb * nk),
+               o_U = ar.add(zb * nw * nw * nk), o_W = ar.add(zb * nb * nw * nk),
                o_Mw = ar.add(zb * nw * nw * nn * nk), o_wk1 = ar.add(sizeof(double) * nk),
-               o_wkom = ar.add(sizeof(double) * nk), o_frz = ar.add(sizeof(double) * nk),
+               o_part = ar.add(sizeof(double) * 2 * nn * nk),
                o_eigopt = ar.add(sizeof(double) * nb * nk), o_log = ar.add(sizeof(double) * 3 * nlog),
                o_st = ar.add(sizeof(DisState)), o_flags = ar.add(sizeof(int));
   DevBuf buf;
@@ -1034,13 +1098,14 @@ int w90gpu_disentangle(w90gpu_ctx* ctx, const w90gpu_problem* prob, int num_band
   p.A = (const zd*)(base + o_A);
   p.Uopt = (zd*)(base + o_Uopt);
   p.Zin = (zd*)(base + o_Zin);
-  p.Zout = (zd*)(base + o_Zout);
+  p.Zpart = (zd*)(base + o_Zpart);
+  p.Vprev = (zd*)(base + o_Vprev);
+  p.ntri = nb * (nb + 1) / 2;
   p.U = (zd*)(base + o_U);
   p.W = (zd*)(base + o_W);
   p.Mw = (zd*)(base + o_Mw);
   p.wk1 = (double*)(base + o_wk1);
-  p.wkom = (double*)(base + o_wkom);
-  p.frz = (double*)(base + o_frz);
+  p.part = (double*)(base + o_part);
   p.eigopt = (double*)(base + o_eigopt);
   p.log = (double*)(base + o_log);
   p.st = (DisState*)(base + o_st);
@@ -1081,15 +1146,15 @@ int w90gpu_disentangle(w90gpu_ctx* ctx, const w90gpu_problem* prob, int num_band
   // the extraction: Z of the initial U_opt, then (update, Z, reduce) per
   // iteration, polled every kPoll iterations
   constexpr int kPoll = 16;
-  hipLaunchKernelGGL(k_dis_zmat, dim3(nk), dim3(kNT), lds, sm, p);
+  hipLaunchKernelGGL(k_dis_zmat, dim3(nk * nn), dim3(kNT), lds, sm, p);
   DisState st;
   int launched = 0;
   while (launched < ctl->num_iter) {
     const int n = std::min(kPoll, ctl->num_iter - launched);
     for (int i = 0; i < n; ++i) {
       hipLaunchKernelGGL(k_dis_update, dim3(nk), dim3(kNT), lds, sm, p);
-      hipLaunchKernelGGL(k_dis_zmat, dim3(nk), dim3(kNT), lds, sm, p);
-      hipLaunchKernelGGL(k_dis_reduce, dim3(1), dim3(64), 0, sm, p);
+      hipLaunchKernelGGL(k_dis_zmat, dim3(nk * nn), dim3(kNT), lds, sm, p);
+      hipLaunchKernelGGL(k_dis_reduce, dim3(1), dim3(kNT), 0, sm, p);
     }
     DCHK(hipGetLastError());
     launched += n;
