@@ -1,8 +1,8 @@
 """Benchmark of the MI355X wannierisation path.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
-                    [--config 1|2|3] [--extras batch,serve|none] [--no-cpu-baseline]
-                    [--workload single|batch|serve|converge|mmn]
+                    [--config 1|2|3] [--extras batch,serve,dis|none] [--no-cpu-baseline]
+                    [--workload single|batch|serve|converge|mmn|dis]
 
 single (default), --config 3 (default): the largest single-GPU
 configuration, a seeded triclinic tight-binding problem (a, b, c ~ U[4,7]
@@ -25,6 +25,9 @@ Prints one JSON line: value = FP64 solves/s on one GPU, with
                 launch in FP64, batched FP64, batched mixed;
   serving       (extras) open-loop Poisson serving with continuous
                 batching: p50/p99 time to converged;
+  disentangle   (extras) the disentanglement of a 48-band fcc problem to
+                24 WFs on 512 k, beside the reference's w90_disentangle
+                (MPI over k-points) on a 20-iteration sample;
   cpu_baseline  the reference (its w90_library built from its sources,
                 driven by oracle/ref_driver.F90) with MPI over k-points on
                 the host's cores, on a bounded sample of the same workload.
@@ -66,15 +69,16 @@ def parse():
                     help="iterations of the CPU baseline sample (0: 100 for configs 1-2, 20 for config 3)")
     ap.add_argument("--cpu-ranks", type=int, default=0,
                     help="MPI ranks of the CPU baseline (0: the host's cores, at most 16)")
-    ap.add_argument("--extras", default="batch,serve",
-                    help="single: comma list of extra workloads reported beside the line (batch, serve) or none")
+    ap.add_argument("--extras", default="batch,serve,dis",
+                    help="single: comma list of extra workloads reported beside the line (batch, serve, dis) "
+                         "or none")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device once (a kernel of known traffic that "
                          "calibrates FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--config", type=int, choices=[1, 2, 3], default=3,
                     help="single: configuration (3: num_wann 64 triclinic 800 k, the largest; 2: fcc 16 "
                          "bands 512 k; 1: sc 4 bands 64 k)")
-    ap.add_argument("--workload", choices=["single", "batch", "serve", "mmn", "converge"], default="single")
+    ap.add_argument("--workload", choices=["single", "batch", "serve", "mmn", "converge", "dis"], default="single")
     ap.add_argument("--problems", type=int, default=256, help="batch: number of problems")
     ap.add_argument("--first", type=int, default=0, help="converge: first batch item")
     ap.add_argument("--no-plain", action="store_true", help="converge: skip the one-per-launch FP64 solves")
@@ -140,6 +144,19 @@ def cpu_baseline(prob, m0, u0, sample_iters, ranks):
     dt = time.perf_counter() - t0
     return dict(seconds=dt, iters=r["iterations"], kind="port", cores=1,
                 sample=f"C restatement of wann_main (1 thread, MKL sequential), num_iter={sample_iters}")
+
+
+def dis_summary(args):
+    """The disentanglement workload (workloads.run_dis) as an extra key, with
+    the reference timed on a 20-iteration sample on the CPU baseline's
+    ranks (or none with --no-cpu-baseline)."""
+    from types import SimpleNamespace
+
+    from wannier90_amd import workloads
+
+    a = SimpleNamespace(reps=3, dis_cpu_iters=0 if args.no_cpu_baseline else 20,
+                        cpu_ranks_resolved=cpu_ranks(args.cpu_ranks))
+    return workloads.run_dis(a)
 
 
 def batch_summary(args):
@@ -249,6 +266,9 @@ def main():
     if args.workload != "single":
         from wannier90_amd import workloads
 
+        args.cpu_ranks_resolved = cpu_ranks(args.cpu_ranks)
+        args.dis_cpu_iters = 0 if args.no_cpu_baseline else 20
+        args.reps = 3
         out = workloads.run(args)
         print(json.dumps(out), flush=True)
         return
@@ -352,6 +372,8 @@ def main():
         out["ragged_batch"] = batch_summary(args)
     if "serve" in extras:
         out["serving"] = serve_summary(args)
+    if "dis" in extras:
+        out["disentangle"] = dis_summary(args)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(prob, m0, u0, args.cpu_sample_iters, cpu_ranks(args.cpu_ranks))
         per_solve = cb["seconds"] * its / max(cb["iters"], 1)

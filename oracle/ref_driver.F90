@@ -15,14 +15,20 @@
 !>   ref_driver.x input.bin output.bin seedname
 !>
 !> input.bin (little endian): int32 nw, nk, nn, mp(3), num_iter,
-!> num_cg_steps, conv_window, num_print_cycles; float64 conv_tol,
-!> trial_step, cell(3,3) (column i = lattice vector i, Ang),
-!> kpt_latt(3,nk); int32 nnlist(nk,nn) (1-based, checked against the
-!> reference's own k-mesh); complex128 M(nw,nw,nn,nk), U(nw,nw,nk).
+!> num_cg_steps, conv_window, num_print_cycles, nb (num_bands),
+!> dis_num_iter, dis_conv_window, frozen_states; float64 conv_tol,
+!> trial_step, dis_win_min, dis_win_max, dis_froz_min, dis_froz_max,
+!> dis_mix_ratio, dis_conv_tol, cell(3,3) (column i = lattice vector i,
+!> Ang), kpt_latt(3,nk); int32 nnlist(nk,nn) (1-based, checked against the
+!> reference's own k-mesh); complex128 M(nb,nb,nn,nk); then for nb = nw
+!> U(nw,nw,nk), for nb > nw the projections A(nb,nw,nk) and float64
+!> eigval(nb,nk) (w90_disentangle runs first; windows beyond +-1e300 are
+!> left at the reference's defaults).
 !> output.bin (rank 0): float64 seconds (wall time of w90_wannierise, the
 !> slowest rank), float64 omega(3) = (invariant, tilde, total), float64
-!> centres(3,nw), spreads(nw); complex128 U(nw,nw,nk).  The CONV lines go
-!> to seedname.wout.
+!> centres(3,nw), spreads(nw); complex128 U(nw,nw,nk); for nb > nw also
+!> float64 seconds of w90_disentangle and complex128 U_opt(nb,nw,nk).  The
+!> CONV and DIS lines go to seedname.wout.
 program ref_driver
   use w90_library
   implicit none
@@ -32,8 +38,10 @@ program ref_driver
   integer, parameter :: dp = kind(1.0d0)
   type(lib_common_type), target :: lib
   integer :: nw, nk, nn, mp(3), num_iter, num_cg_steps, conv_window, num_print_cycles
-  real(dp) :: conv_tol, trial_step, cell(3, 3), t0, t1, tmax
-  real(dp), allocatable :: kpt(:, :)
+  integer :: nb, dis_num_iter, dis_conv_window, frozen_states
+  real(dp) :: conv_tol, trial_step, cell(3, 3), t0, t1, tmax, tdis
+  real(dp) :: win_min, win_max, froz_min, froz_max, dis_mix_ratio, dis_conv_tol
+  real(dp), allocatable :: kpt(:, :), eig(:, :)
   integer, allocatable :: nnlist(:, :), nnref(:, :), distk(:)
   complex(dp), allocatable :: mall(:, :, :, :), mloc(:, :, :, :), u(:, :, :), uopt(:, :, :)
   integer :: rank, nranks, ierr, unit_in, unit_out, unit_wout, unit_err, nkl, k, k0, k1, i, nn_ref, comm
@@ -54,10 +62,18 @@ program ref_driver
 #endif
 
   open (newunit=unit_in, file=trim(fin), access='stream', form='unformatted', status='old')
-  read (unit_in) nw, nk, nn, mp, num_iter, num_cg_steps, conv_window, num_print_cycles
-  read (unit_in) conv_tol, trial_step, cell
-  allocate (kpt(3, nk), nnlist(nk, nn), mall(nw, nw, nn, nk), u(nw, nw, nk), uopt(nw, nw, nk))
-  read (unit_in) kpt, nnlist, mall, u
+  read (unit_in) nw, nk, nn, mp, num_iter, num_cg_steps, conv_window, num_print_cycles, nb, dis_num_iter, &
+    dis_conv_window, frozen_states
+  read (unit_in) conv_tol, trial_step, win_min, win_max, froz_min, froz_max, dis_mix_ratio, dis_conv_tol, cell
+  allocate (kpt(3, nk), nnlist(nk, nn), mall(nb, nb, nn, nk), u(nw, nw, nk), uopt(nb, nw, nk))
+  read (unit_in) kpt, nnlist, mall
+  if (nb == nw) then
+    read (unit_in) u
+  else
+    allocate (eig(nb, nk))
+    read (unit_in) uopt, eig
+    u = (0.0_dp, 0.0_dp)
+  end if
   close (unit_in)
 
   ! contiguous blocks of k-points per rank
@@ -71,7 +87,19 @@ program ref_driver
   end do
 
   call w90_set_option(lib, 'num_wann', nw)
-  call w90_set_option(lib, 'num_bands', nw)
+  call w90_set_option(lib, 'num_bands', nb)
+  if (nb > nw) then
+    call w90_set_option(lib, 'dis_num_iter', dis_num_iter)
+    call w90_set_option(lib, 'dis_conv_window', dis_conv_window)
+    call w90_set_option(lib, 'dis_mix_ratio', dis_mix_ratio)
+    call w90_set_option(lib, 'dis_conv_tol', dis_conv_tol)
+    if (abs(win_min) < 1.0e300_dp) call w90_set_option(lib, 'dis_win_min', win_min)
+    if (abs(win_max) < 1.0e300_dp) call w90_set_option(lib, 'dis_win_max', win_max)
+    if (frozen_states /= 0) then
+      call w90_set_option(lib, 'dis_froz_max', froz_max)
+      if (abs(froz_min) < 1.0e300_dp) call w90_set_option(lib, 'dis_froz_min', froz_min)
+    end if
+  end if
   call w90_set_option(lib, 'unit_cell_cart', cell)
   call w90_set_option(lib, 'mp_grid', mp)
   call w90_set_option(lib, 'kpoints', kpt)
@@ -106,18 +134,40 @@ program ref_driver
     k0 = k0 + 1
   end do
   k1 = k0 + count(distk == rank)
-  allocate (mloc(nw, nw, nn, k1 - k0))
+  allocate (mloc(nb, nb, nn, k1 - k0))
   mloc = mall(:, :, :, k0 + 1:k1)
   deallocate (mall)
-  uopt = (0.0_dp, 0.0_dp)
-  do k = 1, nk
-    do i = 1, nw
-      uopt(i, i, k) = (1.0_dp, 0.0_dp)
+  if (nb == nw) then
+    uopt = (0.0_dp, 0.0_dp)
+    do k = 1, nk
+      do i = 1, nw
+        uopt(i, i, k) = (1.0_dp, 0.0_dp)
+      end do
     end do
-  end do
+  end if
   call w90_set_m_local(lib, mloc)
   call w90_set_u_matrix(lib, u)
   call w90_set_u_opt(lib, uopt)
+
+  tdis = 0.0_dp
+  if (nb > nw) then
+    call w90_set_eigval(lib, eig)
+#ifdef MPI
+    call mpi_barrier(MPI_COMM_WORLD, ierr)
+    t0 = mpi_wtime()
+#else
+    call wall(t0)
+#endif
+    call w90_disentangle(lib, unit_wout, unit_err, ierr)
+#ifdef MPI
+    t1 = mpi_wtime() - t0
+    call mpi_reduce(t1, tdis, 1, MPI_DOUBLE_PRECISION, MPI_MAX, 0, MPI_COMM_WORLD, ierr)
+#else
+    call wall(t1)
+    tdis = t1 - t0
+#endif
+    if (ierr /= 0) call fail('w90_disentangle')
+  end if
 
 #ifdef MPI
   call mpi_barrier(MPI_COMM_WORLD, ierr)
@@ -140,6 +190,7 @@ program ref_driver
     open (newunit=unit_out, file=trim(fout), access='stream', form='unformatted', status='replace')
     write (unit_out) tmax, lib%omega%invariant, lib%omega%tilde, lib%omega%total
     write (unit_out) lib%wannier_data%centres, lib%wannier_data%spreads, u
+    if (nb > nw) write (unit_out) tdis, uopt
     close (unit_out)
   end if
 #ifdef MPI
