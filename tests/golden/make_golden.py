@@ -249,6 +249,8 @@ def golden_dis_case(name, srcdir, seed, mmn_bz2=None):
         os.chmod(os.path.join(tmp, f), 0o644)
     win = os.path.join(tmp, seed + ".win")
     text = open(win).read()
+    # the DIS lines in Angstrom^2 whatever the input's length unit
+    text = set_keyword(text, "length_unit", "ang")
     ctl = control_from_win(text)
     dctl = dis_control_from_win(text)
     open(win, "w").write(set_keyword(text, "num_iter", 0))
@@ -270,7 +272,20 @@ def golden_dis_case(name, srcdir, seed, mmn_bz2=None):
     assert km.nntot == nn and np.array_equal(km.nnlist, nnl) and np.array_equal(km.nncell, nnc), name
     nb, nw, nk = chk0["num_bands"], chk0["num_wann"], chk0["num_kpts"]
     m = capi.read_mmn(os.path.join(tmp, seed + ".mmn"), nb, km)
-    a = capi.read_amn(os.path.join(tmp, seed + ".amn"), nb, nk, nw)
+    # the projections the reference keeps (select_projections, 1-based
+    # numbers and ranges; all of them without the keyword)
+    with open(os.path.join(tmp, seed + ".amn")) as f:
+        f.readline()
+        nproj = int(f.readline().split()[2])
+    a = capi.read_amn(os.path.join(tmp, seed + ".amn"), nb, nk, nproj)
+    sel = re.search(r"^\s*select_projections\s*[=:]?\s*([0-9 ,\-]+)$", text, re.M | re.I)
+    if sel:
+        keep = []
+        for tok in sel.group(1).replace(",", " ").split():
+            lo, _, hi = tok.partition("-")
+            keep += list(range(int(lo), int(hi or lo) + 1))
+        a = np.asfortranarray(a[:, [i - 1 for i in keep], :])
+    assert a.shape[1] == nw, name
     eig = np.loadtxt(os.path.join(tmp, seed + ".eig"))[:, 2].reshape(nk, nb).T.copy(order="F")
     data = dict(
         real_lattice=lat, kpt_latt=kpts, mp_grid=np.array(chk0["mp_grid"]),
@@ -300,6 +315,8 @@ def main_dis(which=None):
          os.path.join(CHECKPOINTS, "si_geninterp", "silicon.mmn.bz2")),
         ("dis_copper", os.path.join(REF, "testw90_example04"), "copper", None),
         ("dis_li_window", os.path.join(REF, "testw90_write_u_matrices_disent"), "wannier", None),
+        ("dis_gaas", os.path.join(REF, "testw90_gaas_disentanglement_issue192"), "gaas", None),
+        ("dis_si_valence", os.path.join(REF, "testw90_example11_1"), "silicon", None),
     ]
     for name, d, seed, bz in cases:
         if which and name not in which:

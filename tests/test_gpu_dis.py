@@ -1,7 +1,9 @@
 """Disentanglement on the GPU (w90gpu_disentangle) against the reference's
-runs on its own inputs (tests/golden/dis/: test-suite example03 silicon and
-example04 copper with frozen windows, write_u_matrices_disent with an outer
-window only) and against the oracle on synthetic problems up to 64 bands.
+runs on its own inputs (tests/golden/dis/: test-suite example03 silicon,
+example04 copper and gaas_disentanglement_issue192 with frozen windows,
+example11_1 silicon valence whose windows hold exactly num_wann states,
+write_u_matrices_disent with an outer window only) and against the oracle on
+synthetic problems up to 64 bands.
 
 Tolerances: Omega_I per iteration 1e-9 relative to the oracle (the DIS
 lines of the reference, printed with 8 decimals, to 6e-9), the final
@@ -105,17 +107,28 @@ def test_wannierise_after_disentangle_matches_reference(ctx, name):
     s.close()
     print(f"[dis+wannierise] {name}: {res.iterations} iterations, Omega {res.omega_total:.10f} "
           f"(reference {z['omega'][3]:.10f})", flush=True)
-    assert res.status == capi.OK and res.iterations == int(z["iterations"])
+    assert res.status == capi.OK
+    if int(z["iterations"]) == ctl["num_iter"]:
+        assert res.iterations == ctl["num_iter"]
+        tol_w = 1e-5
+    else:
+        # a conv_window stop at conv_tol 1e-10 lands on a chaotic iteration:
+        # GaAs stops at 160 in the reference and at 180 in the oracle's
+        # wann_main from the reference's own postdis M and U (176 from the
+        # oracle's disentanglement), all within 1e-9 of the reference's
+        # Omega; U_opt U then differs from the reference's by 6.6e-6 (1.1e-5)
+        assert res.iterations < ctl["num_iter"]
+        tol_w = 5e-5
     assert abs(res.omega_total - float(z["omega"][3])) <= 1e-8 * max(1.0, float(z["omega"][3])) + 5e-9
     # silicon's 50 iterations end while single spreads still move: the
     # oracle's wann_main from the reference's own postdis M and U is 2e-7
     # off the reference's spreads, and from the oracle's disentanglement
-    # 5.4e-7 off; the bounds are ten times that
+    # 5.4e-7 off (GaAs: 7.5e-7 and 9.7e-7); the bounds are five times that
     assert np.abs(np.sort(spr) - np.sort(z["spreads"])).max() <= 5e-6
     # U_opt U of the final state (gauge-invariant) against the reference's
     w = np.einsum("bwk,wvk->bvk", g["u_opt"], u)
     wr = np.einsum("bwk,wvk->bvk", z["u_opt"], z["u_final"])
-    assert np.abs(w - wr).max() <= 1e-5
+    assert np.abs(w - wr).max() <= tol_w
 
 
 SYN = [("fcc", 20, 12, (4, 4, 4), 4), ("sc", 48, 24, (3, 3, 3), 6), ("bcc", 64, 40, (2, 2, 2), 0),
