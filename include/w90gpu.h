@@ -324,9 +324,11 @@ typedef struct {
 typedef struct {
   double omega_invariant; /* final Omega_I of the optimal subspace (Ang^2) */
   double seconds;         /* wall time of the call on the device            */
+  double sweeps_mean;     /* Jacobi sweeps per extraction eigenproblem      */
   int iterations;         /* dis_extract iterations performed               */
   int converged;          /* 1 if the dis_conv_window test was satisfied    */
   int status;             /* status of the call                             */
+  int sweeps_max;         /* most Jacobi sweeps of one extraction eigenproblem */
 } w90gpu_dis_result;
 
 void w90gpu_dis_control_defaults(w90gpu_dis_control* ctl);

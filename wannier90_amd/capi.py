@@ -62,8 +62,8 @@ class DisControl(C.Structure):
 
 
 class DisResult(C.Structure):
-    _fields_ = [("omega_invariant", C.c_double), ("seconds", C.c_double), ("iterations", C.c_int),
-                ("converged", C.c_int), ("status", C.c_int)]
+    _fields_ = [("omega_invariant", C.c_double), ("seconds", C.c_double), ("sweeps_mean", C.c_double),
+                ("iterations", C.c_int), ("converged", C.c_int), ("status", C.c_int), ("sweeps_max", C.c_int)]
 
 
 class Result(C.Structure):
@@ -399,7 +399,8 @@ class Context:
                                                 _d(hist), rows, C.byref(res)))
         return dict(u_opt=uopt, u=u, m=mw, eigval_opt=eigopt, nfirstwin=nfirst, ndimwin=ndim,
                     history=hist[: res.iterations], omega_invariant=res.omega_invariant,
-                    iterations=res.iterations, converged=bool(res.converged), seconds=res.seconds)
+                    iterations=res.iterations, converged=bool(res.converged), seconds=res.seconds,
+                    sweeps_mean=res.sweeps_mean, sweeps_max=res.sweeps_max)
 
     def omega_noalloc(self, kmesh, m, first_pass=True, omega_invariant=0.0):
         """omega() through the allocation-free entry point (static device

@@ -55,6 +55,7 @@ typedef double2 zd;
 
 __device__ __forceinline__ zd zmk(double r, double i) { return make_double2(r, i); }
 __device__ __forceinline__ zd zadd(zd a, zd b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ zd zsub(zd a, zd b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ zd zmul(zd a, zd b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 // conj(a) * b
 __device__ __forceinline__ zd zcmul(zd a, zd b) { return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x); }
@@ -90,6 +91,8 @@ struct DisState {
   int num_iter;
   int conv_window;
   int run;         // trailing iterations with |delta| < conv_tol
+  int sweeps_max;  // most Jacobi sweeps of one extraction eigenproblem
+  unsigned long long sweeps_sum, solves;
   double mix, conv_tol;
   double womegai, womegai1, delta;
 };
@@ -106,6 +109,8 @@ enum : int {
 struct DisProb {
   int nb, nw, nk, nn, ld;  // ld: leading dimension of the LDS matrices (nb + 1)
   int ntri;                // nb (nb + 1) / 2
+  int ldc;                 // k_dis_zmat: LDS leading dimension (odd, >= every ndim)
+  int nzmax, ldz;          // k_dis_update: largest ndim - nfroz, leading dimension (odd)
   double wbtot;
   const int* nnl;      // [k * nn + b], 0-based
   const double* wb;    // [nn]
@@ -114,7 +119,7 @@ struct DisProb {
   const zd* M;         // (nb, nb, nn, nk), all bands
   const zd* A;         // (nb, nw, nk), all bands
   zd* Uopt;            // (nb, nw, nk), rows relative to the window
-  zd* Zin;             // (nb, nb, nk)
+  zd* Zin;             // (nb, nb, nk): the mixed Z, packed upper triangle
   zd* Zpart;           // (ntri, nn, nk): w_b C C^+ per (k, b), packed upper triangle
   zd* Vprev;           // (nb, nb, nk): eigenvectors of the last Z
   zd* U;               // (nw, nw, nk)
@@ -133,7 +138,7 @@ __device__ __forceinline__ int nf_row(const Win& w, int i) { return i < w.fmin ?
 
 // Deterministic block reductions (wavefront butterflies, then the four
 // wavefront results in order); red holds kNT / 64 doubles.
-__device__ double block_sum(double v, double* red) {
+__device__ __forceinline__ double block_sum(double v, double* red) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
@@ -160,102 +165,324 @@ struct JacobiLds {
   double red[kNT / 64];
   double lam[kMaxN];
   int perm[kMaxN];
+  int rank[kMaxN];
 };
 
+// Position (r, c), r <= c, of entry x = c (c + 1) / 2 + r of a packed upper
+// triangle (x < 2^22: the single-precision root is corrected by one step).
+__device__ __forceinline__ void tri_decode(int x, int& r, int& c) {
+  int cc = (int)((sqrtf(8.0f * (float)x + 1.0f) - 1.0f) * 0.5f);
+  if (cc * (cc + 1) / 2 > x) --cc;
+  if ((cc + 1) * (cc + 2) / 2 <= x) ++cc;
+  c = cc;
+  r = x - cc * (cc + 1) / 2;
+}
+
+// Pair (p, q), p < q, of slot s in round r of the round-robin (tournament)
+// schedule on m (even) indices: index 0 stays, the others turn one place
+// per round, and slot s pairs the indices s and m - 1 - s positions away.
+__device__ __forceinline__ void rr_pair(int s, int r, int m, int& p, int& q) {
+  int a = 0;
+  if (s > 0) {
+    a = s - 1 + r;
+    if (a >= m - 1) a -= m - 1;
+    a += 1;
+  }
+  int b = m - 2 - s + r;
+  if (b >= m - 1) b -= m - 1;
+  b += 1;
+  p = min(a, b);
+  q = max(a, b);
+}
+
+// rows of E per lane in a rotation pass (n <= 64, at least 8 lanes a slot)
+constexpr int kRowsPerLane = (kMaxN + kNT / (kMaxN / 2) - 1) / (kNT / (kMaxN / 2));
+
+// The rotation of pair (p, q) is skipped when |a_pq| <= 1e-15 sqrt(|a_pp
+// a_qq|) (relative to the diagonal) or <= 1e-16 ||A||_F (without this floor
+// the rounding noise among the zero eigenvalues of a rank-deficient matrix
+// would be rotated forever).
+__device__ __forceinline__ bool jacobi_rotates(double b2, double app, double aqq, double floor2) {
+  return b2 > floor2 && b2 > 1e-30 * fabs(app * aqq);
+}
+
+// Whether some off-diagonal pair of A still rotates (all lanes, one pass).
+__device__ __forceinline__ bool jacobi_pending(const zd* A, int n, int ld, double floor2) {
+  int any = 0;
+  for (int e = threadIdx.x; e < n * n; e += kNT) {
+    const int i = e % n, j = e / n;
+    if (i < j && jacobi_rotates(zabs2(A[i + j * ld]), A[i + i * ld].x, A[j + j * ld].x, floor2)) any = 1;
+  }
+  return __syncthreads_or(any) != 0;
+}
+
+// The rotation that zeroes a_pq: with D = diag(1, a_pq/|a_pq|) the 2 x 2
+// block D^+ A D is real, and its rotation by the smaller angle (tan 2t =
+// 2|a_pq| / (a_qq - a_pp)) gives J = D R D^+:
+//   t = sign(h) 2|a_pq| / (|h| + sqrt(h^2 + 4|a_pq|^2)), h = a_qq - a_pp,
+//   c = 1 / sqrt(1 + t^2), s = t c a_pq / |a_pq|.
+__device__ __forceinline__ JRot jacobi_rotation(int p, int q, const zd* A, int n, int ld, double floor2) {
+  JRot R;
+  R.p = p;
+  R.q = q;
+  R.c = 1.0;
+  R.s = zmk(0.0, 0.0);
+  if (q < n) {
+    const zd apq = A[p + q * ld];
+    const double app = A[p + p * ld].x, aqq = A[q + q * ld].x;
+    const double b2 = zabs2(apq);
+    if (jacobi_rotates(b2, app, aqq, floor2)) {
+      const double rb = rsqrt(b2);
+      const double h = aqq - app;
+      const double den = fabs(h) + sqrt(fma(h, h, 4.0 * b2));
+      const double tt = copysign(2.0 * b2 * rb, h) / den;
+      const double c = rsqrt(fma(tt, tt, 1.0));
+      const double sr = tt * c * rb;
+      R.c = c;
+      R.s = zmk(sr * apq.x, sr * apq.y);
+    }
+  }
+  return R;
+}
+
 // Eigen-decomposition of the Hermitian n x n matrix A (LDS, column-major,
-// leading dimension ld; the upper triangle and the real diagonal are
-// read, A must be stored whole): on return A is diagonal to the Jacobi
-// tolerance and V (LDS) holds the eigenvectors in its columns.  For each
-// pair, D = diag(1, b/|b|) makes the 2 x 2 block real, and the real
-// rotation that diagonalises it (tan 2t = 2|b| / (a_qq - a_pp), the smaller
-// angle) gives J = D R D^+.  A rotation is skipped when |a_pq| <= 1e-15
-// sqrt(|a_pp a_qq|) (relative to the diagonal) or <= 1e-16 ||A||_F (without
-// this floor the rounding noise among the zero eigenvalues of a
-// rank-deficient matrix would be rotated forever); the sweeps end when a
-// whole sweep skips every pair.  A warm start (a nearly diagonal A) leaves
-// off-diagonal entries just below these bounds, so they are kept tight:
+// leading dimension ld, stored whole) by the two-sided cyclic Jacobi method
+// with round-robin (tournament) ordering: each round rotates n/2 disjoint
+// pairs at once; the rotations go into J.rot, and one parallel pass then
+// applies A <- J^+ A J by 2 x 2 blocks (one lane per block of the upper
+// block triangle, which writes the block and its mirror) and E <- E J to
+// the rows of E (LDS, leading dimension ld).  On return A is diagonal to the
+// tolerance of jacobi_rotates and the columns of E (as it came in) times the
+// rotations are the eigenvectors: E = I gives those of A, E = V for A = V^+
+// Z V those of Z.  The sweeps end when no pair rotates (checked before each
+// sweep in one parallel pass).  Returns the number of sweeps, or -1 if
+// kMaxSweeps were not enough.  A warm start (a nearly diagonal A) leaves
+// off-diagonal entries just below the skip bounds, so they are kept tight:
 // the selected subspace moves by their size over the eigenvalue gap at the
-// selection.  Returns the
-// number of sweeps (kMaxSweeps: not converged).
-__device__ int jacobi_eigh(zd* A, zd* V, int n, int ld, JacobiLds& J) {
+// selection.
+__device__ __forceinline__ int jacobi_run(zd* A, zd* E, int n, int ld, JacobiLds& J) {
   const int t = threadIdx.x;
   double fro = 0.0;
-  for (int e = t; e < n * n; e += kNT) {
-    const int r = e % n, c = e / n;
-    V[r + c * ld] = zmk(r == c ? 1.0 : 0.0, 0.0);
-    fro += zabs2(A[r + c * ld]);
-  }
-  fro = block_sum(fro, J.red);
-  const double floor2 = fro * 1e-32;
+  for (int e = t; e < n * n; e += kNT) fro += zabs2(A[(e % n) + (e / n) * ld]);
+  const double floor2 = block_sum(fro, J.red) * 1e-32;
   const int m = n + (n & 1), half = m / 2;
-  int sweep = 0;
-  for (; sweep < kMaxSweeps; ++sweep) {
-    if (t == 0) J.count = 0;
-    __syncthreads();
+  const int nblk = half * (half + 1) / 2;
+  // this lane's first block of the upper block triangle, and its share of
+  // the rows of E: slot es (g lanes per slot), rows eg, eg + g, ...
+  int bs1 = 0, bs2 = 0;
+  if (t < nblk) tri_decode(t, bs1, bs2);
+  const int g = kNT / half, es = t / g, eg = t % g;
+  int sweeps = 0;
+  while (jacobi_pending(A, n, ld, floor2)) {
+    if (sweeps == kMaxSweeps) return -1;
+    ++sweeps;
     for (int r = 0; r < m - 1; ++r) {
       if (t < half) {
-        const int a = (t == 0) ? 0 : 1 + ((t - 1 + r) % (m - 1));
-        const int b = 1 + ((m - 2 - t + r) % (m - 1));
-        JRot R;
-        R.p = min(a, b);
-        R.q = max(a, b);
-        R.c = 1.0;
-        R.s = zmk(0.0, 0.0);
-        if (R.q < n) {
-          const zd apq = A[R.p + R.q * ld];
-          const double b2 = zabs2(apq);
-          const double app = A[R.p + R.p * ld].x, aqq = A[R.q + R.q * ld].x;
-          if (b2 > floor2 && b2 > 1e-30 * fabs(app * aqq)) {
-            const double bn = sqrt(b2);
-            const double tau = (aqq - app) / (2.0 * bn);
-            double tt;
-            if (fabs(tau) > 1e150) tt = 0.5 / tau;
-            else tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-            const double c = 1.0 / sqrt(1.0 + tt * tt), s = tt * c;
-            R.c = c;
-            R.s = zmk(s * apq.x / bn, s * apq.y / bn);
-            atomicAdd(&J.count, 1);
+        int p, q;
+        rr_pair(t, r, m, p, q);
+        J.rot[t] = jacobi_rotation(p, q, A, n, ld, floor2);
+      }
+      __syncthreads();
+      for (int x = t; x < nblk; x += kNT) {
+        int r1 = bs1, r2 = bs2;
+        if (x != t) tri_decode(x, r1, r2);
+        int p1, q1, p2, q2;
+        rr_pair(r1, r, m, p1, q1);
+        rr_pair(r2, r, m, p2, q2);
+        const bool v1 = q1 < n, v2 = q2 < n;
+        // the block's entries requested with the rotations
+        const zd z0 = zmk(0.0, 0.0);
+        const zd xpp = A[p1 + p2 * ld];
+        const zd xpq = v2 ? A[p1 + q2 * ld] : z0;
+        const zd xqp = v1 ? A[q1 + p2 * ld] : z0;
+        const zd xqq = (v1 && v2) ? A[q1 + q2 * ld] : z0;
+        const double c1 = J.rot[r1].c, c2 = J.rot[r2].c;
+        const zd s1 = J.rot[r1].s, s2 = J.rot[r2].s;
+        if (s1.x == 0.0 && s1.y == 0.0 && s2.x == 0.0 && s2.y == 0.0) continue;
+        // Y = X J2, then J1^+ Y
+        const zd s2c = zconj(s2), s1c = zconj(s1);
+        const zd ypp = zsub(zscale(xpp, c2), zmul(s2c, xpq)), ypq = zadd(zmul(s2, xpp), zscale(xpq, c2));
+        const zd yqp = zsub(zscale(xqp, c2), zmul(s2c, xqq)), yqq = zadd(zmul(s2, xqp), zscale(xqq, c2));
+        const zd npp = zsub(zscale(ypp, c1), zmul(s1, yqp)), nqp = zadd(zmul(s1c, ypp), zscale(yqp, c1));
+        const zd npq = zsub(zscale(ypq, c1), zmul(s1, yqq)), nqq = zadd(zmul(s1c, ypq), zscale(yqq, c1));
+        if (r1 != r2) {
+          A[p1 + p2 * ld] = npp;
+          A[p2 + p1 * ld] = zconj(npp);
+          if (v2) {
+            A[p1 + q2 * ld] = npq;
+            A[q2 + p1 * ld] = zconj(npq);
+          }
+          if (v1) {
+            A[q1 + p2 * ld] = nqp;
+            A[p2 + q1 * ld] = zconj(nqp);
+          }
+          if (v1 && v2) {
+            A[q1 + q2 * ld] = nqq;
+            A[q2 + q1 * ld] = zconj(nqq);
+          }
+        } else {
+          A[p1 + p1 * ld] = zmk(npp.x, 0.0);
+          if (v1) {
+            A[q1 + q1 * ld] = zmk(nqq.x, 0.0);
+            A[p1 + q1 * ld] = npq;
+            A[q1 + p1 * ld] = zconj(npq);
           }
         }
-        J.rot[t] = R;
       }
-      __syncthreads();
-      // columns: A J and V J
-      for (int e = t; e < half * n; e += kNT) {
-        const JRot R = J.rot[e / n];
-        if (R.s.x == 0.0 && R.s.y == 0.0) continue;
-        const int row = e % n;
-        const zd sc = zconj(R.s);
-        zd ap = A[row + R.p * ld], aq = A[row + R.q * ld];
-        A[row + R.p * ld] = zadd(zscale(ap, R.c), zscale(zmul(sc, aq), -1.0));
-        A[row + R.q * ld] = zadd(zmul(R.s, ap), zscale(aq, R.c));
-        ap = V[row + R.p * ld];
-        aq = V[row + R.q * ld];
-        V[row + R.p * ld] = zadd(zscale(ap, R.c), zscale(zmul(sc, aq), -1.0));
-        V[row + R.q * ld] = zadd(zmul(R.s, ap), zscale(aq, R.c));
-      }
-      __syncthreads();
-      // rows: J^+ A
-      for (int e = t; e < half * n; e += kNT) {
-        const JRot R = J.rot[e / n];
-        if (R.s.x == 0.0 && R.s.y == 0.0) continue;
-        const int col = e % n;
-        const zd ap = A[R.p + col * ld], aq = A[R.q + col * ld];
-        A[R.p + col * ld] = zadd(zscale(ap, R.c), zscale(zmul(R.s, aq), -1.0));
-        A[R.q + col * ld] = zadd(zmul(zconj(R.s), ap), zscale(aq, R.c));
+      if (es < half) {
+        int p, q;
+        rr_pair(es, r, m, p, q);
+        if (q < n) {
+          zd ep[kRowsPerLane], eq[kRowsPerLane];
+#pragma unroll
+          for (int j = 0; j < kRowsPerLane; ++j) {
+            const int row = eg + j * g;
+            if (row < n) {
+              ep[j] = E[row + p * ld];
+              eq[j] = E[row + q * ld];
+            }
+          }
+          const double c = J.rot[es].c;
+          const zd sv = J.rot[es].s, sc = zconj(sv);
+          if (sv.x != 0.0 || sv.y != 0.0) {
+#pragma unroll
+            for (int j = 0; j < kRowsPerLane; ++j) {
+              const int row = eg + j * g;
+              if (row < n) {
+                E[row + p * ld] = zsub(zscale(ep[j], c), zmul(sc, eq[j]));
+                E[row + q * ld] = zadd(zmul(sv, ep[j]), zscale(eq[j], c));
+              }
+            }
+          }
+        }
       }
       __syncthreads();
     }
-    const int cnt = J.count;
-    __syncthreads();
-    if (cnt == 0) break;
   }
-  return sweep;
+  return sweeps;
 }
 
-// Eigenvalues (the diagonal of A after jacobi_eigh) into J.lam and their
-// ascending order into J.perm (ties by index).
-__device__ void eig_order(const zd* A, int n, int ld, JacobiLds& J) {
+// jacobi_run from E = I into V (the eigenvectors of A itself).
+__device__ __forceinline__ int jacobi_eigh(zd* A, zd* V, int n, int ld, JacobiLds& J) {
+  for (int e = threadIdx.x; e < n * n; e += kNT) {
+    const int r = e % n, c = e / n;
+    V[r + c * ld] = zmk(r == c ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  return jacobi_run(A, V, n, ld, J);
+}
+
+// ------------------------------------------------------------ products
+
+// out(i, j, C(i, j)) for C = op(A) B (M x N, inner dimension K) by RT x CT
+// register tiles, one tile per lane and pass: op(A)(i, l) = A[i + l * lda],
+// or conj(A[l + i * lda]) when ADJ; B(l, j) = B[l + j * ldb].  Rows and
+// columns past the edge are read clamped and not stored.
+template <int RT, int CT, bool ADJ, class Out>
+__device__ void zgemm_tiles(int M, int N, int K, const zd* A, int lda, const zd* B, int ldb, Out out) {
+  const int tr = (M + RT - 1) / RT, tc = (N + CT - 1) / CT;
+  for (int x = threadIdx.x; x < tr * tc; x += kNT) {
+    const int i0 = (x % tr) * RT, j0 = (x / tr) * CT;
+    int ii[RT], jj[CT];
+#pragma unroll
+    for (int a = 0; a < RT; ++a) ii[a] = min(i0 + a, M - 1);
+#pragma unroll
+    for (int b = 0; b < CT; ++b) jj[b] = min(j0 + b, N - 1);
+    zd acc[RT][CT];
+#pragma unroll
+    for (int a = 0; a < RT; ++a)
+#pragma unroll
+      for (int b = 0; b < CT; ++b) acc[a][b] = zmk(0.0, 0.0);
+#pragma unroll 2
+    for (int l = 0; l < K; ++l) {
+      zd av[RT], bv[CT];
+#pragma unroll
+      for (int a = 0; a < RT; ++a) av[a] = ADJ ? A[l + ii[a] * lda] : A[ii[a] + l * lda];
+#pragma unroll
+      for (int b = 0; b < CT; ++b) bv[b] = B[l + jj[b] * ldb];
+#pragma unroll
+      for (int a = 0; a < RT; ++a)
+#pragma unroll
+        for (int b = 0; b < CT; ++b) {
+          if (ADJ) zcfma(acc[a][b], av[a], bv[b]);
+          else zfma(acc[a][b], av[a], bv[b]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < RT; ++a)
+#pragma unroll
+      for (int b = 0; b < CT; ++b)
+        if (i0 + a < M && j0 + b < N) out(i0 + a, j0 + b, acc[a][b]);
+  }
+}
+
+// zgemm_tiles with the smallest tile that needs at most one pass (M, N <=
+// 64 always fit 4 x 4 tiles).
+template <bool ADJ, class Out>
+__device__ void zgemm(int M, int N, int K, const zd* A, int lda, const zd* B, int ldb, Out out) {
+  auto fits = [&](int r, int c) { return ((M + r - 1) / r) * ((N + c - 1) / c) <= kNT; };
+  if (fits(2, 2)) zgemm_tiles<2, 2, ADJ>(M, N, K, A, lda, B, ldb, out);
+  else if (fits(2, 3)) zgemm_tiles<2, 3, ADJ>(M, N, K, A, lda, B, ldb, out);
+  else if (fits(3, 3)) zgemm_tiles<3, 3, ADJ>(M, N, K, A, lda, B, ldb, out);
+  else if (fits(3, 4)) zgemm_tiles<3, 4, ADJ>(M, N, K, A, lda, B, ldb, out);
+  else zgemm_tiles<4, 4, ADJ>(M, N, K, A, lda, B, ldb, out);
+}
+
+// The upper triangle (i <= j) of a Hermitian N x N product A^+ B by T x T
+// tiles of the upper block triangle: out(i, j, C(i, j)) for i <= j only.
+template <int T, class Out>
+__device__ void zgemm_herm_tiles(int N, int K, const zd* A, int lda, const zd* B, int ldb, Out out) {
+  const int nt = (N + T - 1) / T;
+  for (int x = threadIdx.x; x < nt * (nt + 1) / 2; x += kNT) {
+    int ib, jb;
+    tri_decode(x, ib, jb);
+    const int i0 = ib * T, j0 = jb * T;
+    int ii[T], jj[T];
+#pragma unroll
+    for (int a = 0; a < T; ++a) {
+      ii[a] = min(i0 + a, N - 1);
+      jj[a] = min(j0 + a, N - 1);
+    }
+    zd acc[T][T];
+#pragma unroll
+    for (int a = 0; a < T; ++a)
+#pragma unroll
+      for (int b = 0; b < T; ++b) acc[a][b] = zmk(0.0, 0.0);
+#pragma unroll 2
+    for (int l = 0; l < K; ++l) {
+      zd av[T], bv[T];
+#pragma unroll
+      for (int a = 0; a < T; ++a) {
+        av[a] = A[l + ii[a] * lda];
+        bv[a] = B[l + jj[a] * ldb];
+      }
+#pragma unroll
+      for (int a = 0; a < T; ++a)
+#pragma unroll
+        for (int b = 0; b < T; ++b) zcfma(acc[a][b], av[a], bv[b]);
+    }
+#pragma unroll
+    for (int a = 0; a < T; ++a)
+#pragma unroll
+      for (int b = 0; b < T; ++b)
+        if (i0 + a <= j0 + b && j0 + b < N) out(i0 + a, j0 + b, acc[a][b]);
+  }
+}
+
+template <class Out>
+__device__ void zgemm_herm(int N, int K, const zd* A, int lda, const zd* B, int ldb, Out out) {
+  auto fits = [&](int tl) {
+    const int nt = (N + tl - 1) / tl;
+    return nt * (nt + 1) / 2 <= kNT;
+  };
+  if (fits(2)) zgemm_herm_tiles<2>(N, K, A, lda, B, ldb, out);
+  else if (fits(3)) zgemm_herm_tiles<3>(N, K, A, lda, B, ldb, out);
+  else zgemm_herm_tiles<4>(N, K, A, lda, B, ldb, out);
+}
+
+// Eigenvalues (the diagonal of A after jacobi_run) into J.lam, their
+// ascending order into J.perm (ties by index) and its inverse into J.rank.
+__device__ __forceinline__ void eig_order(const zd* A, int n, int ld, JacobiLds& J) {
   const int t = threadIdx.x;
   if (t < n) J.lam[t] = A[t + t * ld].x;
   __syncthreads();
@@ -264,6 +491,7 @@ __device__ void eig_order(const zd* A, int n, int ld, JacobiLds& J) {
     int rank = 0;
     for (int j = 0; j < n; ++j) rank += (J.lam[j] < l || (J.lam[j] == l && j < t)) ? 1 : 0;
     J.perm[rank] = t;
+    J.rank[t] = rank;
   }
   __syncthreads();
 }
@@ -279,7 +507,7 @@ __device__ void eig_order(const zd* A, int n, int ld, JacobiLds& J) {
 // X, where the polar factor is not unique, as with the SVD) are completed
 // by Gram-Schmidt from unit vectors.  U goes to global memory (leading
 // dimension ldu).  LDS: G (rows x nw, leading dimension ld), V.
-__device__ void polar_factor(const zd* X, int ldx, int rows, int nw, zd* U, int ldu, zd* G, zd* V, int ld,
+__device__ __forceinline__ void polar_factor(const zd* X, int ldx, int rows, int nw, zd* U, int ldu, zd* G, zd* V, int ld,
                              JacobiLds& J, int* flags) {
   const int t = threadIdx.x;
   double fro = 0.0;
@@ -310,10 +538,7 @@ __device__ void polar_factor(const zd* X, int ldx, int rows, int nw, zd* U, int 
       int P = 0, Q = 0;
       bool valid = false;
       if (pi < half) {
-        const int a = (pi == 0) ? 0 : 1 + ((pi - 1 + r) % (m - 1));
-        const int b = 1 + ((m - 2 - pi + r) % (m - 1));
-        P = min(a, b);
-        Q = max(a, b);
+        rr_pair(pi, r, m, P, Q);
         valid = Q < nw;
         if (valid)
           for (int row = lane; row < rows; row += 8) {
@@ -475,7 +700,7 @@ __global__ void __launch_bounds__(kNT) k_dis_proj_froz(DisProb p) {
       Aq[j + i * ld] = zconj(s);
     }
     __syncthreads();
-    if (jacobi_eigh(Aq, V, nd, ld, J) >= kMaxSweeps && t == 0) atomicOr(p.flags, F_JACOBI);
+    if (jacobi_eigh(Aq, V, nd, ld, J) < 0 && t == 0) atomicOr(p.flags, F_JACOBI);
     eig_order(Aq, nd, ld, J);
     if (t == 0) {
       fix_ok = 1;
@@ -558,16 +783,6 @@ __global__ void __launch_bounds__(kNT) k_dis_orthonorm(DisProb p) {
   if (bad) atomicOr(p.flags, F_ORTHONORM);
 }
 
-// Position (m, n), m <= n, of entry e of a packed upper triangle.
-__device__ __forceinline__ void tri_index(int e, int& m, int& n) {
-  // e = n (n + 1) / 2 + m, m <= n
-  int c = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-  while (c * (c + 1) / 2 > e) --c;
-  while ((c + 1) * (c + 2) / 2 <= e) ++c;
-  n = c;
-  m = e - c * (c + 1) / 2;
-}
-
 // internal_zmatrix and the Omega_I terms for one (k, b), one workgroup
 // each, with U_opt of every k as it is now: C = M(k,b) U_opt(k+b) and
 //   Zpart(k,b) = w_b C C^+ over the non-frozen rows (upper triangle,
@@ -576,82 +791,131 @@ __device__ __forceinline__ void tri_index(int e, int& m, int& n) {
 //                by k_dis_reduce;
 //   part(1,k,b) = the same over the frozen rows of U_opt(k)^+ C, the frozen
 //                term of the next extraction step.
+// LDS: C (ndim(k) x nw) and U (ndim(k+b) x nw, then U_opt(k)), leading
+// dimension ldc; the products by register tiles.
+template <int T>
+__device__ void zpart_tiles(const zd* C, int ldc, const Win& w, int nd, int nw, double wbb, zd* Zp) {
+  const int nt = (nd + T - 1) / T;
+  for (int x = threadIdx.x; x < nt * (nt + 1) / 2; x += kNT) {
+    int ib, jb;
+    tri_decode(x, ib, jb);
+    int pr[T], qr[T];
+#pragma unroll
+    for (int a = 0; a < T; ++a) {
+      pr[a] = nf_row(w, min(ib * T + a, nd - 1));
+      qr[a] = nf_row(w, min(jb * T + a, nd - 1));
+    }
+    zd acc[T][T];
+#pragma unroll
+    for (int a = 0; a < T; ++a)
+#pragma unroll
+      for (int b = 0; b < T; ++b) acc[a][b] = zmk(0.0, 0.0);
+#pragma unroll 2
+    for (int l = 0; l < nw; ++l) {
+      zd av[T], bv[T];
+#pragma unroll
+      for (int a = 0; a < T; ++a) {
+        av[a] = C[pr[a] + l * ldc];
+        bv[a] = zconj(C[qr[a] + l * ldc]);
+      }
+#pragma unroll
+      for (int a = 0; a < T; ++a)
+#pragma unroll
+        for (int b = 0; b < T; ++b) zfma(acc[a][b], av[a], bv[b]);
+    }
+#pragma unroll
+    for (int a = 0; a < T; ++a)
+#pragma unroll
+      for (int b = 0; b < T; ++b) {
+        const int i = ib * T + a, j = jb * T + b;
+        if (i <= j && j < nd) Zp[j * (j + 1) / 2 + i] = zscale(acc[a][b], wbb);
+      }
+  }
+}
+
 __global__ void __launch_bounds__(kNT) k_dis_zmat(DisProb p) {
-  __shared__ double red[kNT / 64];
+  __shared__ double red[2][kNT / 64];
   const DisState* st = p.st;
   if (st->done) return;
   const int nn = p.nn;
   const int k = blockIdx.x / nn, b = blockIdx.x % nn, t = threadIdx.x;
   const Win w = p.win[k];
-  const int nb = p.nb, nw = p.nw, ld = p.ld;
+  const int nb = p.nb, nw = p.nw, ldc = p.ldc;
   const int nd = w.ndim - w.nfroz;
-  const int ntri = nd * (nd + 1) / 2;
-  zd* C = dis_lds;            // ndim(k) x nw
-  zd* U2 = dis_lds + nb * ld; // ndim(k+b) x nw
+  zd* C = dis_lds;              // ndim(k) x nw
+  zd* U2 = dis_lds + ldc * nw;  // ndim(k+b) x nw, then ndim(k) x nw
   const zd* Uk = p.Uopt + (size_t)k * nb * nw;
   const int k2 = p.nnl[k * nn + b];
   const Win w2 = p.win[k2];
   const zd* Ukb = p.Uopt + (size_t)k2 * nb * nw;
   for (int e = t; e < w2.ndim * nw; e += kNT) {
     const int l = e % w2.ndim, j = e / w2.ndim;
-    U2[l + j * ld] = Ukb[l + (size_t)j * nb];
+    U2[l + j * ldc] = Ukb[l + (size_t)j * nb];
   }
   __syncthreads();
   const zd* Mkb = p.M + ((size_t)k * nn + b) * nb * nb + w.first + (size_t)w2.first * nb;
-  for (int e = t; e < w.ndim * nw; e += kNT) {
-    const int i = e % w.ndim, j = e / w.ndim;
-    zd c = zmk(0.0, 0.0);
-    for (int l = 0; l < w2.ndim; ++l) zfma(c, Mkb[i + (size_t)l * nb], U2[l + j * ld]);
-    C[i + j * ld] = c;
-  }
+  zgemm<false>(w.ndim, nw, w2.ndim, Mkb, nb, U2, ldc, [&](int i, int j, zd v) { C[i + j * ldc] = v; });
   __syncthreads();
+  for (int e = t; e < w.ndim * nw; e += kNT) {
+    const int l = e % w.ndim, j = e / w.ndim;
+    U2[l + j * ldc] = Uk[l + (size_t)j * nb];
+  }
   const double wbb = p.wb[b];
   const size_t kb = (size_t)k * nn + b;
   if (nw > w.nfroz) {
     zd* Zp = p.Zpart + kb * p.ntri;
-    for (int e = t; e < ntri; e += kNT) {
-      int m, n;
-      tri_index(e, m, n);
-      const int pr = nf_row(w, m), qr = nf_row(w, n);
-      zd cs = zmk(0.0, 0.0);
-      for (int l = 0; l < nw; ++l) zfma(cs, C[pr + l * ld], zconj(C[qr + l * ld]));
-      Zp[e] = zscale(cs, wbb);
-    }
+    const int nt2 = (nd + 1) / 2;
+    if (nt2 * (nt2 + 1) / 2 <= kNT) zpart_tiles<2>(C, ldc, w, nd, nw, wbb, Zp);
+    else zpart_tiles<3>(C, ldc, w, nd, nw, wbb, Zp);
   }
+  __syncthreads();
   double r = 0.0, rf = 0.0;
-  for (int e = t; e < nw * nw; e += kNT) {
-    const int i = e % nw, j = e / nw;
-    zd v = zmk(0.0, 0.0);
-    for (int l = 0; l < w.ndim; ++l) zcfma(v, Uk[l + (size_t)i * nb], C[l + j * ld]);
+  zgemm<true>(nw, nw, w.ndim, U2, ldc, C, ldc, [&](int i, int, zd v) {
     const double a2 = zabs2(v);
     r += a2;
     if (i < w.nfroz) rf += a2;
+  });
+  // both sums in one pass: wavefront butterflies, then the waves in order
+  for (int off = 32; off > 0; off >>= 1) {
+    r += __shfl_xor(r, off, 64);
+    rf += __shfl_xor(rf, off, 64);
   }
-  r = block_sum(r, red);
-  rf = block_sum(rf, red);
+  if ((t & 63) == 0) {
+    red[0][t >> 6] = r;
+    red[1][t >> 6] = rf;
+  }
+  __syncthreads();
   if (t == 0) {
-    p.part[2 * kb] = wbb * r;
-    p.part[2 * kb + 1] = wbb * rf;
+    double s0 = 0.0, s1 = 0.0;
+    for (int i = 0; i < kNT / 64; ++i) {
+      s0 += red[0][i];
+      s1 += red[1][i];
+    }
+    p.part[2 * kb] = wbb * s0;
+    p.part[2 * kb + 1] = wbb * s1;
   }
 }
 
 // One extraction step for one k: Z of the last Z pass (its partial sums
-// over b) mixed with the previous Zin (the first iteration takes it
-// unmixed), its eigenvectors; the leading nw - nfroz replace the
-// non-frozen columns of U_opt(k) (in ascending order of eigenvalue) and
-// their eigenvalues are subtracted from the Omega_I estimate.  From the
-// second iteration on, the eigenproblem starts from the eigenvectors of the
-// previous one (Vprev): A = Vprev^+ Z Vprev is nearly diagonal and the
-// Jacobi sweeps end after one to three; the eigenvectors are Vprev V.
+// over b) mixed with the previous Z (Zin, packed upper triangle; the first
+// iteration takes it unmixed), its eigenvectors; the leading nw - nfroz
+// replace the non-frozen columns of U_opt(k) (in ascending order of
+// eigenvalue) and their eigenvalues are subtracted from the Omega_I
+// estimate.  From the second iteration on, the eigenproblem starts from the
+// eigenvectors of the previous one, E = Vprev: A = E^+ Z E is nearly
+// diagonal and the Jacobi rotations go straight into E.  One Newton-Schulz
+// step E <- E (3 I - E^+ E) / 2 then keeps the product of the iterations'
+// rotations orthonormal to rounding (without it the basis drifts by ~n eps
+// per iteration).  LDS: A and E (nd x nd, leading dimension ldz); Z E
+// passes through the Vprev area of global memory.
 __global__ void __launch_bounds__(kNT) k_dis_update(DisProb p) {
   __shared__ JacobiLds J;
-  const DisState* st = p.st;
+  DisState* st = p.st;
   if (st->done) return;
   const int k = blockIdx.x, t = threadIdx.x;
   const Win w = p.win[k];
-  const int nb = p.nb, nw = p.nw, ld = p.ld, nn = p.nn;
+  const int nb = p.nb, nw = p.nw, ld = p.ldz, nn = p.nn;
   const int nd = w.ndim - w.nfroz, r = nw - w.nfroz;
-  const int ntri = nd * (nd + 1) / 2;
   double fz = 0.0;
   if (t == 0)
     for (int b = 0; b < nn; ++b) fz += p.part[2 * ((size_t)k * nn + b) + 1];
@@ -664,96 +928,65 @@ __global__ void __launch_bounds__(kNT) k_dis_update(DisProb p) {
   zd* Zi = p.Zin + (size_t)k * nb * nb;
   zd* Vp = p.Vprev + (size_t)k * nb * nb;
   zd* A = dis_lds;
-  zd* V = dis_lds + nb * ld;
-  for (int e = t; e < ntri; e += kNT) {
-    int i, j;  // row i <= column j
-    tri_index(e, i, j);
+  zd* E = dis_lds + ld * p.nzmax;
+  const zd* Zp = p.Zpart + (size_t)k * nn * p.ntri;
+  for (int e = t; e < nd * nd; e += kNT) {
+    const int i = e % nd, j = e / nd;
+    E[i + j * ld] = first ? zmk(i == j ? 1.0 : 0.0, 0.0) : Vp[i + (size_t)j * nb];
+    if (i > j) continue;
+    const int x = j * (j + 1) / 2 + i;
     zd z = zmk(0.0, 0.0);
-    for (int b = 0; b < nn; ++b) z = zadd(z, p.Zpart[((size_t)k * nn + b) * p.ntri + e]);
-    zd v;
-    if (first) {
-      v = z;
-    } else {
-      const zd zi = Zi[i + (size_t)j * nb];
-      v = zadd(zscale(z, mix), zscale(zi, 1.0 - mix));
-    }
-    Zi[i + (size_t)j * nb] = v;
-    Zi[j + (size_t)i * nb] = zconj(v);
-    if (i == j) v.y = 0.0;
-    A[i + j * ld] = v;
+    for (int b = 0; b < nn; ++b) z = zadd(z, Zp[(size_t)b * p.ntri + x]);
+    const zd v = first ? z : zadd(zscale(z, mix), zscale(Zi[x], 1.0 - mix));
+    Zi[x] = v;
+    A[i + j * ld] = i == j ? zmk(v.x, 0.0) : v;
     A[j + i * ld] = zconj(v);
   }
   __syncthreads();
   if (!first) {
-    // A <- Vprev^+ A Vprev, through V
-    for (int e = t; e < nd * nd; e += kNT) {
-      const int i = e % nd, j = e / nd;
-      zd s = zmk(0.0, 0.0);
-      for (int l = 0; l < nd; ++l) zfma(s, A[i + l * ld], Vp[l + (size_t)j * nb]);
-      V[i + j * ld] = s;
-    }
+    // Z E into the Vprev area (E holds its contents), then A = E^+ (Z E)
+    zgemm<false>(nd, nd, nd, A, ld, E, ld, [&](int i, int j, zd v) { Vp[i + (size_t)j * nb] = v; });
+    __threadfence_block();
     __syncthreads();
-    for (int e = t; e < nd * nd; e += kNT) {
-      const int i = e % nd, j = e / nd;
-      if (i > j) continue;
-      zd s = zmk(0.0, 0.0);
-      for (int l = 0; l < nd; ++l) zcfma(s, Vp[l + (size_t)i * nb], V[l + j * ld]);
-      if (i == j) s.y = 0.0;
-      A[i + j * ld] = s;
-      A[j + i * ld] = zconj(s);
-    }
+    zgemm_herm(nd, nd, E, ld, Vp, nb, [&](int i, int j, zd v) {
+      A[i + j * ld] = i == j ? zmk(v.x, 0.0) : v;
+      A[j + i * ld] = zconj(v);
+    });
     __syncthreads();
   }
-  if (jacobi_eigh(A, V, nd, ld, J) >= kMaxSweeps && t == 0) atomicOr(p.flags, F_JACOBI);
-  eig_order(A, nd, ld, J);
-  // the eigenvectors of Z: E = Vprev V (or V) into the A area, then one
-  // Newton-Schulz step E <- E (3 I - E^+ E) / 2 to Vprev, which keeps the
-  // product of the iterations' rotations orthonormal to rounding (without
-  // it the basis drifts by ~n eps per iteration)
-  for (int e = t; e < nd * nd; e += kNT) {
-    const int i = e % nd, j = e / nd;
-    zd s;
-    if (first) {
-      s = V[i + j * ld];
+  const int sweeps = jacobi_run(A, E, nd, ld, J);
+  if (t == 0) {
+    if (sweeps < 0) {
+      atomicOr(p.flags, F_JACOBI);
     } else {
-      s = zmk(0.0, 0.0);
-      for (int l = 0; l < nd; ++l) zfma(s, Vp[i + (size_t)l * nb], V[l + j * ld]);
+      atomicMax(&st->sweeps_max, sweeps);
+      atomicAdd(&st->sweeps_sum, (unsigned long long)sweeps);
+      atomicAdd(&st->solves, 1ull);
     }
-    A[i + j * ld] = s;
   }
-  __syncthreads();
-  for (int e = t; e < nd * nd; e += kNT) {
-    const int i = e % nd, j = e / nd;
-    if (i > j) continue;
-    zd s = zmk(0.0, 0.0);
-    for (int l = 0; l < nd; ++l) zcfma(s, A[l + i * ld], A[l + j * ld]);
-    if (i == j) s.y = 0.0;
-    V[i + j * ld] = s;
-    V[j + i * ld] = zconj(s);
-  }
-  __syncthreads();
-  for (int e = t; e < nd * nd; e += kNT) {
-    const int i = e % nd, j = e / nd;
-    zd s = zscale(A[i + j * ld], 1.5);
-    for (int l = 0; l < nd; ++l) zfma(s, A[i + l * ld], zscale(V[l + j * ld], -0.5));
-    Vp[i + (size_t)j * nb] = s;
-  }
-  __threadfence_block();
+  eig_order(A, nd, ld, J);
+  // E^+ E into the A area (the eigenvalues are in J.lam), then the
+  // Newton-Schulz step to Vprev and the selected columns to U_opt
+  zgemm_herm(nd, nd, E, ld, E, ld, [&](int i, int j, zd v) {
+    A[i + j * ld] = i == j ? zmk(v.x, 0.0) : v;
+    A[j + i * ld] = zconj(v);
+  });
   __syncthreads();
   zd* U = p.Uopt + (size_t)k * nb * nw;
-  for (int e = t; e < w.ndim * r; e += kNT) {
-    const int i = e % w.ndim, l = e / w.ndim;
-    U[i + (size_t)(w.nfroz + l) * nb] = zmk(0.0, 0.0);
-  }
-  __syncthreads();
-  for (int e = t; e < nd * r; e += kNT) {
-    const int i = e % nd, l = e / nd;
-    U[nf_row(w, i) + (size_t)(w.nfroz + l) * nb] = Vp[i + (size_t)J.perm[nd - r + l] * nb];
-  }
+  const int lead = nd - r;
+  zgemm<false>(nd, nd, nd, E, ld, A, ld, [&](int i, int j, zd v) {
+    const zd s = zsub(zscale(E[i + j * ld], 1.5), zscale(v, 0.5));
+    Vp[i + (size_t)j * nb] = s;
+    const int c = J.rank[j] - lead;
+    if (c >= 0) U[nf_row(w, i) + (size_t)(w.nfroz + c) * nb] = s;
+  });
+  // the frozen rows of the non-frozen columns
+  for (int e = t; e < w.nfroz * r; e += kNT)
+    U[w.fmin + e % w.nfroz + (size_t)(w.nfroz + e / w.nfroz) * nb] = zmk(0.0, 0.0);
   if (t == 0) {
     double s = (double)nw * p.wbtot;
     if (w.nfroz > 0) s -= fz;
-    for (int l = 0; l < r; ++l) s -= J.lam[J.perm[nd - r + l]];
+    for (int l = 0; l < r; ++l) s -= J.lam[J.perm[lead + l]];
     p.wk1[k] = s;
   }
 }
@@ -814,7 +1047,7 @@ __global__ void __launch_bounds__(kNT) k_dis_ham(DisProb p) {
     H[j + i * ld] = zconj(s);
   }
   __syncthreads();
-  if (jacobi_eigh(H, V, nw, ld, J) >= kMaxSweeps && t == 0) atomicOr(p.flags, F_JACOBI);
+  if (jacobi_eigh(H, V, nw, ld, J) < 0 && t == 0) atomicOr(p.flags, F_JACOBI);
   eig_order(H, nw, ld, J);
   if (t < nw) p.eigopt[(size_t)k * nb + t] = J.lam[J.perm[t]];
   // U_opt V into the H area, then back
@@ -1046,6 +1279,17 @@ int w90gpu_disentangle(w90gpu_ctx* ctx, const w90gpu_problem* prob, int num_band
   DCHK(hipSetDevice(ctx->device));
   const int ld = nb + 1;
   const size_t lds = 2 * (size_t)nb * ld * sizeof(zd);
+  // the per-iteration kernels size their LDS by the windows: k_dis_zmat
+  // holds two ndim x nw blocks, k_dis_update two (ndim - nfroz)^2 blocks
+  // (odd leading dimensions: rows and columns of complex doubles both
+  // spread over the banks)
+  int ndmax = 1, nzmax = 1;
+  for (const Win& wk : win) {
+    ndmax = std::max(ndmax, wk.ndim);
+    if (nw > wk.nfroz) nzmax = std::max(nzmax, wk.ndim - wk.nfroz);
+  }
+  const int ldc = ndmax | 1, ldz = nzmax | 1;
+  const size_t lds_zmat = 2 * (size_t)ldc * nw * sizeof(zd), lds_update = 2 * (size_t)ldz * nzmax * sizeof(zd);
   if ((rc = set_lds_limits(ctx, 2 * (size_t)kMaxN * (kMaxN + 1) * sizeof(zd)))) return rc;
 
   const size_t zb = sizeof(zd);
@@ -1101,6 +1345,9 @@ int w90gpu_disentangle(w90gpu_ctx* ctx, const w90gpu_problem* prob, int num_band
   p.Zpart = (zd*)(base + o_Zpart);
   p.Vprev = (zd*)(base + o_Vprev);
   p.ntri = nb * (nb + 1) / 2;
+  p.ldc = ldc;
+  p.nzmax = nzmax;
+  p.ldz = ldz;
   p.U = (zd*)(base + o_U);
   p.W = (zd*)(base + o_W);
   p.Mw = (zd*)(base + o_Mw);
@@ -1146,14 +1393,14 @@ int w90gpu_disentangle(w90gpu_ctx* ctx, const w90gpu_problem* prob, int num_band
   // the extraction: Z of the initial U_opt, then (update, Z, reduce) per
   // iteration, polled every kPoll iterations
   constexpr int kPoll = 16;
-  hipLaunchKernelGGL(k_dis_zmat, dim3(nk * nn), dim3(kNT), lds, sm, p);
+  hipLaunchKernelGGL(k_dis_zmat, dim3(nk * nn), dim3(kNT), lds_zmat, sm, p);
   DisState st;
   int launched = 0;
   while (launched < ctl->num_iter) {
     const int n = std::min(kPoll, ctl->num_iter - launched);
     for (int i = 0; i < n; ++i) {
-      hipLaunchKernelGGL(k_dis_update, dim3(nk), dim3(kNT), lds, sm, p);
-      hipLaunchKernelGGL(k_dis_zmat, dim3(nk * nn), dim3(kNT), lds, sm, p);
+      hipLaunchKernelGGL(k_dis_update, dim3(nk), dim3(kNT), lds_update, sm, p);
+      hipLaunchKernelGGL(k_dis_zmat, dim3(nk * nn), dim3(kNT), lds_zmat, sm, p);
       hipLaunchKernelGGL(k_dis_reduce, dim3(1), dim3(kNT), 0, sm, p);
     }
     DCHK(hipGetLastError());
@@ -1183,6 +1430,8 @@ int w90gpu_disentangle(w90gpu_ctx* ctx, const w90gpu_problem* prob, int num_band
   }
   if (result) {
     result->omega_invariant = st.womegai;
+    result->sweeps_max = st.sweeps_max;
+    result->sweeps_mean = st.solves ? (double)st.sweeps_sum / (double)st.solves : 0.0;
     result->iterations = st.iter;
     result->converged = st.converged;
     result->status = W90GPU_OK;

@@ -78,9 +78,11 @@ module w90gpu_c
   type, bind(c) :: w90gpu_dis_result
     real(c_double) :: omega_invariant
     real(c_double) :: seconds
+    real(c_double) :: sweeps_mean
     integer(c_int) :: iterations
     integer(c_int) :: converged
     integer(c_int) :: status
+    integer(c_int) :: sweeps_max
   end type w90gpu_dis_result
 
   interface
