@@ -3,7 +3,7 @@ runs on its own inputs (tests/golden/dis/: test-suite example03 silicon,
 example04 copper and gaas_disentanglement_issue192 with frozen windows,
 example11_1 silicon valence whose windows hold exactly num_wann states,
 write_u_matrices_disent with an outer window only) and against the oracle on
-synthetic problems up to 64 bands.
+synthetic problems up to 64 bands and on the benchmark's 512-k problem.
 
 Tolerances: Omega_I per iteration 1e-9 relative to the oracle (the DIS
 lines of the reference, printed with 8 decimals, to 6e-9), the final
@@ -143,6 +143,21 @@ def test_disentangle_matches_oracle_synthetic(ctx, case):
     print(f"[dis synthetic] {case}", flush=True)
     p, ctl = synth.make_dis_problem(7 + nb, kind=kind, num_bands=nb, num_wann=nw, mp=mp, frozen=fr)
     ctl["num_iter"] = 60
+    o = po.dis_main(nb, nw, p.kmesh, ctl, p.eig, p.amat_dis, p.m_orig)
+    g = ctx.disentangle(p.kmesh, nb, ctl, p.eig, p.amat_dis, p.m_orig)
+    check_against(g, o, nw, well_posed=True, tol_w=1e-7)
+
+
+def test_disentangle_benchmark_problem_matches_oracle(ctx):
+    """The benchmark's problem at its full size (fcc, 48 bands to 24, 8x8x8
+    = 512 k, frozen bands): its first 10 iterations against the oracle's,
+    with the per-workgroup LDS and tile shapes of that size."""
+    from wannier90_amd import workloads
+
+    p, ctl = workloads.dis_problem()
+    nb, nw = workloads.DIS_PARAMS["num_bands"], workloads.DIS_PARAMS["num_wann"]
+    ctl = dict(ctl, num_iter=10)
+    print(f"[dis benchmark problem] nb={nb} nw={nw} nk={p.num_kpts}", flush=True)
     o = po.dis_main(nb, nw, p.kmesh, ctl, p.eig, p.amat_dis, p.m_orig)
     g = ctx.disentangle(p.kmesh, nb, ctl, p.eig, p.amat_dis, p.m_orig)
     check_against(g, o, nw, well_posed=True, tol_w=1e-7)
