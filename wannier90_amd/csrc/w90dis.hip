@@ -378,7 +378,7 @@ __device__ __forceinline__ int jacobi_eigh(zd* A, zd* V, int n, int ld, JacobiLd
 // register tiles, one tile per lane and pass: op(A)(i, l) = A[i + l * lda],
 // or conj(A[l + i * lda]) when ADJ; B(l, j) = B[l + j * ldb].  Rows and
 // columns past the edge are read clamped and not stored.
-template <int RT, int CT, bool ADJ, class Out>
+template <int RT, int CT, bool ADJ, int UNR = 2, class Out>
 __device__ void zgemm_tiles(int M, int N, int K, const zd* A, int lda, const zd* B, int ldb, Out out) {
   const int tr = (M + RT - 1) / RT, tc = (N + CT - 1) / CT;
   for (int x = threadIdx.x; x < tr * tc; x += kNT) {
@@ -393,7 +393,7 @@ __device__ void zgemm_tiles(int M, int N, int K, const zd* A, int lda, const zd*
     for (int a = 0; a < RT; ++a)
 #pragma unroll
       for (int b = 0; b < CT; ++b) acc[a][b] = zmk(0.0, 0.0);
-#pragma unroll 2
+#pragma unroll UNR
     for (int l = 0; l < K; ++l) {
       zd av[RT], bv[CT];
 #pragma unroll
@@ -418,6 +418,16 @@ __device__ void zgemm_tiles(int M, int N, int K, const zd* A, int lda, const zd*
 
 // zgemm_tiles with the smallest tile that needs at most one pass (M, N <=
 // 64 always fit 4 x 4 tiles).
+// zgemm with tiles of at most 2 x 3 (several passes for large results):
+// the register budget of k_dis_zmat, whose (k, b) workgroups wait on
+// memory and gain from more of them per CU; UNR K steps unrolled (four
+// when A is in global memory, to request its loads ahead).
+template <bool ADJ, int UNR, class Out>
+__device__ void zgemm_small(int M, int N, int K, const zd* A, int lda, const zd* B, int ldb, Out out) {
+  if (((M + 1) / 2) * ((N + 1) / 2) <= kNT) zgemm_tiles<2, 2, ADJ, UNR>(M, N, K, A, lda, B, ldb, out);
+  else zgemm_tiles<2, 3, ADJ, UNR>(M, N, K, A, lda, B, ldb, out);
+}
+
 template <bool ADJ, class Out>
 __device__ void zgemm(int M, int N, int K, const zd* A, int lda, const zd* B, int ldb, Out out) {
   auto fits = [&](int r, int c) { return ((M + r - 1) / r) * ((N + c - 1) / c) <= kNT; };
@@ -854,7 +864,7 @@ __global__ void __launch_bounds__(kNT) k_dis_zmat(DisProb p) {
   }
   __syncthreads();
   const zd* Mkb = p.M + ((size_t)k * nn + b) * nb * nb + w.first + (size_t)w2.first * nb;
-  zgemm<false>(w.ndim, nw, w2.ndim, Mkb, nb, U2, ldc, [&](int i, int j, zd v) { C[i + j * ldc] = v; });
+  zgemm_small<false, 4>(w.ndim, nw, w2.ndim, Mkb, nb, U2, ldc, [&](int i, int j, zd v) { C[i + j * ldc] = v; });
   __syncthreads();
   for (int e = t; e < w.ndim * nw; e += kNT) {
     const int l = e % w.ndim, j = e / w.ndim;
@@ -864,13 +874,11 @@ __global__ void __launch_bounds__(kNT) k_dis_zmat(DisProb p) {
   const size_t kb = (size_t)k * nn + b;
   if (nw > w.nfroz) {
     zd* Zp = p.Zpart + kb * p.ntri;
-    const int nt2 = (nd + 1) / 2;
-    if (nt2 * (nt2 + 1) / 2 <= kNT) zpart_tiles<2>(C, ldc, w, nd, nw, wbb, Zp);
-    else zpart_tiles<3>(C, ldc, w, nd, nw, wbb, Zp);
+    zpart_tiles<2>(C, ldc, w, nd, nw, wbb, Zp);
   }
   __syncthreads();
   double r = 0.0, rf = 0.0;
-  zgemm<true>(nw, nw, w.ndim, U2, ldc, C, ldc, [&](int i, int, zd v) {
+  zgemm_small<true, 2>(nw, nw, w.ndim, U2, ldc, C, ldc, [&](int i, int, zd v) {
     const double a2 = zabs2(v);
     r += a2;
     if (i < w.nfroz) rf += a2;
