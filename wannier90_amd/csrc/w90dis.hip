@@ -44,6 +44,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/w90gpu.h"
@@ -1228,8 +1229,11 @@ struct DevBuf {
   }
 };
 
+// Once per process (calls from several host threads are serialised).
 int set_lds_limits(w90gpu_ctx* ctx, size_t bytes) {
+  static std::mutex mu;
   static bool done = false;
+  std::lock_guard<std::mutex> lock(mu);
   if (done) return W90GPU_OK;
   const void* ks[] = {(const void*)k_dis_project, (const void*)k_dis_proj_froz, (const void*)k_dis_zmat,
                       (const void*)k_dis_update,  (const void*)k_dis_ham,       (const void*)k_dis_find_u,
