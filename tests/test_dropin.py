@@ -278,6 +278,43 @@ def test_driver_disentangles_like_the_reference(tmp_path):
     assert om2["omega_total"] <= om["omega_total"] + 1e-8
 
 
+@pytest.mark.gpu
+@needs_exe
+def test_driver_writes_u_matrices_of_the_disentanglement(tmp_path):
+    """write_u_matrices after a disentanglement, on the reference's
+    write_u_matrices_disent inputs (8 bands to 4 WFs, outer window only):
+    _u_dis.mat holds U_opt with its window rows at their band positions and
+    zeros elsewhere (plot_u_matrices, src/plot.F90:2266-2285), _u.mat the
+    unitary U.  Compared with the reference's U_opt through the projector
+    onto the optimal subspace, to the files' 10 decimals: U_opt's columns
+    are the subspace Hamiltonian's eigenvectors, whose phases LAPACK and
+    this build fix differently, and U_opt^+ A is rank-deficient here, so U
+    itself is not unique."""
+    from wannier90_amd import w90io
+
+    seed, z = write_dis_inputs("dis_li_window", tmp_path, {"write_u_matrices": "true"})
+    r = run(seed, tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr + open(os.path.join(tmp_path, seed + ".werr")).read()
+    uopt, kpts = w90io.read_u_dis_mat(os.path.join(tmp_path, seed + "_u_dis.mat"))
+    u, kpts2 = w90io.read_u_mat(os.path.join(tmp_path, seed + "_u.mat"))
+    nb, nw = int(z["num_bands"]), int(z["num_wann"])
+    nk = z["u_opt"].shape[2]
+    assert uopt.shape == (nb, nw, nk) and u.shape == (nw, nw, nk)
+    assert np.abs(kpts - z["kpt_latt"]).max() <= 1e-10 and np.abs(kpts2 - z["kpt_latt"]).max() <= 1e-10
+    first = np.argmax(z["lwindow"], axis=0)
+    nd = z["ndimwin"]
+    ref = np.zeros_like(uopt)
+    for k in range(nk):
+        ref[first[k]: first[k] + nd[k], :, k] = z["u_opt"][: nd[k], :, k]
+        outside = np.ones(nb, bool)
+        outside[first[k]: first[k] + nd[k]] = False
+        assert np.abs(uopt[outside, :, k]).max(initial=0.0) == 0.0
+        assert np.abs(u[:, :, k].conj().T @ u[:, :, k] - np.eye(nw)).max() <= 1e-9
+    p = np.einsum("iwk,jwk->ijk", uopt, uopt.conj())
+    pr = np.einsum("iwk,jwk->ijk", ref, ref.conj())
+    assert np.abs(p - pr).max() <= 1e-8
+
+
 def test_pp_accepts_disentanglement_inputs(tmp_path):
     """-pp with num_bands > num_wann and the dis_* keywords: the k-mesh and
     the keyword checks of the disentanglement run before any GPU work."""

@@ -41,7 +41,7 @@ module w90gpu_io
 
   public :: settings_add, settings_find, settings_clear, read_win, lower
   public :: get_int, get_real, get_logical, get_text, get_i1d, get_r2d, get_block
-  public :: read_mmn, read_amn, read_eig, write_chk, read_chk, write_nnkp, write_xyz, write_umat
+  public :: read_mmn, read_amn, read_eig, write_chk, read_chk, write_nnkp, write_xyz, write_umat, write_umat_dis
   public :: datetime
 
 contains
@@ -730,5 +730,35 @@ contains
     end do
     close (u)
   end subroutine write_umat
+
+  !> _u_dis.mat (plot_u_matrices, src/plot.F90:2266-2285): U_opt with its
+  !> window rows at their band positions (zero outside the window), in the
+  !> layout of _u.mat; the header gives num_kpts, num_wann, num_bands.
+  subroutine write_umat_dis(path, num_wann, num_bands, num_kpts, kpt_latt, u_opt, ndimwin, nfirstwin)
+    character(len=*), intent(in) :: path
+    integer, intent(in) :: num_wann, num_bands, num_kpts
+    real(dp), intent(in) :: kpt_latt(3, num_kpts)
+    complex(dp), intent(in) :: u_opt(num_bands, num_wann, num_kpts)
+    integer, intent(in) :: ndimwin(num_kpts), nfirstwin(num_kpts)
+    complex(dp) :: col(num_bands)
+    integer :: u, k, i, j, f, nd
+    character(len=9) :: cdate, ctime
+    call datetime(cdate, ctime)
+    open (newunit=u, file=path, form='formatted')
+    write (u, *) 'written on '//cdate//' at '//ctime
+    write (u, '(3i12)') num_kpts, num_wann, num_bands
+    do k = 1, num_kpts
+      write (u, *)
+      write (u, '(f15.10,sp,f15.10,sp,f15.10)') kpt_latt(:, k)
+      f = nfirstwin(k)
+      nd = ndimwin(k)
+      do j = 1, num_wann
+        col = (0.0_dp, 0.0_dp)
+        col(f:f + nd - 1) = u_opt(1:nd, j, k)
+        write (u, '(f15.10,sp,f15.10)') (col(i), i=1, num_bands)
+      end do
+    end do
+    close (u)
+  end subroutine write_umat_dis
 
 end module w90gpu_io

@@ -923,8 +923,8 @@ contains
   end subroutine print_dis
 
   !> The output writers of plot_main on this path: _centres.xyz (write_xyz)
-  !> and _u.mat (write_u_matrices).  Band/Fermi-surface/WF plotting is not
-  !> carried by this build.
+  !> and _u.mat, with _u_dis.mat after a disentanglement (write_u_matrices).
+  !> Band/Fermi-surface/WF plotting is not carried by this build.
   subroutine w90_plot(common_data, istdout, istderr, ierr)
     type(lib_common_type), intent(inout) :: common_data
     integer, intent(in) :: istdout, istderr
@@ -946,6 +946,14 @@ contains
     if (common_data%write_u_matrices .and. associated(common_data%u_matrix)) then
       call write_umat(trim(common_data%seedname)//'_u.mat', common_data%num_wann, common_data%num_kpts, &
                       common_data%kpt_latt, common_data%u_matrix)
+      if (common_data%have_disentangled .and. associated(common_data%u_opt) .and. &
+          allocated(common_data%ndimwin)) then
+        call write_umat_dis(trim(common_data%seedname)//'_u_dis.mat', common_data%num_wann, &
+                            common_data%num_bands, common_data%num_kpts, common_data%kpt_latt, &
+                            common_data%u_opt(1:common_data%num_bands, 1:common_data%num_wann, &
+                                              1:common_data%num_kpts), common_data%ndimwin, &
+                            common_data%nfirstwin)
+      end if
     end if
     if (istdout < 0 .and. istderr < 0) ierr = 0
   end subroutine w90_plot

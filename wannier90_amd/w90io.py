@@ -1,7 +1,7 @@
 """wannier90 file formats used on the wannierise path.
 
 Writers: .win, .mmn, .amn, .eig (inputs), .chk (unformatted checkpoint),
-_centres.xyz, _u.mat.  Readers: .mmn, .amn, .chk, .nnkp, _u.mat and the
+_centres.xyz, _u.mat.  Readers: .mmn, .amn, .chk, .nnkp, _u.mat, _u_dis.mat and the
 final state of .wout.  Formats follow src/overlap.F90:152-430 (.mmn/.amn),
 src/readwrite.F90:583-635 (.eig), src/wannier90_readwrite.F90:2278-2373
 (.chk), src/plot.F90:2224-2288 (_u.mat) and :2790-2862 (_centres.xyz).
@@ -246,6 +246,26 @@ def read_u_mat(path):
         vals = np.array(toks[p:p + 2 * nw1 * nw2], dtype=float).reshape(-1, 2)
         p += 2 * nw1 * nw2
         u[:, :, k] = (vals[:, 0] + 1j * vals[:, 1]).reshape((nw1, nw2), order="F")
+    return u, kpts
+
+
+def read_u_dis_mat(path):
+    """_u_dis.mat (plot_u_matrices, src/plot.F90:2266-2285): U_opt as
+    (num_bands, num_wann, num_kpts), window rows at their band positions;
+    the header gives num_kpts, num_wann, num_bands."""
+    with open(path) as f:
+        f.readline()
+        nk, nw, nb = (int(x) for x in f.readline().split())
+        toks = f.read().split()
+    u = np.zeros((nb, nw, nk), dtype=np.complex128, order="F")
+    kpts = np.zeros((nk, 3))
+    p = 0
+    for k in range(nk):
+        kpts[k] = [float(x) for x in toks[p:p + 3]]
+        p += 3
+        vals = np.array(toks[p:p + 2 * nb * nw], dtype=float).reshape(-1, 2)
+        p += 2 * nb * nw
+        u[:, :, k] = (vals[:, 0] + 1j * vals[:, 1]).reshape((nb, nw), order="F")
     return u, kpts
 
 
