@@ -53,8 +53,11 @@ void cset_u_opt(void* blob, void* u_opt);
 void cget_nn(void* blob, int* nntot);
 void cget_nnkp(void* blob, int* nnlist);
 
-/* steps (src/c_interface.F90:27, :38).  cdisentangle is a no-op when
- * num_bands = num_wann and an error otherwise (not carried by this build). */
+/* steps (src/c_interface.F90:27, :38).  cdisentangle runs dis_main and
+ * setup_m_loc (w90_disentangle, src/library_interface.F90:362-420) on the
+ * GPU: M (num_bands^2 per (k, b)) through cset_m_local, the projections A
+ * in U_opt, the band energies through cset_eigval; it leaves U_opt, U and
+ * the num_wann^2 M blocks that cwannierise starts from. */
 void cdisentangle(void* blob, int* ierr);
 void cwannierise(void* blob, int* ierr);
 

@@ -148,15 +148,11 @@ def test_host_library_exports_the_reference_c_interface():
     assert not missing, missing
 
 
-@pytest.mark.gpu
-@needs_exe
-def test_c_caller_of_reference_interface(tmp_path):
-    """A C++ program using the reference's C interface (options, k-mesh,
-    caller-owned arrays, project, wannierise, getters) reproduces the
-    reference run of the same problem (cfg1, 60 iterations, no window)."""
+def run_c_demo(tmp_path, nw, nb, mp, ctl, real_lattice, kpts, nn, m, a, eig=None, dctl=None):
+    """Build tests/c_demo/libc_demo.cc against the host library, run it on
+    one problem (m (nb,nb,nn,nk), a (nb,nw,nk), Fortran order) and return
+    ierr, nnlist, U, U_opt, centres, spreads, omega."""
     import struct
-
-    from wannier90_amd import synth
 
     exe = os.path.join(tmp_path, "libc_demo")
     b = subprocess.run(["g++", "-O1", "-std=c++17", f"-I{ROOT}/include", "-I/opt/rocm/llvm/include/flang",
@@ -164,18 +160,22 @@ def test_c_caller_of_reference_interface(tmp_path):
                         f"-L{ROOT}/wannier90_amd/lib", "-lw90host", "-lw90gpu",
                         f"-Wl,-rpath,{ROOT}/wannier90_amd/lib", "-o", exe], capture_output=True, text=True)
     assert b.returncode == 0, b.stderr
-    c = Case("cfg1_nowindow")
-    prob = synth.config(1)
-    nw, nk, nn = prob.num_wann, prob.num_kpts, prob.kmesh.nntot
-    ctl = c.ctl
+    nk = kpts.shape[0]
+    d = dctl or {"num_iter": 0, "conv_window": 0, "win_min": -2e300, "win_max": 2e300, "froz_min": -2e300,
+                 "froz_max": 2e300, "mix_ratio": 0.5, "conv_tol": 1e-10, "frozen_states": 0}
+    froz = (d["froz_min"], d["froz_max"]) if d["frozen_states"] else (-2e300, 2e300)
     with open(os.path.join(tmp_path, "in.bin"), "wb") as f:
-        f.write(struct.pack("<8i", nw, nk, *prob.mp, ctl["num_iter"], ctl["num_cg_steps"], ctl["conv_window"]))
-        f.write(struct.pack("<2d", ctl["conv_tol"], ctl["trial_step"]))
-        f.write(np.ascontiguousarray(prob.real_lattice, dtype="<f8").tobytes())
-        f.write(np.ascontiguousarray(prob.kpts, dtype="<f8").tobytes())
+        f.write(struct.pack("<11i", nw, nb, nk, *mp, ctl["num_iter"], ctl["num_cg_steps"], ctl["conv_window"],
+                            d["num_iter"], d["conv_window"]))
+        f.write(struct.pack("<8d", ctl["conv_tol"], ctl["trial_step"], d["win_min"], d["win_max"], *froz,
+                            d["mix_ratio"], d["conv_tol"]))
+        f.write(np.ascontiguousarray(real_lattice, dtype="<f8").tobytes())
+        f.write(np.ascontiguousarray(kpts, dtype="<f8").tobytes())
         f.write(struct.pack("<i", nn))
-        f.write(np.asarray(prob.m_orig, dtype="<c16").tobytes(order="F"))
-        f.write(np.asarray(np.transpose(prob.amat, (1, 2, 0)), dtype="<c16").tobytes(order="F"))
+        f.write(np.asarray(m, dtype="<c16").tobytes(order="F"))
+        f.write(np.asarray(a, dtype="<c16").tobytes(order="F"))
+        if nb > nw:
+            f.write(np.asarray(eig, dtype="<f8").tobytes(order="F"))
     r = subprocess.run([exe, "in.bin", "out.bin"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     raw = open(os.path.join(tmp_path, "out.bin"), "rb").read()
@@ -185,16 +185,64 @@ def test_c_caller_of_reference_interface(tmp_path):
     pos += 4 * nk * nn_out
     u = np.frombuffer(raw, "<c16", nw * nw * nk, pos).reshape((nw, nw, nk), order="F")
     pos += 16 * nw * nw * nk
+    uopt = np.frombuffer(raw, "<c16", nb * nw * nk, pos).reshape((nb, nw, nk), order="F")
+    pos += 16 * nb * nw * nk
     cen = np.frombuffer(raw, "<f8", 3 * nw, pos).reshape((3, nw), order="F")
     pos += 24 * nw
     spr = np.frombuffer(raw, "<f8", nw, pos)
     pos += 8 * nw
     om = np.frombuffer(raw, "<f8", 4, pos)
-    assert ierr == 0 and nn_out == nn and np.array_equal(nnlist, c.kmesh.nnlist)
+    assert nn_out == nn
+    return ierr, nnlist, u, uopt, cen, spr, om
+
+
+@pytest.mark.gpu
+@needs_exe
+def test_c_caller_of_reference_interface(tmp_path):
+    """A C++ program using the reference's C interface (options, k-mesh,
+    caller-owned arrays, project, wannierise, getters) reproduces the
+    reference run of the same problem (cfg1, 60 iterations, no window)."""
+    from wannier90_amd import synth
+
+    c = Case("cfg1_nowindow")
+    prob = synth.config(1)
+    nw, nn = prob.num_wann, prob.kmesh.nntot
+    ierr, nnlist, u, _, cen, spr, om = run_c_demo(tmp_path, nw, nw, prob.mp, c.ctl, prob.real_lattice, prob.kpts,
+                                                  nn, prob.m_orig, np.transpose(prob.amat, (1, 2, 0)))
+    assert ierr == 0 and np.array_equal(nnlist, c.kmesh.nnlist)
     assert np.abs(u - c.z["u_final"]).max() <= 1e-7
     assert np.abs(cen - c.z["centres"]).max() <= 1e-6
     assert np.abs(spr - c.z["spreads"]).max() <= 1e-8
     assert abs(om[3] - c.z["omega"][3]) <= 2e-9
+
+
+@pytest.mark.gpu
+@needs_exe
+def test_c_caller_disentangles_like_the_reference(tmp_path):
+    """The same C++ caller with num_bands > num_wann (the reference's silicon
+    example, 12 bands to 8 WFs, outer and frozen windows): band energies
+    through cset_eigval, the projections in U_opt, cdisentangle, then
+    cwannierise, against the reference's run: the optimal subspace (the
+    projector U_opt U_opt^+), U_opt U, the spreads and Omega (tolerances of
+    the driver's test of the same run)."""
+    import json
+
+    z = np.load(os.path.join(ROOT, "tests", "golden", "dis", "dis_silicon.npz"))
+    nb, nw = int(z["num_bands"]), int(z["num_wann"])
+    ctl = json.loads(str(z["control"]))
+    dctl = json.loads(str(z["dis_control"]))
+    ierr, nnlist, u, uopt, cen, spr, om = run_c_demo(tmp_path, nw, nb, tuple(int(x) for x in z["mp_grid"]), ctl,
+                                                     z["real_lattice"], z["kpt_latt"], int(z["nnlist"].shape[1]),
+                                                     z["m"], z["a"], z["eig"], dctl)
+    assert ierr == 0 and np.array_equal(nnlist, z["nnlist"])
+    p = np.einsum("iwk,jwk->ijk", uopt, uopt.conj())
+    pr = np.einsum("iwk,jwk->ijk", z["u_opt"], z["u_opt"].conj())
+    assert np.abs(p - pr).max() <= 1e-8
+    w = np.einsum("bwk,wvk->bvk", uopt, u)
+    wr = np.einsum("bwk,wvk->bvk", z["u_opt"], z["u_final"])
+    assert np.abs(w - wr).max() <= 1e-5
+    assert np.abs(np.sort(spr) - np.sort(z["spreads"])).max() <= 5e-6
+    assert abs(om[3] - float(z["omega"][3])) <= 2e-8
 
 
 @pytest.mark.gpu
