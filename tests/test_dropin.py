@@ -380,3 +380,78 @@ def test_pp_accepts_disentanglement_inputs(tmp_path):
         r = run(seed, tmp_path, "-pp")
         assert r.returncode != 0
         assert msg in open(os.path.join(tmp_path, seed + ".werr")).read()
+
+
+def _library_with_options(z, seedname):
+    """wannier90_amd.library with the k-point keywords as options and the
+    rest of seedname.win through the reader (the reference's order:
+    input_setopt, then input_reader)."""
+    from wannier90_amd import library
+
+    lib = library.Library()
+    lib.set_option("num_wann", int(z["num_wann"]))
+    lib.set_option("num_bands", int(z["num_bands"]))
+    lib.set_option("unit_cell_cart", np.asarray(z["real_lattice"], dtype=float))
+    lib.set_option("mp_grid", np.asarray(z["mp_grid"], dtype=np.int32))
+    lib.set_option("kpoints", np.asarray(z["kpt_latt"], dtype=float))
+    assert lib.input_setopt(seedname) == 0
+    assert lib.input_reader() == 0
+    return lib
+
+
+def test_python_library_interface_reads_options_and_win(tmp_path):
+    """The Python mirror of the library interface (wannier90_amd.library,
+    over libw90host.so's C interface): options, input_setopt, the .win
+    through input_reader, and the k-mesh getters, which need no GPU; the
+    reader with unspent options and no setopt is refused as the reference
+    refuses it (src/library_interface.F90:312-318)."""
+    if not os.path.exists(HOST_LIB):
+        pytest.skip("Fortran host not built")
+    from wannier90_amd import library
+
+    seed, z = write_dis_inputs("dis_silicon", tmp_path)
+    lib = _library_with_options(z, os.path.join(str(tmp_path), seed))
+    nk = z["kpt_latt"].shape[0]
+    assert lib.get_nn() == z["nnlist"].shape[1]
+    assert np.array_equal(lib.get_nnkp(nk), z["nnlist"])
+    lib.close()
+    lib2 = library.Library()
+    lib2.set_option("num_wann", int(z["num_wann"]))
+    assert lib2.input_reader() > 0
+    lib2.close()
+
+
+@pytest.mark.gpu
+@needs_exe
+def test_python_library_interface_disentangles_like_the_reference(tmp_path):
+    """The reference's silicon run (12 bands to 8 WFs, outer and frozen
+    windows) from Python through the library interface: overlaps and band
+    energies as caller-owned arrays, disentangle, wannierise, the getters;
+    against the reference's run with the tolerances of the driver's test."""
+    from wannier90_amd import capi
+
+    seed, z = write_dis_inputs("dis_silicon", tmp_path)
+    lib = _library_with_options(z, os.path.join(str(tmp_path), seed))
+    nb, nw = int(z["num_bands"]), int(z["num_wann"])
+    nk = z["kpt_latt"].shape[0]
+    km = capi.kmesh_get(np.asarray(z["real_lattice"]), np.asarray(z["kpt_latt"]))
+    m = capi.read_mmn(os.path.join(tmp_path, seed + ".mmn"), nb, km)
+    a = capi.read_amn(os.path.join(tmp_path, seed + ".amn"), nb, nk, nw)
+    eig = np.loadtxt(os.path.join(tmp_path, seed + ".eig"))[:, 2].reshape(nk, nb).T.copy(order="F")
+    u = np.zeros((nw, nw, nk), dtype=np.complex128, order="F")
+    lib.set_m_local(m)
+    lib.set_u_matrix(u)
+    lib.set_u_opt(a)
+    lib.set_eigval(eig)
+    assert lib.disentangle() == 0
+    assert lib.wannierise() <= 0
+    spr, om = lib.get_spreads(nw), lib.get_omega()
+    lib.close()
+    p = np.einsum("iwk,jwk->ijk", a, a.conj())
+    pr = np.einsum("iwk,jwk->ijk", z["u_opt"], z["u_opt"].conj())
+    assert np.abs(p - pr).max() <= 1e-8
+    w = np.einsum("bwk,wvk->bvk", a, u)
+    wr = np.einsum("bwk,wvk->bvk", z["u_opt"], z["u_final"])
+    assert np.abs(w - wr).max() <= 1e-5
+    assert np.abs(np.sort(spr) - np.sort(z["spreads"])).max() <= 5e-6
+    assert abs(om[3] - float(z["omega"][3])) <= 2e-8

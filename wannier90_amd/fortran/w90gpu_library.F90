@@ -89,6 +89,7 @@ module w90_library
     ! options (set_option entries or .win data)
     type(w90_settings) :: settings
     logical :: have_win = .false.
+    logical :: setopt_done = .false.  ! input_setopt applied the interface's options
     type(c_ptr) :: ctx = c_null_ptr
   end type lib_common_type
 
@@ -261,22 +262,30 @@ contains
     end if
     common_data%seedname = seedname
     call apply_settings(common_data, ierr, msg)
-    if (ierr > 0) call report(istdout, istderr, ierr, msg, ierr)
+    if (ierr > 0) then
+      call report(istdout, istderr, ierr, msg, ierr)
+      return
+    end if
+    common_data%setopt_done = .true.
     if (comm < -huge(comm)) ierr = ierr  ! single device: the communicator is not used
   end subroutine w90_input_setopt
 
-  !> Read seedname.win (w90_input_reader, src/library_interface.F90:294)
+  !> Read seedname.win (w90_input_reader, src/library_interface.F90:294).
+  !> After input_setopt (which sets the seedname) the options given through
+  !> the interface stay and take precedence: the first occurrence of a
+  !> keyword is the one used, and the .win's entries follow them.
   subroutine w90_input_reader(common_data, istdout, istderr, ierr)
     type(lib_common_type), intent(inout) :: common_data
     integer, intent(in) :: istdout, istderr
     integer, intent(out) :: ierr
     character(len=256) :: msg
     ierr = 0
-    if (common_data%settings%n > 0 .and. .not. common_data%have_win) then
-      call report(istdout, istderr, 1, 'Error: (library use) options interface and .win parsing clash.', ierr)
+    if (common_data%settings%n > 0 .and. .not. common_data%have_win .and. .not. common_data%setopt_done) then
+      call report(istdout, istderr, 1, &
+                  'Error: input reader called when unspent options present (setopt must be called first)', ierr)
       return
     end if
-    call settings_clear(common_data%settings)
+    if (.not. common_data%setopt_done) call settings_clear(common_data%settings)
     call read_win(trim(common_data%seedname)//'.win', common_data%settings, ierr, msg)
     if (ierr > 0) then
       call report(istdout, istderr, ierr, msg, ierr)
