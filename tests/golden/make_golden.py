@@ -3,7 +3,8 @@
 Runs the reference wannier90.x built by oracle/Makefile.ref (oracle/_ref)
 in this container on
   * the reference's own hot-path regression inputs (test-suite/tests/
-    testw90_basic1, testw90_example01, testw90_example02, testw90_example05),
+    testw90_basic1, testw90_example01, testw90_example02, testw90_example05,
+    testw90_write_u_matrices),
   * seeded synthetic tight-binding problems (wannier90_amd.synth) that cover
     the control-flow branches: CG + line search, fixed step, pure steepest
     descent, no convergence window, several lattices and nntot = 6/8/12,
@@ -263,6 +264,8 @@ def golden_dis_case(name, srcdir, seed, mmn_bz2=None):
     cen, spr, om = (None, None, {})
     if ctl["num_iter"] > 0:
         cen, spr, om = w90io.read_wout_final(os.path.join(tmp, seed + ".wout"))
+    # -pp writes the .nnkp (a postproc_setup = F in the .win would override it)
+    open(win, "w").write(set_keyword(text, "postproc_setup", "true"))
     subprocess.run([WANNIER, "-pp", seed], cwd=tmp, check=True, stdout=subprocess.DEVNULL,
                    stderr=subprocess.DEVNULL)
     nn, nnl, nnc = w90io.read_nnkp(os.path.join(tmp, seed + ".nnkp"))
@@ -317,6 +320,8 @@ def main_dis(which=None):
         ("dis_li_window", os.path.join(REF, "testw90_write_u_matrices_disent"), "wannier", None),
         ("dis_gaas", os.path.join(REF, "testw90_gaas_disentanglement_issue192"), "gaas", None),
         ("dis_si_valence", os.path.join(REF, "testw90_example11_1"), "silicon", None),
+        ("dis_basic2", os.path.join(REF, "testw90_basic2"), "wannier", None),
+        ("dis_si_conduction", os.path.join(REF, "testw90_example11_2"), "silicon", None),
     ]
     for name, d, seed, bz in cases:
         if which and name not in which:
@@ -332,6 +337,7 @@ def main(which=None):
         ("ref_example01", os.path.join(REF, "testw90_example01"), "gaas", True, None),
         ("ref_example02", os.path.join(REF, "testw90_example02"), "lead", True, None),
         ("ref_example05", os.path.join(REF, "testw90_example05"), "diamond", True, None),
+        ("ref_write_u_matrices", os.path.join(REF, "testw90_write_u_matrices"), "gaas", True, None),
     ]
     syn = [
         ("cfg1", synth.config(1), {}),

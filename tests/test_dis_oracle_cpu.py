@@ -1,8 +1,9 @@
 """The C restatement of dis_main (oracle/disentangle_oracle.c) pinned
 against the reference wannier90.x on its own disentanglement inputs
 (test-suite testw90_example03 silicon, testw90_example04 copper,
-testw90_gaas_disentanglement_issue192, testw90_example11_1 silicon valence
-with select_projections, testw90_write_u_matrices_disent), whose runs the
+testw90_gaas_disentanglement_issue192, testw90_example11_1 and _11_2
+silicon valence and conduction with select_projections, testw90_basic2,
+testw90_write_u_matrices_disent), whose runs the
 fixtures under
 tests/golden/dis/ hold, and against the mathematics of the extraction.
 

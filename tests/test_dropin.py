@@ -1,7 +1,8 @@
 """The drop-in boundary: the wannier90.x-style driver (wannier90_gpu.x, the
 Fortran host over libw90gpu.so) on the reference's own regression inputs
-(test-suite/tests/testw90_basic1, _example02, _example05; copied as data
-under tests/golden/inputs) and on seeded synthetic inputs.
+(test-suite/tests/testw90_basic1, _example02, _example05,
+_write_u_matrices; copied as data under tests/golden/inputs) and on seeded
+synthetic inputs.
 
 -pp needs no GPU (the k-mesh is host code) and runs everywhere; the full
 wannierisation runs on the GPU and is compared with the reference's
@@ -23,6 +24,7 @@ REF_CASES = {
     "ref_basic1": ("testw90_basic1", "wannier"),
     "ref_example02": ("testw90_example02", "lead"),
     "ref_example05": ("testw90_example05", "diamond"),
+    "ref_write_u_matrices": ("testw90_write_u_matrices", "gaas"),
 }
 
 needs_exe = pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran host not built")
@@ -126,6 +128,26 @@ def test_driver_outputs_and_restart(tmp_path):
     ctx.close()
     assert np.abs(chk_r["u_matrix"] - u).max() < 1e-12
     assert np.abs(chk_r["centres"] - cen).max() < 1e-12
+
+
+@pytest.mark.gpu
+@needs_exe
+def test_driver_writes_u_matrices_like_the_reference(tmp_path):
+    """The reference's write_u_matrices case (GaAs, 4 WFs on a 2x2x2 mesh,
+    write_u_matrices in its .win): _u.mat holds the k-points and the final
+    U of the run, the checkpoint's to the file's 10 decimals, and the
+    reference's final U to the checkpoint test's bound."""
+    from wannier90_amd import w90io
+
+    seed = unpack("ref_write_u_matrices", tmp_path)
+    r = run(seed, tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    u, kp = w90io.read_u_mat(os.path.join(tmp_path, seed + "_u.mat"))
+    chk = w90io.read_chk(os.path.join(tmp_path, seed + ".chk"))
+    c = Case("ref_write_u_matrices")
+    assert np.abs(kp - c.z["kpt_latt"]).max() <= 1e-10
+    assert np.abs(u - chk["u_matrix"]).max() <= 1e-9
+    assert np.abs(u - c.z["u_final"]).max() <= max(1e-7, 20 * c.sens_u)
 
 
 HOST_LIB = os.path.join(ROOT, "wannier90_amd", "lib", "libw90host.so")

@@ -1,7 +1,8 @@
 """Disentanglement on the GPU (w90gpu_disentangle) against the reference's
 runs on its own inputs (tests/golden/dis/: test-suite example03 silicon,
-example04 copper and gaas_disentanglement_issue192 with frozen windows,
-example11_1 silicon valence whose windows hold exactly num_wann states,
+example04 copper, gaas_disentanglement_issue192, example11_2 silicon
+conduction and basic2 with frozen windows, example11_1 silicon valence whose
+windows hold exactly num_wann states,
 write_u_matrices_disent with an outer window only) and against the oracle on
 synthetic problems up to 64 bands and on the benchmark's 512-k problem.
 
