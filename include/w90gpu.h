@@ -306,8 +306,9 @@ int w90gpu_batch_info(w90gpu_batch* batch, size_t* pool_bytes, size_t* used_byte
  * bands of the inner (frozen) window kept, found by the iteration of
  * dis_extract (:1893-2864).  Not covered: gamma_only, site symmetry and
  * dis_spheres (W90GPU_ERR_INPUT).  Limits: num_bands <= W90GPU_DIS_MAX_BANDS
- * (each k's subspace problems run in one workgroup's LDS). */
-#define W90GPU_DIS_MAX_BANDS 64
+ * (each k's subspace problems run in one workgroup: in its LDS up to 64
+ * bands, above that in an area of global memory per workgroup). */
+#define W90GPU_DIS_MAX_BANDS 128
 
 /* dis_control_type and dis_manifold_type (src/wannier90_types.F90:127-136,
  * src/types.F90:179-195): field meaning and defaults as there. */

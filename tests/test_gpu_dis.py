@@ -4,7 +4,8 @@ example04 copper, gaas_disentanglement_issue192, example11_2 silicon
 conduction and basic2 with frozen windows, example11_1 silicon valence whose
 windows hold exactly num_wann states,
 write_u_matrices_disent with an outer window only) and against the oracle on
-synthetic problems up to 64 bands and on the benchmark's 512-k problem.
+synthetic problems up to 128 bands (above 64 the workgroups' matrices are in
+global memory) and on the benchmark's 512-k problem.
 
 Tolerances: Omega_I per iteration 1e-9 relative to the oracle (the DIS
 lines of the reference, printed with 8 decimals, to 6e-9), the final
@@ -132,19 +133,29 @@ def test_wannierise_after_disentangle_matches_reference(ctx, name):
     assert np.abs(w - wr).max() <= tol_w
 
 
-# the last case fills the workgroup LDS of the extraction kernels (64 x 65
-# complex blocks) and gives the Z pass 60-column tiles
-SYN = [("fcc", 20, 12, (4, 4, 4), 4), ("sc", 48, 24, (3, 3, 3), 6), ("bcc", 64, 40, (2, 2, 2), 0),
-       ("hex", 33, 17, (3, 3, 2), 10), ("sc", 64, 60, (2, 2, 2), 0)]
+# (kind, num_bands, num_wann, mesh, frozen, outer): frozen > 0 freezes the
+# bands below the lowest energy of band `frozen`; outer > 0 puts the top of
+# the outer window at the lowest energy of band `outer` (at most `outer`
+# bands in any window).  "sc-64-60" fills the workgroup LDS of the
+# extraction kernels (64 x 65 complex blocks) and gives the Z pass
+# 60-column tiles.  Above 64 bands: "fcc-72-12" has the one-time kernels and
+# the extraction step in global memory and the Z pass (73 x 12 blocks) in
+# LDS; "hex-80-24" (windows of at most 60 bands) only the one-time kernels;
+# "sc-96-48" and "bcc-128-100" every kernel, the last at the largest size.
+SYN = [("fcc", 20, 12, (4, 4, 4), 4, 0), ("sc", 48, 24, (3, 3, 3), 6, 0), ("bcc", 64, 40, (2, 2, 2), 0, 0),
+       ("hex", 33, 17, (3, 3, 2), 10, 0), ("sc", 64, 60, (2, 2, 2), 0, 0), ("fcc", 72, 12, (2, 2, 2), 0, 0),
+       ("hex", 80, 24, (2, 2, 2), 4, 60), ("sc", 96, 48, (2, 2, 2), 8, 0), ("bcc", 128, 100, (2, 2, 2), 0, 0)]
 
 
-@pytest.mark.parametrize("case", SYN, ids=lambda c: f"{c[0]}-nb{c[1]}-nw{c[2]}-froz{c[4]}")
+@pytest.mark.parametrize("case", SYN, ids=lambda c: f"{c[0]}-nb{c[1]}-nw{c[2]}-froz{c[4]}-outer{c[5]}")
 def test_disentangle_matches_oracle_synthetic(ctx, case):
     from wannier90_amd import synth
 
-    kind, nb, nw, mp, fr = case
+    kind, nb, nw, mp, fr, outer = case
     print(f"[dis synthetic] {case}", flush=True)
     p, ctl = synth.make_dis_problem(7 + nb, kind=kind, num_bands=nb, num_wann=nw, mp=mp, frozen=fr)
+    if outer > 0:
+        ctl["win_max"] = float(p.evals[:, outer - 1].min())
     ctl["num_iter"] = 60
     o = po.dis_main(nb, nw, p.kmesh, ctl, p.eig, p.amat_dis, p.m_orig)
     g = ctx.disentangle(p.kmesh, nb, ctl, p.eig, p.amat_dis, p.m_orig)

@@ -28,11 +28,14 @@
 //   k_dis_rotate      one workgroup per (k, b): M(k,b) <- W(k)^+ M(k,b)
 //                     W(k+b), the dis_main rotation (:229-241) and
 //                     setup_m_loc in one product.
-// Hermitian eigenproblems (of Z, Q P Q and the subspace Hamiltonian, n <=
-// 64) are solved in LDS by a two-sided cyclic Jacobi method with round-robin
-// (tournament) ordering: n/2 disjoint complex rotations per round, applied
-// to columns and then to rows in two parallel passes; the polar factors (of
-// A and of U_opt^+ A) by the one-sided Jacobi SVD with the same ordering.  The
+// Hermitian eigenproblems (of Z, Q P Q and the subspace Hamiltonian) are
+// solved in the workgroup by a two-sided cyclic Jacobi method with
+// round-robin (tournament) ordering: n/2 disjoint complex rotations per
+// round, applied to A (by 2 x 2 blocks) and to the basis in one parallel
+// pass; the polar factors (of A and of U_opt^+ A) by the one-sided Jacobi
+// SVD with the same ordering.  A workgroup's matrices live in LDS up to 64
+// bands (two 64 x 65 complex blocks) and in its own area of global memory
+// above that, up to 128 bands (the same code, instantiated for each).  The
 // extraction only uses the spanned subspaces and the eigenvalue sums, which
 // do not depend on the eigenvector phases; the final U_opt and U do, so they
 // agree with the reference up to the phase of each Hamiltonian eigenvector
@@ -78,6 +81,14 @@ __device__ __forceinline__ void zcfma(zd& a, zd b, zd c) {
 
 constexpr int kNT = 256;             // threads per workgroup
 constexpr int kMaxN = W90GPU_DIS_MAX_BANDS;
+// Largest matrix order of the LDS kernels: two kLdsN x (kLdsN + 1) complex
+// blocks fill 133 KB of a workgroup's 160 KB.  Above it the same kernels
+// run with their blocks in a per-workgroup area of global memory
+// (instantiated with N = kMaxN), launched in chunks of kGwsBlocks
+// workgroups that share those areas.
+constexpr int kLdsN = 64;
+constexpr size_t kLdsBytes = 2 * (size_t)kLdsN * (kLdsN + 1) * sizeof(double2);
+constexpr int kGwsBlocks = 1024;
 
 // Window of one k: first band (0-based), size, frozen count and the
 // window row of the first frozen band (the frozen bands are contiguous).
@@ -108,7 +119,7 @@ enum : int {
 };
 
 struct DisProb {
-  int nb, nw, nk, nn, ld;  // ld: leading dimension of the LDS matrices (nb + 1)
+  int nb, nw, nk, nn, ld;  // ld: leading dimension of the workgroup matrices (nb + 1)
   int ntri;                // nb (nb + 1) / 2
   int ldc;                 // k_dis_zmat: LDS leading dimension (odd, >= every ndim)
   int nzmax, ldz;          // k_dis_update: largest ndim - nfroz, leading dimension (odd)
@@ -132,7 +143,24 @@ struct DisProb {
   double* log;         // (3, num_iter)
   DisState* st;
   int* flags;
+  zd* gws;            // global-memory kernels: gws_stride complex per workgroup
+  size_t gws_stride;
+  int blk0;           // global-memory kernels: first block of this launch
 };
+
+// The workgroup's matrix area (LDS, or its slice of gws above kLdsN) and
+// its block index (the launch's offset added above kLdsN).
+extern __shared__ zd dis_lds[];
+template <int N>
+__device__ __forceinline__ zd* dis_ws(const DisProb& p) {
+  if constexpr (N > kLdsN) return p.gws + (size_t)blockIdx.x * p.gws_stride;
+  else return dis_lds;
+}
+template <int N>
+__device__ __forceinline__ int dis_blk(const DisProb& p) {
+  if constexpr (N > kLdsN) return (int)blockIdx.x + p.blk0;
+  else return (int)blockIdx.x;
+}
 
 // window row of the i-th non-frozen state
 __device__ __forceinline__ int nf_row(const Win& w, int i) { return i < w.fmin ? i : i + w.nfroz; }
@@ -160,13 +188,15 @@ struct JRot {
   zd s;
 };
 
+template <int N>
 struct JacobiLds {
-  JRot rot[kMaxN / 2];
+  static constexpr int kN = N;
+  JRot rot[N / 2];
   int count;
   double red[kNT / 64];
-  double lam[kMaxN];
-  int perm[kMaxN];
-  int rank[kMaxN];
+  double lam[N];
+  int perm[N];
+  int rank[N];
 };
 
 // Position (r, c), r <= c, of entry x = c (c + 1) / 2 + r of a packed upper
@@ -196,8 +226,13 @@ __device__ __forceinline__ void rr_pair(int s, int r, int m, int& p, int& q) {
   q = max(a, b);
 }
 
-// rows of E per lane in a rotation pass (n <= 64, at least 8 lanes a slot)
-constexpr int kRowsPerLane = (kMaxN + kNT / (kMaxN / 2) - 1) / (kNT / (kMaxN / 2));
+// rows of E per lane and chunk in a rotation pass; n <= 64 takes one chunk
+// (at least 8 lanes a slot), n <= 128 up to four
+constexpr int kRowsPerLane = 8;
+template <int N>
+constexpr int row_chunks() {
+  return (N + (kNT / (N / 2)) * kRowsPerLane - 1) / ((kNT / (N / 2)) * kRowsPerLane);
+}
 
 // The rotation of pair (p, q) is skipped when |a_pq| <= 1e-15 sqrt(|a_pp
 // a_qq|) (relative to the diagonal) or <= 1e-16 ||A||_F (without this floor
@@ -261,7 +296,9 @@ __device__ __forceinline__ JRot jacobi_rotation(int p, int q, const zd* A, int n
 // off-diagonal entries just below the skip bounds, so they are kept tight:
 // the selected subspace moves by their size over the eigenvalue gap at the
 // selection.
-__device__ __forceinline__ int jacobi_run(zd* A, zd* E, int n, int ld, JacobiLds& J) {
+template <class JL>
+__device__ __forceinline__ int jacobi_run(zd* A, zd* E, int n, int ld, JL& J) {
+  constexpr int CH = row_chunks<JL::kN>();
   const int t = threadIdx.x;
   double fro = 0.0;
   for (int e = t; e < n * n; e += kNT) fro += zabs2(A[(e % n) + (e / n) * ld]);
@@ -334,24 +371,28 @@ __device__ __forceinline__ int jacobi_run(zd* A, zd* E, int n, int ld, JacobiLds
         int p, q;
         rr_pair(es, r, m, p, q);
         if (q < n) {
-          zd ep[kRowsPerLane], eq[kRowsPerLane];
 #pragma unroll
-          for (int j = 0; j < kRowsPerLane; ++j) {
-            const int row = eg + j * g;
-            if (row < n) {
-              ep[j] = E[row + p * ld];
-              eq[j] = E[row + q * ld];
-            }
-          }
-          const double c = J.rot[es].c;
-          const zd sv = J.rot[es].s, sc = zconj(sv);
-          if (sv.x != 0.0 || sv.y != 0.0) {
+          for (int ch = 0; ch < CH; ++ch) {
+            const int row0 = eg + ch * kRowsPerLane * g;
+            zd ep[kRowsPerLane], eq[kRowsPerLane];
 #pragma unroll
             for (int j = 0; j < kRowsPerLane; ++j) {
-              const int row = eg + j * g;
+              const int row = row0 + j * g;
               if (row < n) {
-                E[row + p * ld] = zsub(zscale(ep[j], c), zmul(sc, eq[j]));
-                E[row + q * ld] = zadd(zmul(sv, ep[j]), zscale(eq[j], c));
+                ep[j] = E[row + p * ld];
+                eq[j] = E[row + q * ld];
+              }
+            }
+            const double c = J.rot[es].c;
+            const zd sv = J.rot[es].s, sc = zconj(sv);
+            if (sv.x != 0.0 || sv.y != 0.0) {
+#pragma unroll
+              for (int j = 0; j < kRowsPerLane; ++j) {
+                const int row = row0 + j * g;
+                if (row < n) {
+                  E[row + p * ld] = zsub(zscale(ep[j], c), zmul(sc, eq[j]));
+                  E[row + q * ld] = zadd(zmul(sv, ep[j]), zscale(eq[j], c));
+                }
               }
             }
           }
@@ -364,7 +405,8 @@ __device__ __forceinline__ int jacobi_run(zd* A, zd* E, int n, int ld, JacobiLds
 }
 
 // jacobi_run from E = I into V (the eigenvectors of A itself).
-__device__ __forceinline__ int jacobi_eigh(zd* A, zd* V, int n, int ld, JacobiLds& J) {
+template <class JL>
+__device__ __forceinline__ int jacobi_eigh(zd* A, zd* V, int n, int ld, JL& J) {
   for (int e = threadIdx.x; e < n * n; e += kNT) {
     const int r = e % n, c = e / n;
     V[r + c * ld] = zmk(r == c ? 1.0 : 0.0, 0.0);
@@ -417,8 +459,6 @@ __device__ void zgemm_tiles(int M, int N, int K, const zd* A, int lda, const zd*
   }
 }
 
-// zgemm_tiles with the smallest tile that needs at most one pass (M, N <=
-// 64 always fit 4 x 4 tiles).
 // zgemm with tiles of at most 2 x 3 (several passes for large results):
 // the register budget of k_dis_zmat, whose (k, b) workgroups wait on
 // memory and gain from more of them per CU; UNR K steps unrolled (four
@@ -429,6 +469,8 @@ __device__ void zgemm_small(int M, int N, int K, const zd* A, int lda, const zd*
   else zgemm_tiles<2, 3, ADJ, UNR>(M, N, K, A, lda, B, ldb, out);
 }
 
+// zgemm_tiles with the smallest tile that needs at most one pass (M, N <=
+// 64 fit 4 x 4 tiles; larger results take several passes of them).
 template <bool ADJ, class Out>
 __device__ void zgemm(int M, int N, int K, const zd* A, int lda, const zd* B, int ldb, Out out) {
   auto fits = [&](int r, int c) { return ((M + r - 1) / r) * ((N + c - 1) / c) <= kNT; };
@@ -493,7 +535,8 @@ __device__ void zgemm_herm(int N, int K, const zd* A, int lda, const zd* B, int 
 
 // Eigenvalues (the diagonal of A after jacobi_run) into J.lam, their
 // ascending order into J.perm (ties by index) and its inverse into J.rank.
-__device__ __forceinline__ void eig_order(const zd* A, int n, int ld, JacobiLds& J) {
+template <class JL>
+__device__ __forceinline__ void eig_order(const zd* A, int n, int ld, JL& J) {
   const int t = threadIdx.x;
   if (t < n) J.lam[t] = A[t + t * ld].x;
   __syncthreads();
@@ -518,8 +561,9 @@ __device__ __forceinline__ void eig_order(const zd* A, int n, int ld, JacobiLds&
 // X, where the polar factor is not unique, as with the SVD) are completed
 // by Gram-Schmidt from unit vectors.  U goes to global memory (leading
 // dimension ldu).  LDS: G (rows x nw, leading dimension ld), V.
-__device__ __forceinline__ void polar_factor(const zd* X, int ldx, int rows, int nw, zd* U, int ldu, zd* G, zd* V, int ld,
-                             JacobiLds& J, int* flags) {
+template <class JL>
+__device__ __forceinline__ void polar_factor(const zd* X, int ldx, int rows, int nw, zd* U, int ldu, zd* G, zd* V,
+                                             int ld, JL& J, int* flags) {
   const int t = threadIdx.x;
   double fro = 0.0;
   for (int e = t; e < rows * nw; e += kNT) {
@@ -542,48 +586,50 @@ __device__ __forceinline__ void polar_factor(const zd* X, int ldx, int rows, int
     if (t == 0) J.count = 0;
     __syncthreads();
     for (int r = 0; r < m - 1; ++r) {
-      // Gram entries of each pair, 8 lanes per pair
-      const int pi = t >> 3, lane = t & 7;
-      double al = 0.0, be = 0.0;
-      zd ga = zmk(0.0, 0.0);
-      int P = 0, Q = 0;
-      bool valid = false;
-      if (pi < half) {
-        rr_pair(pi, r, m, P, Q);
-        valid = Q < nw;
-        if (valid)
-          for (int row = lane; row < rows; row += 8) {
-            const zd gp = G[row + P * ld], gq = G[row + Q * ld];
-            al += zabs2(gp);
-            be += zabs2(gq);
-            zcfma(ga, gp, gq);
-          }
-      }
-      for (int off = 1; off < 8; off <<= 1) {
-        al += __shfl_xor(al, off, 64);
-        be += __shfl_xor(be, off, 64);
-        ga.x += __shfl_xor(ga.x, off, 64);
-        ga.y += __shfl_xor(ga.y, off, 64);
-      }
-      if (pi < half && lane == 0) {
-        JRot R;
-        R.p = P;
-        R.q = Q;
-        R.c = 1.0;
-        R.s = zmk(0.0, 0.0);
-        const double b2 = zabs2(ga);
-        if (valid && b2 > 0.0 && b2 > 1e-26 * al * be && fmin(al, be) > zero2) {
-          const double bn = sqrt(b2);
-          const double tau = (be - al) / (2.0 * bn);
-          double tt;
-          if (fabs(tau) > 1e150) tt = 0.5 / tau;
-          else tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-          const double c = 1.0 / sqrt(1.0 + tt * tt), s = tt * c;
-          R.c = c;
-          R.s = zmk(s * ga.x / bn, s * ga.y / bn);
-          atomicAdd(&J.count, 1);
+      // Gram entries of each pair, 8 lanes per pair (kNT / 8 pairs a pass)
+      for (int pb = 0; pb < half; pb += kNT / 8) {
+        const int pi = pb + (t >> 3), lane = t & 7;
+        double al = 0.0, be = 0.0;
+        zd ga = zmk(0.0, 0.0);
+        int P = 0, Q = 0;
+        bool valid = false;
+        if (pi < half) {
+          rr_pair(pi, r, m, P, Q);
+          valid = Q < nw;
+          if (valid)
+            for (int row = lane; row < rows; row += 8) {
+              const zd gp = G[row + P * ld], gq = G[row + Q * ld];
+              al += zabs2(gp);
+              be += zabs2(gq);
+              zcfma(ga, gp, gq);
+            }
         }
-        J.rot[pi] = R;
+        for (int off = 1; off < 8; off <<= 1) {
+          al += __shfl_xor(al, off, 64);
+          be += __shfl_xor(be, off, 64);
+          ga.x += __shfl_xor(ga.x, off, 64);
+          ga.y += __shfl_xor(ga.y, off, 64);
+        }
+        if (pi < half && lane == 0) {
+          JRot R;
+          R.p = P;
+          R.q = Q;
+          R.c = 1.0;
+          R.s = zmk(0.0, 0.0);
+          const double b2 = zabs2(ga);
+          if (valid && b2 > 0.0 && b2 > 1e-26 * al * be && fmin(al, be) > zero2) {
+            const double bn = sqrt(b2);
+            const double tau = (be - al) / (2.0 * bn);
+            double tt;
+            if (fabs(tau) > 1e150) tt = 0.5 / tau;
+            else tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            const double c = 1.0 / sqrt(1.0 + tt * tt), s = tt * c;
+            R.c = c;
+            R.s = zmk(s * ga.x / bn, s * ga.y / bn);
+            atomicAdd(&J.count, 1);
+          }
+          J.rot[pi] = R;
+        }
       }
       __syncthreads();
       for (int e = t; e < half * (rows + nw); e += kNT) {
@@ -652,18 +698,21 @@ __device__ __forceinline__ void polar_factor(const zd* X, int ldx, int rows, int
 }
 
 // ------------------------------------------------------------ kernels
-
-extern __shared__ zd dis_lds[];
+//
+// Each kernel with matrices per workgroup is a template on the largest
+// matrix order N: N = kLdsN keeps them in LDS, N = kMaxN in the
+// workgroup's area of global memory (dis_ws, dis_blk).
 
 // dis_project: U_opt(k) = polar factor of A(window rows, :); its unitarity
 // checked to 1e-5; rows below the window zero.
+template <int N>
 __global__ void __launch_bounds__(kNT) k_dis_project(DisProb p) {
-  __shared__ JacobiLds J;
-  const int k = blockIdx.x;
+  __shared__ JacobiLds<N> J;
+  const int k = dis_blk<N>(p);
   const Win w = p.win[k];
   const int nb = p.nb, nw = p.nw, ld = p.ld;
-  zd* S = dis_lds;
-  zd* V = dis_lds + nb * ld;
+  zd* S = dis_ws<N>(p);
+  zd* V = S + nb * ld;
   zd* U = p.Uopt + (size_t)k * nb * nw;
   const zd* X = p.A + (size_t)k * nb * nw + w.first;
   polar_factor(X, nb, w.ndim, nw, U, nb, S, V, ld, J, p.flags);
@@ -687,15 +736,16 @@ __global__ void __launch_bounds__(kNT) k_dis_project(DisProb p) {
 // order of eigenvalue; when some of those eigenvalues are below 1e-8 the
 // reference's ortho-fix picks, from the top down, eigenvectors orthogonal
 // to the frozen columns.
+template <int N>
 __global__ void __launch_bounds__(kNT) k_dis_proj_froz(DisProb p) {
-  __shared__ JacobiLds J;
-  __shared__ int vmap[kMaxN];
+  __shared__ JacobiLds<N> J;
+  __shared__ int vmap[N];
   __shared__ int fix_ok;
-  const int k = blockIdx.x;
+  const int k = dis_blk<N>(p);
   const Win w = p.win[k];
   const int nb = p.nb, nw = p.nw, ld = p.ld, nd = w.ndim, t = threadIdx.x;
-  zd* Aq = dis_lds;
-  zd* V = dis_lds + nb * ld;
+  zd* Aq = dis_ws<N>(p);
+  zd* V = Aq + nb * ld;
   zd* U = p.Uopt + (size_t)k * nb * nw;
   const int r = nw - w.nfroz;
   auto frozen = [&](int i) { return i >= w.fmin && i < w.fmin + w.nfroz; };
@@ -727,7 +777,7 @@ __global__ void __launch_bounds__(kNT) k_dis_proj_froz(DisProb p) {
       } else {
         // ortho-fix (:1780-1850): the good eigenvectors from the top, then
         // eigenvectors orthogonal to the frozen columns, searched from the top
-        int used[kMaxN];
+        int used[N];
         for (int j = 0; j < nd; ++j) used[j] = 0;
         int cnt = 0;
         for (int j = nd - 1; j >= nd - goods; --j) {
@@ -844,17 +894,19 @@ __device__ void zpart_tiles(const zd* C, int ldc, const Win& w, int nd, int nw, 
   }
 }
 
+template <int N>
 __global__ void __launch_bounds__(kNT) k_dis_zmat(DisProb p) {
   __shared__ double red[2][kNT / 64];
   const DisState* st = p.st;
   if (st->done) return;
   const int nn = p.nn;
-  const int k = blockIdx.x / nn, b = blockIdx.x % nn, t = threadIdx.x;
+  const int blk = dis_blk<N>(p);
+  const int k = blk / nn, b = blk % nn, t = threadIdx.x;
   const Win w = p.win[k];
   const int nb = p.nb, nw = p.nw, ldc = p.ldc;
   const int nd = w.ndim - w.nfroz;
-  zd* C = dis_lds;              // ndim(k) x nw
-  zd* U2 = dis_lds + ldc * nw;  // ndim(k+b) x nw, then ndim(k) x nw
+  zd* C = dis_ws<N>(p);    // ndim(k) x nw
+  zd* U2 = C + ldc * nw;   // ndim(k+b) x nw, then ndim(k) x nw
   const zd* Uk = p.Uopt + (size_t)k * nb * nw;
   const int k2 = p.nnl[k * nn + b];
   const Win w2 = p.win[k2];
@@ -917,11 +969,12 @@ __global__ void __launch_bounds__(kNT) k_dis_zmat(DisProb p) {
 // rotations orthonormal to rounding (without it the basis drifts by ~n eps
 // per iteration).  LDS: A and E (nd x nd, leading dimension ldz); Z E
 // passes through the Vprev area of global memory.
+template <int N>
 __global__ void __launch_bounds__(kNT) k_dis_update(DisProb p) {
-  __shared__ JacobiLds J;
+  __shared__ JacobiLds<N> J;
   DisState* st = p.st;
   if (st->done) return;
-  const int k = blockIdx.x, t = threadIdx.x;
+  const int k = dis_blk<N>(p), t = threadIdx.x;
   const Win w = p.win[k];
   const int nb = p.nb, nw = p.nw, ld = p.ldz, nn = p.nn;
   const int nd = w.ndim - w.nfroz, r = nw - w.nfroz;
@@ -936,8 +989,8 @@ __global__ void __launch_bounds__(kNT) k_dis_update(DisProb p) {
   const double mix = st->mix;
   zd* Zi = p.Zin + (size_t)k * nb * nb;
   zd* Vp = p.Vprev + (size_t)k * nb * nb;
-  zd* A = dis_lds;
-  zd* E = dis_lds + ld * p.nzmax;
+  zd* A = dis_ws<N>(p);
+  zd* E = A + ld * p.nzmax;
   const zd* Zp = p.Zpart + (size_t)k * nn * p.ntri;
   for (int e = t; e < nd * nd; e += kNT) {
     const int i = e % nd, j = e / nd;
@@ -1037,15 +1090,16 @@ __global__ void __launch_bounds__(kNT) k_dis_reduce(DisProb p) {
 
 // The Hamiltonian in the optimal subspace, H = U_opt^+ diag(e) U_opt, its
 // eigenvalues into eigopt(1:nw) and U_opt <- U_opt V (ascending order).
+template <int N>
 __global__ void __launch_bounds__(kNT) k_dis_ham(DisProb p) {
-  __shared__ JacobiLds J;
-  const int k = blockIdx.x, t = threadIdx.x;
+  __shared__ JacobiLds<N> J;
+  const int k = dis_blk<N>(p), t = threadIdx.x;
   const Win w = p.win[k];
   const int nb = p.nb, nw = p.nw, ld = p.ld;
   zd* U = p.Uopt + (size_t)k * nb * nw;
   const double* e = p.eig + (size_t)k * nb + w.first;
-  zd* H = dis_lds;
-  zd* V = dis_lds + nb * ld;
+  zd* H = dis_ws<N>(p);
+  zd* V = H + nb * ld;
   for (int q = t; q < nw * nw; q += kNT) {
     const int i = q % nw, j = q / nw;
     if (i > j) continue;
@@ -1076,9 +1130,10 @@ __global__ void __launch_bounds__(kNT) k_dis_ham(DisProb p) {
 
 // internal_find_u: U(k) = polar factor of X = U_opt(k)^+ A(k) (X staged in
 // the Zin area, free after the extraction), then W(k) = U_opt(k) U(k).
+template <int N>
 __global__ void __launch_bounds__(kNT) k_dis_find_u(DisProb p) {
-  __shared__ JacobiLds J;
-  const int k = blockIdx.x, t = threadIdx.x;
+  __shared__ JacobiLds<N> J;
+  const int k = dis_blk<N>(p), t = threadIdx.x;
   const Win w = p.win[k];
   const int nb = p.nb, nw = p.nw, ld = p.ld;
   const zd* Uo = p.Uopt + (size_t)k * nb * nw;
@@ -1093,7 +1148,8 @@ __global__ void __launch_bounds__(kNT) k_dis_find_u(DisProb p) {
   __threadfence_block();
   __syncthreads();
   zd* U = p.U + (size_t)k * nw * nw;
-  polar_factor(X, nw, nw, nw, U, nw, dis_lds, dis_lds + nb * ld, ld, J, p.flags);
+  zd* G = dis_ws<N>(p);
+  polar_factor(X, nw, nw, nw, U, nw, G, G + nb * ld, ld, J, p.flags);
   __threadfence_block();
   __syncthreads();
   zd* W = p.W + (size_t)k * nb * nw;
@@ -1107,14 +1163,16 @@ __global__ void __launch_bounds__(kNT) k_dis_find_u(DisProb p) {
 
 // M(k,b) <- W(k)^+ M(k,b) W(k+b) (num_wann x num_wann), one workgroup per
 // (k, b).
+template <int N>
 __global__ void __launch_bounds__(kNT) k_dis_rotate(DisProb p) {
-  const int k = blockIdx.x / p.nn, b = blockIdx.x % p.nn, t = threadIdx.x;
+  const int blk = dis_blk<N>(p);
+  const int k = blk / p.nn, b = blk % p.nn, t = threadIdx.x;
   const int nb = p.nb, nw = p.nw, ld = p.ld, nn = p.nn;
   const Win w = p.win[k];
   const int k2 = p.nnl[k * nn + b];
   const Win w2 = p.win[k2];
-  zd* P = dis_lds;            // ndim(k) x nw
-  zd* W2 = dis_lds + nb * ld; // ndim(k+b) x nw
+  zd* P = dis_ws<N>(p);  // ndim(k) x nw
+  zd* W2 = P + nb * ld;   // ndim(k+b) x nw
   const zd* Wk = p.W + (size_t)k * nb * nw;
   const zd* Wkb = p.W + (size_t)k2 * nb * nw;
   for (int e = t; e < w2.ndim * nw; e += kNT) {
@@ -1235,14 +1293,44 @@ int set_lds_limits(w90gpu_ctx* ctx, size_t bytes) {
   static bool done = false;
   std::lock_guard<std::mutex> lock(mu);
   if (done) return W90GPU_OK;
-  const void* ks[] = {(const void*)k_dis_project, (const void*)k_dis_proj_froz, (const void*)k_dis_zmat,
-                      (const void*)k_dis_update,  (const void*)k_dis_ham,       (const void*)k_dis_find_u,
-                      (const void*)k_dis_rotate};
+  const void* ks[] = {(const void*)k_dis_project<kLdsN>, (const void*)k_dis_proj_froz<kLdsN>,
+                      (const void*)k_dis_zmat<kLdsN>,    (const void*)k_dis_update<kLdsN>,
+                      (const void*)k_dis_ham<kLdsN>,     (const void*)k_dis_find_u<kLdsN>,
+                      (const void*)k_dis_rotate<kLdsN>};
   for (const void* f : ks)
     DCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
   done = true;
   return W90GPU_OK;
 }
+
+// A kernel over nblocks workgroups: the LDS instantiation in one launch,
+// the global-memory one in launches of at most kGwsBlocks workgroups (each
+// launch reuses the workgroup areas of gws; the stream orders them).
+template <int N>
+void launch_blocks(void (*kern)(DisProb), int nblocks, size_t lds, hipStream_t sm, DisProb p) {
+  if constexpr (N <= kLdsN) {
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kNT), lds, sm, p);
+  } else {
+    for (int b0 = 0; b0 < nblocks; b0 += kGwsBlocks) {
+      p.blk0 = b0;
+      hipLaunchKernelGGL(kern, dim3(std::min(kGwsBlocks, nblocks - b0)), dim3(kNT), 0, sm, p);
+    }
+  }
+}
+
+// The instantiation that holds a kernel's matrices: LDS when they fit.
+template <template <int> class K>
+void launch_sized(bool global, int nblocks, size_t lds, hipStream_t sm, const DisProb& p) {
+  if (global) launch_blocks<kMaxN>(K<kMaxN>::fn, nblocks, 0, sm, p);
+  else launch_blocks<kLdsN>(K<kLdsN>::fn, nblocks, lds, sm, p);
+}
+template <int N> struct KProject { static constexpr void (*fn)(DisProb) = k_dis_project<N>; };
+template <int N> struct KProjFroz { static constexpr void (*fn)(DisProb) = k_dis_proj_froz<N>; };
+template <int N> struct KZmat { static constexpr void (*fn)(DisProb) = k_dis_zmat<N>; };
+template <int N> struct KUpdate { static constexpr void (*fn)(DisProb) = k_dis_update<N>; };
+template <int N> struct KHam { static constexpr void (*fn)(DisProb) = k_dis_ham<N>; };
+template <int N> struct KFindU { static constexpr void (*fn)(DisProb) = k_dis_find_u<N>; };
+template <int N> struct KRotate { static constexpr void (*fn)(DisProb) = k_dis_rotate<N>; };
 
 }  // namespace
 
@@ -1302,7 +1390,14 @@ int w90gpu_disentangle(w90gpu_ctx* ctx, const w90gpu_problem* prob, int num_band
   }
   const int ldc = ndmax | 1, ldz = nzmax | 1;
   const size_t lds_zmat = 2 * (size_t)ldc * nw * sizeof(zd), lds_update = 2 * (size_t)ldz * nzmax * sizeof(zd);
-  if ((rc = set_lds_limits(ctx, 2 * (size_t)kMaxN * (kMaxN + 1) * sizeof(zd)))) return rc;
+  if ((rc = set_lds_limits(ctx, kLdsBytes))) return rc;
+  // matrices beyond the LDS go to per-workgroup areas of global memory:
+  // those of the one-time kernels (two nb x (nb + 1) blocks) above kLdsN
+  // bands, those of the extraction kernels when their window blocks do not
+  // fit
+  const bool g_once = nb > kLdsN, g_zmat = lds_zmat > kLdsBytes, g_update = nzmax > kLdsN;
+  const size_t gws_stride = 2 * (size_t)nb * ld;
+  const size_t gws_blocks = (g_once || g_zmat || g_update) ? std::min<size_t>(kGwsBlocks, (size_t)nk * nn) : 0;
 
   const size_t zb = sizeof(zd);
   const int nlog = std::max(ctl->num_iter, 1);
@@ -1316,7 +1411,8 @@ int w90gpu_disentangle(w90gpu_ctx* ctx, const w90gpu_problem* prob, int num_band
                o_Mw = ar.add(zb * nw * nw * nn * nk), o_wk1 = ar.add(sizeof(double) * nk),
                o_part = ar.add(sizeof(double) * 2 * nn * nk),
                o_eigopt = ar.add(sizeof(double) * nb * nk), o_log = ar.add(sizeof(double) * 3 * nlog),
-               o_st = ar.add(sizeof(DisState)), o_flags = ar.add(sizeof(int));
+               o_st = ar.add(sizeof(DisState)), o_flags = ar.add(sizeof(int)),
+               o_gws = ar.add(zb * gws_stride * gws_blocks);
   DevBuf buf;
   DCHK(hipMalloc(&buf.p, ar.total));
   char* base = (char*)buf.p;
@@ -1369,6 +1465,9 @@ int w90gpu_disentangle(w90gpu_ctx* ctx, const w90gpu_problem* prob, int num_band
   p.log = (double*)(base + o_log);
   p.st = (DisState*)(base + o_st);
   p.flags = (int*)(base + o_flags);
+  p.gws = (zd*)(base + o_gws);
+  p.gws_stride = gws_stride;
+  p.blk0 = 0;
 
   DisState s0;
   std::memset(&s0, 0, sizeof(s0));
@@ -1379,8 +1478,8 @@ int w90gpu_disentangle(w90gpu_ctx* ctx, const w90gpu_problem* prob, int num_band
   s0.done = ctl->num_iter == 0 ? 1 : 0;
   hipStream_t sm = nullptr;
   hipLaunchKernelGGL(k_dis_init_state, dim3(1), dim3(1), 0, sm, p.st, s0);
-  hipLaunchKernelGGL(k_dis_project, dim3(nk), dim3(kNT), lds, sm, p);
-  if (linner) hipLaunchKernelGGL(k_dis_proj_froz, dim3(nk), dim3(kNT), lds, sm, p);
+  launch_sized<KProject>(g_once, nk, lds, sm, p);
+  if (linner) launch_sized<KProjFroz>(g_once, nk, lds, sm, p);
   hipLaunchKernelGGL(k_dis_orthonorm, dim3(nk), dim3(kNT), 0, sm, p);
   DCHK(hipGetLastError());
   int flags = 0;
@@ -1405,14 +1504,14 @@ int w90gpu_disentangle(w90gpu_ctx* ctx, const w90gpu_problem* prob, int num_band
   // the extraction: Z of the initial U_opt, then (update, Z, reduce) per
   // iteration, polled every kPoll iterations
   constexpr int kPoll = 16;
-  hipLaunchKernelGGL(k_dis_zmat, dim3(nk * nn), dim3(kNT), lds_zmat, sm, p);
+  launch_sized<KZmat>(g_zmat, nk * nn, lds_zmat, sm, p);
   DisState st;
   int launched = 0;
   while (launched < ctl->num_iter) {
     const int n = std::min(kPoll, ctl->num_iter - launched);
     for (int i = 0; i < n; ++i) {
-      hipLaunchKernelGGL(k_dis_update, dim3(nk), dim3(kNT), lds_update, sm, p);
-      hipLaunchKernelGGL(k_dis_zmat, dim3(nk * nn), dim3(kNT), lds_zmat, sm, p);
+      launch_sized<KUpdate>(g_update, nk, lds_update, sm, p);
+      launch_sized<KZmat>(g_zmat, nk * nn, lds_zmat, sm, p);
       hipLaunchKernelGGL(k_dis_reduce, dim3(1), dim3(kNT), 0, sm, p);
     }
     DCHK(hipGetLastError());
@@ -1420,9 +1519,9 @@ int w90gpu_disentangle(w90gpu_ctx* ctx, const w90gpu_problem* prob, int num_band
     DCHK(hipMemcpy(&st, p.st, sizeof(DisState), hipMemcpyDeviceToHost));
     if (st.done) break;
   }
-  hipLaunchKernelGGL(k_dis_ham, dim3(nk), dim3(kNT), lds, sm, p);
-  hipLaunchKernelGGL(k_dis_find_u, dim3(nk), dim3(kNT), lds, sm, p);
-  hipLaunchKernelGGL(k_dis_rotate, dim3(nk * nn), dim3(kNT), lds, sm, p);
+  launch_sized<KHam>(g_once, nk, lds, sm, p);
+  launch_sized<KFindU>(g_once, nk, lds, sm, p);
+  launch_sized<KRotate>(g_once, nk * nn, lds, sm, p);
   DCHK(hipGetLastError());
   DCHK(hipMemcpy(&st, p.st, sizeof(DisState), hipMemcpyDeviceToHost));
   DCHK(hipMemcpy(&flags, p.flags, sizeof(int), hipMemcpyDeviceToHost));
