@@ -35,7 +35,7 @@ extern "C" {
 #define W90GPU_WARN (-2)
 
 #define W90GPU_NUM_NNMAX 12      /* types.F90:116 num_nnmax */
-#define W90GPU_MAX_WANN 128      /* largest num_wann the kernels accept */
+#define W90GPU_MAX_WANN 256      /* largest num_wann the kernels accept */
 
 /* arithmetic of the rotation / gradient pass */
 #define W90GPU_PREC_FP64 0       /* plain drop-in: everything in FP64 */
@@ -308,7 +308,7 @@ int w90gpu_batch_info(w90gpu_batch* batch, size_t* pool_bytes, size_t* used_byte
  * dis_spheres (W90GPU_ERR_INPUT).  Limits: num_bands <= W90GPU_DIS_MAX_BANDS
  * (each k's subspace problems run in one workgroup: in its LDS up to 64
  * bands, above that in an area of global memory per workgroup). */
-#define W90GPU_DIS_MAX_BANDS 128
+#define W90GPU_DIS_MAX_BANDS 256
 
 /* dis_control_type and dis_manifold_type (src/wannier90_types.F90:127-136,
  * src/types.F90:179-195): field meaning and defaults as there. */
@@ -362,7 +362,7 @@ int w90gpu_disentangle(w90gpu_ctx* ctx, const w90gpu_problem* prob, int num_band
  * No counterpart in the reference.  The M rotation has several kernels, by
  * tile shape, per num_wann class: "w16" (up to 16 bands), "fused32" or
  * "fused64" (17-32), "fused64" or "direct64" (33-64), "gemm64" or "gemm32"
- * (65-128).  A problem's kernel is fixed by (num_wann, nntot) when it is
+ * (65-256).  A problem's kernel is fixed by (num_wann, nntot) when it is
  * admitted: a choice made earlier in the process (w90gpu_tune_set, or a
  * measurement of the online tuner), else the table measured on MI355X that
  * is compiled into the library, else the class default.  With the

@@ -4,7 +4,7 @@ example04 copper, gaas_disentanglement_issue192, example11_2 silicon
 conduction and basic2 with frozen windows, example11_1 silicon valence whose
 windows hold exactly num_wann states,
 write_u_matrices_disent with an outer window only) and against the oracle on
-synthetic problems up to 128 bands (above 64 the workgroups' matrices are in
+synthetic problems up to 256 bands (above 64 the workgroups' matrices are in
 global memory) and on the benchmark's 512-k problem.
 
 Tolerances: Omega_I per iteration 1e-9 relative to the oracle (the DIS
@@ -141,7 +141,7 @@ def test_wannierise_after_disentangle_matches_reference(ctx, name):
 # 60-column tiles.  Above 64 bands: "fcc-72-12" has the one-time kernels and
 # the extraction step in global memory and the Z pass (73 x 12 blocks) in
 # LDS; "hex-80-24" (windows of at most 60 bands) only the one-time kernels;
-# "sc-96-48" and "bcc-128-100" every kernel, the last at the largest size.
+# "sc-96-48" and "bcc-128-100" every kernel.
 SYN = [("fcc", 20, 12, (4, 4, 4), 4, 0), ("sc", 48, 24, (3, 3, 3), 6, 0), ("bcc", 64, 40, (2, 2, 2), 0, 0),
        ("hex", 33, 17, (3, 3, 2), 10, 0), ("sc", 64, 60, (2, 2, 2), 0, 0), ("fcc", 72, 12, (2, 2, 2), 0, 0),
        ("hex", 80, 24, (2, 2, 2), 4, 60), ("sc", 96, 48, (2, 2, 2), 8, 0), ("bcc", 128, 100, (2, 2, 2), 0, 0)]
@@ -157,6 +157,21 @@ def test_disentangle_matches_oracle_synthetic(ctx, case):
     if outer > 0:
         ctl["win_max"] = float(p.evals[:, outer - 1].min())
     ctl["num_iter"] = 60
+    o = po.dis_main(nb, nw, p.kmesh, ctl, p.eig, p.amat_dis, p.m_orig)
+    g = ctx.disentangle(p.kmesh, nb, ctl, p.eig, p.amat_dis, p.m_orig)
+    check_against(g, o, nw, well_posed=True, tol_w=1e-7)
+
+
+def test_disentangle_largest_size_matches_oracle(ctx):
+    """num_bands = 256, the largest the library takes (W90GPU_DIS_MAX_BANDS),
+    to num_wann = 200 with frozen bands: 20 iterations against the oracle,
+    every kernel with its matrices in global memory."""
+    from wannier90_amd import synth
+
+    nb, nw = 256, 200
+    print(f"[dis largest] nb={nb} nw={nw}", flush=True)
+    p, ctl = synth.make_dis_problem(7 + nb, kind="sc", num_bands=nb, num_wann=nw, mp=(2, 2, 2), frozen=16)
+    ctl["num_iter"] = 20
     o = po.dis_main(nb, nw, p.kmesh, ctl, p.eig, p.amat_dis, p.m_orig)
     g = ctx.disentangle(p.kmesh, nb, ctl, p.eig, p.amat_dis, p.m_orig)
     check_against(g, o, nw, well_posed=True, tol_w=1e-7)

@@ -1,4 +1,4 @@
-"""num_wann above 16 (17-128): the MFMA GEMM rotation, checked
+"""num_wann above 16 (17-256): the MFMA GEMM rotation, checked
 against the oracle on seeded synthetic problems (sc, bcc, fcc, triclinic
 lattices; nntot 6-12) small enough in k for the oracle to finish in seconds.
 
@@ -61,9 +61,7 @@ def ids(v):
     return f"{v[0]}-nw{v[1]}"
 
 
-@pytest.mark.parametrize("case", LARGE, ids=ids)
-def test_large_phases_match_oracle(ctx, case):
-    print(f"[large phases] {case}", flush=True)
+def check_phases(ctx, case):
     p, u0, m0 = problem(*case)
     op = oprob(p)
     w = antihermitian(u0.shape, 0.05, 7)
@@ -83,13 +81,11 @@ def test_large_phases_match_oracle(ctx, case):
     assert np.abs(g - g_ref).max() <= 1e-12 * max(1.0, np.abs(g_ref).max())
 
 
-@pytest.mark.parametrize("case", LARGE, ids=ids)
-def test_large_trajectory_matches_oracle(ctx, case):
+def check_trajectory(ctx, case, num_iter):
     from wannier90_amd import capi
 
-    print(f"[large trajectory] {case}", flush=True)
     p, u0, m0 = problem(*case)
-    ctl = dict(num_iter=20, num_cg_steps=5, conv_window=-1, conv_tol=1e-10, trial_step=2.0)
+    ctl = dict(num_iter=num_iter, num_cg_steps=5, conv_window=-1, conv_tol=1e-10, trial_step=2.0)
     o = po.wann_main(oprob(p), ctl, m0, u0)
     s = ctx.session(p.kmesh, p.num_wann, p.num_kpts, capi.control(**ctl))
     s.load(np.array(m0, order="F"), np.array(u0, order="F"))
@@ -107,11 +103,34 @@ def test_large_trajectory_matches_oracle(ctx, case):
         s_u = max(s_u, np.abs(o2["u"] - o["u"]).max())
         s_c = max(s_c, np.abs(o2["centres"] - o["centres"]).max())
     print(f"[large trajectory] sensitivity: Omega {s_h:.1e} U {s_u:.1e} centres {s_c:.1e}", flush=True)
-    assert res.iterations == o["iterations"] == 20
+    assert res.iterations == o["iterations"] == num_iter
     assert np.abs(hist[:, 0] - o["history"][:, 0]).max() <= tol(1e-8, s_h)
     assert abs(res.omega_total - o["omega"][3]) <= tol(1e-8, s_h)
     assert np.abs(cen - o["centres"]).max() <= tol(1e-6, s_c)
     assert np.abs(u - o["u"]).max() <= tol(1e-7, s_u)
+
+
+@pytest.mark.parametrize("case", LARGE, ids=ids)
+def test_large_phases_match_oracle(ctx, case):
+    print(f"[large phases] {case}", flush=True)
+    check_phases(ctx, case)
+
+
+@pytest.mark.parametrize("case", LARGE, ids=ids)
+def test_large_trajectory_matches_oracle(ctx, case):
+    print(f"[large trajectory] {case}", flush=True)
+    check_trajectory(ctx, case, 20)
+
+
+def test_largest_num_wann_matches_oracle(ctx):
+    """num_wann = 256, the largest the library takes (the per-k partial
+    sums give each Wannier function a thread of a 256-thread workgroup):
+    the single phases and a 5-iteration trajectory on a 2x2x2 simple-cubic
+    mesh (64 x 64 panels, four per dimension)."""
+    case = ("sc", 256, (2, 2, 2))
+    print(f"[largest num_wann] {case}", flush=True)
+    check_phases(ctx, case)
+    check_trajectory(ctx, case, 5)
 
 
 def test_triclinic_800k_matches_reference(ctx):

@@ -25,7 +25,8 @@ po = pytest.importorskip("oracle.pyoracle")
 CASES = [(("fcc", 24, (3, 3, 3)), ("fused32", "fused64")),
          (("bcc", 48, (3, 3, 3)), ("fused64", "direct64")),
          (("hex", 70, (2, 2, 2)), ("gemm64", "gemm32")),
-         (("sc", 96, (2, 2, 2)), ("gemm64", "gemm32"))]
+         (("sc", 96, (2, 2, 2)), ("gemm64", "gemm32")),
+         (("sc", 160, (2, 2, 2)), ("gemm64", "gemm32"))]
 PARAMS = [(case, k) for case, ks in CASES for k in ks]
 
 

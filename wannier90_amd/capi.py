@@ -26,7 +26,7 @@ ERR_INPUT = 5
 ERR_FILE = 6
 WARN_UNCONVERGED = -1
 NUM_NNMAX = 12
-MAX_WANN = 128
+MAX_WANN = 256
 PREC_FP64 = 0
 PREC_MIXED = 1
 

@@ -35,7 +35,7 @@
 // pass; the polar factors (of A and of U_opt^+ A) by the one-sided Jacobi
 // SVD with the same ordering.  A workgroup's matrices live in LDS up to 64
 // bands (two 64 x 65 complex blocks) and in its own area of global memory
-// above that, up to 128 bands (the same code, instantiated for each).  The
+// above that, up to 256 bands (the same code, instantiated for each).  The
 // extraction only uses the spanned subspaces and the eigenvalue sums, which
 // do not depend on the eigenvector phases; the final U_opt and U do, so they
 // agree with the reference up to the phase of each Hamiltonian eigenvector
@@ -227,7 +227,7 @@ __device__ __forceinline__ void rr_pair(int s, int r, int m, int& p, int& q) {
 }
 
 // rows of E per lane and chunk in a rotation pass; n <= 64 takes one chunk
-// (at least 8 lanes a slot), n <= 128 up to four
+// (at least 8 lanes a slot), n <= 256 up to eight
 constexpr int kRowsPerLane = 8;
 template <int N>
 constexpr int row_chunks() {
@@ -371,7 +371,7 @@ __device__ __forceinline__ int jacobi_run(zd* A, zd* E, int n, int ld, JL& J) {
         int p, q;
         rr_pair(es, r, m, p, q);
         if (q < n) {
-#pragma unroll
+#pragma unroll 1
           for (int ch = 0; ch < CH; ++ch) {
             const int row0 = eg + ch * kRowsPerLane * g;
             zd ep[kRowsPerLane], eq[kRowsPerLane];

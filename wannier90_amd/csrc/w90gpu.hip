@@ -91,7 +91,7 @@ int wann_class(int nw) { return nw <= 16 ? 0 : nw <= kLdsMaxWann ? 1 : nw <= W90
 // and candidates of other tile shapes:
 //   17-32: fused32 (32 x 32 tile, T in LDS) | fused64 (64 x 64 tile)
 //   33-64: fused64 | direct64 (operands straight into MFMA fragments)
-//   65-128: gemm64 (two launches over 64 x 64 panels) | gemm32 (32 x 32
+//   65-256: gemm64 (two launches over 64 x 64 panels) | gemm32 (32 x 32
 //           panels: less padding for 65-96 bands, lower reuse per load)
 // The choice is made per (num_wann, nntot) by the autotuner (rot_choice,
 // tune_member) and fixed for the life of the process, so a problem is

@@ -518,8 +518,9 @@ __device__ __forceinline__ const cplx* rot_factor(const Prob& p, int, int) { ret
 // --------------------------------------------------------- M rotation
 
 // Tiles per block M(k,b) of the MFMA rotation at most: 32 x 32 panels over
-// 128 bands.
-#define W90_MAX_PANELS 16
+// 256 bands.
+#define W90_MAX_PANELS 64
+static_assert(W90GPU_MAX_WANN <= W90_NT, "k_kpart: one thread per Wannier function");
 
 // Per-k partial vector from the per-tile outputs of the MFMA rotation
 // (num_wann above the LDS kernels): diagonal phases and |M_nn|^2, and the

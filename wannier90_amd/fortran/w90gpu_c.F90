@@ -14,7 +14,7 @@ module w90gpu_c
   integer(c_int), parameter :: W90GPU_NUM_NNMAX = 12
   integer(c_int), parameter :: W90GPU_PREC_FP64 = 0
   integer(c_int), parameter :: W90GPU_PREC_MIXED = 1
-  integer(c_int), parameter :: W90GPU_DIS_MAX_BANDS = 128
+  integer(c_int), parameter :: W90GPU_DIS_MAX_BANDS = 256
 
   !> wann_control_type fields on this path (w90gpu_control)
   type, bind(c) :: w90gpu_control
