@@ -178,7 +178,6 @@ __device__ __forceinline__ void rot_gemm_tile(const Prob& p, int k, int b, int t
                                               typename Real<T2>::type (*Bs)[W90_GK][GT + 1], double* red) {
   constexpr int FR = GT / 32;  // fragments per wavefront along rows and columns
   typedef typename Real<T2>::type R;
-  typedef typename Mfma<R>::acc Acc;
   const int nw = p.nw, nn = p.nn;
   const size_t kb = (size_t)k * nn + b;
   const size_t nw2 = p.nw2;
@@ -356,7 +355,6 @@ __device__ __forceinline__ void rot_fused_tile(const Prob& p, int k, int b, int 
   constexpr int FR = GT / 32;
   constexpr int LDA = GT + 1;
   typedef typename Real<T2>::type R;
-  typedef typename Mfma<R>::acc Acc;
   const int nw = p.nw, nn = p.nn;
   const size_t kb = (size_t)k * nn + b;
   const size_t nw2 = p.nw2;
@@ -1158,7 +1156,6 @@ template <class T2>
 __global__ void __launch_bounds__(64 * W90GPU_NUM_NNMAX) k_rotate_w16(const Prob* probs, const Item* items,
                                                                       int nitems, int mode) {
   typedef typename Real<T2>::type R;
-  typedef typename Mfma<R>::acc Acc;
   if (blockIdx.x >= (unsigned)nitems) return;
   const Item it = items[blockIdx.x];
   const Prob& p = probs[it.prob];
