@@ -973,7 +973,8 @@ int launch_step(Engine* E, const Lists& L, int mode) {
   if (c.n)
     hipLaunchKernelGGL(k_step_mfma, dim3(c.n), dim3(256), step_mfma_lds(), s, E->d_probs, L.items + c.off, c.n,
                        mode);
-  if (d.n) hipLaunchKernelGGL(k_step, dim3(d.n), dim3(W90_NT), 0, s, E->d_probs, L.items + d.off, d.n, mode);
+  if (d.n)
+    hipLaunchKernelGGL(k_step, dim3(d.n), dim3(W90_STEP_NT), 0, s, E->d_probs, L.items + d.off, d.n, mode);
   HIPCHK(E->ctx, hipGetLastError());
   return W90GPU_OK;
 }

@@ -42,6 +42,7 @@
 typedef double2 cplx;
 
 #define W90_NT 256          // threads per workgroup of the per-k kernels
+#define W90_STEP_NT 1024    // threads of the global-memory retraction (above 64 bands)
 #define W90_FINAL_NT 1024   // threads of the per-problem omega kernel
 #define W90_MAX_GROUPS 32   // k-ranges of the first omega reduction level
 #define W90_MIXED_RTOL 1e-8 // relative FP32 noise floor of Omega differences in the bulk phase
@@ -1305,9 +1306,13 @@ __device__ void search_control(State* st, double gg, double gd) {
 // must be W90_NT (the summation order depends on it).
 __device__ void search_in_step(const Prob& p, int k, State* lst, double* red) {
   constexpr int NSD = sizeof(State) / sizeof(double);
-  // the partial sums are requested first, the state copy overlaps them
+  // the partial sums are requested first, the state copy overlaps them;
+  // the k range is split over W90_NT lanes whatever the workgroup size
+  // (lanes beyond add exact zeros), so every retraction kernel forms the
+  // same sums
   double gg = 0.0, gd = 0.0;
-  const int per = (p.nk + blockDim.x - 1) / blockDim.x;
+  const int nsplit = min((int)blockDim.x, W90_NT);
+  const int per = (p.nk + nsplit - 1) / nsplit;
   const int kb = threadIdx.x * per, ke = min(p.nk, kb + per);
   // chunks of 4: the loads of a chunk are issued together, then added in
   // k order
@@ -1439,7 +1444,7 @@ __device__ void wg_gemm_mfma(cplx* C, const cplx* A, const cplx* B, int n, doubl
 // X is anti-Hermitian, so exp(X) is the same unitary that
 // internal_new_u_and_m builds from the eigenvectors of i X
 // (wannierise.F90:1662-1689, 1729-1734).
-__global__ void __launch_bounds__(W90_NT) k_step(const Prob* probs, const Item* items, int nitems, int mode) {
+__global__ void __launch_bounds__(W90_STEP_NT) k_step(const Prob* probs, const Item* items, int nitems, int mode) {
   if (blockIdx.x >= (unsigned)nitems) return;
   const Item it = items[blockIdx.x];
   const Prob& p = probs[it.prob];
