@@ -1011,8 +1011,10 @@ int launch_domega(Engine* E, const Lists& L, int mode) {
           hipLaunchKernelGGL((k_domega_w64<float2>), dim3(g.n), dim3(domega_nt<W90_STEP_N>()),
                              (domega_lds<W90_STEP_N, float2>()), s, P, it, g.n, mode);
       } else {
-        if (q == 0) hipLaunchKernelGGL(k_domega<cplx>, dim3(g.n), dim3(1024), 0, s, P, it, g.n, mode);
-        else hipLaunchKernelGGL(k_domega<float2>, dim3(g.n), dim3(1024), 0, s, P, it, g.n, mode);
+        if (q == 0)
+          hipLaunchKernelGGL(k_domega<cplx>, dim3(g.n), dim3(1024), domega_t_lds<cplx>(), s, P, it, g.n, mode);
+        else
+          hipLaunchKernelGGL(k_domega<float2>, dim3(g.n), dim3(1024), domega_t_lds<float2>(), s, P, it, g.n, mode);
       }
       HIPCHK(E->ctx, hipGetLastError());
     }
@@ -1561,6 +1563,8 @@ int w90gpu_create(int device, w90gpu_ctx** ctx) {
       {(const void*)k_domega_k<kLdsMaxWann, float2>, domega_lds<kLdsMaxWann, float2>()},
       {(const void*)k_domega_w64<cplx>, domega_lds<W90_STEP_N, cplx>()},
       {(const void*)k_domega_w64<float2>, domega_lds<W90_STEP_N, float2>()},
+      {(const void*)k_domega<cplx>, domega_t_lds<cplx>()},
+      {(const void*)k_domega<float2>, domega_t_lds<float2>()},
   };
   for (const auto& l : lds) {
     if (hipFuncSetAttribute(l.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.bytes) != hipSuccess) {

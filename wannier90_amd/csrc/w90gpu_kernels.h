@@ -940,9 +940,23 @@ __device__ __forceinline__ cplx domega_term_r(cplx acc, double w, cplx mmn, cplx
   return c_sub(acc, c_mul(c_add(t_mn, c_conj(t_nm)), mhalf_i));
 }
 
-// wann_domega (wannierise.F90:2287-2569), non-SLWF gradient, one
-// workgroup per (problem, k), fused with the per-k partial sums Re<G,G> and
-// Re<G,D_old> needed by internal_search_direction.
+// wann_domega (wannierise.F90:2287-2569), non-SLWF gradient, above 64
+// bands: one workgroup of 1024 threads per (problem, k), fused with the
+// per-k partial sums Re<G,G> and Re<G,D_old> needed by
+// internal_search_direction.  Element (m, n) needs M_mn and M_nm, so the
+// block is taken in pairs of 32 x 32 tiles (I, J) and (J, I), I <= J: for
+// each b both tiles are staged in LDS from coalesced loads (the next b's
+// requested while this one is used), and each thread forms one element of
+// each tile, its sum over b in b order.  The diagonals M_nn and the phases
+// w_b Im ln M_nn of every b are staged once per k.  G is stored, and the
+// per-k sums are then formed over G in element order e = t + 1024 i per
+// thread, which makes them independent of the tiling.
+constexpr int kDtTile = 32;
+template <class T2>
+__host__ __device__ constexpr size_t domega_t_lds() {
+  return (sizeof(T2) + sizeof(double)) * W90GPU_NUM_NNMAX * W90GPU_MAX_WANN +
+         sizeof(T2) * 2 * kDtTile * (kDtTile + 1);
+}
 template <class T2>
 __global__ void __launch_bounds__(1024) k_domega(const Prob* probs, const Item* items, int nitems, int mode) {
   if (blockIdx.x >= (unsigned)nitems) return;
@@ -951,35 +965,82 @@ __global__ void __launch_bounds__(1024) k_domega(const Prob* probs, const Item* 
   const State* st = p.st;
   const int cur = gate_cur(st, (mode == MODE_BENCH ? MODE_BENCH : MODE_TRIAL));
   if (cur < 0) return;
+  constexpr int TS = kDtTile, LD = kDtTile + 1;
   const int nw = p.nw, nn = p.nn, nw2 = p.nw2;
-  const int k = it.k;
+  const int k = it.k, tid = threadIdx.x;
+  const int lr = tid % TS, lc = tid / TS;
   const double* ln = p.ln[cur];
   const double* rave = p.rave[cur];
   const T2* Mk = mbuf<T2>(p, cur) + (size_t)k * nn * nw2;
+  extern __shared__ double lds_dt[];
+  T2* ta = reinterpret_cast<T2*>(lds_dt);                // tile (I, J), column-major
+  T2* tb = ta + TS * LD;                                  // tile (J, I)
+  T2* dg = tb + TS * LD;                                  // [b][n] M_nn
+  double* qv = reinterpret_cast<double*>(dg + nn * nw);  // [b][n] w_b Im ln M_nn
+  __shared__ double wbL[W90GPU_NUM_NNMAX], bkL[3 * W90GPU_NUM_NNMAX];
   __shared__ double red[1024];
+  kmesh_stage(p, k, wbL, bkL);
+  for (int t = tid; t < nn * nw; t += blockDim.x) {
+    const int b = t / nw, n = t % nw;
+    dg[t] = Mk[(size_t)b * nw2 + n + (size_t)n * nw];
+    qv[t] = p.wb[b] * ln[((size_t)k * nn + b) * nw + n];
+  }
   const double rnk = (double)p.nk;
-  double gg = 0.0, gd = 0.0;
-  for (int e = threadIdx.x; e < nw2; e += blockDim.x) {
-    const int m = e % nw, n = e / nw;
-    cplx acc = c_make(0.0, 0.0);
+  cplx* Gk = p.G + (size_t)k * nw2;
+  const int nt = (nw + TS - 1) / TS;
+  for (int J = 0, I = 0; J < nt;) {
+    const bool diag = I == J;
+    const int m1 = I * TS + lr, n1 = J * TS + lc;  // element of tile (I, J)
+    const int m2 = J * TS + lr, n2 = I * TS + lc;  // element of tile (J, I)
+    const bool v1 = m1 < nw && n1 < nw, v2 = !diag && m2 < nw && n2 < nw;
+    T2 xa = {}, xb = {};
+    if (v1) xa = Mk[m1 + (size_t)n1 * nw];
+    if (v2) xb = Mk[m2 + (size_t)n2 * nw];
+    cplx acc1 = c_make(0.0, 0.0), acc2 = acc1;
     for (int b = 0; b < nn; ++b) {
-      const T2* Mb = Mk + (size_t)b * nw2;
-      const double w = p.wb[b];
-      const double* bv = p.bk + ((size_t)k * nn + b) * 3;
-      const double* lnb = ln + ((size_t)k * nn + b) * nw;
-      const double rn = bv[0] * rave[0 + 3 * n] + bv[1] * rave[1 + 3 * n] + bv[2] * rave[2 + 3 * n];
-      const double rm = bv[0] * rave[0 + 3 * m] + bv[1] * rave[1 + 3 * m] + bv[2] * rave[2 + 3 * m];
-      acc = domega_term(acc, w, c_wide(Mb[m + n * nw]), c_wide(Mb[n + m * nw]), c_wide(Mb[n + n * nw]),
-                        c_wide(Mb[m + m * nw]), w * lnb[n], w * lnb[m], rn, rm);
+      __syncthreads();  // the tiles of b - 1 are used (and, for b = 0, the staging above is done)
+      ta[lr + lc * LD] = xa;
+      tb[lr + lc * LD] = xb;
+      if (b + 1 < nn) {
+        const T2* Mb1 = Mk + (size_t)(b + 1) * nw2;
+        if (v1) xa = Mb1[m1 + (size_t)n1 * nw];
+        if (v2) xb = Mb1[m2 + (size_t)n2 * nw];
+      }
+      __syncthreads();
+      const double w = wbL[b];
+      const double* bv = bkL + 3 * b;
+      const T2* dgb = dg + b * nw;
+      const double* qb = qv + b * nw;
+      if (v1) {
+        const double rn = bv[0] * rave[0 + 3 * n1] + bv[1] * rave[1 + 3 * n1] + bv[2] * rave[2 + 3 * n1];
+        const double rm = bv[0] * rave[0 + 3 * m1] + bv[1] * rave[1 + 3 * m1] + bv[2] * rave[2 + 3 * m1];
+        acc1 = domega_term(acc1, w, c_wide(ta[lr + lc * LD]), c_wide((diag ? ta : tb)[lc + lr * LD]),
+                           c_wide(dgb[n1]), c_wide(dgb[m1]), qb[n1], qb[m1], rn, rm);
+      }
+      if (v2) {
+        const double rn = bv[0] * rave[0 + 3 * n2] + bv[1] * rave[1 + 3 * n2] + bv[2] * rave[2 + 3 * n2];
+        const double rm = bv[0] * rave[0 + 3 * m2] + bv[1] * rave[1 + 3 * m2] + bv[2] * rave[2 + 3 * m2];
+        acc2 = domega_term(acc2, w, c_wide(tb[lr + lc * LD]), c_wide(ta[lc + lr * LD]), c_wide(dgb[n2]),
+                           c_wide(dgb[m2]), qb[n2], qb[m2], rn, rm);
+      }
     }
-    const cplx g = c_scale(make_double2(acc.x / rnk, acc.y / rnk), 4.0);
-    p.G[(size_t)k * nw2 + e] = g;
+    if (v1) Gk[m1 + (size_t)n1 * nw] = c_scale(make_double2(acc1.x / rnk, acc1.y / rnk), 4.0);
+    if (v2) Gk[m2 + (size_t)n2 * nw] = c_scale(make_double2(acc2.x / rnk, acc2.y / rnk), 4.0);
+    if (++I > J) {
+      ++J;
+      I = 0;
+    }
+  }
+  __syncthreads();  // G of the k is stored
+  double gg = 0.0, gd = 0.0;
+  for (int e = tid; e < nw2; e += blockDim.x) {
+    const cplx g = Gk[e];
     gg += c_abs2(g);
     const cplx dold = p.D[(size_t)k * nw2 + e];
     gd += g.x * dold.x + g.y * dold.y;
   }
   block_sum2(gg, gd, red);
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     p.part_gg[k] = gg;
     p.part_gd[k] = gd;
   }
