@@ -361,8 +361,8 @@ int w90gpu_disentangle(w90gpu_ctx* ctx, const w90gpu_problem* prob, int num_band
  *
  * No counterpart in the reference.  The M rotation has several kernels, by
  * tile shape, per num_wann class: "w16" (up to 16 bands), "fused32" or
- * "fused64" (17-32), "fused64" or "direct64" (33-64), "gemm64" or "gemm32"
- * (65-256).  A problem's kernel is fixed by (num_wann, nntot) when it is
+ * "fused64" (17-32), "fused64", "direct64" or "fused48" (33-64, fused48 up
+ * to 48), "gemm64" or "gemm32" (65-256).  A problem's kernel is fixed by (num_wann, nntot) when it is
  * admitted: a choice made earlier in the process (w90gpu_tune_set, or a
  * measurement of the online tuner), else the table measured on MI355X that
  * is compiled into the library, else the class default.  With the

@@ -8,7 +8,7 @@ import pytest
 
 from wannier90_amd import tune
 
-CLASS_KERNELS = {1: {"fused32", "fused64"}, 2: {"fused64", "direct64"}, 3: {"gemm64", "gemm32"}}
+CLASS_KERNELS = {1: {"fused32", "fused64"}, 2: {"fused64", "direct64", "fused48"}, 3: {"gemm64", "gemm32"}}
 
 
 def test_nn_ranges_split_at_midpoints():
@@ -67,6 +67,9 @@ def test_choice_api_follows_table_and_fixed_choices():
         for nn in (6, 8, 12):
             want = covered.get((nw, nn), tune.DEFAULTS[tune.wann_class(nw)])
             assert capi.tune_get(nw, nn) == want
+    capi.tune_set(48, 6, "fused48")
+    assert capi.tune_get(48, 6) == "fused48"
+    capi.tune_set(48, 6, None)
     # a fixed choice, then back
     before = capi.tune_get(80, 6)
     other = ({"gemm64", "gemm32"} - {before}).pop()
@@ -74,7 +77,9 @@ def test_choice_api_follows_table_and_fixed_choices():
     assert capi.tune_get(80, 6) == other
     capi.tune_set(80, 6, None)
     assert capi.tune_get(80, 6) == before
-    # a kernel outside the class, an unknown kernel, an invalid key
-    for bad in ((80, 6, "fused32"), (20, 6, "gemm32"), (40, 6, "nope"), (0, 6, "w16"), (20, 13, "fused32")):
+    # a kernel outside the class, a 48 tile for more than 48 bands, an
+    # unknown kernel, an invalid key
+    for bad in ((80, 6, "fused32"), (20, 6, "gemm32"), (49, 6, "fused48"), (40, 6, "nope"), (0, 6, "w16"),
+                (20, 13, "fused32")):
         with pytest.raises(capi.W90Error):
             capi.tune_set(*bad)

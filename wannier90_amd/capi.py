@@ -212,7 +212,7 @@ class KMesh:
         self.wbtot = wbtot
 
 
-ROT_KERNELS = ("w16", "fused32", "fused64", "direct64", "gemm64", "gemm32")
+ROT_KERNELS = ("w16", "fused32", "fused64", "direct64", "gemm64", "gemm32", "fused48")
 
 
 def tune_get(num_wann, nntot):

@@ -90,29 +90,37 @@ int wann_class(int nw) { return nw <= 16 ? 0 : nw <= kLdsMaxWann ? 1 : nw <= W90
 // Rotation kernels by tile shape.  Each class above 16 bands has a default
 // and candidates of other tile shapes:
 //   17-32: fused32 (32 x 32 tile, T in LDS) | fused64 (64 x 64 tile)
-//   33-64: fused64 | direct64 (operands straight into MFMA fragments)
+//   33-64: fused64 | direct64 (operands straight into MFMA fragments) |
+//          fused48 (a 48 x 48 tile over three wavefronts, up to 48 bands)
 //   65-256: gemm64 (two launches over 64 x 64 panels) | gemm32 (32 x 32
 //           panels: less padding for 65-96 bands, lower reuse per load)
 // The choice is made per (num_wann, nntot) by the autotuner (rot_choice,
 // tune_member) and fixed for the life of the process, so a problem is
 // rotated by the same kernel alone and in a batch.
-enum RotKernel { RK_W16 = 0, RK_FUSED32, RK_FUSED64, RK_DIRECT64, RK_GEMM64, RK_GEMM32, kRotKernels };
-const char* const kRotName[kRotKernels] = {"w16", "fused32", "fused64", "direct64", "gemm64", "gemm32"};
+enum RotKernel { RK_W16 = 0, RK_FUSED32, RK_FUSED64, RK_DIRECT64, RK_GEMM64, RK_GEMM32, RK_FUSED48, kRotKernels };
+const char* const kRotName[kRotKernels] = {"w16", "fused32", "fused64", "direct64", "gemm64", "gemm32", "fused48"};
 // output tile edge (the initial reduction k_reduce_tiles uses the same tiles)
-constexpr int kRotEdge[kRotKernels] = {16, 32, 64, 64, 64, 32};
+constexpr int kRotEdge[kRotKernels] = {16, 32, 64, 64, 64, 32, 48};
 
 // Launch groups: (class, rotation kernel) pairs in class order.  Item lists
 // are sorted by class, then storage precision, then group, so that one
 // (class, precision) is a contiguous range for the gradient kernels and one
 // (group, precision) a contiguous range for the rotation.
-constexpr int kGroups = 7;
-constexpr int kGroupClass[kGroups] = {0, 1, 1, 2, 2, 3, 3};
-constexpr int kGroupRot[kGroups] = {RK_W16, RK_FUSED32, RK_FUSED64, RK_FUSED64, RK_DIRECT64, RK_GEMM64, RK_GEMM32};
+constexpr int kGroups = 8;
+constexpr int kGroupClass[kGroups] = {0, 1, 1, 2, 2, 2, 3, 3};
+constexpr int kGroupRot[kGroups] = {RK_W16,      RK_FUSED32, RK_FUSED64, RK_FUSED64,
+                                    RK_DIRECT64, RK_FUSED48, RK_GEMM64,  RK_GEMM32};
 
 int group_of(int cls, int rk) {
   for (int g = 0; g < kGroups; ++g)
     if (kGroupClass[g] == cls && kGroupRot[g] == rk) return g;
   return -1;
+}
+
+// Whether kernel rk rotates num_wann nw: it serves nw's class, and the one
+// tile of fused48 covers the block.
+bool rot_serves(int rk, int nw) {
+  return rk >= 0 && group_of(wann_class(nw), rk) >= 0 && (rk != RK_FUSED48 || nw <= 48);
 }
 int default_rot(int cls) { return cls == 0 ? RK_W16 : cls == 1 ? RK_FUSED32 : cls == 2 ? RK_FUSED64 : RK_GEMM64; }
 int first_group(int cls) {
@@ -141,7 +149,8 @@ static_assert(((W90GPU_MAX_WANN + 31) / 32) * ((W90GPU_MAX_WANN + 31) / 32) <= W
 int max_panels(int nw) {
   const int c = wann_class(nw);
   int mx = 1;
-  for (int g = first_group(c); g <= last_group(c); ++g) mx = std::max(mx, rot_panels(kGroupRot[g], nw));
+  for (int g = first_group(c); g <= last_group(c); ++g)
+    if (rot_serves(kGroupRot[g], nw)) mx = std::max(mx, rot_panels(kGroupRot[g], nw));
   return mx;
 }
 
@@ -668,7 +677,7 @@ void tune_refresh() {
       char name[32] = {0};
       if (line[0] == '#' || std::sscanf(line, "%d %d %31s", &nw, &nn, name) != 3) continue;
       const int r = rot_by_name(name);
-      if (r >= 0 && nw >= 1 && nw <= W90GPU_MAX_WANN && group_of(wann_class(nw), r) >= 0)
+      if (r >= 0 && nw >= 1 && nw <= W90GPU_MAX_WANN && rot_serves(r, nw))
         tc.chosen.emplace(std::make_pair(nw, nn), r);
     }
     std::fclose(fp);
@@ -683,7 +692,7 @@ int rot_choice(int nw, int nn, bool* pending) {
   std::lock_guard<std::mutex> lock(tc.mu);
   const int c = wann_class(nw);
   if (pending) *pending = false;
-  if (tc.forced >= 0 && group_of(c, tc.forced) >= 0) return tc.forced;
+  if (tc.forced >= 0 && rot_serves(tc.forced, nw)) return tc.forced;
   auto it = tc.chosen.find({nw, nn});
   if (it != tc.chosen.end()) return it->second;
   if (tc.online) {
@@ -691,7 +700,7 @@ int rot_choice(int nw, int nn, bool* pending) {
     return default_rot(c);
   }
   for (const TuneRow& r : kTuneTable)
-    if (nw >= r.nw_lo && nw <= r.nw_hi && nn >= r.nn_lo && nn <= r.nn_hi && group_of(c, r.rk) >= 0) return r.rk;
+    if (nw >= r.nw_lo && nw <= r.nw_hi && nn >= r.nn_lo && nn <= r.nn_hi && rot_serves(r.rk, nw)) return r.rk;
   return default_rot(c);
 }
 
@@ -893,6 +902,7 @@ void launch_rot_kernel(hipStream_t s, const Prob* P, const Item* it, const Seg& 
     return;
   }
   if (mode == MODE_INIT) {
+    // fused48 writes one tile per block, as the 64 tiles do up to 64 bands
     if (kRotEdge[rk] == 32) hipLaunchKernelGGL((k_reduce_tiles<T2, 32>), g2, dim3(256), 0, s, P, it, g.n, mode);
     else hipLaunchKernelGGL((k_reduce_tiles<T2, 64>), g2, dim3(256), 0, s, P, it, g.n, mode);
     return;
@@ -903,6 +913,9 @@ void launch_rot_kernel(hipStream_t s, const Prob* P, const Item* it, const Seg& 
       break;
     case RK_FUSED64:
       hipLaunchKernelGGL((k_rot_fused<T2, 64>), gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
+      break;
+    case RK_FUSED48:
+      hipLaunchKernelGGL((k_rot_fused<T2, 48>), gb, dim3(FusedWaves<48>::NT), 0, s, P, it, g.n, g.nnmax, mode);
       break;
     case RK_DIRECT64:
       hipLaunchKernelGGL(k_rot_direct<T2>, gb, dim3(256), 0, s, P, it, g.n, g.nnmax, mode);
@@ -1066,7 +1079,8 @@ int tune_member(Engine* E, Member* m) {
     for (int k = 0; k < m->nk; ++k) items[(size_t)r * m->nk + k] = Item{m->slot, k};
   HIPCHK(E->ctx, hipMemcpyAsync(E->d_aux_items, items.data(), sizeof(Item) * n, hipMemcpyHostToDevice, E->stream));
   std::vector<int> cand;
-  for (int g = first_group(c); g <= last_group(c); ++g) cand.push_back(kGroupRot[g]);
+  for (int g = first_group(c); g <= last_group(c); ++g)
+    if (rot_serves(kGroupRot[g], nw)) cand.push_back(kGroupRot[g]);
   double t[kRotKernels];
   for (double& x : t) x = 1e30;
   auto time_one = [&](int rk) -> int {
@@ -1629,7 +1643,7 @@ int w90gpu_tune_set(int num_wann, int nntot, const char* name) {
     return W90GPU_OK;
   }
   const int r = rot_by_name(name);
-  if (r < 0 || group_of(wann_class(num_wann), r) < 0) {
+  if (!rot_serves(r, num_wann)) {
     set_error(nullptr, "w90gpu_tune_set: kernel '%s' does not rotate num_wann=%d", name, num_wann);
     return W90GPU_ERR_INPUT;
   }

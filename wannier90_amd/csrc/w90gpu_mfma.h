@@ -332,6 +332,19 @@ __global__ void __launch_bounds__(256, 2) k_rot_gemm(const Prob* probs, const It
 // K chunk of the second product of k_rot_fused (T held in LDS).
 #define W90_GK2 8
 
+// Wavefront layout of k_rot_fused by tile edge GT: WR x WC wavefronts, each
+// holding (GT / 16 WR) x (GT / 16 WC) fragments of 16 x 16.  A 64 or 32
+// tile is split in quadrants over four wavefronts; a 48 tile (num_wann up
+// to 48: 3 x 3 fragments, which four wavefronts cannot share evenly) in
+// three column strips of 48 x 16 over three wavefronts.
+template <int GT>
+struct FusedWaves {
+  static constexpr int WR = GT == 48 ? 1 : 2, WC = GT == 48 ? 3 : 2;
+  static constexpr int NT = 64 * WR * WC;
+  static constexpr int FRR = GT / (16 * WR), FRC = GT / (16 * WC);
+  static constexpr int MINB = GT == 48 ? 3 : 2;  // workgroups per CU the registers allow
+};
+
 // LDS elements (storage-precision reals) of k_rot_fused: the staging of the
 // first product (A and B chunks) and, after it, T with the B chunks of the
 // second product share one buffer.
@@ -352,7 +365,8 @@ __host__ __device__ constexpr int rot_fused_elems() {
 template <class T2, int GT>
 __device__ __forceinline__ void rot_fused_tile(const Prob& p, int k, int b, int mode, int cur,
                                                typename Real<T2>::type* lds, double* red) {
-  constexpr int FR = GT / 32;
+  typedef FusedWaves<GT> WL;
+  constexpr int FRR = WL::FRR, FRC = WL::FRC, FRM = FRR > FRC ? FRR : FRC, NT = WL::NT;
   constexpr int LDA = GT + 1;
   typedef typename Real<T2>::type R;
   const int nw = p.nw, nn = p.nn;
@@ -374,22 +388,22 @@ __device__ __forceinline__ void rot_fused_tile(const Prob& p, int k, int b, int 
 
   const int tid = threadIdx.x;
   const int w = tid >> 6, lane = tid & 63;
-  const int wr = (w >> 1) * (GT / 2), wc = (w & 1) * (GT / 2);
+  const int wr = (w / WL::WC) * (GT / WL::WR), wc = (w % WL::WC) * (GT / WL::WC);
   const int lr = lane & 15, lk = lane >> 4;
-  Cfrag<R> C[FR][FR];
+  Cfrag<R> C[FRR][FRC];
 #pragma unroll
-  for (int fi = 0; fi < FR; ++fi)
+  for (int fi = 0; fi < FRR; ++fi)
 #pragma unroll
-    for (int fj = 0; fj < FR; ++fj) C[fi][fj].zero();
+    for (int fj = 0; fj < FRC; ++fj) C[fi][fj].zero();
 
   // ---- T = E_k^+ src, K chunks of GK loaded one chunk ahead
-  constexpr int NE = W90_GK * GT / 256;
-  static_assert(NE * 256 == W90_GK * GT, "whole chunk per thread slot");
+  constexpr int NE = W90_GK * GT / NT;
+  static_assert(NE * NT == W90_GK * GT, "whole chunk per thread slot");
   cplx pa[NE], pb[NE];
   auto load1 = [&](int kk) {
 #pragma unroll
     for (int u = 0; u < NE; ++u) {
-      const int e = tid + 256 * u;
+      const int e = tid + NT * u;
       const int l = e % W90_GK, x = e / W90_GK;
       const int gl = kk + l;
       cplx v = c_make(0.0, 0.0), v2 = c_make(0.0, 0.0);
@@ -407,7 +421,7 @@ __device__ __forceinline__ void rot_fused_tile(const Prob& p, int k, int b, int 
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < NE; ++u) {
-      const int e = tid + 256 * u;
+      const int e = tid + NT * u;
       const int l = e % W90_GK, x = e / W90_GK;
       As[(0 * W90_GK + l) * LDA + x] = (R)pa[u].x;
       As[(1 * W90_GK + l) * LDA + x] = (R)pa[u].y;
@@ -418,26 +432,30 @@ __device__ __forceinline__ void rot_fused_tile(const Prob& p, int k, int b, int 
     __syncthreads();
 #pragma unroll
     for (int ks = 0; ks < W90_GK; ks += 4) {
-      R ar[FR], ai[FR], br[FR], bi[FR];
+      R ar[FRR], ai[FRR], br[FRC], bi[FRC];
 #pragma unroll
-      for (int f = 0; f < FR; ++f) {
-        ar[f] = As[(0 * W90_GK + ks + lk) * LDA + wr + f * 16 + lr];
-        ai[f] = As[(1 * W90_GK + ks + lk) * LDA + wr + f * 16 + lr];
-        br[f] = Bs[(0 * W90_GK + ks + lk) * LDA + wc + f * 16 + lr];
-        bi[f] = Bs[(1 * W90_GK + ks + lk) * LDA + wc + f * 16 + lr];
+      for (int f = 0; f < FRM; ++f) {
+        if (f < FRR) {
+          ar[f] = As[(0 * W90_GK + ks + lk) * LDA + wr + f * 16 + lr];
+          ai[f] = As[(1 * W90_GK + ks + lk) * LDA + wr + f * 16 + lr];
+        }
+        if (f < FRC) {
+          br[f] = Bs[(0 * W90_GK + ks + lk) * LDA + wc + f * 16 + lr];
+          bi[f] = Bs[(1 * W90_GK + ks + lk) * LDA + wc + f * 16 + lr];
+        }
       }
 #pragma unroll
-      for (int fi = 0; fi < FR; ++fi)
+      for (int fi = 0; fi < FRR; ++fi)
 #pragma unroll
-        for (int fj = 0; fj < FR; ++fj) C[fi][fj].mma(ar[fi], ai[fi], br[fj], bi[fj]);
+        for (int fj = 0; fj < FRC; ++fj) C[fi][fj].mma(ar[fi], ai[fi], br[fj], bi[fj]);
     }
   }
   // every wavefront is done with the staging buffers before T overwrites them
   __syncthreads();
 #pragma unroll
-  for (int fi = 0; fi < FR; ++fi)
+  for (int fi = 0; fi < FRR; ++fi)
 #pragma unroll
-    for (int fj = 0; fj < FR; ++fj)
+    for (int fj = 0; fj < FRC; ++fj)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = wr + fi * 16 + Mfma<R>::row(lk, i);
@@ -446,18 +464,18 @@ __device__ __forceinline__ void rot_fused_tile(const Prob& p, int k, int b, int 
         Ts[(1 * GT + c) * LDA + r] = C[fi][fj].im(i);
       }
 #pragma unroll
-  for (int fi = 0; fi < FR; ++fi)
+  for (int fi = 0; fi < FRR; ++fi)
 #pragma unroll
-    for (int fj = 0; fj < FR; ++fj) C[fi][fj].zero();
+    for (int fj = 0; fj < FRC; ++fj) C[fi][fj].zero();
 
   // ---- R = T E_{k+b}, K chunks of GK2 rows of E_{k+b} loaded one ahead
-  constexpr int NE2 = W90_GK2 * GT / 256;
-  static_assert(NE2 * 256 == W90_GK2 * GT, "whole chunk per thread slot");
+  constexpr int NE2 = W90_GK2 * GT / NT;
+  static_assert(NE2 * NT == W90_GK2 * GT, "whole chunk per thread slot");
   cplx pc[NE2];
   auto load2 = [&](int kk) {
 #pragma unroll
     for (int u = 0; u < NE2; ++u) {
-      const int e = tid + 256 * u;
+      const int e = tid + NT * u;
       const int l = e % W90_GK2, j = e / W90_GK2;
       const int gl = kk + l;
       cplx v = c_make(0.0, 0.0);
@@ -470,7 +488,7 @@ __device__ __forceinline__ void rot_fused_tile(const Prob& p, int k, int b, int 
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < NE2; ++u) {
-      const int e = tid + 256 * u;
+      const int e = tid + NT * u;
       const int l = e % W90_GK2, j = e / W90_GK2;
       B2[(0 * W90_GK2 + l) * LDA + j] = (R)pc[u].x;
       B2[(1 * W90_GK2 + l) * LDA + j] = (R)pc[u].y;
@@ -479,27 +497,31 @@ __device__ __forceinline__ void rot_fused_tile(const Prob& p, int k, int b, int 
     __syncthreads();
 #pragma unroll
     for (int ks = 0; ks < W90_GK2; ks += 4) {
-      R ar[FR], ai[FR], br[FR], bi[FR];
+      R ar[FRR], ai[FRR], br[FRC], bi[FRC];
 #pragma unroll
-      for (int f = 0; f < FR; ++f) {
-        ar[f] = Ts[(0 * GT + kk + ks + lk) * LDA + wr + f * 16 + lr];
-        ai[f] = Ts[(1 * GT + kk + ks + lk) * LDA + wr + f * 16 + lr];
-        br[f] = B2[(0 * W90_GK2 + ks + lk) * LDA + wc + f * 16 + lr];
-        bi[f] = B2[(1 * W90_GK2 + ks + lk) * LDA + wc + f * 16 + lr];
+      for (int f = 0; f < FRM; ++f) {
+        if (f < FRR) {
+          ar[f] = Ts[(0 * GT + kk + ks + lk) * LDA + wr + f * 16 + lr];
+          ai[f] = Ts[(1 * GT + kk + ks + lk) * LDA + wr + f * 16 + lr];
+        }
+        if (f < FRC) {
+          br[f] = B2[(0 * W90_GK2 + ks + lk) * LDA + wc + f * 16 + lr];
+          bi[f] = B2[(1 * W90_GK2 + ks + lk) * LDA + wc + f * 16 + lr];
+        }
       }
 #pragma unroll
-      for (int fi = 0; fi < FR; ++fi)
+      for (int fi = 0; fi < FRR; ++fi)
 #pragma unroll
-        for (int fj = 0; fj < FR; ++fj) C[fi][fj].mma(ar[fi], ai[fi], br[fj], bi[fj]);
+        for (int fj = 0; fj < FRC; ++fj) C[fi][fj].mma(ar[fi], ai[fi], br[fj], bi[fj]);
     }
   }
 
   // ---- epilogue: R to the other buffer, diagonal phases and tile sums
   double od = 0.0, all = 0.0;
 #pragma unroll
-  for (int fi = 0; fi < FR; ++fi)
+  for (int fi = 0; fi < FRR; ++fi)
 #pragma unroll
-    for (int fj = 0; fj < FR; ++fj)
+    for (int fj = 0; fj < FRC; ++fj)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = wr + fi * 16 + Mfma<R>::row(lk, i);
@@ -535,8 +557,8 @@ __device__ __forceinline__ void rot_fused_tile(const Prob& p, int k, int b, int 
 // (t, b) = divmod(w / 8, nnmax): the nnmax workgroups of one k run back to
 // back on one XCD and share E_k through its L2.
 template <class T2, int GT>
-__global__ void __launch_bounds__(256, 2) k_rot_fused(const Prob* probs, const Item* items, int nitems, int nnmax,
-                                                      int mode) {
+__global__ void __launch_bounds__(FusedWaves<GT>::NT, FusedWaves<GT>::MINB) k_rot_fused(const Prob* probs, const Item* items, int nitems,
+                                                                     int nnmax, int mode) {
   typedef typename Real<T2>::type R;
   const int w = blockIdx.x;
   const int r = w >> 3;

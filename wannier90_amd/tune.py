@@ -28,7 +28,7 @@ import numpy as np
 # lattices giving each nntot on a cubic mesh: sc 6, fcc 8, bcc 12
 LATTICE_OF_NN = {6: "sc", 8: "fcc", 12: "bcc"}
 DEFAULTS = {1: "fused32", 2: "fused64", 3: "gemm64"}
-KERNELS = ["w16", "fused32", "fused64", "direct64", "gemm64", "gemm32"]
+KERNELS = ["w16", "fused32", "fused64", "direct64", "gemm64", "gemm32", "fused48"]
 HEADER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "w90gpu_tune_table.h")
 
 
