@@ -157,6 +157,7 @@ int max_panels(int nw) {
 // A contiguous range of an item list with the launch shape its kernels need.
 struct Seg {
   int off = 0, n = 0, nnmax = 0, npanmax = 0;
+  int nwmin = 0, nwmax = 0;  // num_wann range of the items (0, 0 when empty)
 };
 
 // Device lists of one launch sequence: items sorted by (class, precision,
@@ -172,14 +173,14 @@ struct Lists {
 
 // Lists over n items of one group and precision (the segments before it
 // empty at 0, those after it empty at n).
-void one_group_lists(Lists& L, int g0, int q0, int n, int nn, int npan) {
+void one_group_lists(Lists& L, int g0, int q0, int n, int nn, int npan, int nw) {
   for (int c = 0; c < kClasses; ++c)
     for (int q = 0; q < 2; ++q)
       for (int g = first_group(c); g <= last_group(c); ++g) {
         const bool after = c > kGroupClass[g0] || (c == kGroupClass[g0] && (q > q0 || (q == q0 && g > g0)));
         L.seg[g][q] = Seg{after ? n : 0, 0, 0, 0};
       }
-  L.seg[g0][q0] = Seg{0, n, nn, npan};
+  L.seg[g0][q0] = Seg{0, n, nn, npan, nw, nw};
 }
 
 struct Pool {
@@ -815,6 +816,8 @@ void build_lists(const std::vector<Member*>& ms, F prec_of, std::vector<Item>& i
           for (int k = 0; k < m->nk; ++k) items.push_back(Item{m->slot, k});
           s.nnmax = std::max(s.nnmax, m->nn);
           s.npanmax = std::max(s.npanmax, rot_panels(m->rk, m->nw));
+          s.nwmin = s.nwmin ? std::min(s.nwmin, m->nw) : m->nw;
+          s.nwmax = std::max(s.nwmax, m->nw);
         }
         s.n = (int)items.size() - s.off;
       }
@@ -870,6 +873,13 @@ Seg class_range(const Lists& L, int c0, int c1) {
   r.off = L.seg[first_group(c0)][0].off;
   const Seg& last = L.seg[last_group(c1)][1];
   r.n = last.off + last.n - r.off;
+  for (int g = first_group(c0); g <= last_group(c1); ++g)
+    for (int q = 0; q < 2; ++q) {
+      const Seg& x = L.seg[g][q];
+      if (x.n == 0) continue;
+      r.nwmin = r.nwmin ? std::min(r.nwmin, x.nwmin) : x.nwmin;
+      r.nwmax = std::max(r.nwmax, x.nwmax);
+    }
   return r;
 }
 
@@ -983,9 +993,14 @@ int launch_step(Engine* E, const Lists& L, int mode) {
   if (b.n)
     hipLaunchKernelGGL(k_step_k<kLdsMaxWann>, dim3(b.n), dim3(W90_NT), 0, s, E->d_probs, L.items + b.off, b.n,
                        mode);
-  if (c.n)
-    hipLaunchKernelGGL(k_step_mfma, dim3(c.n), dim3(256), step_mfma_lds(), s, E->d_probs, L.items + c.off, c.n,
-                       mode);
+  // num_wann up to 48 on the 48 layout, above on the 64 one (each launch
+  // covers the class; workgroups of the other's items return at once)
+  if (c.n && c.nwmin <= 48)
+    hipLaunchKernelGGL(k_step_mfma<48>, dim3(c.n), dim3(StepWaves<48>::NT), step_mfma_lds<48>(), s, E->d_probs,
+                       L.items + c.off, c.n, mode);
+  if (c.n && c.nwmax > 48)
+    hipLaunchKernelGGL(k_step_mfma<64>, dim3(c.n), dim3(StepWaves<64>::NT), step_mfma_lds<64>(), s, E->d_probs,
+                       L.items + c.off, c.n, mode);
   if (d.n)
     hipLaunchKernelGGL(k_step, dim3(d.n), dim3(W90_STEP_NT), 0, s, E->d_probs, L.items + d.off, d.n, mode);
   HIPCHK(E->ctx, hipGetLastError());
@@ -1087,7 +1102,7 @@ int tune_member(Engine* E, Member* m) {
     m->p.npan = rot_panels(rk, nw);
     HIPCHK(E->ctx, hipMemcpyAsync(E->d_probs + m->slot, &m->p, sizeof(Prob), hipMemcpyHostToDevice, E->stream));
     Lists L;
-    one_group_lists(L, group_of(c, rk), 0, n, nn, m->p.npan);
+    one_group_lists(L, group_of(c, rk), 0, n, nn, m->p.npan, nw);
     L.items = E->d_aux_items;
     int r = launch_rotate(E, L, MODE_BENCH);
     if (r) return r;
@@ -1570,7 +1585,8 @@ int w90gpu_create(int device, w90gpu_ctx** ctx) {
   } lds[] = {
       {(const void*)k_rotate_w16<cplx>, rotate_w16_lds<cplx>(W90GPU_NUM_NNMAX)},
       {(const void*)k_rotate_w16<float2>, rotate_w16_lds<float2>(W90GPU_NUM_NNMAX)},
-      {(const void*)k_step_mfma, step_mfma_lds()},
+      {(const void*)k_step_mfma<48>, step_mfma_lds<48>()},
+      {(const void*)k_step_mfma<64>, step_mfma_lds<64>()},
       {(const void*)k_domega_k<16, cplx>, domega_lds<16, cplx>()},
       {(const void*)k_domega_k<16, float2>, domega_lds<16, float2>()},
       {(const void*)k_domega_k<kLdsMaxWann, cplx>, domega_lds<kLdsMaxWann, cplx>()},
@@ -2063,7 +2079,7 @@ int w90gpu_omega_noalloc(w90gpu_ctx* ctx, const w90gpu_problem* prob, const doub
   L.np = 1;
   // the segments build_lists gives a single member: empty before and after
   // its group, all nk items in (group, FP64)
-  one_group_lists(L, group_of(m.cls, m.rk), 0, nk, nn, rot_panels(m.rk, nw));
+  one_group_lists(L, group_of(m.cls, m.rk), 0, nk, nn, rot_panels(m.rk, nw), nw);
   L.any_group = m.p.ngroup > 0;
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, nullptr, E.d_probs, L.plist, 1);
   hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(1024, (nd + 255) / 256)), dim3(256), 0, nullptr,

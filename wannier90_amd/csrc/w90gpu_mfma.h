@@ -857,16 +857,28 @@ __global__ void __launch_bounds__(256) k_reduce_tiles(const Prob* probs, const I
 
 // ---------------------------------------------------------- retraction
 
-// Square complex matrices of edge W90_STEP_N in LDS for k_step_mfma:
-// planar re/im, column-major with leading dimension W90_STEP_N + 1 (MFMA
-// fragment loads of A along a column and of B along a row are both
-// conflict-free with this padding).
+// Square complex matrices of edge N in LDS for k_step_mfma<N> (N = 64 for
+// num_wann 49-64, 48 up to 48): planar re/im, column-major with leading
+// dimension N + 1 (MFMA fragment loads of A along a column and of B along
+// a row are both conflict-free with this padding).  Wavefront layout: a 64
+// matrix in quadrants over four wavefronts, a 48 matrix in three 48 x 16
+// column strips over three (3 x 3 fragments do not split evenly over four),
+// which also leaves room in LDS for two workgroups per CU.
 #define W90_STEP_N 64
-#define W90_STEP_LD (W90_STEP_N + 1)
-#define W90_STEP_MAT (W90_STEP_N * W90_STEP_LD)
+template <int N>
+struct StepWaves {
+  static constexpr int LD = N + 1, MAT = N * LD;
+  static constexpr int WR = N == 48 ? 1 : 2, WC = N == 48 ? 3 : 2;
+  static constexpr int NT = 64 * WR * WC;
+  static constexpr int FRR = N / (16 * WR), FRC = N / (16 * WC), FRM = FRR > FRC ? FRR : FRC;
+  static constexpr int MINB = N == 48 ? 2 : 1;  // workgroups per CU (LDS: 75 KB at 48, 133 KB at 64)
+};
 
-// LDS bytes of k_step_mfma: X and P (re, im each) and a reduction buffer.
-__host__ __device__ constexpr size_t step_mfma_lds() { return sizeof(double) * (4 * W90_STEP_MAT + 256); }
+// LDS bytes of k_step_mfma<N>: X and P (re, im each) and a reduction buffer.
+template <int N>
+__host__ __device__ constexpr size_t step_mfma_lds() {
+  return sizeof(double) * (4 * StepWaves<N>::MAT + 256);
+}
 
 // C = A B for the 64 x 64 LDS matrices (planar at Ar/Ai, Br/Bi) over the
 // first 4 KS columns of A (rows of B): each wavefront accumulates its 32 x 32 quadrant as 2 x 2 complex
@@ -876,26 +888,32 @@ __host__ __device__ constexpr size_t step_mfma_lds() { return sizeof(double) * (
 // steps are unrolled without branches and the operands of step ks + 1 are
 // read from LDS before the products of step ks are issued: with one
 // wavefront per SIMD nothing else covers the LDS latency.
-template <int KS>
-__device__ __forceinline__ void step_gemm(Cfrag<double> (&C)[2][2], const double* Ar, const double* Ai,
-                                          const double* Br, const double* Bi) {
+template <int KS, int N>
+__device__ __forceinline__ void step_gemm(Cfrag<double> (&C)[StepWaves<N>::FRR][StepWaves<N>::FRC], const double* Ar,
+                                          const double* Ai, const double* Br, const double* Bi) {
+  typedef StepWaves<N> SW;
+  constexpr int FRR = SW::FRR, FRC = SW::FRC, LD = SW::LD;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  const int wr = (w / SW::WC) * (N / SW::WR), wc = (w % SW::WC) * (N / SW::WC);
   const int lr = lane & 15, lk = lane >> 4;
 #pragma unroll
-  for (int fi = 0; fi < 2; ++fi)
+  for (int fi = 0; fi < FRR; ++fi)
 #pragma unroll
-    for (int fj = 0; fj < 2; ++fj) C[fi][fj].zero();
-  double ar[2][2], ai[2][2], br[2][2], bi[2][2];  // [buffer][fragment]
+    for (int fj = 0; fj < FRC; ++fj) C[fi][fj].zero();
+  double ar[2][FRR], ai[2][FRR], br[2][FRC], bi[2][FRC];  // [buffer][fragment]
   auto load = [&](int ks, int buf) {
 #pragma unroll
-    for (int f = 0; f < 2; ++f) {
-      const int ia = (4 * ks + lk) * W90_STEP_LD + wr + f * 16 + lr;  // A[row][4 ks + lk]
-      ar[buf][f] = Ar[ia];
-      ai[buf][f] = Ai[ia];
-      const int ib = (wc + f * 16 + lr) * W90_STEP_LD + 4 * ks + lk;  // B[4 ks + lk][col]
-      br[buf][f] = Br[ib];
-      bi[buf][f] = Bi[ib];
+    for (int f = 0; f < SW::FRM; ++f) {
+      if (f < FRR) {
+        const int ia = (4 * ks + lk) * LD + wr + f * 16 + lr;  // A[row][4 ks + lk]
+        ar[buf][f] = Ar[ia];
+        ai[buf][f] = Ai[ia];
+      }
+      if (f < FRC) {
+        const int ib = (wc + f * 16 + lr) * LD + 4 * ks + lk;  // B[4 ks + lk][col]
+        br[buf][f] = Br[ib];
+        bi[buf][f] = Bi[ib];
+      }
     }
   };
   load(0, 0);
@@ -904,24 +922,30 @@ __device__ __forceinline__ void step_gemm(Cfrag<double> (&C)[2][2], const double
     const int cb = ks & 1;
     if (ks + 1 < KS) load(ks + 1, cb ^ 1);
 #pragma unroll
-    for (int fi = 0; fi < 2; ++fi)
+    for (int fi = 0; fi < FRR; ++fi)
 #pragma unroll
-      for (int fj = 0; fj < 2; ++fj) C[fi][fj].mma(ar[cb][fi], ai[cb][fi], br[cb][fj], bi[cb][fj]);
+      for (int fj = 0; fj < FRC; ++fj) C[fi][fj].mma(ar[cb][fi], ai[cb][fi], br[cb][fj], bi[cb][fj]);
   }
 }
 
 // step_gemm over the K steps num_wann needs: 12 up to 48 bands, else 16.
-__device__ __forceinline__ void step_gemm_n(Cfrag<double> (&C)[2][2], const double* Ar, const double* Ai,
-                                            const double* Br, const double* Bi, int n) {
-  if (n > 48) step_gemm<16>(C, Ar, Ai, Br, Bi);
-  else step_gemm<12>(C, Ar, Ai, Br, Bi);
+// Each output fragment is the same MFMA sequence in either layout, so a
+// product's result does not depend on N.
+template <int N>
+__device__ __forceinline__ void step_gemm_n(Cfrag<double> (&C)[StepWaves<N>::FRR][StepWaves<N>::FRC],
+                                            const double* Ar, const double* Ai, const double* Br, const double* Bi,
+                                            int n) {
+  if (N > 48 && n > 48) step_gemm<16, N>(C, Ar, Ai, Br, Bi);
+  else step_gemm<12, N>(C, Ar, Ai, Br, Bi);
 }
 
 // Position (row, column) of accumulator element i of fragment (fi, fj).
+template <int N>
 __device__ __forceinline__ void step_pos(int fi, int fj, int i, int* r, int* c) {
+  typedef StepWaves<N> SW;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  *r = (w >> 1) * 32 + fi * 16 + Mfma<double>::row(lane >> 4, i);
-  *c = (w & 1) * 32 + fj * 16 + (lane & 15);
+  *r = (w / SW::WC) * (N / SW::WR) + fi * 16 + Mfma<double>::row(lane >> 4, i);
+  *c = (w % SW::WC) * (N / SW::WC) + fj * 16 + (lane & 15);
 }
 
 // Retraction of one (problem, k) for 32 < num_wann <= 64, every matrix in
@@ -929,11 +953,19 @@ __device__ __forceinline__ void step_pos(int fi, int fj, int i, int* r, int* c) 
 // E = exp(X) by the scaling and squaring of the Taylor polynomial that
 // k_step uses (same plan), and U_new = U E.  The exponential of the
 // anti-Hermitian X is the unitary internal_new_u_and_m builds from the
-// eigenvectors of i X (wannierise.F90:1662-1689, 1729-1734).
-__global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item* items, int nitems, int mode) {
+// eigenvectors of i X (wannierise.F90:1662-1689, 1729-1734).  N = 48 takes
+// the items of num_wann up to 48 and N = 64 the others (a workgroup whose
+// item is the other's returns at once).
+template <int N>
+__global__ void __launch_bounds__(StepWaves<N>::NT, StepWaves<N>::MINB) k_step_mfma(const Prob* probs,
+                                                                                   const Item* items, int nitems,
+                                                                                   int mode) {
+  typedef StepWaves<N> SW;
+  constexpr int NT = SW::NT, LD = SW::LD, MAT = SW::MAT, FRR = SW::FRR, FRC = SW::FRC;
   if (blockIdx.x >= (unsigned)nitems) return;
   const Item it = items[blockIdx.x];
   const Prob& p = probs[it.prob];
+  if ((p.nw <= 48) != (N == 48)) return;
   const State* st = p.st;
   const int cur = gate_cur(st, mode);
   if (cur < 0) return;
@@ -942,24 +974,25 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
   const int tid = threadIdx.x;
   extern __shared__ double lds_step[];
   double* Xr = lds_step;
-  double* Xi = Xr + W90_STEP_MAT;
-  double* Pr = Xi + W90_STEP_MAT;
-  double* Pi = Pr + W90_STEP_MAT;
-  double* red = Pi + W90_STEP_MAT;
+  double* Xi = Xr + MAT;
+  double* Pr = Xi + MAT;
+  double* Pi = Pr + MAT;
+  double* red = Pi + MAT;
   cplx* Dk = p.D + (size_t)k * nw2;
   // every G, D and U element of the thread is requested before the search
   // and the exponential (one workgroup fills a CU's LDS, so there are no
   // other wavefronts to cover the latency of loads issued one loop trip
   // at a time)
-  constexpr int SL = W90_STEP_N * W90_STEP_N / 256;
+  constexpr int SL = N * N / NT;
+  static_assert(SL * NT == N * N, "whole matrix per thread slot");
   cplx gpre[SL], dpre[SL], upre[SL];
   {
     const cplx* Gk = p.G + (size_t)k * nw2;
     const cplx* Uc = p.U[cur] + (size_t)k * nw2;
 #pragma unroll
     for (int i = 0; i < SL; ++i) {
-      const int e = tid + 256 * i;
-      const int r = e % W90_STEP_N, c = e / W90_STEP_N;
+      const int e = tid + NT * i;
+      const int r = e % N, c = e / N;
       cplx g = c_make(0.0, 0.0), d = g, u = g;
       if (r < nw && c < nw) {
         const int q = r + c * nw;
@@ -985,8 +1018,8 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
   // wavefront 0, the maximum by lane exchange
 #pragma unroll
   for (int i = 0; i < SL; ++i) {
-    const int e = tid + 256 * i;
-    const int r = e % W90_STEP_N, c = e / W90_STEP_N;
+    const int e = tid + NT * i;
+    const int r = e % N, c = e / N;
     cplx x = c_make(0.0, 0.0);
     if (r < nw && c < nw) {
       cplx d = dpre[i];
@@ -996,16 +1029,16 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
       }
       x = c_scale(d, scale);
     }
-    Xr[c * W90_STEP_LD + r] = x.x;
-    Xi[c * W90_STEP_LD + r] = x.y;
-    Pr[c * W90_STEP_LD + r] = sqrt(c_abs2(x));
+    Xr[c * LD + r] = x.x;
+    Xi[c * LD + r] = x.y;
+    Pr[c * LD + r] = sqrt(c_abs2(x));
   }
   __syncthreads();
   double cs = 0.0;
   if (tid < 64) {
     if (tid < nw) {
 #pragma unroll 4
-      for (int r = 0; r < nw; ++r) cs += Pr[tid * W90_STEP_LD + r];
+      for (int r = 0; r < nw; ++r) cs += Pr[tid * LD + r];
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) cs = fmax(cs, __shfl_xor(cs, o));
@@ -1017,9 +1050,9 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
   taylor_plan(cs, &sq, &m);
   const double f = ldexp(1.0, -sq);
   const double invm = 1.0 / (double)m;
-  for (int e = tid; e < W90_STEP_N * W90_STEP_N; e += 256) {
-    const int r = e % W90_STEP_N, c = e / W90_STEP_N;
-    const int q = c * W90_STEP_LD + r;
+  for (int e = tid; e < N * N; e += NT) {
+    const int r = e % N, c = e / N;
+    const int q = c * LD + r;
     const double xr = Xr[q] * f, xi = Xi[q] * f;
     Xr[q] = xr;
     Xi[q] = xi;
@@ -1027,21 +1060,21 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
     Pi[q] = xi * invm;
   }
   __syncthreads();
-  Cfrag<double> C[2][2];
+  Cfrag<double> C[FRR][FRC];
   // Horner: P <- I + X P / j
   for (int j = m - 1; j >= 1; --j) {
-    step_gemm_n(C, Xr, Xi, Pr, Pi, nw);
+    step_gemm_n<N>(C, Xr, Xi, Pr, Pi, nw);
     __syncthreads();
     const double invj = 1.0 / (double)j;
 #pragma unroll
-    for (int fi = 0; fi < 2; ++fi)
+    for (int fi = 0; fi < FRR; ++fi)
 #pragma unroll
-      for (int fj = 0; fj < 2; ++fj)
+      for (int fj = 0; fj < FRC; ++fj)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           int r, c;
-          step_pos(fi, fj, i, &r, &c);
-          const int q = c * W90_STEP_LD + r;
+          step_pos<N>(fi, fj, i, &r, &c);
+          const int q = c * LD + r;
           Pr[q] = C[fi][fj].re(i) * invj + (r == c ? 1.0 : 0.0);
           Pi[q] = C[fi][fj].im(i) * invj;
         }
@@ -1049,17 +1082,17 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
   }
   // squarings: P <- P P
   for (int s = 0; s < sq; ++s) {
-    step_gemm_n(C, Pr, Pi, Pr, Pi, nw);
+    step_gemm_n<N>(C, Pr, Pi, Pr, Pi, nw);
     __syncthreads();
 #pragma unroll
-    for (int fi = 0; fi < 2; ++fi)
+    for (int fi = 0; fi < FRR; ++fi)
 #pragma unroll
-      for (int fj = 0; fj < 2; ++fj)
+      for (int fj = 0; fj < FRC; ++fj)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           int r, c;
-          step_pos(fi, fj, i, &r, &c);
-          const int q = c * W90_STEP_LD + r;
+          step_pos<N>(fi, fj, i, &r, &c);
+          const int q = c * LD + r;
           Pr[q] = C[fi][fj].re(i);
           Pi[q] = C[fi][fj].im(i);
         }
@@ -1071,28 +1104,28 @@ __global__ void __launch_bounds__(256) k_step_mfma(const Prob* probs, const Item
   cplx* ETk = p.ET + (size_t)k * nw2;
 #pragma unroll
   for (int i = 0; i < SL; ++i) {
-    const int e = tid + 256 * i;
-    const int r = e % W90_STEP_N, c = e / W90_STEP_N;
-    const int q = c * W90_STEP_LD + r;
+    const int e = tid + NT * i;
+    const int r = e % N, c = e / N;
+    const int q = c * LD + r;
     if (r < nw && c < nw) {
       st_stream(&Ek[r + c * nw], c_make(Pr[q], Pi[q]));
-      const int qt = r * W90_STEP_LD + c;  // E[c][r]
+      const int qt = r * LD + c;  // E[c][r]
       st_stream(&ETk[r + c * nw], c_make(Pr[qt], Pi[qt]));
     }
     Xr[q] = upre[i].x;
     Xi[q] = upre[i].y;
   }
   __syncthreads();
-  step_gemm_n(C, Xr, Xi, Pr, Pi, nw);
+  step_gemm_n<N>(C, Xr, Xi, Pr, Pi, nw);
   cplx* Un = p.U[1 - cur] + (size_t)k * nw2;
 #pragma unroll
-  for (int fi = 0; fi < 2; ++fi)
+  for (int fi = 0; fi < FRR; ++fi)
 #pragma unroll
-    for (int fj = 0; fj < 2; ++fj)
+    for (int fj = 0; fj < FRC; ++fj)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         int r, c;
-        step_pos(fi, fj, i, &r, &c);
+        step_pos<N>(fi, fj, i, &r, &c);
         if (r < nw && c < nw) st_stream(&Un[r + c * nw], c_make(C[fi][fj].re(i), C[fi][fj].im(i)));
       }
 }
