@@ -861,14 +861,16 @@ __global__ void __launch_bounds__(256) k_reduce_tiles(const Prob* probs, const I
 // num_wann 49-64, 48 up to 48): planar re/im, column-major with leading
 // dimension N + 1 (MFMA fragment loads of A along a column and of B along
 // a row are both conflict-free with this padding).  Wavefront layout: a 64
-// matrix in quadrants over four wavefronts, a 48 matrix in three 48 x 16
-// column strips over three (3 x 3 fragments do not split evenly over four),
-// which also leaves room in LDS for two workgroups per CU.
+// matrix in 32 x 16 blocks over eight wavefronts (two per SIMD, so one
+// covers the other's LDS waits; the matrices fill the CU's LDS, so there
+// is no second workgroup), a 48 matrix in three 48 x 16 column strips over
+// three (3 x 3 fragments do not split evenly over four), which leaves room
+// in LDS for two workgroups per CU.
 #define W90_STEP_N 64
 template <int N>
 struct StepWaves {
   static constexpr int LD = N + 1, MAT = N * LD;
-  static constexpr int WR = N == 48 ? 1 : 2, WC = N == 48 ? 3 : 2;
+  static constexpr int WR = N == 48 ? 1 : 2, WC = N == 48 ? 3 : 4;
   static constexpr int NT = 64 * WR * WC;
   static constexpr int FRR = N / (16 * WR), FRC = N / (16 * WC), FRM = FRR > FRC ? FRR : FRC;
   static constexpr int MINB = N == 48 ? 2 : 1;  // workgroups per CU (LDS: 75 KB at 48, 133 KB at 64)
