@@ -76,9 +76,10 @@ struct Member {
 // kernel classes by num_wann: the wavefront-per-neighbour MFMA rotation up
 // to 16; the one-workgroup-per-block MFMA rotation (32 x 32 tile, T in LDS)
 // with the per-k LDS retraction and gradient up to 32; the same rotation
-// with a 64 x 64 tile, the MFMA retraction and the LDS gradient up to 64;
-// the two-launch MFMA rotation (64 x 64 tiles over panels) with the
-// global-memory retraction and gradient up to W90GPU_MAX_WANN
+// with a 64 x 64 (or, up to 48 bands, 48 x 48) tile, the MFMA retraction
+// and the LDS gradient up to 64; the two-launch MFMA rotation (tiles over
+// panels) with the global-memory retraction and the tiled gradient up to
+// W90GPU_MAX_WANN
 constexpr int kClasses = 4;
 constexpr int kLdsMaxWann = 32;
 // iterations per captured block (W90GPU_GRAPH_BLOCK overrides it, 1-64);

@@ -6,12 +6,13 @@
 //
 // num_wann <= 16 (k_rotate_w16): one 16 x 16 tile per block, one wavefront
 // per neighbour b.  num_wann 17-64 (k_rot_fused): a workgroup computes both
-// products of one (k, b) block as one GT x GT tile (GT = 32 or 64), T kept
-// in LDS between them.  num_wann 65-128 (k_rot_gemm): each product is its
-// own launch over 2 x 2 tiles of 64, T through a scratch buffer.  Four
-// wavefronts hold a GT/2 x GT/2 quadrant of a tile each, as 16 x 16
-// fragments; a complex fragment product is three real MFMA products per K
-// step (Cfrag, Gauss's form).  K runs in chunks staged in LDS with
+// products of one (k, b) block as one GT x GT tile (GT = 32, 48 or 64), T
+// kept in LDS between them.  num_wann 65-256 (k_rot_gemm): each product is
+// its own launch over up to 4 x 4 tiles of 64 (or 8 x 8 of 32), T through
+// a scratch buffer.  Four wavefronts hold a quadrant of a tile each (three
+// hold a 48 x 16 strip of a 48 tile), as 16 x 16 fragments; a complex
+// fragment product is three real MFMA products per K step (Cfrag, Gauss's
+// form).  K runs in chunks staged in LDS with
 // coalesced global loads (planar re/im, padded rows), the next chunk
 // loaded into registers while the current one is multiplied.  Fragments
 // wholly outside num_wann are skipped.  The second product's epilogue
