@@ -6,7 +6,9 @@ problems: the single phases (rotation, wann_omega with the kernel's tiles,
 gradient) to 1e-12 relative, and a 20-iteration trajectory to 1e-8 Ang^2 in
 Omega at every iteration, 1e-6 Ang in the centres and 1e-7 in U (or 20x the
 trajectory's own sensitivity to rounding, measured on the oracle, where that
-is larger).  The online tuner (W90GPU_TUNE=1) is checked to time both
+is larger).  The FP32 instances of the 33-48-band kernels are checked in
+the mixed bulk phase (fused48 against fused64 and against FP64).  The
+online tuner (W90GPU_TUNE=1) is checked to time both
 candidates of a key, write them to its file, keep the faster one only by its
 3% rule, and give the same result bits on a second solve of the key.
 """
@@ -146,6 +148,38 @@ def test_rotation_kernel_trajectory_matches_oracle(ctx, param):
     assert np.abs(hist[:, 0] - o["history"][:, 0]).max() <= tol(1e-8, s_h)
     assert np.abs(cen - o["centres"]).max() <= tol(1e-6, s_c)
     assert np.abs(u - o["u"]).max() <= tol(1e-7, s_u)
+
+
+def test_mixed_bulk_phase_with_the_48_tile(ctx):
+    """The FP32 instances of the 33-48-band rotation kernels in the mixed
+    bulk phase (num_wann 40): the Omega history of a 10-iteration mixed
+    solve with fused48 equals the one with fused64 to 1e-8 relative (both
+    form every fragment by the same FP32 MFMA sequence; only the order of
+    the per-tile spread sums differs), and follows the FP64 solve's over
+    the bulk iterations to 1e-4 relative (FP32 rounding of M)."""
+    from wannier90_amd import capi
+
+    case = ("sc", 40, (3, 3, 3))
+    print(f"[tune mixed] {case}", flush=True)
+    p, u0, m0 = problem(*case)
+    nw, nn = p.num_wann, p.kmesh.nntot
+    ctl = dict(num_iter=10, num_cg_steps=5, conv_window=-1, conv_tol=1e-10, trial_step=2.0)
+    r64, h64, _, _ = solve(ctx, p, u0, m0, ctl)
+    runs = {}
+    for kernel in ("fused48", "fused64"):
+        with fixed_kernel(nw, nn, kernel):
+            runs[kernel] = solve(ctx, p, u0, m0, dict(ctl, precision=capi.PREC_MIXED))
+    r48, h48, _, _ = runs["fused48"]
+    rf64, hf64, _, _ = runs["fused64"]
+    assert r48.unitarity_ok and rf64.unitarity_ok
+    assert r48.iterations == rf64.iterations and r48.bulk_iterations == rf64.bulk_iterations
+    nb = r48.bulk_iterations
+    assert nb > 0
+    print(f"[tune mixed] bulk {nb} of {r48.iterations}; Omega {h48[nb - 1, 0]:.10f} (fp64 {h64[nb - 1, 0]:.10f})",
+          flush=True)
+    scale = np.abs(h64[:nb, 0]).max()
+    assert np.abs(h48[:, 0] - hf64[:, 0]).max() <= 1e-8 * scale
+    assert np.abs(h48[:nb, 0] - h64[:nb, 0]).max() <= 1e-4 * scale
 
 
 def test_online_tuner_times_candidates_and_keeps_its_choice(tmp_path):
