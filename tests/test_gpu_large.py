@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 po = pytest.importorskip("oracle.pyoracle")
 
 # (lattice, num_wann, MP grid): 32x32 MFMA tiles (17-32), 64x64 tiles with
-# the LDS retraction (33-64) and with the global-memory retraction (65-128)
+# the LDS retraction (33-64) and with the global-memory retraction (65-256)
 LARGE = [("sc", 40, (3, 3, 3)), ("bcc", 48, (3, 3, 3)), ("fcc", 33, (2, 2, 2)),
          ("triclinic", 64, (2, 2, 3)), ("fcc", 24, (3, 3, 3)), ("hex", 17, (3, 3, 3)),
          ("bcc", 32, (3, 3, 3)), ("sc", 96, (2, 2, 2)), ("hex", 70, (2, 2, 2))]

@@ -10,7 +10,7 @@ every iteration, centres 1e-6 Ang, U 1e-7, or 20x the trajectory's own
 measured sensitivity to rounding where that is larger).
 
 Serving: requests of the serving distribution (num_wann up to 96, so the
-65-128 kernel class runs), arriving while others are on the device, under
+65-256 kernel class runs), arriving while others are on the device, under
 the continuous-batching loop of the serving workload: each result equals
 the same request solved alone.
 """
@@ -131,7 +131,7 @@ def test_ragged_batch_small_problems_match_oracle(ctx, batch_problems):
     assert n >= 4
 
 
-# serving requests: num_wann 74 and 71 (the 65-128 class, same lane), 24,
+# serving requests: num_wann 74 and 71 (the 65-256 class, same lane), 24,
 # 26, 18 (4^3 meshes) and 14 (6^3)
 SERVE_IDS = [6, 13, 1, 10, 31, 54]
 
