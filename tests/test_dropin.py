@@ -16,7 +16,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import ROOT, Case
+from conftest import ROOT, Case, tol
 
 EXE = os.path.join(ROOT, "wannier90_amd", "lib", "wannier90_gpu.x")
 INPUTS = os.path.join(ROOT, "tests", "golden", "inputs")
@@ -265,6 +265,59 @@ def test_c_caller_disentangles_like_the_reference(tmp_path):
     assert np.abs(w - wr).max() <= 1e-5
     assert np.abs(np.sort(spr) - np.sort(z["spreads"])).max() <= 5e-6
     assert abs(om[3] - float(z["omega"][3])) <= 2e-8
+
+
+@pytest.mark.gpu
+@needs_exe
+def test_c_caller_disentangles_above_64_bands(tmp_path):
+    """The same C++ caller on a synthetic problem of 72 bands to 12 WFs (fcc,
+    2x2x2 mesh): the host library's size checks and the disentanglement
+    kernels whose matrices live in global memory, behind cdisentangle, then
+    cwannierise.  The optimal subspace against the oracle's dis_main (1e-8);
+    Omega against the same two steps through the GPU's C ABI (1e-10
+    relative: the host adds no arithmetic) and against the oracle's
+    wann_main from that disentanglement's M and U (1e-8 Ang^2, or 20x the
+    trajectory's own sensitivity to rounding where that is larger: here a
+    1e-15 perturbation of M moves the final Omega by 2e-6 Ang^2, and the
+    GPU ends 1.5e-5 from the oracle)."""
+    po = pytest.importorskip("oracle.pyoracle")
+    from wannier90_amd import capi, synth
+
+    nb, nw = 72, 12
+    p, dc = synth.make_dis_problem(7 + nb, kind="fcc", num_bands=nb, num_wann=nw, mp=(2, 2, 2))
+    dc = dict(dc, num_iter=30)
+    ctl = dict(num_iter=20, num_cg_steps=5, conv_window=-1, conv_tol=1e-10, trial_step=2.0)
+    dctl = {"num_iter": dc["num_iter"], "conv_window": dc["conv_window"], "win_min": -2e300, "win_max": 2e300,
+            "froz_min": -2e300, "froz_max": 2e300, "mix_ratio": dc["mix_ratio"], "conv_tol": dc["conv_tol"],
+            "frozen_states": 0}
+    print(f"[c demo] dis nb={nb} nw={nw} nk={p.num_kpts}", flush=True)
+    ierr, _, _, uopt, _, _, om = run_c_demo(tmp_path, nw, nb, p.mp, ctl, p.real_lattice, p.kpts, p.kmesh.nntot,
+                                            p.m_orig, p.amat_dis, p.eig, dctl)
+    assert ierr == 0
+    o = po.dis_main(nb, nw, p.kmesh, dc, p.eig, p.amat_dis, p.m_orig)
+
+    def proj(x):
+        return np.einsum("iwk,jwk->ijk", x, x.conj())
+
+    assert np.abs(proj(uopt) - proj(o["u_opt"])).max() <= 1e-8
+    ctx = capi.Context(0)
+    g = ctx.disentangle(p.kmesh, nb, dc, p.eig, p.amat_dis, p.m_orig)
+    sess = ctx.session(p.kmesh, nw, p.num_kpts, capi.control(**ctl))
+    sess.load(np.asfortranarray(g["m"]), np.asfortranarray(g["u"]))
+    res = sess.solve()
+    sess.close()
+    ctx.close()
+    assert abs(om[3] - res.omega_total) <= 1e-10 * max(1.0, abs(res.omega_total))
+    op = po.Problem(nw, p.num_kpts, p.kmesh)
+    mg, ug = np.asfortranarray(g["m"]), np.asfortranarray(g["u"])
+    ow = po.wann_main(op, ctl, mg, ug)
+    sens = 0.0
+    for seed in (1, 2):
+        rng = np.random.default_rng(seed)
+        o2 = po.wann_main(op, ctl, mg * (1.0 + 1e-15 * rng.standard_normal(mg.shape)), ug)
+        sens = max(sens, abs(o2["omega"][3] - ow["omega"][3]))
+    print(f"[c demo] Omega {om[3]:.10f}, oracle {ow['omega'][3]:.10f}, sensitivity {sens:.1e}", flush=True)
+    assert abs(om[3] - ow["omega"][3]) <= tol(1e-8 * max(1.0, abs(ow["omega"][3])), sens)
 
 
 @pytest.mark.gpu
