@@ -141,9 +141,11 @@ def test_wannierise_after_disentangle_matches_reference(ctx, name):
 # 60-column tiles.  Above 64 bands: "fcc-72-12" has the one-time kernels and
 # the extraction step in global memory and the Z pass (73 x 12 blocks) in
 # LDS; "hex-80-24" (windows of at most 60 bands) only the one-time kernels;
-# "sc-96-48" and "bcc-128-100" every kernel.
+# "sc-96-48" and "bcc-128-100" every kernel; "sc-65-33" is the first size
+# past the LDS kernels (its Z pass, 65 x 33 blocks, still in LDS).
 SYN = [("fcc", 20, 12, (4, 4, 4), 4, 0), ("sc", 48, 24, (3, 3, 3), 6, 0), ("bcc", 64, 40, (2, 2, 2), 0, 0),
-       ("hex", 33, 17, (3, 3, 2), 10, 0), ("sc", 64, 60, (2, 2, 2), 0, 0), ("fcc", 72, 12, (2, 2, 2), 0, 0),
+       ("hex", 33, 17, (3, 3, 2), 10, 0), ("sc", 64, 60, (2, 2, 2), 0, 0), ("sc", 65, 33, (2, 2, 2), 0, 0),
+       ("fcc", 72, 12, (2, 2, 2), 0, 0),
        ("hex", 80, 24, (2, 2, 2), 4, 60), ("sc", 96, 48, (2, 2, 2), 8, 0), ("bcc", 128, 100, (2, 2, 2), 0, 0)]
 
 

@@ -197,3 +197,13 @@ def test_triclinic_800k_matches_reference(ctx):
     assert abs(rf.omega_invariant) <= 1e-10
     assert abs(sprf.sum() - rf.omega_total) <= 1e-9 * rf.omega_total
     assert rf.omega_total < hist[0, 0]
+
+
+def test_num_wann_above_the_limit_is_rejected(ctx):
+    """num_wann above W90GPU_MAX_WANN (256) is an input error, as every other
+    size the kernels do not take."""
+    from wannier90_amd import capi
+
+    p, _, _ = problem("sc", 40, (2, 2, 2))
+    with pytest.raises(capi.W90Error, match=r"num_wann=257 outside \[1, 256\]"):
+        ctx.session(p.kmesh, capi.MAX_WANN + 1, p.num_kpts, capi.control(num_iter=1))
